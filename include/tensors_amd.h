@@ -1,0 +1,123 @@
+/*
+ * tensors_amd.h -- C ABI of the MI355X (gfx950) sparse hot path.
+ *
+ * Plain pointers and sizes only.  Every pointer to matrix or vector data is a
+ * DEVICE pointer (hipMalloc'd or otherwise GPU-resident) unless a comment says
+ * "host".  Every call that takes a `stream` (a hipStream_t passed as void*,
+ * NULL = the null stream) is stream-ordered: it enqueues work and returns
+ * without synchronising, except where a comment says it synchronises.
+ *
+ * Return values are tg_status codes; tg_last_error() gives a message for the
+ * calling thread's most recent non-zero status.
+ *
+ * Index types: column indices are int32 (`Int` of the reference); row
+ * pointers (`LInt`) are int32 or int64, named by an rp_type argument.
+ * Values / dense blocks are FP64 or FP32 (val_type); A, X and Y share it.
+ * Dense blocks are row-major with a leading dimension (ld >= nrhs), as in the
+ * reference's Tensor2 (src/Tensor2.hpp:9-30).
+ */
+#ifndef TENSORS_AMD_H
+#define TENSORS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum tg_dtype { TG_F64 = 0, TG_F32 = 1, TG_I32 = 2, TG_I64 = 3 } tg_dtype;
+
+typedef enum tg_status {
+    TG_OK = 0,
+    TG_ERR_INVALID = -1,  /* bad argument (negative size, null pointer, ld < nrhs, ...) */
+    TG_ERR_ALIAS = -2,    /* X == Y: the reference prints and does nothing (SpMM.hpp:236-243) */
+    TG_ERR_NRHS = -3,     /* fixed NRHS != nrhs: reference no-op (Multiply_DenseMatrix.hpp:19-26) */
+    TG_ERR_DTYPE = -4,    /* unsupported index / value type */
+    TG_ERR_HIP = -5,      /* a HIP runtime call failed */
+    TG_ERR_WORKSPACE = -6,/* workspace missing or too small */
+    TG_ERR_SHAPE = -7     /* shape mismatch (MatrixCSR.hpp:999-1002,1018-1021) */
+} tg_status;
+
+/* Message for the calling thread's last non-zero status ("" if none). */
+const char* tg_last_error(void);
+/* Library version string. */
+const char* tg_version(void);
+
+/* ------------------------------------------------------------------------
+ * Row partition -- replaces Tools' JobPointers<Int>(m, outer, T, false) that
+ * PatternCSR::RequireJobPtr builds and caches (src/Sparse/PatternCSR.hpp:389-398).
+ * Writes job_ptr[0..blocks] (device, int32): block b owns rows
+ * [job_ptr[b], job_ptr[b+1]), cut where rp[i] + i first reaches
+ * b * (nnz + m) / blocks, so every block carries about the same nonzeros + rows.
+ * ------------------------------------------------------------------------ */
+int tg_csr_job_pointers(int32_t m, const void* rp, tg_dtype rp_type, int32_t blocks, int32_t* job_ptr,
+                        void* stream);
+
+/* Number of partition blocks tg_csr_spmm uses for a matrix of m rows and nnz
+ * nonzeros multiplied against nrhs columns (host arithmetic only). */
+int32_t tg_csr_default_blocks(int32_t m, int64_t nnz, int32_t nrhs);
+
+/* ------------------------------------------------------------------------
+ * Y <- alpha * A X + beta * Y   (A: m x n CSR; X: n x nrhs, ldX; Y: m x nrhs, ldY)
+ *
+ * Replaces SparseBLAS<Scal,Int,LInt>::Multiply_DenseMatrix<NRHS>
+ * (src/SparseBLAS/Multiply_DenseMatrix.hpp:9-45), i.e. the SpMV
+ * (src/SparseBLAS/SpMV.hpp:3-288) and SpMM (src/SparseBLAS/SpMM.hpp:3-569)
+ * kernels behind Sparse::MatrixCSR::Dot (src/Sparse/MatrixCSR.hpp:965-1086)
+ * and PatternCSR::Dot_ (src/Sparse/PatternCSR.hpp:844-890).
+ *
+ *  a == NULL   : binary matrix, every stored entry is 1 (a_flag Plus).
+ *  NRHS > 0    : the fixed-NRHS instantiation; NRHS != nrhs -> TG_ERR_NRHS, no-op.
+ *  alpha == 0  : Y <- beta * Y only (beta 0: zero, 1: untouched).
+ *  beta == 0   : Y is written without being read (NaNs in Y do not propagate).
+ *  X == Y      : TG_ERR_ALIAS, no-op.
+ *  empty rows  : Y row <- beta * Y row (beta 0: zero, 1: untouched).
+ *  nnz         : rp[m] - rp[0], known to the caller (MatrixCSR::NonzeroCount);
+ *                it picks the lane layout, it is not read from the device.
+ *  job_ptr     : optional device partition from tg_csr_job_pointers with
+ *                `blocks` blocks; NULL = rows split evenly inside the kernel.
+ * ------------------------------------------------------------------------ */
+int tg_csr_spmm(int32_t m, int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, const int32_t* ci, const void* a,
+                tg_dtype val_type, double alpha, const void* X, int64_t ldX, double beta, void* Y,
+                int64_t ldY, int32_t nrhs, int32_t NRHS, const int32_t* job_ptr, int32_t blocks,
+                void* stream);
+
+/* Y <- alpha * X + beta * Y over an m x nrhs block (flag shortcuts of
+ * combine_matrices_auto: beta 0 -> Y not read; alpha 0 -> X not read).
+ * Replaces Tools' combine_matrices_auto / scale_matrix as used by
+ * ConjugateGradient.hpp:342-346 and GMRES.hpp:835-845. */
+int tg_axpby(int32_t m, int32_t nrhs, tg_dtype val_type, double alpha, const void* X, int64_t ldX,
+             double beta, void* Y, int64_t ldY, void* stream);
+
+/* Streaming probes (no reference counterpart): dst <- src over `bytes`
+ * (multiple of 16), and a read-only pass over src (sink: one device float that
+ * is practically never written).  They measure the achievable HBM bandwidth
+ * that the SpMM roofline fraction is quoted against. */
+int tg_probe_copy(const void* src, void* dst, int64_t bytes, void* stream);
+int tg_probe_read(const void* src, int64_t bytes, float* sink, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Synthetic workload generators (HOST buffers; no device work).  No
+ * reference counterpart: the reference draws its test inputs from
+ * std::random_device (src/Tensor_Common.hpp:141-211), which cannot be
+ * reproduced; these are counter-based (SplitMix64 of seed, stream, index) so
+ * every run and every thread count yields the same inputs.  Row pointers are
+ * int64, column indices int32.  Size-returning functions return the nonzero
+ * count or a negative tg_status.
+ * ------------------------------------------------------------------------ */
+int tg_gen_uniform(uint64_t seed, uint64_t offset, int64_t count, tg_dtype dtype, void* host_out);
+int64_t tg_gen_random_csr_rowptr(int64_t m, double mean, int32_t lo, int32_t hi, uint64_t seed, int64_t* rp);
+int tg_gen_random_csr_fill(int64_t m, int64_t n, const int64_t* rp, uint64_t seed, int32_t* ci, double* val64,
+                           float* val32);
+int64_t tg_gen_laplacian_3d(int32_t g, double shift, int64_t* rp, int32_t* ci, double* val);
+int64_t tg_gen_weighted_grid(int32_t dims, int32_t s0, int32_t s1, int32_t s2, double shift, uint64_t seed,
+                             int64_t* rp, int32_t* ci, double* val);
+int64_t tg_gen_convection_diffusion_2d(int32_t g, double peclet, double theta, int64_t* rp, int32_t* ci,
+                                       double* val);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TENSORS_AMD_H */

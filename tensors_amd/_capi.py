@@ -1,0 +1,86 @@
+"""ctypes bindings of libtensors_amd.so (the C ABI declared in include/tensors_amd.h).
+
+The library is built in-tree by ``make -C tensors_amd/csrc`` (``__graft_entry__.build``).
+Loading fails loudly when it is missing: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtensors_amd.so")
+
+# tg_dtype
+F64, F32, I32, I64 = 0, 1, 2, 3
+# tg_status
+OK, ERR_INVALID, ERR_ALIAS, ERR_NRHS, ERR_DTYPE, ERR_HIP, ERR_WORKSPACE, ERR_SHAPE = 0, -1, -2, -3, -4, -5, -6, -7
+
+STATUS_NAMES = {
+    OK: "TG_OK", ERR_INVALID: "TG_ERR_INVALID", ERR_ALIAS: "TG_ERR_ALIAS", ERR_NRHS: "TG_ERR_NRHS",
+    ERR_DTYPE: "TG_ERR_DTYPE", ERR_HIP: "TG_ERR_HIP", ERR_WORKSPACE: "TG_ERR_WORKSPACE",
+    ERR_SHAPE: "TG_ERR_SHAPE",
+}
+
+i32, i64, u64, dbl, vp = C.c_int32, C.c_int64, C.c_uint64, C.c_double, C.c_void_p
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "tg_last_error": (C.c_char_p, []),
+    "tg_version": (C.c_char_p, []),
+    "tg_csr_job_pointers": (C.c_int, [i32, vp, C.c_int, i32, vp, vp]),
+    "tg_csr_default_blocks": (i32, [i32, i64, i32]),
+    "tg_csr_spmm": (C.c_int, [i32, i32, i64, vp, C.c_int, vp, vp, C.c_int, dbl, vp, i64, dbl, vp, i64,
+                              i32, i32, vp, i32, vp]),
+    "tg_axpby": (C.c_int, [i32, i32, C.c_int, dbl, vp, i64, dbl, vp, i64, vp]),
+    "tg_probe_copy": (C.c_int, [vp, vp, i64, vp]),
+    "tg_probe_read": (C.c_int, [vp, i64, vp, vp]),
+    "tg_gen_uniform": (C.c_int, [u64, u64, i64, C.c_int, vp]),
+    "tg_gen_random_csr_rowptr": (i64, [i64, dbl, i32, i32, u64, vp]),
+    "tg_gen_random_csr_fill": (C.c_int, [i64, i64, vp, u64, vp, vp, vp]),
+    "tg_gen_laplacian_3d": (i64, [i32, dbl, vp, vp, vp]),
+    "tg_gen_weighted_grid": (i64, [i32, i32, i32, i32, dbl, u64, vp, vp, vp]),
+    "tg_gen_convection_diffusion_2d": (i64, [i32, dbl, dbl, vp, vp, vp]),
+}
+
+_lib = None
+
+
+class TensorsError(RuntimeError):
+    """A tg_* call returned a non-zero status."""
+
+    def __init__(self, status: int, where: str, message: str):
+        self.status = status
+        super().__init__(f"{where}: {STATUS_NAMES.get(status, status)}: {message}")
+
+
+def lib() -> C.CDLL:
+    """The loaded library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `make -C tensors_amd/csrc` "
+                "(or __graft_entry__.build()); there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().tg_last_error().decode()
+
+
+def check(status: int, where: str) -> int:
+    if status < 0:
+        raise TensorsError(int(status), where, last_error())
+    return status
+
+
+def call(name: str, *args) -> int:
+    """Call tg_<name> and raise TensorsError on a negative status."""
+    return check(getattr(lib(), name)(*args), name)

@@ -1,0 +1,485 @@
+// CSR SpMV / SpMM for gfx950:  Y <- alpha * A X + beta * Y.
+//
+// Reference semantics: SparseBLAS::Multiply_DenseMatrix routing
+// (src/SparseBLAS/Multiply_DenseMatrix.hpp:9-45), SpMV (src/SparseBLAS/SpMV.hpp:
+// 3-288), SpMM (src/SparseBLAS/SpMM.hpp:3-569).  The reference's compile-time
+// flag dispatch (a_flag x alpha_flag x beta_flag, SpMM.hpp:83-175) becomes
+// wave-uniform runtime branches here: they cost nothing in a bandwidth-bound
+// kernel and keep the instantiation count down.
+//
+// Two kernel families, both HBM-bound (no MFMA: ~0.1-5 flop/byte):
+//  * spmv_team<TEAM>: nrhs == 1.  A team of TEAM lanes (power of two, picked
+//    from the mean row length) owns one row; lanes stride through the row so
+//    the int32 column indices and values stream coalesced; the row sum is a
+//    butterfly over the team.  64/TEAM rows per wave.
+//  * spmm_wave<G,V>: nrhs >= 2.  One wavefront owns one row.  G lanes cover a
+//    tile of G*V right-hand-side columns (V-wide vector loads of the row-major
+//    X rows), and the 64/G lane groups of the wave split the row's nonzeros
+//    between them.  The wave loads 64 (column, value) pairs coalesced and
+//    hands each group its pair through a cross-lane read; partial sums of the
+//    groups are combined by butterfly at the end of the row.
+// Rows are distributed over workgroups either evenly (rows_per_block) or by a
+// device partition (job_ptr) that balances nonzeros + rows per block.
+
+#include <algorithm>
+
+#include "common.hpp"
+
+namespace tg {
+
+enum BetaMode : int { kBetaZero = 0, kBetaOne = 1, kBetaGeneric = 2 };
+
+template <typename T, int V>
+struct VecOf {
+    typedef T type __attribute__((ext_vector_type(V)));
+};
+
+template <typename T, int V>
+__device__ __forceinline__ void load_vec(const T* __restrict__ p, T (&out)[V]) {
+    if constexpr (V == 1) {
+        out[0] = *p;
+    } else {
+        using VT = typename VecOf<T, V>::type;
+        const VT v = *reinterpret_cast<const VT*>(p);
+#pragma unroll
+        for (int i = 0; i < V; ++i) out[i] = v[i];
+    }
+}
+
+template <typename T, int V>
+__device__ __forceinline__ void store_vec(T* __restrict__ p, const T (&in)[V]) {
+    if constexpr (V == 1) {
+        *p = in[0];
+    } else {
+        using VT = typename VecOf<T, V>::type;
+        VT v;
+#pragma unroll
+        for (int i = 0; i < V; ++i) v[i] = in[i];
+        *reinterpret_cast<VT*>(p) = v;
+    }
+}
+
+// Row epilogue for V consecutive entries of one Y row: the combine_scalars /
+// combine_buffers contract y = (alpha * z) + (beta * y) (SpMV.hpp:224-225,
+// SpMM.hpp:433-438) and the empty-row rules (SpMM.hpp:440-455).
+template <typename T, int V>
+__device__ __forceinline__ void epilogue(T* __restrict__ yp, const T (&z)[V], bool empty, T alpha,
+                                         int alpha_one, T beta, int beta_mode) {
+    T out[V];
+    if (empty) {
+        if (beta_mode == kBetaOne) return;
+        if (beta_mode == kBetaZero) {
+#pragma unroll
+            for (int i = 0; i < V; ++i) out[i] = T(0);
+        } else {
+            load_vec<T, V>(yp, out);
+#pragma unroll
+            for (int i = 0; i < V; ++i) out[i] = beta * out[i];
+        }
+        store_vec<T, V>(yp, out);
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < V; ++i) out[i] = alpha_one ? z[i] : alpha * z[i];
+    if (beta_mode != kBetaZero) {
+        T yv[V];
+        load_vec<T, V>(yp, yv);
+        if (beta_mode == kBetaOne) {
+#pragma unroll
+            for (int i = 0; i < V; ++i) out[i] = out[i] + yv[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < V; ++i) out[i] = out[i] + beta * yv[i];
+        }
+    }
+    store_vec<T, V>(yp, out);
+}
+
+__device__ __forceinline__ void block_rows(int m, const int32_t* __restrict__ job_ptr, int rows_per_block,
+                                           int& r0, int& r1) {
+    const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+    if (job_ptr) {
+        r0 = job_ptr[b];
+        r1 = job_ptr[b + 1];
+    } else {
+        r0 = b * rows_per_block;
+        r1 = min(m, r0 + rows_per_block);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// nrhs == 1
+// ---------------------------------------------------------------------------
+template <typename T, typename RP, int TEAM>
+__global__ __launch_bounds__(kBlock) void spmv_team_kernel(
+    int m, const RP* __restrict__ rp, const int32_t* __restrict__ ci, const T* __restrict__ a, T alpha,
+    const T* __restrict__ x, int64_t ldx, T beta, T* __restrict__ y, int64_t ldy, int alpha_one,
+    int beta_mode, const int32_t* __restrict__ job_ptr, int rows_per_block) {
+    constexpr int TEAMS = kBlock / TEAM;
+    const int lt = threadIdx.x % TEAM;
+    const int team = threadIdx.x / TEAM;
+    int r0, r1;
+    block_rows(m, job_ptr, rows_per_block, r0, r1);
+    for (int row = r0 + team; row < r1; row += TEAMS) {
+        const RP lb = rp[row], le = rp[row + 1];
+        T z = T(0);
+        if (a) {
+            for (RP l = lb + lt; l < le; l += TEAM) {
+                const int32_t c = __builtin_nontemporal_load(ci + l);
+                const T av = __builtin_nontemporal_load(a + l);
+                z += av * x[(int64_t)c * ldx];
+            }
+        } else {
+            for (RP l = lb + lt; l < le; l += TEAM) {
+                const int32_t c = __builtin_nontemporal_load(ci + l);
+                z += x[(int64_t)c * ldx];
+            }
+        }
+        z = group_sum<TEAM>(z);
+        if (lt == 0) {
+            T zz[1] = {z};
+            epilogue<T, 1>(y + (int64_t)row * ldy, zz, le == lb, alpha, alpha_one, beta, beta_mode);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// nrhs >= 2
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T lane_read(T v, int src) {
+    return __shfl(v, src, kWave);
+}
+
+// Broadcast of lane `src` (wave-uniform) through the scalar unit.
+__device__ __forceinline__ int32_t lane_read_uniform(int32_t v, int src) {
+    return __builtin_amdgcn_readlane(v, src);
+}
+__device__ __forceinline__ float lane_read_uniform(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+__device__ __forceinline__ double lane_read_uniform(double v, int src) {
+    const int64_t b = __double_as_longlong(v);
+    const int32_t lo = __builtin_amdgcn_readlane((int32_t)(b & 0xffffffff), src);
+    const int32_t hi = __builtin_amdgcn_readlane((int32_t)(b >> 32), src);
+    return __longlong_as_double(((int64_t)hi << 32) | (uint32_t)lo);
+}
+
+template <typename T, typename RP, int G, int V>
+__global__ __launch_bounds__(kBlock) void spmm_wave_kernel(
+    int m, const RP* __restrict__ rp, const int32_t* __restrict__ ci, const T* __restrict__ a, T alpha,
+    const T* __restrict__ X, int64_t ldX, T beta, T* __restrict__ Y, int64_t ldY, int nrhs, int alpha_one,
+    int beta_mode, const int32_t* __restrict__ job_ptr, int rows_per_block) {
+    constexpr int S = kWave / G;  // lane groups splitting a row's nonzeros
+    constexpr int KT = G * V;     // columns per tile
+    constexpr int WAVES = kBlock / kWave;
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+    const int s = lane / G, g = lane % G;
+    int r0, r1;
+    block_rows(m, job_ptr, rows_per_block, r0, r1);
+    for (int row = r0 + wave; row < r1; row += WAVES) {
+        const RP lb = rp[row], le = rp[row + 1];
+        T* __restrict__ yrow = Y + (int64_t)row * ldY;
+        for (int c0 = 0; c0 < nrhs; c0 += KT) {
+            const int col = c0 + g * V;
+            const bool colok = col < nrhs;
+            T acc[V];
+#pragma unroll
+            for (int i = 0; i < V; ++i) acc[i] = T(0);
+            for (RP base = lb; base < le; base += kWave) {
+                const RP e = base + lane;
+                int32_t myc = 0;
+                T mya = T(0);
+                if (e < le) {
+                    myc = __builtin_nontemporal_load(ci + e);
+                    mya = a ? __builtin_nontemporal_load(a + e) : T(1);
+                }
+                const int cnt = (int)((le - base) < (RP)kWave ? (le - base) : (RP)kWave);
+                // G steps of S nonzeros cover the 64 loaded pairs.  Steps run in
+                // groups of U without branches inside a group, so the X loads of
+                // a group are all in flight together; the wave-uniform test
+                // between groups skips the tail of short rows.
+                constexpr int U = G < 4 ? G : 4;
+                for (int st0 = 0; st0 < G; st0 += U) {
+                    if (st0 * S >= cnt) break;
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int j = (st0 + u) * S + s;
+                        int32_t c;
+                        T av;
+                        if constexpr (S == 1) {
+                            c = lane_read_uniform(myc, st0 + u);
+                            av = lane_read_uniform(mya, st0 + u);
+                        } else {
+                            c = lane_read(myc, j);
+                            av = lane_read(mya, j);
+                        }
+                        const bool live = (j < cnt) && colok;
+                        T xv[V];
+                        load_vec<T, V>(X + (int64_t)(live ? c : 0) * ldX + (live ? col : 0), xv);
+#pragma unroll
+                        for (int i = 0; i < V; ++i) acc[i] += live ? av * xv[i] : T(0);
+                    }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < V; ++i) {
+#pragma unroll
+                for (int off = G; off < kWave; off <<= 1) acc[i] += shfl_xor(acc[i], off);
+            }
+            if (s == 0 && colok) epilogue<T, V>(yrow + col, acc, le == lb, alpha, alpha_one, beta, beta_mode);
+        }
+    }
+}
+
+// Y <- beta * Y over m x nrhs (alpha == 0 path of SpMM.hpp:65-71 / SpMV.hpp:15-30)
+// and the general axpby used by the solvers.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void axpby_kernel(int m, int nrhs, T alpha, const T* __restrict__ X,
+                                                       int64_t ldX, T beta, T* __restrict__ Y, int64_t ldY,
+                                                       int alpha_mode, int beta_mode) {
+    const int64_t total = (int64_t)m * nrhs;
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < total; e += (int64_t)gridDim.x * kBlock) {
+        const int64_t i = e / nrhs;
+        const int k = (int)(e - i * nrhs);
+        T* yp = Y + i * ldY + k;
+        T out = T(0);
+        if (alpha_mode != kBetaZero) {
+            const T xv = X[i * ldX + k];
+            out = (alpha_mode == kBetaOne) ? xv : alpha * xv;
+        }
+        if (beta_mode == kBetaOne)
+            out = out + *yp;
+        else if (beta_mode == kBetaGeneric)
+            out = out + beta * *yp;
+        *yp = out;
+    }
+}
+
+template <typename RP>
+__global__ void job_pointers_kernel(int m, const RP* __restrict__ rp, int blocks, int32_t* __restrict__ jp) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > blocks) return;
+    if (b == 0) {
+        jp[0] = 0;
+        return;
+    }
+    if (b == blocks) {
+        jp[blocks] = m;
+        return;
+    }
+    const int64_t total = (int64_t)rp[m] - (int64_t)rp[0] + m;
+    const int64_t target = (int64_t)(((__int128)total * b) / blocks) + (int64_t)rp[0];
+    // first i in [0, m] with rp[i] + i >= target (rp[i] + i is strictly increasing)
+    int lo = 0, hi = m;
+    while (lo < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if ((int64_t)rp[mid] + mid >= target)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    jp[b] = lo;
+}
+
+// ---------------------------------------------------------------------------
+// Host-side dispatch.
+// ---------------------------------------------------------------------------
+static int pow2ceil(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+constexpr int kRowsPerTeam = 8;
+
+static int spmv_team_width(int32_t m, int64_t nnz) {
+    const double avg = m > 0 ? (double)nnz / m : 0.0;
+    int t = pow2ceil((int)std::ceil(avg));
+    return std::min(64, std::max(4, t));
+}
+
+static int rows_per_block_for(int32_t m, int64_t nnz, int32_t nrhs) {
+    if (nrhs == 1) return (kBlock / spmv_team_width(m, nnz)) * kRowsPerTeam;
+    return (kBlock / kWave) * kRowsPerTeam;
+}
+
+template <typename T, typename RP>
+static int launch_spmv(int32_t m, int64_t nnz, const RP* rp, const int32_t* ci, const T* a, T alpha,
+                       const T* X, int64_t ldX, T beta, T* Y, int64_t ldY, int alpha_one, int beta_mode,
+                       const int32_t* job_ptr, int32_t blocks, hipStream_t st) {
+    const int team = spmv_team_width(m, nnz);
+    const int rpb = (kBlock / team) * kRowsPerTeam;
+    const int grid = job_ptr ? blocks : (m + rpb - 1) / rpb;
+    if (grid <= 0) return TG_OK;
+#define TG_SPMV(TM)                                                                                     \
+    case TM:                                                                                            \
+        hipLaunchKernelGGL((spmv_team_kernel<T, RP, TM>), dim3(grid), dim3(kBlock), 0, st, m, rp, ci, a, \
+                           alpha, X, ldX, beta, Y, ldY, alpha_one, beta_mode, job_ptr, rpb);            \
+        break;
+    switch (team) {
+        TG_SPMV(4)
+        TG_SPMV(8)
+        TG_SPMV(16)
+        TG_SPMV(32)
+        TG_SPMV(64)
+        default: return fail(TG_ERR_INVALID, "internal: bad team width %d", team);
+    }
+#undef TG_SPMV
+    TG_LAUNCH_CHECK();
+    return TG_OK;
+}
+
+template <typename T, typename RP, int V>
+static int launch_spmm_v(int G, int32_t m, const RP* rp, const int32_t* ci, const T* a, T alpha, const T* X,
+                         int64_t ldX, T beta, T* Y, int64_t ldY, int32_t nrhs, int alpha_one, int beta_mode,
+                         const int32_t* job_ptr, int32_t blocks, hipStream_t st) {
+    const int rpb = (kBlock / kWave) * kRowsPerTeam;
+    const int grid = job_ptr ? blocks : (m + rpb - 1) / rpb;
+    if (grid <= 0) return TG_OK;
+#define TG_SPMM(GG)                                                                                      \
+    case GG:                                                                                             \
+        hipLaunchKernelGGL((spmm_wave_kernel<T, RP, GG, V>), dim3(grid), dim3(kBlock), 0, st, m, rp, ci, a, \
+                           alpha, X, ldX, beta, Y, ldY, nrhs, alpha_one, beta_mode, job_ptr, rpb);      \
+        break;
+    switch (G) {
+        TG_SPMM(1)
+        TG_SPMM(2)
+        TG_SPMM(4)
+        TG_SPMM(8)
+        TG_SPMM(16)
+        TG_SPMM(32)
+        TG_SPMM(64)
+        default: return fail(TG_ERR_INVALID, "internal: bad lane group %d", G);
+    }
+#undef TG_SPMM
+    TG_LAUNCH_CHECK();
+    return TG_OK;
+}
+
+template <typename T, typename RP>
+static int spmm_typed(int32_t m, int64_t nnz, const RP* rp, const int32_t* ci, const T* a, double alpha_d,
+                      const T* X, int64_t ldX, double beta_d, T* Y, int64_t ldY, int32_t nrhs,
+                      const int32_t* job_ptr, int32_t blocks, hipStream_t st) {
+    const T alpha = (T)alpha_d, beta = (T)beta_d;
+    const int alpha_one = (alpha_d == 1.0);
+    const int beta_mode = (beta_d == 0.0) ? kBetaZero : (beta_d == 1.0) ? kBetaOne : kBetaGeneric;
+    if (nrhs == 1)
+        return launch_spmv<T, RP>(m, nnz, rp, ci, a, alpha, X, ldX, beta, Y, ldY, alpha_one, beta_mode, job_ptr,
+                                  blocks, st);
+    // Vector width: 16-byte loads when X and Y rows allow it.
+    const auto aligned = [&](int bytes) {
+        return ((uintptr_t)X % bytes == 0) && ((uintptr_t)Y % bytes == 0) &&
+               ((ldX * (int64_t)sizeof(T)) % bytes == 0) && ((ldY * (int64_t)sizeof(T)) % bytes == 0);
+    };
+    int V = 1;
+    if (sizeof(T) == 8) {
+        if (nrhs % 2 == 0 && aligned(16)) V = 2;
+    } else {
+        if (nrhs % 4 == 0 && aligned(16))
+            V = 4;
+        else if (nrhs % 2 == 0 && aligned(8))
+            V = 2;
+    }
+    const int G = std::min(64, pow2ceil((nrhs + V - 1) / V));
+    switch (V) {
+        case 1:
+            return launch_spmm_v<T, RP, 1>(G, m, rp, ci, a, alpha, X, ldX, beta, Y, ldY, nrhs, alpha_one,
+                                           beta_mode, job_ptr, blocks, st);
+        case 2:
+            return launch_spmm_v<T, RP, 2>(G, m, rp, ci, a, alpha, X, ldX, beta, Y, ldY, nrhs, alpha_one,
+                                           beta_mode, job_ptr, blocks, st);
+        default:
+            if constexpr (sizeof(T) == 4)
+                return launch_spmm_v<T, RP, 4>(G, m, rp, ci, a, alpha, X, ldX, beta, Y, ldY, nrhs, alpha_one,
+                                               beta_mode, job_ptr, blocks, st);
+            return fail(TG_ERR_INVALID, "internal: bad vector width");
+    }
+}
+
+template <typename T>
+static int axpby_typed(int32_t m, int32_t nrhs, double alpha_d, const T* X, int64_t ldX, double beta_d, T* Y,
+                       int64_t ldY, hipStream_t st) {
+    const int alpha_mode = (alpha_d == 0.0) ? kBetaZero : (alpha_d == 1.0) ? kBetaOne : kBetaGeneric;
+    const int beta_mode = (beta_d == 0.0) ? kBetaZero : (beta_d == 1.0) ? kBetaOne : kBetaGeneric;
+    if (alpha_mode == kBetaZero && beta_mode == kBetaOne) return TG_OK;  // nothing to do
+    const int64_t total = (int64_t)m * nrhs;
+    if (total <= 0) return TG_OK;
+    const int grid = (int)std::min<int64_t>((total + kBlock - 1) / kBlock, 65536);
+    hipLaunchKernelGGL((axpby_kernel<T>), dim3(grid), dim3(kBlock), 0, st, m, nrhs, (T)alpha_d, X, ldX,
+                       (T)beta_d, Y, ldY, alpha_mode, beta_mode);
+    TG_LAUNCH_CHECK();
+    return TG_OK;
+}
+
+}  // namespace tg
+
+using namespace tg;
+
+extern "C" int32_t tg_csr_default_blocks(int32_t m, int64_t nnz, int32_t nrhs) {
+    if (m <= 0) return 0;
+    const int rpb = rows_per_block_for(m, nnz, nrhs < 1 ? 1 : nrhs);
+    return (m + rpb - 1) / rpb;
+}
+
+extern "C" int tg_csr_job_pointers(int32_t m, const void* rp, tg_dtype rp_type, int32_t blocks, int32_t* job_ptr,
+                                   void* stream) {
+    if (m < 0 || blocks < 1 || !rp || !job_ptr) return fail(TG_ERR_INVALID, "tg_csr_job_pointers: bad argument");
+    const int threads = 256, grid = (blocks + 1 + threads - 1) / threads;
+    hipStream_t st = as_stream(stream);
+    if (rp_type == TG_I32)
+        hipLaunchKernelGGL((job_pointers_kernel<int32_t>), dim3(grid), dim3(threads), 0, st, m,
+                           (const int32_t*)rp, blocks, job_ptr);
+    else if (rp_type == TG_I64)
+        hipLaunchKernelGGL((job_pointers_kernel<int64_t>), dim3(grid), dim3(threads), 0, st, m,
+                           (const int64_t*)rp, blocks, job_ptr);
+    else
+        return fail(TG_ERR_DTYPE, "tg_csr_job_pointers: rp_type must be TG_I32 or TG_I64");
+    TG_LAUNCH_CHECK();
+    return TG_OK;
+}
+
+extern "C" int tg_axpby(int32_t m, int32_t nrhs, tg_dtype val_type, double alpha, const void* X, int64_t ldX,
+                        double beta, void* Y, int64_t ldY, void* stream) {
+    if (m < 0 || nrhs < 0 || !Y || ldY < nrhs || (alpha != 0.0 && (!X || ldX < nrhs)))
+        return fail(TG_ERR_INVALID, "tg_axpby: bad argument");
+    hipStream_t st = as_stream(stream);
+    if (val_type == TG_F64) return axpby_typed<double>(m, nrhs, alpha, (const double*)X, ldX, beta, (double*)Y, ldY, st);
+    if (val_type == TG_F32) return axpby_typed<float>(m, nrhs, alpha, (const float*)X, ldX, beta, (float*)Y, ldY, st);
+    return fail(TG_ERR_DTYPE, "tg_axpby: val_type must be TG_F64 or TG_F32");
+}
+
+extern "C" int tg_csr_spmm(int32_t m, int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, const int32_t* ci, const void* a,
+                           tg_dtype val_type, double alpha, const void* X, int64_t ldX, double beta, void* Y,
+                           int64_t ldY, int32_t nrhs, int32_t NRHS, const int32_t* job_ptr, int32_t blocks,
+                           void* stream) {
+    if (NRHS > 0 && nrhs != NRHS)
+        return fail(TG_ERR_NRHS, "tg_csr_spmm: nrhs (%d) != NRHS (%d). Doing nothing.", nrhs, NRHS);
+    if (m < 0 || n < 0 || nnz < 0 || nrhs < 0) return fail(TG_ERR_INVALID, "tg_csr_spmm: negative size");
+    if (val_type != TG_F64 && val_type != TG_F32) return fail(TG_ERR_DTYPE, "tg_csr_spmm: val_type must be F64/F32");
+    if (rp_type != TG_I32 && rp_type != TG_I64) return fail(TG_ERR_DTYPE, "tg_csr_spmm: rp_type must be I32/I64");
+    if (m == 0 || nrhs == 0) return TG_OK;
+    if (!Y || ldY < nrhs) return fail(TG_ERR_INVALID, "tg_csr_spmm: Y is null or ldY < nrhs");
+    if (job_ptr && blocks < 1) return fail(TG_ERR_INVALID, "tg_csr_spmm: job_ptr given with blocks < 1");
+    hipStream_t st = as_stream(stream);
+    if (alpha == 0.0) {  // SpMM.hpp:65-71, SpMV.hpp:15-30
+        if (val_type == TG_F64) return axpby_typed<double>(m, nrhs, 0.0, nullptr, 0, beta, (double*)Y, ldY, st);
+        return axpby_typed<float>(m, nrhs, 0.0, nullptr, 0, beta, (float*)Y, ldY, st);
+    }
+    if (X == Y) return fail(TG_ERR_ALIAS, "tg_csr_spmm: input and output pointer coincide. Doing nothing.");
+    if (!rp || !X || ldX < nrhs) return fail(TG_ERR_INVALID, "tg_csr_spmm: rp/X null or ldX < nrhs");
+    if (nnz > 0 && !ci) return fail(TG_ERR_INVALID, "tg_csr_spmm: ci is null");
+    if (val_type == TG_F64) {
+        if (rp_type == TG_I32)
+            return spmm_typed<double, int32_t>(m, nnz, (const int32_t*)rp, ci, (const double*)a, alpha,
+                                               (const double*)X, ldX, beta, (double*)Y, ldY, nrhs, job_ptr, blocks, st);
+        return spmm_typed<double, int64_t>(m, nnz, (const int64_t*)rp, ci, (const double*)a, alpha, (const double*)X,
+                                           ldX, beta, (double*)Y, ldY, nrhs, job_ptr, blocks, st);
+    }
+    if (rp_type == TG_I32)
+        return spmm_typed<float, int32_t>(m, nnz, (const int32_t*)rp, ci, (const float*)a, alpha, (const float*)X,
+                                          ldX, beta, (float*)Y, ldY, nrhs, job_ptr, blocks, st);
+    return spmm_typed<float, int64_t>(m, nnz, (const int64_t*)rp, ci, (const float*)a, alpha, (const float*)X, ldX,
+                                      beta, (float*)Y, ldY, nrhs, job_ptr, blocks, st);
+}

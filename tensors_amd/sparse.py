@@ -1,0 +1,146 @@
+"""Python mirror of ``Tensors::Sparse::MatrixCSR`` over the gfx950 C ABI.
+
+Host arrays stay authoritative (as in the reference); a device mirror of the row
+pointers, column indices and values is made on first use and reused, like the lazily
+built job pointers of ``PatternCSR::RequireJobPtr`` (src/Sparse/PatternCSR.hpp:389-398).
+Dense blocks passed to ``Dot`` are torch CUDA tensors (row-major, a leading dimension
+given by their row stride).  torch only supplies device memory and streams here.
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import torch
+
+from . import _capi as capi
+
+_TORCH_TO_TG = {torch.float64: capi.F64, torch.float32: capi.F32}
+
+
+def stream_handle(stream=None) -> int:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+def _eprint(msg: str) -> None:
+    print(msg, file=sys.stderr)
+
+
+def spmm_raw(m, n, nnz, rp, ci, a, alpha, X, ldX, beta, Y, ldY, nrhs, NRHS=0, job_ptr=None, blocks=0,
+             stream=None):
+    """Direct tg_csr_spmm on torch CUDA tensors; returns the status (no raise for the
+    reference's no-op cases TG_ERR_ALIAS / TG_ERR_NRHS)."""
+    rp_type = capi.I64 if rp.dtype == torch.int64 else capi.I32
+    val_type = _TORCH_TO_TG[Y.dtype]
+    st = capi.lib().tg_csr_spmm(
+        m, n, nnz, rp.data_ptr(), rp_type, ci.data_ptr() if ci is not None and ci.numel() else None,
+        a.data_ptr() if a is not None else None, val_type, float(alpha),
+        X.data_ptr() if X is not None else None, ldX, float(beta), Y.data_ptr(), ldY, nrhs, NRHS,
+        job_ptr.data_ptr() if job_ptr is not None else None, blocks, stream_handle(stream))
+    if st in (capi.ERR_ALIAS, capi.ERR_NRHS):
+        return st
+    return capi.check(st, "tg_csr_spmm")
+
+
+class MatrixCSR:
+    """m x n CSR matrix: ``outer`` (m+1, int32/int64), ``inner`` (nnz, int32), ``values``
+    (nnz, float64/float32, or None for a pattern/binary matrix).
+
+    Mirrors MatrixCSR<Scal,Int,LInt> (src/Sparse/MatrixCSR.hpp:9-10, 87-100) and its Dot
+    overloads (src/Sparse/MatrixCSR.hpp:965-1086)."""
+
+    def __init__(self, outer, inner, values, m, n, thread_count=1, device="cuda"):
+        self.m, self.n = int(m), int(n)
+        self.outer = np.ascontiguousarray(outer)
+        self.inner = np.ascontiguousarray(inner, dtype=np.int32)
+        self.values = None if values is None else np.ascontiguousarray(values)
+        self.thread_count = int(thread_count)
+        self.device = device
+        self._dev = None
+        self._job_ptr = None
+
+    # -- accessors (PatternCSR.hpp / MatrixCSR.hpp) --------------------------------
+    def RowCount(self): return self.m
+    def ColCount(self): return self.n
+    def NonzeroCount(self): return int(self.outer[self.m]) if self.m >= 0 else 0
+    def ThreadCount(self): return self.thread_count
+    def Outer(self): return self.outer
+    def Inner(self): return self.inner
+    def Values(self): return self.values
+
+    def SetThreadCount(self, t):
+        # the GPU path ignores the host thread count; kept for API parity
+        self.thread_count = int(t)
+
+    def WellFormedQ(self):  # PatternCSR.hpp:1146-1167
+        return self.outer.size > 1 and int(self.outer[-1]) > 0
+
+    # -- device mirror -------------------------------------------------------------
+    def device_arrays(self, dtype=None):
+        """(rp, ci, values) torch CUDA tensors; values converted to `dtype` if given."""
+        if self._dev is None:
+            rp = torch.from_numpy(self.outer).to(self.device)
+            ci = torch.from_numpy(self.inner).to(self.device)
+            vals = None if self.values is None else torch.from_numpy(self.values).to(self.device)
+            self._dev = {"rp": rp, "ci": ci, "vals": {}}
+            if vals is not None:
+                self._dev["vals"][vals.dtype] = vals
+        vals = None
+        if self.values is not None:
+            cache = self._dev["vals"]
+            want = dtype if dtype is not None else next(iter(cache))
+            if want not in cache:
+                cache[want] = next(iter(cache.values())).to(want)
+            vals = cache[want]
+        return self._dev["rp"], self._dev["ci"], vals
+
+    def JobPtr(self, nrhs=1):
+        """Device row partition (tg_csr_job_pointers), cached per nrhs."""
+        if self._job_ptr is None:
+            self._job_ptr = {}
+        if nrhs not in self._job_ptr:
+            rp, _, _ = self.device_arrays()
+            blocks = capi.lib().tg_csr_default_blocks(self.m, self.NonzeroCount(), nrhs)
+            jp = torch.empty(max(blocks, 0) + 1, dtype=torch.int32, device=self.device)
+            if blocks > 0:
+                capi.call("tg_csr_job_pointers", self.m, rp.data_ptr(),
+                          capi.I64 if rp.dtype == torch.int64 else capi.I32, blocks, jp.data_ptr(),
+                          stream_handle())
+            self._job_ptr[nrhs] = (jp, blocks)
+        return self._job_ptr[nrhs]
+
+    # -- products ----------------------------------------------------------------
+    def Dot(self, alpha, X, beta, Y, nrhs=None, NRHS=0, ext_values=None, use_job_ptr=False, stream=None):
+        """Y <- alpha * A X + beta * Y on CUDA tensors.  1-D X/Y: the Tensor1 overload
+        (MatrixCSR.hpp:987-1003); 2-D: the Tensor2 overload (:1006-1022) with nrhs =
+        X.shape[1].  A shape mismatch prints the reference's message and does nothing."""
+        if X.dim() == 1:
+            if X.shape[0] != self.n or Y.shape[0] != self.m:
+                _eprint(self._name("Dot") + ": shapes of matrix, input, and output do not match.")
+                return capi.ERR_SHAPE
+            k, ldX, ldY = 1, 1, 1
+        else:
+            if X.shape[0] != self.n or Y.shape[0] != self.m or X.shape[1] != Y.shape[1]:
+                _eprint(self._name("Dot") + ": shapes of matrix, input, and output do not match.")
+                return capi.ERR_SHAPE
+            k = X.shape[1] if nrhs is None else int(nrhs)
+            ldX, ldY = X.stride(0), Y.stride(0)
+        if not self.WellFormedQ():
+            _eprint(self._name("Dot_") + ": Not WellFormedQ(). Doing nothing.")
+        rp, ci, vals = self.device_arrays(Y.dtype)
+        if ext_values is not None:
+            vals = ext_values
+        jp, blocks = self.JobPtr(k) if use_job_ptr else (None, 0)
+        st = spmm_raw(self.m, self.n, self.NonzeroCount(), rp, ci, vals, alpha, X, ldX, beta, Y, ldY, k,
+                      NRHS, jp, blocks, stream)
+        if st == capi.ERR_ALIAS:
+            _eprint(self._name("Dot") + ": Input and output pointer coincide. This is not safe. Aborting.")
+        elif st == capi.ERR_NRHS:
+            _eprint(self._name("Dot") + ": nrhs != NRHS. Doing nothing.")
+        return st
+
+    def _name(self, method):
+        vt = "Real64" if self.values is None or self.values.dtype == np.float64 else "Real32"
+        it = "Int64" if self.outer.dtype == np.int64 else "Int32"
+        return f"Sparse::MatrixCSR<{vt},Int32,{it}>::{method}"
