@@ -28,6 +28,12 @@ extern "C" {
 
 typedef enum tg_dtype { TG_F64 = 0, TG_F32 = 1, TG_I32 = 2, TG_I64 = 3 } tg_dtype;
 
+/* Operator callback used by the *_op solver entry points: y <- Op(x) on
+ * n x nrhs row-major DEVICE blocks (ld = nrhs).  The library calls it from the
+ * host thread that called the solver; the callback must enqueue its work on
+ * `stream` (or finish it before returning).  Non-zero return = failure. */
+typedef int (*tg_apply_fn)(void* ctx, const void* x, void* y, int32_t nrhs, void* stream);
+
 typedef enum tg_status {
     TG_OK = 0,
     TG_ERR_INVALID = -1,  /* bad argument (negative size, null pointer, ld < nrhs, ...) */
@@ -89,6 +95,75 @@ int tg_csr_spmm(int32_t m, int32_t n, int64_t nnz, const void* rp, tg_dtype rp_t
  * ConjugateGradient.hpp:342-346 and GMRES.hpp:835-845. */
 int tg_axpby(int32_t m, int32_t nrhs, tg_dtype val_type, double alpha, const void* X, int64_t ldX,
              double beta, void* Y, int64_t ldY, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Block preconditioned conjugate gradient -- replaces
+ * ConjugateGradient<NRHS,Scal,Int>::operator()(A, P, a, B, ldB, b, X, ldX,
+ * tol, use_initial_guessQ) (ConjugateGradient.hpp:105-353) for the case where
+ * A is a CSR matrix and P is the identity (prec_kind 0) or a diagonal scaling
+ * z_i = prec_diag[i] * r_i (prec_kind 1, e.g. a Jacobi / lumped-mass inverse).
+ *
+ *   X <- a_coef * A^{-1} B + b_coef * X        (n x nrhs blocks, row-major)
+ *
+ * Same recurrences, per-column scalars and stopping rule as the reference:
+ * all columns satisfy |r.z|_k <= tol^2 |b.Pb|_k, or max_iter iterations.
+ * Special cases as the reference: a_coef == 0 or B == 0 -> X <- b_coef X,
+ * succeeded; NaN in B -> nothing written, not succeeded.  x starts at 0.
+ * workspace: device memory of tg_cg_workspace_bytes() bytes (not zeroed).
+ * Synchronises `stream` (the host polls convergence between batches of
+ * iterations); *succeeded, *iterations and rel_residuals[nrhs] (sqrt(|r.z| /
+ * |b.Pb|) per column, as ConjugateGradient::RelativeResiduals) are host
+ * outputs.  nrhs <= 256.
+ * ------------------------------------------------------------------------ */
+size_t tg_cg_workspace_bytes(int32_t n, int32_t nrhs, tg_dtype val_type, int32_t prec_kind);
+/* The generic-operator form: A and P are callables, as in the reference's
+ * A(cptr x, mptr y) / P(cptr x, mptr y) contract (ConjugateGradient.hpp:
+ * 120-128), here called with device pointers (apply_P NULL = identity).
+ * Workspace: tg_cg_workspace_bytes(n, nrhs, val_type, apply_P != NULL). */
+int tg_cg_solve_op(int32_t n, tg_apply_fn apply_A, void* ctx_A, tg_apply_fn apply_P, void* ctx_P,
+                   tg_dtype val_type, int32_t nrhs, int32_t max_iter, double a_coef, const void* B, int64_t ldB,
+                   double b_coef, void* X, int64_t ldX, double tol, int32_t use_guess, void* workspace,
+                   size_t workspace_bytes, int32_t* succeeded, int32_t* iterations, double* rel_residuals,
+                   void* stream);
+int tg_cg_solve(int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, const int32_t* ci, const void* a,
+                tg_dtype val_type, int32_t prec_kind, const void* prec_diag, int32_t nrhs, int32_t max_iter,
+                double a_coef, const void* B, int64_t ldB, double b_coef, void* X, int64_t ldX, double tol,
+                int32_t use_guess, void* workspace, size_t workspace_bytes, int32_t* succeeded,
+                int32_t* iterations, double* rel_residuals, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Restarted GMRES(m) -- replaces GMRES<NRHS,Scal,Int,Side>::operator()(A, P,
+ * a, B, ldB, b, X, ldX, tol, max_restarts, use_initial_guessQ)
+ * (GMRES.hpp:127-260) for a CSR operator and P = identity (prec_kind 0) or a
+ * diagonal scaling (prec_kind 1); side 0 = Side::Left (default), 1 = Right.
+ *
+ *   X <- a_coef * A^{-1} B + b_coef * X        (n x nrhs blocks, row-major)
+ *
+ * m = min(max_iter, n + 1) (GMRES.hpp:89), m <= 63; nrhs <= 32.  Stops when
+ * every column has |beta_iter|_k <= tol ||P b||_k; restarts while not
+ * converged and restarts <= max_restarts.  Orthogonalisation is classical
+ * Gram-Schmidt applied twice (the reference applies modified Gram-Schmidt
+ * twice: same projection in exact arithmetic); Hessenberg, Givens rotations
+ * and the triangular solve are FP64 on the device.  Special cases as the
+ * reference: B == 0 -> X <- b X (zeroed if b == 0); NaN in B -> nothing
+ * written.  Synchronises `stream`.  Host outputs: *succeeded, *iterations
+ * (of the last cycle, as GMRES::IterationCount), *restarts
+ * (GMRES::RestartCount) and rel_residuals[nrhs] = |beta_iter| / ||P b||.
+ * ------------------------------------------------------------------------ */
+size_t tg_gmres_workspace_bytes(int32_t n, int32_t max_iter, int32_t nrhs, tg_dtype val_type);
+/* Generic-operator form (callables A and P with device pointers, apply_P
+ * NULL = identity), as the reference's A / P contract (GMRES.hpp:143-151). */
+int tg_gmres_solve_op(int32_t n, tg_apply_fn apply_A, void* ctx_A, tg_apply_fn apply_P, void* ctx_P,
+                      tg_dtype val_type, int32_t side, int32_t nrhs, int32_t max_iter, double a_coef, const void* B,
+                      int64_t ldB, double b_coef, void* X, int64_t ldX, double tol, int32_t max_restarts,
+                      int32_t use_guess, void* workspace, size_t workspace_bytes, int32_t* succeeded,
+                      int32_t* iterations, int32_t* restarts, double* rel_residuals, void* stream);
+int tg_gmres_solve(int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, const int32_t* ci, const void* a,
+                   tg_dtype val_type, int32_t prec_kind, const void* prec_diag, int32_t side, int32_t nrhs,
+                   int32_t max_iter, double a_coef, const void* B, int64_t ldB, double b_coef, void* X,
+                   int64_t ldX, double tol, int32_t max_restarts, int32_t use_guess, void* workspace,
+                   size_t workspace_bytes, int32_t* succeeded, int32_t* iterations, int32_t* restarts,
+                   double* rel_residuals, void* stream);
 
 /* Streaming probes (no reference counterpart): dst <- src over `bytes`
  * (multiple of 16), and a read-only pass over src (sink: one device float that

@@ -33,6 +33,16 @@ SIGNATURES = {
     "tg_csr_spmm": (C.c_int, [i32, i32, i64, vp, C.c_int, vp, vp, C.c_int, dbl, vp, i64, dbl, vp, i64,
                               i32, i32, vp, i32, vp]),
     "tg_axpby": (C.c_int, [i32, i32, C.c_int, dbl, vp, i64, dbl, vp, i64, vp]),
+    "tg_cg_workspace_bytes": (C.c_size_t, [i32, i32, C.c_int, i32]),
+    "tg_cg_solve": (C.c_int, [i32, i64, vp, C.c_int, vp, vp, C.c_int, i32, vp, i32, i32, dbl, vp, i64, dbl, vp,
+                              i64, dbl, i32, vp, C.c_size_t, vp, vp, vp, vp]),
+    "tg_gmres_workspace_bytes": (C.c_size_t, [i32, i32, i32, C.c_int]),
+    "tg_gmres_solve": (C.c_int, [i32, i64, vp, C.c_int, vp, vp, C.c_int, i32, vp, i32, i32, i32, dbl, vp, i64,
+                                 dbl, vp, i64, dbl, i32, i32, vp, C.c_size_t, vp, vp, vp, vp, vp]),
+    "tg_cg_solve_op": (C.c_int, [i32, vp, vp, vp, vp, C.c_int, i32, i32, dbl, vp, i64, dbl, vp, i64, dbl, i32,
+                                 vp, C.c_size_t, vp, vp, vp, vp]),
+    "tg_gmres_solve_op": (C.c_int, [i32, vp, vp, vp, vp, C.c_int, i32, i32, i32, dbl, vp, i64, dbl, vp, i64,
+                                    dbl, i32, i32, vp, C.c_size_t, vp, vp, vp, vp, vp]),
     "tg_probe_copy": (C.c_int, [vp, vp, i64, vp]),
     "tg_probe_read": (C.c_int, [vp, i64, vp, vp]),
     "tg_gen_uniform": (C.c_int, [u64, u64, i64, C.c_int, vp]),
@@ -42,6 +52,8 @@ SIGNATURES = {
     "tg_gen_weighted_grid": (i64, [i32, i32, i32, i32, dbl, u64, vp, vp, vp]),
     "tg_gen_convection_diffusion_2d": (i64, [i32, dbl, dbl, vp, vp, vp]),
 }
+
+APPLY_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p)
 
 _lib = None
 

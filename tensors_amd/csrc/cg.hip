@@ -1,0 +1,554 @@
+// Block preconditioned conjugate gradient on the GPU:
+//   X <- a * A^{-1} B + b * X    for nrhs right-hand sides at once.
+//
+// The recurrences, special cases and stopping rule are those of
+// ConjugateGradient::operator() (ConjugateGradient.hpp:105-353): per-column
+// alpha_k, beta_k, rho_k; stop when every column has |r.z|_k <= tol^2 |b.Pb|_k
+// (:176-181, 470-480) or after max_iter iterations.  One departure: x starts
+// at zero (the reference leaves it uninitialised when the loop never runs
+// without an initial guess, :214-250, 342-346).
+//
+// Layout: every n x nrhs block is row-major with ld = nrhs (the reference's
+// Tensor2 workspaces r, u, p, x, z, :38-42).  Kernels per iteration:
+//   spmm (u = A p)  |  dot(p,u) -> alpha  |  x,r update + z = P r + dot(r,z)
+//   -> rho, beta, convergence  |  p = z + beta p
+// Dots are deterministic: each thread sums its rows in order, the workgroup
+// combines threads in a fixed tree, and a one-workgroup kernel adds the
+// workgroup partials in index order (replacing ComputeScalarProducts +
+// ThreadTensor2::AddReduce, ConjugateGradient.hpp:395-438).
+// Convergence lives on the device: once `done` is set every kernel returns at
+// once, so the host enqueues iterations in batches and polls the flag between
+// batches instead of synchronising every iteration.
+
+#include "common.hpp"
+#include "krylov.hpp"
+
+namespace tg {
+
+__device__ __forceinline__ void block_row_range(int n, int& i0, int& i1) {
+    const int b = blockIdx.x, B = gridDim.x;
+    i0 = (int)((int64_t)n * b / B);
+    i1 = (int)((int64_t)n * (b + 1) / B);
+}
+
+// Thread layout of the column-blocked elementwise kernels: c = column lane,
+// ro = row lane; kBlock / kpad rows are processed per sweep.
+struct Lanes {
+    int c, ro, rows;
+};
+__device__ __forceinline__ Lanes lanes_of(int kpad) {
+    Lanes L;
+    L.c = threadIdx.x % kpad;
+    L.ro = threadIdx.x / kpad;
+    L.rows = kBlock / kpad;
+    return L;
+}
+
+// Reduce per-thread partials of the same column over the row lanes (fixed
+// tree) and store the workgroup partial partial[block * k + c].
+__device__ __forceinline__ void column_partial(double v, const Lanes& L, int k, int kpad, double* partial) {
+    __shared__ double sh[kBlock];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = L.rows / 2; s > 0; s >>= 1) {
+        if (L.ro < s) sh[threadIdx.x] += sh[threadIdx.x + s * kpad];
+        __syncthreads();
+    }
+    if (L.ro == 0 && L.c < k) partial[(int64_t)blockIdx.x * k + L.c] = sh[L.c];
+}
+
+// Sum of the B workgroup partials of each column in index order, by one
+// workgroup: thread (c, j) adds partials j, j+J, ...; then a fixed tree over j.
+__device__ __forceinline__ double column_total(const double* partial, int B, int k, int kpad, int c, int j,
+                                               int J) {
+    __shared__ double sh[kBlock];
+    double s = 0.0;
+    if (c < k)
+        for (int b = j; b < B; b += J) s += partial[(int64_t)b * k + c];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = J / 2; h > 0; h >>= 1) {
+        if (j < h) sh[threadIdx.x] += sh[threadIdx.x + h * kpad];
+        __syncthreads();
+    }
+    const double t = sh[c];
+    __syncthreads();
+    return t;
+}
+
+// --- setup ----------------------------------------------------------------
+// r = B;  z = P r;  x = 0 (unless guessing);  partial(r.z)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void cg_init_kernel(int n, int k, int kpad, const T* __restrict__ Bm,
+                                                         int64_t ldB, const T* __restrict__ d, T* __restrict__ r,
+                                                         T* __restrict__ z, T* __restrict__ x, int zero_x,
+                                                         int zmode, double* __restrict__ partial) {
+    int i0, i1;
+    block_row_range(n, i0, i1);
+    const Lanes L = lanes_of(kpad);
+    double acc = 0.0;
+    if (L.c < k)
+        for (int i = i0 + L.ro; i < i1; i += L.rows) {
+            const int64_t e = (int64_t)i * k + L.c;
+            const T rv = Bm[(int64_t)i * ldB + L.c];
+            r[e] = rv;
+            T zv = rv;
+            if (d) {
+                zv = d[i] * rv;
+                z[e] = zv;
+            }
+            if (zero_x) x[e] = T(0);
+            if (zmode != 2) acc += (double)rv * (double)zv;
+        }
+    column_partial(acc, L, k, kpad, partial);
+}
+
+// With an initial guess: r -= u (u = A x);  z = P r;  partial(r.z)   (:214-246)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void cg_guess_kernel(int n, int k, int kpad, const T* __restrict__ u,
+                                                          const T* __restrict__ d, T* __restrict__ r,
+                                                          T* __restrict__ z, int zmode, double* __restrict__ partial) {
+    int i0, i1;
+    block_row_range(n, i0, i1);
+    const Lanes L = lanes_of(kpad);
+    double acc = 0.0;
+    if (L.c < k)
+        for (int i = i0 + L.ro; i < i1; i += L.rows) {
+            const int64_t e = (int64_t)i * k + L.c;
+            const T rv = r[e] - u[e];
+            r[e] = rv;
+            T zv = rv;
+            if (d) {
+                zv = d[i] * rv;
+                z[e] = zv;
+            }
+            if (zmode != 2) acc += (double)rv * (double)zv;
+        }
+    column_partial(acc, L, k, kpad, partial);
+}
+
+// mode 0: rho = sum, bsq = |rho|, TOL = bsq * tol^2, nan/zero flags (:171-211)
+// mode 1: rho = sum (after the initial-guess residual)
+__global__ __launch_bounds__(kBlock) void cg_setup_finalize_kernel(int k, int kpad, int B, int mode, double tol2,
+                                                                   const double* __restrict__ partial,
+                                                                   CGState* __restrict__ st) {
+    const int c = threadIdx.x % kpad, j = threadIdx.x / kpad, J = kBlock / kpad;
+    const double tot = column_total(partial, B, k, kpad, c, j, J);
+    if (j == 0 && c < k) {
+        st->rho[c] = tot;
+        if (mode == 0) {
+            st->bsq[c] = fabs(tot);
+            st->TOL[c] = fabs(tot) * tol2;
+        }
+    }
+}
+
+// After p = z: succeeded = all |rho| <= TOL; done if succeeded or max_iter == 0.
+__global__ void cg_check_kernel(int k, CGState* __restrict__ st) {
+    if (threadIdx.x != 0) return;
+    bool ok = true;
+    for (int c = 0; c < k; ++c) ok = ok && (fabs(st->rho[c]) <= st->TOL[c]);
+    st->succeeded = ok ? 1 : 0;
+    st->iter = 0;
+    st->done = (ok || st->max_iter <= 0) ? 1 : 0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void cg_copy_kernel(int64_t count, const T* __restrict__ s, T* __restrict__ d) {
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < count; e += (int64_t)gridDim.x * kBlock)
+        d[e] = s[e];
+}
+
+// x = X (row-major ld) for the initial guess
+template <typename T>
+__global__ __launch_bounds__(kBlock) void cg_load_guess_kernel(int n, int k, const T* __restrict__ X, int64_t ldX,
+                                                               T* __restrict__ x) {
+    const int64_t total = (int64_t)n * k;
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < total; e += (int64_t)gridDim.x * kBlock) {
+        const int64_t i = e / k;
+        x[e] = X[i * ldX + (e - i * k)];
+    }
+}
+
+// --- iteration --------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void cg_dot_pu_kernel(int n, int k, int kpad, const T* __restrict__ p,
+                                                           const T* __restrict__ u, double* __restrict__ partial,
+                                                           const CGState* __restrict__ st) {
+    if (st && st->done) return;
+    int i0, i1;
+    block_row_range(n, i0, i1);
+    const Lanes L = lanes_of(kpad);
+    double acc = 0.0;
+    if (L.c < k)
+        for (int i = i0 + L.ro; i < i1; i += L.rows) {
+            const int64_t e = (int64_t)i * k + L.c;
+            acc += (double)p[e] * (double)u[e];
+        }
+    column_partial(acc, L, k, kpad, partial);
+}
+
+// alpha = rho / (p.u)   (:255-260)
+__global__ __launch_bounds__(kBlock) void cg_alpha_kernel(int k, int kpad, int B, const double* __restrict__ partial,
+                                                          CGState* __restrict__ st) {
+    if (st->done) return;
+    const int c = threadIdx.x % kpad, j = threadIdx.x / kpad, J = kBlock / kpad;
+    const double pu = column_total(partial, B, k, kpad, c, j, J);
+    if (j == 0 && c < k) st->alpha[c] = st->rho[c] / pu;
+}
+
+// x = (first ? 0 : x) + alpha p;  r -= alpha u;  z = P r;  partial(r.z)   (:262-313)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void cg_update_kernel(int n, int k, int kpad, const T* __restrict__ p,
+                                                           const T* __restrict__ u, const T* __restrict__ d,
+                                                           T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
+                                                           int zmode, double* __restrict__ partial,
+                                                           const CGState* __restrict__ st) {
+    if (st->done) return;
+    int i0, i1;
+    block_row_range(n, i0, i1);
+    const Lanes L = lanes_of(kpad);
+    const bool first = (st->iter == 0) && !st->use_guess;
+    double acc = 0.0;
+    if (L.c < k) {
+        const T al = (T)st->alpha[L.c];
+        for (int i = i0 + L.ro; i < i1; i += L.rows) {
+            const int64_t e = (int64_t)i * k + L.c;
+            const T pv = p[e];
+            x[e] = first ? al * pv : x[e] + al * pv;
+            const T rv = r[e] - al * u[e];
+            r[e] = rv;
+            T zv = rv;
+            if (d) {
+                zv = d[i] * rv;
+                z[e] = zv;
+            }
+            if (zmode != 2) acc += (double)rv * (double)zv;
+        }
+    }
+    column_partial(acc, L, k, kpad, partial);
+}
+
+// rho_old = rho; rho = r.z; beta = rho / rho_old; convergence; ++iter  (:309-338)
+__global__ __launch_bounds__(kBlock) void cg_rho_kernel(int k, int kpad, int B, const double* __restrict__ partial,
+                                                        CGState* __restrict__ st) {
+    if (st->done) return;
+    const int c = threadIdx.x % kpad, j = threadIdx.x / kpad, J = kBlock / kpad;
+    const double rz = column_total(partial, B, k, kpad, c, j, J);
+    __shared__ int conv;
+    if (threadIdx.x == 0) conv = 1;
+    __syncthreads();
+    if (j == 0 && c < k) {
+        const double old = st->rho[c];
+        st->rho_old[c] = old;
+        st->rho[c] = rz;
+        st->beta[c] = rz / old;
+        if (!(fabs(rz) <= st->TOL[c])) atomicAnd(&conv, 0);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        st->succeeded = conv;
+        st->iter = st->iter + 1;
+        st->done = (conv || st->iter >= st->max_iter) ? 1 : 0;
+    }
+}
+
+// p = z + beta p   (:321-335)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void cg_p_kernel(int n, int k, int kpad, const T* __restrict__ z,
+                                                      T* __restrict__ p, const CGState* __restrict__ st) {
+    if (st->done) return;
+    int i0, i1;
+    block_row_range(n, i0, i1);
+    const Lanes L = lanes_of(kpad);
+    if (L.c < k) {
+        const T be = (T)st->beta[L.c];
+        for (int i = i0 + L.ro; i < i1; i += L.rows) {
+            const int64_t e = (int64_t)i * k + L.c;
+            p[e] = z[e] + be * p[e];
+        }
+    }
+}
+
+// X <- a x + b X  with combine_matrices_auto's shortcuts (:342-346)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void cg_write_kernel(int n, int k, T a, const T* __restrict__ x, T b,
+                                                          T* __restrict__ X, int64_t ldX, int a_one, int b_mode) {
+    const int64_t total = (int64_t)n * k;
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < total; e += (int64_t)gridDim.x * kBlock) {
+        const int64_t i = e / k;
+        T* Xp = X + i * ldX + (e - i * k);
+        T v = a_one ? x[e] : a * x[e];
+        if (b_mode == 1)
+            v = v + *Xp;
+        else if (b_mode == 2)
+            v = v + b * *Xp;
+        *Xp = v;
+    }
+}
+
+// X <- b X literally (scale_matrix at :151-153, :201-203)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void scale_kernel(int n, int k, T b, T* __restrict__ X, int64_t ldX) {
+    const int64_t total = (int64_t)n * k;
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < total; e += (int64_t)gridDim.x * kBlock) {
+        const int64_t i = e / k;
+        X[i * ldX + (e - i * k)] *= b;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host driver.
+// ---------------------------------------------------------------------------
+static int pow2ceil_i(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+size_t cg_workspace_bytes(int32_t n, int32_t k, size_t vs, int prec_kind) {
+    const size_t vec = align256((size_t)n * k * vs);
+    const size_t nvec = prec_kind ? 5 : 4;  // r, u, p, x (+ z)
+    return align256(sizeof(CGState)) + align256((size_t)kSolverBlocks * k * sizeof(double)) + nvec * vec;
+}
+
+template <typename T>
+static int grid_for(int64_t count) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>((count + kBlock - 1) / kBlock, 8192));
+}
+
+template <typename T>
+int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, const OpRef& Pop, int32_t k,
+               int32_t max_iter, double a, const T* Bm, int64_t ldB, double b, T* X, int64_t ldX, double tol,
+               int use_guess, void* ws, size_t ws_bytes, CGResult* res, hipStream_t s) {
+    const int n = A.n;
+    res->succeeded = 0;
+    res->iterations = 0;
+    if (k < 1 || k > kBlock) return fail(TG_ERR_INVALID, "cg: nrhs must be in [1, %d]", kBlock);
+    if (n < 0 || A.m != n) return fail(TG_ERR_SHAPE, "cg: the operator must be square");
+    if (!ws || ws_bytes < cg_workspace_bytes(n, k, sizeof(T), prec_kind))
+        return fail(TG_ERR_WORKSPACE, "cg: workspace missing or smaller than tg_cg_workspace_bytes()");
+    if (prec_kind == 1 && !d) return fail(TG_ERR_INVALID, "cg: diagonal preconditioner without a diagonal");
+    if (prec_kind == 2 && !Pop.fn) return fail(TG_ERR_INVALID, "cg: callback preconditioner without a function");
+    max_iter = std::max(0, max_iter);
+    const int64_t N = (int64_t)n * k;
+    if (a == 0.0) {  // :147-163
+        if (N > 0) hipLaunchKernelGGL((scale_kernel<T>), dim3(grid_for<T>(N)), dim3(kBlock), 0, s, n, k, (T)b, X, ldX);
+        TG_LAUNCH_CHECK();
+        res->succeeded = 1;
+        return TG_OK;
+    }
+    char* w = (char*)ws;
+    CGState* st = (CGState*)w;
+    w += align256(sizeof(CGState));
+    double* partial = (double*)w;
+    w += align256((size_t)kSolverBlocks * k * sizeof(double));
+    const size_t vec = align256((size_t)N * sizeof(T));
+    T* r = (T*)w;
+    T* u = (T*)(w + vec);
+    T* p = (T*)(w + 2 * vec);
+    T* x = (T*)(w + 3 * vec);
+    T* z = prec_kind ? (T*)(w + 4 * vec) : r;
+    const T* dd = prec_kind == 1 ? d : nullptr;
+    const int zmode = prec_kind;  // 0: z is r, 1: z = d r in the kernels, 2: z = P(r) by callback
+    const int kpad = pow2ceil_i(k);
+    const int B = kSolverBlocks;
+    const bool host_ops = Aop.is_callback() || prec_kind == 2;
+
+    // z = P r by callback, then partial(r.z) -- the external-preconditioner step
+    auto apply_P = [&]() -> int {
+        int rc = op_apply(Pop, r, z, k, nullptr, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL((cg_dot_pu_kernel<T>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, (const T*)r, (const T*)z,
+                           partial, (const CGState*)nullptr);
+        return TG_OK;
+    };
+
+    CGState h{};
+    h.max_iter = max_iter;
+    h.use_guess = use_guess ? 1 : 0;
+    TG_HIP(hipMemcpyAsync(st, &h, sizeof(CGState), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL((cg_init_kernel<T>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, Bm, ldB, dd, r, z, x,
+                       use_guess ? 0 : 1, zmode, partial);
+    if (zmode == 2) {
+        int rc = apply_P();
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(cg_setup_finalize_kernel, dim3(1), dim3(kBlock), 0, s, k, kpad, B, 0, tol * tol, partial, st);
+    TG_LAUNCH_CHECK();
+    CGState hs;
+    TG_HIP(hipMemcpyAsync(&hs, st, sizeof(CGState), hipMemcpyDeviceToHost, s));
+    TG_HIP(hipStreamSynchronize(s));
+    double bmax = 0.0;
+    for (int c = 0; c < k; ++c) {
+        if (std::isnan(hs.bsq[c])) {  // :185-195
+            fail(TG_OK, "cg: right-hand side contains NaNs. Doing nothing.");
+            return TG_OK;
+        }
+        bmax = std::max(bmax, hs.bsq[c]);
+    }
+    if (!(bmax > 0.0)) {  // :197-211
+        if (N > 0) hipLaunchKernelGGL((scale_kernel<T>), dim3(grid_for<T>(N)), dim3(kBlock), 0, s, n, k, (T)b, X, ldX);
+        TG_LAUNCH_CHECK();
+        res->succeeded = 1;
+        return TG_OK;
+    }
+    if (use_guess) {
+        hipLaunchKernelGGL((cg_load_guess_kernel<T>), dim3(grid_for<T>(N)), dim3(kBlock), 0, s, n, k, X, ldX, x);
+        int rc = op_apply(Aop, x, u, k, nullptr, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL((cg_guess_kernel<T>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, u, dd, r, z, zmode, partial);
+        if (zmode == 2) {
+            rc = apply_P();
+            if (rc) return rc;
+        }
+        hipLaunchKernelGGL(cg_setup_finalize_kernel, dim3(1), dim3(kBlock), 0, s, k, kpad, B, 1, 0.0, partial, st);
+    }
+    hipLaunchKernelGGL((cg_copy_kernel<T>), dim3(grid_for<T>(N)), dim3(kBlock), 0, s, N, (const T*)z, p);
+    hipLaunchKernelGGL(cg_check_kernel, dim3(1), dim3(64), 0, s, k, st);
+    TG_LAUNCH_CHECK();
+
+    // Iterations.  With device operators every kernel checks `done` itself and
+    // the host polls in growing batches; host callbacks cannot check it, so
+    // then the host polls after every iteration.
+    int launched = 0, batch = host_ops ? 1 : 4;
+    int hflag[2] = {0, 0};
+    TG_HIP(hipMemcpyAsync(hflag, &st->iter, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    TG_HIP(hipStreamSynchronize(s));
+    int rc = TG_OK;
+    while (!hflag[1] && launched < max_iter) {
+        for (int it = 0; it < batch && launched < max_iter; ++it, ++launched) {
+            rc = op_apply(Aop, p, u, k, &st->done, s);
+            if (rc) break;
+            hipLaunchKernelGGL((cg_dot_pu_kernel<T>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, (const T*)p,
+                               (const T*)u, partial, (const CGState*)st);
+            hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kBlock), 0, s, k, kpad, B, (const double*)partial, st);
+            hipLaunchKernelGGL((cg_update_kernel<T>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, (const T*)p,
+                               (const T*)u, dd, x, r, z, zmode, partial, (const CGState*)st);
+            if (zmode == 2) {
+                rc = apply_P();
+                if (rc) break;
+            }
+            hipLaunchKernelGGL(cg_rho_kernel, dim3(1), dim3(kBlock), 0, s, k, kpad, B, (const double*)partial, st);
+            hipLaunchKernelGGL((cg_p_kernel<T>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, (const T*)z, p,
+                               (const CGState*)st);
+        }
+        if (rc) break;
+        if (hipGetLastError() != hipSuccess) {
+            rc = fail(TG_ERR_HIP, "cg: kernel launch failed");
+            break;
+        }
+        if (hipMemcpyAsync(hflag, &st->iter, 2 * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+            rc = fail(TG_ERR_HIP, "cg: polling the convergence flag failed");
+            break;
+        }
+        if (!host_ops) batch = std::min(batch * 2, 64);
+    }
+    if (rc) return rc;
+    const int a_one = (a == 1.0);
+    const int b_mode = (b == 0.0) ? 0 : (b == 1.0) ? 1 : 2;
+    hipLaunchKernelGGL((cg_write_kernel<T>), dim3(grid_for<T>(N)), dim3(kBlock), 0, s, n, k, (T)a, (const T*)x, (T)b,
+                       X, ldX, a_one, b_mode);
+    TG_LAUNCH_CHECK();
+    TG_HIP(hipMemcpyAsync(&hs, st, sizeof(CGState), hipMemcpyDeviceToHost, s));
+    TG_HIP(hipStreamSynchronize(s));
+    res->succeeded = hs.succeeded;
+    res->iterations = hs.iter;
+    for (int c = 0; c < k && c < kMaxCols; ++c) res->rel_res[c] = std::sqrt(std::fabs(hs.rho[c]) / hs.bsq[c]);
+    return TG_OK;
+}
+
+template int cg_solve_t<double>(const CsrView&, const OpRef&, int, const double*, const OpRef&, int32_t, int32_t,
+                                double, const double*, int64_t, double, double*, int64_t, double, int, void*, size_t,
+                                CGResult*, hipStream_t);
+template int cg_solve_t<float>(const CsrView&, const OpRef&, int, const float*, const OpRef&, int32_t, int32_t,
+                               double, const float*, int64_t, double, float*, int64_t, double, int, void*, size_t,
+                               CGResult*, hipStream_t);
+
+}  // namespace tg
+
+using namespace tg;
+
+extern "C" size_t tg_cg_workspace_bytes(int32_t n, int32_t nrhs, tg_dtype val_type, int32_t prec_kind) {
+    if (n < 0 || nrhs < 1) return 0;
+    return cg_workspace_bytes(n, nrhs, val_type == TG_F32 ? 4 : 8, prec_kind ? 1 : 0);
+}
+
+template <typename T>
+static int cg_entry(const CsrView& A, const OpRef& Aop, int32_t prec_kind, const void* prec_diag, const OpRef& Pop,
+                    int32_t nrhs, int32_t max_iter, double a_coef, const void* B, int64_t ldB, double b_coef, void* X,
+                    int64_t ldX, double tol, int32_t use_guess, void* workspace, size_t workspace_bytes,
+                    int32_t* succeeded, int32_t* iterations, double* rel_residuals, hipStream_t s) {
+    CGResult res{};
+    const int rc = cg_solve_t<T>(A, Aop, prec_kind, (const T*)prec_diag, Pop, nrhs, max_iter, a_coef, (const T*)B,
+                                 ldB, b_coef, (T*)X, ldX, tol, use_guess, workspace, workspace_bytes, &res, s);
+    if (rc) return rc;
+    if (succeeded) *succeeded = res.succeeded;
+    if (iterations) *iterations = res.iterations;
+    if (rel_residuals)
+        for (int c = 0; c < nrhs; ++c) rel_residuals[c] = res.rel_res[c];
+    return TG_OK;
+}
+
+extern "C" int tg_cg_solve(int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, const int32_t* ci, const void* a,
+                           tg_dtype val_type, int32_t prec_kind, const void* prec_diag, int32_t nrhs, int32_t max_iter,
+                           double a_coef, const void* B, int64_t ldB, double b_coef, void* X, int64_t ldX, double tol,
+                           int32_t use_guess, void* workspace, size_t workspace_bytes, int32_t* succeeded,
+                           int32_t* iterations, double* rel_residuals, void* stream) {
+    if (succeeded) *succeeded = 0;
+    if (iterations) *iterations = 0;
+    if (n < 0 || nrhs < 1 || !B || !X || ldB < nrhs || ldX < nrhs || (prec_kind != 0 && prec_kind != 1))
+        return fail(TG_ERR_INVALID, "tg_cg_solve: bad argument");
+    if (rp_type != TG_I32 && rp_type != TG_I64) return fail(TG_ERR_DTYPE, "tg_cg_solve: rp_type must be I32/I64");
+    CsrView A;
+    A.m = n;
+    A.n = n;
+    A.nnz = nnz;
+    A.rp = rp;
+    A.rp_type = rp_type;
+    A.ci = ci;
+    A.val = a;
+    A.val_type = val_type;
+    OpRef Aop;
+    Aop.csr = &A;
+    OpRef Pop;
+    hipStream_t s = as_stream(stream);
+    if (val_type == TG_F64)
+        return cg_entry<double>(A, Aop, prec_kind, prec_diag, Pop, nrhs, max_iter, a_coef, B, ldB, b_coef, X, ldX, tol,
+                                use_guess, workspace, workspace_bytes, succeeded, iterations, rel_residuals, s);
+    if (val_type == TG_F32)
+        return cg_entry<float>(A, Aop, prec_kind, prec_diag, Pop, nrhs, max_iter, a_coef, B, ldB, b_coef, X, ldX, tol,
+                               use_guess, workspace, workspace_bytes, succeeded, iterations, rel_residuals, s);
+    return fail(TG_ERR_DTYPE, "tg_cg_solve: val_type must be F64/F32");
+}
+
+extern "C" int tg_cg_solve_op(int32_t n, tg_apply_fn apply_A, void* ctx_A, tg_apply_fn apply_P, void* ctx_P,
+                              tg_dtype val_type, int32_t nrhs, int32_t max_iter, double a_coef, const void* B,
+                              int64_t ldB, double b_coef, void* X, int64_t ldX, double tol, int32_t use_guess,
+                              void* workspace, size_t workspace_bytes, int32_t* succeeded, int32_t* iterations,
+                              double* rel_residuals, void* stream) {
+    if (succeeded) *succeeded = 0;
+    if (iterations) *iterations = 0;
+    if (n < 0 || nrhs < 1 || !B || !X || ldB < nrhs || ldX < nrhs || !apply_A)
+        return fail(TG_ERR_INVALID, "tg_cg_solve_op: bad argument");
+    CsrView A;  // shape only
+    A.m = n;
+    A.n = n;
+    A.val_type = val_type;
+    OpRef Aop;
+    Aop.fn = apply_A;
+    Aop.ctx = ctx_A;
+    OpRef Pop;
+    Pop.fn = apply_P;
+    Pop.ctx = ctx_P;
+    const int prec_kind = apply_P ? 2 : 0;
+    hipStream_t s = as_stream(stream);
+    if (val_type == TG_F64)
+        return cg_entry<double>(A, Aop, prec_kind, nullptr, Pop, nrhs, max_iter, a_coef, B, ldB, b_coef, X, ldX, tol,
+                                use_guess, workspace, workspace_bytes, succeeded, iterations, rel_residuals, s);
+    if (val_type == TG_F32)
+        return cg_entry<float>(A, Aop, prec_kind, nullptr, Pop, nrhs, max_iter, a_coef, B, ldB, b_coef, X, ldX, tol,
+                               use_guess, workspace, workspace_bytes, succeeded, iterations, rel_residuals, s);
+    return fail(TG_ERR_DTYPE, "tg_cg_solve_op: val_type must be F64/F32");
+}

@@ -1,0 +1,805 @@
+// Restarted GMRES(m) on the GPU:  X <- a * A^{-1} B + b * X  for nrhs columns.
+//
+// Follows GMRES::operator() / Solve / ArnoldiStep / ApplyGivensRotations
+// (GMRES.hpp:127-260, 264-485, 526-601, 603-658): residual convention
+// z = A x - b, left (default) or right diagonal preconditioning, Givens QR of
+// the Hessenberg matrix per column, all columns stop together when every
+// |beta_iter|_k <= tol * ||P b||_k, upper back substitution for y
+// (GMRES.hpp:385-412, the CBLAS trsv of src/BLAS/trsv.hpp:7-48), solution
+// -= Q y, restarts up to max_restarts (:244-250).
+//
+// Orthogonalisation: classical Gram-Schmidt applied twice (CGS2) instead of the
+// reference's modified Gram-Schmidt applied twice (GMRES.hpp:555-585).  In exact
+// arithmetic both give the same projection and the same H (each pass adds its
+// coefficients into H); CGS2 needs O(1) passes over the basis per step instead
+// of O(j) dependent dot/axpy pairs, which is what a GPU wants.  The Krylov
+// vectors live in the vector type T (FP64 or FP32); the Hessenberg matrix,
+// rotations, beta, y and all reductions are FP64.
+//
+// Kernels of Arnoldi step j (q = Q_{j+1}):
+//   spmm (left: z = A Q_j; right: z = P Q_j then q = A z)
+//   dots1:  q = P z (left), h1_i = Q_i.q             i <= j
+//   upd1:   q -= sum h1_i Q_i,  h2_i = Q_i.q
+//   upd2:   q -= sum h2_i Q_i,  |q|^2
+//   givens: H[j+1][j] = |q|, rotations, beta, convergence (one workgroup)
+//   scale:  q /= |q|
+// Every kernel returns at once when the device `done` flag is set, so the host
+// enqueues steps in batches and polls between batches.
+
+#include "common.hpp"
+#include "krylov.hpp"
+
+namespace tg {
+
+constexpr int kGmresMaxCols = 32;  // nrhs limit (LDS of the multi-dot)
+constexpr int kGmresMaxM = 63;     // restart length limit (m + 1 basis vectors <= 64)
+constexpr int kGB = 512;           // workgroups of the vector kernels
+
+struct GmresHdr {
+    int iter;
+    int done;
+    int succ;
+    int m;
+    int pad[12];
+};
+
+// Small FP64 arrays (offsets in doubles from the start of the small region).
+struct GmresLayout {
+    int m, k;
+    size_t H, cs, sn, beta, y, TOL, bnorm, rnorm, h, inv, total;
+    GmresLayout(int m_, int k_) : m(m_), k(k_) {
+        size_t o = 0;
+        H = o;
+        o += (size_t)(m + 1) * m * k;
+        cs = o;
+        o += (size_t)m * k;
+        sn = o;
+        o += (size_t)m * k;
+        beta = o;
+        o += (size_t)(m + 1) * k;
+        y = o;
+        o += (size_t)m * k;
+        TOL = o;
+        o += k;
+        bnorm = o;
+        o += k;
+        rnorm = o;
+        o += k;
+        h = o;
+        o += (size_t)(m + 1) * k;
+        inv = o;
+        o += k;
+        total = o;
+    }
+};
+
+__device__ __forceinline__ void grow_range(int n, int& i0, int& i1) {
+    const int b = blockIdx.x, B = gridDim.x;
+    i0 = (int)((int64_t)n * b / B);
+    i1 = (int)((int64_t)n * (b + 1) / B);
+}
+
+// Per-thread partial dots v[0..nq) of column c, reduced over the row lanes of the
+// workgroup in a fixed order; writes partial[(block * nq + i) * k + c].  Uses
+// (kBlock / kWave) * nq * k doubles of dynamic LDS.
+template <int NQMAX>
+__device__ __forceinline__ void multi_partial(double (&v)[NQMAX], int nq, int k, int kpad,
+                                              double* __restrict__ partial) {
+    extern __shared__ double gm_sh[];
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+#pragma unroll
+    for (int i = 0; i < NQMAX; ++i) {
+        if (i < nq) {
+            double t = v[i];
+            for (int off = kpad; off < kWave; off <<= 1) t += shfl_xor(t, off);
+            if (lane < kpad && lane < k) gm_sh[(wave * nq + i) * k + lane] = t;
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < nq * k; t += kBlock) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < kBlock / kWave; ++w) s += gm_sh[w * nq * k + t];
+        partial[(int64_t)blockIdx.x * nq * k + t] = s;
+    }
+}
+
+inline size_t multi_lds_bytes(int nq, int k) { return (size_t)(kBlock / kWave) * nq * k * sizeof(double); }
+
+// One workgroup per (i, c): out[i*k + c] = sum_b partial[(b*nq + i)*k + c] (fixed tree).
+__global__ __launch_bounds__(kBlock) void gm_total_kernel(int nq, int k, int B, const double* __restrict__ partial,
+                                                          double* __restrict__ out, const GmresHdr* __restrict__ hd) {
+    if (hd->done) return;
+    const int ic = blockIdx.x, i = ic / k, c = ic - i * k;
+    __shared__ double sh[kBlock];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < B; b += kBlock) s += partial[((int64_t)b * nq + i) * k + c];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[ic] = sh[0];
+}
+
+// dots1: q = P z (left, diag) or q already holds A Q_j; h1_i = Q_i . q
+template <typename T, int NQMAX>
+__global__ __launch_bounds__(kBlock) void gm_dots1_kernel(int n, int k, int kpad, int nq, const T* __restrict__ Q,
+                                                          int64_t qstride, const T* __restrict__ z,
+                                                          const T* __restrict__ d, T* __restrict__ q,
+                                                          double* __restrict__ partial, const GmresHdr* __restrict__ hd) {
+    if (hd->done) return;
+    int i0, i1;
+    grow_range(n, i0, i1);
+    const int c = threadIdx.x % kpad, ro = threadIdx.x / kpad, rows = kBlock / kpad;
+    double v[NQMAX];
+#pragma unroll
+    for (int i = 0; i < NQMAX; ++i) v[i] = 0.0;
+    if (c < k)
+        for (int r = i0 + ro; r < i1; r += rows) {
+            const int64_t e = (int64_t)r * k + c;
+            T qv;
+            if (d) {
+                qv = d[r] * z[e];
+                q[e] = qv;
+            } else {
+                qv = q[e];
+            }
+#pragma unroll
+            for (int i = 0; i < NQMAX; ++i)
+                if (i < nq) v[i] += (double)Q[i * qstride + e] * (double)qv;
+        }
+    multi_partial<NQMAX>(v, nq, k, kpad, partial);
+}
+
+// upd: q -= sum_i h_i Q_i; then either h'_i = Q_i . q (norm_only == 0) or |q|^2.
+template <typename T, int NQMAX>
+__global__ __launch_bounds__(kBlock) void gm_update_kernel(int n, int k, int kpad, int nq, const T* __restrict__ Q,
+                                                           int64_t qstride, T* __restrict__ q,
+                                                           const double* __restrict__ h, int norm_only,
+                                                           double* __restrict__ partial,
+                                                           const GmresHdr* __restrict__ hd) {
+    if (hd->done) return;
+    int i0, i1;
+    grow_range(n, i0, i1);
+    const int c = threadIdx.x % kpad, ro = threadIdx.x / kpad, rows = kBlock / kpad;
+    double v[NQMAX];
+#pragma unroll
+    for (int i = 0; i < NQMAX; ++i) v[i] = 0.0;
+    if (c < k) {
+        T hc[NQMAX];
+#pragma unroll
+        for (int i = 0; i < NQMAX; ++i) hc[i] = i < nq ? (T)h[i * k + c] : T(0);
+        for (int r = i0 + ro; r < i1; r += rows) {
+            const int64_t e = (int64_t)r * k + c;
+            T qi[NQMAX];
+            T qv = q[e];
+#pragma unroll
+            for (int i = 0; i < NQMAX; ++i)
+                if (i < nq) {
+                    qi[i] = Q[i * qstride + e];
+                    qv -= hc[i] * qi[i];
+                }
+            q[e] = qv;
+            if (norm_only) {
+                v[0] += (double)qv * (double)qv;
+            } else {
+#pragma unroll
+                for (int i = 0; i < NQMAX; ++i)
+                    if (i < nq) v[i] += (double)qi[i] * (double)qv;
+            }
+        }
+    }
+    multi_partial<NQMAX>(v, norm_only ? 1 : nq, k, kpad, partial);
+}
+
+// H[0..nq)[j] += h  (each Gram-Schmidt pass adds its coefficients, GMRES.hpp:564-568)
+__global__ void gm_addh_kernel(int j, int nq, int m, int k, const double* __restrict__ h, double* __restrict__ H,
+                               const GmresHdr* __restrict__ hd) {
+    if (hd->done) return;
+    for (int t = threadIdx.x; t < nq * k; t += blockDim.x) {
+        const int i = t / k, c = t - i * k;
+        H[((int64_t)i * m + j) * k + c] += h[t];
+    }
+}
+
+// H[j+1][j] = |q|; Givens (GMRES.hpp:603-658); beta; convergence; ++iter.
+__global__ void gm_givens_kernel(int j, int m, int k, const double* __restrict__ qn2, double* __restrict__ H,
+                                 double* __restrict__ cs, double* __restrict__ sn, double* __restrict__ beta,
+                                 const double* __restrict__ TOL, double* __restrict__ inv, GmresHdr* __restrict__ hd) {
+    if (hd->done) return;
+    __shared__ int conv;
+    if (threadIdx.x == 0) conv = 1;
+    __syncthreads();
+    for (int c = threadIdx.x; c < k; c += blockDim.x) {
+        const double qn = sqrt(qn2[c]);
+        auto Hx = [&](int r) -> double& { return H[((int64_t)r * m + j) * k + c]; };
+        Hx(j + 1) = qn;
+        inv[c] = 1.0 / qn;
+        for (int i = 0; i < j; ++i) {
+            const double xi = Hx(i), eta = Hx(i + 1);
+            const double co = cs[i * k + c], si = sn[i * k + c];
+            Hx(i) = co * xi + si * eta;
+            Hx(i + 1) = -si * xi + co * eta;
+        }
+        const double xi = Hx(j), eta = Hx(j + 1);
+        const double r = sqrt(xi * xi + eta * eta);
+        double co, si;
+        if (fabs(xi) <= 2.220446049250313e-16 * r) {
+            co = 0.0;
+            si = 1.0;
+        } else {
+            const double rinv = 1.0 / r;
+            co = fabs(xi) * rinv;
+            si = (xi / fabs(xi)) * eta * rinv;
+        }
+        cs[j * k + c] = co;
+        sn[j * k + c] = si;
+        Hx(j) = co * xi + si * eta;
+        Hx(j + 1) = 0.0;
+        const double bj = beta[j * k + c];
+        beta[(j + 1) * k + c] = -si * bj;
+        beta[j * k + c] = co * bj;
+        if (!(fabs(beta[(j + 1) * k + c]) <= TOL[c])) atomicAnd(&conv, 0);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        hd->iter = j + 1;
+        hd->succ = conv;
+        hd->done = (conv || j + 1 >= hd->m) ? 1 : 0;
+    }
+}
+
+// q *= inv (per column); runs even when the step just set `done` is not needed,
+// so it keeps the guard.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void gm_scale_kernel(int n, int k, T* __restrict__ q,
+                                                          const double* __restrict__ inv, const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int64_t total = (int64_t)n * k;
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < total; e += (int64_t)gridDim.x * kBlock)
+        q[e] = q[e] * (T)inv[e % k];
+}
+
+// v = P u (diag) or copy
+template <typename T>
+__global__ __launch_bounds__(kBlock) void gm_prec_kernel(int n, int k, const T* __restrict__ u, const T* __restrict__ d,
+                                                         T* __restrict__ v, const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int64_t total = (int64_t)n * k;
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < total; e += (int64_t)gridDim.x * kBlock)
+        v[e] = d ? d[e / k] * u[e] : u[e];
+}
+
+// z = s_x * x + s_b * B (row-major ld for B), used for z = A x - b and z = -b.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void gm_residual_kernel(int n, int k, const T* __restrict__ Bm, int64_t ldB,
+                                                             T* __restrict__ z, int has_ax) {
+    const int64_t total = (int64_t)n * k;
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < total; e += (int64_t)gridDim.x * kBlock) {
+        const int64_t i = e / k;
+        const T bv = Bm[i * ldB + (e - i * k)];
+        z[e] = has_ax ? z[e] + (-bv) : -bv;
+    }
+}
+
+// |v|^2 per column (for b norms and r norms) -> partial
+template <typename T>
+__global__ __launch_bounds__(kBlock) void gm_norm_kernel(int n, int k, int kpad, const T* __restrict__ v,
+                                                         double* __restrict__ partial) {
+    int i0, i1;
+    grow_range(n, i0, i1);
+    const int c = threadIdx.x % kpad, ro = threadIdx.x / kpad, rows = kBlock / kpad;
+    double acc[1] = {0.0};
+    if (c < k)
+        for (int r = i0 + ro; r < i1; r += rows) {
+            const double x = (double)v[(int64_t)r * k + c];
+            acc[0] += x * x;
+        }
+    multi_partial<1>(acc, 1, k, kpad, partial);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void gm_zero_kernel(int64_t count, T* __restrict__ d) {
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < count; e += (int64_t)gridDim.x * kBlock)
+        d[e] = T(0);
+}
+
+// After the cycle's first residual: r_norm, beta[0] = r_norm, inv = 1/r_norm,
+// H = 0, TOL (first call), convergence at iteration 0 (GMRES.hpp:352-375).
+__global__ void gm_start_kernel(int m, int k, const double* __restrict__ nn2, double* __restrict__ small,
+                                GmresLayout L, double tol, int set_tol, GmresHdr* __restrict__ hd) {
+    __shared__ int conv;
+    if (threadIdx.x == 0) conv = 1;
+    __syncthreads();
+    for (int t = threadIdx.x; t < (m + 1) * m * k; t += blockDim.x) small[L.H + t] = 0.0;
+    for (int t = threadIdx.x; t < (m + 1) * k; t += blockDim.x) small[L.beta + t] = 0.0;
+    __syncthreads();
+    for (int c = threadIdx.x; c < k; c += blockDim.x) {
+        const double nv = sqrt(nn2[c]);
+        if (set_tol) {
+            small[L.bnorm + c] = nv;
+            small[L.TOL + c] = nv * tol;
+        } else {
+            small[L.rnorm + c] = nv;
+            small[L.beta + c] = nv;
+            small[L.inv + c] = 1.0 / nv;
+            if (!(fabs(nv) <= small[L.TOL + c])) atomicAnd(&conv, 0);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && !set_tol) {
+        hd->iter = 0;
+        hd->succ = conv;
+        hd->m = m;
+        hd->done = (conv || m <= 0) ? 1 : 0;
+    }
+}
+
+// y = H^{-1} beta per column (upper back substitution, iter x iter)
+__global__ void gm_lsq_kernel(int m, int k, double* __restrict__ small, GmresLayout L, const GmresHdr* __restrict__ hd) {
+    const int it = hd->iter;
+    for (int c = threadIdx.x; c < k; c += blockDim.x) {
+        for (int i = it - 1; i >= 0; --i) {
+            double s = small[L.beta + i * k + c];
+            for (int j = i + 1; j < it; ++j) s -= small[L.H + ((int64_t)i * m + j) * k + c] * small[L.y + j * k + c];
+            small[L.y + i * k + c] = s / small[L.H + ((int64_t)i * m + i) * k + c];
+        }
+    }
+}
+
+// z = sum_{j < iter} y_j Q_j;  (left) x = z;  solution = (implicit ? 0 : solution) - x.
+// Right preconditioning applies P to z first (x = P z, GMRES.hpp:444-448).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void gm_synth_kernel(int n, int k, const T* __restrict__ Q, int64_t qstride,
+                                                          const double* __restrict__ small, GmresLayout L,
+                                                          const T* __restrict__ d_right, T* __restrict__ sol,
+                                                          int implicit_zero, const GmresHdr* __restrict__ hd) {
+    const int it = hd->iter;
+    const int64_t total = (int64_t)n * k;
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < total; e += (int64_t)gridDim.x * kBlock) {
+        const int c = (int)(e % k);
+        T zv = T(0);
+        for (int j = 0; j < it; ++j) zv += (T)small[L.y + j * k + c] * Q[j * qstride + e];
+        if (d_right) zv = d_right[e / k] * zv;
+        sol[e] = implicit_zero ? -zv : sol[e] - zv;
+    }
+}
+
+// solution = (implicit ? 0 : solution) + v, where v = P(-Q y)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void gm_sub_kernel(int64_t count, const T* __restrict__ v, T* __restrict__ sol,
+                                                        int implicit_zero) {
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < count; e += (int64_t)gridDim.x * kBlock)
+        sol[e] = implicit_zero ? v[e] : sol[e] + v[e];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void gm_copy_kernel(int64_t count, const T* __restrict__ s, T* __restrict__ d) {
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < count; e += (int64_t)gridDim.x * kBlock)
+        d[e] = s[e];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void gm_load_kernel(int n, int k, const T* __restrict__ X, int64_t ldX,
+                                                         T* __restrict__ x) {
+    const int64_t total = (int64_t)n * k;
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < total; e += (int64_t)gridDim.x * kBlock) {
+        const int64_t i = e / k;
+        x[e] = X[i * ldX + (e - i * k)];
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void gm_write_kernel(int n, int k, T a, const T* __restrict__ x, T b,
+                                                          T* __restrict__ X, int64_t ldX, int a_one, int b_mode) {
+    const int64_t total = (int64_t)n * k;
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < total; e += (int64_t)gridDim.x * kBlock) {
+        const int64_t i = e / k;
+        T* Xp = X + i * ldX + (e - i * k);
+        T v = a_one ? x[e] : a * x[e];
+        if (b_mode == 1)
+            v = v + *Xp;
+        else if (b_mode == 2)
+            v = v + b * *Xp;
+        *Xp = v;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void gm_zero_or_scale_kernel(int n, int k, T b, T* __restrict__ X, int64_t ldX) {
+    const int64_t total = (int64_t)n * k;
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < total; e += (int64_t)gridDim.x * kBlock) {
+        const int64_t i = e / k;
+        T* Xp = X + i * ldX + (e - i * k);
+        *Xp = (b == T(0)) ? T(0) : *Xp * b;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host driver.
+// ---------------------------------------------------------------------------
+static int pow2ceil_g(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+template <typename T, int NQMAX>
+static void launch_dots1(hipStream_t s, int n, int k, int kpad, int nq, const T* Q, int64_t qs, const T* z,
+                         const T* d, T* q, double* partial, const GmresHdr* hd) {
+    hipLaunchKernelGGL((gm_dots1_kernel<T, NQMAX>), dim3(kGB), dim3(kBlock), multi_lds_bytes(nq, k), s, n, k, kpad,
+                       nq, Q, qs, z, d, q, partial, hd);
+}
+template <typename T, int NQMAX>
+static void launch_update(hipStream_t s, int n, int k, int kpad, int nq, const T* Q, int64_t qs, T* q,
+                          const double* h, int norm_only, double* partial, const GmresHdr* hd) {
+    hipLaunchKernelGGL((gm_update_kernel<T, NQMAX>), dim3(kGB), dim3(kBlock), multi_lds_bytes(nq, k), s, n, k, kpad,
+                       nq, Q, qs, q, h, norm_only, partial, hd);
+}
+template <typename T>
+static void dots1(hipStream_t s, int n, int k, int kpad, int nq, const T* Q, int64_t qs, const T* z, const T* d,
+                  T* q, double* partial, const GmresHdr* hd) {
+    if (nq <= 4) launch_dots1<T, 4>(s, n, k, kpad, nq, Q, qs, z, d, q, partial, hd);
+    else if (nq <= 8) launch_dots1<T, 8>(s, n, k, kpad, nq, Q, qs, z, d, q, partial, hd);
+    else if (nq <= 16) launch_dots1<T, 16>(s, n, k, kpad, nq, Q, qs, z, d, q, partial, hd);
+    else if (nq <= 32) launch_dots1<T, 32>(s, n, k, kpad, nq, Q, qs, z, d, q, partial, hd);
+    else launch_dots1<T, 64>(s, n, k, kpad, nq, Q, qs, z, d, q, partial, hd);
+}
+template <typename T>
+static void update(hipStream_t s, int n, int k, int kpad, int nq, const T* Q, int64_t qs, T* q, const double* h,
+                   int norm_only, double* partial, const GmresHdr* hd) {
+    if (nq <= 4) launch_update<T, 4>(s, n, k, kpad, nq, Q, qs, q, h, norm_only, partial, hd);
+    else if (nq <= 8) launch_update<T, 8>(s, n, k, kpad, nq, Q, qs, q, h, norm_only, partial, hd);
+    else if (nq <= 16) launch_update<T, 16>(s, n, k, kpad, nq, Q, qs, q, h, norm_only, partial, hd);
+    else if (nq <= 32) launch_update<T, 32>(s, n, k, kpad, nq, Q, qs, q, h, norm_only, partial, hd);
+    else launch_update<T, 64>(s, n, k, kpad, nq, Q, qs, q, h, norm_only, partial, hd);
+}
+
+static int grid_elems(int64_t count) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>((count + kBlock - 1) / kBlock, 8192));
+}
+
+struct GmresWs {
+    GmresHdr* hd;
+    double* small;
+    double* partial;
+    double* tot;  // (m+1) * k totals
+    void* Q;
+    void* x;
+    void* z;
+    void* sol;
+    size_t bytes;
+};
+
+static size_t gmres_layout(int32_t n, int32_t m, int32_t k, size_t vs, GmresWs* w, char* base) {
+    GmresLayout L(m, k);
+    size_t off = 0;
+    auto take = [&](size_t b) {
+        const size_t o = off;
+        off += align256(b);
+        return base ? (void*)(base + o) : nullptr;
+    };
+    GmresWs t;
+    t.hd = (GmresHdr*)take(sizeof(GmresHdr));
+    t.small = (double*)take(L.total * sizeof(double));
+    t.partial = (double*)take((size_t)kGB * (m + 1) * k * sizeof(double));
+    t.tot = (double*)take((size_t)(m + 1) * k * sizeof(double));
+    t.Q = take((size_t)(m + 1) * n * k * vs);
+    t.x = take((size_t)n * k * vs);
+    t.z = take((size_t)n * k * vs);
+    t.sol = take((size_t)n * k * vs);
+    t.bytes = off;
+    if (w) *w = t;
+    return off;
+}
+
+size_t gmres_workspace_bytes(int32_t n, int32_t max_iter, int32_t k, size_t vs) {
+    const int m = std::max(0, std::min(max_iter, n + 1));
+    return gmres_layout(n, std::max(m, 1), k, vs, nullptr, nullptr);
+}
+
+template <typename T>
+int gmres_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, const OpRef& Pop, int side,
+                  int32_t k, int32_t max_iter, double a, const T* Bm, int64_t ldB, double b, T* X, int64_t ldX,
+                  double tol, int32_t max_restarts, int use_guess, void* ws, size_t ws_bytes, GmresResult* res,
+                  hipStream_t s) {
+    const int n = A.n;
+    res->succeeded = 0;
+    res->iterations = 0;
+    res->restarts = 0;
+    if (k < 1 || k > kGmresMaxCols) return fail(TG_ERR_INVALID, "gmres: nrhs must be in [1, %d]", kGmresMaxCols);
+    if (A.m != n || n < 1) return fail(TG_ERR_SHAPE, "gmres: the operator must be square and non-empty");
+    const int m = std::min(max_iter, n + 1);  // GMRES.hpp:89
+    if (m < 0 || m > kGmresMaxM) return fail(TG_ERR_INVALID, "gmres: restart length must be in [0, %d]", kGmresMaxM);
+    const int mm = std::max(m, 1);
+    if (!ws || ws_bytes < gmres_layout(n, mm, k, sizeof(T), nullptr, nullptr))
+        return fail(TG_ERR_WORKSPACE, "gmres: workspace missing or smaller than tg_gmres_workspace_bytes()");
+    if (prec_kind == 1 && !d) return fail(TG_ERR_INVALID, "gmres: diagonal preconditioner without a diagonal");
+    if (prec_kind == 2 && !Pop.fn) return fail(TG_ERR_INVALID, "gmres: callback preconditioner without a function");
+    const bool host_ops = Aop.is_callback() || prec_kind == 2;
+    GmresWs w;
+    gmres_layout(n, mm, k, sizeof(T), &w, (char*)ws);
+    GmresLayout L(mm, k);
+    const T* dd = prec_kind == 1 ? d : nullptr;
+    // v = P u: device scaling (identity / diagonal) or the host callback
+    auto applyP = [&](const T* u_, T* v_, const int* skip) -> int {
+        if (prec_kind == 2) return op_apply(Pop, u_, v_, k, skip, s);
+        hipLaunchKernelGGL((gm_prec_kernel<T>), dim3(grid_elems((int64_t)n * k)), dim3(kBlock), 0, s, n, k, u_, dd, v_,
+                           skip);
+        return TG_OK;
+    };
+    const int64_t N = (int64_t)n * k;
+    const int64_t qstride = N;
+    T* Q = (T*)w.Q;
+    T* x = (T*)w.x;
+    T* z = (T*)w.z;
+    T* sol = (T*)w.sol;
+    const int kpad = pow2ceil_g(k);
+    const int ge = grid_elems(N);
+
+    // b norms: x = B; z = P x (left) or z = B (right)  (GMRES.hpp:168-185)
+    GmresHdr h0{};
+    h0.m = m;
+    h0.done = 0;
+    TG_HIP(hipMemcpyAsync(w.hd, &h0, sizeof(h0), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL((gm_load_kernel<T>), dim3(ge), dim3(kBlock), 0, s, n, k, Bm, ldB, x);
+    if (side == 0) {
+        int rcp = applyP((const T*)x, z, nullptr);
+        if (rcp) return rcp;
+    } else {
+        hipLaunchKernelGGL((gm_copy_kernel<T>), dim3(ge), dim3(kBlock), 0, s, N, (const T*)x, z);
+    }
+    hipLaunchKernelGGL((gm_norm_kernel<T>), dim3(kGB), dim3(kBlock), multi_lds_bytes(1, k), s, n, k, kpad, (const T*)z,
+                       w.partial);
+    hipLaunchKernelGGL(gm_total_kernel, dim3(k), dim3(kBlock), 0, s, 1, k, kGB, (const double*)w.partial, w.tot,
+                       (const GmresHdr*)w.hd);
+    hipLaunchKernelGGL(gm_start_kernel, dim3(1), dim3(kBlock), 0, s, mm, k, (const double*)w.tot, w.small, L, tol, 1,
+                       w.hd);
+    TG_LAUNCH_CHECK();
+    std::vector<double> bn(k);
+    TG_HIP(hipMemcpyAsync(bn.data(), w.small + L.bnorm, k * sizeof(double), hipMemcpyDeviceToHost, s));
+    TG_HIP(hipStreamSynchronize(s));
+    double bmax = 0.0;
+    for (int c = 0; c < k; ++c) {
+        if (std::isnan(bn[c])) {  // GMRES.hpp:193-204
+            fail(TG_OK, "gmres: right-hand side contains NaNs. Doing nothing.");
+            return TG_OK;
+        }
+        bmax = std::max(bmax, bn[c]);
+    }
+    if (!(bmax > 0.0)) {  // :206-229
+        hipLaunchKernelGGL((gm_zero_or_scale_kernel<T>), dim3(ge), dim3(kBlock), 0, s, n, k, (T)b, X, ldX);
+        TG_LAUNCH_CHECK();
+        res->succeeded = 1;
+        return TG_OK;
+    }
+    if (use_guess) hipLaunchKernelGGL((gm_load_kernel<T>), dim3(ge), dim3(kBlock), 0, s, n, k, (const T*)X, ldX, sol);
+    else hipLaunchKernelGGL((gm_zero_kernel<T>), dim3(ge), dim3(kBlock), 0, s, N, sol);
+
+    auto solve = [&](bool implicit_zero, int& succ_out) -> int {
+        // residual: z = A x - b (or -b), x = solution   (GMRES.hpp:282-350)
+        T* target = (side == 0) ? z : Q;
+        if (implicit_zero) {
+            hipLaunchKernelGGL((gm_residual_kernel<T>), dim3(ge), dim3(kBlock), 0, s, n, k, Bm, ldB, target, 0);
+        } else {
+            int rc = op_apply(Aop, (const T*)sol, target, k, nullptr, s);
+            if (rc) return rc;
+            hipLaunchKernelGGL((gm_residual_kernel<T>), dim3(ge), dim3(kBlock), 0, s, n, k, Bm, ldB, target, 1);
+        }
+        if (side == 0) {
+            int rcp = applyP((const T*)z, Q, nullptr);
+            if (rcp) return rcp;
+        }
+        hipLaunchKernelGGL((gm_norm_kernel<T>), dim3(kGB), dim3(kBlock), multi_lds_bytes(1, k), s, n, k, kpad,
+                           (const T*)Q, w.partial);
+        TG_LAUNCH_CHECK();
+        // the totals kernel returns early while `done` is set: clear it first
+        GmresHdr clr{};
+        clr.m = m;
+        TG_HIP(hipMemcpyAsync(w.hd, &clr, sizeof(clr), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(gm_total_kernel, dim3(k), dim3(kBlock), 0, s, 1, k, kGB, (const double*)w.partial, w.tot,
+                           (const GmresHdr*)w.hd);
+        hipLaunchKernelGGL(gm_start_kernel, dim3(1), dim3(kBlock), 0, s, mm, k, (const double*)w.tot, w.small, L, tol,
+                           0, w.hd);
+        hipLaunchKernelGGL((gm_scale_kernel<T>), dim3(ge), dim3(kBlock), 0, s, n, k, Q,
+                           (const double*)(w.small + L.inv), (const int*)nullptr);
+        TG_LAUNCH_CHECK();
+        int j = 0, batch = host_ops ? 1 : 4;
+        GmresHdr hh{};
+        while (j < m) {
+            for (int t = 0; t < batch && j < m; ++t, ++j) {
+                const int nq = j + 1;
+                T* Qj = Q + j * qstride;
+                T* q = Q + (int64_t)(j + 1) * qstride;
+                int rc;
+                if (side == 0) {
+                    // z = A Q_j; q = P z (diagonal P is folded into dots1)
+                    rc = op_apply(Aop, (const T*)Qj, (dd || prec_kind == 2) ? z : q, k, &w.hd->done, s);
+                    if (!rc && prec_kind == 2) rc = applyP((const T*)z, q, &w.hd->done);
+                } else {
+                    rc = applyP((const T*)Qj, z, &w.hd->done);
+                    if (!rc) rc = op_apply(Aop, (const T*)z, q, k, &w.hd->done, s);
+                }
+                if (rc) return rc;
+                dots1<T>(s, n, k, kpad, nq, (const T*)Q, qstride, (const T*)z, side == 0 ? dd : nullptr, q, w.partial,
+                         (const GmresHdr*)w.hd);
+                hipLaunchKernelGGL(gm_total_kernel, dim3(nq * k), dim3(kBlock), 0, s, nq, k, kGB,
+                                   (const double*)w.partial, w.tot, (const GmresHdr*)w.hd);
+                hipLaunchKernelGGL(gm_addh_kernel, dim3(1), dim3(kBlock), 0, s, j, nq, mm, k, (const double*)w.tot,
+                                   w.small + L.H, (const GmresHdr*)w.hd);
+                update<T>(s, n, k, kpad, nq, (const T*)Q, qstride, q, (const double*)w.tot, 0, w.partial,
+                          (const GmresHdr*)w.hd);
+                hipLaunchKernelGGL(gm_total_kernel, dim3(nq * k), dim3(kBlock), 0, s, nq, k, kGB,
+                                   (const double*)w.partial, w.tot, (const GmresHdr*)w.hd);
+                hipLaunchKernelGGL(gm_addh_kernel, dim3(1), dim3(kBlock), 0, s, j, nq, mm, k, (const double*)w.tot,
+                                   w.small + L.H, (const GmresHdr*)w.hd);
+                update<T>(s, n, k, kpad, nq, (const T*)Q, qstride, q, (const double*)w.tot, 1, w.partial,
+                          (const GmresHdr*)w.hd);
+                hipLaunchKernelGGL(gm_total_kernel, dim3(k), dim3(kBlock), 0, s, 1, k, kGB, (const double*)w.partial,
+                                   w.tot, (const GmresHdr*)w.hd);
+                hipLaunchKernelGGL(gm_givens_kernel, dim3(1), dim3(64), 0, s, j, mm, k, (const double*)w.tot,
+                                   w.small + L.H, w.small + L.cs, w.small + L.sn, w.small + L.beta,
+                                   (const double*)(w.small + L.TOL), w.small + L.inv, w.hd);
+                hipLaunchKernelGGL((gm_scale_kernel<T>), dim3(ge), dim3(kBlock), 0, s, n, k, q,
+                                   (const double*)(w.small + L.inv), (const int*)&w.hd->done);
+            }
+            TG_LAUNCH_CHECK();
+            TG_HIP(hipMemcpyAsync(&hh, w.hd, sizeof(hh), hipMemcpyDeviceToHost, s));
+            TG_HIP(hipStreamSynchronize(s));
+            if (hh.done) break;
+            if (!host_ops) batch = std::min(batch * 2, 16);
+        }
+        TG_HIP(hipMemcpyAsync(&hh, w.hd, sizeof(hh), hipMemcpyDeviceToHost, s));
+        TG_HIP(hipStreamSynchronize(s));
+        succ_out = hh.succ;
+        res->iterations = hh.iter;
+        if (hh.succ && hh.iter == 0) return TG_OK;  // converged before any step: solution untouched (:371-375)
+        hipLaunchKernelGGL(gm_lsq_kernel, dim3(1), dim3(64), 0, s, mm, k, w.small, L, (const GmresHdr*)w.hd);
+        if (side == 1 && prec_kind == 2) {
+            // z = Q y;  x = P z;  solution -= x   (GMRES.hpp:416-480)
+            hipLaunchKernelGGL((gm_synth_kernel<T>), dim3(ge), dim3(kBlock), 0, s, n, k, (const T*)Q, qstride,
+                               (const double*)w.small, L, (const T*)nullptr, z, 1, (const GmresHdr*)w.hd);
+            int rcp = applyP((const T*)z, x, nullptr);
+            if (rcp) return rcp;
+            hipLaunchKernelGGL((gm_sub_kernel<T>), dim3(ge), dim3(kBlock), 0, s, N, (const T*)x, sol,
+                               implicit_zero ? 1 : 0);
+        } else {
+            hipLaunchKernelGGL((gm_synth_kernel<T>), dim3(ge), dim3(kBlock), 0, s, n, k, (const T*)Q, qstride,
+                               (const double*)w.small, L, side == 1 ? dd : nullptr, sol, implicit_zero ? 1 : 0,
+                               (const GmresHdr*)w.hd);
+        }
+        TG_LAUNCH_CHECK();
+        return TG_OK;
+    };
+
+    int restarts = 0, succ = 0;
+    int rc = solve(!use_guess, succ);
+    if (rc) return rc;
+    ++restarts;
+    while (!succ && restarts <= max_restarts) {
+        rc = solve(false, succ);
+        if (rc) return rc;
+        ++restarts;
+    }
+    --restarts;
+    const int a_one = (a == 1.0);
+    const int b_mode = (b == 0.0) ? 0 : (b == 1.0) ? 1 : 2;
+    hipLaunchKernelGGL((gm_write_kernel<T>), dim3(ge), dim3(kBlock), 0, s, n, k, (T)a, (const T*)sol, (T)b, X, ldX,
+                       a_one, b_mode);
+    TG_LAUNCH_CHECK();
+    std::vector<double> beta((size_t)(mm + 1) * k);
+    TG_HIP(hipMemcpyAsync(beta.data(), w.small + L.beta, beta.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    TG_HIP(hipStreamSynchronize(s));
+    res->succeeded = succ;
+    res->restarts = restarts;
+    for (int c = 0; c < k; ++c) res->rel_res[c] = std::fabs(beta[(size_t)res->iterations * k + c]) / bn[c];
+    return TG_OK;
+}
+
+template int gmres_solve_t<double>(const CsrView&, const OpRef&, int, const double*, const OpRef&, int, int32_t,
+                                   int32_t, double, const double*, int64_t, double, double*, int64_t, double, int32_t,
+                                   int, void*, size_t, GmresResult*, hipStream_t);
+template int gmres_solve_t<float>(const CsrView&, const OpRef&, int, const float*, const OpRef&, int, int32_t,
+                                  int32_t, double, const float*, int64_t, double, float*, int64_t, double, int32_t, int,
+                                  void*, size_t, GmresResult*, hipStream_t);
+
+}  // namespace tg
+
+using namespace tg;
+
+extern "C" size_t tg_gmres_workspace_bytes(int32_t n, int32_t max_iter, int32_t nrhs, tg_dtype val_type) {
+    if (n < 1 || nrhs < 1) return 0;
+    return gmres_workspace_bytes(n, max_iter, nrhs, val_type == TG_F32 ? 4 : 8);
+}
+
+template <typename T>
+static int gmres_entry(const CsrView& A, const OpRef& Aop, int32_t prec_kind, const void* prec_diag,
+                       const OpRef& Pop, int32_t side, int32_t nrhs, int32_t max_iter, double a_coef, const void* B,
+                       int64_t ldB, double b_coef, void* X, int64_t ldX, double tol, int32_t max_restarts,
+                       int32_t use_guess, void* workspace, size_t workspace_bytes, int32_t* succeeded,
+                       int32_t* iterations, int32_t* restarts, double* rel_residuals, hipStream_t s) {
+    GmresResult res{};
+    const int rc = gmres_solve_t<T>(A, Aop, prec_kind, (const T*)prec_diag, Pop, side, nrhs, max_iter, a_coef,
+                                    (const T*)B, ldB, b_coef, (T*)X, ldX, tol, max_restarts, use_guess, workspace,
+                                    workspace_bytes, &res, s);
+    if (rc) return rc;
+    if (succeeded) *succeeded = res.succeeded;
+    if (iterations) *iterations = res.iterations;
+    if (restarts) *restarts = res.restarts;
+    if (rel_residuals)
+        for (int c = 0; c < nrhs; ++c) rel_residuals[c] = res.rel_res[c];
+    return TG_OK;
+}
+
+extern "C" int tg_gmres_solve(int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, const int32_t* ci,
+                              const void* a, tg_dtype val_type, int32_t prec_kind, const void* prec_diag,
+                              int32_t side, int32_t nrhs, int32_t max_iter, double a_coef, const void* B,
+                              int64_t ldB, double b_coef, void* X, int64_t ldX, double tol, int32_t max_restarts,
+                              int32_t use_guess, void* workspace, size_t workspace_bytes, int32_t* succeeded,
+                              int32_t* iterations, int32_t* restarts, double* rel_residuals, void* stream) {
+    if (succeeded) *succeeded = 0;
+    if (iterations) *iterations = 0;
+    if (restarts) *restarts = 0;
+    if (n < 1 || nrhs < 1 || !B || !X || ldB < nrhs || ldX < nrhs || (prec_kind != 0 && prec_kind != 1) ||
+        (side != 0 && side != 1))
+        return fail(TG_ERR_INVALID, "tg_gmres_solve: bad argument");
+    if (rp_type != TG_I32 && rp_type != TG_I64) return fail(TG_ERR_DTYPE, "tg_gmres_solve: rp_type must be I32/I64");
+    CsrView A;
+    A.m = n;
+    A.n = n;
+    A.nnz = nnz;
+    A.rp = rp;
+    A.rp_type = rp_type;
+    A.ci = ci;
+    A.val = a;
+    A.val_type = val_type;
+    OpRef Aop;
+    Aop.csr = &A;
+    OpRef Pop;
+    hipStream_t s = as_stream(stream);
+    if (val_type == TG_F64)
+        return gmres_entry<double>(A, Aop, prec_kind, prec_diag, Pop, side, nrhs, max_iter, a_coef, B, ldB, b_coef, X,
+                                   ldX, tol, max_restarts, use_guess, workspace, workspace_bytes, succeeded,
+                                   iterations, restarts, rel_residuals, s);
+    if (val_type == TG_F32)
+        return gmres_entry<float>(A, Aop, prec_kind, prec_diag, Pop, side, nrhs, max_iter, a_coef, B, ldB, b_coef, X,
+                                  ldX, tol, max_restarts, use_guess, workspace, workspace_bytes, succeeded, iterations,
+                                  restarts, rel_residuals, s);
+    return fail(TG_ERR_DTYPE, "tg_gmres_solve: val_type must be F64/F32");
+}
+
+extern "C" int tg_gmres_solve_op(int32_t n, tg_apply_fn apply_A, void* ctx_A, tg_apply_fn apply_P, void* ctx_P,
+                                 tg_dtype val_type, int32_t side, int32_t nrhs, int32_t max_iter, double a_coef,
+                                 const void* B, int64_t ldB, double b_coef, void* X, int64_t ldX, double tol,
+                                 int32_t max_restarts, int32_t use_guess, void* workspace, size_t workspace_bytes,
+                                 int32_t* succeeded, int32_t* iterations, int32_t* restarts, double* rel_residuals,
+                                 void* stream) {
+    if (succeeded) *succeeded = 0;
+    if (iterations) *iterations = 0;
+    if (restarts) *restarts = 0;
+    if (n < 1 || nrhs < 1 || !B || !X || ldB < nrhs || ldX < nrhs || !apply_A || (side != 0 && side != 1))
+        return fail(TG_ERR_INVALID, "tg_gmres_solve_op: bad argument");
+    CsrView A;
+    A.m = n;
+    A.n = n;
+    A.val_type = val_type;
+    OpRef Aop;
+    Aop.fn = apply_A;
+    Aop.ctx = ctx_A;
+    OpRef Pop;
+    Pop.fn = apply_P;
+    Pop.ctx = ctx_P;
+    const int prec_kind = apply_P ? 2 : 0;
+    hipStream_t s = as_stream(stream);
+    if (val_type == TG_F64)
+        return gmres_entry<double>(A, Aop, prec_kind, nullptr, Pop, side, nrhs, max_iter, a_coef, B, ldB, b_coef, X,
+                                   ldX, tol, max_restarts, use_guess, workspace, workspace_bytes, succeeded,
+                                   iterations, restarts, rel_residuals, s);
+    if (val_type == TG_F32)
+        return gmres_entry<float>(A, Aop, prec_kind, nullptr, Pop, side, nrhs, max_iter, a_coef, B, ldB, b_coef, X,
+                                  ldX, tol, max_restarts, use_guess, workspace, workspace_bytes, succeeded, iterations,
+                                  restarts, rel_residuals, s);
+    return fail(TG_ERR_DTYPE, "tg_gmres_solve_op: val_type must be F64/F32");
+}

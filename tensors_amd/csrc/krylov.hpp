@@ -1,0 +1,96 @@
+// Internal interfaces shared by the SpMM and Krylov translation units.
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+
+namespace tg {
+
+// Device CSR operand of the solvers (all pointers device-resident).
+struct CsrView {
+    int32_t m = 0, n = 0;
+    int64_t nnz = 0;
+    const void* rp = nullptr;
+    tg_dtype rp_type = TG_I32;
+    const int32_t* ci = nullptr;
+    const void* val = nullptr;  // null: binary matrix
+    tg_dtype val_type = TG_F64;
+};
+
+// Y <- alpha A X + beta Y with the SpMM kernels; when `skip` is non-null and
+// *skip != 0 at kernel start, the kernels return without touching Y.
+int spmm_dispatch(const CsrView& A, double alpha, const void* X, int64_t ldX, double beta, void* Y, int64_t ldY,
+                  int32_t nrhs, const int32_t* job_ptr, int32_t blocks, const int* skip, hipStream_t s);
+
+// An operator y <- Op(x) on n x k row-major device blocks: either a device CSR
+// matrix or a host callback (tg_apply_fn) that the library calls with device
+// pointers.
+struct OpRef {
+    const CsrView* csr = nullptr;
+    tg_apply_fn fn = nullptr;
+    void* ctx = nullptr;
+    bool is_callback() const { return csr == nullptr; }
+};
+
+int op_apply(const OpRef& op, const void* x, void* y, int k, const int* skip, hipStream_t s);
+
+template <typename T>
+inline int csr_apply(const CsrView& A, const T* X, T* Y, int k, hipStream_t s) {
+    return spmm_dispatch(A, 1.0, X, k, 0.0, Y, k, k, nullptr, 0, nullptr, s);
+}
+template <typename T>
+inline int csr_apply_guarded(const CsrView& A, const T* X, T* Y, int k, const int* skip, hipStream_t s) {
+    return spmm_dispatch(A, 1.0, X, k, 0.0, Y, k, k, nullptr, 0, skip, s);
+}
+
+constexpr int kSolverBlocks = 1024;  // workgroups of the column-blocked vector kernels
+constexpr int kMaxCols = 256;        // nrhs limit of the solvers
+
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Device-resident CG state (one per solve).
+struct CGState {
+    int iter;
+    int done;
+    int succeeded;
+    int max_iter;
+    int use_guess;
+    int pad[11];
+    double rho[kMaxCols];
+    double rho_old[kMaxCols];
+    double alpha[kMaxCols];
+    double beta[kMaxCols];
+    double TOL[kMaxCols];
+    double bsq[kMaxCols];
+};
+
+struct CGResult {
+    int succeeded;
+    int iterations;
+    double rel_res[kMaxCols];
+};
+
+struct GmresResult {
+    int succeeded;
+    int iterations;
+    int restarts;
+    double rel_res[kMaxCols];
+};
+
+size_t gmres_workspace_bytes(int32_t n, int32_t max_iter, int32_t k, size_t vs);
+
+template <typename T>
+int gmres_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, const OpRef& Pop, int side,
+                  int32_t k, int32_t max_iter, double a, const T* Bm, int64_t ldB, double b, T* X, int64_t ldX,
+                  double tol, int32_t max_restarts, int use_guess, void* ws, size_t ws_bytes, GmresResult* res,
+                  hipStream_t s);
+
+size_t cg_workspace_bytes(int32_t n, int32_t k, size_t vs, int prec_kind);
+
+template <typename T>
+int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, const OpRef& Pop, int32_t k, int32_t max_iter, double a, const T* Bm,
+               int64_t ldB, double b, T* X, int64_t ldX, double tol, int use_guess, void* ws, size_t ws_bytes,
+               CGResult* res, hipStream_t s);
+
+}  // namespace tg

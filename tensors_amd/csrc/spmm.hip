@@ -24,6 +24,7 @@
 #include <algorithm>
 
 #include "common.hpp"
+#include "krylov.hpp"
 
 namespace tg {
 
@@ -114,7 +115,8 @@ template <typename T, typename RP, int TEAM>
 __global__ __launch_bounds__(kBlock) void spmv_team_kernel(
     int m, const RP* __restrict__ rp, const int32_t* __restrict__ ci, const T* __restrict__ a, T alpha,
     const T* __restrict__ x, int64_t ldx, T beta, T* __restrict__ y, int64_t ldy, int alpha_one,
-    int beta_mode, const int32_t* __restrict__ job_ptr, int rows_per_block) {
+    int beta_mode, const int32_t* __restrict__ job_ptr, int rows_per_block, const int* __restrict__ skip) {
+    if (skip && *skip) return;
     constexpr int TEAMS = kBlock / TEAM;
     const int lt = threadIdx.x % TEAM;
     const int team = threadIdx.x / TEAM;
@@ -169,7 +171,8 @@ template <typename T, typename RP, int G, int V>
 __global__ __launch_bounds__(kBlock) void spmm_wave_kernel(
     int m, const RP* __restrict__ rp, const int32_t* __restrict__ ci, const T* __restrict__ a, T alpha,
     const T* __restrict__ X, int64_t ldX, T beta, T* __restrict__ Y, int64_t ldY, int nrhs, int alpha_one,
-    int beta_mode, const int32_t* __restrict__ job_ptr, int rows_per_block) {
+    int beta_mode, const int32_t* __restrict__ job_ptr, int rows_per_block, const int* __restrict__ skip) {
+    if (skip && *skip) return;
     constexpr int S = kWave / G;  // lane groups splitting a row's nonzeros
     constexpr int KT = G * V;     // columns per tile
     constexpr int WAVES = kBlock / kWave;
@@ -308,7 +311,7 @@ static int rows_per_block_for(int32_t m, int64_t nnz, int32_t nrhs) {
 template <typename T, typename RP>
 static int launch_spmv(int32_t m, int64_t nnz, const RP* rp, const int32_t* ci, const T* a, T alpha,
                        const T* X, int64_t ldX, T beta, T* Y, int64_t ldY, int alpha_one, int beta_mode,
-                       const int32_t* job_ptr, int32_t blocks, hipStream_t st) {
+                       const int32_t* job_ptr, int32_t blocks, const int* skip, hipStream_t st) {
     const int team = spmv_team_width(m, nnz);
     const int rpb = (kBlock / team) * kRowsPerTeam;
     const int grid = job_ptr ? blocks : (m + rpb - 1) / rpb;
@@ -316,7 +319,7 @@ static int launch_spmv(int32_t m, int64_t nnz, const RP* rp, const int32_t* ci, 
 #define TG_SPMV(TM)                                                                                     \
     case TM:                                                                                            \
         hipLaunchKernelGGL((spmv_team_kernel<T, RP, TM>), dim3(grid), dim3(kBlock), 0, st, m, rp, ci, a, \
-                           alpha, X, ldX, beta, Y, ldY, alpha_one, beta_mode, job_ptr, rpb);            \
+                           alpha, X, ldX, beta, Y, ldY, alpha_one, beta_mode, job_ptr, rpb, skip);      \
         break;
     switch (team) {
         TG_SPMV(4)
@@ -334,14 +337,14 @@ static int launch_spmv(int32_t m, int64_t nnz, const RP* rp, const int32_t* ci, 
 template <typename T, typename RP, int V>
 static int launch_spmm_v(int G, int32_t m, const RP* rp, const int32_t* ci, const T* a, T alpha, const T* X,
                          int64_t ldX, T beta, T* Y, int64_t ldY, int32_t nrhs, int alpha_one, int beta_mode,
-                         const int32_t* job_ptr, int32_t blocks, hipStream_t st) {
+                         const int32_t* job_ptr, int32_t blocks, const int* skip, hipStream_t st) {
     const int rpb = (kBlock / kWave) * kRowsPerTeam;
     const int grid = job_ptr ? blocks : (m + rpb - 1) / rpb;
     if (grid <= 0) return TG_OK;
 #define TG_SPMM(GG)                                                                                      \
     case GG:                                                                                             \
         hipLaunchKernelGGL((spmm_wave_kernel<T, RP, GG, V>), dim3(grid), dim3(kBlock), 0, st, m, rp, ci, a, \
-                           alpha, X, ldX, beta, Y, ldY, nrhs, alpha_one, beta_mode, job_ptr, rpb);      \
+                           alpha, X, ldX, beta, Y, ldY, nrhs, alpha_one, beta_mode, job_ptr, rpb, skip); \
         break;
     switch (G) {
         TG_SPMM(1)
@@ -361,13 +364,13 @@ static int launch_spmm_v(int G, int32_t m, const RP* rp, const int32_t* ci, cons
 template <typename T, typename RP>
 static int spmm_typed(int32_t m, int64_t nnz, const RP* rp, const int32_t* ci, const T* a, double alpha_d,
                       const T* X, int64_t ldX, double beta_d, T* Y, int64_t ldY, int32_t nrhs,
-                      const int32_t* job_ptr, int32_t blocks, hipStream_t st) {
+                      const int32_t* job_ptr, int32_t blocks, const int* skip, hipStream_t st) {
     const T alpha = (T)alpha_d, beta = (T)beta_d;
     const int alpha_one = (alpha_d == 1.0);
     const int beta_mode = (beta_d == 0.0) ? kBetaZero : (beta_d == 1.0) ? kBetaOne : kBetaGeneric;
     if (nrhs == 1)
         return launch_spmv<T, RP>(m, nnz, rp, ci, a, alpha, X, ldX, beta, Y, ldY, alpha_one, beta_mode, job_ptr,
-                                  blocks, st);
+                                  blocks, skip, st);
     // Vector width: 16-byte loads when X and Y rows allow it.
     const auto aligned = [&](int bytes) {
         return ((uintptr_t)X % bytes == 0) && ((uintptr_t)Y % bytes == 0) &&
@@ -386,14 +389,14 @@ static int spmm_typed(int32_t m, int64_t nnz, const RP* rp, const int32_t* ci, c
     switch (V) {
         case 1:
             return launch_spmm_v<T, RP, 1>(G, m, rp, ci, a, alpha, X, ldX, beta, Y, ldY, nrhs, alpha_one,
-                                           beta_mode, job_ptr, blocks, st);
+                                           beta_mode, job_ptr, blocks, skip, st);
         case 2:
             return launch_spmm_v<T, RP, 2>(G, m, rp, ci, a, alpha, X, ldX, beta, Y, ldY, nrhs, alpha_one,
-                                           beta_mode, job_ptr, blocks, st);
+                                           beta_mode, job_ptr, blocks, skip, st);
         default:
             if constexpr (sizeof(T) == 4)
                 return launch_spmm_v<T, RP, 4>(G, m, rp, ci, a, alpha, X, ldX, beta, Y, ldY, nrhs, alpha_one,
-                                               beta_mode, job_ptr, blocks, st);
+                                               beta_mode, job_ptr, blocks, skip, st);
             return fail(TG_ERR_INVALID, "internal: bad vector width");
     }
 }
@@ -450,36 +453,86 @@ extern "C" int tg_axpby(int32_t m, int32_t nrhs, tg_dtype val_type, double alpha
     return fail(TG_ERR_DTYPE, "tg_axpby: val_type must be TG_F64 or TG_F32");
 }
 
-extern "C" int tg_csr_spmm(int32_t m, int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, const int32_t* ci, const void* a,
-                           tg_dtype val_type, double alpha, const void* X, int64_t ldX, double beta, void* Y,
-                           int64_t ldY, int32_t nrhs, int32_t NRHS, const int32_t* job_ptr, int32_t blocks,
+namespace tg {
+
+// Random gathers of X rows only stay cheap while X fits the 256 MB Infinity
+// Cache; wider blocks are processed in column tiles of at most this many
+// bytes of X (A is re-streamed once per tile).
+constexpr int64_t kXTileBytes = 128ll << 20;
+
+int spmm_dispatch(const CsrView& A, double alpha, const void* X, int64_t ldX, double beta, void* Y, int64_t ldY,
+                  int32_t nrhs, const int32_t* job_ptr, int32_t blocks, const int* skip, hipStream_t st) {
+    const int32_t m = A.m, n = A.n;
+    if (m < 0 || n < 0 || A.nnz < 0 || nrhs < 0) return fail(TG_ERR_INVALID, "spmm: negative size");
+    if (A.val_type != TG_F64 && A.val_type != TG_F32) return fail(TG_ERR_DTYPE, "spmm: val_type must be F64/F32");
+    if (A.rp_type != TG_I32 && A.rp_type != TG_I64) return fail(TG_ERR_DTYPE, "spmm: rp_type must be I32/I64");
+    if (m == 0 || nrhs == 0) return TG_OK;
+    if (!Y || ldY < nrhs) return fail(TG_ERR_INVALID, "spmm: Y is null or ldY < nrhs");
+    if (job_ptr && blocks < 1) return fail(TG_ERR_INVALID, "spmm: job_ptr given with blocks < 1");
+    const bool f64 = A.val_type == TG_F64;
+    if (alpha == 0.0) {  // SpMM.hpp:65-71, SpMV.hpp:15-30
+        if (f64) return axpby_typed<double>(m, nrhs, 0.0, nullptr, 0, beta, (double*)Y, ldY, st);
+        return axpby_typed<float>(m, nrhs, 0.0, nullptr, 0, beta, (float*)Y, ldY, st);
+    }
+    if (X == Y) return fail(TG_ERR_ALIAS, "spmm: input and output pointer coincide. Doing nothing.");
+    if (!A.rp || !X || ldX < nrhs) return fail(TG_ERR_INVALID, "spmm: rp/X null or ldX < nrhs");
+    if (A.nnz > 0 && !A.ci) return fail(TG_ERR_INVALID, "spmm: ci is null");
+    const size_t vs = f64 ? 8 : 4;
+    int32_t kt = nrhs;
+    if ((int64_t)n * nrhs * (int64_t)vs > kXTileBytes && nrhs > 8) {
+        kt = 8;
+        while ((int64_t)n * (2 * kt) * (int64_t)vs <= kXTileBytes && 2 * kt < nrhs) kt *= 2;
+    }
+    for (int32_t c0 = 0; c0 < nrhs; c0 += kt) {
+        const int32_t kc = std::min(kt, nrhs - c0);
+        int rc;
+        if (f64) {
+            const double* Xc = (const double*)X + c0;
+            double* Yc = (double*)Y + c0;
+            if (A.rp_type == TG_I32)
+                rc = spmm_typed<double, int32_t>(m, A.nnz, (const int32_t*)A.rp, A.ci, (const double*)A.val, alpha, Xc,
+                                                 ldX, beta, Yc, ldY, kc, job_ptr, blocks, skip, st);
+            else
+                rc = spmm_typed<double, int64_t>(m, A.nnz, (const int64_t*)A.rp, A.ci, (const double*)A.val, alpha, Xc,
+                                                 ldX, beta, Yc, ldY, kc, job_ptr, blocks, skip, st);
+        } else {
+            const float* Xc = (const float*)X + c0;
+            float* Yc = (float*)Y + c0;
+            if (A.rp_type == TG_I32)
+                rc = spmm_typed<float, int32_t>(m, A.nnz, (const int32_t*)A.rp, A.ci, (const float*)A.val, alpha, Xc,
+                                                ldX, beta, Yc, ldY, kc, job_ptr, blocks, skip, st);
+            else
+                rc = spmm_typed<float, int64_t>(m, A.nnz, (const int64_t*)A.rp, A.ci, (const float*)A.val, alpha, Xc,
+                                                ldX, beta, Yc, ldY, kc, job_ptr, blocks, skip, st);
+        }
+        if (rc) return rc;
+    }
+    return TG_OK;
+}
+
+int op_apply(const OpRef& op, const void* x, void* y, int k, const int* skip, hipStream_t s) {
+    if (op.csr) return spmm_dispatch(*op.csr, 1.0, x, k, 0.0, y, k, k, nullptr, 0, skip, s);
+    const int rc = op.fn(op.ctx, x, y, k, (void*)s);
+    if (rc) return fail(TG_ERR_INVALID, "operator callback returned %d", rc);
+    return TG_OK;
+}
+
+}  // namespace tg
+
+extern "C" int tg_csr_spmm(int32_t m, int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, const int32_t* ci,
+                           const void* a, tg_dtype val_type, double alpha, const void* X, int64_t ldX, double beta,
+                           void* Y, int64_t ldY, int32_t nrhs, int32_t NRHS, const int32_t* job_ptr, int32_t blocks,
                            void* stream) {
     if (NRHS > 0 && nrhs != NRHS)
         return fail(TG_ERR_NRHS, "tg_csr_spmm: nrhs (%d) != NRHS (%d). Doing nothing.", nrhs, NRHS);
-    if (m < 0 || n < 0 || nnz < 0 || nrhs < 0) return fail(TG_ERR_INVALID, "tg_csr_spmm: negative size");
-    if (val_type != TG_F64 && val_type != TG_F32) return fail(TG_ERR_DTYPE, "tg_csr_spmm: val_type must be F64/F32");
-    if (rp_type != TG_I32 && rp_type != TG_I64) return fail(TG_ERR_DTYPE, "tg_csr_spmm: rp_type must be I32/I64");
-    if (m == 0 || nrhs == 0) return TG_OK;
-    if (!Y || ldY < nrhs) return fail(TG_ERR_INVALID, "tg_csr_spmm: Y is null or ldY < nrhs");
-    if (job_ptr && blocks < 1) return fail(TG_ERR_INVALID, "tg_csr_spmm: job_ptr given with blocks < 1");
-    hipStream_t st = as_stream(stream);
-    if (alpha == 0.0) {  // SpMM.hpp:65-71, SpMV.hpp:15-30
-        if (val_type == TG_F64) return axpby_typed<double>(m, nrhs, 0.0, nullptr, 0, beta, (double*)Y, ldY, st);
-        return axpby_typed<float>(m, nrhs, 0.0, nullptr, 0, beta, (float*)Y, ldY, st);
-    }
-    if (X == Y) return fail(TG_ERR_ALIAS, "tg_csr_spmm: input and output pointer coincide. Doing nothing.");
-    if (!rp || !X || ldX < nrhs) return fail(TG_ERR_INVALID, "tg_csr_spmm: rp/X null or ldX < nrhs");
-    if (nnz > 0 && !ci) return fail(TG_ERR_INVALID, "tg_csr_spmm: ci is null");
-    if (val_type == TG_F64) {
-        if (rp_type == TG_I32)
-            return spmm_typed<double, int32_t>(m, nnz, (const int32_t*)rp, ci, (const double*)a, alpha,
-                                               (const double*)X, ldX, beta, (double*)Y, ldY, nrhs, job_ptr, blocks, st);
-        return spmm_typed<double, int64_t>(m, nnz, (const int64_t*)rp, ci, (const double*)a, alpha, (const double*)X,
-                                           ldX, beta, (double*)Y, ldY, nrhs, job_ptr, blocks, st);
-    }
-    if (rp_type == TG_I32)
-        return spmm_typed<float, int32_t>(m, nnz, (const int32_t*)rp, ci, (const float*)a, alpha, (const float*)X,
-                                          ldX, beta, (float*)Y, ldY, nrhs, job_ptr, blocks, st);
-    return spmm_typed<float, int64_t>(m, nnz, (const int64_t*)rp, ci, (const float*)a, alpha, (const float*)X, ldX,
-                                      beta, (float*)Y, ldY, nrhs, job_ptr, blocks, st);
+    CsrView A;
+    A.m = m;
+    A.n = n;
+    A.nnz = nnz;
+    A.rp = rp;
+    A.rp_type = rp_type;
+    A.ci = ci;
+    A.val = a;
+    A.val_type = val_type;
+    return spmm_dispatch(A, alpha, X, ldX, beta, Y, ldY, nrhs, job_ptr, blocks, nullptr, as_stream(stream));
 }

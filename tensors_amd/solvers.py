@@ -1,0 +1,149 @@
+"""Python mirror of the ConjugateGradient / GMRES drivers over the C ABI.
+
+``ConjugateGradient(n, max_iter, nrhs)`` keeps its device workspace between calls, as the
+reference keeps its Tensor2 workspaces (ConjugateGradient.hpp:38-52, 64-88); calling it
+runs tg_cg_solve on CUDA tensors.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _capi as capi
+from .sparse import MatrixCSR, stream_handle
+
+_TG = {torch.float64: capi.F64, torch.float32: capi.F32}
+
+
+class ConjugateGradient:
+    """ConjugateGradient<VarSize,Scal,Int>(n, max_iter, nrhs) (ConjugateGradient.hpp:64-88)."""
+
+    def __init__(self, n, max_iter, nrhs=1, dtype=torch.float64, device="cuda"):
+        self.n, self.max_iter, self.nrhs = int(n), int(max(0, max_iter)), int(nrhs)
+        self.dtype, self.device = dtype, device
+        self._ws = None
+        self.iter = 0
+        self.succeeded = False
+        self.rel_res = np.zeros(self.nrhs)
+
+    def workspace(self, prec_kind):
+        nb = capi.lib().tg_cg_workspace_bytes(self.n, self.nrhs, _TG[self.dtype], prec_kind)
+        if self._ws is None or self._ws.numel() < nb:
+            self._ws = torch.empty(nb, dtype=torch.uint8, device=self.device)
+        return self._ws, nb
+
+    def __call__(self, A: MatrixCSR, P, a, B, b, X, tol, use_initial_guess=False, stream=None):
+        """X <- a A^{-1} B + b X.  P: None (identity) or a CUDA tensor d (z = d * r)."""
+        prec_kind = 0 if P is None else 1
+        ws, nb = self.workspace(prec_kind)
+        rp, ci, vals = A.device_arrays(self.dtype)
+        ok = C.c_int32(0)
+        it = C.c_int32(0)
+        rr = np.zeros(self.nrhs, np.float64)
+        capi.call("tg_cg_solve", self.n, A.NonzeroCount(), rp.data_ptr(),
+                  capi.I64 if rp.dtype == torch.int64 else capi.I32, ci.data_ptr(),
+                  vals.data_ptr() if vals is not None else None, _TG[self.dtype], prec_kind,
+                  P.data_ptr() if P is not None else None, self.nrhs, self.max_iter, float(a), B.data_ptr(),
+                  B.stride(0) if B.dim() == 2 else 1, float(b), X.data_ptr(),
+                  X.stride(0) if X.dim() == 2 else 1, float(tol), int(use_initial_guess), ws.data_ptr(), nb,
+                  C.byref(ok), C.byref(it), rr.ctypes.data, stream_handle(stream))
+        self.iter, self.succeeded, self.rel_res = it.value, bool(ok.value), rr
+        return self.succeeded
+
+    def IterationCount(self):
+        return self.iter
+
+    def RelativeResiduals(self):
+        return self.rel_res
+
+
+class GMRES:
+    """GMRES<VarSize,Scal,Int,Side>(n, max_iter, nrhs) (GMRES.hpp:82-108); side 0 = Left, 1 = Right."""
+
+    def __init__(self, n, max_iter, nrhs=1, side=0, dtype=torch.float64, device="cuda"):
+        self.n, self.max_iter, self.nrhs, self.side = int(n), int(max_iter), int(nrhs), int(side)
+        self.dtype, self.device = dtype, device
+        self._ws = None
+        self.iter = 0
+        self.restarts = 0
+        self.succeeded = False
+        self.rel_res = np.zeros(self.nrhs)
+
+    def workspace(self):
+        nb = capi.lib().tg_gmres_workspace_bytes(self.n, self.max_iter, self.nrhs, _TG[self.dtype])
+        if self._ws is None or self._ws.numel() < nb:
+            self._ws = torch.empty(nb, dtype=torch.uint8, device=self.device)
+        return self._ws, nb
+
+    def __call__(self, A: MatrixCSR, P, a, B, b, X, tol, max_restarts, use_initial_guess=False, stream=None):
+        prec_kind = 0 if P is None else 1
+        ws, nb = self.workspace()
+        rp, ci, vals = A.device_arrays(self.dtype)
+        ok = C.c_int32(0)
+        it = C.c_int32(0)
+        rs = C.c_int32(0)
+        rr = np.zeros(self.nrhs, np.float64)
+        capi.call("tg_gmres_solve", self.n, A.NonzeroCount(), rp.data_ptr(),
+                  capi.I64 if rp.dtype == torch.int64 else capi.I32, ci.data_ptr(),
+                  vals.data_ptr() if vals is not None else None, _TG[self.dtype], prec_kind,
+                  P.data_ptr() if P is not None else None, self.side, self.nrhs, self.max_iter, float(a),
+                  B.data_ptr(), B.stride(0) if B.dim() == 2 else 1, float(b), X.data_ptr(),
+                  X.stride(0) if X.dim() == 2 else 1, float(tol), int(max_restarts), int(use_initial_guess),
+                  ws.data_ptr(), nb, C.byref(ok), C.byref(it), C.byref(rs), rr.ctypes.data,
+                  stream_handle(stream))
+        self.iter, self.restarts, self.succeeded, self.rel_res = it.value, rs.value, bool(ok.value), rr
+        return self.succeeded
+
+    def IterationCount(self):
+        return self.iter
+
+    def RestartCount(self):
+        return self.restarts
+
+    def RelativeResiduals(self):
+        return self.rel_res
+
+
+def csr_callback(A: MatrixCSR, dtype=torch.float64):
+    """An operator callback (tg_apply_fn) that applies the CSR matrix A with tg_csr_spmm:
+    the analogue of the reference's `[&](cptr x, mptr y){ A.Dot(1, x, 0, y, nrhs); }`
+    lambdas (Mathematica/Dev_CG.nb:104-123).  Keep the returned object alive."""
+    rp, ci, vals = A.device_arrays(dtype)
+    tg = _TG[dtype]
+    rpt = capi.I64 if rp.dtype == torch.int64 else capi.I32
+
+    def fn(ctx, x, y, k, stream):
+        return capi.lib().tg_csr_spmm(A.m, A.n, A.NonzeroCount(), rp.data_ptr(), rpt, ci.data_ptr(),
+                                      vals.data_ptr() if vals is not None else None, tg, 1.0, x, k, 0.0, y, k, k,
+                                      0, None, 0, stream)
+
+    return capi.APPLY_FN(fn)
+
+
+def cg_solve_op(n, nrhs, max_iter, apply_A, apply_P, a, B, b, X, tol, use_initial_guess=False,
+                dtype=torch.float64, stream=None):
+    """tg_cg_solve_op with callbacks; returns (succeeded, iterations, rel_res)."""
+    nb = capi.lib().tg_cg_workspace_bytes(n, nrhs, _TG[dtype], 1 if apply_P is not None else 0)
+    ws = torch.empty(nb, dtype=torch.uint8, device=B.device)
+    ok, it = C.c_int32(0), C.c_int32(0)
+    rr = np.zeros(nrhs)
+    capi.call("tg_cg_solve_op", n, apply_A, None, apply_P, None, _TG[dtype], nrhs, max_iter, float(a),
+              B.data_ptr(), B.stride(0), float(b), X.data_ptr(), X.stride(0), float(tol), int(use_initial_guess),
+              ws.data_ptr(), nb, C.byref(ok), C.byref(it), rr.ctypes.data, stream_handle(stream))
+    return bool(ok.value), it.value, rr
+
+
+def gmres_solve_op(n, nrhs, max_iter, apply_A, apply_P, a, B, b, X, tol, max_restarts, side=0,
+                   use_initial_guess=False, dtype=torch.float64, stream=None):
+    """tg_gmres_solve_op with callbacks; returns (succeeded, iterations, restarts, rel_res)."""
+    nb = capi.lib().tg_gmres_workspace_bytes(n, max_iter, nrhs, _TG[dtype])
+    ws = torch.empty(nb, dtype=torch.uint8, device=B.device)
+    ok, it, rs = C.c_int32(0), C.c_int32(0), C.c_int32(0)
+    rr = np.zeros(nrhs)
+    capi.call("tg_gmres_solve_op", n, apply_A, None, apply_P, None, _TG[dtype], side, nrhs, max_iter, float(a),
+              B.data_ptr(), B.stride(0), float(b), X.data_ptr(), X.stride(0), float(tol), int(max_restarts),
+              int(use_initial_guess), ws.data_ptr(), nb, C.byref(ok), C.byref(it), C.byref(rs), rr.ctypes.data,
+              stream_handle(stream))
+    return bool(ok.value), it.value, rs.value, rr
