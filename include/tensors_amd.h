@@ -165,6 +165,46 @@ int tg_gmres_solve(int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, con
                    size_t workspace_bytes, int32_t* succeeded, int32_t* iterations, int32_t* restarts,
                    double* rel_residuals, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Structure work on the GPU (integer results are exact and deterministic).
+ *
+ * Triples -> CSR: replaces MatrixCSR::FromTriples (MatrixCSR/FromTriples.hpp:
+ * 3-209) with SortInner (MatrixCSR/SortInner.hpp:3-49) and Compress
+ * (PatternCSR/Compress.hpp:3-214): rows bucketed, each row sorted by column,
+ * duplicates summed in input order when `compress`, (j,i,a) added for every
+ * off-diagonal (i,j,a) when `symmetrize`.  V may be NULL (pattern only).
+ * Outputs: rp_out[m+1] (rp_type), ci_out / val_out of `capacity` >=
+ * count * (symmetrize ? 2 : 1) entries; *nnz_out (host) = the nonzero count.
+ * Out-of-range indices -> TG_ERR_INVALID.  Synchronises `stream`.
+ * ------------------------------------------------------------------------ */
+size_t tg_csr_from_triples_workspace_bytes(int64_t count, int32_t m, int32_t symmetrize);
+int tg_csr_from_triples(int64_t count, const int32_t* I, const int32_t* J, const void* V, tg_dtype val_type,
+                        int32_t m, int32_t n, int32_t compress, int32_t symmetrize, void* rp_out, tg_dtype rp_type,
+                        int32_t* ci_out, void* val_out, int64_t capacity, void* workspace, size_t workspace_bytes,
+                        int64_t* nnz_out, void* stream);
+
+/* Transpose: replaces MatrixCSR::Op<Op::Trans> / Transpose (MatrixCSR.hpp:
+ * 389-495, CreateTransposeCounters PatternCSR.hpp:642-684).  out_rp[n+1],
+ * out_ci / out_val[nnz] (val may be NULL).  Synchronises `stream`. */
+size_t tg_csr_transpose_workspace_bytes(int32_t m, int32_t n, int64_t nnz);
+int tg_csr_transpose(int32_t m, int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, const int32_t* ci,
+                     const void* val, tg_dtype val_type, void* out_rp, int32_t* out_ci, void* out_val,
+                     void* workspace, size_t workspace_bytes, void* stream);
+
+/* Sparse x sparse C = A B (A m x k, B k x n): replaces MatrixCSR::Dot(MatrixCSR)
+ * (MatrixCSR.hpp:835-921) and, with both value arrays NULL, the pattern product
+ * PatternCSR::DotBinary_ (PatternCSR.hpp:731-839).  Step 1 gives the expanded
+ * count E (workspace >= 2*align256(8(m+1)) + scan scratch; 1 MiB + 16(m+1)
+ * bytes is enough); step 2 writes C with capacity E and returns its nnz.
+ * Both synchronise `stream`. */
+int tg_csr_spgemm_expanded_count(int32_t m, const void* Arp, const int32_t* Aci, const void* Brp, tg_dtype rp_type,
+                                 int64_t* count_out, void* workspace, size_t workspace_bytes, void* stream);
+size_t tg_csr_spgemm_workspace_bytes(int32_t m, int32_t n, int64_t expanded, tg_dtype val_type);
+int tg_csr_spgemm(int32_t m, int32_t k, int32_t n, const void* Arp, const int32_t* Aci, const void* Aval,
+                  const void* Brp, const int32_t* Bci, const void* Bval, tg_dtype rp_type, tg_dtype val_type,
+                  int64_t expanded, void* Crp, int32_t* Cci, void* Cval, void* workspace, size_t workspace_bytes,
+                  int64_t* nnz_out, void* stream);
+
 /* Streaming probes (no reference counterpart): dst <- src over `bytes`
  * (multiple of 16), and a read-only pass over src (sink: one device float that
  * is practically never written).  They measure the achievable HBM bandwidth

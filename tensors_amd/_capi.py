@@ -43,6 +43,15 @@ SIGNATURES = {
                                  vp, C.c_size_t, vp, vp, vp, vp]),
     "tg_gmres_solve_op": (C.c_int, [i32, vp, vp, vp, vp, C.c_int, i32, i32, i32, dbl, vp, i64, dbl, vp, i64,
                                     dbl, i32, i32, vp, C.c_size_t, vp, vp, vp, vp, vp]),
+    "tg_csr_from_triples_workspace_bytes": (C.c_size_t, [i64, i32, i32]),
+    "tg_csr_from_triples": (C.c_int, [i64, vp, vp, vp, C.c_int, i32, i32, i32, i32, vp, C.c_int, vp, vp, i64,
+                                      vp, C.c_size_t, vp, vp]),
+    "tg_csr_transpose_workspace_bytes": (C.c_size_t, [i32, i32, i64]),
+    "tg_csr_transpose": (C.c_int, [i32, i32, i64, vp, C.c_int, vp, vp, C.c_int, vp, vp, vp, vp, C.c_size_t, vp]),
+    "tg_csr_spgemm_expanded_count": (C.c_int, [i32, vp, vp, vp, C.c_int, vp, vp, C.c_size_t, vp]),
+    "tg_csr_spgemm_workspace_bytes": (C.c_size_t, [i32, i32, i64, C.c_int]),
+    "tg_csr_spgemm": (C.c_int, [i32, i32, i32, vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, i64, vp, vp, vp, vp,
+                                C.c_size_t, vp, vp]),
     "tg_probe_copy": (C.c_int, [vp, vp, i64, vp]),
     "tg_probe_read": (C.c_int, [vp, i64, vp, vp]),
     "tg_gen_uniform": (C.c_int, [u64, u64, i64, C.c_int, vp]),
