@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cg", action="store_true", help="skip the CG (configs 3) section")
+    ap.add_argument("--cg-grid", type=int, default=128)
+    ap.add_argument("--cg-reps", type=int, default=2)
     return ap.parse_args()
 
 
@@ -95,6 +98,77 @@ def cpu_baseline(rp, ci, v64, v32, X64, X32, budget_s):
         "kind": "port",
         "sample": f"the same 8 SpMMs (FP64/FP32 x nrhs 1,4,16,64) on the full n=1e6 matrix, "
                   f"{reps_done} timed calls in ~{budget_s:.0f}s, fixed-NRHS path, -O3 -ffast-math",
+    }
+
+
+def cg_section(args, dev):
+    """Single SPD solve: 3D 7-point Laplacian on a g^3 grid + 1e-2 shift, 32 RHS
+    uniform[-1,1] (seed 3003), to 1e-10 relative residual: plain FP64 CG (the
+    ConjugateGradient recurrences, P = I, |r.r| <= tol^2 |b.b|) vs FP32-inner CG with
+    FP64 refinement (true ||b - A x|| / ||b|| <= 1e-10).  Reports solves/s, per-solve
+    latencies, iterations, true residuals and the mixed result's error vs FP64."""
+    import torch
+
+    from tensors_amd import _capi as capi
+    from tensors_amd import generators as gen
+    from tensors_amd.solvers import ConjugateGradient, cg_solve_mixed
+    from tensors_amd.sparse import MatrixCSR, spmm_raw
+
+    g, k, tol = args.cg_grid, 32, 1e-10
+    rp, ci, val = gen.laplacian_3d(g, 1e-2)
+    n = g ** 3
+    A = MatrixCSR(rp.astype(np.int32), ci, val, n, n)
+    B = torch.from_numpy(gen.uniform(3003, n * k).reshape(n, k)).to(dev)
+    rpd, cid, vd = A.device_arrays(torch.float64)
+    nnz = A.NonzeroCount()
+
+    def true_res(X):
+        R = B.clone()
+        spmm_raw(n, n, nnz, rpd, cid, vd, -1.0, X, k, 1.0, R, k, k)
+        return (R.norm(dim=0) / B.norm(dim=0)).max().item()
+
+    cg = ConjugateGradient(n, 5000, k)
+    X64 = torch.zeros(n, k, dtype=torch.float64, device=dev)
+    cg(A, None, 1.0, B, 0.0, X64, tol)  # warm-up (also allocates)
+    lat64 = []
+    for _ in range(args.cg_reps):
+        X64.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ok64 = cg(A, None, 1.0, B, 0.0, X64, tol)
+        torch.cuda.synchronize()
+        lat64.append(time.perf_counter() - t0)
+    it64 = cg.IterationCount()
+    res64 = true_res(X64)
+    Xm = torch.zeros(n, k, dtype=torch.float64, device=dev)
+    _, a32 = None, A.device_arrays(torch.float32)[2]
+    cg_solve_mixed(A, B, Xm, tol=tol, a32=a32)  # warm-up
+    latm = []
+    for _ in range(args.cg_reps):
+        Xm.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        okm, outer, inner, rr = cg_solve_mixed(A, B, Xm, tol=tol, a32=a32)
+        torch.cuda.synchronize()
+        latm.append(time.perf_counter() - t0)
+    resm = true_res(Xm)
+    err = ((Xm - X64).abs().max() / X64.abs().max()).item()
+    # algorithmic bytes of one FP64 CG iteration (P = I): SpMM + 11 vector passes
+    spmm_b = nnz * 12 + 4 * (n + 1) + 2 * n * k * 8
+    it_bytes = spmm_b + 11 * n * k * 8
+    t64 = float(np.median(lat64))
+    tm = float(np.median(latm))
+    return {
+        "workload": f"3D 7-point Laplacian {g}^3 + 1e-2 shift, n={n}, nnz={nnz}, 32 RHS uniform[-1,1], "
+                    f"tol {tol} relative residual, P=I",
+        "fp64": {"solves_per_s": round(1.0 / t64, 3), "ms_per_solve": round(t64 * 1e3, 2),
+                 "latencies_ms": [round(x * 1e3, 2) for x in lat64], "iterations": it64, "succeeded": bool(ok64),
+                 "true_rel_residual": res64,
+                 "GBps_per_iteration": round(it_bytes * it64 / t64 / 1e9, 1)},
+        "mixed": {"solves_per_s": round(1.0 / tm, 3), "ms_per_solve": round(tm * 1e3, 2),
+                  "latencies_ms": [round(x * 1e3, 2) for x in latm], "outer": outer, "inner_iterations": inner,
+                  "succeeded": bool(okm), "true_rel_residual": resm, "error_vs_fp64": err},
+        "speedup_mixed_over_fp64": round(t64 / tm, 3),
     }
 
 
@@ -209,6 +283,11 @@ def main():
         "hbm_measured": {"copy_GBps": round(copy_gbps, 1), "read_GBps": round(read_gbps, 1)},
         "breakdown": breakdown,
     }
+    # free the SpMM suite's buffers before the solver section
+    del cases, vals, Xs, rp32, cid
+    torch.cuda.empty_cache()
+    if not args.no_cg:
+        out["cg"] = cg_section(args, dev)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(rp, ci, v64, v32, X64h, X32h, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -132,6 +132,30 @@ int tg_cg_solve(int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, const 
                 int32_t* iterations, double* rel_residuals, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Mixed-precision CG: FP32 inner CG sweeps (FP32 values a32 on the same
+ * pattern, FP32 Krylov vectors, the recurrences of tg_cg_solve) inside an FP64
+ * iterative-refinement loop whose residual b - A x, solution update and
+ * stopping norms are FP64.  No reference counterpart (the reference CG is
+ * single-precision-agnostic); it solves what tg_cg_solve solves:
+ *
+ *   X <- a_coef * A^{-1} B + b_coef * X      (FP64 in and out)
+ *
+ * Stops when every column has ||B - A x||_2 / ||B||_2 <= tol (true FP64
+ * residual), or after max_outer refinement steps; each inner solve runs at most
+ * max_inner iterations down to max(inner_tol, 0.3 tol / current residual).
+ * a32 may be NULL (the FP64 values are rounded into the workspace; pass
+ * with_a32 = 0 to the workspace query then).  rel_residuals[nrhs] are the final
+ * true relative residuals.  Synchronises `stream`.
+ * ------------------------------------------------------------------------ */
+size_t tg_cg_mixed_workspace_bytes(int32_t n, int64_t nnz, int32_t nrhs, int32_t prec_kind, int32_t with_a32);
+int tg_cg_solve_mixed(int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, const int32_t* ci,
+                      const double* a64, const float* a32, int32_t prec_kind, const double* prec_diag,
+                      int32_t nrhs, int32_t max_outer, int32_t max_inner, double inner_tol, double a_coef,
+                      const double* B, int64_t ldB, double b_coef, double* X, int64_t ldX, double tol,
+                      int32_t use_guess, void* workspace, size_t workspace_bytes, int32_t* succeeded,
+                      int32_t* outer_iterations, int32_t* inner_iterations, double* rel_residuals, void* stream);
+
+/* ------------------------------------------------------------------------
  * Restarted GMRES(m) -- replaces GMRES<NRHS,Scal,Int,Side>::operator()(A, P,
  * a, B, ldB, b, X, ldX, tol, max_restarts, use_initial_guessQ)
  * (GMRES.hpp:127-260) for a CSR operator and P = identity (prec_kind 0) or a

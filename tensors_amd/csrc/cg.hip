@@ -22,59 +22,9 @@
 
 #include "common.hpp"
 #include "krylov.hpp"
+#include "reduce.hpp"
 
 namespace tg {
-
-__device__ __forceinline__ void block_row_range(int n, int& i0, int& i1) {
-    const int b = blockIdx.x, B = gridDim.x;
-    i0 = (int)((int64_t)n * b / B);
-    i1 = (int)((int64_t)n * (b + 1) / B);
-}
-
-// Thread layout of the column-blocked elementwise kernels: c = column lane,
-// ro = row lane; kBlock / kpad rows are processed per sweep.
-struct Lanes {
-    int c, ro, rows;
-};
-__device__ __forceinline__ Lanes lanes_of(int kpad) {
-    Lanes L;
-    L.c = threadIdx.x % kpad;
-    L.ro = threadIdx.x / kpad;
-    L.rows = kBlock / kpad;
-    return L;
-}
-
-// Reduce per-thread partials of the same column over the row lanes (fixed
-// tree) and store the workgroup partial partial[block * k + c].
-__device__ __forceinline__ void column_partial(double v, const Lanes& L, int k, int kpad, double* partial) {
-    __shared__ double sh[kBlock];
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = L.rows / 2; s > 0; s >>= 1) {
-        if (L.ro < s) sh[threadIdx.x] += sh[threadIdx.x + s * kpad];
-        __syncthreads();
-    }
-    if (L.ro == 0 && L.c < k) partial[(int64_t)blockIdx.x * k + L.c] = sh[L.c];
-}
-
-// Sum of the B workgroup partials of each column in index order, by one
-// workgroup: thread (c, j) adds partials j, j+J, ...; then a fixed tree over j.
-__device__ __forceinline__ double column_total(const double* partial, int B, int k, int kpad, int c, int j,
-                                               int J) {
-    __shared__ double sh[kBlock];
-    double s = 0.0;
-    if (c < k)
-        for (int b = j; b < B; b += J) s += partial[(int64_t)b * k + c];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int h = J / 2; h > 0; h >>= 1) {
-        if (j < h) sh[threadIdx.x] += sh[threadIdx.x + h * kpad];
-        __syncthreads();
-    }
-    const double t = sh[c];
-    __syncthreads();
-    return t;
-}
 
 // --- setup ----------------------------------------------------------------
 // r = B;  z = P r;  x = 0 (unless guessing);  partial(r.z)

@@ -1,0 +1,58 @@
+"""Mixed-precision CG (FP32 inner sweeps + FP64 refinement) against the FP64 oracle:
+the true FP64 residual reaches 1e-10 and the solution matches the oracle's FP64 CG."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+import oracle
+from helpers import rel_err
+from tensors_amd import generators as gen
+from tensors_amd.solvers import cg_solve_mixed
+from tensors_amd.sparse import MatrixCSR
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("k", [1, 4, 32])
+@pytest.mark.parametrize("prec", [False, True])
+def test_mixed_reaches_true_residual_and_matches_oracle(cuda, k, prec):
+    g = 24
+    rp, ci, val = gen.laplacian_3d(g, 1e-2)
+    n = g ** 3
+    B = gen.uniform(3003 + k, n * k).reshape(n, k)
+    d = 1.0 / sp.csr_matrix((val, ci, rp), shape=(n, n)).diagonal() if prec else None
+    A = MatrixCSR(rp.astype(np.int32), ci, val, n, n)
+    Xd = torch.zeros(n, k, dtype=torch.float64, device=cuda)
+    ok, outer, inner, rr = cg_solve_mixed(A, torch.from_numpy(B).to(cuda), Xd, tol=1e-10,
+                                          P=None if d is None else torch.from_numpy(d).to(cuda))
+    assert ok and outer >= 1 and np.all(rr <= 1e-10)
+    X = Xd.cpu().numpy()
+    Dm = sp.csr_matrix((val, ci, rp), shape=(n, n))
+    true = np.linalg.norm(B - Dm @ X, axis=0) / np.linalg.norm(B, axis=0)
+    assert np.all(true <= 1.5e-10)
+    Xo = np.zeros((n, k))
+    oracle.cg(n, rp, ci, val, B, Xo, k, 5000, 1e-12, threads=8)
+    assert rel_err(X, Xo) < 1e-7
+
+
+def test_mixed_alpha_beta_and_special_cases(cuda):
+    g = 10
+    rp, ci, val = gen.laplacian_3d(g, 1e-2)
+    n, k = g ** 3, 2
+    A = MatrixCSR(rp, ci, val, n, n)
+    rng = np.random.default_rng(0)
+    B = rng.uniform(-1, 1, (n, k))
+    X0 = rng.uniform(-1, 1, (n, k))
+    Xd = torch.from_numpy(X0.copy()).to(cuda)
+    ok, _, _, _ = cg_solve_mixed(A, torch.from_numpy(B).to(cuda), Xd, tol=1e-11, a=-2.0, b=0.5)
+    Dm = sp.csr_matrix((val, ci, rp), shape=(n, n)).toarray()
+    assert ok and rel_err(Xd.cpu().numpy(), -2.0 * np.linalg.solve(Dm, B) + 0.5 * X0) < 1e-9
+    Xd = torch.full((n, k), 3.0, dtype=torch.float64, device=cuda)
+    ok, outer, inner, _ = cg_solve_mixed(A, torch.zeros(n, k, dtype=torch.float64, device=cuda), Xd, b=2.0)
+    assert ok and outer == 0 and torch.all(Xd == 6.0)
+    Bn = torch.ones(n, k, dtype=torch.float64, device=cuda)
+    Bn[1, 1] = float("nan")
+    Xd = torch.full((n, k), 3.0, dtype=torch.float64, device=cuda)
+    ok, _, _, _ = cg_solve_mixed(A, Bn, Xd)
+    assert not ok and torch.all(Xd == 3.0)
