@@ -89,6 +89,13 @@ int tg_csr_spmm(int32_t m, int32_t n, int64_t nnz, const void* rp, tg_dtype rp_t
                 int64_t ldY, int32_t nrhs, int32_t NRHS, const int32_t* job_ptr, int32_t blocks,
                 void* stream);
 
+/* Kernel-shape knobs read by tg_csr_spmm (no reference counterpart; the
+ * reference picks its kernel by template arguments).  Keys:
+ *   "spmv_kernel": nrhs == 1 kernel, 0 = streaming (default), 1 = team-per-row.
+ * Process-wide; not thread-safe against concurrent SpMM calls. */
+int tg_tune_set(const char* key, int32_t value);
+int32_t tg_tune_get(const char* key);
+
 /* Y <- alpha * X + beta * Y over an m x nrhs block (flag shortcuts of
  * combine_matrices_auto: beta 0 -> Y not read; alpha 0 -> X not read).
  * Replaces Tools' combine_matrices_auto / scale_matrix as used by

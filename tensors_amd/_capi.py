@@ -32,6 +32,8 @@ SIGNATURES = {
     "tg_csr_default_blocks": (i32, [i32, i64, i32]),
     "tg_csr_spmm": (C.c_int, [i32, i32, i64, vp, C.c_int, vp, vp, C.c_int, dbl, vp, i64, dbl, vp, i64,
                               i32, i32, vp, i32, vp]),
+    "tg_tune_set": (C.c_int, [C.c_char_p, i32]),
+    "tg_tune_get": (i32, [C.c_char_p]),
     "tg_axpby": (C.c_int, [i32, i32, C.c_int, dbl, vp, i64, dbl, vp, i64, vp]),
     "tg_cg_workspace_bytes": (C.c_size_t, [i32, i32, C.c_int, i32]),
     "tg_cg_solve": (C.c_int, [i32, i64, vp, C.c_int, vp, vp, C.c_int, i32, vp, i32, i32, dbl, vp, i64, dbl, vp,

@@ -22,6 +22,8 @@
 // device partition (job_ptr) that balances nonzeros + rows per block.
 
 #include <algorithm>
+#include <cstdlib>
+#include <string>
 
 #include "common.hpp"
 #include "krylov.hpp"
@@ -142,6 +144,72 @@ __global__ __launch_bounds__(kBlock) void spmv_team_kernel(
             T zz[1] = {z};
             epilogue<T, 1>(y + (int64_t)row * ldy, zz, le == lb, alpha, alpha_one, beta, beta_mode);
         }
+    }
+}
+
+// nrhs == 1, streaming form: the workgroup walks its nonzeros in chunks of
+// kBlock * ELEMS.  Every lane loads ELEMS (column, value) pairs of the chunk
+// with coalesced loads and gathers the x entries -- all independent, so a wave
+// keeps 2*ELEMS loads in flight instead of one dependent rp -> ci -> x chain
+// per row -- and leaves the products in LDS.  Teams of TEAM lanes then sum each
+// row's segment out of LDS (strided, butterfly); rows that span chunks
+// accumulate in LDS.  Rows are taken RMAX at a time.
+template <typename T, typename RP, int ELEMS, int TEAM>
+__global__ __launch_bounds__(kBlock) void spmv_stream_kernel(
+    int m, const RP* __restrict__ rp, const int32_t* __restrict__ ci, const T* __restrict__ a, T alpha,
+    const T* __restrict__ x, int64_t ldx, T beta, T* __restrict__ y, int64_t ldy, int alpha_one, int beta_mode,
+    const int32_t* __restrict__ job_ptr, int rows_per_block, const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    constexpr int CAP = kBlock * ELEMS;
+    constexpr int RMAX = 1024;
+    constexpr int TEAMS = kBlock / TEAM;
+    __shared__ T prod[CAP];
+    __shared__ RP srp[RMAX + 1];
+    __shared__ T racc[RMAX];
+    int r0, r1;
+    block_rows(m, job_ptr, rows_per_block, r0, r1);
+    const int lt = threadIdx.x % TEAM, team = threadIdx.x / TEAM;
+    for (int rs = r0; rs < r1; rs += RMAX) {
+        const int nr = min(RMAX, r1 - rs);
+        for (int i = threadIdx.x; i <= nr; i += kBlock) srp[i] = rp[rs + i];
+        for (int i = threadIdx.x; i < nr; i += kBlock) racc[i] = T(0);
+        __syncthreads();
+        const RP L0 = srp[0], L1 = srp[nr];
+        for (RP cs = L0; cs < L1; cs += CAP) {
+            const int cnt = (int)((L1 - cs) < (RP)CAP ? (L1 - cs) : (RP)CAP);
+            int32_t c[ELEMS];
+            T av[ELEMS];
+#pragma unroll
+            for (int j = 0; j < ELEMS; ++j) {
+                const int e = threadIdx.x + j * kBlock;
+                c[j] = 0;
+                av[j] = T(0);
+                if (e < cnt) {
+                    c[j] = __builtin_nontemporal_load(ci + cs + e);
+                    av[j] = a ? __builtin_nontemporal_load(a + cs + e) : T(1);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < ELEMS; ++j) {
+                const int e = threadIdx.x + j * kBlock;
+                if (e < cnt) prod[e] = av[j] * x[(int64_t)c[j] * ldx];
+            }
+            __syncthreads();
+            for (int i = team; i < nr; i += TEAMS) {
+                const RP lb = srp[i] > cs ? srp[i] : cs;
+                const RP le = srp[i + 1] < cs + cnt ? srp[i + 1] : cs + cnt;
+                T sum = T(0);
+                for (RP l = lb + lt; l < le; l += TEAM) sum += prod[l - cs];
+                sum = group_sum<TEAM>(sum);
+                if (lt == 0 && le > lb) racc[i] += sum;
+            }
+            __syncthreads();
+        }
+        for (int i = threadIdx.x; i < nr; i += kBlock) {
+            T zz[1] = {racc[i]};
+            epilogue<T, 1>(y + (int64_t)(rs + i) * ldy, zz, srp[i + 1] == srp[i], alpha, alpha_one, beta, beta_mode);
+        }
+        __syncthreads();
     }
 }
 
@@ -303,9 +371,34 @@ static int spmv_team_width(int32_t m, int64_t nnz) {
     return std::min(64, std::max(4, t));
 }
 
+static int stream_rows_per_block(int32_t m, int64_t nnz);
+static int spmv_variant();
+
 static int rows_per_block_for(int32_t m, int64_t nnz, int32_t nrhs) {
-    if (nrhs == 1) return (kBlock / spmv_team_width(m, nnz)) * kRowsPerTeam;
+    if (nrhs == 1)
+        return spmv_variant() == 0 ? stream_rows_per_block(m, nnz) : (kBlock / spmv_team_width(m, nnz)) * kRowsPerTeam;
     return (kBlock / kWave) * kRowsPerTeam;
+}
+
+constexpr int kStreamElems = 8;
+constexpr int kStreamCap = kBlock * kStreamElems;
+
+// nrhs == 1 kernel choice: 0 = streaming (default), 1 = team-per-row.  Set
+// through tg_tune_set("spmv_kernel", v); the environment variable
+// TG_SPMV_KERNEL ("stream" / "team") gives the initial value.
+static int g_spmv_variant = -1;
+static int spmv_variant() {
+    if (g_spmv_variant < 0) {
+        const char* e = std::getenv("TG_SPMV_KERNEL");
+        g_spmv_variant = (e && std::string(e) == "team") ? 1 : 0;
+    }
+    return g_spmv_variant;
+}
+
+static int stream_rows_per_block(int32_t m, int64_t nnz) {
+    const double avg = m > 0 ? (double)nnz / m : 0.0;
+    int r = (int)(kStreamCap / (avg + 1.0));
+    return std::max(1, std::min(1024, r));
 }
 
 template <typename T, typename RP>
@@ -313,6 +406,27 @@ static int launch_spmv(int32_t m, int64_t nnz, const RP* rp, const int32_t* ci, 
                        const T* X, int64_t ldX, T beta, T* Y, int64_t ldY, int alpha_one, int beta_mode,
                        const int32_t* job_ptr, int32_t blocks, const int* skip, hipStream_t st) {
     const int team = spmv_team_width(m, nnz);
+    if (spmv_variant() == 0) {
+        const int rpb = stream_rows_per_block(m, nnz);
+        const int grid = job_ptr ? blocks : (m + rpb - 1) / rpb;
+        if (grid <= 0) return TG_OK;
+        const int steam = std::min(team, 32);
+#define TG_SPMVS(TM)                                                                                          \
+    case TM:                                                                                                  \
+        hipLaunchKernelGGL((spmv_stream_kernel<T, RP, kStreamElems, TM>), dim3(grid), dim3(kBlock), 0, st, m, rp, \
+                           ci, a, alpha, X, ldX, beta, Y, ldY, alpha_one, beta_mode, job_ptr, rpb, skip);      \
+        break;
+        switch (steam) {
+            TG_SPMVS(4)
+            TG_SPMVS(8)
+            TG_SPMVS(16)
+            TG_SPMVS(32)
+            default: return fail(TG_ERR_INVALID, "internal: bad team width %d", steam);
+        }
+#undef TG_SPMVS
+        TG_LAUNCH_CHECK();
+        return TG_OK;
+    }
     const int rpb = (kBlock / team) * kRowsPerTeam;
     const int grid = job_ptr ? blocks : (m + rpb - 1) / rpb;
     if (grid <= 0) return TG_OK;
@@ -419,6 +533,21 @@ static int axpby_typed(int32_t m, int32_t nrhs, double alpha_d, const T* X, int6
 }  // namespace tg
 
 using namespace tg;
+
+extern "C" int tg_tune_set(const char* key, int32_t value) {
+    if (!key) return fail(TG_ERR_INVALID, "tg_tune_set: null key");
+    if (std::string(key) == "spmv_kernel") {
+        if (value != 0 && value != 1) return fail(TG_ERR_INVALID, "tg_tune_set: spmv_kernel must be 0 or 1");
+        g_spmv_variant = value;
+        return TG_OK;
+    }
+    return fail(TG_ERR_INVALID, "tg_tune_set: unknown key '%s'", key);
+}
+
+extern "C" int32_t tg_tune_get(const char* key) {
+    if (key && std::string(key) == "spmv_kernel") return spmv_variant();
+    return -1;
+}
 
 extern "C" int32_t tg_csr_default_blocks(int32_t m, int64_t nnz, int32_t nrhs) {
     if (m <= 0) return 0;

@@ -234,3 +234,32 @@ def test_matrixcsr_dot_overloads(cuda):
     assert rel_err(Y.cpu().numpy(), 2.0 * dense_of(m, n, rp, ci, val) @ X.cpu().numpy()) <= 1e-12
     Ybad = torch.full((m + 1, 5), 3.0, dtype=torch.float64, device=cuda)
     assert A.Dot(1.0, X, 0.0, Ybad) == capi.ERR_SHAPE and torch.all(Ybad == 3.0)
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_spmv_kernel_variants(cuda, variant, dtype):
+    """Both nrhs == 1 kernels (streaming and team-per-row) against the oracle, including
+    rows longer than one streaming chunk and blocks of more than 1024 rows."""
+    L = capi.lib()
+    old = L.tg_tune_get(b"spmv_kernel")
+    try:
+        assert L.tg_tune_set(b"spmv_kernel", variant) == 0
+        # long rows (up to 5000 nonzeros) next to many empty / short rows
+        rng = np.random.default_rng(11)
+        m, n = 6000, 7000
+        lens = rng.integers(0, 4, m)
+        lens[rng.choice(m, 5, replace=False)] = rng.integers(2500, 5000, 5)
+        rp = np.r_[0, np.cumsum(lens)].astype(np.int64)
+        ci = np.concatenate([np.sort(rng.choice(n, L_, replace=False)) for L_ in lens]).astype(np.int32)
+        val = rng.normal(size=len(ci)).astype(dtype)
+        X = rng.uniform(-1, 1, (n, 1)).astype(dtype)
+        Y0 = rng.uniform(-1, 1, (m, 1)).astype(dtype)
+        for alpha, beta in ((1.0, 0.0), (-2.0, 1.0), (0.5, np.pi)):
+            st, Y = run_gpu(cuda, rp, ci, val, m, n, alpha, X, beta, Y0, 1)
+            Yo = Y0.copy()
+            oracle.spmm(rp, ci, val, m, n, alpha, X, 1, beta, Yo, 1, 1)
+            tol = 1e-12 if dtype == np.float64 else 1e-5
+            assert st == 0 and rel_err(Y, Yo) <= tol
+    finally:
+        L.tg_tune_set(b"spmv_kernel", old)
