@@ -236,6 +236,38 @@ int tg_csr_spgemm(int32_t m, int32_t k, int32_t n, const void* Arp, const int32_
                   int64_t expanded, void* Crp, int32_t* Cci, void* Cval, void* workspace, size_t workspace_bytes,
                   int64_t* nnz_out, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Batched, continuously fed CG (no single reference counterpart: it runs many
+ * ConjugateGradient::operator() solves, ConjugateGradient.hpp:105-353, at
+ * once).  Each system: n x n CSR (int32 rp/ci, FP64 values, device), optional
+ * Jacobi diagonal `diag` (z = diag * r), nrhs in 1..8, B and X n x nrhs
+ * row-major device blocks (ld = nrhs), X <- A^{-1} B.
+ *   mixed == 0: FP64 CG with the reference stopping rule |r.z| <= tol^2 |b.Pb|.
+ *   mixed != 0: FP32 inner CG sweeps + FP64 refinement until the true
+ *               ||B - A X|| / ||B|| <= tol per column (max_iter bounds each
+ *               inner solve).
+ * All active systems advance together (one launch per CG phase covers every
+ * system's row tiles); a system's arithmetic does not depend on the others, so
+ * its result is bitwise the same alone and in any mix.  Systems may be added
+ * between steps.  The caller's arrays must stay valid until the system has
+ * finished; tg_batch_release frees the system's workspace.  Calls synchronise
+ * the batch's stream.
+ * ------------------------------------------------------------------------ */
+typedef struct tg_batch_s* tg_batch_t;
+int tg_batch_create(tg_batch_t* out, void* stream);
+int tg_batch_destroy(tg_batch_t batch);
+int tg_batch_add_cg(tg_batch_t batch, int32_t n, int64_t nnz, const int32_t* rp, const int32_t* ci,
+                    const double* val, const double* diag, int32_t nrhs, const double* B, double* X, double tol,
+                    int32_t max_iter, int32_t mixed, int64_t* id_out);
+/* Advances every active system by up to `iterations` CG iterations. */
+int tg_batch_step(tg_batch_t batch, int32_t iterations);
+/* Ids of systems finished since the last call (at most cap; *count = number). */
+int tg_batch_finished(tg_batch_t batch, int64_t* ids, int32_t cap, int32_t* count);
+int tg_batch_result(tg_batch_t batch, int64_t id, int32_t* finished, int32_t* succeeded, int32_t* iterations,
+                    int32_t* outer, double* rel_residuals);
+int tg_batch_release(tg_batch_t batch, int64_t id);
+int tg_batch_active(tg_batch_t batch, int32_t* systems, int64_t* bytes_in_use);
+
 /* Streaming probes (no reference counterpart): dst <- src over `bytes`
  * (multiple of 16), and a read-only pass over src (sink: one device float that
  * is practically never written).  They measure the achievable HBM bandwidth

@@ -57,6 +57,14 @@ SIGNATURES = {
     "tg_cg_mixed_workspace_bytes": (C.c_size_t, [i32, i64, i32, i32, i32]),
     "tg_cg_solve_mixed": (C.c_int, [i32, i64, vp, C.c_int, vp, vp, vp, i32, vp, i32, i32, i32, dbl, dbl, vp, i64,
                                     dbl, vp, i64, dbl, i32, vp, C.c_size_t, vp, vp, vp, vp, vp]),
+    "tg_batch_create": (C.c_int, [vp, vp]),
+    "tg_batch_destroy": (C.c_int, [vp]),
+    "tg_batch_add_cg": (C.c_int, [vp, i32, i64, vp, vp, vp, vp, i32, vp, vp, dbl, i32, i32, vp]),
+    "tg_batch_step": (C.c_int, [vp, i32]),
+    "tg_batch_finished": (C.c_int, [vp, vp, i32, vp]),
+    "tg_batch_result": (C.c_int, [vp, i64, vp, vp, vp, vp, vp]),
+    "tg_batch_release": (C.c_int, [vp, i64]),
+    "tg_batch_active": (C.c_int, [vp, vp, vp]),
     "tg_probe_copy": (C.c_int, [vp, vp, i64, vp]),
     "tg_probe_read": (C.c_int, [vp, i64, vp, vp]),
     "tg_gen_uniform": (C.c_int, [u64, u64, i64, C.c_int, vp]),

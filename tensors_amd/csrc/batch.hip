@@ -1,0 +1,814 @@
+// Batched, continuously fed CG engine: many SPD systems of uneven size with
+// 1..8 right-hand sides each, advanced together.
+//
+// Every system runs the recurrences of ConjugateGradient::operator()
+// (ConjugateGradient.hpp:105-353) with its own per-column scalars, Jacobi or
+// identity preconditioner, tolerance and iteration limit.  The GPU work of one
+// iteration for ALL active systems is seven launches:
+//   spmm+dot(p,u)  |  alpha (per system)  |  x,r,z update + dot(r,z)  |
+//   rho/beta/convergence (per system)  |  p update
+// Each launch covers the row tiles (kTileRows rows of one system) of every
+// active system, so small and large systems share the machine and a system
+// retires as soon as it converges (its tiles drop out of the next tile list).
+// The host enqueues iterations in batches and polls the per-system flags in
+// between; systems can be added at any poll (continuous batching).
+//
+// Determinism: a system's reductions go through tile partials that depend only
+// on the system itself (its n and k), then a fixed-order per-system sum, so a
+// system's result is bitwise the same alone and in any mix.
+//
+// Mixed systems: FP32 inner CG sweeps in this engine, inside an FP64 refinement
+// loop (r = b - A64 x, true-residual stopping test, x += d) run per system by
+// the host between polls.
+
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "krylov.hpp"
+
+namespace tg {
+
+constexpr int kBatchMaxK = 8;
+constexpr int kTileRows = 2048;
+
+template <typename T> struct SysDev {
+    const int32_t* rp;
+    const int32_t* ci;
+    const T* val;
+    const T* d;  // Jacobi diagonal or null
+    const T* b;
+    T* x;
+    T* r;
+    T* u;
+    T* p;
+    T* z;            // == r without a preconditioner
+    double* partial; // tiles * k
+    int n, k, kpad, tiles;
+};
+
+struct SysState {
+    double rho[kBatchMaxK], rho_old[kBatchMaxK], alpha[kBatchMaxK], beta[kBatchMaxK], TOL[kBatchMaxK],
+        bsq[kBatchMaxK];
+    int iter, done, succ, max_iter, nan, pad[3];
+};
+
+struct Tile {
+    int slot, r0, r1, t;
+};
+
+// ---- tile kernels --------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void tile_partial(double v, int k, int kpad, double* partial, int t) {
+    __shared__ double sh[kBlock];
+    const int c = threadIdx.x % kpad, ro = threadIdx.x / kpad, rows = kBlock / kpad;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = rows / 2; s > 0; s >>= 1) {
+        if (ro < s) sh[threadIdx.x] += sh[threadIdx.x + s * kpad];
+        __syncthreads();
+    }
+    if (ro == 0 && c < k) partial[(int64_t)t * k + c] = sh[c];
+}
+
+// mode 0: r = b, z = P r, x = 0, partial r.z
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bk_init_kernel(const Tile* __restrict__ tiles, const SysDev<T>* __restrict__ sys,
+                                                         const SysState* __restrict__ st) {
+    const Tile tl = tiles[blockIdx.x];
+    const SysDev<T> S = sys[tl.slot];
+    const int c = threadIdx.x % S.kpad, ro = threadIdx.x / S.kpad, rows = kBlock / S.kpad;
+    double acc = 0.0;
+    if (c < S.k)
+        for (int i = tl.r0 + ro; i < tl.r1; i += rows) {
+            const int64_t e = (int64_t)i * S.k + c;
+            const T rv = S.b[e];
+            S.r[e] = rv;
+            T zv = rv;
+            if (S.d) {
+                zv = S.d[i] * rv;
+                S.z[e] = zv;
+            }
+            S.x[e] = T(0);
+            acc += (double)rv * (double)zv;
+        }
+    tile_partial<T>(acc, S.k, S.kpad, S.partial, tl.t);
+}
+
+// u = A p (rows of the tile), partial p.u
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bk_spmm_kernel(const Tile* __restrict__ tiles, const SysDev<T>* __restrict__ sys,
+                                                         const SysState* __restrict__ st) {
+    const Tile tl = tiles[blockIdx.x];
+    if (st[tl.slot].done) return;
+    const SysDev<T> S = sys[tl.slot];
+    const int c = threadIdx.x % S.kpad, ro = threadIdx.x / S.kpad, rows = kBlock / S.kpad;
+    double acc = 0.0;
+    if (c < S.k)
+        for (int i = tl.r0 + ro; i < tl.r1; i += rows) {
+            T s = T(0);
+            for (int l = S.rp[i]; l < S.rp[i + 1]; ++l) s += S.val[l] * S.p[(int64_t)S.ci[l] * S.k + c];
+            const int64_t e = (int64_t)i * S.k + c;
+            S.u[e] = s;
+            acc += (double)S.p[e] * (double)s;
+        }
+    tile_partial<T>(acc, S.k, S.kpad, S.partial, tl.t);
+}
+
+// x = (first ? 0 : x) + alpha p; r -= alpha u; z = P r; partial r.z
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bk_update_kernel(const Tile* __restrict__ tiles, const SysDev<T>* __restrict__ sys,
+                                                           const SysState* __restrict__ st) {
+    const Tile tl = tiles[blockIdx.x];
+    const SysState& ss = st[tl.slot];
+    if (ss.done) return;
+    const SysDev<T> S = sys[tl.slot];
+    const int c = threadIdx.x % S.kpad, ro = threadIdx.x / S.kpad, rows = kBlock / S.kpad;
+    const bool first = ss.iter == 0;
+    double acc = 0.0;
+    if (c < S.k) {
+        const T al = (T)ss.alpha[c];
+        for (int i = tl.r0 + ro; i < tl.r1; i += rows) {
+            const int64_t e = (int64_t)i * S.k + c;
+            const T pv = S.p[e];
+            S.x[e] = first ? al * pv : S.x[e] + al * pv;
+            const T rv = S.r[e] - al * S.u[e];
+            S.r[e] = rv;
+            T zv = rv;
+            if (S.d) {
+                zv = S.d[i] * rv;
+                S.z[e] = zv;
+            }
+            acc += (double)rv * (double)zv;
+        }
+    }
+    tile_partial<T>(acc, S.k, S.kpad, S.partial, tl.t);
+}
+
+// p = z + beta p; with beta_zero p = z (the initial p = z)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bk_p_kernel(const Tile* __restrict__ tiles, const SysDev<T>* __restrict__ sys,
+                                                      const SysState* __restrict__ st, int initial) {
+    const Tile tl = tiles[blockIdx.x];
+    const SysState& ss = st[tl.slot];
+    if (!initial && ss.done) return;
+    const SysDev<T> S = sys[tl.slot];
+    const int c = threadIdx.x % S.kpad, ro = threadIdx.x / S.kpad, rows = kBlock / S.kpad;
+    if (c >= S.k) return;
+    const T be = initial ? T(0) : (T)ss.beta[c];
+    for (int i = tl.r0 + ro; i < tl.r1; i += rows) {
+        const int64_t e = (int64_t)i * S.k + c;
+        S.p[e] = initial ? S.z[e] : S.z[e] + be * S.p[e];
+    }
+}
+
+// ---- per-system finalisation (one workgroup per listed system) ------------------
+__device__ __forceinline__ double sys_total(const double* partial, int tiles, int k, int kpad, int c, int j, int J) {
+    __shared__ double sh[kBlock];
+    double s = 0.0;
+    if (c < k)
+        for (int t = j; t < tiles; t += J) s += partial[(int64_t)t * k + c];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = J / 2; h > 0; h >>= 1) {
+        if (j < h) sh[threadIdx.x] += sh[threadIdx.x + h * kpad];
+        __syncthreads();
+    }
+    const double v = sh[c];
+    __syncthreads();
+    return v;
+}
+
+// mode 0: setup (rho, bsq, TOL, NaN / zero handling, initial convergence)
+// mode 1: alpha = rho / (p.u)
+// mode 2: rho/beta update, convergence, ++iter
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bk_finalize_kernel(const int* __restrict__ slots,
+                                                             const SysDev<T>* __restrict__ sys,
+                                                             SysState* __restrict__ st, int mode,
+                                                             const double* __restrict__ tol2) {
+    const int slot = slots[blockIdx.x];
+    SysState& ss = st[slot];
+    if (mode != 0 && ss.done) return;
+    const SysDev<T> S = sys[slot];
+    const int c = threadIdx.x % S.kpad, j = threadIdx.x / S.kpad, J = kBlock / S.kpad;
+    const double tot = sys_total(S.partial, S.tiles, S.k, S.kpad, c, j, J);
+    __shared__ int flag, nanf;
+    if (threadIdx.x == 0) {
+        flag = 1;
+        nanf = 0;
+    }
+    __syncthreads();
+    if (j == 0 && c < S.k) {
+        if (mode == 0) {
+            ss.rho[c] = tot;
+            ss.bsq[c] = fabs(tot);
+            ss.TOL[c] = fabs(tot) * tol2[slot];
+            if (isnan(tot)) atomicOr(&nanf, 1);
+            if (!(fabs(tot) <= ss.TOL[c])) atomicAnd(&flag, 0);
+        } else if (mode == 1) {
+            ss.alpha[c] = ss.rho[c] / tot;
+        } else {
+            const double old = ss.rho[c];
+            ss.rho_old[c] = old;
+            ss.rho[c] = tot;
+            ss.beta[c] = tot / old;
+            if (!(fabs(tot) <= ss.TOL[c])) atomicAnd(&flag, 0);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (mode == 0) {
+            ss.iter = 0;
+            ss.nan = nanf;
+            ss.succ = nanf ? 0 : flag;
+            ss.done = (nanf || flag || ss.max_iter <= 0) ? 1 : 0;
+        } else if (mode == 2) {
+            ss.iter = ss.iter + 1;
+            ss.succ = flag;
+            ss.done = (flag || ss.iter >= ss.max_iter) ? 1 : 0;
+        }
+    }
+}
+
+// ---- host engine ------------------------------------------------------------------
+static int ceil_div(int a, int b) { return (a + b - 1) / b; }
+static int pow2_at_least(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// One engine per vector type.  Slots are fixed per system while it is active.
+template <typename T> class CGBatch {
+  public:
+    explicit CGBatch(hipStream_t s) : s_(s) {}
+    ~CGBatch() { release(); }
+
+    // Adds a system; its setup runs at once.  b, x: n x k device blocks (ld = k).
+    int add(int n, const int32_t* rp, const int32_t* ci, const T* val, const T* d, int k, const T* b, T* x,
+            double tol, int max_iter, int* slot_out) {
+        if (n < 1 || k < 1 || k > kBatchMaxK) return fail(TG_ERR_INVALID, "batch: n >= 1 and 1 <= nrhs <= 8");
+        int slot = -1;
+        for (int i = 0; i < (int)used_.size(); ++i)
+            if (!used_[i]) {
+                slot = i;
+                break;
+            }
+        if (slot < 0) {
+            slot = (int)used_.size();
+            int rc = grow(slot + 1);
+            if (rc) return rc;
+        }
+        SysDev<T> S{};
+        S.rp = rp;
+        S.ci = ci;
+        S.val = val;
+        S.d = d;
+        S.b = b;
+        S.x = x;
+        S.n = n;
+        S.k = k;
+        S.kpad = pow2_at_least(k);
+        S.tiles = ceil_div(n, kTileRows);
+        const size_t N = (size_t)n * k;
+        const size_t nvec = d ? 4 : 3;
+        const size_t bytes = nvec * align256(N * sizeof(T)) + align256((size_t)S.tiles * k * sizeof(double));
+        char* mem = nullptr;
+        TG_HIP(hipMallocAsync((void**)&mem, bytes, s_));
+        S.r = (T*)mem;
+        S.u = (T*)(mem + align256(N * sizeof(T)));
+        S.p = (T*)(mem + 2 * align256(N * sizeof(T)));
+        S.z = d ? (T*)(mem + 3 * align256(N * sizeof(T))) : S.r;
+        S.partial = (double*)(mem + nvec * align256(N * sizeof(T)));
+        mem_[slot] = mem;
+        bytes_[slot] = bytes;
+        used_[slot] = 1;
+        parked_[slot] = 0;  // a reused slot starts iterating again
+        host_[slot] = S;
+        SysState z{};
+        z.max_iter = std::max(0, max_iter);
+        const double t2 = tol * tol;
+        TG_HIP(hipMemcpyAsync(dsys_ + slot, &S, sizeof(S), hipMemcpyHostToDevice, s_));
+        TG_HIP(hipMemcpyAsync(dst_ + slot, &z, sizeof(z), hipMemcpyHostToDevice, s_));
+        TG_HIP(hipMemcpyAsync(dtol2_ + slot, &t2, sizeof(double), hipMemcpyHostToDevice, s_));
+        // setup for this system alone
+        std::vector<Tile> tl;
+        tiles_of(slot, tl);
+        TG_HIP(hipMemcpyAsync(dsetup_tiles_, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice, s_));
+        TG_HIP(hipMemcpyAsync(dsetup_slot_, &slot, sizeof(int), hipMemcpyHostToDevice, s_));
+        hipLaunchKernelGGL((bk_init_kernel<T>), dim3((unsigned)tl.size()), dim3(kBlock), 0, s_, dsetup_tiles_, dsys_,
+                           dst_);
+        hipLaunchKernelGGL((bk_finalize_kernel<T>), dim3(1), dim3(kBlock), 0, s_, dsetup_slot_, dsys_, dst_, 0,
+                           (const double*)dtol2_);
+        hipLaunchKernelGGL((bk_p_kernel<T>), dim3((unsigned)tl.size()), dim3(kBlock), 0, s_, dsetup_tiles_, dsys_,
+                           dst_, 1);
+        TG_LAUNCH_CHECK();
+        // the setup buffers are reused by the next add: finish this one first
+        TG_HIP(hipStreamSynchronize(s_));
+        dirty_ = true;
+        *slot_out = slot;
+        return TG_OK;
+    }
+
+    // Enqueues `iters` iterations for every active (not done) system, then polls.
+    int step(int iters) {
+        if (dirty_) {
+            int rc = rebuild();
+            if (rc) return rc;
+        }
+        if (ntiles_ == 0) return TG_OK;
+        for (int it = 0; it < iters; ++it) {
+            hipLaunchKernelGGL((bk_spmm_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s_, dtiles_, dsys_, dst_);
+            hipLaunchKernelGGL((bk_finalize_kernel<T>), dim3(nslots_), dim3(kBlock), 0, s_, dslots_, dsys_, dst_, 1,
+                               (const double*)dtol2_);
+            hipLaunchKernelGGL((bk_update_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s_, dtiles_, dsys_, dst_);
+            hipLaunchKernelGGL((bk_finalize_kernel<T>), dim3(nslots_), dim3(kBlock), 0, s_, dslots_, dsys_, dst_, 2,
+                               (const double*)dtol2_);
+            hipLaunchKernelGGL((bk_p_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s_, dtiles_, dsys_, dst_, 0);
+        }
+        TG_LAUNCH_CHECK();
+        return TG_OK;
+    }
+
+    // Reads the states of all used slots (synchronises).
+    int poll(std::vector<SysState>& out) {
+        out.resize(used_.size());
+        if (!used_.empty())
+            TG_HIP(hipMemcpyAsync(out.data(), dst_, used_.size() * sizeof(SysState), hipMemcpyDeviceToHost, s_));
+        TG_HIP(hipStreamSynchronize(s_));
+        return TG_OK;
+    }
+
+    // Frees a slot (after its result has been consumed).
+    int retire(int slot) {
+        if (slot < 0 || slot >= (int)used_.size() || !used_[slot]) return fail(TG_ERR_INVALID, "batch: bad slot");
+        TG_HIP(hipFreeAsync(mem_[slot], s_));
+        mem_[slot] = nullptr;
+        bytes_[slot] = 0;
+        used_[slot] = 0;
+        dirty_ = true;
+        return TG_OK;
+    }
+
+    // Marks a slot as no longer iterating (its tiles leave the list).
+    void park(int slot) {
+        if (slot >= 0 && slot < (int)parked_.size()) parked_[slot] = 1;
+        dirty_ = true;
+    }
+    void unpark(int slot) {
+        if (slot >= 0 && slot < (int)parked_.size()) parked_[slot] = 0;
+        dirty_ = true;
+    }
+
+    size_t bytes_in_use() const {
+        size_t s = 0;
+        for (size_t b : bytes_) s += b;
+        return s;
+    }
+    int active_tiles() const { return ntiles_; }
+    const SysDev<T>& desc(int slot) const { return host_[slot]; }
+
+    void mark_done_host(int slot) {
+        if (slot >= 0 && slot < (int)parked_.size()) parked_[slot] = 1;
+        dirty_ = true;
+    }
+
+  private:
+    void tiles_of(int slot, std::vector<Tile>& tl) const {
+        const SysDev<T>& S = host_[slot];
+        for (int t = 0; t < S.tiles; ++t) tl.push_back(Tile{slot, t * kTileRows, std::min(S.n, (t + 1) * kTileRows), t});
+    }
+
+    int grow(int need) {
+        int cap = std::max(need, 2 * (int)cap_);
+        if (cap <= (int)cap_) return TG_OK;
+        SysDev<T>* nsys = nullptr;
+        SysState* nst = nullptr;
+        double* ntol = nullptr;
+        TG_HIP(hipMallocAsync((void**)&nsys, cap * sizeof(SysDev<T>), s_));
+        TG_HIP(hipMallocAsync((void**)&nst, cap * sizeof(SysState), s_));
+        TG_HIP(hipMallocAsync((void**)&ntol, cap * sizeof(double), s_));
+        if (cap_) {
+            TG_HIP(hipMemcpyAsync(nsys, dsys_, cap_ * sizeof(SysDev<T>), hipMemcpyDeviceToDevice, s_));
+            TG_HIP(hipMemcpyAsync(nst, dst_, cap_ * sizeof(SysState), hipMemcpyDeviceToDevice, s_));
+            TG_HIP(hipMemcpyAsync(ntol, dtol2_, cap_ * sizeof(double), hipMemcpyDeviceToDevice, s_));
+            TG_HIP(hipFreeAsync(dsys_, s_));
+            TG_HIP(hipFreeAsync(dst_, s_));
+            TG_HIP(hipFreeAsync(dtol2_, s_));
+        }
+        dsys_ = nsys;
+        dst_ = nst;
+        dtol2_ = ntol;
+        cap_ = cap;
+        used_.resize(cap, 0);
+        parked_.resize(cap, 0);
+        mem_.resize(cap, nullptr);
+        bytes_.resize(cap, 0);
+        host_.resize(cap);
+        if (!dsetup_tiles_) {
+            TG_HIP(hipMallocAsync((void**)&dsetup_tiles_, sizeof(Tile) * ((size_t)INT32_MAX / kTileRows + 1), s_));
+            TG_HIP(hipMallocAsync((void**)&dsetup_slot_, sizeof(int), s_));
+        }
+        return TG_OK;
+    }
+
+    // Tile list of every used, not parked, not done system (host reads the flags).
+    int rebuild() {
+        std::vector<SysState> states;
+        int rc = poll(states);
+        if (rc) return rc;
+        std::vector<Tile> tl;
+        std::vector<int> sl;
+        for (int i = 0; i < (int)used_.size(); ++i)
+            if (used_[i] && !parked_[i] && !states[i].done) {
+                tiles_of(i, tl);
+                sl.push_back(i);
+            }
+        if (tl.size() > tiles_cap_) {
+            if (dtiles_) TG_HIP(hipFreeAsync(dtiles_, s_));
+            tiles_cap_ = std::max(tl.size(), 2 * tiles_cap_);
+            TG_HIP(hipMallocAsync((void**)&dtiles_, tiles_cap_ * sizeof(Tile), s_));
+        }
+        if (sl.size() > slots_cap_) {
+            if (dslots_) TG_HIP(hipFreeAsync(dslots_, s_));
+            slots_cap_ = std::max(sl.size(), 2 * slots_cap_);
+            TG_HIP(hipMallocAsync((void**)&dslots_, slots_cap_ * sizeof(int), s_));
+        }
+        if (!tl.empty()) TG_HIP(hipMemcpyAsync(dtiles_, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice, s_));
+        if (!sl.empty()) TG_HIP(hipMemcpyAsync(dslots_, sl.data(), sl.size() * sizeof(int), hipMemcpyHostToDevice, s_));
+        TG_HIP(hipStreamSynchronize(s_));  // host vectors go out of scope
+        ntiles_ = (int)tl.size();
+        nslots_ = (int)sl.size();
+        dirty_ = false;
+        return TG_OK;
+    }
+
+    void release() {
+        for (auto* m : mem_)
+            if (m) (void)hipFreeAsync(m, s_);
+        if (dsys_) (void)hipFreeAsync(dsys_, s_);
+        if (dst_) (void)hipFreeAsync(dst_, s_);
+        if (dtol2_) (void)hipFreeAsync(dtol2_, s_);
+        if (dtiles_) (void)hipFreeAsync(dtiles_, s_);
+        if (dslots_) (void)hipFreeAsync(dslots_, s_);
+        if (dsetup_tiles_) (void)hipFreeAsync(dsetup_tiles_, s_);
+        if (dsetup_slot_) (void)hipFreeAsync(dsetup_slot_, s_);
+        (void)hipStreamSynchronize(s_);
+    }
+
+    hipStream_t s_;
+    size_t cap_ = 0;
+    std::vector<char> used_, parked_;
+    std::vector<char*> mem_;
+    std::vector<size_t> bytes_;
+    std::vector<SysDev<T>> host_;
+    SysDev<T>* dsys_ = nullptr;
+    SysState* dst_ = nullptr;
+    double* dtol2_ = nullptr;
+    Tile* dtiles_ = nullptr;
+    int* dslots_ = nullptr;
+    size_t tiles_cap_ = 0, slots_cap_ = 0;
+    Tile* dsetup_tiles_ = nullptr;
+    int* dsetup_slot_ = nullptr;
+    int ntiles_ = 0, nslots_ = 0;
+    bool dirty_ = true;
+};
+
+// ---- FP64 refinement kernels for mixed systems (one system per launch) -----------
+// r32 = float(b - A x) rows; partial ||b - A x||^2 per tile (tiles of kTileRows)
+__global__ __launch_bounds__(kBlock) void bm_residual_kernel(int n, int k, int kpad, const int32_t* __restrict__ rp,
+                                                             const int32_t* __restrict__ ci,
+                                                             const double* __restrict__ val,
+                                                             const double* __restrict__ b, const double* __restrict__ x,
+                                                             float* __restrict__ r32, double* __restrict__ partial,
+                                                             int first) {
+    const int r0 = blockIdx.x * kTileRows, r1 = min(n, r0 + kTileRows);
+    const int c = threadIdx.x % kpad, ro = threadIdx.x / kpad, rows = kBlock / kpad;
+    double acc = 0.0;
+    if (c < k)
+        for (int i = r0 + ro; i < r1; i += rows) {
+            double ax = 0.0;
+            if (!first)
+                for (int l = rp[i]; l < rp[i + 1]; ++l) ax += val[l] * x[(int64_t)ci[l] * k + c];
+            const int64_t e = (int64_t)i * k + c;
+            const double r = b[e] - ax;
+            r32[e] = (float)r;
+            acc += r * r;
+        }
+    tile_partial<double>(acc, k, kpad, partial, blockIdx.x);
+}
+
+__global__ __launch_bounds__(kBlock) void bm_norm_total_kernel(int tiles, int k, int kpad, const double* __restrict__ partial,
+                                                               double* __restrict__ out) {
+    const int c = threadIdx.x % kpad, j = threadIdx.x / kpad, J = kBlock / kpad;
+    const double t = sys_total(partial, tiles, k, kpad, c, j, J);
+    if (j == 0 && c < k) out[c] = sqrt(t);
+}
+
+__global__ __launch_bounds__(kBlock) void bm_axpy_kernel(int64_t count, const float* __restrict__ d,
+                                                         double* __restrict__ x) {
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < count; e += (int64_t)gridDim.x * kBlock)
+        x[e] += (double)d[e];
+}
+
+__global__ __launch_bounds__(kBlock) void bm_zero_kernel(int64_t count, double* __restrict__ x) {
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < count; e += (int64_t)gridDim.x * kBlock)
+        x[e] = 0.0;
+}
+
+__global__ __launch_bounds__(kBlock) void bm_tofloat_kernel(int64_t count, const double* __restrict__ a,
+                                                            float* __restrict__ b) {
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < count; e += (int64_t)gridDim.x * kBlock)
+        b[e] = (float)a[e];
+}
+
+// ---- the batch object behind the C ABI ------------------------------------------
+struct BatchSystem {
+    int64_t id;
+    int mixed;
+    int n, k;
+    int64_t nnz;
+    const int32_t* rp;
+    const int32_t* ci;
+    const double* val64;
+    const double* d64;
+    const double* b64;
+    double* x64;  // caller's X
+    double tol;
+    int max_iter;
+    // plain FP64
+    int slot64 = -1;
+    // mixed
+    int slot32 = -1;
+    int state = 0;  // 0: needs residual, 1: inner running, 2: done
+    int outer = 0, inner_total = 0, max_outer = 30;
+    double inner_tol = 1e-5;
+    char* mem = nullptr;  // a32, d32, r32, dx32, partial, norms
+    size_t mem_bytes = 0;
+    float* a32 = nullptr;
+    float* d32 = nullptr;
+    float* r32 = nullptr;
+    float* dx32 = nullptr;
+    double* partial = nullptr;
+    double* norms = nullptr;  // 2k
+    std::vector<double> bnorm;
+    // results
+    int finished = 0, succeeded = 0, iterations = 0;
+    std::vector<double> rel;
+};
+
+struct Batch {
+    hipStream_t s;
+    CGBatch<double> b64;
+    CGBatch<float> b32;
+    std::map<int64_t, std::unique_ptr<BatchSystem>> sys;
+    std::vector<int64_t> finished_queue;
+    int64_t next_id = 1;
+    explicit Batch(hipStream_t st) : s(st), b64(st), b32(st) {}
+};
+
+static int mixed_start(Batch& B, BatchSystem& S);
+
+static int finish(Batch& B, BatchSystem& S, int succeeded, int iterations) {
+    S.finished = 1;
+    S.succeeded = succeeded;
+    S.iterations = iterations;
+    B.finished_queue.push_back(S.id);
+    return TG_OK;
+}
+
+// Computes the FP64 residual of a mixed system; decides done / next inner solve.
+static int mixed_residual(Batch& B, BatchSystem& S) {
+    const int tiles = ceil_div(S.n, kTileRows), kpad = pow2_at_least(S.k);
+    hipLaunchKernelGGL(bm_residual_kernel, dim3(tiles), dim3(kBlock), 0, B.s, S.n, S.k, kpad, S.rp, S.ci, S.val64, S.b64,
+                       (const double*)S.x64, S.r32, S.partial, S.outer == 0 ? 1 : 0);
+    hipLaunchKernelGGL(bm_norm_total_kernel, dim3(1), dim3(kBlock), 0, B.s, tiles, S.k, kpad, (const double*)S.partial,
+                       S.norms + S.k);
+    TG_LAUNCH_CHECK();
+    std::vector<double> rn(S.k);
+    TG_HIP(hipMemcpyAsync(rn.data(), S.norms + S.k, S.k * 8, hipMemcpyDeviceToHost, B.s));
+    TG_HIP(hipStreamSynchronize(B.s));
+    double worst = 0.0;
+    for (int c = 0; c < S.k; ++c) {
+        S.rel[c] = S.bnorm[c] > 0 ? rn[c] / S.bnorm[c] : 0.0;
+        worst = std::max(worst, S.rel[c]);
+    }
+    if (worst <= S.tol || S.outer >= S.max_outer) {
+        S.state = 2;
+        return finish(B, S, worst <= S.tol ? 1 : 0, S.inner_total);
+    }
+    const double tin = std::min(0.5, std::max(S.inner_tol, 0.3 * S.tol / worst));
+    if (S.slot32 >= 0) {
+        int rc = B.b32.retire(S.slot32);
+        if (rc) return rc;
+        S.slot32 = -1;
+    }
+    int slot = -1;
+    int rc = B.b32.add(S.n, S.rp, S.ci, S.a32, S.d32, S.k, S.r32, S.dx32, tin, S.max_iter, &slot);
+    if (rc) return rc;
+    S.slot32 = slot;
+    S.state = 1;
+    return TG_OK;
+}
+
+static int mixed_start(Batch& B, BatchSystem& S) {
+    const size_t N = (size_t)S.n * S.k;
+    const int tiles = ceil_div(S.n, kTileRows);
+    const size_t bytes = align256((size_t)S.nnz * 4) + (S.d64 ? align256((size_t)S.n * 4) : 0) + 2 * align256(N * 4) +
+                         align256((size_t)tiles * S.k * 8) + align256((size_t)2 * S.k * 8);
+    TG_HIP(hipMallocAsync((void**)&S.mem, bytes, B.s));
+    S.mem_bytes = bytes;
+    char* p = S.mem;
+    S.a32 = (float*)p;
+    p += align256((size_t)S.nnz * 4);
+    if (S.d64) {
+        S.d32 = (float*)p;
+        p += align256((size_t)S.n * 4);
+    }
+    S.r32 = (float*)p;
+    p += align256(N * 4);
+    S.dx32 = (float*)p;
+    p += align256(N * 4);
+    S.partial = (double*)p;
+    p += align256((size_t)tiles * S.k * 8);
+    S.norms = (double*)p;
+    auto grid = [](int64_t c) { return (int)std::max<int64_t>(1, std::min<int64_t>((c + kBlock - 1) / kBlock, 8192)); };
+    hipLaunchKernelGGL(bm_tofloat_kernel, dim3(grid(S.nnz)), dim3(kBlock), 0, B.s, S.nnz, S.val64, S.a32);
+    if (S.d64) hipLaunchKernelGGL(bm_tofloat_kernel, dim3(grid(S.n)), dim3(kBlock), 0, B.s, (int64_t)S.n, S.d64, S.d32);
+    // ||b||: the residual kernel with x = 0 gives it (first pass)
+    const int kpad = pow2_at_least(S.k);
+    hipLaunchKernelGGL(bm_residual_kernel, dim3(tiles), dim3(kBlock), 0, B.s, S.n, S.k, kpad, S.rp, S.ci, S.val64, S.b64,
+                       (const double*)S.x64, S.r32, S.partial, 1);
+    hipLaunchKernelGGL(bm_norm_total_kernel, dim3(1), dim3(kBlock), 0, B.s, tiles, S.k, kpad, (const double*)S.partial,
+                       S.norms);
+    TG_LAUNCH_CHECK();
+    S.bnorm.assign(S.k, 0.0);
+    TG_HIP(hipMemcpyAsync(S.bnorm.data(), S.norms, S.k * 8, hipMemcpyDeviceToHost, B.s));
+    TG_HIP(hipStreamSynchronize(B.s));
+    for (int c = 0; c < S.k; ++c)
+        if (std::isnan(S.bnorm[c])) {
+            S.state = 2;
+            return finish(B, S, 0, 0);
+        }
+    // x = 0 (the caller's X is the FP64 iterate)
+    hipLaunchKernelGGL(bm_zero_kernel, dim3(grid((int64_t)N)), dim3(kBlock), 0, B.s, (int64_t)N, S.x64);
+    TG_LAUNCH_CHECK();
+    S.outer = 0;
+    return mixed_residual(B, S);
+}
+
+static int batch_advance(Batch& B, int iters) {
+    int rc = B.b64.step(iters);
+    if (rc) return rc;
+    rc = B.b32.step(iters);
+    if (rc) return rc;
+    std::vector<SysState> s64, s32;
+    rc = B.b64.poll(s64);
+    if (rc) return rc;
+    rc = B.b32.poll(s32);
+    if (rc) return rc;
+    for (auto& kv : B.sys) {
+        BatchSystem& S = *kv.second;
+        if (S.finished) continue;
+        if (!S.mixed) {
+            const SysState& st = s64[S.slot64];
+            if (st.done) {
+                for (int c = 0; c < S.k; ++c) S.rel[c] = st.bsq[c] > 0 ? std::sqrt(std::fabs(st.rho[c]) / st.bsq[c]) : 0.0;
+                B.b64.park(S.slot64);
+                rc = finish(B, S, st.nan ? 0 : st.succ, st.iter);
+                if (rc) return rc;
+            }
+        } else if (S.state == 1) {
+            const SysState& st = s32[S.slot32];
+            if (st.done) {
+                S.inner_total += st.iter;
+                const int64_t N = (int64_t)S.n * S.k;
+                hipLaunchKernelGGL(bm_axpy_kernel, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((N + 255) / 256, 8192))),
+                                   dim3(kBlock), 0, B.s, N, (const float*)S.dx32, S.x64);
+                TG_LAUNCH_CHECK();
+                B.b32.park(S.slot32);
+                ++S.outer;
+                rc = mixed_residual(B, S);
+                if (rc) return rc;
+            }
+        }
+    }
+    return TG_OK;
+}
+
+}  // namespace tg
+
+using namespace tg;
+
+struct tg_batch_s {
+    Batch* b;
+};
+
+extern "C" int tg_batch_create(tg_batch_t* out, void* stream) {
+    if (!out) return fail(TG_ERR_INVALID, "tg_batch_create: null handle");
+    *out = new tg_batch_s{new Batch(as_stream(stream))};
+    return TG_OK;
+}
+
+extern "C" int tg_batch_destroy(tg_batch_t h) {
+    if (!h) return TG_OK;
+    for (auto& kv : h->b->sys)
+        if (kv.second->mem) (void)hipFreeAsync(kv.second->mem, h->b->s);
+    delete h->b;
+    delete h;
+    return TG_OK;
+}
+
+extern "C" int tg_batch_add_cg(tg_batch_t h, int32_t n, int64_t nnz, const int32_t* rp, const int32_t* ci,
+                               const double* val, const double* diag, int32_t nrhs, const double* B, double* X,
+                               double tol, int32_t max_iter, int32_t mixed, int64_t* id_out) {
+    if (!h || !id_out || n < 1 || nrhs < 1 || nrhs > kBatchMaxK || !rp || !ci || !val || !B || !X)
+        return fail(TG_ERR_INVALID, "tg_batch_add_cg: bad argument (1 <= nrhs <= 8, device pointers)");
+    Batch& Bt = *h->b;
+    auto S = std::make_unique<BatchSystem>();
+    S->id = Bt.next_id++;
+    S->mixed = mixed ? 1 : 0;
+    S->n = n;
+    S->k = nrhs;
+    S->nnz = nnz;
+    S->rp = rp;
+    S->ci = ci;
+    S->val64 = val;
+    S->d64 = diag;
+    S->b64 = B;
+    S->x64 = X;
+    S->tol = tol;
+    S->max_iter = max_iter;
+    S->rel.assign(nrhs, 0.0);
+    BatchSystem& ref = *S;
+    Bt.sys[S->id] = std::move(S);
+    *id_out = ref.id;
+    if (!ref.mixed) return Bt.b64.add(n, rp, ci, val, diag, nrhs, B, X, tol, max_iter, &ref.slot64);
+    return mixed_start(Bt, ref);
+}
+
+extern "C" int tg_batch_step(tg_batch_t h, int32_t iterations) {
+    if (!h || iterations < 1) return fail(TG_ERR_INVALID, "tg_batch_step: bad argument");
+    return batch_advance(*h->b, iterations);
+}
+
+extern "C" int tg_batch_finished(tg_batch_t h, int64_t* ids, int32_t cap, int32_t* count) {
+    if (!h || !count || (cap > 0 && !ids)) return fail(TG_ERR_INVALID, "tg_batch_finished: bad argument");
+    auto& q = h->b->finished_queue;
+    const int32_t c = (int32_t)std::min<size_t>(q.size(), (size_t)std::max(cap, 0));
+    for (int32_t i = 0; i < c; ++i) ids[i] = q[i];
+    q.erase(q.begin(), q.begin() + c);
+    *count = c;
+    return TG_OK;
+}
+
+extern "C" int tg_batch_result(tg_batch_t h, int64_t id, int32_t* finished, int32_t* succeeded, int32_t* iterations,
+                               int32_t* outer, double* rel_residuals) {
+    if (!h) return fail(TG_ERR_INVALID, "tg_batch_result: null handle");
+    auto it = h->b->sys.find(id);
+    if (it == h->b->sys.end()) return fail(TG_ERR_INVALID, "tg_batch_result: unknown id");
+    const BatchSystem& S = *it->second;
+    if (finished) *finished = S.finished;
+    if (succeeded) *succeeded = S.succeeded;
+    if (iterations) *iterations = S.iterations;
+    if (outer) *outer = S.outer;
+    if (rel_residuals)
+        for (int c = 0; c < S.k; ++c) rel_residuals[c] = S.rel[c];
+    return TG_OK;
+}
+
+// Frees the device workspace of a finished system (its X stays with the caller).
+extern "C" int tg_batch_release(tg_batch_t h, int64_t id) {
+    if (!h) return fail(TG_ERR_INVALID, "tg_batch_release: null handle");
+    Batch& Bt = *h->b;
+    auto it = Bt.sys.find(id);
+    if (it == Bt.sys.end()) return fail(TG_ERR_INVALID, "tg_batch_release: unknown id");
+    BatchSystem& S = *it->second;
+    if (!S.finished) return fail(TG_ERR_INVALID, "tg_batch_release: system %lld still running", (long long)id);
+    if (S.slot64 >= 0) {
+        int rc = Bt.b64.retire(S.slot64);
+        if (rc) return rc;
+    }
+    if (S.slot32 >= 0) {
+        int rc = Bt.b32.retire(S.slot32);
+        if (rc) return rc;
+    }
+    if (S.mem) TG_HIP(hipFreeAsync(S.mem, Bt.s));
+    Bt.sys.erase(it);
+    return TG_OK;
+}
+
+extern "C" int tg_batch_active(tg_batch_t h, int32_t* systems, int64_t* bytes_in_use) {
+    if (!h) return fail(TG_ERR_INVALID, "tg_batch_active: null handle");
+    int c = 0;
+    size_t extra = 0;
+    for (auto& kv : h->b->sys) {
+        c += kv.second->finished ? 0 : 1;
+        extra += kv.second->mem_bytes;
+    }
+    if (systems) *systems = c;
+    if (bytes_in_use) *bytes_in_use = (int64_t)(h->b->b64.bytes_in_use() + h->b->b32.bytes_in_use() + extra);
+    return TG_OK;
+}
