@@ -32,6 +32,14 @@ KS = (1, 4, 16, 64)
 SEED = 2002
 
 
+_T0 = time.perf_counter()
+
+
+def log(msg):
+    """Progress line on stderr (stdout carries only the JSON line)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f}s] {msg}", file=sys.stderr, flush=True)
+
+
 def alg_bytes(nnz, m, n, k, sv):
     return nnz * (sv + 4) + 4 * (m + 1) + n * k * sv + m * k * sv
 
@@ -46,6 +54,15 @@ def parse():
     ap.add_argument("--no-cg", action="store_true", help="skip the CG (configs 3) section")
     ap.add_argument("--cg-grid", type=int, default=128)
     ap.add_argument("--cg-reps", type=int, default=2)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven CG section")
+    ap.add_argument("--batch-systems", type=int, default=2000)
+    ap.add_argument("--batch-iters", type=int, default=16, help="CG iterations enqueued per poll")
+    ap.add_argument("--batch-seq-sample", type=int, default=32,
+                    help="systems solved one at a time by the drop-in FP64 CG for comparison")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop section")
+    ap.add_argument("--serve-requests", type=int, default=160)
+    ap.add_argument("--serve-rates", type=str, default="0.25,0.5,1,2",
+                    help="arrival rates as multiples of the measured saturated service rate")
     return ap.parse_args()
 
 
@@ -138,6 +155,7 @@ def cg_section(args, dev):
         ok64 = cg(A, None, 1.0, B, 0.0, X64, tol)
         torch.cuda.synchronize()
         lat64.append(time.perf_counter() - t0)
+        log(f"CG fp64 solve {lat64[-1]:.3f}s")
     it64 = cg.IterationCount()
     res64 = true_res(X64)
     Xm = torch.zeros(n, k, dtype=torch.float64, device=dev)
@@ -151,6 +169,7 @@ def cg_section(args, dev):
         okm, outer, inner, rr = cg_solve_mixed(A, B, Xm, tol=tol, a32=a32)
         torch.cuda.synchronize()
         latm.append(time.perf_counter() - t0)
+        log(f"CG mixed solve {latm[-1]:.3f}s")
     resm = true_res(Xm)
     err = ((Xm - X64).abs().max() / X64.abs().max()).item()
     # algorithmic bytes of one FP64 CG iteration (P = I): SpMM + 11 vector passes
@@ -169,6 +188,261 @@ def cg_section(args, dev):
                   "latencies_ms": [round(x * 1e3, 2) for x in latm], "outer": outer, "inner_iterations": inner,
                   "succeeded": bool(okm), "true_rel_residual": resm, "error_vs_fp64": err},
         "speedup_mixed_over_fp64": round(t64 / tm, 3),
+    }
+
+
+def batch_specs(count, seed=4004):
+    """Uneven SPD systems: n log-uniform in [1e3, 2e5], 2D or 3D weighted grid Laplacian
+    (log-normal(0,1) edge coefficients, diagonal shift 1e-3), 1-8 RHS each."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(count):
+        n_t = float(np.exp(rng.uniform(np.log(1e3), np.log(2e5))))
+        dims = 2 if rng.random() < 0.5 else 3
+        side = max(2, int(round(n_t ** (1.0 / dims))))
+        specs.append((dims, side, int(rng.integers(1, 9))))
+    return specs
+
+
+def batch_section(args, dev):
+    """Batched uneven solves to 1e-10: every system added at t0 to the batched CG engine
+    (Jacobi), FP64 recurrences vs FP32-inner CG with FP64 refinement.  Reports solves/s,
+    p50/p99 completion latency, iterations and the true residuals reached; the drop-in
+    FP64 CG solving a sample of the systems one at a time is timed beside it."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import torch
+
+    from tensors_amd import generators as gen
+    from tensors_amd.batch import BatchCG
+    from tensors_amd.solvers import ConjugateGradient
+    from tensors_amd.sparse import MatrixCSR, spmm_raw
+
+    specs = batch_specs(args.batch_systems)
+    tol = 1e-10
+
+    def make(i):
+        d, side, k = specs[i]
+        return gen.weighted_grid(d, (side,) * d, 1e-3, seed=9000 + i)
+
+    t_gen = time.perf_counter()
+    with ThreadPoolExecutor(8) as ex:
+        host = list(ex.map(make, range(len(specs))))
+    t_gen = time.perf_counter() - t_gen
+    log(f"batch: generated {len(host)} systems in {t_gen:.1f}s")
+    g = torch.Generator(device=dev)
+    g.manual_seed(4004)
+    systems = []
+    for (rp, ci, val), (_, _, k) in zip(host, specs):
+        n = len(rp) - 1
+        rpd = torch.from_numpy(rp.astype(np.int32)).to(dev)
+        cid = torch.from_numpy(ci).to(dev)
+        vd = torch.from_numpy(val).to(dev)
+        rows = torch.repeat_interleave(torch.arange(n, device=dev, dtype=torch.int32), rpd[1:] - rpd[:-1])
+        dg = torch.zeros(n, dtype=torch.float64, device=dev)
+        m = cid == rows
+        dg[rows[m].long()] = vd[m]
+        B = torch.rand(n, k, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+        systems.append({"n": n, "k": k, "nnz": len(ci), "rp": rpd, "ci": cid, "val": vd, "d": 1.0 / dg, "B": B})
+    del host
+    torch.cuda.synchronize()
+    total_rows = sum(s["n"] for s in systems)
+    total_nnz = sum(s["nnz"] for s in systems)
+
+    def true_res(s, X):
+        R = s["B"].clone()
+        spmm_raw(s["n"], s["n"], s["nnz"], s["rp"], s["ci"], s["val"], -1.0, X, s["k"], 1.0, R, s["k"], s["k"])
+        return (R.norm(dim=0) / s["B"].norm(dim=0)).max().item()
+
+    def run(mixed):
+        Xs = [torch.zeros(s["n"], s["k"], dtype=torch.float64, device=dev) for s in systems]
+        torch.cuda.synchronize()
+        bt = BatchCG()
+        t0 = time.perf_counter()
+        ids = {}
+        for i, s in enumerate(systems):
+            ids[bt.add(s["rp"], s["ci"], s["val"], s["B"], Xs[i], tol, 20000, diag=s["d"], mixed=mixed)] = i
+        done_t = {}
+        for sid in bt.finished():
+            done_t[sid] = time.perf_counter() - t0
+        last = 0.0
+        while True:
+            active = bt.active()[0]
+            if active == 0:
+                break
+            bt.step(args.batch_iters)
+            now = time.perf_counter() - t0
+            for sid in bt.finished():
+                done_t[sid] = now
+            if now - last > 10.0:
+                log(f"batch {'mixed' if mixed else 'fp64'}: {active} systems active at {now:.1f}s")
+                last = now
+        wall = time.perf_counter() - t0
+        res = {sid: bt.result(sid) for sid in ids}
+        bt.close()
+        lat = np.array(sorted(done_t.values()))
+        tr = np.array([true_res(systems[i], Xs[i]) for sid, i in ids.items()])
+        iters = np.array([res[sid]["iterations"] for sid in ids])
+        return Xs, {
+            "solves_per_s": round(len(systems) / wall, 2), "wall_s": round(wall, 3),
+            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 1),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 1),
+            "succeeded": int(sum(r["succeeded"] for r in res.values())),
+            "iterations_mean": round(float(iters.mean()), 1), "iterations_max": int(iters.max()),
+            "outer_mean": round(float(np.mean([r["outer"] for r in res.values()])), 2),
+            "true_rel_residual_max": float(tr.max()),
+            "frac_true_rel_residual_le_1e-10": round(float((tr <= 1e-10).mean()), 4)}
+
+    log(f"batch: {len(systems)} systems resident, {total_rows} rows, {total_nnz} nnz")
+    X64, r64 = run(False)
+    log(f"batch fp64: {r64}")
+    Xm, rm = run(True)
+    log(f"batch mixed: {rm}")
+    err = max(((a - b).abs().max() / b.abs().max()).item() for a, b in zip(Xm, X64))
+    rm["max_error_vs_fp64"] = err
+    del Xm, X64
+    # the drop-in FP64 CG, one system at a time, on a sample
+    sample = systems[: max(0, args.batch_seq_sample)]
+    seq = None
+    if sample:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for j, s in enumerate(sample):
+            if j % 8 == 0:
+                log(f"batch: sequential drop-in CG system {j}/{len(sample)}")
+            A = MatrixCSR.from_device(s["rp"], s["ci"], s["val"], s["n"], s["n"])
+            X = torch.zeros(s["n"], s["k"], dtype=torch.float64, device=dev)
+            ConjugateGradient(s["n"], 20000, s["k"])(A, s["d"], 1.0, s["B"], 0.0, X, tol)
+        torch.cuda.synchronize()
+        tseq = time.perf_counter() - t0
+        seq = {"systems": len(sample), "solves_per_s": round(len(sample) / tseq, 2),
+               "note": "the same first systems of the batch, tg_cg_solve one at a time"}
+    return {
+        "workload": f"{len(systems)} SPD systems, n log-uniform [1e3, 2e5] (total rows {total_rows}, nnz {total_nnz}), "
+                    "2D/3D grid Laplacians with log-normal(0,1) edge coefficients + 1e-3 shift, 1-8 RHS "
+                    "uniform[-1,1], Jacobi, tol 1e-10; all added at t0",
+        "generation_s": round(t_gen, 2),
+        "fp64_batched": r64, "mixed_batched": rm,
+        "speedup_mixed_over_fp64_batched": round(rm["solves_per_s"] / r64["solves_per_s"], 3),
+        "fp64_dropin_sequential_sample": seq,
+    }
+
+
+def serve_requests(count, seed=5005):
+    """Request mix: half SPD CG requests shaped like the batched systems (mixed, Jacobi,
+    1e-10), half 2D upwind convection-diffusion GMRES(30) requests (n log-uniform in
+    [1e4, 1e6], Peclet log-uniform in [1, 100], P = I, 1e-8, 100 restarts)."""
+    import scipy.sparse  # noqa: F401  (only to keep import cost out of the timed loop)
+
+    from tensors_amd import generators as gen
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(count):
+        if rng.random() < 0.5:
+            n_t = float(np.exp(rng.uniform(np.log(1e3), np.log(2e5))))
+            dims = 2 if rng.random() < 0.5 else 3
+            side = max(2, int(round(n_t ** (1.0 / dims))))
+            k = int(rng.integers(1, 9))
+            rp, ci, val = gen.weighted_grid(dims, (side,) * dims, 1e-3, seed=seed * 10 + s)
+            n = len(rp) - 1
+            rows = np.repeat(np.arange(n, dtype=np.int32), np.diff(rp))
+            dg = np.zeros(n)
+            m = ci == rows
+            dg[rows[m]] = val[m]
+            B = gen.uniform(seed * 100 + s, n * k).reshape(n, k)
+            out.append(("cg", rp.astype(np.int32), ci, val, B, 1.0 / dg))
+        else:
+            n_t = float(np.exp(rng.uniform(np.log(1e4), np.log(1e6))))
+            g = max(2, int(round(np.sqrt(n_t))))
+            pe = float(np.exp(rng.uniform(0.0, np.log(100.0))))
+            rp, ci, val = gen.convection_diffusion_2d(g, pe)
+            n = g * g
+            B = gen.uniform(seed * 100 + s, n).reshape(n, 1)
+            out.append(("gmres", rp.astype(np.int32), ci, val, B, None))
+    return out
+
+
+def serve_section(args, dev):
+    """Serving loop: Poisson arrivals of a CG/GMRES request mix into tg_server (200 GB
+    admission budget, host arrays in and out, so latency includes the PCIe transfers).
+    First every request at once (saturated service rate), then Poisson arrivals at the
+    requested multiples of that rate; p50/p99 latency = completion - arrival."""
+    from tensors_amd.serving import Server
+
+    t_gen = time.perf_counter()
+    reqs = serve_requests(args.serve_requests)
+    t_gen = time.perf_counter() - t_gen
+    log(f"serve: generated {len(reqs)} requests in {t_gen:.1f}s")
+    budget = 200 << 30
+
+    def submit(srv, r):
+        kind, rp, ci, val, B, d = r
+        if kind == "cg":
+            return srv.submit_cg(rp, ci, val, B, tol=1e-10, max_iter=20000, inv_diag=d, mixed=True)
+        return srv.submit_gmres(rp, ci, val, B, tol=1e-8, restart=30, max_restarts=100)
+
+    def run(rate, seed):
+        srv = Server(budget, 3, args.batch_iters)
+        rng = np.random.default_rng(seed)
+        gaps = rng.exponential(1.0 / rate, len(reqs)) if rate > 0 else np.zeros(len(reqs))
+        arrivals = np.cumsum(gaps) - gaps[0]
+        ids, results = [], {}
+        t0 = time.perf_counter()
+        last = t0
+        for r, a in zip(reqs, arrivals):
+            while True:
+                now = time.perf_counter() - t0
+                if now >= a:
+                    break
+                for sid in srv.wait(min(0.002, a - now)):
+                    results[sid] = srv.result(sid)
+            ids.append(submit(srv, r))
+            if time.perf_counter() - last > 10:
+                log(f"serve rate {rate:.2f}/s: {len(ids)} submitted, {len(results)} done")
+                last = time.perf_counter()
+        while len(results) < len(ids):
+            for sid in srv.wait(0.05):
+                results[sid] = srv.result(sid)
+            if time.perf_counter() - last > 10:
+                log(f"serve rate {rate:.2f}/s: draining, {len(results)}/{len(ids)} done")
+                last = time.perf_counter()
+        st = srv.stats()
+        srv.close()
+        res = [results[i] for i in ids]
+        lat = np.array([r["t_done"] - r["t_submit"] for r in res])
+        t_first = min(r["t_submit"] for r in res)
+        t_last = max(r["t_done"] for r in res)
+        kinds = [q[0] for q in reqs]
+        ok = [r["succeeded"] and r["status"] == 0 for r in res]
+        lat_k = {k: lat[[i for i, q in enumerate(kinds) if q == k]] for k in ("cg", "gmres")}
+        return {
+            "offered_rate_per_s": None if rate <= 0 else round(rate, 3),
+            "solves_per_s": round(len(res) / (t_last - t_first), 3),
+            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 1),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 1),
+            "cg_p50_ms": round(float(np.percentile(lat_k["cg"], 50)) * 1e3, 1) if len(lat_k["cg"]) else None,
+            "gmres_p50_ms": round(float(np.percentile(lat_k["gmres"], 50)) * 1e3, 1) if len(lat_k["gmres"]) else None,
+            "succeeded": int(sum(ok)), "requests": len(res),
+            "peak_admitted_GB": round(st["peak_bytes"] / 1e9, 3),
+        }
+
+    sat = run(0.0, 5005)
+    log(f"serve saturated: {sat}")
+    sweep = []
+    for mult in [float(x) for x in args.serve_rates.split(",") if x]:
+        rate = mult * sat["solves_per_s"]
+        sweep.append(dict(run(rate, 5005 + len(sweep) + 1), load=mult))
+        log(f"serve load {mult}: {sweep[-1]}")
+    n_cg = sum(1 for q in reqs if q[0] == "cg")
+    return {
+        "workload": f"{len(reqs)} requests: {n_cg} SPD CG (weighted 2D/3D grids, n log-uniform [1e3, 2e5], "
+                    f"1-8 RHS, Jacobi, FP32 inner + FP64 refinement to 1e-10) and {len(reqs) - n_cg} GMRES(30) "
+                    "(2D upwind convection-diffusion, n log-uniform [1e4, 1e6], Peclet log-uniform [1, 100], "
+                    "P = I, 1e-8, 100 restarts); host arrays in/out; 200 GB admission budget; 3 GMRES streams",
+        "generation_s": round(t_gen, 2),
+        "saturated": sat,
+        "poisson_sweep": sweep,
     }
 
 
@@ -207,6 +481,7 @@ def main():
     def launch(c):
         return spmm_raw(M, N, nnz, rp32, cid, c["A"], 1.0, c["X"], c["k"], 0.0, c["Y"], c["k"], c["k"])
 
+    log(f"SpMM suite: nnz={nnz}, {len(cases)} launches per step")
     for _ in range(args.warmup):
         for c in cases:
             launch(c)
@@ -230,6 +505,7 @@ def main():
     kern_ms = sum(per_case_ms)
     value = step_bytes / (kern_ms * 1e-3) / 1e9
 
+    log(f"SpMM suite timed: {value:.1f} GB/s")
     # measured streaming bandwidth (copy and read) on a 2 GiB buffer
     nb = 2 << 30
     src = torch.empty(nb // 4, dtype=torch.float32, device=dev).fill_(1.0)
@@ -287,8 +563,18 @@ def main():
     del cases, vals, Xs, rp32, cid
     torch.cuda.empty_cache()
     if not args.no_cg:
+        log("CG section")
         out["cg"] = cg_section(args, dev)
+        torch.cuda.empty_cache()
+    if not args.no_batch:
+        log("batched CG section")
+        out["batch"] = batch_section(args, dev)
+        torch.cuda.empty_cache()
+    if not args.no_serve:
+        log("serving section")
+        out["serve"] = serve_section(args, dev)
     if not args.no_cpu:
+        log("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(rp, ci, v64, v32, X64h, X32h, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 

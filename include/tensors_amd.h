@@ -92,6 +92,11 @@ int tg_csr_spmm(int32_t m, int32_t n, int64_t nnz, const void* rp, tg_dtype rp_t
 /* Kernel-shape knobs read by tg_csr_spmm (no reference counterpart; the
  * reference picks its kernel by template arguments).  Keys:
  *   "spmv_kernel": nrhs == 1 kernel, 0 = streaming (default), 1 = team-per-row.
+ *   "spmv_team":   lanes per row for nrhs == 1, 0 = from the mean row length,
+ *                  else a power of two in [4, 64] (the streaming kernel caps it at 32).
+ *   "spmm_group":  lanes per RHS tile of the nrhs >= 2 kernel (a wave's 64/G lane
+ *                  groups split each row's nonzeros), 0 = enough lanes for all
+ *                  columns, else a power of two in [1, 64].
  * Process-wide; not thread-safe against concurrent SpMM calls. */
 int tg_tune_set(const char* key, int32_t value);
 int32_t tg_tune_get(const char* key);
@@ -246,9 +251,9 @@ int tg_csr_spgemm(int32_t m, int32_t k, int32_t n, const void* Arp, const int32_
  *   mixed != 0: FP32 inner CG sweeps + FP64 refinement until the true
  *               ||B - A X|| / ||B|| <= tol per column (max_iter bounds each
  *               inner solve).
- * All active systems advance together (one launch per CG phase covers every
- * system's row tiles); a system's arithmetic does not depend on the others, so
- * its result is bitwise the same alone and in any mix.  Systems may be added
+ * All active systems advance together (two launches per CG iteration cover
+ * every system's row tiles); a system's arithmetic does not depend on the
+ * others, so its result is bitwise the same alone and in any mix.  Systems may be added
  * between steps.  The caller's arrays must stay valid until the system has
  * finished; tg_batch_release frees the system's workspace.  Calls synchronise
  * the batch's stream.
@@ -267,6 +272,45 @@ int tg_batch_result(tg_batch_t batch, int64_t id, int32_t* finished, int32_t* su
                     int32_t* outer, double* rel_residuals);
 int tg_batch_release(tg_batch_t batch, int64_t id);
 int tg_batch_active(tg_batch_t batch, int32_t* systems, int64_t* bytes_in_use);
+/* Device bytes the engine itself allocates for one such system. */
+size_t tg_batch_request_bytes(int32_t n, int64_t nnz, int32_t nrhs, int32_t has_diag, int32_t mixed);
+
+/* ------------------------------------------------------------------------
+ * Serving loop (no reference counterpart).  Requests carry HOST arrays (int32
+ * rp/ci, FP64 values, B and X n x nrhs row-major with ld = nrhs); the caller
+ * keeps them valid until the request has finished.  A request is admitted, in
+ * arrival order, once its device bytes (matrix, vectors, solver workspace) fit
+ * in the budget next to the requests already admitted; it is then uploaded and
+ * solved while other requests run:
+ *   CG requests join the batched CG engine (tg_batch_*), which advances all of
+ *   them together on one stream (Jacobi `inv_diag` optional, mixed as there);
+ *   GMRES(restart) requests (P = I, left side) run tg_gmres_solve on one of
+ *   `gmres_workers` host threads, each with its own stream.
+ * On completion X is downloaded and all of the request's device memory freed.
+ * Results do not depend on what else is in flight.  Times are seconds since
+ * tg_server_create on the host steady clock.
+ * ------------------------------------------------------------------------ */
+typedef struct tg_server_s* tg_server_t;
+int tg_server_create(int64_t budget_bytes, int32_t gmres_workers, int32_t cg_iterations_per_poll, tg_server_t* out);
+/* Waits for every admitted and queued request, then frees the server. */
+int tg_server_destroy(tg_server_t server);
+int tg_server_submit_cg(tg_server_t server, int32_t n, int64_t nnz, const int32_t* rp, const int32_t* ci,
+                        const double* val, const double* inv_diag, int32_t nrhs, const double* B, double* X,
+                        double tol, int32_t max_iter, int32_t mixed, int64_t* id_out);
+int tg_server_submit_gmres(tg_server_t server, int32_t n, int64_t nnz, const int32_t* rp, const int32_t* ci,
+                           const double* val, int32_t nrhs, const double* B, double* X, double tol, int32_t restart,
+                           int32_t max_restarts, int64_t* id_out);
+/* Ids finished since the last call (at most cap), waiting up to timeout_s for one. */
+int tg_server_wait(tg_server_t server, int64_t* ids, int32_t cap, int32_t* count, double timeout_s);
+/* status: tg_status of the request; times[3] = submit, admit, done. */
+int tg_server_result(tg_server_t server, int64_t id, int32_t* status, int32_t* succeeded, int32_t* iterations,
+                     int32_t* restarts, double* rel_residuals, double* times);
+int tg_server_release(tg_server_t server, int64_t id);
+int tg_server_stats(tg_server_t server, int64_t* bytes_in_use, int64_t* peak_bytes, int32_t* queued,
+                    int32_t* running);
+/* Bytes a request is charged against the budget. */
+size_t tg_server_request_bytes(int32_t kind, int32_t n, int64_t nnz, int32_t nrhs, int32_t has_diag, int32_t mixed,
+                               int32_t restart);
 
 /* Streaming probes (no reference counterpart): dst <- src over `bytes`
  * (multiple of 16), and a read-only pass over src (sink: one device float that

@@ -65,6 +65,16 @@ SIGNATURES = {
     "tg_batch_result": (C.c_int, [vp, i64, vp, vp, vp, vp, vp]),
     "tg_batch_release": (C.c_int, [vp, i64]),
     "tg_batch_active": (C.c_int, [vp, vp, vp]),
+    "tg_batch_request_bytes": (C.c_size_t, [i32, i64, i32, i32, i32]),
+    "tg_server_create": (C.c_int, [i64, i32, i32, vp]),
+    "tg_server_destroy": (C.c_int, [vp]),
+    "tg_server_submit_cg": (C.c_int, [vp, i32, i64, vp, vp, vp, vp, i32, vp, vp, dbl, i32, i32, vp]),
+    "tg_server_submit_gmres": (C.c_int, [vp, i32, i64, vp, vp, vp, i32, vp, vp, dbl, i32, i32, vp]),
+    "tg_server_wait": (C.c_int, [vp, vp, i32, vp, dbl]),
+    "tg_server_result": (C.c_int, [vp, i64, vp, vp, vp, vp, vp, vp]),
+    "tg_server_release": (C.c_int, [vp, i64]),
+    "tg_server_stats": (C.c_int, [vp, vp, vp, vp, vp]),
+    "tg_server_request_bytes": (C.c_size_t, [i32, i32, i64, i32, i32, i32, i32]),
     "tg_probe_copy": (C.c_int, [vp, vp, i64, vp]),
     "tg_probe_read": (C.c_int, [vp, i64, vp, vp]),
     "tg_gen_uniform": (C.c_int, [u64, u64, i64, C.c_int, vp]),

@@ -3,19 +3,24 @@
 //
 // Every system runs the recurrences of ConjugateGradient::operator()
 // (ConjugateGradient.hpp:105-353) with its own per-column scalars, Jacobi or
-// identity preconditioner, tolerance and iteration limit.  The GPU work of one
-// iteration for ALL active systems is seven launches:
-//   spmm+dot(p,u)  |  alpha (per system)  |  x,r,z update + dot(r,z)  |
-//   rho/beta/convergence (per system)  |  p update
-// Each launch covers the row tiles (kTileRows rows of one system) of every
-// active system, so small and large systems share the machine and a system
-// retires as soon as it converges (its tiles drop out of the next tile list).
-// The host enqueues iterations in batches and polls the per-system flags in
-// between; systems can be added at any poll (continuous batching).
+// identity preconditioner, tolerance and iteration limit.  One iteration for ALL
+// active systems is two launches over the row tiles (kTileRows rows of one
+// system) of every active system:
+//   K1  beta = rz_i / rz_{i-1}, convergence test, p_i = z + beta p_{i-1}
+//       (own rows written, neighbour rows recomputed on the fly), u = A p_i,
+//       tile partials of p.u
+//   K2  alpha = rz_i / (p.u), x += alpha p, r -= alpha u, z = P r, tile
+//       partials of r.z
+// The per-system scalars are not kept in device state between launches: every
+// block of a system sums that system's tile partials itself, in one fixed order,
+// so all blocks see bitwise the same alpha / beta / convergence decision and no
+// block reads a value another block of the same launch writes.  p and the r.z
+// partials are double-buffered by iteration parity.  A system leaves the tile
+// list at the next host poll after it converged; until then its blocks return at
+// once.  Systems can be added at any poll (continuous batching).
 //
-// Determinism: a system's reductions go through tile partials that depend only
-// on the system itself (its n and k), then a fixed-order per-system sum, so a
-// system's result is bitwise the same alone and in any mix.
+// Determinism: a system's reductions depend only on the system itself (its n, k
+// and tile partials), so its result is bitwise the same alone and in any mix.
 //
 // Mixed systems: FP32 inner CG sweeps in this engine, inside an FP64 refinement
 // loop (r = b - A64 x, true-residual stopping test, x += d) run per system by
@@ -37,29 +42,35 @@ template <typename T> struct SysDev {
     const int32_t* rp;
     const int32_t* ci;
     const T* val;
-    const T* d;  // Jacobi diagonal or null
+    const T* d;  // Jacobi diagonal (inverse) or null
     const T* b;
     T* x;
     T* r;
     T* u;
-    T* p;
+    T* p[2];         // direction, by iteration parity
     T* z;            // == r without a preconditioner
-    double* partial; // tiles * k
+    double* ppu;     // tiles * k partials of p.u
+    double* prz[2];  // tiles * k partials of r.z, by parity
     int n, k, kpad, tiles;
 };
 
 struct SysState {
-    double rho[kBatchMaxK], rho_old[kBatchMaxK], alpha[kBatchMaxK], beta[kBatchMaxK], TOL[kBatchMaxK],
-        bsq[kBatchMaxK];
-    int iter, done, succ, max_iter, nan, pad[3];
+    double rho[kBatchMaxK], TOL[kBatchMaxK], bsq[kBatchMaxK];
+    int iter, done, succ, max_iter, nan;
+    int it_a;  // iteration index, written by K1 (tile 0), read by K2
+    int it_b;  // completed iterations, written by K2 (tile 0), read by K1
+    int pad;
 };
 
 struct Tile {
     int slot, r0, r1, t;
 };
 
-// ---- tile kernels --------------------------------------------------------------
-template <typename T>
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// ---- reductions ----------------------------------------------------------------
+// Block tree over the kBlock/kpad row groups; row group 0 column c writes partial[t*k + c].
 __device__ __forceinline__ void tile_partial(double v, int k, int kpad, double* partial, int t) {
     __shared__ double sh[kBlock];
     const int c = threadIdx.x % kpad, ro = threadIdx.x / kpad, rows = kBlock / kpad;
@@ -72,10 +83,26 @@ __device__ __forceinline__ void tile_partial(double v, int k, int kpad, double* 
     if (ro == 0 && c < k) partial[(int64_t)t * k + c] = sh[c];
 }
 
-// mode 0: r = b, z = P r, x = 0, partial r.z
+// Fixed-order sum over a system's tile partials (column c); every thread gets its column's total.
+__device__ __forceinline__ double sys_total(const double* partial, int tiles, int k, int kpad, int c, int j, int J) {
+    __shared__ double sh[kBlock];
+    double s = 0.0;
+    if (c < k)
+        for (int t = j; t < tiles; t += J) s += partial[(int64_t)t * k + c];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = J / 2; h > 0; h >>= 1) {
+        if (j < h) sh[threadIdx.x] += sh[threadIdx.x + h * kpad];
+        __syncthreads();
+    }
+    const double v = sh[c];
+    __syncthreads();
+    return v;
+}
+
+// ---- setup: r = b, z = P r, x = 0, partial r.z (parity 0) -------------------------
 template <typename T>
-__global__ __launch_bounds__(kBlock) void bk_init_kernel(const Tile* __restrict__ tiles, const SysDev<T>* __restrict__ sys,
-                                                         const SysState* __restrict__ st) {
+__global__ __launch_bounds__(kBlock) void bk_init_kernel(const Tile* __restrict__ tiles, const SysDev<T>* __restrict__ sys) {
     const Tile tl = tiles[blockIdx.x];
     const SysDev<T> S = sys[tl.slot];
     const int c = threadIdx.x % S.kpad, ro = threadIdx.x / S.kpad, rows = kBlock / S.kpad;
@@ -93,46 +120,113 @@ __global__ __launch_bounds__(kBlock) void bk_init_kernel(const Tile* __restrict_
             S.x[e] = T(0);
             acc += (double)rv * (double)zv;
         }
-    tile_partial<T>(acc, S.k, S.kpad, S.partial, tl.t);
+    tile_partial(acc, S.k, S.kpad, S.prz[0], tl.t);
 }
 
-// u = A p (rows of the tile), partial p.u
+// Setup totals of one system (one workgroup): rho_0, TOL = tol^2 |r.z|, initial test.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void bk_spmm_kernel(const Tile* __restrict__ tiles, const SysDev<T>* __restrict__ sys,
-                                                         const SysState* __restrict__ st) {
+__global__ __launch_bounds__(kBlock) void bk_setup_kernel(const int* __restrict__ slotp, const SysDev<T>* __restrict__ sys,
+                                                          SysState* __restrict__ st, const double* __restrict__ tol2) {
+    const int slot = *slotp;
+    SysState& ss = st[slot];
+    const SysDev<T> S = sys[slot];
+    const int c = threadIdx.x % S.kpad, j = threadIdx.x / S.kpad, J = kBlock / S.kpad;
+    const double tot = sys_total(S.prz[0], S.tiles, S.k, S.kpad, c, j, J);
+    __shared__ int flag, nanf;
+    if (threadIdx.x == 0) {
+        flag = 1;
+        nanf = 0;
+    }
+    __syncthreads();
+    if (j == 0 && c < S.k) {
+        ss.rho[c] = tot;
+        ss.bsq[c] = fabs(tot);
+        ss.TOL[c] = fabs(tot) * tol2[slot];
+        if (isnan(tot)) nanf = 1;
+        if (!(fabs(tot) <= ss.TOL[c])) flag = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ss.iter = 0;
+        ss.it_a = 0;
+        ss.it_b = 0;
+        ss.nan = nanf;
+        ss.succ = nanf ? 0 : flag;
+        ss.done = (nanf || flag || ss.max_iter <= 0) ? 1 : 0;
+    }
+}
+
+// ---- K1: direction + SpMM + partial p.u -----------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bk_dir_spmm_kernel(const Tile* __restrict__ tiles, const SysDev<T>* __restrict__ sys,
+                                                             SysState* __restrict__ st) {
     const Tile tl = tiles[blockIdx.x];
-    if (st[tl.slot].done) return;
+    SysState& ss = st[tl.slot];
     const SysDev<T> S = sys[tl.slot];
+    const int it = ss.it_b;
     const int c = threadIdx.x % S.kpad, ro = threadIdx.x / S.kpad, rows = kBlock / S.kpad;
-    double acc = 0.0;
-    if (c < S.k)
-        for (int i = tl.r0 + ro; i < tl.r1; i += rows) {
-            T s = T(0);
-            for (int l = S.rp[i]; l < S.rp[i + 1]; ++l) s += S.val[l] * S.p[(int64_t)S.ci[l] * S.k + c];
-            const int64_t e = (int64_t)i * S.k + c;
-            S.u[e] = s;
-            acc += (double)S.p[e] * (double)s;
+    const double rz = sys_total(S.prz[it & 1], S.tiles, S.k, S.kpad, c, ro, rows);
+    const double rz_prev = it > 0 ? sys_total(S.prz[(it + 1) & 1], S.tiles, S.k, S.kpad, c, ro, rows) : 1.0;
+    __shared__ int conv;
+    if (threadIdx.x == 0) conv = 1;
+    __syncthreads();
+    if (ro == 0 && c < S.k && !(fabs(rz) <= ss.TOL[c])) conv = 0;
+    __syncthreads();
+    if (conv || it >= ss.max_iter) {
+        if (tl.t == 0) {
+            if (ro == 0 && c < S.k) ss.rho[c] = rz;
+            if (threadIdx.x == 0) {
+                ss.iter = it;
+                ss.succ = conv;
+                ss.done = 1;
+            }
         }
-    tile_partial<T>(acc, S.k, S.kpad, S.partial, tl.t);
-}
-
-// x = (first ? 0 : x) + alpha p; r -= alpha u; z = P r; partial r.z
-template <typename T>
-__global__ __launch_bounds__(kBlock) void bk_update_kernel(const Tile* __restrict__ tiles, const SysDev<T>* __restrict__ sys,
-                                                           const SysState* __restrict__ st) {
-    const Tile tl = tiles[blockIdx.x];
-    const SysState& ss = st[tl.slot];
-    if (ss.done) return;
-    const SysDev<T> S = sys[tl.slot];
-    const int c = threadIdx.x % S.kpad, ro = threadIdx.x / S.kpad, rows = kBlock / S.kpad;
-    const bool first = ss.iter == 0;
+        return;
+    }
+    if (tl.t == 0 && threadIdx.x == 0) ss.it_a = it;
     double acc = 0.0;
     if (c < S.k) {
-        const T al = (T)ss.alpha[c];
+        const bool first = it == 0;
+        const T be = (T)(rz / rz_prev);
+        const T* __restrict__ pold = S.p[(it + 1) & 1];
+        T* __restrict__ pnew = S.p[it & 1];
+        const T* __restrict__ z = S.z;
+        for (int i = tl.r0 + ro; i < tl.r1; i += rows) {
+            T s = T(0);
+            for (int l = S.rp[i]; l < S.rp[i + 1]; ++l) {
+                const int64_t f = (int64_t)S.ci[l] * S.k + c;
+                const T pj = first ? z[f] : fma_t(be, pold[f], z[f]);
+                s += S.val[l] * pj;
+            }
+            const int64_t e = (int64_t)i * S.k + c;
+            const T pi = first ? z[e] : fma_t(be, pold[e], z[e]);
+            pnew[e] = pi;
+            S.u[e] = s;
+            acc += (double)pi * (double)s;
+        }
+    }
+    tile_partial(acc, S.k, S.kpad, S.ppu, tl.t);
+}
+
+// ---- K2: x, r, z update + partial r.z (next parity) -------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bk_update_kernel(const Tile* __restrict__ tiles, const SysDev<T>* __restrict__ sys,
+                                                           SysState* __restrict__ st) {
+    const Tile tl = tiles[blockIdx.x];
+    SysState& ss = st[tl.slot];
+    if (ss.done) return;
+    const SysDev<T> S = sys[tl.slot];
+    const int it = ss.it_a;
+    const int c = threadIdx.x % S.kpad, ro = threadIdx.x / S.kpad, rows = kBlock / S.kpad;
+    const double rz = sys_total(S.prz[it & 1], S.tiles, S.k, S.kpad, c, ro, rows);
+    const double pu = sys_total(S.ppu, S.tiles, S.k, S.kpad, c, ro, rows);
+    double acc = 0.0;
+    if (c < S.k) {
+        const T al = (T)(rz / pu);
+        const T* __restrict__ p = S.p[it & 1];
         for (int i = tl.r0 + ro; i < tl.r1; i += rows) {
             const int64_t e = (int64_t)i * S.k + c;
-            const T pv = S.p[e];
-            S.x[e] = first ? al * pv : S.x[e] + al * pv;
+            S.x[e] += al * p[e];
             const T rv = S.r[e] - al * S.u[e];
             S.r[e] = rv;
             T zv = rv;
@@ -143,93 +237,8 @@ __global__ __launch_bounds__(kBlock) void bk_update_kernel(const Tile* __restric
             acc += (double)rv * (double)zv;
         }
     }
-    tile_partial<T>(acc, S.k, S.kpad, S.partial, tl.t);
-}
-
-// p = z + beta p; with beta_zero p = z (the initial p = z)
-template <typename T>
-__global__ __launch_bounds__(kBlock) void bk_p_kernel(const Tile* __restrict__ tiles, const SysDev<T>* __restrict__ sys,
-                                                      const SysState* __restrict__ st, int initial) {
-    const Tile tl = tiles[blockIdx.x];
-    const SysState& ss = st[tl.slot];
-    if (!initial && ss.done) return;
-    const SysDev<T> S = sys[tl.slot];
-    const int c = threadIdx.x % S.kpad, ro = threadIdx.x / S.kpad, rows = kBlock / S.kpad;
-    if (c >= S.k) return;
-    const T be = initial ? T(0) : (T)ss.beta[c];
-    for (int i = tl.r0 + ro; i < tl.r1; i += rows) {
-        const int64_t e = (int64_t)i * S.k + c;
-        S.p[e] = initial ? S.z[e] : S.z[e] + be * S.p[e];
-    }
-}
-
-// ---- per-system finalisation (one workgroup per listed system) ------------------
-__device__ __forceinline__ double sys_total(const double* partial, int tiles, int k, int kpad, int c, int j, int J) {
-    __shared__ double sh[kBlock];
-    double s = 0.0;
-    if (c < k)
-        for (int t = j; t < tiles; t += J) s += partial[(int64_t)t * k + c];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int h = J / 2; h > 0; h >>= 1) {
-        if (j < h) sh[threadIdx.x] += sh[threadIdx.x + h * kpad];
-        __syncthreads();
-    }
-    const double v = sh[c];
-    __syncthreads();
-    return v;
-}
-
-// mode 0: setup (rho, bsq, TOL, NaN / zero handling, initial convergence)
-// mode 1: alpha = rho / (p.u)
-// mode 2: rho/beta update, convergence, ++iter
-template <typename T>
-__global__ __launch_bounds__(kBlock) void bk_finalize_kernel(const int* __restrict__ slots,
-                                                             const SysDev<T>* __restrict__ sys,
-                                                             SysState* __restrict__ st, int mode,
-                                                             const double* __restrict__ tol2) {
-    const int slot = slots[blockIdx.x];
-    SysState& ss = st[slot];
-    if (mode != 0 && ss.done) return;
-    const SysDev<T> S = sys[slot];
-    const int c = threadIdx.x % S.kpad, j = threadIdx.x / S.kpad, J = kBlock / S.kpad;
-    const double tot = sys_total(S.partial, S.tiles, S.k, S.kpad, c, j, J);
-    __shared__ int flag, nanf;
-    if (threadIdx.x == 0) {
-        flag = 1;
-        nanf = 0;
-    }
-    __syncthreads();
-    if (j == 0 && c < S.k) {
-        if (mode == 0) {
-            ss.rho[c] = tot;
-            ss.bsq[c] = fabs(tot);
-            ss.TOL[c] = fabs(tot) * tol2[slot];
-            if (isnan(tot)) atomicOr(&nanf, 1);
-            if (!(fabs(tot) <= ss.TOL[c])) atomicAnd(&flag, 0);
-        } else if (mode == 1) {
-            ss.alpha[c] = ss.rho[c] / tot;
-        } else {
-            const double old = ss.rho[c];
-            ss.rho_old[c] = old;
-            ss.rho[c] = tot;
-            ss.beta[c] = tot / old;
-            if (!(fabs(tot) <= ss.TOL[c])) atomicAnd(&flag, 0);
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        if (mode == 0) {
-            ss.iter = 0;
-            ss.nan = nanf;
-            ss.succ = nanf ? 0 : flag;
-            ss.done = (nanf || flag || ss.max_iter <= 0) ? 1 : 0;
-        } else if (mode == 2) {
-            ss.iter = ss.iter + 1;
-            ss.succ = flag;
-            ss.done = (flag || ss.iter >= ss.max_iter) ? 1 : 0;
-        }
-    }
+    tile_partial(acc, S.k, S.kpad, S.prz[(it + 1) & 1], tl.t);
+    if (tl.t == 0 && threadIdx.x == 0) ss.it_b = it + 1;
 }
 
 // ---- host engine ------------------------------------------------------------------
@@ -273,15 +282,19 @@ template <typename T> class CGBatch {
         S.kpad = pow2_at_least(k);
         S.tiles = ceil_div(n, kTileRows);
         const size_t N = (size_t)n * k;
-        const size_t nvec = d ? 4 : 3;
-        const size_t bytes = nvec * align256(N * sizeof(T)) + align256((size_t)S.tiles * k * sizeof(double));
+        const size_t nvec = d ? 5 : 4;  // r, u, p0, p1 (+ z)
+        const size_t vb = align256(N * sizeof(T)), pb = align256((size_t)S.tiles * k * sizeof(double));
+        const size_t bytes = nvec * vb + 3 * pb;
         char* mem = nullptr;
         TG_HIP(hipMallocAsync((void**)&mem, bytes, s_));
         S.r = (T*)mem;
-        S.u = (T*)(mem + align256(N * sizeof(T)));
-        S.p = (T*)(mem + 2 * align256(N * sizeof(T)));
-        S.z = d ? (T*)(mem + 3 * align256(N * sizeof(T))) : S.r;
-        S.partial = (double*)(mem + nvec * align256(N * sizeof(T)));
+        S.u = (T*)(mem + vb);
+        S.p[0] = (T*)(mem + 2 * vb);
+        S.p[1] = (T*)(mem + 3 * vb);
+        S.z = d ? (T*)(mem + 4 * vb) : S.r;
+        S.ppu = (double*)(mem + nvec * vb);
+        S.prz[0] = (double*)(mem + nvec * vb + pb);
+        S.prz[1] = (double*)(mem + nvec * vb + 2 * pb);
         mem_[slot] = mem;
         bytes_[slot] = bytes;
         used_[slot] = 1;
@@ -298,12 +311,9 @@ template <typename T> class CGBatch {
         tiles_of(slot, tl);
         TG_HIP(hipMemcpyAsync(dsetup_tiles_, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice, s_));
         TG_HIP(hipMemcpyAsync(dsetup_slot_, &slot, sizeof(int), hipMemcpyHostToDevice, s_));
-        hipLaunchKernelGGL((bk_init_kernel<T>), dim3((unsigned)tl.size()), dim3(kBlock), 0, s_, dsetup_tiles_, dsys_,
-                           dst_);
-        hipLaunchKernelGGL((bk_finalize_kernel<T>), dim3(1), dim3(kBlock), 0, s_, dsetup_slot_, dsys_, dst_, 0,
+        hipLaunchKernelGGL((bk_init_kernel<T>), dim3((unsigned)tl.size()), dim3(kBlock), 0, s_, dsetup_tiles_, dsys_);
+        hipLaunchKernelGGL((bk_setup_kernel<T>), dim3(1), dim3(kBlock), 0, s_, dsetup_slot_, dsys_, dst_,
                            (const double*)dtol2_);
-        hipLaunchKernelGGL((bk_p_kernel<T>), dim3((unsigned)tl.size()), dim3(kBlock), 0, s_, dsetup_tiles_, dsys_,
-                           dst_, 1);
         TG_LAUNCH_CHECK();
         // the setup buffers are reused by the next add: finish this one first
         TG_HIP(hipStreamSynchronize(s_));
@@ -320,13 +330,8 @@ template <typename T> class CGBatch {
         }
         if (ntiles_ == 0) return TG_OK;
         for (int it = 0; it < iters; ++it) {
-            hipLaunchKernelGGL((bk_spmm_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s_, dtiles_, dsys_, dst_);
-            hipLaunchKernelGGL((bk_finalize_kernel<T>), dim3(nslots_), dim3(kBlock), 0, s_, dslots_, dsys_, dst_, 1,
-                               (const double*)dtol2_);
+            hipLaunchKernelGGL((bk_dir_spmm_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s_, dtiles_, dsys_, dst_);
             hipLaunchKernelGGL((bk_update_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s_, dtiles_, dsys_, dst_);
-            hipLaunchKernelGGL((bk_finalize_kernel<T>), dim3(nslots_), dim3(kBlock), 0, s_, dslots_, dsys_, dst_, 2,
-                               (const double*)dtol2_);
-            hipLaunchKernelGGL((bk_p_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s_, dtiles_, dsys_, dst_, 0);
         }
         TG_LAUNCH_CHECK();
         return TG_OK;
@@ -420,27 +425,16 @@ template <typename T> class CGBatch {
         int rc = poll(states);
         if (rc) return rc;
         std::vector<Tile> tl;
-        std::vector<int> sl;
         for (int i = 0; i < (int)used_.size(); ++i)
-            if (used_[i] && !parked_[i] && !states[i].done) {
-                tiles_of(i, tl);
-                sl.push_back(i);
-            }
+            if (used_[i] && !parked_[i] && !states[i].done) tiles_of(i, tl);
         if (tl.size() > tiles_cap_) {
             if (dtiles_) TG_HIP(hipFreeAsync(dtiles_, s_));
             tiles_cap_ = std::max(tl.size(), 2 * tiles_cap_);
             TG_HIP(hipMallocAsync((void**)&dtiles_, tiles_cap_ * sizeof(Tile), s_));
         }
-        if (sl.size() > slots_cap_) {
-            if (dslots_) TG_HIP(hipFreeAsync(dslots_, s_));
-            slots_cap_ = std::max(sl.size(), 2 * slots_cap_);
-            TG_HIP(hipMallocAsync((void**)&dslots_, slots_cap_ * sizeof(int), s_));
-        }
         if (!tl.empty()) TG_HIP(hipMemcpyAsync(dtiles_, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice, s_));
-        if (!sl.empty()) TG_HIP(hipMemcpyAsync(dslots_, sl.data(), sl.size() * sizeof(int), hipMemcpyHostToDevice, s_));
         TG_HIP(hipStreamSynchronize(s_));  // host vectors go out of scope
         ntiles_ = (int)tl.size();
-        nslots_ = (int)sl.size();
         dirty_ = false;
         return TG_OK;
     }
@@ -452,7 +446,6 @@ template <typename T> class CGBatch {
         if (dst_) (void)hipFreeAsync(dst_, s_);
         if (dtol2_) (void)hipFreeAsync(dtol2_, s_);
         if (dtiles_) (void)hipFreeAsync(dtiles_, s_);
-        if (dslots_) (void)hipFreeAsync(dslots_, s_);
         if (dsetup_tiles_) (void)hipFreeAsync(dsetup_tiles_, s_);
         if (dsetup_slot_) (void)hipFreeAsync(dsetup_slot_, s_);
         (void)hipStreamSynchronize(s_);
@@ -468,11 +461,10 @@ template <typename T> class CGBatch {
     SysState* dst_ = nullptr;
     double* dtol2_ = nullptr;
     Tile* dtiles_ = nullptr;
-    int* dslots_ = nullptr;
-    size_t tiles_cap_ = 0, slots_cap_ = 0;
+    size_t tiles_cap_ = 0;
     Tile* dsetup_tiles_ = nullptr;
     int* dsetup_slot_ = nullptr;
-    int ntiles_ = 0, nslots_ = 0;
+    int ntiles_ = 0;
     bool dirty_ = true;
 };
 
@@ -497,7 +489,7 @@ __global__ __launch_bounds__(kBlock) void bm_residual_kernel(int n, int k, int k
             r32[e] = (float)r;
             acc += r * r;
         }
-    tile_partial<double>(acc, k, kpad, partial, blockIdx.x);
+    tile_partial(acc, k, kpad, partial, blockIdx.x);
 }
 
 __global__ __launch_bounds__(kBlock) void bm_norm_total_kernel(int tiles, int k, int kpad, const double* __restrict__ partial,
@@ -798,6 +790,18 @@ extern "C" int tg_batch_release(tg_batch_t h, int64_t id) {
     if (S.mem) TG_HIP(hipFreeAsync(S.mem, Bt.s));
     Bt.sys.erase(it);
     return TG_OK;
+}
+
+// Device bytes the engine allocates for one system (its own workspace, not the
+// caller's matrix and vectors).
+extern "C" size_t tg_batch_request_bytes(int32_t n, int64_t nnz, int32_t nrhs, int32_t has_diag, int32_t mixed) {
+    if (n < 1 || nrhs < 1 || nrhs > kBatchMaxK || nnz < 0) return 0;
+    const size_t N = (size_t)n * nrhs, tiles = (size_t)ceil_div(n, kTileRows);
+    const size_t pb = align256(tiles * nrhs * sizeof(double));
+    const size_t nvec = has_diag ? 5 : 4;
+    if (!mixed) return nvec * align256(N * sizeof(double)) + 3 * pb;
+    return nvec * align256(N * sizeof(float)) + 3 * pb + align256((size_t)nnz * 4) + (has_diag ? align256((size_t)n * 4) : 0) +
+           2 * align256(N * 4) + pb + align256((size_t)2 * nrhs * 8);
 }
 
 extern "C" int tg_batch_active(tg_batch_t h, int32_t* systems, int64_t* bytes_in_use) {

@@ -365,7 +365,13 @@ static int pow2ceil(int v) {
 
 constexpr int kRowsPerTeam = 8;
 
+// Shape overrides (0 = automatic), set through tg_tune_set by the autotuner:
+// lanes per row of the nrhs == 1 kernels, and lanes per RHS tile of spmm_wave.
+static int g_spmv_team = 0;
+static int g_spmm_group = 0;
+
 static int spmv_team_width(int32_t m, int64_t nnz) {
+    if (g_spmv_team) return g_spmv_team;
     const double avg = m > 0 ? (double)nnz / m : 0.0;
     int t = pow2ceil((int)std::ceil(avg));
     return std::min(64, std::max(4, t));
@@ -499,7 +505,7 @@ static int spmm_typed(int32_t m, int64_t nnz, const RP* rp, const int32_t* ci, c
         else if (nrhs % 2 == 0 && aligned(8))
             V = 2;
     }
-    const int G = std::min(64, pow2ceil((nrhs + V - 1) / V));
+    const int G = g_spmm_group ? g_spmm_group : std::min(64, pow2ceil((nrhs + V - 1) / V));
     switch (V) {
         case 1:
             return launch_spmm_v<T, RP, 1>(G, m, rp, ci, a, alpha, X, ldX, beta, Y, ldY, nrhs, alpha_one,
@@ -534,18 +540,37 @@ static int axpby_typed(int32_t m, int32_t nrhs, double alpha_d, const T* X, int6
 
 using namespace tg;
 
+static bool pow2_in(int v, int lo, int hi) { return v >= lo && v <= hi && (v & (v - 1)) == 0; }
+
 extern "C" int tg_tune_set(const char* key, int32_t value) {
     if (!key) return fail(TG_ERR_INVALID, "tg_tune_set: null key");
-    if (std::string(key) == "spmv_kernel") {
+    const std::string k(key);
+    if (k == "spmv_kernel") {
         if (value != 0 && value != 1) return fail(TG_ERR_INVALID, "tg_tune_set: spmv_kernel must be 0 or 1");
         g_spmv_variant = value;
+        return TG_OK;
+    }
+    if (k == "spmv_team") {
+        if (value != 0 && !pow2_in(value, 4, 64))
+            return fail(TG_ERR_INVALID, "tg_tune_set: spmv_team must be 0 (auto) or a power of two in [4, 64]");
+        g_spmv_team = value;
+        return TG_OK;
+    }
+    if (k == "spmm_group") {
+        if (value != 0 && !pow2_in(value, 1, 64))
+            return fail(TG_ERR_INVALID, "tg_tune_set: spmm_group must be 0 (auto) or a power of two in [1, 64]");
+        g_spmm_group = value;
         return TG_OK;
     }
     return fail(TG_ERR_INVALID, "tg_tune_set: unknown key '%s'", key);
 }
 
 extern "C" int32_t tg_tune_get(const char* key) {
-    if (key && std::string(key) == "spmv_kernel") return spmv_variant();
+    if (!key) return -1;
+    const std::string k(key);
+    if (k == "spmv_kernel") return spmv_variant();
+    if (k == "spmv_team") return g_spmv_team;
+    if (k == "spmm_group") return g_spmm_group;
     return -1;
 }
 

@@ -60,6 +60,15 @@ class MatrixCSR:
         self._dev = None
         self._job_ptr = None
 
+    @classmethod
+    def from_device(cls, rp, ci, vals, m, n):
+        """Wraps CUDA tensors that already hold the CSR arrays (host copies are made for
+        the accessors; the device arrays are used as they are)."""
+        A = cls(rp.cpu().numpy(), ci.cpu().numpy(), None if vals is None else vals.cpu().numpy(), m, n,
+                device=rp.device)
+        A._dev = {"rp": rp, "ci": ci, "vals": {} if vals is None else {vals.dtype: vals}}
+        return A
+
     # -- accessors (PatternCSR.hpp / MatrixCSR.hpp) --------------------------------
     def RowCount(self): return self.m
     def ColCount(self): return self.n

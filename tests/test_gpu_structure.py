@@ -37,11 +37,10 @@ def test_from_triples_matches_oracle(cuda, compress, symmetrize):
     else:  # duplicates kept: compare the multiset of values per (row, col)
         key = lambda rp, ci, v: sorted(zip(np.repeat(np.arange(m), np.diff(rp)), ci, v))
         assert key(grp, gci, gval) == key(orp, oci, oval)
-    A = sp.coo_matrix((V, (I, J)), shape=(m, n))
-    if symmetrize:
+    if symmetrize:  # one coo of all triples: sparse "+" would drop sums that cancel to 0
         off = I != J
-        A = A + sp.coo_matrix((V[off], (J[off], I[off])), shape=(m, n))
-    A = A.tocsr()
+        I, J, V = np.concatenate([I, J[off]]), np.concatenate([J, I[off]]), np.concatenate([V, V[off]])
+    A = sp.coo_matrix((V, (I, J)), shape=(m, n)).tocsr()
     A.sum_duplicates()
     if compress:
         assert np.array_equal(grp, A.indptr) and np.array_equal(gci, A.indices)
