@@ -248,9 +248,13 @@ int tg_csr_spgemm(int32_t m, int32_t k, int32_t n, const void* Arp, const int32_
  * Jacobi diagonal `diag` (z = diag * r), nrhs in 1..8, B and X n x nrhs
  * row-major device blocks (ld = nrhs), X <- A^{-1} B.
  *   mixed == 0: FP64 CG with the reference stopping rule |r.z| <= tol^2 |b.Pb|.
- *   mixed != 0: FP32 inner CG sweeps + FP64 refinement until the true
- *               ||B - A X|| / ||B|| <= tol per column (max_iter bounds each
- *               inner solve).
+ *   mixed != 0: CG in FP32 (FP32 copy of A, FP32 vectors, FP64 reductions)
+ *               with FP64 residual replacement: every 16 iterations the host
+ *               may replace the recursive residual by b - A x computed in FP64
+ *               (x accumulated in FP64), keeping the search direction; done
+ *               when the true ||B - A X|| / ||B|| <= tol in every column
+ *               (max_iter bounds the total iterations).  `outer` in
+ *               tg_batch_result counts the replacements.
  * All active systems advance together (two launches per CG iteration cover
  * every system's row tiles); a system's arithmetic does not depend on the
  * others, so its result is bitwise the same alone and in any mix.  Systems may be added

@@ -1,0 +1,78 @@
+"""Shape autotuner for the SpMM kernels.
+
+The kernels take their shapes from process-wide knobs (tg_tune_set):
+  nrhs == 1: "spmv_kernel" (0 streaming, 1 team-per-row) and "spmv_team" (lanes per
+             row, 4..64; the streaming kernel caps it at 32);
+  nrhs >= 2: "spmm_group" (lanes per RHS tile, 1..64; a wave's 64/G lane groups split
+             each row's nonzeros, and columns beyond G*V are done in further tiles).
+0 means "pick from the matrix" (mean row length, nrhs).  tune_spmm times every
+candidate on the caller's own matrix and block with HIP events on the launch stream,
+keeps the fastest, and leaves it set (or restores the previous setting).  Results
+are identical for every shape up to floating-point summation order within a row's
+lane groups.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _capi as capi
+from .sparse import MatrixCSR, spmm_raw
+
+KNOBS = ("spmv_kernel", "spmv_team", "spmm_group")
+
+
+def get_knobs():
+    return {k: int(capi.lib().tg_tune_get(k.encode())) for k in KNOBS}
+
+
+def set_knobs(cfg):
+    for k, v in cfg.items():
+        capi.call("tg_tune_set", k.encode(), int(v))
+
+
+def candidates(nrhs):
+    if nrhs == 1:
+        out = [{"spmv_kernel": 0, "spmv_team": 0}, {"spmv_kernel": 1, "spmv_team": 0}]
+        out += [{"spmv_kernel": 0, "spmv_team": t} for t in (4, 8, 16, 32)]
+        out += [{"spmv_kernel": 1, "spmv_team": t} for t in (4, 8, 16, 32, 64)]
+        return out
+    gmax = 1
+    while gmax < nrhs and gmax < 64:
+        gmax *= 2
+    return [{"spmm_group": 0}] + [{"spmm_group": g} for g in (1, 2, 4, 8, 16, 32, 64) if g <= gmax]
+
+
+def tune_spmm(A: MatrixCSR, X: torch.Tensor, Y: torch.Tensor, reps: int = 5, keep: bool = True):
+    """Times Y <- A X for every shape candidate; returns (best_cfg, table of (cfg, ms)).
+    X, Y: CUDA tensors of A's value dtype, n x nrhs / m x nrhs (or 1-D for nrhs = 1)."""
+    nrhs = 1 if X.dim() == 1 else X.shape[1]
+    rp, ci, vals = A.device_arrays(X.dtype)
+    ld_x = 1 if X.dim() == 1 else X.stride(0)
+    ld_y = 1 if Y.dim() == 1 else Y.stride(0)
+    stream = torch.cuda.current_stream()
+    before = get_knobs()
+    table = []
+    try:
+        for cfg in candidates(nrhs):
+            set_knobs(cfg)
+
+            def run():
+                spmm_raw(A.m, A.n, A.NonzeroCount(), rp, ci, vals, 1.0, X, ld_x, 0.0, Y, ld_y, nrhs)
+
+            run()
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            ev[0].record(stream)
+            for _ in range(reps):
+                run()
+            ev[1].record(stream)
+            torch.cuda.synchronize()
+            table.append((dict(cfg), ev[0].elapsed_time(ev[1]) / reps))
+    finally:
+        set_knobs(before)
+    best = min(table, key=lambda t: t[1])[0]
+    if keep:
+        set_knobs(best)
+    return best, table

@@ -22,9 +22,10 @@
 // Determinism: a system's reductions depend only on the system itself (its n, k
 // and tile partials), so its result is bitwise the same alone and in any mix.
 //
-// Mixed systems: FP32 inner CG sweeps in this engine, inside an FP64 refinement
-// loop (r = b - A64 x, true-residual stopping test, x += d) run per system by
-// the host between polls.
+// Mixed systems run the same recurrences in FP32 in a second engine, with FP64
+// residual replacement at checkpoints every 16 of the system's own iterations
+// (see the replacement section below); the stopping test is on the FP64 true
+// residual.
 
 #include <map>
 #include <memory>
@@ -52,14 +53,16 @@ template <typename T> struct SysDev {
     double* ppu;     // tiles * k partials of p.u
     double* prz[2];  // tiles * k partials of r.z, by parity
     int n, k, kpad, tiles;
+    int check_every;  // > 0: pause at every multiple of this many iterations for the host
 };
 
 struct SysState {
     double rho[kBatchMaxK], TOL[kBatchMaxK], bsq[kBatchMaxK];
     int iter, done, succ, max_iter, nan;
     int it_a;  // iteration index, written by K1 (tile 0), read by K2
-    int it_b;  // completed iterations, written by K2 (tile 0), read by K1
-    int pad;
+    int it_b;     // completed iterations, written by K2 (tile 0), read by K1
+    int paused;   // done because of a checkpoint (written by K1 tile 0, read by the host)
+    int resumed;  // iteration at which the host last resumed the system (no pause there)
 };
 
 struct Tile {
@@ -150,6 +153,8 @@ __global__ __launch_bounds__(kBlock) void bk_setup_kernel(const int* __restrict_
         ss.iter = 0;
         ss.it_a = 0;
         ss.it_b = 0;
+        ss.paused = 0;
+        ss.resumed = 0;
         ss.nan = nanf;
         ss.succ = nanf ? 0 : flag;
         ss.done = (nanf || flag || ss.max_iter <= 0) ? 1 : 0;
@@ -172,14 +177,16 @@ __global__ __launch_bounds__(kBlock) void bk_dir_spmm_kernel(const Tile* __restr
     __syncthreads();
     if (ro == 0 && c < S.k && !(fabs(rz) <= ss.TOL[c])) conv = 0;
     __syncthreads();
-    if (conv || it >= ss.max_iter) {
-        if (tl.t == 0) {
-            if (ro == 0 && c < S.k) ss.rho[c] = rz;
-            if (threadIdx.x == 0) {
-                ss.iter = it;
-                ss.succ = conv;
-                ss.done = 1;
-            }
+    if (tl.t == 0 && ro == 0 && c < S.k) ss.rho[c] = rz;  // current r.z, read by the host between polls
+    // Checkpoints depend only on the system's own iteration count, so the host's
+    // decisions there do not depend on when it polls or what else runs.
+    const bool pause = S.check_every > 0 && it > 0 && it % S.check_every == 0 && it != ss.resumed;
+    if (conv || it >= ss.max_iter || pause) {
+        if (tl.t == 0 && threadIdx.x == 0) {
+            ss.iter = it;
+            ss.succ = conv;
+            ss.paused = (!conv && it < ss.max_iter) ? 1 : 0;
+            ss.done = 1;
         }
         return;
     }
@@ -241,6 +248,24 @@ __global__ __launch_bounds__(kBlock) void bk_update_kernel(const Tile* __restric
     if (tl.t == 0 && threadIdx.x == 0) ss.it_b = it + 1;
 }
 
+struct Resume {
+    int slot;
+    double scale;
+};
+
+// One workgroup per resumed system: tolerance scaled, pause cleared, iterating again.
+__global__ void bk_resume_kernel(const Resume* __restrict__ list, SysState* __restrict__ st) {
+    const Resume rs = list[blockIdx.x];
+    SysState& ss = st[rs.slot];
+    if (threadIdx.x < kBatchMaxK) ss.TOL[threadIdx.x] *= rs.scale;
+    if (threadIdx.x == 0) {
+        ss.resumed = ss.iter;
+        ss.paused = 0;
+        ss.done = 0;
+        ss.succ = 0;
+    }
+}
+
 // ---- host engine ------------------------------------------------------------------
 static int ceil_div(int a, int b) { return (a + b - 1) / b; }
 static int pow2_at_least(int v) {
@@ -257,7 +282,7 @@ template <typename T> class CGBatch {
 
     // Adds a system; its setup runs at once.  b, x: n x k device blocks (ld = k).
     int add(int n, const int32_t* rp, const int32_t* ci, const T* val, const T* d, int k, const T* b, T* x,
-            double tol, int max_iter, int* slot_out) {
+            double tol, int max_iter, int* slot_out, int check_every = 0) {
         if (n < 1 || k < 1 || k > kBatchMaxK) return fail(TG_ERR_INVALID, "batch: n >= 1 and 1 <= nrhs <= 8");
         int slot = -1;
         for (int i = 0; i < (int)used_.size(); ++i)
@@ -279,6 +304,7 @@ template <typename T> class CGBatch {
         S.x = x;
         S.n = n;
         S.k = k;
+        S.check_every = check_every;
         S.kpad = pow2_at_least(k);
         S.tiles = ceil_div(n, kTileRows);
         const size_t N = (size_t)n * k;
@@ -323,7 +349,25 @@ template <typename T> class CGBatch {
     }
 
     // Enqueues `iters` iterations for every active (not done) system, then polls.
+    int flush_resumes() {
+        if (resumes_.empty()) return TG_OK;
+        if (resumes_.size() > resume_cap_) {
+            if (dresume_) TG_HIP(hipFreeAsync(dresume_, s_));
+            resume_cap_ = std::max(resumes_.size(), 2 * resume_cap_);
+            TG_HIP(hipMallocAsync((void**)&dresume_, resume_cap_ * sizeof(Resume), s_));
+        }
+        TG_HIP(hipMemcpyAsync(dresume_, resumes_.data(), resumes_.size() * sizeof(Resume), hipMemcpyHostToDevice, s_));
+        hipLaunchKernelGGL(bk_resume_kernel, dim3((unsigned)resumes_.size()), dim3(kWave), 0, s_, dresume_, dst_);
+        TG_LAUNCH_CHECK();
+        // the host list is read by the copy: keep it until the stream passes it
+        TG_HIP(hipStreamSynchronize(s_));
+        resumes_.clear();
+        return TG_OK;
+    }
+
     int step(int iters) {
+        int rc0 = flush_resumes();
+        if (rc0) return rc0;
         if (dirty_) {
             int rc = rebuild();
             if (rc) return rc;
@@ -339,6 +383,8 @@ template <typename T> class CGBatch {
 
     // Reads the states of all used slots (synchronises).
     int poll(std::vector<SysState>& out) {
+        int rc0 = flush_resumes();
+        if (rc0) return rc0;
         out.resize(used_.size());
         if (!used_.empty())
             TG_HIP(hipMemcpyAsync(out.data(), dst_, used_.size() * sizeof(SysState), hipMemcpyDeviceToHost, s_));
@@ -373,7 +419,12 @@ template <typename T> class CGBatch {
         return s;
     }
     int active_tiles() const { return ntiles_; }
+    int capacity() const { return (int)cap_; }
     const SysDev<T>& desc(int slot) const { return host_[slot]; }
+
+    // Continues a system the device marked done, with its tolerance scaled by `scale`.
+    // Queued; applied by one launch before the next iterations (or poll).
+    void resume(int slot, double scale) { resumes_.push_back(Resume{slot, scale}); }
 
     void mark_done_host(int slot) {
         if (slot >= 0 && slot < (int)parked_.size()) parked_[slot] = 1;
@@ -448,11 +499,15 @@ template <typename T> class CGBatch {
         if (dtiles_) (void)hipFreeAsync(dtiles_, s_);
         if (dsetup_tiles_) (void)hipFreeAsync(dsetup_tiles_, s_);
         if (dsetup_slot_) (void)hipFreeAsync(dsetup_slot_, s_);
+        if (dresume_) (void)hipFreeAsync(dresume_, s_);
         (void)hipStreamSynchronize(s_);
     }
 
     hipStream_t s_;
     size_t cap_ = 0;
+    std::vector<Resume> resumes_;
+    Resume* dresume_ = nullptr;
+    size_t resume_cap_ = 0;
     std::vector<char> used_, parked_;
     std::vector<char*> mem_;
     std::vector<size_t> bytes_;
@@ -468,28 +523,51 @@ template <typename T> class CGBatch {
     bool dirty_ = true;
 };
 
-// ---- FP64 refinement kernels for mixed systems (one system per launch) -----------
-// r32 = float(b - A x) rows; partial ||b - A x||^2 per tile (tiles of kTileRows)
-__global__ __launch_bounds__(kBlock) void bm_residual_kernel(int n, int k, int kpad, const int32_t* __restrict__ rp,
-                                                             const int32_t* __restrict__ ci,
-                                                             const double* __restrict__ val,
-                                                             const double* __restrict__ b, const double* __restrict__ x,
-                                                             float* __restrict__ r32, double* __restrict__ partial,
-                                                             int first) {
+// ---- FP64 residual replacement for mixed systems ----------------------------------
+// A mixed system runs FP32 CG in the engine (A32, r, z, p, u and the increment x32
+// in FP32; reductions in FP64).  Between polls the host replaces the recursive
+// residual by the true one (reliable updates: van der Vorst & Ye, "Residual
+// replacement strategies for Krylov subspace iterative methods for the
+// convergence of true residuals", SIAM J. Sci. Comput. 22 (2000); the
+// mixed-precision form of Clark et al., Comput. Phys. Commun. 181 (2010)):
+//   r = b - A64 (x64 + x32) in FP64, r32 = float(r), z32 = P r32, new r.z
+//   partials; then x64 += x32, x32 = 0.
+// The search direction p is kept, so CG continues instead of restarting.
+
+// r = b - A (x64 + x32) on the engine's tile; writes r32, z32 (if P), r.z partial
+// (the engine's parity slot) and ||r||^2 partial.  x64 and x32 are only read.
+__global__ __launch_bounds__(kBlock) void bm_replace_kernel(int n, int k, int kpad, const int32_t* __restrict__ rp,
+                                                            const int32_t* __restrict__ ci,
+                                                            const double* __restrict__ val,
+                                                            const double* __restrict__ b, const double* __restrict__ x64,
+                                                            const float* __restrict__ x32, const float* __restrict__ d32,
+                                                            float* __restrict__ r32, float* __restrict__ z32,
+                                                            double* __restrict__ prz, double* __restrict__ pnorm) {
     const int r0 = blockIdx.x * kTileRows, r1 = min(n, r0 + kTileRows);
     const int c = threadIdx.x % kpad, ro = threadIdx.x / kpad, rows = kBlock / kpad;
-    double acc = 0.0;
+    double acc_rz = 0.0, acc_n = 0.0;
     if (c < k)
         for (int i = r0 + ro; i < r1; i += rows) {
             double ax = 0.0;
-            if (!first)
-                for (int l = rp[i]; l < rp[i + 1]; ++l) ax += val[l] * x[(int64_t)ci[l] * k + c];
+            for (int l = rp[i]; l < rp[i + 1]; ++l) {
+                const int64_t f = (int64_t)ci[l] * k + c;
+                ax += val[l] * (x64[f] + (double)x32[f]);
+            }
             const int64_t e = (int64_t)i * k + c;
             const double r = b[e] - ax;
-            r32[e] = (float)r;
-            acc += r * r;
+            const float rf = (float)r;
+            r32[e] = rf;
+            float zf = rf;
+            if (d32) {
+                zf = d32[i] * rf;
+                z32[e] = zf;
+            }
+            acc_rz += (double)rf * (double)zf;
+            acc_n += r * r;
         }
-    tile_partial(acc, k, kpad, partial, blockIdx.x);
+    tile_partial(acc_rz, k, kpad, prz, blockIdx.x);
+    __syncthreads();
+    tile_partial(acc_n, k, kpad, pnorm, blockIdx.x);
 }
 
 __global__ __launch_bounds__(kBlock) void bm_norm_total_kernel(int tiles, int k, int kpad, const double* __restrict__ partial,
@@ -499,10 +577,26 @@ __global__ __launch_bounds__(kBlock) void bm_norm_total_kernel(int tiles, int k,
     if (j == 0 && c < k) out[c] = sqrt(t);
 }
 
-__global__ __launch_bounds__(kBlock) void bm_axpy_kernel(int64_t count, const float* __restrict__ d,
-                                                         double* __restrict__ x) {
-    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < count; e += (int64_t)gridDim.x * kBlock)
-        x[e] += (double)d[e];
+// ||b||^2 partials per tile
+__global__ __launch_bounds__(kBlock) void bm_bnorm_kernel(int n, int k, int kpad, const double* __restrict__ b,
+                                                          double* __restrict__ partial) {
+    const int r0 = blockIdx.x * kTileRows, r1 = min(n, r0 + kTileRows);
+    const int c = threadIdx.x % kpad, ro = threadIdx.x / kpad, rows = kBlock / kpad;
+    double acc = 0.0;
+    if (c < k)
+        for (int i = r0 + ro; i < r1; i += rows) {
+            const double v = b[(int64_t)i * k + c];
+            acc += v * v;
+        }
+    tile_partial(acc, k, kpad, partial, blockIdx.x);
+}
+
+// x64 += x32, x32 = 0
+__global__ __launch_bounds__(kBlock) void bm_flush_kernel(int64_t count, float* __restrict__ x32, double* __restrict__ x64) {
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < count; e += (int64_t)gridDim.x * kBlock) {
+        x64[e] += (double)x32[e];
+        x32[e] = 0.0f;
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void bm_zero_kernel(int64_t count, double* __restrict__ x) {
@@ -530,21 +624,18 @@ struct BatchSystem {
     double* x64;  // caller's X
     double tol;
     int max_iter;
-    // plain FP64
-    int slot64 = -1;
+    int slot = -1;  // engine slot (FP64 engine for plain systems, FP32 engine for mixed ones)
     // mixed
-    int slot32 = -1;
-    int state = 0;  // 0: needs residual, 1: inner running, 2: done
-    int outer = 0, inner_total = 0, max_outer = 30;
-    double inner_tol = 1e-5;
-    char* mem = nullptr;  // a32, d32, r32, dx32, partial, norms
+    int replacements = 0;
+    double est_last = 1.0;   // preconditioned residual estimate at the last replacement
+    char* mem = nullptr;     // a32, d32, b32, x32, norm partials, norms
     size_t mem_bytes = 0;
     float* a32 = nullptr;
     float* d32 = nullptr;
-    float* r32 = nullptr;
-    float* dx32 = nullptr;
-    double* partial = nullptr;
-    double* norms = nullptr;  // 2k
+    float* b32 = nullptr;
+    float* x32 = nullptr;
+    double* pnorm = nullptr;
+    double* norms = nullptr;  // k
     std::vector<double> bnorm;
     // results
     int finished = 0, succeeded = 0, iterations = 0;
@@ -558,10 +649,18 @@ struct Batch {
     std::map<int64_t, std::unique_ptr<BatchSystem>> sys;
     std::vector<int64_t> finished_queue;
     int64_t next_id = 1;
+    double* dnorms = nullptr;  // kBatchMaxK true-residual norms per FP32-engine slot
+    int dnorms_cap = 0;
+    std::vector<double> hnorms;
     explicit Batch(hipStream_t st) : s(st), b64(st), b32(st) {}
+    ~Batch() {
+        if (dnorms) (void)hipFreeAsync(dnorms, s);
+    }
 };
 
-static int mixed_start(Batch& B, BatchSystem& S);
+constexpr double kReplaceDrop = 0.1;  // replace when the residual estimate fell by this factor
+constexpr int kCheckEvery = 16;       // mixed systems pause every 16 iterations for that test
+constexpr int kMaxReplacements = 200;
 
 static int finish(Batch& B, BatchSystem& S, int succeeded, int iterations) {
     S.finished = 1;
@@ -571,45 +670,28 @@ static int finish(Batch& B, BatchSystem& S, int succeeded, int iterations) {
     return TG_OK;
 }
 
-// Computes the FP64 residual of a mixed system; decides done / next inner solve.
-static int mixed_residual(Batch& B, BatchSystem& S) {
+static int grid_of(int64_t c) { return (int)std::max<int64_t>(1, std::min<int64_t>((c + kBlock - 1) / kBlock, 8192)); }
+
+// Replaces a mixed system's recursive residual by the true one (launches only);
+// its true residual norms land in B.dnorms[slot * kBatchMaxK + c].
+static void mixed_replace(Batch& B, BatchSystem& S, int it) {
+    const SysDev<float>& E = B.b32.desc(S.slot);
     const int tiles = ceil_div(S.n, kTileRows), kpad = pow2_at_least(S.k);
-    hipLaunchKernelGGL(bm_residual_kernel, dim3(tiles), dim3(kBlock), 0, B.s, S.n, S.k, kpad, S.rp, S.ci, S.val64, S.b64,
-                       (const double*)S.x64, S.r32, S.partial, S.outer == 0 ? 1 : 0);
-    hipLaunchKernelGGL(bm_norm_total_kernel, dim3(1), dim3(kBlock), 0, B.s, tiles, S.k, kpad, (const double*)S.partial,
-                       S.norms + S.k);
-    TG_LAUNCH_CHECK();
-    std::vector<double> rn(S.k);
-    TG_HIP(hipMemcpyAsync(rn.data(), S.norms + S.k, S.k * 8, hipMemcpyDeviceToHost, B.s));
-    TG_HIP(hipStreamSynchronize(B.s));
-    double worst = 0.0;
-    for (int c = 0; c < S.k; ++c) {
-        S.rel[c] = S.bnorm[c] > 0 ? rn[c] / S.bnorm[c] : 0.0;
-        worst = std::max(worst, S.rel[c]);
-    }
-    if (worst <= S.tol || S.outer >= S.max_outer) {
-        S.state = 2;
-        return finish(B, S, worst <= S.tol ? 1 : 0, S.inner_total);
-    }
-    const double tin = std::min(0.5, std::max(S.inner_tol, 0.3 * S.tol / worst));
-    if (S.slot32 >= 0) {
-        int rc = B.b32.retire(S.slot32);
-        if (rc) return rc;
-        S.slot32 = -1;
-    }
-    int slot = -1;
-    int rc = B.b32.add(S.n, S.rp, S.ci, S.a32, S.d32, S.k, S.r32, S.dx32, tin, S.max_iter, &slot);
-    if (rc) return rc;
-    S.slot32 = slot;
-    S.state = 1;
-    return TG_OK;
+    hipLaunchKernelGGL(bm_replace_kernel, dim3(tiles), dim3(kBlock), 0, B.s, S.n, S.k, kpad, S.rp, S.ci, S.val64, S.b64,
+                       (const double*)S.x64, (const float*)S.x32, (const float*)S.d32, E.r, S.d32 ? E.z : nullptr,
+                       E.prz[it & 1], S.pnorm);
+    hipLaunchKernelGGL(bm_flush_kernel, dim3(grid_of((int64_t)S.n * S.k)), dim3(kBlock), 0, B.s, (int64_t)S.n * S.k,
+                       S.x32, S.x64);
+    hipLaunchKernelGGL(bm_norm_total_kernel, dim3(1), dim3(kBlock), 0, B.s, tiles, S.k, kpad, (const double*)S.pnorm,
+                       B.dnorms + (size_t)S.slot * kBatchMaxK);
+    ++S.replacements;
 }
 
 static int mixed_start(Batch& B, BatchSystem& S) {
     const size_t N = (size_t)S.n * S.k;
     const int tiles = ceil_div(S.n, kTileRows);
     const size_t bytes = align256((size_t)S.nnz * 4) + (S.d64 ? align256((size_t)S.n * 4) : 0) + 2 * align256(N * 4) +
-                         align256((size_t)tiles * S.k * 8) + align256((size_t)2 * S.k * 8);
+                         align256((size_t)tiles * S.k * 8) + align256((size_t)S.k * 8);
     TG_HIP(hipMallocAsync((void**)&S.mem, bytes, B.s));
     S.mem_bytes = bytes;
     char* p = S.mem;
@@ -619,36 +701,30 @@ static int mixed_start(Batch& B, BatchSystem& S) {
         S.d32 = (float*)p;
         p += align256((size_t)S.n * 4);
     }
-    S.r32 = (float*)p;
+    S.b32 = (float*)p;
     p += align256(N * 4);
-    S.dx32 = (float*)p;
+    S.x32 = (float*)p;
     p += align256(N * 4);
-    S.partial = (double*)p;
+    S.pnorm = (double*)p;
     p += align256((size_t)tiles * S.k * 8);
     S.norms = (double*)p;
-    auto grid = [](int64_t c) { return (int)std::max<int64_t>(1, std::min<int64_t>((c + kBlock - 1) / kBlock, 8192)); };
-    hipLaunchKernelGGL(bm_tofloat_kernel, dim3(grid(S.nnz)), dim3(kBlock), 0, B.s, S.nnz, S.val64, S.a32);
-    if (S.d64) hipLaunchKernelGGL(bm_tofloat_kernel, dim3(grid(S.n)), dim3(kBlock), 0, B.s, (int64_t)S.n, S.d64, S.d32);
-    // ||b||: the residual kernel with x = 0 gives it (first pass)
+    hipLaunchKernelGGL(bm_tofloat_kernel, dim3(grid_of(S.nnz)), dim3(kBlock), 0, B.s, S.nnz, S.val64, S.a32);
+    if (S.d64) hipLaunchKernelGGL(bm_tofloat_kernel, dim3(grid_of(S.n)), dim3(kBlock), 0, B.s, (int64_t)S.n, S.d64, S.d32);
+    hipLaunchKernelGGL(bm_tofloat_kernel, dim3(grid_of((int64_t)N)), dim3(kBlock), 0, B.s, (int64_t)N, S.b64, S.b32);
+    hipLaunchKernelGGL(bm_zero_kernel, dim3(grid_of((int64_t)N)), dim3(kBlock), 0, B.s, (int64_t)N, S.x64);
     const int kpad = pow2_at_least(S.k);
-    hipLaunchKernelGGL(bm_residual_kernel, dim3(tiles), dim3(kBlock), 0, B.s, S.n, S.k, kpad, S.rp, S.ci, S.val64, S.b64,
-                       (const double*)S.x64, S.r32, S.partial, 1);
-    hipLaunchKernelGGL(bm_norm_total_kernel, dim3(1), dim3(kBlock), 0, B.s, tiles, S.k, kpad, (const double*)S.partial,
+    hipLaunchKernelGGL(bm_bnorm_kernel, dim3(tiles), dim3(kBlock), 0, B.s, S.n, S.k, kpad, S.b64, S.pnorm);
+    hipLaunchKernelGGL(bm_norm_total_kernel, dim3(1), dim3(kBlock), 0, B.s, tiles, S.k, kpad, (const double*)S.pnorm,
                        S.norms);
     TG_LAUNCH_CHECK();
     S.bnorm.assign(S.k, 0.0);
     TG_HIP(hipMemcpyAsync(S.bnorm.data(), S.norms, S.k * 8, hipMemcpyDeviceToHost, B.s));
     TG_HIP(hipStreamSynchronize(B.s));
     for (int c = 0; c < S.k; ++c)
-        if (std::isnan(S.bnorm[c])) {
-            S.state = 2;
-            return finish(B, S, 0, 0);
-        }
-    // x = 0 (the caller's X is the FP64 iterate)
-    hipLaunchKernelGGL(bm_zero_kernel, dim3(grid((int64_t)N)), dim3(kBlock), 0, B.s, (int64_t)N, S.x64);
-    TG_LAUNCH_CHECK();
-    S.outer = 0;
-    return mixed_residual(B, S);
+        if (std::isnan(S.bnorm[c])) return finish(B, S, 0, 0);
+    // The engine's own test (preconditioned FP32 estimate <= tol) only pauses the
+    // system; the host then decides on the FP64 true residual.
+    return B.b32.add(S.n, S.rp, S.ci, S.a32, S.d32, S.k, S.b32, S.x32, S.tol, S.max_iter, &S.slot, kCheckEvery);
 }
 
 static int batch_advance(Batch& B, int iters) {
@@ -657,35 +733,75 @@ static int batch_advance(Batch& B, int iters) {
     rc = B.b32.step(iters);
     if (rc) return rc;
     std::vector<SysState> s64, s32;
+    std::vector<BatchSystem*> replaced;
     rc = B.b64.poll(s64);
     if (rc) return rc;
     rc = B.b32.poll(s32);
     if (rc) return rc;
+    if (B.dnorms_cap < B.b32.capacity()) {  // sized before any replacement of this poll writes it
+        if (B.dnorms) TG_HIP(hipFreeAsync(B.dnorms, B.s));
+        B.dnorms_cap = std::max(B.b32.capacity(), 2 * B.dnorms_cap);
+        TG_HIP(hipMallocAsync((void**)&B.dnorms, (size_t)B.dnorms_cap * kBatchMaxK * sizeof(double), B.s));
+    }
     for (auto& kv : B.sys) {
         BatchSystem& S = *kv.second;
-        if (S.finished) continue;
+        if (S.finished || S.slot < 0) continue;
         if (!S.mixed) {
-            const SysState& st = s64[S.slot64];
+            const SysState& st = s64[S.slot];
             if (st.done) {
                 for (int c = 0; c < S.k; ++c) S.rel[c] = st.bsq[c] > 0 ? std::sqrt(std::fabs(st.rho[c]) / st.bsq[c]) : 0.0;
-                B.b64.park(S.slot64);
+                B.b64.park(S.slot);
                 rc = finish(B, S, st.nan ? 0 : st.succ, st.iter);
                 if (rc) return rc;
             }
-        } else if (S.state == 1) {
-            const SysState& st = s32[S.slot32];
-            if (st.done) {
-                S.inner_total += st.iter;
-                const int64_t N = (int64_t)S.n * S.k;
-                hipLaunchKernelGGL(bm_axpy_kernel, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((N + 255) / 256, 8192))),
-                                   dim3(kBlock), 0, B.s, N, (const float*)S.dx32, S.x64);
-                TG_LAUNCH_CHECK();
-                B.b32.park(S.slot32);
-                ++S.outer;
-                rc = mixed_residual(B, S);
-                if (rc) return rc;
-            }
+            continue;
         }
+        const SysState& st = s32[S.slot];
+        if (st.nan) {
+            B.b32.park(S.slot);
+            rc = finish(B, S, 0, st.iter);
+            if (rc) return rc;
+            continue;
+        }
+        if (!st.done) continue;  // decisions only at the system's own checkpoints / stops
+        double est = 0.0;
+        for (int c = 0; c < S.k; ++c) est = std::max(est, st.bsq[c] > 0 ? std::sqrt(std::fabs(st.rho[c]) / st.bsq[c]) : 0.0);
+        if (st.paused && !(est < kReplaceDrop * S.est_last)) {
+            B.b32.resume(S.slot, 1.0);
+            continue;
+        }
+        mixed_replace(B, S, st.it_b);
+        S.est_last = est;
+        replaced.push_back(&S);
+    }
+    if (replaced.empty()) return TG_OK;
+    TG_LAUNCH_CHECK();
+    B.hnorms.resize((size_t)B.dnorms_cap * kBatchMaxK);
+    TG_HIP(hipMemcpyAsync(B.hnorms.data(), B.dnorms, B.hnorms.size() * sizeof(double), hipMemcpyDeviceToHost, B.s));
+    TG_HIP(hipStreamSynchronize(B.s));
+    for (BatchSystem* Sp : replaced) {
+        BatchSystem& S = *Sp;
+        const SysState& st = s32[S.slot];
+        double worst = 0.0;
+        for (int c = 0; c < S.k; ++c) {
+            const double rn = B.hnorms[(size_t)S.slot * kBatchMaxK + c];
+            S.rel[c] = S.bnorm[c] > 0 ? rn / S.bnorm[c] : 0.0;
+            worst = std::max(worst, S.rel[c]);
+        }
+        if (worst <= S.tol) {
+            B.b32.park(S.slot);
+            rc = finish(B, S, 1, st.it_b);
+        } else if (st.it_b >= S.max_iter || S.replacements >= kMaxReplacements || std::isnan(worst)) {
+            B.b32.park(S.slot);
+            rc = finish(B, S, 0, st.it_b);
+        } else if (!st.paused) {
+            // the FP32 estimate met the tolerance but the true residual did not: tighten it
+            const double f = S.tol / worst;
+            B.b32.resume(S.slot, std::min(0.25, std::max(1e-4, 0.25 * f * f)));
+        } else {
+            B.b32.resume(S.slot, 1.0);
+        }
+        if (rc) return rc;
     }
     return TG_OK;
 }
@@ -737,7 +853,7 @@ extern "C" int tg_batch_add_cg(tg_batch_t h, int32_t n, int64_t nnz, const int32
     BatchSystem& ref = *S;
     Bt.sys[S->id] = std::move(S);
     *id_out = ref.id;
-    if (!ref.mixed) return Bt.b64.add(n, rp, ci, val, diag, nrhs, B, X, tol, max_iter, &ref.slot64);
+    if (!ref.mixed) return Bt.b64.add(n, rp, ci, val, diag, nrhs, B, X, tol, max_iter, &ref.slot);
     return mixed_start(Bt, ref);
 }
 
@@ -765,7 +881,7 @@ extern "C" int tg_batch_result(tg_batch_t h, int64_t id, int32_t* finished, int3
     if (finished) *finished = S.finished;
     if (succeeded) *succeeded = S.succeeded;
     if (iterations) *iterations = S.iterations;
-    if (outer) *outer = S.outer;
+    if (outer) *outer = S.replacements;
     if (rel_residuals)
         for (int c = 0; c < S.k; ++c) rel_residuals[c] = S.rel[c];
     return TG_OK;
@@ -779,12 +895,8 @@ extern "C" int tg_batch_release(tg_batch_t h, int64_t id) {
     if (it == Bt.sys.end()) return fail(TG_ERR_INVALID, "tg_batch_release: unknown id");
     BatchSystem& S = *it->second;
     if (!S.finished) return fail(TG_ERR_INVALID, "tg_batch_release: system %lld still running", (long long)id);
-    if (S.slot64 >= 0) {
-        int rc = Bt.b64.retire(S.slot64);
-        if (rc) return rc;
-    }
-    if (S.slot32 >= 0) {
-        int rc = Bt.b32.retire(S.slot32);
+    if (S.slot >= 0) {
+        int rc = S.mixed ? Bt.b32.retire(S.slot) : Bt.b64.retire(S.slot);
         if (rc) return rc;
     }
     if (S.mem) TG_HIP(hipFreeAsync(S.mem, Bt.s));
