@@ -283,8 +283,21 @@ def batch_section(args, dev):
         lat = np.array(sorted(done_t.values()))
         tr = np.array([true_res(systems[i], Xs[i]) for sid, i in ids.items()])
         iters = np.array([res[sid]["iterations"] for sid in ids])
+        # algorithmic bytes: per iteration nnz*(s_v + 4) + 4(n+1) + 2*n*s_v (Jacobi diagonal,
+        # read by both kernels) + 10*n*k*s (K1: r, p_old read, p, u written; K2: p, u, x, r
+        # read, x, r written); a residual replacement adds an FP64 SpMV over x64 + x32 plus
+        # b read, r written and the x fold
+        sv = 4 if mixed else 8
+        alg = 0.0
+        for sid, i in ids.items():
+            s_ = systems[i]
+            n_, k_, z_ = s_["n"], s_["k"], s_["nnz"]
+            alg += res[sid]["iterations"] * (z_ * (sv + 4) + 4 * (n_ + 1) + 2 * n_ * sv + 10 * n_ * k_ * sv)
+            if mixed:
+                alg += res[sid]["outer"] * (z_ * 12 + 4 * (n_ + 1) + n_ * k_ * (8 + 12 + 8 + 4 + 12))
         return Xs, {
             "solves_per_s": round(len(systems) / wall, 2), "wall_s": round(wall, 3),
+            "alg_GB": round(alg / 1e9, 1), "alg_GBps": round(alg / wall / 1e9, 1),
             "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 1),
             "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 1),
             "succeeded": int(sum(r["succeeded"] for r in res.values())),

@@ -48,8 +48,7 @@ template <typename T> struct SysDev {
     T* x;
     T* r;
     T* u;
-    T* p[2];         // direction, by iteration parity
-    T* z;            // == r without a preconditioner
+    T* p[2];         // direction, by iteration parity (z = d * r is never stored)
     double* ppu;     // tiles * k partials of p.u
     double* prz[2];  // tiles * k partials of r.z, by parity
     int n, k, kpad, tiles;
@@ -115,11 +114,7 @@ __global__ __launch_bounds__(kBlock) void bk_init_kernel(const Tile* __restrict_
             const int64_t e = (int64_t)i * S.k + c;
             const T rv = S.b[e];
             S.r[e] = rv;
-            T zv = rv;
-            if (S.d) {
-                zv = S.d[i] * rv;
-                S.z[e] = zv;
-            }
+            const T zv = S.d ? S.d[i] * rv : rv;
             S.x[e] = T(0);
             acc += (double)rv * (double)zv;
         }
@@ -197,16 +192,20 @@ __global__ __launch_bounds__(kBlock) void bk_dir_spmm_kernel(const Tile* __restr
         const T be = (T)(rz / rz_prev);
         const T* __restrict__ pold = S.p[(it + 1) & 1];
         T* __restrict__ pnew = S.p[it & 1];
-        const T* __restrict__ z = S.z;
+        const T* __restrict__ r = S.r;
+        const T* __restrict__ d = S.d;
         for (int i = tl.r0 + ro; i < tl.r1; i += rows) {
             T s = T(0);
             for (int l = S.rp[i]; l < S.rp[i + 1]; ++l) {
-                const int64_t f = (int64_t)S.ci[l] * S.k + c;
-                const T pj = first ? z[f] : fma_t(be, pold[f], z[f]);
+                const int32_t j = S.ci[l];
+                const int64_t f = (int64_t)j * S.k + c;
+                const T zj = d ? d[j] * r[f] : r[f];
+                const T pj = first ? zj : fma_t(be, pold[f], zj);
                 s += S.val[l] * pj;
             }
             const int64_t e = (int64_t)i * S.k + c;
-            const T pi = first ? z[e] : fma_t(be, pold[e], z[e]);
+            const T zi = d ? d[i] * r[e] : r[e];
+            const T pi = first ? zi : fma_t(be, pold[e], zi);
             pnew[e] = pi;
             S.u[e] = s;
             acc += (double)pi * (double)s;
@@ -236,11 +235,7 @@ __global__ __launch_bounds__(kBlock) void bk_update_kernel(const Tile* __restric
             S.x[e] += al * p[e];
             const T rv = S.r[e] - al * S.u[e];
             S.r[e] = rv;
-            T zv = rv;
-            if (S.d) {
-                zv = S.d[i] * rv;
-                S.z[e] = zv;
-            }
+            const T zv = S.d ? S.d[i] * rv : rv;
             acc += (double)rv * (double)zv;
         }
     }
@@ -308,7 +303,7 @@ template <typename T> class CGBatch {
         S.kpad = pow2_at_least(k);
         S.tiles = ceil_div(n, kTileRows);
         const size_t N = (size_t)n * k;
-        const size_t nvec = d ? 5 : 4;  // r, u, p0, p1 (+ z)
+        const size_t nvec = 4;  // r, u, p0, p1
         const size_t vb = align256(N * sizeof(T)), pb = align256((size_t)S.tiles * k * sizeof(double));
         const size_t bytes = nvec * vb + 3 * pb;
         char* mem = nullptr;
@@ -317,7 +312,6 @@ template <typename T> class CGBatch {
         S.u = (T*)(mem + vb);
         S.p[0] = (T*)(mem + 2 * vb);
         S.p[1] = (T*)(mem + 3 * vb);
-        S.z = d ? (T*)(mem + 4 * vb) : S.r;
         S.ppu = (double*)(mem + nvec * vb);
         S.prz[0] = (double*)(mem + nvec * vb + pb);
         S.prz[1] = (double*)(mem + nvec * vb + 2 * pb);
@@ -530,19 +524,19 @@ template <typename T> class CGBatch {
 // replacement strategies for Krylov subspace iterative methods for the
 // convergence of true residuals", SIAM J. Sci. Comput. 22 (2000); the
 // mixed-precision form of Clark et al., Comput. Phys. Commun. 181 (2010)):
-//   r = b - A64 (x64 + x32) in FP64, r32 = float(r), z32 = P r32, new r.z
+//   r = b - A64 (x64 + x32) in FP64, r32 = float(r), new r.z
 //   partials; then x64 += x32, x32 = 0.
 // The search direction p is kept, so CG continues instead of restarting.
 
-// r = b - A (x64 + x32) on the engine's tile; writes r32, z32 (if P), r.z partial
+// r = b - A (x64 + x32) on the engine's tile; writes r32, the r.z partial
 // (the engine's parity slot) and ||r||^2 partial.  x64 and x32 are only read.
 __global__ __launch_bounds__(kBlock) void bm_replace_kernel(int n, int k, int kpad, const int32_t* __restrict__ rp,
                                                             const int32_t* __restrict__ ci,
                                                             const double* __restrict__ val,
                                                             const double* __restrict__ b, const double* __restrict__ x64,
                                                             const float* __restrict__ x32, const float* __restrict__ d32,
-                                                            float* __restrict__ r32, float* __restrict__ z32,
-                                                            double* __restrict__ prz, double* __restrict__ pnorm) {
+                                                            float* __restrict__ r32, double* __restrict__ prz,
+                                                            double* __restrict__ pnorm) {
     const int r0 = blockIdx.x * kTileRows, r1 = min(n, r0 + kTileRows);
     const int c = threadIdx.x % kpad, ro = threadIdx.x / kpad, rows = kBlock / kpad;
     double acc_rz = 0.0, acc_n = 0.0;
@@ -557,11 +551,7 @@ __global__ __launch_bounds__(kBlock) void bm_replace_kernel(int n, int k, int kp
             const double r = b[e] - ax;
             const float rf = (float)r;
             r32[e] = rf;
-            float zf = rf;
-            if (d32) {
-                zf = d32[i] * rf;
-                z32[e] = zf;
-            }
+            const float zf = d32 ? d32[i] * rf : rf;
             acc_rz += (double)rf * (double)zf;
             acc_n += r * r;
         }
@@ -678,8 +668,7 @@ static void mixed_replace(Batch& B, BatchSystem& S, int it) {
     const SysDev<float>& E = B.b32.desc(S.slot);
     const int tiles = ceil_div(S.n, kTileRows), kpad = pow2_at_least(S.k);
     hipLaunchKernelGGL(bm_replace_kernel, dim3(tiles), dim3(kBlock), 0, B.s, S.n, S.k, kpad, S.rp, S.ci, S.val64, S.b64,
-                       (const double*)S.x64, (const float*)S.x32, (const float*)S.d32, E.r, S.d32 ? E.z : nullptr,
-                       E.prz[it & 1], S.pnorm);
+                       (const double*)S.x64, (const float*)S.x32, (const float*)S.d32, E.r, E.prz[it & 1], S.pnorm);
     hipLaunchKernelGGL(bm_flush_kernel, dim3(grid_of((int64_t)S.n * S.k)), dim3(kBlock), 0, B.s, (int64_t)S.n * S.k,
                        S.x32, S.x64);
     hipLaunchKernelGGL(bm_norm_total_kernel, dim3(1), dim3(kBlock), 0, B.s, tiles, S.k, kpad, (const double*)S.pnorm,
@@ -910,7 +899,7 @@ extern "C" size_t tg_batch_request_bytes(int32_t n, int64_t nnz, int32_t nrhs, i
     if (n < 1 || nrhs < 1 || nrhs > kBatchMaxK || nnz < 0) return 0;
     const size_t N = (size_t)n * nrhs, tiles = (size_t)ceil_div(n, kTileRows);
     const size_t pb = align256(tiles * nrhs * sizeof(double));
-    const size_t nvec = has_diag ? 5 : 4;
+    const size_t nvec = 4;
     if (!mixed) return nvec * align256(N * sizeof(double)) + 3 * pb;
     return nvec * align256(N * sizeof(float)) + 3 * pb + align256((size_t)nnz * 4) + (has_diag ? align256((size_t)n * 4) : 0) +
            2 * align256(N * 4) + pb + align256((size_t)2 * nrhs * 8);
