@@ -38,6 +38,8 @@ namespace tg {
 
 constexpr int kBatchMaxK = 8;
 constexpr int kTileRows = 2048;
+constexpr int kGather = 8;     // nonzeros per gather chunk in the SpMM kernel
+constexpr int kRowUnroll = 4;  // rows per step in the update kernel
 
 template <typename T> struct SysDev {
     const int32_t* rp;
@@ -67,6 +69,12 @@ struct SysState {
 struct Tile {
     int slot, r0, r1, t;
 };
+
+// Parity selection without indexing a local array (which would live in scratch).
+template <typename P> __host__ __device__ __forceinline__ P pick(P const (&a)[2], int it) {
+    const P a0 = a[0], a1 = a[1];
+    return (it & 1) ? a1 : a0;
+}
 
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
@@ -106,7 +114,7 @@ __device__ __forceinline__ double sys_total(const double* partial, int tiles, in
 template <typename T>
 __global__ __launch_bounds__(kBlock) void bk_init_kernel(const Tile* __restrict__ tiles, const SysDev<T>* __restrict__ sys) {
     const Tile tl = tiles[blockIdx.x];
-    const SysDev<T> S = sys[tl.slot];
+    const SysDev<T>& S = sys[tl.slot];
     const int c = threadIdx.x % S.kpad, ro = threadIdx.x / S.kpad, rows = kBlock / S.kpad;
     double acc = 0.0;
     if (c < S.k)
@@ -127,7 +135,7 @@ __global__ __launch_bounds__(kBlock) void bk_setup_kernel(const int* __restrict_
                                                           SysState* __restrict__ st, const double* __restrict__ tol2) {
     const int slot = *slotp;
     SysState& ss = st[slot];
-    const SysDev<T> S = sys[slot];
+    const SysDev<T>& S = sys[slot];
     const int c = threadIdx.x % S.kpad, j = threadIdx.x / S.kpad, J = kBlock / S.kpad;
     const double tot = sys_total(S.prz[0], S.tiles, S.k, S.kpad, c, j, J);
     __shared__ int flag, nanf;
@@ -162,11 +170,11 @@ __global__ __launch_bounds__(kBlock) void bk_dir_spmm_kernel(const Tile* __restr
                                                              SysState* __restrict__ st) {
     const Tile tl = tiles[blockIdx.x];
     SysState& ss = st[tl.slot];
-    const SysDev<T> S = sys[tl.slot];
+    const SysDev<T>& S = sys[tl.slot];
     const int it = ss.it_b;
     const int c = threadIdx.x % S.kpad, ro = threadIdx.x / S.kpad, rows = kBlock / S.kpad;
-    const double rz = sys_total(S.prz[it & 1], S.tiles, S.k, S.kpad, c, ro, rows);
-    const double rz_prev = it > 0 ? sys_total(S.prz[(it + 1) & 1], S.tiles, S.k, S.kpad, c, ro, rows) : 1.0;
+    const double rz = sys_total(pick(S.prz, it), S.tiles, S.k, S.kpad, c, ro, rows);
+    const double rz_prev = it > 0 ? sys_total(pick(S.prz, it + 1), S.tiles, S.k, S.kpad, c, ro, rows) : 1.0;
     __shared__ int conv;
     if (threadIdx.x == 0) conv = 1;
     __syncthreads();
@@ -190,20 +198,40 @@ __global__ __launch_bounds__(kBlock) void bk_dir_spmm_kernel(const Tile* __restr
     if (c < S.k) {
         const bool first = it == 0;
         const T be = (T)(rz / rz_prev);
-        const T* __restrict__ pold = S.p[(it + 1) & 1];
-        T* __restrict__ pnew = S.p[it & 1];
+        const T* __restrict__ pold = pick(S.p, it + 1);
+        T* __restrict__ pnew = pick(S.p, it);
         const T* __restrict__ r = S.r;
         const T* __restrict__ d = S.d;
+        const int32_t* __restrict__ rp = S.rp;
+        const int32_t* __restrict__ ci = S.ci;
+        const T* __restrict__ val = S.val;
+        const int k = S.k;
         for (int i = tl.r0 + ro; i < tl.r1; i += rows) {
+            // A row's nonzeros in chunks of kGather: all index/value loads of a chunk,
+            // then all its gathers, are issued before any is used (memory-level
+            // parallelism); the sum still runs in nonzero order.
+            const int lb = rp[i], le = rp[i + 1];
             T s = T(0);
-            for (int l = S.rp[i]; l < S.rp[i + 1]; ++l) {
-                const int32_t j = S.ci[l];
-                const int64_t f = (int64_t)j * S.k + c;
-                const T zj = d ? d[j] * r[f] : r[f];
-                const T pj = first ? zj : fma_t(be, pold[f], zj);
-                s += S.val[l] * pj;
+            for (int l0 = lb; l0 < le; l0 += kGather) {
+                int32_t jj[kGather];
+                T vv[kGather], pj[kGather];
+#pragma unroll
+                for (int q = 0; q < kGather; ++q) {
+                    const bool ok = l0 + q < le;
+                    jj[q] = ok ? ci[l0 + q] : i;
+                    vv[q] = ok ? val[l0 + q] : T(0);
+                }
+#pragma unroll
+                for (int q = 0; q < kGather; ++q) {
+                    const int64_t f = (int64_t)jj[q] * k + c;
+                    const T zj = d ? d[jj[q]] * r[f] : r[f];
+                    pj[q] = first ? zj : fma_t(be, pold[f], zj);
+                }
+#pragma unroll
+                for (int q = 0; q < kGather; ++q)
+                    if (l0 + q < le) s += vv[q] * pj[q];
             }
-            const int64_t e = (int64_t)i * S.k + c;
+            const int64_t e = (int64_t)i * k + c;
             const T zi = d ? d[i] * r[e] : r[e];
             const T pi = first ? zi : fma_t(be, pold[e], zi);
             pnew[e] = pi;
@@ -221,25 +249,50 @@ __global__ __launch_bounds__(kBlock) void bk_update_kernel(const Tile* __restric
     const Tile tl = tiles[blockIdx.x];
     SysState& ss = st[tl.slot];
     if (ss.done) return;
-    const SysDev<T> S = sys[tl.slot];
+    const SysDev<T>& S = sys[tl.slot];
     const int it = ss.it_a;
     const int c = threadIdx.x % S.kpad, ro = threadIdx.x / S.kpad, rows = kBlock / S.kpad;
-    const double rz = sys_total(S.prz[it & 1], S.tiles, S.k, S.kpad, c, ro, rows);
+    const double rz = sys_total(pick(S.prz, it), S.tiles, S.k, S.kpad, c, ro, rows);
     const double pu = sys_total(S.ppu, S.tiles, S.k, S.kpad, c, ro, rows);
     double acc = 0.0;
     if (c < S.k) {
         const T al = (T)(rz / pu);
-        const T* __restrict__ p = S.p[it & 1];
-        for (int i = tl.r0 + ro; i < tl.r1; i += rows) {
-            const int64_t e = (int64_t)i * S.k + c;
-            S.x[e] += al * p[e];
-            const T rv = S.r[e] - al * S.u[e];
-            S.r[e] = rv;
-            const T zv = S.d ? S.d[i] * rv : rv;
-            acc += (double)rv * (double)zv;
+        const T* __restrict__ p = pick(S.p, it);
+        const T* __restrict__ u = S.u;
+        const T* __restrict__ d = S.d;
+        T* __restrict__ x = S.x;
+        T* __restrict__ r = S.r;
+        const int k = S.k;
+        // kRowUnroll rows per step: all loads first, then the stores (the arrays are
+        // distinct, so nothing a step stores is read by it)
+        for (int i0 = tl.r0 + ro; i0 < tl.r1; i0 += kRowUnroll * rows) {
+            T pv[kRowUnroll], xv[kRowUnroll], rv[kRowUnroll], uv[kRowUnroll], dv[kRowUnroll];
+#pragma unroll
+            for (int q = 0; q < kRowUnroll; ++q) {
+                const int i = i0 + q * rows;
+                const bool ok = i < tl.r1;
+                const int64_t e = (int64_t)(ok ? i : i0) * k + c;
+                pv[q] = p[e];
+                xv[q] = x[e];
+                rv[q] = r[e];
+                uv[q] = u[e];
+                dv[q] = d ? d[ok ? i : i0] : T(1);
+            }
+#pragma unroll
+            for (int q = 0; q < kRowUnroll; ++q) {
+                const int i = i0 + q * rows;
+                if (i < tl.r1) {
+                    const int64_t e = (int64_t)i * k + c;
+                    x[e] = xv[q] + al * pv[q];
+                    const T rn = rv[q] - al * uv[q];
+                    r[e] = rn;
+                    const T zv = d ? dv[q] * rn : rn;
+                    acc += (double)rn * (double)zv;
+                }
+            }
         }
     }
-    tile_partial(acc, S.k, S.kpad, S.prz[(it + 1) & 1], tl.t);
+    tile_partial(acc, S.k, S.kpad, pick(S.prz, it + 1), tl.t);
     if (tl.t == 0 && threadIdx.x == 0) ss.it_b = it + 1;
 }
 
