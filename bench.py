@@ -468,7 +468,7 @@ def main():
 
     from tensors_amd import _capi as capi
     from tensors_amd import generators as gen
-    from tensors_amd.sparse import spmm_raw
+    from tensors_amd.sparse import SlicePlan
 
     dev = torch.device("cuda:0")
     rp, ci, v64, v32 = gen.random_csr(M, N, 32.0, 1, 512, SEED, want64=True, want32=True)
@@ -490,9 +490,13 @@ def main():
             cases.append({"dtype": "f64" if sv == 8 else "f32", "k": k, "X": X, "Y": Y, "A": vals[dtype],
                           "bytes": alg_bytes(nnz, M, N, k, sv)})
     stream = torch.cuda.current_stream()
+    # the path MatrixCSR.Dot takes: tg_csr_spmm_plan (column slices where X is a few L2s)
+    plan = SlicePlan(M, N, nnz, rp32, cid)
+    for c in cases:
+        c["slices"] = plan.slices(c["Y"].dtype, c["k"])
 
     def launch(c):
-        return spmm_raw(M, N, nnz, rp32, cid, c["A"], 1.0, c["X"], c["k"], 0.0, c["Y"], c["k"], c["k"])
+        return plan.spmm(c["A"], 1.0, c["X"], c["k"], 0.0, c["Y"], c["k"], c["k"])
 
     log(f"SpMM suite: nnz={nnz}, {len(cases)} launches per step")
     for _ in range(args.warmup):
@@ -544,7 +548,7 @@ def main():
     dc = cases[dom]
     achieved = dc["bytes"] / (per_case_ms[dom] * 1e-3) / 1e9
     traffic = read_profile_traffic()
-    breakdown = [{"dtype": c["dtype"], "nrhs": c["k"], "ms": round(t, 4),
+    breakdown = [{"dtype": c["dtype"], "nrhs": c["k"], "slices": c["slices"], "ms": round(t, 4),
                   "GBps": round(c["bytes"] / (t * 1e-3) / 1e9, 1),
                   "frac_peak": round(c["bytes"] / (t * 1e-3) / 1e9 / PEAK_HBM_GBPS, 4),
                   "frac_measured_read": round(c["bytes"] / (t * 1e-3) / 1e9 / read_gbps, 4)}
@@ -563,7 +567,8 @@ def main():
         "dtype": "f64+f32",
         "data": "synthetic",
         "config": {"workload": "random CSR SpMM suite: n=1e6, Poisson(32) rows clipped [1,512], "
-                               "FP64 and FP32, nrhs 1/4/16/64, alpha=1 beta=0, int32 indices",
+                               "FP64 and FP32, nrhs 1/4/16/64, alpha=1 beta=0, int32 indices, MatrixCSR.Dot path "
+                               "(tg_csr_spmm_plan)",
                    "nnz": nnz, "nrhs": list(KS)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": traffic,
@@ -573,7 +578,7 @@ def main():
         "breakdown": breakdown,
     }
     # free the SpMM suite's buffers before the solver section
-    del cases, vals, Xs, rp32, cid
+    del cases, vals, Xs, rp32, cid, plan
     torch.cuda.empty_cache()
     if not args.no_cg:
         log("CG section")

@@ -158,6 +158,7 @@ template <typename Scal_, typename Int_, typename LInt_, Parallel_T parQ_ = Para
     static constexpr bool kRp64 = sizeof(LInt) > 4;
     void RequireDevice() const {
         if (dev_valid) return;
+        plan.Reset();  // it points at the device arrays about to be replaced
         const int64_t nnz = (int64_t)NonzeroCount();
         if (kRp64) {
             std::vector<int64_t> r(outer.Size());
@@ -173,6 +174,16 @@ template <typename Scal_, typename Int_, typename LInt_, Parallel_T parQ_ = Para
         d_ci.Upload(c.data(), c.size());
         d_val.Upload(values.data(), (Size_T)std::min<int64_t>(nnz, (int64_t)values.Size()));
         dev_valid = true;
+    }
+    // Column-slice plan of the device pattern (tg_csr_plan_create), built on first
+    // use like the reference's lazily built job pointers (PatternCSR.hpp:389-398).
+    tg_csr_plan_t DevicePlan() const {
+        RequireDevice();
+        if (!plan.p)
+            Device::Status(tg_csr_plan_create((int32_t)m, (int32_t)n, (int64_t)NonzeroCount(), DeviceRowPointers(),
+                                              DeviceRowPointerType(), d_ci.data(), &plan.p, nullptr),
+                           MethodName("RequirePlan"));
+        return plan.p;
     }
     const void* DeviceRowPointers() const { RequireDevice(); return kRp64 ? (const void*)d_rp64.data() : (const void*)d_rp32.data(); }
     tg_dtype DeviceRowPointerType() const { return kRp64 ? TG_I64 : TG_I32; }
@@ -306,10 +317,8 @@ template <typename Scal_, typename Int_, typename LInt_, Parallel_T parQ_ = Para
         const bool xd = X == nullptr || Device::IsDevicePointer(X), yd = Device::IsDevicePointer(Y);
         const std::string where = MethodName("Dot");
         if (xd && yd) {
-            Device::Status(tg_csr_spmm((int32_t)m, (int32_t)n, nnz, DeviceRowPointers(), DeviceRowPointerType(),
-                                       d_ci.data(), vals, Device::DType<Scal>(), (double)alpha, X, (int64_t)ldX,
-                                       (double)beta, Y, (int64_t)ldY, (int32_t)nrhs, (int32_t)NRHS, nullptr, 0,
-                                       nullptr),
+            Device::Status(tg_csr_spmm_plan(DevicePlan(), vals, Device::DType<Scal>(), (double)alpha, X, (int64_t)ldX,
+                                            (double)beta, Y, (int64_t)ldY, (int32_t)nrhs, (int32_t)NRHS, nullptr),
                            where);
             Device::Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
             return;
@@ -333,9 +342,9 @@ template <typename Scal_, typename Int_, typename LInt_, Parallel_T parQ_ = Para
                 Device::Copy2D(Yd, (Size_T)nrhs, Y, (Size_T)ldY, (Size_T)m, (Size_T)nrhs, hipMemcpyHostToDevice);
         }
         const int st = Device::Status(
-            tg_csr_spmm((int32_t)m, (int32_t)n, nnz, DeviceRowPointers(), DeviceRowPointerType(), d_ci.data(), vals,
-                        Device::DType<Scal>(), (double)alpha, Xd, xd ? (int64_t)ldX : (int64_t)nrhs, (double)beta, Yd,
-                        yd ? (int64_t)ldY : (int64_t)nrhs, (int32_t)nrhs, (int32_t)NRHS, nullptr, 0, nullptr),
+            tg_csr_spmm_plan(DevicePlan(), vals, Device::DType<Scal>(), (double)alpha, Xd,
+                             xd ? (int64_t)ldX : (int64_t)nrhs, (double)beta, Yd, yd ? (int64_t)ldY : (int64_t)nrhs,
+                             (int32_t)nrhs, (int32_t)NRHS, nullptr),
             where);
         if (st == TG_OK && !yd)
             Device::Copy2D(Y, (Size_T)ldY, Yd, (Size_T)nrhs, (Size_T)m, (Size_T)nrhs, hipMemcpyDeviceToHost);
@@ -404,7 +413,34 @@ template <typename Scal_, typename Int_, typename LInt_, Parallel_T parQ_ = Para
     Int n = 0;
     Int thread_count = 1;
 
+    // Owns a tg_csr_plan_t; a copied matrix starts without one (the plan points at
+    // the device arrays of the matrix that built it), a moved one takes it along.
+    struct PlanHolder {
+        mutable tg_csr_plan_t p = nullptr;
+        PlanHolder() = default;
+        PlanHolder(const PlanHolder&) {}
+        PlanHolder(PlanHolder&& o) noexcept : p(o.p) { o.p = nullptr; }
+        PlanHolder& operator=(const PlanHolder&) {
+            Reset();
+            return *this;
+        }
+        PlanHolder& operator=(PlanHolder&& o) noexcept {
+            if (this != &o) {
+                Reset();
+                p = o.p;
+                o.p = nullptr;
+            }
+            return *this;
+        }
+        ~PlanHolder() { Reset(); }
+        void Reset() const {
+            if (p) tg_csr_plan_destroy(p);
+            p = nullptr;
+        }
+    };
+
     mutable bool dev_valid = false;
+    mutable PlanHolder plan;
     mutable Device::Buffer<int64_t> d_rp64;
     mutable Device::Buffer<int32_t> d_rp32;
     mutable Device::Buffer<int32_t> d_ci;

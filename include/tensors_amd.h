@@ -89,6 +89,24 @@ int tg_csr_spmm(int32_t m, int32_t n, int64_t nnz, const void* rp, tg_dtype rp_t
                 int64_t ldY, int32_t nrhs, int32_t NRHS, const int32_t* job_ptr, int32_t blocks,
                 void* stream);
 
+/* Column-sliced execution plan of one CSR pattern (no reference counterpart;
+ * cached per matrix like PatternCSR::RequireJobPtr, PatternCSR.hpp:389-398).
+ * For dense blocks of 6-32 MB (nrhs <= 4) the columns are cut into 2-8 ranges,
+ * one per group of XCDs, so every XCD gathers from a slice of X that stays in
+ * its L2; the slices' partial sums are added in slice order.  The plan keeps
+ * the rp/ci pointers (they must stay valid) and 7 int32 per row; it needs
+ * sorted columns within rows (checked when built, else the plain kernels
+ * run).  tg_csr_spmm_plan has tg_csr_spmm's semantics and falls back to it
+ * whenever slicing does not pay; calls on one plan must not overlap.
+ * tg_csr_plan_slices returns the slice count it would use (0 = plain). */
+typedef struct tg_csr_plan_s* tg_csr_plan_t;
+int tg_csr_plan_create(int32_t m, int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, const int32_t* ci,
+                       tg_csr_plan_t* out, void* stream);
+int tg_csr_plan_destroy(tg_csr_plan_t plan);
+int32_t tg_csr_plan_slices(tg_csr_plan_t plan, tg_dtype val_type, int32_t nrhs);
+int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, double alpha, const void* X, int64_t ldX,
+                     double beta, void* Y, int64_t ldY, int32_t nrhs, int32_t NRHS, void* stream);
+
 /* Kernel-shape knobs read by tg_csr_spmm (no reference counterpart; the
  * reference picks its kernel by template arguments).  Keys:
  *   "spmv_kernel": nrhs == 1 kernel, 0 = streaming (default), 1 = team-per-row.
@@ -97,6 +115,8 @@ int tg_csr_spmm(int32_t m, int32_t n, int64_t nnz, const void* rp, tg_dtype rp_t
  *   "spmm_group":  lanes per RHS tile of the nrhs >= 2 kernel (a wave's 64/G lane
  *                  groups split each row's nonzeros), 0 = enough lanes for all
  *                  columns, else a power of two in [1, 64].
+ *   "spmm_slices": column slices of tg_csr_spmm_plan, 0 = from the size of X,
+ *                  1 = never, 2 / 4 / 8 = always that many (nrhs <= 4).
  * Process-wide; not thread-safe against concurrent SpMM calls. */
 int tg_tune_set(const char* key, int32_t value);
 int32_t tg_tune_get(const char* key);

@@ -32,6 +32,10 @@ SIGNATURES = {
     "tg_csr_default_blocks": (i32, [i32, i64, i32]),
     "tg_csr_spmm": (C.c_int, [i32, i32, i64, vp, C.c_int, vp, vp, C.c_int, dbl, vp, i64, dbl, vp, i64,
                               i32, i32, vp, i32, vp]),
+    "tg_csr_plan_create": (C.c_int, [i32, i32, i64, vp, C.c_int, vp, vp, vp]),
+    "tg_csr_plan_destroy": (C.c_int, [vp]),
+    "tg_csr_plan_slices": (i32, [vp, C.c_int, i32]),
+    "tg_csr_spmm_plan": (C.c_int, [vp, vp, C.c_int, dbl, vp, i64, dbl, vp, i64, i32, i32, vp]),
     "tg_tune_set": (C.c_int, [C.c_char_p, i32]),
     "tg_tune_get": (i32, [C.c_char_p]),
     "tg_axpby": (C.c_int, [i32, i32, C.c_int, dbl, vp, i64, dbl, vp, i64, vp]),

@@ -369,6 +369,7 @@ constexpr int kRowsPerTeam = 8;
 // lanes per row of the nrhs == 1 kernels, and lanes per RHS tile of spmm_wave.
 static int g_spmv_team = 0;
 static int g_spmm_group = 0;
+static int g_spmm_slices = 0;  // planned SpMM: 0 = by X size, 1 = never sliced, 2/4/8 = that many slices
 
 static int spmv_team_width(int32_t m, int64_t nnz) {
     if (g_spmv_team) return g_spmv_team;
@@ -536,6 +537,167 @@ static int axpby_typed(int32_t m, int32_t nrhs, double alpha_d, const T* X, int6
     return TG_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// Column-sliced SpMM for dense blocks a few L2s large (nrhs <= 4).
+//
+// Each XCD has its own 4 MB L2, and a random-column gather from an X of 8-32 MB
+// misses it almost every time (a 128-byte line fill per nonzero).  Here the
+// columns are cut into S equal ranges (S = 2, 4, 8), and the workgroups that
+// the hardware places on XCD x (blockIdx % 8 == x) only gather from range
+// x % S, so each XCD's L2 holds its slice of X.  Columns are sorted within a
+// row, so a row's nonzeros of one range form one run; the plan stores the run
+// boundaries of the 8-way cut (7 int32 per row; S = 4 and 2 use every second
+// / fourth).  The slices' partial sums go to S partial blocks that a second
+// kernel adds in slice order, with the alpha/beta epilogue.  Extra traffic:
+// S partial blocks written and read, 7 cut arrays read.
+// ---------------------------------------------------------------------------
+constexpr int kCuts = kXcds - 1;
+
+template <typename RP>
+__global__ __launch_bounds__(kBlock) void slice_plan_kernel(int m, int n, const RP* __restrict__ rp,
+                                                            const int32_t* __restrict__ ci,
+                                                            int32_t* __restrict__ cuts, int* __restrict__ unsorted) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < m; i += gridDim.x * kBlock) {
+        const int64_t lb = rp[i], le = rp[i + 1];
+        bool bad = false;
+        for (int64_t l = lb + 1; l < le; ++l) bad |= ci[l] < ci[l - 1];
+        if (bad) *unsorted = 1;
+        for (int s = 1; s <= kCuts; ++s) {
+            const int32_t bound = (int32_t)((int64_t)n * s / kXcds);
+            int64_t lo = lb, hi = le;  // first l with ci[l] >= bound
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) >> 1;
+                if (ci[mid] < bound)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            cuts[(int64_t)(s - 1) * m + i] = (int32_t)(lo - lb);  // offset within the row
+        }
+    }
+}
+
+template <typename T, typename RP, int TEAM, int K>
+__global__ __launch_bounds__(kBlock) void spmm_slice_kernel(int m, int S, const RP* __restrict__ rp,
+                                                            const int32_t* __restrict__ cuts,
+                                                            const int32_t* __restrict__ ci, const T* __restrict__ a,
+                                                            const T* __restrict__ X, int64_t ldX, T* __restrict__ part,
+                                                            int rows_per_block) {
+    constexpr int TEAMS = kBlock / TEAM;
+    const int b = blockIdx.x, xcd = b % kXcds;
+    const int s = xcd % S, per = kXcds / S;
+    const int rb = (b / kXcds) * per + xcd / S;
+    const int r0 = rb * rows_per_block, r1 = min(m, r0 + rows_per_block);
+    const int step = kXcds / S;  // cut index stride of this S
+    const int lt = threadIdx.x % TEAM, team = threadIdx.x / TEAM;
+    for (int row = r0 + team; row < r1; row += TEAMS) {
+        const RP base = rp[row];
+        const RP lb = s == 0 ? base : base + cuts[(int64_t)(s * step - 1) * m + row];
+        const RP le = s == S - 1 ? rp[row + 1] : base + cuts[(int64_t)((s + 1) * step - 1) * m + row];
+        T acc[K];
+#pragma unroll
+        for (int q = 0; q < K; ++q) acc[q] = T(0);
+        for (RP l = lb + lt; l < le; l += TEAM) {
+            const int32_t c = __builtin_nontemporal_load(ci + l);
+            const T av = a ? __builtin_nontemporal_load(a + l) : T(1);
+            const T* xr = X + (int64_t)c * ldX;
+#pragma unroll
+            for (int q = 0; q < K; ++q) acc[q] += av * xr[q];
+        }
+#pragma unroll
+        for (int q = 0; q < K; ++q) acc[q] = group_sum<TEAM>(acc[q]);
+        if (lt == 0) {
+            T* pr = part + ((int64_t)s * m + row) * K;
+#pragma unroll
+            for (int q = 0; q < K; ++q) pr[q] = acc[q];
+        }
+    }
+}
+
+// Y[i, :] = alpha * sum_s part_s[i, :] + beta * Y[i, :] (slices added in order)
+template <typename T, typename RP, int K>
+__global__ __launch_bounds__(kBlock) void slice_reduce_kernel(int m, int S, const RP* __restrict__ rp,
+                                                              const T* __restrict__ part, T alpha, T* __restrict__ Y,
+                                                              int64_t ldY, T beta, int alpha_one, int beta_mode) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < m; i += gridDim.x * kBlock) {
+        T z[K];
+#pragma unroll
+        for (int q = 0; q < K; ++q) z[q] = part[(int64_t)i * K + q];
+        for (int s = 1; s < S; ++s)
+#pragma unroll
+            for (int q = 0; q < K; ++q) z[q] += part[((int64_t)s * m + i) * K + q];
+        const bool empty = rp[i + 1] == rp[i];
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            const T zq[1] = {z[q]};
+            epilogue<T, 1>(Y + (int64_t)i * ldY + q, zq, empty, alpha, alpha_one, beta, beta_mode);
+        }
+    }
+}
+
+// Slices for an X of `xbytes`: 0 = not worth it (X fits an L2 already, or is so
+// large that the partial blocks cost more than the misses they save).
+static int slice_count(int64_t xbytes) {
+    constexpr int64_t kL2Share = 2ll << 20;  // X bytes per XCD slice that stay L2-resident
+    if (xbytes < 6 * kL2Share / 2 || xbytes > 16 * kL2Share) return 0;
+    int S = 2;
+    while (S < kXcds && xbytes / S > kL2Share) S *= 2;
+    return S;
+}
+
+template <typename T, typename RP, int TEAM>
+static void launch_slice_team(int S, int K, int grid, int m, const RP* rp, const int32_t* cuts, const int32_t* ci,
+                              const T* a, const T* X, int64_t ldX, T* part, int rpb, hipStream_t st) {
+#define TG_SLICE(KK)                                                                                         \
+    case KK:                                                                                                 \
+        hipLaunchKernelGGL((spmm_slice_kernel<T, RP, TEAM, KK>), dim3(grid), dim3(kBlock), 0, st, m, S, rp, cuts, \
+                           ci, a, X, ldX, part, rpb);                                                        \
+        break;
+    switch (K) {
+        TG_SLICE(1)
+        TG_SLICE(2)
+        TG_SLICE(3)
+        TG_SLICE(4)
+    }
+#undef TG_SLICE
+}
+
+template <typename T, typename RP>
+static int spmm_sliced_typed(int S, int32_t m, int64_t nnz, const RP* rp, const int32_t* cuts, const int32_t* ci,
+                             const T* a, double alpha_d, const T* X, int64_t ldX, double beta_d, T* Y, int64_t ldY,
+                             int32_t K, T* part, hipStream_t st) {
+    const double avg = m > 0 ? (double)nnz / m / S : 0.0;
+    const int team = std::min(32, std::max(4, pow2ceil((int)std::ceil(avg))));
+    const int rpb = (kBlock / team) * kRowsPerTeam;
+    const int row_blocks = (m + rpb - 1) / rpb;
+    const int per = kXcds / S;
+    const int grid = kXcds * ((row_blocks + per - 1) / per);
+    switch (team) {
+        case 4: launch_slice_team<T, RP, 4>(S, K, grid, m, rp, cuts, ci, a, X, ldX, part, rpb, st); break;
+        case 8: launch_slice_team<T, RP, 8>(S, K, grid, m, rp, cuts, ci, a, X, ldX, part, rpb, st); break;
+        case 16: launch_slice_team<T, RP, 16>(S, K, grid, m, rp, cuts, ci, a, X, ldX, part, rpb, st); break;
+        default: launch_slice_team<T, RP, 32>(S, K, grid, m, rp, cuts, ci, a, X, ldX, part, rpb, st); break;
+    }
+    const int alpha_one = (alpha_d == 1.0);
+    const int beta_mode = (beta_d == 0.0) ? kBetaZero : (beta_d == 1.0) ? kBetaOne : kBetaGeneric;
+    const int rgrid = (int)std::min<int64_t>((m + kBlock - 1) / kBlock, 65536);
+#define TG_SRED(KK)                                                                                       \
+    case KK:                                                                                              \
+        hipLaunchKernelGGL((slice_reduce_kernel<T, RP, KK>), dim3(rgrid), dim3(kBlock), 0, st, m, S, rp, \
+                           (const T*)part, (T)alpha_d, Y, ldY, (T)beta_d, alpha_one, beta_mode);          \
+        break;
+    switch (K) {
+        TG_SRED(1)
+        TG_SRED(2)
+        TG_SRED(3)
+        TG_SRED(4)
+    }
+#undef TG_SRED
+    TG_LAUNCH_CHECK();
+    return TG_OK;
+}
+
 }  // namespace tg
 
 using namespace tg;
@@ -562,6 +724,12 @@ extern "C" int tg_tune_set(const char* key, int32_t value) {
         g_spmm_group = value;
         return TG_OK;
     }
+    if (k == "spmm_slices") {
+        if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8)
+            return fail(TG_ERR_INVALID, "tg_tune_set: spmm_slices must be 0 (auto), 1 (off), 2, 4 or 8");
+        g_spmm_slices = value;
+        return TG_OK;
+    }
     return fail(TG_ERR_INVALID, "tg_tune_set: unknown key '%s'", key);
 }
 
@@ -571,6 +739,7 @@ extern "C" int32_t tg_tune_get(const char* key) {
     if (k == "spmv_kernel") return spmv_variant();
     if (k == "spmv_team") return g_spmv_team;
     if (k == "spmm_group") return g_spmm_group;
+    if (k == "spmm_slices") return g_spmm_slices;
     return -1;
 }
 
@@ -672,6 +841,125 @@ int op_apply(const OpRef& op, const void* x, void* y, int k, const int* skip, hi
 }
 
 }  // namespace tg
+
+struct tg_csr_plan_s {
+    int32_t m = 0, n = 0;
+    int64_t nnz = 0;
+    const void* rp = nullptr;
+    tg_dtype rp_type = TG_I32;
+    const int32_t* ci = nullptr;
+    int32_t* cuts = nullptr;  // kCuts x m row offsets of the 8-way column cut
+    int sorted = 0;
+    void* part = nullptr;     // partial blocks of the sliced path (grown on demand)
+    size_t part_bytes = 0;
+};
+
+extern "C" int tg_csr_plan_create(int32_t m, int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type,
+                                  const int32_t* ci, tg_csr_plan_t* out, void* stream) {
+    if (!out || m < 0 || n < 0 || nnz < 0 || !rp || (nnz > 0 && !ci))
+        return fail(TG_ERR_INVALID, "tg_csr_plan_create: bad argument");
+    if (rp_type != TG_I32 && rp_type != TG_I64) return fail(TG_ERR_DTYPE, "tg_csr_plan_create: rp_type must be I32/I64");
+    hipStream_t st = as_stream(stream);
+    auto* P = new tg_csr_plan_s();
+    P->m = m;
+    P->n = n;
+    P->nnz = nnz;
+    P->rp = rp;
+    P->rp_type = rp_type;
+    P->ci = ci;
+    int* flag = nullptr;
+    if (m > 0 && n > 0) {
+        if (hipMallocAsync((void**)&P->cuts, (size_t)kCuts * m * sizeof(int32_t), st) != hipSuccess ||
+            hipMallocAsync((void**)&flag, sizeof(int), st) != hipSuccess) {
+            delete P;
+            return fail(TG_ERR_HIP, "tg_csr_plan_create: allocation failed");
+        }
+        int zero = 0;
+        (void)hipMemcpyAsync(flag, &zero, sizeof(int), hipMemcpyHostToDevice, st);
+        const int grid = (int)std::min<int64_t>((m + kBlock - 1) / kBlock, 65536);
+        if (rp_type == TG_I32)
+            hipLaunchKernelGGL((slice_plan_kernel<int32_t>), dim3(grid), dim3(kBlock), 0, st, m, n, (const int32_t*)rp,
+                               ci, P->cuts, flag);
+        else
+            hipLaunchKernelGGL((slice_plan_kernel<int64_t>), dim3(grid), dim3(kBlock), 0, st, m, n, (const int64_t*)rp,
+                               ci, P->cuts, flag);
+        int unsorted = 1;
+        (void)hipMemcpyAsync(&unsorted, flag, sizeof(int), hipMemcpyDeviceToHost, st);
+        (void)hipFreeAsync(flag, st);
+        if (hipStreamSynchronize(st) != hipSuccess || hipGetLastError() != hipSuccess) {
+            (void)hipFree(P->cuts);
+            delete P;
+            return fail(TG_ERR_HIP, "tg_csr_plan_create: plan kernel failed");
+        }
+        P->sorted = unsorted ? 0 : 1;
+    }
+    *out = P;
+    return TG_OK;
+}
+
+extern "C" int tg_csr_plan_destroy(tg_csr_plan_t plan) {
+    if (!plan) return TG_OK;
+    if (plan->cuts) (void)hipFree(plan->cuts);
+    if (plan->part) (void)hipFree(plan->part);
+    delete plan;
+    return TG_OK;
+}
+
+extern "C" int32_t tg_csr_plan_slices(tg_csr_plan_t plan, tg_dtype val_type, int32_t nrhs) {
+    if (!plan || !plan->sorted || nrhs < 1 || nrhs > 4 || plan->m == 0 || plan->n == 0) return 0;
+    if (g_spmm_slices == 1) return 0;
+    if (g_spmm_slices > 1) return g_spmm_slices;
+    const int64_t vs = val_type == TG_F64 ? 8 : 4;
+    return slice_count((int64_t)plan->n * nrhs * vs);
+}
+
+extern "C" int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, double alpha, const void* X,
+                                int64_t ldX, double beta, void* Y, int64_t ldY, int32_t nrhs, int32_t NRHS,
+                                void* stream) {
+    if (!plan) return fail(TG_ERR_INVALID, "tg_csr_spmm_plan: null plan");
+    if (NRHS > 0 && nrhs != NRHS)
+        return fail(TG_ERR_NRHS, "tg_csr_spmm: nrhs (%d) != NRHS (%d). Doing nothing.", nrhs, NRHS);
+    CsrView A;
+    A.m = plan->m;
+    A.n = plan->n;
+    A.nnz = plan->nnz;
+    A.rp = plan->rp;
+    A.rp_type = plan->rp_type;
+    A.ci = plan->ci;
+    A.val = a;
+    A.val_type = val_type;
+    hipStream_t st = as_stream(stream);
+    const int S = (val_type == TG_F64 || val_type == TG_F32) ? tg_csr_plan_slices(plan, val_type, nrhs) : 0;
+    // the plain path covers every special case (alpha = 0, aliasing, bad arguments)
+    if (S == 0 || alpha == 0.0 || X == Y || !X || !Y || ldX < nrhs || ldY < nrhs)
+        return spmm_dispatch(A, alpha, X, ldX, beta, Y, ldY, nrhs, nullptr, 0, nullptr, st);
+    const size_t vs = val_type == TG_F64 ? 8 : 4;
+    const size_t need = (size_t)S * plan->m * nrhs * vs;
+    if (plan->part_bytes < need) {
+        if (plan->part) TG_HIP(hipFreeAsync(plan->part, st));
+        plan->part = nullptr;
+        plan->part_bytes = 0;
+        TG_HIP(hipMallocAsync(&plan->part, need, st));
+        plan->part_bytes = need;
+    }
+    const int32_t m = plan->m;
+    if (val_type == TG_F64) {
+        if (plan->rp_type == TG_I32)
+            return spmm_sliced_typed<double, int32_t>(S, m, plan->nnz, (const int32_t*)plan->rp, plan->cuts, plan->ci,
+                                                      (const double*)a, alpha, (const double*)X, ldX, beta, (double*)Y,
+                                                      ldY, nrhs, (double*)plan->part, st);
+        return spmm_sliced_typed<double, int64_t>(S, m, plan->nnz, (const int64_t*)plan->rp, plan->cuts, plan->ci,
+                                                  (const double*)a, alpha, (const double*)X, ldX, beta, (double*)Y,
+                                                  ldY, nrhs, (double*)plan->part, st);
+    }
+    if (plan->rp_type == TG_I32)
+        return spmm_sliced_typed<float, int32_t>(S, m, plan->nnz, (const int32_t*)plan->rp, plan->cuts, plan->ci,
+                                                 (const float*)a, alpha, (const float*)X, ldX, beta, (float*)Y, ldY,
+                                                 nrhs, (float*)plan->part, st);
+    return spmm_sliced_typed<float, int64_t>(S, m, plan->nnz, (const int64_t*)plan->rp, plan->cuts, plan->ci,
+                                             (const float*)a, alpha, (const float*)X, ldX, beta, (float*)Y, ldY, nrhs,
+                                             (float*)plan->part, st);
+}
 
 extern "C" int tg_csr_spmm(int32_t m, int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, const int32_t* ci,
                            const void* a, tg_dtype val_type, double alpha, const void* X, int64_t ldX, double beta,

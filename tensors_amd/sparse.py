@@ -43,6 +43,38 @@ def spmm_raw(m, n, nnz, rp, ci, a, alpha, X, ldX, beta, Y, ldY, nrhs, NRHS=0, jo
     return capi.check(st, "tg_csr_spmm")
 
 
+class SlicePlan:
+    """tg_csr_plan_t of one device CSR pattern (column-sliced SpMM for small X)."""
+
+    def __init__(self, m, n, nnz, rp, ci, stream=None):
+        import ctypes as C
+
+        self.h = C.c_void_p()
+        self._keep = (rp, ci)
+        capi.call("tg_csr_plan_create", m, n, nnz, rp.data_ptr(), capi.I64 if rp.dtype == torch.int64 else capi.I32,
+                  ci.data_ptr() if ci.numel() else None, C.byref(self.h), stream_handle(stream))
+
+    def slices(self, dtype, nrhs):
+        return capi.lib().tg_csr_plan_slices(self.h, _TORCH_TO_TG[dtype], nrhs)
+
+    def spmm(self, a, alpha, X, ldX, beta, Y, ldY, nrhs, NRHS=0, stream=None):
+        """tg_csr_spmm_plan; returns the status like spmm_raw."""
+        st = capi.lib().tg_csr_spmm_plan(
+            self.h, a.data_ptr() if a is not None else None, _TORCH_TO_TG[Y.dtype], float(alpha),
+            X.data_ptr() if X is not None else None, ldX, float(beta), Y.data_ptr(), ldY, nrhs, NRHS,
+            stream_handle(stream))
+        if st in (capi.ERR_ALIAS, capi.ERR_NRHS):
+            return st
+        return capi.check(st, "tg_csr_spmm_plan")
+
+    def __del__(self):
+        try:
+            if self.h:
+                capi.lib().tg_csr_plan_destroy(self.h)
+        except Exception:
+            pass
+
+
 class MatrixCSR:
     """m x n CSR matrix: ``outer`` (m+1, int32/int64), ``inner`` (nnz, int32), ``values``
     (nnz, float64/float32, or None for a pattern/binary matrix).
@@ -59,6 +91,7 @@ class MatrixCSR:
         self.device = device
         self._dev = None
         self._job_ptr = None
+        self._plan = None
 
     @classmethod
     def from_device(cls, rp, ci, vals, m, n):
@@ -104,6 +137,13 @@ class MatrixCSR:
             vals = cache[want]
         return self._dev["rp"], self._dev["ci"], vals
 
+    def Plan(self):
+        """Column-slice plan of the device pattern (tg_csr_plan_create), made on first use."""
+        if self._plan is None:
+            rp, ci, _ = self.device_arrays()
+            self._plan = SlicePlan(self.m, self.n, self.NonzeroCount(), rp, ci)
+        return self._plan
+
     def JobPtr(self, nrhs=1):
         """Device row partition (tg_csr_job_pointers), cached per nrhs."""
         if self._job_ptr is None:
@@ -140,9 +180,12 @@ class MatrixCSR:
         rp, ci, vals = self.device_arrays(Y.dtype)
         if ext_values is not None:
             vals = ext_values
-        jp, blocks = self.JobPtr(k) if use_job_ptr else (None, 0)
-        st = spmm_raw(self.m, self.n, self.NonzeroCount(), rp, ci, vals, alpha, X, ldX, beta, Y, ldY, k,
-                      NRHS, jp, blocks, stream)
+        if use_job_ptr:
+            jp, blocks = self.JobPtr(k)
+            st = spmm_raw(self.m, self.n, self.NonzeroCount(), rp, ci, vals, alpha, X, ldX, beta, Y, ldY, k,
+                          NRHS, jp, blocks, stream)
+        else:
+            st = self.Plan().spmm(vals, alpha, X, ldX, beta, Y, ldY, k, NRHS, stream)
         if st == capi.ERR_ALIAS:
             _eprint(self._name("Dot") + ": Input and output pointer coincide. This is not safe. Aborting.")
         elif st == capi.ERR_NRHS:

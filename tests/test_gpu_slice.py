@@ -1,0 +1,122 @@
+"""Column-sliced SpMM (tg_csr_plan_* / tg_csr_spmm_plan, the path MatrixCSR.Dot takes):
+every forced slice count and nrhs <= 4 stays inside the per-entry rounding bound of the
+exact product (any summation order), keeps the alpha/beta and empty-row rules, falls
+back to the plain kernels for unsorted rows, and picks slices by itself for an X of a
+few L2s."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+import oracle
+from helpers import random_csr_np, rel_err, spmm_bound
+from tensors_amd import _capi as capi
+from tensors_amd import autotune
+from tensors_amd.sparse import MatrixCSR, SlicePlan
+
+pytestmark = pytest.mark.gpu
+
+
+def dev_arrays(rp, ci, val, dtype, cuda):
+    return (torch.from_numpy(rp.astype(np.int32)).to(cuda), torch.from_numpy(ci).to(cuda),
+            torch.from_numpy(val.astype(dtype)).to(cuda))
+
+
+@pytest.fixture
+def knobs():
+    before = autotune.get_knobs()
+    yield
+    autotune.set_knobs(before)
+
+
+@pytest.mark.parametrize("S", [2, 4, 8])
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_forced_slices_within_rounding_bound(cuda, knobs, S, k, dtype):
+    m, n = 2500, 3100
+    rp, ci, val = random_csr_np(m, n, 0.01, seed=S * 10 + k, empty_rows=0.05)
+    rng = np.random.default_rng(k)
+    X = rng.uniform(-1, 1, (n, k))
+    Y0 = rng.uniform(-1, 1, (m, k))
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    rpd, cid, vd = dev_arrays(rp, ci, val, dtype, cuda)
+    plan = SlicePlan(m, n, len(ci), rpd, cid)
+    autotune.set_knobs({"spmm_slices": S})
+    assert plan.slices(tdt, k) == S
+    u = 2.0 ** -53 if dtype == np.float64 else 2.0 ** -24
+    for alpha, beta in [(1.0, 0.0), (2.0, 1.0), (-1.0, 0.5), (0.0, 2.0)]:
+        Xd = torch.from_numpy(X.astype(dtype)).to(cuda)
+        Yd = torch.from_numpy(Y0.astype(dtype)).to(cuda)
+        assert plan.spmm(vd, alpha, Xd, k, beta, Yd, k, k) == 0
+        Xe, Ye, Ve = X.astype(dtype).astype(np.float64), Y0.astype(dtype).astype(np.float64), val.astype(dtype)
+        Yo = Ye.copy()
+        oracle.spmm(rp, ci, Ve.astype(np.float64), m, n, alpha, Xe, k, beta, Yo, k, k)
+        bound = spmm_bound(m, n, rp, ci, Ve.astype(np.float64), alpha, Xe, beta, Ye, u)
+        assert np.all(np.abs(Yd.cpu().numpy().astype(np.float64) - Yo) <= bound), (alpha, beta)
+        empty = np.diff(rp) == 0
+        if beta == 1.0:  # empty rows untouched
+            assert np.array_equal(Yd.cpu().numpy()[empty], Y0.astype(dtype)[empty])
+
+
+def test_beta_zero_ignores_nan_and_matches_plain(cuda, knobs):
+    m, n, k = 1800, 2000, 2
+    rp, ci, val = random_csr_np(m, n, 0.02, seed=3, empty_rows=0.1)
+    rpd, cid, vd = dev_arrays(rp, ci, val, np.float64, cuda)
+    X = torch.rand(n, k, dtype=torch.float64, device=cuda)
+    plan = SlicePlan(m, n, len(ci), rpd, cid)
+    autotune.set_knobs({"spmm_slices": 8})
+    Y = torch.full((m, k), float("nan"), dtype=torch.float64, device=cuda)
+    plan.spmm(vd, 1.0, X, k, 0.0, Y, k, k)
+    assert not torch.isnan(Y).any()
+    autotune.set_knobs({"spmm_slices": 1})
+    Yp = torch.full((m, k), float("nan"), dtype=torch.float64, device=cuda)
+    plan.spmm(vd, 1.0, X, k, 0.0, Yp, k, k)
+    assert rel_err(Y.cpu().numpy(), Yp.cpu().numpy()) <= 1e-14
+    # NRHS mismatch and aliasing keep the reference's no-op behaviour
+    assert plan.spmm(vd, 1.0, X, k, 0.0, Y, k, k, NRHS=3) == capi.ERR_NRHS
+    assert plan.spmm(vd, 1.0, X, k, 0.0, X, k, k) == capi.ERR_ALIAS
+
+
+def test_unsorted_rows_fall_back(cuda, knobs):
+    m = n = 1000
+    rp, ci, val = random_csr_np(m, n, 0.02, seed=9)
+    ci2 = ci.copy()
+    for i in range(0, m, 7):  # reverse some rows
+        ci2[rp[i]:rp[i + 1]] = ci2[rp[i]:rp[i + 1]][::-1].copy()
+        val[rp[i]:rp[i + 1]] = val[rp[i]:rp[i + 1]][::-1].copy()
+    rpd, cid, vd = dev_arrays(rp, ci2, val, np.float64, cuda)
+    plan = SlicePlan(m, n, len(ci2), rpd, cid)
+    autotune.set_knobs({"spmm_slices": 4})
+    assert plan.slices(torch.float64, 1) == 0
+    X = torch.rand(n, 1, dtype=torch.float64, device=cuda)
+    Y = torch.zeros(m, 1, dtype=torch.float64, device=cuda)
+    plan.spmm(vd, 1.0, X, 1, 0.0, Y, 1, 1)
+    A = sp.csr_matrix((val, ci2, rp), shape=(m, n))
+    assert rel_err(Y.cpu().numpy(), A @ X.cpu().numpy()) <= 1e-13
+
+
+@pytest.mark.parametrize("k,dtype", [(1, torch.float64), (4, torch.float32)])
+def test_auto_slices_large_matrix_dot(cuda, knobs, k, dtype):
+    """MatrixCSR.Dot on an 800k x 800k matrix (X of 6.4 / 12.8 MB): slices chosen by size."""
+    autotune.set_knobs({"spmm_slices": 0})
+    m = n = 800_000
+    rng = np.random.default_rng(11)
+    lens = rng.integers(1, 12, m)
+    rp = np.zeros(m + 1, np.int64)
+    rp[1:] = np.cumsum(lens)
+    # random columns, sorted within each row
+    ci = rng.integers(0, n, rp[-1]).astype(np.int32)
+    for_sort = np.repeat(np.arange(m), lens) * np.int64(n) + ci
+    ci = (np.sort(for_sort) % n).astype(np.int32)
+    val = rng.normal(size=rp[-1])
+    A = MatrixCSR(rp.astype(np.int32), ci, val, m, n)
+    X = torch.rand(n, k, dtype=dtype, device=cuda) * 2 - 1
+    Y = torch.zeros(m, k, dtype=dtype, device=cuda)
+    assert A.Plan().slices(dtype, k) in (4, 8)
+    assert A.Dot(1.0, X, 0.0, Y) == 0
+    Xh = X.double().cpu().numpy()
+    Yo = np.zeros((m, k))
+    oracle.spmm(rp, ci, val.astype(np.float32).astype(np.float64) if dtype == torch.float32 else val, m, n, 1.0,
+                Xh, k, 0.0, Yo, k, k)
+    tol = 1e-12 if dtype == torch.float64 else 2e-6
+    assert rel_err(Y.double().cpu().numpy(), Yo) <= tol
