@@ -116,7 +116,7 @@ int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, doubl
  *                  groups split each row's nonzeros), 0 = enough lanes for all
  *                  columns, else a power of two in [1, 64].
  *   "spmm_slices": column slices of tg_csr_spmm_plan, 0 = from the size of X,
- *                  1 = never, 2 / 4 / 8 = always that many (nrhs <= 4).
+ *                  1 = never (default), 2 / 4 / 8 = always that many (nrhs <= 4).
  * Process-wide; not thread-safe against concurrent SpMM calls. */
 int tg_tune_set(const char* key, int32_t value);
 int32_t tg_tune_get(const char* key);

@@ -19,9 +19,9 @@ import numpy as np
 import torch
 
 from . import _capi as capi
-from .sparse import MatrixCSR, spmm_raw
+from .sparse import MatrixCSR
 
-KNOBS = ("spmv_kernel", "spmv_team", "spmm_group")
+KNOBS = ("spmv_kernel", "spmv_team", "spmm_group", "spmm_slices")
 
 
 def get_knobs():
@@ -45,31 +45,42 @@ def candidates(nrhs):
     return [{"spmm_group": 0}] + [{"spmm_group": g} for g in (1, 2, 4, 8, 16, 32, 64) if g <= gmax]
 
 
+def slice_candidates(nrhs):
+    """Column-slice counts of the planned path (nrhs <= 4 only)."""
+    return [{"spmm_slices": s} for s in ((1, 2, 4, 8) if nrhs <= 4 else (1,))]
+
+
 def tune_spmm(A: MatrixCSR, X: torch.Tensor, Y: torch.Tensor, reps: int = 5, keep: bool = True):
-    """Times Y <- A X for every shape candidate; returns (best_cfg, table of (cfg, ms)).
-    X, Y: CUDA tensors of A's value dtype, n x nrhs / m x nrhs (or 1-D for nrhs = 1)."""
+    """Times Y <- A X (the MatrixCSR.Dot path, tg_csr_spmm_plan) for every kernel shape with
+    column slicing off, then every slice count with the best shape; returns (best_cfg,
+    table of (cfg, ms)).  X, Y: CUDA tensors of A's value dtype, n x nrhs / m x nrhs (or
+    1-D for nrhs = 1)."""
     nrhs = 1 if X.dim() == 1 else X.shape[1]
-    rp, ci, vals = A.device_arrays(X.dtype)
+    _, _, vals = A.device_arrays(X.dtype)
+    plan = A.Plan()
     ld_x = 1 if X.dim() == 1 else X.stride(0)
     ld_y = 1 if Y.dim() == 1 else Y.stride(0)
     stream = torch.cuda.current_stream()
     before = get_knobs()
     table = []
+
+    def timed(cfg):
+        set_knobs(cfg)
+        plan.spmm(vals, 1.0, X, ld_x, 0.0, Y, ld_y, nrhs)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record(stream)
+        for _ in range(reps):
+            plan.spmm(vals, 1.0, X, ld_x, 0.0, Y, ld_y, nrhs)
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        table.append((dict(cfg), ev[0].elapsed_time(ev[1]) / reps))
+        return table[-1][1]
+
     try:
-        for cfg in candidates(nrhs):
-            set_knobs(cfg)
-
-            def run():
-                spmm_raw(A.m, A.n, A.NonzeroCount(), rp, ci, vals, 1.0, X, ld_x, 0.0, Y, ld_y, nrhs)
-
-            run()
-            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-            ev[0].record(stream)
-            for _ in range(reps):
-                run()
-            ev[1].record(stream)
-            torch.cuda.synchronize()
-            table.append((dict(cfg), ev[0].elapsed_time(ev[1]) / reps))
+        shapes = [dict(c, spmm_slices=1) for c in candidates(nrhs)]
+        best = min(shapes, key=timed)
+        for sc in slice_candidates(nrhs)[1:]:
+            timed(dict(best, **sc))
     finally:
         set_knobs(before)
     best = min(table, key=lambda t: t[1])[0]

@@ -113,7 +113,7 @@ __device__ __forceinline__ double sys_total(const double* partial, int tiles, in
 // ---- setup: r = b, z = P r, x = 0, partial r.z (parity 0) -------------------------
 template <typename T>
 __global__ __launch_bounds__(kBlock) void bk_init_kernel(const Tile* __restrict__ tiles, const SysDev<T>* __restrict__ sys) {
-    const Tile tl = tiles[blockIdx.x];
+    const Tile tl = tiles[xcd_swizzle(blockIdx.x, gridDim.x)];
     const SysDev<T>& S = sys[tl.slot];
     const int c = threadIdx.x % S.kpad, ro = threadIdx.x / S.kpad, rows = kBlock / S.kpad;
     double acc = 0.0;
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void bk_setup_kernel(const int* __restrict_
 template <typename T>
 __global__ __launch_bounds__(kBlock) void bk_dir_spmm_kernel(const Tile* __restrict__ tiles, const SysDev<T>* __restrict__ sys,
                                                              SysState* __restrict__ st) {
-    const Tile tl = tiles[blockIdx.x];
+    const Tile tl = tiles[xcd_swizzle(blockIdx.x, gridDim.x)];
     SysState& ss = st[tl.slot];
     const SysDev<T>& S = sys[tl.slot];
     const int it = ss.it_b;
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(kBlock) void bk_dir_spmm_kernel(const Tile* __restr
 template <typename T>
 __global__ __launch_bounds__(kBlock) void bk_update_kernel(const Tile* __restrict__ tiles, const SysDev<T>* __restrict__ sys,
                                                            SysState* __restrict__ st) {
-    const Tile tl = tiles[blockIdx.x];
+    const Tile tl = tiles[xcd_swizzle(blockIdx.x, gridDim.x)];
     SysState& ss = st[tl.slot];
     if (ss.done) return;
     const SysDev<T>& S = sys[tl.slot];
@@ -298,6 +298,7 @@ __global__ __launch_bounds__(kBlock) void bk_update_kernel(const Tile* __restric
 
 struct Resume {
     int slot;
+    int it;  // the system's completed iterations: no checkpoint pause there
     double scale;
 };
 
@@ -307,7 +308,7 @@ __global__ void bk_resume_kernel(const Resume* __restrict__ list, SysState* __re
     SysState& ss = st[rs.slot];
     if (threadIdx.x < kBatchMaxK) ss.TOL[threadIdx.x] *= rs.scale;
     if (threadIdx.x == 0) {
-        ss.resumed = ss.iter;
+        ss.resumed = rs.it;
         ss.paused = 0;
         ss.done = 0;
         ss.succ = 0;
@@ -471,7 +472,7 @@ template <typename T> class CGBatch {
 
     // Continues a system the device marked done, with its tolerance scaled by `scale`.
     // Queued; applied by one launch before the next iterations (or poll).
-    void resume(int slot, double scale) { resumes_.push_back(Resume{slot, scale}); }
+    void resume(int slot, int it, double scale) { resumes_.push_back(Resume{slot, it, scale}); }
 
     void mark_done_host(int slot) {
         if (slot >= 0 && slot < (int)parked_.size()) parked_[slot] = 1;
@@ -581,36 +582,94 @@ template <typename T> class CGBatch {
 //   partials; then x64 += x32, x32 = 0.
 // The search direction p is kept, so CG continues instead of restarting.
 
-// r = b - A (x64 + x32) on the engine's tile; writes r32, the r.z partial
-// (the engine's parity slot) and ||r||^2 partial.  x64 and x32 are only read.
-__global__ __launch_bounds__(kBlock) void bm_replace_kernel(int n, int k, int kpad, const int32_t* __restrict__ rp,
-                                                            const int32_t* __restrict__ ci,
-                                                            const double* __restrict__ val,
-                                                            const double* __restrict__ b, const double* __restrict__ x64,
-                                                            const float* __restrict__ x32, const float* __restrict__ d32,
-                                                            float* __restrict__ r32, double* __restrict__ prz,
-                                                            double* __restrict__ pnorm) {
-    const int r0 = blockIdx.x * kTileRows, r1 = min(n, r0 + kTileRows);
+// r = b - A (x64 + x32) for every system being replaced, one workgroup per (system,
+// row tile of the engine's tiling); writes r32, the r.z partials of the engine's
+// current parity and the ||r||^2 partials.  x64 and x32 are only read here.
+struct ReplaceDesc {
+    const int32_t* rp;
+    const int32_t* ci;
+    const double* val;
+    const double* b;
+    double* x64;
+    float* x32;
+    const float* d32;
+    float* r32;      // the FP32 engine's r
+    double* prz;     // the engine's r.z partials of the current parity
+    double* pnorm;   // ||r||^2 partials
+    double* norms;   // k true-residual norms out
+    int n, k, kpad, tiles;
+};
+
+struct ReplaceTile {
+    int desc, r0, r1, t;
+};
+
+__global__ __launch_bounds__(kBlock) void bm_replace_many_kernel(const ReplaceTile* __restrict__ tiles,
+                                                                 const ReplaceDesc* __restrict__ descs) {
+    const ReplaceTile tl = tiles[xcd_swizzle(blockIdx.x, gridDim.x)];
+    const ReplaceDesc& D = descs[tl.desc];
+    const int k = D.k, kpad = D.kpad;
     const int c = threadIdx.x % kpad, ro = threadIdx.x / kpad, rows = kBlock / kpad;
     double acc_rz = 0.0, acc_n = 0.0;
-    if (c < k)
-        for (int i = r0 + ro; i < r1; i += rows) {
+    if (c < k) {
+        const int32_t* __restrict__ rp = D.rp;
+        const int32_t* __restrict__ ci = D.ci;
+        const double* __restrict__ val = D.val;
+        const double* __restrict__ x64 = D.x64;
+        const float* __restrict__ x32 = D.x32;
+        for (int i = tl.r0 + ro; i < tl.r1; i += rows) {
+            const int lb = rp[i], le = rp[i + 1];
             double ax = 0.0;
-            for (int l = rp[i]; l < rp[i + 1]; ++l) {
-                const int64_t f = (int64_t)ci[l] * k + c;
-                ax += val[l] * (x64[f] + (double)x32[f]);
+            for (int l0 = lb; l0 < le; l0 += kGather) {
+                int32_t jj[kGather];
+                double vv[kGather], xv[kGather];
+#pragma unroll
+                for (int q = 0; q < kGather; ++q) {
+                    const bool ok = l0 + q < le;
+                    jj[q] = ok ? ci[l0 + q] : i;
+                    vv[q] = ok ? val[l0 + q] : 0.0;
+                }
+#pragma unroll
+                for (int q = 0; q < kGather; ++q) {
+                    const int64_t f = (int64_t)jj[q] * k + c;
+                    xv[q] = x64[f] + (double)x32[f];
+                }
+#pragma unroll
+                for (int q = 0; q < kGather; ++q)
+                    if (l0 + q < le) ax += vv[q] * xv[q];
             }
             const int64_t e = (int64_t)i * k + c;
-            const double r = b[e] - ax;
+            const double r = D.b[e] - ax;
             const float rf = (float)r;
-            r32[e] = rf;
-            const float zf = d32 ? d32[i] * rf : rf;
+            D.r32[e] = rf;
+            const float zf = D.d32 ? D.d32[i] * rf : rf;
             acc_rz += (double)rf * (double)zf;
             acc_n += r * r;
         }
-    tile_partial(acc_rz, k, kpad, prz, blockIdx.x);
+    }
+    tile_partial(acc_rz, k, kpad, D.prz, tl.t);
     __syncthreads();
-    tile_partial(acc_n, k, kpad, pnorm, blockIdx.x);
+    tile_partial(acc_n, k, kpad, D.pnorm, tl.t);
+}
+
+// x64 += x32, x32 = 0 on the replaced systems' tiles
+__global__ __launch_bounds__(kBlock) void bm_flush_many_kernel(const ReplaceTile* __restrict__ tiles,
+                                                               const ReplaceDesc* __restrict__ descs) {
+    const ReplaceTile tl = tiles[xcd_swizzle(blockIdx.x, gridDim.x)];
+    const ReplaceDesc& D = descs[tl.desc];
+    const int64_t e0 = (int64_t)tl.r0 * D.k, e1 = (int64_t)tl.r1 * D.k;
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += kBlock) {
+        D.x64[e] += (double)D.x32[e];
+        D.x32[e] = 0.0f;
+    }
+}
+
+// one workgroup per replaced system: sqrt of the tile-partial totals
+__global__ __launch_bounds__(kBlock) void bm_norm_many_kernel(const ReplaceDesc* __restrict__ descs) {
+    const ReplaceDesc& D = descs[blockIdx.x];
+    const int c = threadIdx.x % D.kpad, j = threadIdx.x / D.kpad, J = kBlock / D.kpad;
+    const double t = sys_total(D.pnorm, D.tiles, D.k, D.kpad, c, j, J);
+    if (j == 0 && c < D.k) D.norms[c] = sqrt(t);
 }
 
 __global__ __launch_bounds__(kBlock) void bm_norm_total_kernel(int tiles, int k, int kpad, const double* __restrict__ partial,
@@ -632,14 +691,6 @@ __global__ __launch_bounds__(kBlock) void bm_bnorm_kernel(int n, int k, int kpad
             acc += v * v;
         }
     tile_partial(acc, k, kpad, partial, blockIdx.x);
-}
-
-// x64 += x32, x32 = 0
-__global__ __launch_bounds__(kBlock) void bm_flush_kernel(int64_t count, float* __restrict__ x32, double* __restrict__ x64) {
-    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < count; e += (int64_t)gridDim.x * kBlock) {
-        x64[e] += (double)x32[e];
-        x32[e] = 0.0f;
-    }
 }
 
 __global__ __launch_bounds__(kBlock) void bm_zero_kernel(int64_t count, double* __restrict__ x) {
@@ -685,6 +736,78 @@ struct BatchSystem {
     std::vector<double> rel;
 };
 
+// Collects the residual replacements of one poll; run() launches them all at once
+// (replacement, fold of x32 into x64, norms) and reads every norm back in one copy.
+struct ReplaceBatch {
+    std::vector<ReplaceDesc> descs;
+    std::vector<ReplaceTile> tiles;
+    std::vector<BatchSystem*> sys;
+    ReplaceDesc* ddesc = nullptr;
+    ReplaceTile* dtile = nullptr;
+    size_t desc_cap = 0, tile_cap = 0;
+
+    void add(const SysDev<float>& E, double* norms_out, BatchSystem& S, int it) {
+        ReplaceDesc D{};
+        D.rp = S.rp;
+        D.ci = S.ci;
+        D.val = S.val64;
+        D.b = S.b64;
+        D.x64 = S.x64;
+        D.x32 = S.x32;
+        D.d32 = S.d32;
+        D.r32 = E.r;
+        D.prz = pick(E.prz, it);
+        D.pnorm = S.pnorm;
+        D.norms = norms_out;
+        D.n = S.n;
+        D.k = S.k;
+        D.kpad = pow2_at_least(S.k);
+        D.tiles = ceil_div(S.n, kTileRows);
+        const int di = (int)descs.size();
+        for (int t = 0; t < D.tiles; ++t) tiles.push_back(ReplaceTile{di, t * kTileRows, std::min(S.n, (t + 1) * kTileRows), t});
+        descs.push_back(D);
+        sys.push_back(&S);
+        ++S.replacements;
+    }
+
+    int run(hipStream_t s) {
+        if (descs.empty()) return TG_OK;
+        if (descs.size() > desc_cap) {
+            if (ddesc) TG_HIP(hipFreeAsync(ddesc, s));
+            desc_cap = std::max(descs.size(), 2 * desc_cap);
+            TG_HIP(hipMallocAsync((void**)&ddesc, desc_cap * sizeof(ReplaceDesc), s));
+        }
+        if (tiles.size() > tile_cap) {
+            if (dtile) TG_HIP(hipFreeAsync(dtile, s));
+            tile_cap = std::max(tiles.size(), 2 * tile_cap);
+            TG_HIP(hipMallocAsync((void**)&dtile, tile_cap * sizeof(ReplaceTile), s));
+        }
+        TG_HIP(hipMemcpyAsync(ddesc, descs.data(), descs.size() * sizeof(ReplaceDesc), hipMemcpyHostToDevice, s));
+        TG_HIP(hipMemcpyAsync(dtile, tiles.data(), tiles.size() * sizeof(ReplaceTile), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(bm_replace_many_kernel, dim3((unsigned)tiles.size()), dim3(kBlock), 0, s,
+                           (const ReplaceTile*)dtile, (const ReplaceDesc*)ddesc);
+        hipLaunchKernelGGL(bm_flush_many_kernel, dim3((unsigned)tiles.size()), dim3(kBlock), 0, s,
+                           (const ReplaceTile*)dtile, (const ReplaceDesc*)ddesc);
+        hipLaunchKernelGGL(bm_norm_many_kernel, dim3((unsigned)descs.size()), dim3(kBlock), 0, s,
+                           (const ReplaceDesc*)ddesc);
+        TG_LAUNCH_CHECK();
+        return TG_OK;
+    }
+
+    void clear() {
+        descs.clear();
+        tiles.clear();
+        sys.clear();
+    }
+
+    void release(hipStream_t s) {
+        if (ddesc) (void)hipFreeAsync(ddesc, s);
+        if (dtile) (void)hipFreeAsync(dtile, s);
+        ddesc = nullptr;
+        dtile = nullptr;
+    }
+};
+
 struct Batch {
     hipStream_t s;
     CGBatch<double> b64;
@@ -695,9 +818,11 @@ struct Batch {
     double* dnorms = nullptr;  // kBatchMaxK true-residual norms per FP32-engine slot
     int dnorms_cap = 0;
     std::vector<double> hnorms;
+    ReplaceBatch rb;
     explicit Batch(hipStream_t st) : s(st), b64(st), b32(st) {}
     ~Batch() {
         if (dnorms) (void)hipFreeAsync(dnorms, s);
+        rb.release(s);
     }
 };
 
@@ -714,20 +839,6 @@ static int finish(Batch& B, BatchSystem& S, int succeeded, int iterations) {
 }
 
 static int grid_of(int64_t c) { return (int)std::max<int64_t>(1, std::min<int64_t>((c + kBlock - 1) / kBlock, 8192)); }
-
-// Replaces a mixed system's recursive residual by the true one (launches only);
-// its true residual norms land in B.dnorms[slot * kBatchMaxK + c].
-static void mixed_replace(Batch& B, BatchSystem& S, int it) {
-    const SysDev<float>& E = B.b32.desc(S.slot);
-    const int tiles = ceil_div(S.n, kTileRows), kpad = pow2_at_least(S.k);
-    hipLaunchKernelGGL(bm_replace_kernel, dim3(tiles), dim3(kBlock), 0, B.s, S.n, S.k, kpad, S.rp, S.ci, S.val64, S.b64,
-                       (const double*)S.x64, (const float*)S.x32, (const float*)S.d32, E.r, E.prz[it & 1], S.pnorm);
-    hipLaunchKernelGGL(bm_flush_kernel, dim3(grid_of((int64_t)S.n * S.k)), dim3(kBlock), 0, B.s, (int64_t)S.n * S.k,
-                       S.x32, S.x64);
-    hipLaunchKernelGGL(bm_norm_total_kernel, dim3(1), dim3(kBlock), 0, B.s, tiles, S.k, kpad, (const double*)S.pnorm,
-                       B.dnorms + (size_t)S.slot * kBatchMaxK);
-    ++S.replacements;
-}
 
 static int mixed_start(Batch& B, BatchSystem& S) {
     const size_t N = (size_t)S.n * S.k;
@@ -775,7 +886,6 @@ static int batch_advance(Batch& B, int iters) {
     rc = B.b32.step(iters);
     if (rc) return rc;
     std::vector<SysState> s64, s32;
-    std::vector<BatchSystem*> replaced;
     rc = B.b64.poll(s64);
     if (rc) return rc;
     rc = B.b32.poll(s32);
@@ -805,22 +915,29 @@ static int batch_advance(Batch& B, int iters) {
             if (rc) return rc;
             continue;
         }
-        if (!st.done) continue;  // decisions only at the system's own checkpoints / stops
+        // Decisions only at the system's own checkpoints (every kCheckEvery iterations:
+        // paused there by the device, or standing exactly there at this poll) and stops.
+        const bool stopped = st.done && !st.paused;
+        const bool at_check = st.done ? (bool)st.paused
+                                      : (st.it_b > 0 && st.it_b % kCheckEvery == 0 && st.it_b != st.resumed);
+        if (!at_check && !stopped) continue;
         double est = 0.0;
         for (int c = 0; c < S.k; ++c) est = std::max(est, st.bsq[c] > 0 ? std::sqrt(std::fabs(st.rho[c]) / st.bsq[c]) : 0.0);
-        if (st.paused && !(est < kReplaceDrop * S.est_last)) {
-            B.b32.resume(S.slot, 1.0);
+        if (!stopped && !(est < kReplaceDrop * S.est_last)) {
+            B.b32.resume(S.slot, st.it_b, 1.0);
             continue;
         }
-        mixed_replace(B, S, st.it_b);
+        B.rb.add(B.b32.desc(S.slot), B.dnorms + (size_t)S.slot * kBatchMaxK, S, st.it_b);
         S.est_last = est;
-        replaced.push_back(&S);
     }
-    if (replaced.empty()) return TG_OK;
-    TG_LAUNCH_CHECK();
+    if (B.rb.sys.empty()) return TG_OK;
+    rc = B.rb.run(B.s);
+    if (rc) return rc;
     B.hnorms.resize((size_t)B.dnorms_cap * kBatchMaxK);
     TG_HIP(hipMemcpyAsync(B.hnorms.data(), B.dnorms, B.hnorms.size() * sizeof(double), hipMemcpyDeviceToHost, B.s));
     TG_HIP(hipStreamSynchronize(B.s));
+    std::vector<BatchSystem*> replaced = B.rb.sys;
+    B.rb.clear();
     for (BatchSystem* Sp : replaced) {
         BatchSystem& S = *Sp;
         const SysState& st = s32[S.slot];
@@ -836,12 +953,12 @@ static int batch_advance(Batch& B, int iters) {
         } else if (st.it_b >= S.max_iter || S.replacements >= kMaxReplacements || std::isnan(worst)) {
             B.b32.park(S.slot);
             rc = finish(B, S, 0, st.it_b);
-        } else if (!st.paused) {
+        } else if (st.done && !st.paused) {
             // the FP32 estimate met the tolerance but the true residual did not: tighten it
             const double f = S.tol / worst;
-            B.b32.resume(S.slot, std::min(0.25, std::max(1e-4, 0.25 * f * f)));
+            B.b32.resume(S.slot, st.it_b, std::min(0.25, std::max(1e-4, 0.25 * f * f)));
         } else {
-            B.b32.resume(S.slot, 1.0);
+            B.b32.resume(S.slot, st.it_b, 1.0);
         }
         if (rc) return rc;
     }

@@ -464,6 +464,7 @@ extern "C" int tg_cg_solve(int32_t n, int64_t nnz, const void* rp, tg_dtype rp_t
     Aop.csr = &A;
     OpRef Pop;
     hipStream_t s = as_stream(stream);
+    if (spmm_may_slab(A, nrhs)) A.band = csr_band(A, s);
     if (val_type == TG_F64)
         return cg_entry<double>(A, Aop, prec_kind, prec_diag, Pop, nrhs, max_iter, a_coef, B, ldB, b_coef, X, ldX, tol,
                                 use_guess, workspace, workspace_bytes, succeeded, iterations, rel_residuals, s);

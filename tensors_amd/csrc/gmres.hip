@@ -759,6 +759,7 @@ extern "C" int tg_gmres_solve(int32_t n, int64_t nnz, const void* rp, tg_dtype r
     Aop.csr = &A;
     OpRef Pop;
     hipStream_t s = as_stream(stream);
+    if (spmm_may_slab(A, nrhs)) A.band = csr_band(A, s);
     if (val_type == TG_F64)
         return gmres_entry<double>(A, Aop, prec_kind, prec_diag, Pop, side, nrhs, max_iter, a_coef, B, ldB, b_coef, X,
                                    ldX, tol, max_restarts, use_guess, workspace, workspace_bytes, succeeded,

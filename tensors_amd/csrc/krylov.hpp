@@ -16,7 +16,15 @@ struct CsrView {
     const int32_t* ci = nullptr;
     const void* val = nullptr;  // null: binary matrix
     tg_dtype val_type = TG_F64;
+    int64_t band = -1;          // max |column - row| if known (csr_band), else -1
 };
+
+// Bandwidth max |ci - row| of a device CSR (one pass over ci; synchronises s).
+// Returns -1 if it could not be computed.
+int64_t csr_band(const CsrView& A, hipStream_t s);
+// Whether spmm_dispatch would cut X into column slabs for this shape, so that
+// knowing the band can matter.
+bool spmm_may_slab(const CsrView& A, int32_t nrhs);
 
 // Y <- alpha A X + beta Y with the SpMM kernels; when `skip` is non-null and
 // *skip != 0 at kernel start, the kernels return without touching Y.

@@ -306,6 +306,7 @@ extern "C" int tg_cg_solve_mixed(int32_t n, int64_t nnz, const void* rp, tg_dtyp
     A.ci = ci;
     A.val = a64;
     A.val_type = TG_F64;
+    if (spmm_may_slab(A, nrhs)) A.band = csr_band(A, as_stream(stream));
     MixedResult res{};
     const int rc = cg_mixed(A, a32, prec_kind, prec_diag, nrhs, max_outer, max_inner, inner_tol, a_coef, B, ldB,
                             b_coef, X, ldX, tol, use_guess, workspace, workspace_bytes, &res, as_stream(stream));

@@ -369,7 +369,9 @@ constexpr int kRowsPerTeam = 8;
 // lanes per row of the nrhs == 1 kernels, and lanes per RHS tile of spmm_wave.
 static int g_spmv_team = 0;
 static int g_spmm_group = 0;
-static int g_spmm_slices = 0;  // planned SpMM: 0 = by X size, 1 = never sliced, 2/4/8 = that many slices
+// planned SpMM: 0 = by X size, 1 = never sliced (default: the sliced kernel measured
+// slower than the plain ones on the 1e6-row random matrix), 2/4/8 = that many slices
+static int g_spmm_slices = 1;
 
 static int spmv_team_width(int32_t m, int64_t nnz) {
     if (g_spmv_team) return g_spmv_team;
@@ -783,6 +785,50 @@ namespace tg {
 // bytes of X (A is re-streamed once per tile).
 constexpr int64_t kXTileBytes = 128ll << 20;
 
+// Per-block maxima of |ci - row| (no atomics; the host takes the max of the blocks).
+template <typename RP>
+__global__ __launch_bounds__(kBlock) void band_kernel(int m, const RP* __restrict__ rp, const int32_t* __restrict__ ci,
+                                                      int32_t* __restrict__ block_max) {
+    __shared__ int32_t sh[kBlock];
+    int32_t b = 0;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < m; i += gridDim.x * kBlock) {
+        const RP lb = rp[i], le = rp[i + 1];
+        if (le > lb) {  // columns need not be sorted: look at all of them
+            for (RP l = lb; l < le; ++l) b = max(b, abs(ci[l] - i));
+        }
+    }
+    sh[threadIdx.x] = b;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) sh[threadIdx.x] = max(sh[threadIdx.x], sh[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) block_max[blockIdx.x] = sh[0];
+}
+
+int64_t csr_band(const CsrView& A, hipStream_t s) {
+    if (A.m <= 0 || A.nnz <= 0 || !A.ci) return 0;
+    const int grid = (int)std::min<int64_t>((A.m + kBlock - 1) / kBlock, 1024);
+    int32_t* d = nullptr;
+    if (hipMallocAsync((void**)&d, grid * sizeof(int32_t), s) != hipSuccess) return -1;
+    if (A.rp_type == TG_I32)
+        hipLaunchKernelGGL((band_kernel<int32_t>), dim3(grid), dim3(kBlock), 0, s, A.m, (const int32_t*)A.rp, A.ci, d);
+    else
+        hipLaunchKernelGGL((band_kernel<int64_t>), dim3(grid), dim3(kBlock), 0, s, A.m, (const int64_t*)A.rp, A.ci, d);
+    std::vector<int32_t> h(grid, 0);
+    const bool ok = hipMemcpyAsync(h.data(), d, grid * sizeof(int32_t), hipMemcpyDeviceToHost, s) == hipSuccess;
+    (void)hipFreeAsync(d, s);
+    if (!ok || hipStreamSynchronize(s) != hipSuccess) return -1;
+    int64_t b = 0;
+    for (int32_t v : h) b = std::max<int64_t>(b, v);
+    return b;
+}
+
+bool spmm_may_slab(const CsrView& A, int32_t nrhs) {
+    const int64_t vs = A.val_type == TG_F64 ? 8 : 4;
+    return (int64_t)A.n * nrhs * vs > kXTileBytes && nrhs > 8;
+}
+
 int spmm_dispatch(const CsrView& A, double alpha, const void* X, int64_t ldX, double beta, void* Y, int64_t ldY,
                   int32_t nrhs, const int32_t* job_ptr, int32_t blocks, const int* skip, hipStream_t st) {
     const int32_t m = A.m, n = A.n;
@@ -802,7 +848,11 @@ int spmm_dispatch(const CsrView& A, double alpha, const void* X, int64_t ldX, do
     if (A.nnz > 0 && !A.ci) return fail(TG_ERR_INVALID, "spmm: ci is null");
     const size_t vs = f64 ? 8 : 4;
     int32_t kt = nrhs;
-    if ((int64_t)n * nrhs * (int64_t)vs > kXTileBytes && nrhs > 8) {
+    // Column slabs keep a random-column X resident in the 256 MB Infinity Cache; a
+    // banded matrix (known band) only touches a window of X around the current
+    // rows, so there they would just re-read A once per slab.
+    const bool banded = A.band >= 0 && (2 * A.band + 1) * (int64_t)nrhs * (int64_t)vs <= kXTileBytes / 2;
+    if ((int64_t)n * nrhs * (int64_t)vs > kXTileBytes && nrhs > 8 && !banded) {
         kt = 8;
         while ((int64_t)n * (2 * kt) * (int64_t)vs <= kXTileBytes && 2 * kt < nrhs) kt *= 2;
     }
@@ -850,6 +900,7 @@ struct tg_csr_plan_s {
     const int32_t* ci = nullptr;
     int32_t* cuts = nullptr;  // kCuts x m row offsets of the 8-way column cut
     int sorted = 0;
+    int64_t band = -1;        // max |column - row|
     void* part = nullptr;     // partial blocks of the sliced path (grown on demand)
     size_t part_bytes = 0;
 };
@@ -892,6 +943,14 @@ extern "C" int tg_csr_plan_create(int32_t m, int32_t n, int64_t nnz, const void*
             return fail(TG_ERR_HIP, "tg_csr_plan_create: plan kernel failed");
         }
         P->sorted = unsorted ? 0 : 1;
+        CsrView V;
+        V.m = m;
+        V.n = n;
+        V.nnz = nnz;
+        V.rp = rp;
+        V.rp_type = rp_type;
+        V.ci = ci;
+        P->band = csr_band(V, st);
     }
     *out = P;
     return TG_OK;
@@ -928,6 +987,7 @@ extern "C" int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_
     A.ci = plan->ci;
     A.val = a;
     A.val_type = val_type;
+    A.band = plan->band;
     hipStream_t st = as_stream(stream);
     const int S = (val_type == TG_F64 || val_type == TG_F32) ? tg_csr_plan_slices(plan, val_type, nrhs) : 0;
     // the plain path covers every special case (alpha = 0, aliasing, bad arguments)

@@ -45,7 +45,7 @@ def test_tune_spmm_keeps_best_and_restores(cuda):
     before = autotune.get_knobs()
     best, table = autotune.tune_spmm(A, X, Y, reps=2, keep=False)
     assert autotune.get_knobs() == before
-    assert len(table) == len(autotune.candidates(4)) and all(t > 0 for _, t in table)
+    assert len(table) == len(autotune.candidates(4)) + 3 and all(t > 0 for _, t in table)
     best, _ = autotune.tune_spmm(A, X, Y, reps=2, keep=True)
     assert all(autotune.get_knobs()[k] == v for k, v in best.items())
     autotune.set_knobs(before)
