@@ -54,11 +54,14 @@ def parse():
     ap.add_argument("--no-cg", action="store_true", help="skip the CG (configs 3) section")
     ap.add_argument("--cg-grid", type=int, default=128)
     ap.add_argument("--cg-reps", type=int, default=2)
+    ap.add_argument("--cpu-cg-iters", type=int, default=2, help="oracle CG iterations timed on the host")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven CG section")
     ap.add_argument("--batch-systems", type=int, default=2000)
     ap.add_argument("--batch-iters", type=int, default=16, help="CG iterations enqueued per poll")
     ap.add_argument("--batch-seq-sample", type=int, default=32,
                     help="systems solved one at a time by the drop-in FP64 CG for comparison")
+    ap.add_argument("--slices", type=int, default=-1,
+                    help="column slices of the planned SpMM (tg_tune_set spmm_slices; -1 = library default)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop section")
     ap.add_argument("--serve-requests", type=int, default=160)
     ap.add_argument("--serve-rates", type=str, default="0.25,0.5,1,2",
@@ -116,6 +119,25 @@ def cpu_baseline(rp, ci, v64, v32, X64, X32, budget_s):
         "sample": f"the same 8 SpMMs (FP64/FP32 x nrhs 1,4,16,64) on the full n=1e6 matrix, "
                   f"{reps_done} timed calls in ~{budget_s:.0f}s, fixed-NRHS path, -O3 -ffast-math",
     }
+
+
+def cpu_cg_sample(g, k, iters):
+    """The oracle CG (reference release flags, all host cores) on the CG section's system
+    for a fixed number of iterations; returns seconds per iteration."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle  # test-side checker; here only as the timed CPU baseline
+
+    from tensors_amd import generators as gen
+
+    threads = min(16, os.cpu_count() or 1)
+    rp, ci, val = gen.laplacian_3d(g, 1e-2)
+    n = g ** 3
+    B = gen.uniform(3003, n * k).reshape(n, k)
+    X = np.zeros((n, k))
+    oracle.cg(n, rp, ci, val, B, X, k, 1, 1e-30, threads=threads, fast=True)  # warm-up
+    t0 = time.perf_counter()
+    oracle.cg(n, rp, ci, val, B, X, k, iters, 1e-30, threads=threads, fast=True)
+    return (time.perf_counter() - t0) / iters, threads
 
 
 def cg_section(args, dev):
@@ -188,7 +210,18 @@ def cg_section(args, dev):
                   "latencies_ms": [round(x * 1e3, 2) for x in latm], "outer": outer, "inner_iterations": inner,
                   "succeeded": bool(okm), "true_rel_residual": resm, "error_vs_fp64": err},
         "speedup_mixed_over_fp64": round(t64 / tm, 3),
+        "cpu": cpu_cg_part(args, g, k, it64),
     }
+
+
+def cpu_cg_part(args, g, k, it64):
+    if args.no_cpu:
+        return None
+    per_it, threads = cpu_cg_sample(g, k, args.cpu_cg_iters)
+    return {"seconds_per_iteration": round(per_it, 4), "cores": threads, "kind": "port",
+            "est_solve_s": round(per_it * it64, 2), "est_solves_per_s": round(1.0 / (per_it * it64), 4),
+            "sample": f"oracle CG (FP64, P = I, -O3 -ffast-math) on the same system, {args.cpu_cg_iters} iterations "
+                      f"timed, solve time extrapolated to the GPU's {it64} iterations"}
 
 
 def batch_specs(count, seed=4004):
@@ -491,6 +524,8 @@ def main():
                           "bytes": alg_bytes(nnz, M, N, k, sv)})
     stream = torch.cuda.current_stream()
     # the path MatrixCSR.Dot takes: tg_csr_spmm_plan (column slices where X is a few L2s)
+    if args.slices >= 0:
+        capi.call("tg_tune_set", b"spmm_slices", args.slices)
     plan = SlicePlan(M, N, nnz, rp32, cid)
     for c in cases:
         c["slices"] = plan.slices(c["Y"].dtype, c["k"])

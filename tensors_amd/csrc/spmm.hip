@@ -580,40 +580,133 @@ __global__ __launch_bounds__(kBlock) void slice_plan_kernel(int m, int n, const 
     }
 }
 
-template <typename T, typename RP, int TEAM, int K>
-__global__ __launch_bounds__(kBlock) void spmm_slice_kernel(int m, int S, const RP* __restrict__ rp,
-                                                            const int32_t* __restrict__ cuts,
-                                                            const int32_t* __restrict__ ci, const T* __restrict__ a,
-                                                            const T* __restrict__ X, int64_t ldX, T* __restrict__ part,
-                                                            int rows_per_block) {
+// Exclusive scan of v[0..n) in place into v[0..n] (v[n] = total), n <= 4 * kBlock,
+// by one workgroup: 4 entries per thread, a wave scan of the thread totals, then
+// the wave totals.  All threads must call it.
+__device__ __forceinline__ void block_exclusive_scan(int* v, int n) {
+    __shared__ int wsum[kBlock / kWave];
+    const int t = threadIdx.x, base = 4 * t;
+    int loc[4], tot = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        loc[q] = base + q < n ? v[base + q] : 0;
+        tot += loc[q];
+    }
+    // inclusive scan of the thread totals within the wave
+    const int lane = t % kWave, w = t / kWave;
+    int inc = tot;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int o = __shfl_up(inc, off, kWave);
+        if (lane >= off) inc += o;
+    }
+    if (lane == kWave - 1) wsum[w] = inc;
+    __syncthreads();
+    int wpre = 0;
+    for (int q = 0; q < w; ++q) wpre += wsum[q];
+    int run = wpre + inc - tot;  // exclusive prefix of this thread
+    __syncthreads();             // everyone has read v before it is overwritten
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (base + q < n) v[base + q] = run;
+        run += loc[q];
+    }
+    if (t == kBlock - 1) v[n] = wpre + inc;  // last thread holds the block total
+    __syncthreads();
+}
+
+// Streaming form of the sliced product: the workgroup's rows' runs in its column
+// range are laid end to end (exclusive scan of the run lengths in LDS); every lane
+// takes ELEMS positions of that sequence (consecutive lanes, consecutive positions,
+// so loads of a run coalesce), finds its row by binary search in LDS, gathers the
+// X row and leaves the K products in LDS; teams of TEAM lanes then add each row's
+// products (rows spanning chunks accumulate in LDS).
+template <typename T, typename RP, int ELEMS, int TEAM, int K>
+__global__ __launch_bounds__(kBlock) void spmm_slice_stream_kernel(int m, int S, const RP* __restrict__ rp,
+                                                                   const int32_t* __restrict__ cuts,
+                                                                   const int32_t* __restrict__ ci,
+                                                                   const T* __restrict__ a, const T* __restrict__ X,
+                                                                   int64_t ldX, T* __restrict__ part,
+                                                                   int rows_per_block) {
+    constexpr int CAP = kBlock * ELEMS;
+    constexpr int RMAX = 1024 / K;
     constexpr int TEAMS = kBlock / TEAM;
+    __shared__ T prod[CAP * K];
+    __shared__ int soff[RMAX + 1];
+    __shared__ RP slo[RMAX];
+    __shared__ T racc[RMAX * K];
     const int b = blockIdx.x, xcd = b % kXcds;
-    const int s = xcd % S, per = kXcds / S;
+    const int s = xcd % S, per = kXcds / S, step = kXcds / S;
     const int rb = (b / kXcds) * per + xcd / S;
     const int r0 = rb * rows_per_block, r1 = min(m, r0 + rows_per_block);
-    const int step = kXcds / S;  // cut index stride of this S
     const int lt = threadIdx.x % TEAM, team = threadIdx.x / TEAM;
-    for (int row = r0 + team; row < r1; row += TEAMS) {
-        const RP base = rp[row];
-        const RP lb = s == 0 ? base : base + cuts[(int64_t)(s * step - 1) * m + row];
-        const RP le = s == S - 1 ? rp[row + 1] : base + cuts[(int64_t)((s + 1) * step - 1) * m + row];
-        T acc[K];
-#pragma unroll
-        for (int q = 0; q < K; ++q) acc[q] = T(0);
-        for (RP l = lb + lt; l < le; l += TEAM) {
-            const int32_t c = __builtin_nontemporal_load(ci + l);
-            const T av = a ? __builtin_nontemporal_load(a + l) : T(1);
-            const T* xr = X + (int64_t)c * ldX;
-#pragma unroll
-            for (int q = 0; q < K; ++q) acc[q] += av * xr[q];
+    for (int rs = r0; rs < r1; rs += RMAX) {
+        const int nr = min(RMAX, r1 - rs);
+        for (int i = threadIdx.x; i < nr; i += kBlock) {
+            const int row = rs + i;
+            const RP base = rp[row];
+            const RP lb = s == 0 ? base : base + cuts[(int64_t)(s * step - 1) * m + row];
+            const RP le = s == S - 1 ? rp[row + 1] : base + cuts[(int64_t)((s + 1) * step - 1) * m + row];
+            slo[i] = lb;
+            soff[i] = (int)(le - lb);
         }
+        for (int i = threadIdx.x; i < nr * K; i += kBlock) racc[i] = T(0);
+        __syncthreads();
+        block_exclusive_scan(soff, nr);
+        const int total = soff[nr];
+        for (int cs = 0; cs < total; cs += CAP) {
+            const int cnt = min(CAP, total - cs);
+            int32_t c[ELEMS];
+            T av[ELEMS];
 #pragma unroll
-        for (int q = 0; q < K; ++q) acc[q] = group_sum<TEAM>(acc[q]);
-        if (lt == 0) {
-            T* pr = part + ((int64_t)s * m + row) * K;
+            for (int j = 0; j < ELEMS; ++j) {
+                const int e = threadIdx.x + j * kBlock;
+                c[j] = 0;
+                av[j] = T(0);
+                if (e < cnt) {
+                    const int v = cs + e;
+                    int lo = 0, hi = nr - 1;  // last row with soff <= v
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (soff[mid] <= v)
+                            lo = mid;
+                        else
+                            hi = mid - 1;
+                    }
+                    const RP l = slo[lo] + (v - soff[lo]);
+                    c[j] = __builtin_nontemporal_load(ci + l);
+                    av[j] = a ? __builtin_nontemporal_load(a + l) : T(1);
+                }
+            }
 #pragma unroll
-            for (int q = 0; q < K; ++q) pr[q] = acc[q];
+            for (int j = 0; j < ELEMS; ++j) {
+                const int e = threadIdx.x + j * kBlock;
+                if (e < cnt) {
+                    const T* xr = X + (int64_t)c[j] * ldX;
+#pragma unroll
+                    for (int q = 0; q < K; ++q) prod[e * K + q] = av[j] * xr[q];
+                }
+            }
+            __syncthreads();
+            for (int i = team; i < nr; i += TEAMS) {
+                const int lb = max(soff[i], cs), le = min(soff[i + 1], cs + cnt);
+                if (le <= lb) continue;  // team-uniform
+                T sum[K];
+#pragma unroll
+                for (int q = 0; q < K; ++q) sum[q] = T(0);
+                for (int l = lb + lt; l < le; l += TEAM)
+#pragma unroll
+                    for (int q = 0; q < K; ++q) sum[q] += prod[(l - cs) * K + q];
+#pragma unroll
+                for (int q = 0; q < K; ++q) {
+                    sum[q] = group_sum<TEAM>(sum[q]);
+                    if (lt == 0) racc[i * K + q] += sum[q];
+                }
+            }
+            __syncthreads();
         }
+        for (int i = threadIdx.x; i < nr * K; i += kBlock) part[((int64_t)s * m + rs) * K + i] = racc[i];
+        __syncthreads();
     }
 }
 
@@ -648,38 +741,43 @@ static int slice_count(int64_t xbytes) {
     return S;
 }
 
-template <typename T, typename RP, int TEAM>
-static void launch_slice_team(int S, int K, int grid, int m, const RP* rp, const int32_t* cuts, const int32_t* ci,
-                              const T* a, const T* X, int64_t ldX, T* part, int rpb, hipStream_t st) {
-#define TG_SLICE(KK)                                                                                         \
-    case KK:                                                                                                 \
-        hipLaunchKernelGGL((spmm_slice_kernel<T, RP, TEAM, KK>), dim3(grid), dim3(kBlock), 0, st, m, S, rp, cuts, \
-                           ci, a, X, ldX, part, rpb);                                                        \
+template <typename T, typename RP, int K>
+static void launch_slice_stream(int S, int team, int grid, int m, const RP* rp, const int32_t* cuts, const int32_t* ci,
+                                const T* a, const T* X, int64_t ldX, T* part, int rpb, hipStream_t st) {
+    // ELEMS keeps the product stage at 2048 entries (16 KB in FP64)
+    constexpr int E = K == 1 ? 8 : K == 2 ? 4 : 2;
+#define TG_SLS(TM)                                                                                            \
+    case TM:                                                                                                  \
+        hipLaunchKernelGGL((spmm_slice_stream_kernel<T, RP, E, TM, K>), dim3(grid), dim3(kBlock), 0, st, m, S, rp, \
+                           cuts, ci, a, X, ldX, part, rpb);                                                   \
         break;
-    switch (K) {
-        TG_SLICE(1)
-        TG_SLICE(2)
-        TG_SLICE(3)
-        TG_SLICE(4)
+    switch (team) {
+        TG_SLS(2)
+        TG_SLS(4)
+        TG_SLS(8)
+        default:
+            TG_SLS(16)
     }
-#undef TG_SLICE
+#undef TG_SLS
 }
 
 template <typename T, typename RP>
 static int spmm_sliced_typed(int S, int32_t m, int64_t nnz, const RP* rp, const int32_t* cuts, const int32_t* ci,
                              const T* a, double alpha_d, const T* X, int64_t ldX, double beta_d, T* Y, int64_t ldY,
                              int32_t K, T* part, hipStream_t st) {
-    const double avg = m > 0 ? (double)nnz / m / S : 0.0;
-    const int team = std::min(32, std::max(4, pow2ceil((int)std::ceil(avg))));
-    const int rpb = (kBlock / team) * kRowsPerTeam;
+    const double avg = m > 0 ? (double)nnz / m / S : 0.0;  // mean run length per row and slice
+    const int team = std::min(16, std::max(2, pow2ceil((int)std::ceil(avg))));
+    const int cap = kBlock * (K == 1 ? 8 : K == 2 ? 4 : 2);
+    const int rmax = 1024 / (K == 3 ? 4 : K);
+    const int rpb = std::max(1, std::min(rmax, (int)(cap / (avg + 1.0))));
     const int row_blocks = (m + rpb - 1) / rpb;
     const int per = kXcds / S;
     const int grid = kXcds * ((row_blocks + per - 1) / per);
-    switch (team) {
-        case 4: launch_slice_team<T, RP, 4>(S, K, grid, m, rp, cuts, ci, a, X, ldX, part, rpb, st); break;
-        case 8: launch_slice_team<T, RP, 8>(S, K, grid, m, rp, cuts, ci, a, X, ldX, part, rpb, st); break;
-        case 16: launch_slice_team<T, RP, 16>(S, K, grid, m, rp, cuts, ci, a, X, ldX, part, rpb, st); break;
-        default: launch_slice_team<T, RP, 32>(S, K, grid, m, rp, cuts, ci, a, X, ldX, part, rpb, st); break;
+    switch (K) {
+        case 1: launch_slice_stream<T, RP, 1>(S, team, grid, m, rp, cuts, ci, a, X, ldX, part, rpb, st); break;
+        case 2: launch_slice_stream<T, RP, 2>(S, team, grid, m, rp, cuts, ci, a, X, ldX, part, rpb, st); break;
+        case 3: launch_slice_stream<T, RP, 3>(S, team, grid, m, rp, cuts, ci, a, X, ldX, part, rpb, st); break;
+        default: launch_slice_stream<T, RP, 4>(S, team, grid, m, rp, cuts, ci, a, X, ldX, part, rpb, st); break;
     }
     const int alpha_one = (alpha_d == 1.0);
     const int beta_mode = (beta_d == 0.0) ? kBetaZero : (beta_d == 1.0) ? kBetaOne : kBetaGeneric;
