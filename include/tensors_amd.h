@@ -115,6 +115,9 @@ int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, doubl
  *   "spmm_group":  lanes per RHS tile of the nrhs >= 2 kernel (a wave's 64/G lane
  *                  groups split each row's nonzeros), 0 = enough lanes for all
  *                  columns, else a power of two in [1, 64].
+ *   "spmm_rows":   nrhs >= 2 kernel, 0 = from the mean row length (up to 16:
+ *                  row per lane group), 1 = the wave's lane groups split one row,
+ *                  2 = each lane group owns a row.
  *   "spmm_slices": column slices of tg_csr_spmm_plan, 0 = from the size of X,
  *                  1 = never (default), 2 / 4 / 8 = always that many (nrhs <= 4).
  * Process-wide; not thread-safe against concurrent SpMM calls. */

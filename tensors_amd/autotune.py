@@ -3,8 +3,9 @@
 The kernels take their shapes from process-wide knobs (tg_tune_set):
   nrhs == 1: "spmv_kernel" (0 streaming, 1 team-per-row) and "spmv_team" (lanes per
              row, 4..64; the streaming kernel caps it at 32);
-  nrhs >= 2: "spmm_group" (lanes per RHS tile, 1..64; a wave's 64/G lane groups split
-             each row's nonzeros, and columns beyond G*V are done in further tiles).
+  nrhs >= 2: "spmm_group" (lanes per RHS tile, 1..64; columns beyond G*V are done in
+             further tiles) and "spmm_rows" (1: the wave's 64/G lane groups split one
+             row's nonzeros; 2: each lane group owns its own row);
 0 means "pick from the matrix" (mean row length, nrhs).  tune_spmm times every
 candidate on the caller's own matrix and block with HIP events on the launch stream,
 keeps the fastest, and leaves it set (or restores the previous setting).  Results
@@ -21,7 +22,7 @@ import torch
 from . import _capi as capi
 from .sparse import MatrixCSR
 
-KNOBS = ("spmv_kernel", "spmv_team", "spmm_group", "spmm_slices")
+KNOBS = ("spmv_kernel", "spmv_team", "spmm_group", "spmm_rows", "spmm_slices")
 
 
 def get_knobs():
@@ -42,7 +43,11 @@ def candidates(nrhs):
     gmax = 1
     while gmax < nrhs and gmax < 64:
         gmax *= 2
-    return [{"spmm_group": 0}] + [{"spmm_group": g} for g in (1, 2, 4, 8, 16, 32, 64) if g <= gmax]
+    out = [{"spmm_group": 0, "spmm_rows": 0}]
+    for rows in (1, 2):  # row per wave / row per lane group
+        out += [{"spmm_group": g, "spmm_rows": rows} for g in (1, 2, 4, 8, 16, 32, 64)
+                if g <= gmax and not (rows == 2 and g == 64)]
+    return out
 
 
 def slice_candidates(nrhs):

@@ -206,19 +206,45 @@ __global__ __launch_bounds__(kBlock) void bk_dir_spmm_kernel(const Tile* __restr
         const int32_t* __restrict__ ci = S.ci;
         const T* __restrict__ val = S.val;
         const int k = S.k;
-        for (int i = tl.r0 + ro; i < tl.r1; i += rows) {
-            // A row's nonzeros in chunks of kGather: all index/value loads of a chunk,
-            // then all its gathers, are issued before any is used (memory-level
-            // parallelism); the sum still runs in nonzero order.
-            const int lb = rp[i], le = rp[i + 1];
+        // Rows are software-pipelined: while row i's gathers are in flight, the row
+        // bounds of row i + 2*rows and the first kGather column indices of row
+        // i + rows are being loaded, so each row costs about one memory round trip
+        // instead of three dependent ones (rp -> ci -> gathers).  Nonzeros past the
+        // first kGather of a row are loaded in place.
+        int i = tl.r0 + ro;
+        int lb0 = 0, le0 = 0, lb1 = 0, le1 = 0;
+        int32_t jj0[kGather];
+        if (i < tl.r1) {
+            lb0 = rp[i];
+            le0 = rp[i + 1];
+        }
+        if (i + rows < tl.r1) {
+            lb1 = rp[i + rows];
+            le1 = rp[i + rows + 1];
+        }
+#pragma unroll
+        for (int q = 0; q < kGather; ++q) jj0[q] = lb0 + q < le0 ? ci[lb0 + q] : i;
+        for (; i < tl.r1; i += rows) {
+            // prefetch: bounds two rows ahead, first indices one row ahead
+            const int i2 = i + 2 * rows;
+            int lb2 = 0, le2 = 0;
+            if (i2 < tl.r1) {
+                lb2 = rp[i2];
+                le2 = rp[i2 + 1];
+            }
+            int32_t jj1[kGather];
+            const int i1 = i + rows;
+#pragma unroll
+            for (int q = 0; q < kGather; ++q) jj1[q] = lb1 + q < le1 ? ci[lb1 + q] : i1;
+            // row i: first chunk from the prefetched indices, the rest in place
             T s = T(0);
-            for (int l0 = lb; l0 < le; l0 += kGather) {
+            for (int l0 = lb0; l0 < le0; l0 += kGather) {
                 int32_t jj[kGather];
                 T vv[kGather], pj[kGather];
 #pragma unroll
                 for (int q = 0; q < kGather; ++q) {
-                    const bool ok = l0 + q < le;
-                    jj[q] = ok ? ci[l0 + q] : i;
+                    const bool ok = l0 + q < le0;
+                    jj[q] = l0 == lb0 ? jj0[q] : (ok ? ci[l0 + q] : i);
                     vv[q] = ok ? val[l0 + q] : T(0);
                 }
 #pragma unroll
@@ -229,7 +255,7 @@ __global__ __launch_bounds__(kBlock) void bk_dir_spmm_kernel(const Tile* __restr
                 }
 #pragma unroll
                 for (int q = 0; q < kGather; ++q)
-                    if (l0 + q < le) s += vv[q] * pj[q];
+                    if (l0 + q < le0) s += vv[q] * pj[q];
             }
             const int64_t e = (int64_t)i * k + c;
             const T zi = d ? d[i] * r[e] : r[e];
@@ -237,6 +263,12 @@ __global__ __launch_bounds__(kBlock) void bk_dir_spmm_kernel(const Tile* __restr
             pnew[e] = pi;
             S.u[e] = s;
             acc += (double)pi * (double)s;
+            lb0 = lb1;
+            le0 = le1;
+            lb1 = lb2;
+            le1 = le2;
+#pragma unroll
+            for (int q = 0; q < kGather; ++q) jj0[q] = jj1[q];
         }
     }
     tile_partial(acc, S.k, S.kpad, S.ppu, tl.t);

@@ -304,6 +304,72 @@ __global__ __launch_bounds__(kBlock) void spmm_wave_kernel(
     }
 }
 
+// spmm_rows<G,V>: short rows (grid and FEM matrices, a few nonzeros per row) with
+// many right-hand sides.  Splitting one such row over a whole wave leaves most
+// lanes idle and makes every row a chain of dependent loads; here each of the
+// 64/G lane groups of a wave owns its own row: the group loads G (column, value)
+// pairs of the row at once, hands them out with cross-lane reads, and gathers the
+// X rows U at a time, so a wave keeps 64/G rows and 4 gathers per lane in flight.
+// A row's sum stays in its group (no butterfly); the summation order is the
+// nonzero order.
+template <typename T, typename RP, int G, int V>
+__global__ __launch_bounds__(kBlock) void spmm_rows_kernel(
+    int m, const RP* __restrict__ rp, const int32_t* __restrict__ ci, const T* __restrict__ a, T alpha,
+    const T* __restrict__ X, int64_t ldX, T beta, T* __restrict__ Y, int64_t ldY, int nrhs, int alpha_one,
+    int beta_mode, const int32_t* __restrict__ job_ptr, int rows_per_block, const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    constexpr int S = kWave / G;  // rows in flight per wave
+    constexpr int KT = G * V;     // columns per tile
+    constexpr int WAVES = kBlock / kWave;
+    constexpr int U = 4;
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    const int s = lane / G, g = lane % G;
+    int r0, r1;
+    block_rows(m, job_ptr, rows_per_block, r0, r1);
+    for (int rbase = r0 + wave * S; rbase < r1; rbase += WAVES * S) {
+        const int row = rbase + s;
+        const bool rok = row < r1;
+        const RP lb = rok ? rp[row] : RP(0), le = rok ? rp[row + 1] : RP(0);
+        const int len = (int)(le - lb);
+        int wmax = len;  // longest row of the wave (uniform loop bound)
+#pragma unroll
+        for (int off = G; off < kWave; off <<= 1) wmax = max(wmax, __shfl_xor(wmax, off, kWave));
+        for (int c0 = 0; c0 < nrhs; c0 += KT) {
+            const int col = c0 + g * V;
+            const bool colok = col < nrhs;
+            T acc[V];
+#pragma unroll
+            for (int i = 0; i < V; ++i) acc[i] = T(0);
+            for (int base = 0; base < wmax; base += G) {
+                const int e = base + g;
+                int32_t myc = 0;
+                T mya = T(0);
+                if (e < len) {
+                    myc = __builtin_nontemporal_load(ci + lb + e);
+                    mya = a ? __builtin_nontemporal_load(a + lb + e) : T(1);
+                }
+                const int cnt = min(G, len - base);  // this group's pairs in the chunk
+                const int wcnt = min(G, wmax - base);
+                for (int u0 = 0; u0 < wcnt; u0 += U) {
+#pragma unroll
+                    for (int uu = 0; uu < U; ++uu) {
+                        const int j = u0 + uu;
+                        const int src = s * G + (j < G ? j : 0);
+                        const int32_t c = __shfl(myc, src, kWave);
+                        const T av = __shfl(mya, src, kWave);
+                        const bool live = (j < cnt) && colok;
+                        T xv[V];
+                        load_vec<T, V>(X + (int64_t)(live ? c : 0) * ldX + (live ? col : 0), xv);
+#pragma unroll
+                        for (int i = 0; i < V; ++i) acc[i] += live ? av * xv[i] : T(0);
+                    }
+                }
+            }
+            if (rok && colok) epilogue<T, V>(Y + (int64_t)row * ldY + col, acc, le == lb, alpha, alpha_one, beta, beta_mode);
+        }
+    }
+}
+
 // Y <- beta * Y over m x nrhs (alpha == 0 path of SpMM.hpp:65-71 / SpMV.hpp:15-30)
 // and the general axpby used by the solvers.
 template <typename T>
@@ -369,6 +435,7 @@ constexpr int kRowsPerTeam = 8;
 // lanes per row of the nrhs == 1 kernels, and lanes per RHS tile of spmm_wave.
 static int g_spmv_team = 0;
 static int g_spmm_group = 0;
+static int g_spmm_rows = 0;  // nrhs >= 2 kernel: 0 = by mean row length, 1 = row per wave, 2 = row per lane group
 // planned SpMM: 0 = by X size, 1 = never sliced (default: the sliced kernel measured
 // slower than the plain ones on the 1e6-row random matrix), 2/4/8 = that many slices
 static int g_spmm_slices = 1;
@@ -458,9 +525,32 @@ static int launch_spmv(int32_t m, int64_t nnz, const RP* rp, const int32_t* ci, 
 }
 
 template <typename T, typename RP, int V>
-static int launch_spmm_v(int G, int32_t m, const RP* rp, const int32_t* ci, const T* a, T alpha, const T* X,
-                         int64_t ldX, T beta, T* Y, int64_t ldY, int32_t nrhs, int alpha_one, int beta_mode,
-                         const int32_t* job_ptr, int32_t blocks, const int* skip, hipStream_t st) {
+static int launch_spmm_v(int G, bool rows_mode, int32_t m, const RP* rp, const int32_t* ci, const T* a, T alpha,
+                         const T* X, int64_t ldX, T beta, T* Y, int64_t ldY, int32_t nrhs, int alpha_one,
+                         int beta_mode, const int32_t* job_ptr, int32_t blocks, const int* skip, hipStream_t st) {
+    if (rows_mode && G < kWave) {
+        // 64/G rows per wave at a time; kRowsPerTeam rounds per wave per block
+        const int rpb = (kBlock / G) * kRowsPerTeam;
+        const int grid = job_ptr ? blocks : (m + rpb - 1) / rpb;
+        if (grid <= 0) return TG_OK;
+#define TG_SPMMR(GG)                                                                                          \
+    case GG:                                                                                                  \
+        hipLaunchKernelGGL((spmm_rows_kernel<T, RP, GG, V>), dim3(grid), dim3(kBlock), 0, st, m, rp, ci, a, alpha, \
+                           X, ldX, beta, Y, ldY, nrhs, alpha_one, beta_mode, job_ptr, rpb, skip);             \
+        break;
+        switch (G) {
+            TG_SPMMR(1)
+            TG_SPMMR(2)
+            TG_SPMMR(4)
+            TG_SPMMR(8)
+            TG_SPMMR(16)
+            TG_SPMMR(32)
+            default: return fail(TG_ERR_INVALID, "internal: bad lane group %d", G);
+        }
+#undef TG_SPMMR
+        TG_LAUNCH_CHECK();
+        return TG_OK;
+    }
     const int rpb = (kBlock / kWave) * kRowsPerTeam;
     const int grid = job_ptr ? blocks : (m + rpb - 1) / rpb;
     if (grid <= 0) return TG_OK;
@@ -509,16 +599,19 @@ static int spmm_typed(int32_t m, int64_t nnz, const RP* rp, const int32_t* ci, c
             V = 2;
     }
     const int G = g_spmm_group ? g_spmm_group : std::min(64, pow2ceil((nrhs + V - 1) / V));
+    // short rows: one row per lane group (auto: mean row length up to 16)
+    const double avg = m > 0 ? (double)nnz / m : 0.0;
+    const bool rows_mode = g_spmm_rows == 2 || (g_spmm_rows == 0 && avg <= 16.0);
     switch (V) {
         case 1:
-            return launch_spmm_v<T, RP, 1>(G, m, rp, ci, a, alpha, X, ldX, beta, Y, ldY, nrhs, alpha_one,
+            return launch_spmm_v<T, RP, 1>(G, rows_mode, m, rp, ci, a, alpha, X, ldX, beta, Y, ldY, nrhs, alpha_one,
                                            beta_mode, job_ptr, blocks, skip, st);
         case 2:
-            return launch_spmm_v<T, RP, 2>(G, m, rp, ci, a, alpha, X, ldX, beta, Y, ldY, nrhs, alpha_one,
+            return launch_spmm_v<T, RP, 2>(G, rows_mode, m, rp, ci, a, alpha, X, ldX, beta, Y, ldY, nrhs, alpha_one,
                                            beta_mode, job_ptr, blocks, skip, st);
         default:
             if constexpr (sizeof(T) == 4)
-                return launch_spmm_v<T, RP, 4>(G, m, rp, ci, a, alpha, X, ldX, beta, Y, ldY, nrhs, alpha_one,
+                return launch_spmm_v<T, RP, 4>(G, rows_mode, m, rp, ci, a, alpha, X, ldX, beta, Y, ldY, nrhs, alpha_one,
                                                beta_mode, job_ptr, blocks, skip, st);
             return fail(TG_ERR_INVALID, "internal: bad vector width");
     }
@@ -824,6 +917,11 @@ extern "C" int tg_tune_set(const char* key, int32_t value) {
         g_spmm_group = value;
         return TG_OK;
     }
+    if (k == "spmm_rows") {
+        if (value < 0 || value > 2) return fail(TG_ERR_INVALID, "tg_tune_set: spmm_rows must be 0 (auto), 1 or 2");
+        g_spmm_rows = value;
+        return TG_OK;
+    }
     if (k == "spmm_slices") {
         if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8)
             return fail(TG_ERR_INVALID, "tg_tune_set: spmm_slices must be 0 (auto), 1 (off), 2, 4 or 8");
@@ -840,6 +938,7 @@ extern "C" int32_t tg_tune_get(const char* key) {
     if (k == "spmv_team") return g_spmv_team;
     if (k == "spmm_group") return g_spmm_group;
     if (k == "spmm_slices") return g_spmm_slices;
+    if (k == "spmm_rows") return g_spmm_rows;
     return -1;
 }
 

@@ -14,10 +14,11 @@ from tensors_amd.sparse import MatrixCSR, spmm_raw
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("density", [0.01, 0.002])
 @pytest.mark.parametrize("nrhs", [1, 3, 8, 33])
-def test_every_shape_matches_oracle(cuda, nrhs):
+def test_every_shape_matches_oracle(cuda, nrhs, density):
     m, n = 3000, 2500
-    rp, ci, val = random_csr_np(m, n, 0.01, seed=77 + nrhs, empty_rows=0.05)
+    rp, ci, val = random_csr_np(m, n, density, seed=77 + nrhs, empty_rows=0.05)
     rng = np.random.default_rng(nrhs)
     X = rng.uniform(-1, 1, (n, nrhs))
     Yo = np.zeros((m, nrhs))
