@@ -120,6 +120,8 @@ int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, doubl
  *                  2 = each lane group owns a row.
  *   "spmm_slices": column slices of tg_csr_spmm_plan, 0 = from the size of X,
  *                  1 = never (default), 2 / 4 / 8 = always that many (nrhs <= 4).
+ *   "batch_tile_rows": rows per workgroup tile of systems added to a batched
+ *                  engine from now on, a power of two in [256, 8192] (2048).
  * Process-wide; not thread-safe against concurrent SpMM calls. */
 int tg_tune_set(const char* key, int32_t value);
 int32_t tg_tune_get(const char* key);

@@ -29,6 +29,7 @@
 
 #include <map>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -37,7 +38,9 @@
 namespace tg {
 
 constexpr int kBatchMaxK = 8;
-constexpr int kTileRows = 2048;
+constexpr int kTileRows = 2048;  // default rows per tile (tg_tune_set "batch_tile_rows")
+constexpr int kMinTileRows = 256;
+static int g_batch_tile_rows = kTileRows;
 constexpr int kGather = 8;     // nonzeros per gather chunk in the SpMM kernel
 constexpr int kRowUnroll = 4;  // rows per step in the update kernel
 
@@ -53,7 +56,7 @@ template <typename T> struct SysDev {
     T* p[2];         // direction, by iteration parity (z = d * r is never stored)
     double* ppu;     // tiles * k partials of p.u
     double* prz[2];  // tiles * k partials of r.z, by parity
-    int n, k, kpad, tiles;
+    int n, k, kpad, tiles, tile_rows;
     int check_every;  // > 0: pause at every multiple of this many iterations for the host
 };
 
@@ -206,45 +209,19 @@ __global__ __launch_bounds__(kBlock) void bk_dir_spmm_kernel(const Tile* __restr
         const int32_t* __restrict__ ci = S.ci;
         const T* __restrict__ val = S.val;
         const int k = S.k;
-        // Rows are software-pipelined: while row i's gathers are in flight, the row
-        // bounds of row i + 2*rows and the first kGather column indices of row
-        // i + rows are being loaded, so each row costs about one memory round trip
-        // instead of three dependent ones (rp -> ci -> gathers).  Nonzeros past the
-        // first kGather of a row are loaded in place.
-        int i = tl.r0 + ro;
-        int lb0 = 0, le0 = 0, lb1 = 0, le1 = 0;
-        int32_t jj0[kGather];
-        if (i < tl.r1) {
-            lb0 = rp[i];
-            le0 = rp[i + 1];
-        }
-        if (i + rows < tl.r1) {
-            lb1 = rp[i + rows];
-            le1 = rp[i + rows + 1];
-        }
-#pragma unroll
-        for (int q = 0; q < kGather; ++q) jj0[q] = lb0 + q < le0 ? ci[lb0 + q] : i;
-        for (; i < tl.r1; i += rows) {
-            // prefetch: bounds two rows ahead, first indices one row ahead
-            const int i2 = i + 2 * rows;
-            int lb2 = 0, le2 = 0;
-            if (i2 < tl.r1) {
-                lb2 = rp[i2];
-                le2 = rp[i2 + 1];
-            }
-            int32_t jj1[kGather];
-            const int i1 = i + rows;
-#pragma unroll
-            for (int q = 0; q < kGather; ++q) jj1[q] = lb1 + q < le1 ? ci[lb1 + q] : i1;
-            // row i: first chunk from the prefetched indices, the rest in place
+        for (int i = tl.r0 + ro; i < tl.r1; i += rows) {
+            // A row's nonzeros in chunks of kGather: all index/value loads of a chunk,
+            // then all its gathers, are issued before any is used (memory-level
+            // parallelism); the sum still runs in nonzero order.
+            const int lb = rp[i], le = rp[i + 1];
             T s = T(0);
-            for (int l0 = lb0; l0 < le0; l0 += kGather) {
+            for (int l0 = lb; l0 < le; l0 += kGather) {
                 int32_t jj[kGather];
                 T vv[kGather], pj[kGather];
 #pragma unroll
                 for (int q = 0; q < kGather; ++q) {
-                    const bool ok = l0 + q < le0;
-                    jj[q] = l0 == lb0 ? jj0[q] : (ok ? ci[l0 + q] : i);
+                    const bool ok = l0 + q < le;
+                    jj[q] = ok ? ci[l0 + q] : i;
                     vv[q] = ok ? val[l0 + q] : T(0);
                 }
 #pragma unroll
@@ -255,7 +232,7 @@ __global__ __launch_bounds__(kBlock) void bk_dir_spmm_kernel(const Tile* __restr
                 }
 #pragma unroll
                 for (int q = 0; q < kGather; ++q)
-                    if (l0 + q < le0) s += vv[q] * pj[q];
+                    if (l0 + q < le) s += vv[q] * pj[q];
             }
             const int64_t e = (int64_t)i * k + c;
             const T zi = d ? d[i] * r[e] : r[e];
@@ -263,12 +240,6 @@ __global__ __launch_bounds__(kBlock) void bk_dir_spmm_kernel(const Tile* __restr
             pnew[e] = pi;
             S.u[e] = s;
             acc += (double)pi * (double)s;
-            lb0 = lb1;
-            le0 = le1;
-            lb1 = lb2;
-            le1 = le2;
-#pragma unroll
-            for (int q = 0; q < kGather; ++q) jj0[q] = jj1[q];
         }
     }
     tile_partial(acc, S.k, S.kpad, S.ppu, tl.t);
@@ -387,7 +358,8 @@ template <typename T> class CGBatch {
         S.k = k;
         S.check_every = check_every;
         S.kpad = pow2_at_least(k);
-        S.tiles = ceil_div(n, kTileRows);
+        S.tile_rows = g_batch_tile_rows;
+        S.tiles = ceil_div(n, S.tile_rows);
         const size_t N = (size_t)n * k;
         const size_t nvec = 4;  // r, u, p0, p1
         const size_t vb = align256(N * sizeof(T)), pb = align256((size_t)S.tiles * k * sizeof(double));
@@ -415,6 +387,11 @@ template <typename T> class CGBatch {
         // setup for this system alone
         std::vector<Tile> tl;
         tiles_of(slot, tl);
+        if (tl.size() > setup_cap_) {
+            if (dsetup_tiles_) TG_HIP(hipFreeAsync(dsetup_tiles_, s_));
+            setup_cap_ = std::max(tl.size(), 2 * setup_cap_);
+            TG_HIP(hipMallocAsync((void**)&dsetup_tiles_, setup_cap_ * sizeof(Tile), s_));
+        }
         TG_HIP(hipMemcpyAsync(dsetup_tiles_, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice, s_));
         TG_HIP(hipMemcpyAsync(dsetup_slot_, &slot, sizeof(int), hipMemcpyHostToDevice, s_));
         hipLaunchKernelGGL((bk_init_kernel<T>), dim3((unsigned)tl.size()), dim3(kBlock), 0, s_, dsetup_tiles_, dsys_);
@@ -514,7 +491,8 @@ template <typename T> class CGBatch {
   private:
     void tiles_of(int slot, std::vector<Tile>& tl) const {
         const SysDev<T>& S = host_[slot];
-        for (int t = 0; t < S.tiles; ++t) tl.push_back(Tile{slot, t * kTileRows, std::min(S.n, (t + 1) * kTileRows), t});
+        for (int t = 0; t < S.tiles; ++t)
+            tl.push_back(Tile{slot, t * S.tile_rows, std::min(S.n, (t + 1) * S.tile_rows), t});
     }
 
     int grow(int need) {
@@ -543,10 +521,7 @@ template <typename T> class CGBatch {
         mem_.resize(cap, nullptr);
         bytes_.resize(cap, 0);
         host_.resize(cap);
-        if (!dsetup_tiles_) {
-            TG_HIP(hipMallocAsync((void**)&dsetup_tiles_, sizeof(Tile) * ((size_t)INT32_MAX / kTileRows + 1), s_));
-            TG_HIP(hipMallocAsync((void**)&dsetup_slot_, sizeof(int), s_));
-        }
+        if (!dsetup_slot_) TG_HIP(hipMallocAsync((void**)&dsetup_slot_, sizeof(int), s_));
         return TG_OK;
     }
 
@@ -598,6 +573,7 @@ template <typename T> class CGBatch {
     Tile* dtiles_ = nullptr;
     size_t tiles_cap_ = 0;
     Tile* dsetup_tiles_ = nullptr;
+    size_t setup_cap_ = 0;
     int* dsetup_slot_ = nullptr;
     int ntiles_ = 0;
     bool dirty_ = true;
@@ -794,9 +770,10 @@ struct ReplaceBatch {
         D.n = S.n;
         D.k = S.k;
         D.kpad = pow2_at_least(S.k);
-        D.tiles = ceil_div(S.n, kTileRows);
+        D.tiles = E.tiles;
         const int di = (int)descs.size();
-        for (int t = 0; t < D.tiles; ++t) tiles.push_back(ReplaceTile{di, t * kTileRows, std::min(S.n, (t + 1) * kTileRows), t});
+        for (int t = 0; t < E.tiles; ++t)
+            tiles.push_back(ReplaceTile{di, t * E.tile_rows, std::min(S.n, (t + 1) * E.tile_rows), t});
         descs.push_back(D);
         sys.push_back(&S);
         ++S.replacements;
@@ -874,9 +851,10 @@ static int grid_of(int64_t c) { return (int)std::max<int64_t>(1, std::min<int64_
 
 static int mixed_start(Batch& B, BatchSystem& S) {
     const size_t N = (size_t)S.n * S.k;
-    const int tiles = ceil_div(S.n, kTileRows);
+    const int tiles = ceil_div(S.n, kTileRows);        // the ||b|| pass
+    const int ptiles = ceil_div(S.n, kMinTileRows);    // any engine tiling
     const size_t bytes = align256((size_t)S.nnz * 4) + (S.d64 ? align256((size_t)S.n * 4) : 0) + 2 * align256(N * 4) +
-                         align256((size_t)tiles * S.k * 8) + align256((size_t)S.k * 8);
+                         align256((size_t)ptiles * S.k * 8) + align256((size_t)S.k * 8);
     TG_HIP(hipMallocAsync((void**)&S.mem, bytes, B.s));
     S.mem_bytes = bytes;
     char* p = S.mem;
@@ -891,7 +869,7 @@ static int mixed_start(Batch& B, BatchSystem& S) {
     S.x32 = (float*)p;
     p += align256(N * 4);
     S.pnorm = (double*)p;
-    p += align256((size_t)tiles * S.k * 8);
+    p += align256((size_t)ptiles * S.k * 8);
     S.norms = (double*)p;
     hipLaunchKernelGGL(bm_tofloat_kernel, dim3(grid_of(S.nnz)), dim3(kBlock), 0, B.s, S.nnz, S.val64, S.a32);
     if (S.d64) hipLaunchKernelGGL(bm_tofloat_kernel, dim3(grid_of(S.n)), dim3(kBlock), 0, B.s, (int64_t)S.n, S.d64, S.d32);
@@ -997,6 +975,18 @@ static int batch_advance(Batch& B, int iters) {
     return TG_OK;
 }
 
+int batch_tune_set(const char* key, int32_t value, bool* known) {
+    *known = false;
+    if (std::string(key) != "batch_tile_rows") return TG_OK;
+    *known = true;
+    if (value < kMinTileRows || value > 8192 || (value & (value - 1)))
+        return fail(TG_ERR_INVALID, "tg_tune_set: batch_tile_rows must be a power of two in [256, 8192]");
+    g_batch_tile_rows = value;
+    return TG_OK;
+}
+
+int32_t batch_tune_get(const char* key) { return std::string(key) == "batch_tile_rows" ? g_batch_tile_rows : -1; }
+
 }  // namespace tg
 
 using namespace tg;
@@ -1099,12 +1089,12 @@ extern "C" int tg_batch_release(tg_batch_t h, int64_t id) {
 // caller's matrix and vectors).
 extern "C" size_t tg_batch_request_bytes(int32_t n, int64_t nnz, int32_t nrhs, int32_t has_diag, int32_t mixed) {
     if (n < 1 || nrhs < 1 || nrhs > kBatchMaxK || nnz < 0) return 0;
-    const size_t N = (size_t)n * nrhs, tiles = (size_t)ceil_div(n, kTileRows);
+    const size_t N = (size_t)n * nrhs, tiles = (size_t)ceil_div(n, g_batch_tile_rows);
     const size_t pb = align256(tiles * nrhs * sizeof(double));
     const size_t nvec = 4;
     if (!mixed) return nvec * align256(N * sizeof(double)) + 3 * pb;
     return nvec * align256(N * sizeof(float)) + 3 * pb + align256((size_t)nnz * 4) + (has_diag ? align256((size_t)n * 4) : 0) +
-           2 * align256(N * 4) + pb + align256((size_t)2 * nrhs * 8);
+           2 * align256(N * 4) + align256((size_t)ceil_div(n, kMinTileRows) * nrhs * 8) + align256((size_t)2 * nrhs * 8);
 }
 
 extern "C" int tg_batch_active(tg_batch_t h, int32_t* systems, int64_t* bytes_in_use) {

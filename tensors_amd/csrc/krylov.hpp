@@ -26,6 +26,10 @@ int64_t csr_band(const CsrView& A, hipStream_t s);
 // knowing the band can matter.
 bool spmm_may_slab(const CsrView& A, int32_t nrhs);
 
+// Tuning keys of the batched engine, forwarded by tg_tune_set / tg_tune_get.
+int batch_tune_set(const char* key, int32_t value, bool* known);
+int32_t batch_tune_get(const char* key);
+
 // Y <- alpha A X + beta Y with the SpMM kernels; when `skip` is non-null and
 // *skip != 0 at kernel start, the kernels return without touching Y.
 int spmm_dispatch(const CsrView& A, double alpha, const void* X, int64_t ldX, double beta, void* Y, int64_t ldY,

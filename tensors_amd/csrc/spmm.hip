@@ -928,6 +928,9 @@ extern "C" int tg_tune_set(const char* key, int32_t value) {
         g_spmm_slices = value;
         return TG_OK;
     }
+    bool known = false;
+    const int rc = batch_tune_set(key, value, &known);
+    if (known) return rc;
     return fail(TG_ERR_INVALID, "tg_tune_set: unknown key '%s'", key);
 }
 
@@ -939,7 +942,7 @@ extern "C" int32_t tg_tune_get(const char* key) {
     if (k == "spmm_group") return g_spmm_group;
     if (k == "spmm_slices") return g_spmm_slices;
     if (k == "spmm_rows") return g_spmm_rows;
-    return -1;
+    return batch_tune_get(key);
 }
 
 extern "C" int32_t tg_csr_default_blocks(int32_t m, int64_t nnz, int32_t nrhs) {
