@@ -121,7 +121,7 @@ int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, doubl
  *   "spmm_slices": column slices of tg_csr_spmm_plan, 0 = from the size of X,
  *                  1 = never (default), 2 / 4 / 8 = always that many (nrhs <= 4).
  *   "batch_tile_rows": rows per workgroup tile of systems added to a batched
- *                  engine from now on, a power of two in [256, 8192] (2048).
+ *                  engine from now on, a power of two in [256, 8192] (1024).
  * Process-wide; not thread-safe against concurrent SpMM calls. */
 int tg_tune_set(const char* key, int32_t value);
 int32_t tg_tune_get(const char* key);
@@ -169,25 +169,28 @@ int tg_cg_solve(int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, const 
                 int32_t* iterations, double* rel_residuals, void* stream);
 
 /* ------------------------------------------------------------------------
- * Mixed-precision CG: FP32 inner CG sweeps (FP32 values a32 on the same
- * pattern, FP32 Krylov vectors, the recurrences of tg_cg_solve) inside an FP64
- * iterative-refinement loop whose residual b - A x, solution update and
- * stopping norms are FP64.  No reference counterpart (the reference CG is
- * single-precision-agnostic); it solves what tg_cg_solve solves:
+ * Mixed-precision CG: the recurrences of tg_cg_solve in FP32 (FP32 values a32
+ * on the same pattern, FP32 Krylov vectors, FP64 reductions) with FP64
+ * residual replacement: every 16 iterations, once the FP32 residual estimate
+ * has fallen by the factor replace_drop (0 = 0.1) since the last replacement,
+ * or when the FP32 iteration meets its own test, the FP32 correction is folded
+ * into the FP64 solution and the residual is recomputed as b - A x in FP64; the
+ * search direction is kept.  No reference counterpart (the reference CG is
+ * precision-agnostic); it solves what tg_cg_solve solves:
  *
  *   X <- a_coef * A^{-1} B + b_coef * X      (FP64 in and out)
  *
  * Stops when every column has ||B - A x||_2 / ||B||_2 <= tol (true FP64
- * residual), or after max_outer refinement steps; each inner solve runs at most
- * max_inner iterations down to max(inner_tol, 0.3 tol / current residual).
+ * residual), after max_outer replacements or max_inner FP32 iterations.
  * a32 may be NULL (the FP64 values are rounded into the workspace; pass
  * with_a32 = 0 to the workspace query then).  rel_residuals[nrhs] are the final
- * true relative residuals.  Synchronises `stream`.
+ * true relative residuals; outer_iterations counts the replacements.
+ * Synchronises `stream`.
  * ------------------------------------------------------------------------ */
 size_t tg_cg_mixed_workspace_bytes(int32_t n, int64_t nnz, int32_t nrhs, int32_t prec_kind, int32_t with_a32);
 int tg_cg_solve_mixed(int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, const int32_t* ci,
                       const double* a64, const float* a32, int32_t prec_kind, const double* prec_diag,
-                      int32_t nrhs, int32_t max_outer, int32_t max_inner, double inner_tol, double a_coef,
+                      int32_t nrhs, int32_t max_outer, int32_t max_inner, double replace_drop, double a_coef,
                       const double* B, int64_t ldB, double b_coef, double* X, int64_t ldX, double tol,
                       int32_t use_guess, void* workspace, size_t workspace_bytes, int32_t* succeeded,
                       int32_t* outer_iterations, int32_t* inner_iterations, double* rel_residuals, void* stream);

@@ -38,7 +38,7 @@
 namespace tg {
 
 constexpr int kBatchMaxK = 8;
-constexpr int kTileRows = 2048;  // default rows per tile (tg_tune_set "batch_tile_rows")
+constexpr int kTileRows = 1024;  // default rows per tile (tg_tune_set "batch_tile_rows")
 constexpr int kMinTileRows = 256;
 static int g_batch_tile_rows = kTileRows;
 constexpr int kGather = 8;     // nonzeros per gather chunk in the SpMM kernel

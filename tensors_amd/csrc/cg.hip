@@ -270,7 +270,7 @@ static int grid_for(int64_t count) {
 template <typename T>
 int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, const OpRef& Pop, int32_t k,
                int32_t max_iter, double a, const T* Bm, int64_t ldB, double b, T* X, int64_t ldX, double tol,
-               int use_guess, void* ws, size_t ws_bytes, CGResult* res, hipStream_t s) {
+               int use_guess, void* ws, size_t ws_bytes, CGResult* res, hipStream_t s, const CGHook* hook) {
     const int n = A.n;
     res->succeeded = 0;
     res->iterations = 0;
@@ -361,11 +361,22 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
     // Iterations.  With device operators every kernel checks `done` itself and
     // the host polls in growing batches; host callbacks cannot check it, so
     // then the host polls after every iteration.
-    int launched = 0, batch = host_ops ? 1 : 4;
+    int launched = 0, batch = host_ops ? 1 : hook ? std::max(1, hook->every) : 4;
     int hflag[2] = {0, 0};
     TG_HIP(hipMemcpyAsync(hflag, &st->iter, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
     TG_HIP(hipStreamSynchronize(s));
     int rc = TG_OK;
+    if (hook && hflag[1]) {  // converged before the first iteration: let the hook see it
+        TG_HIP(hipMemcpyAsync(&hs, st, sizeof(CGState), hipMemcpyDeviceToHost, s));
+        TG_HIP(hipStreamSynchronize(s));
+        bool stop = false;
+        rc = hook->fn(hs, r, z, x, partial, st, &stop);
+        if (rc) return rc;
+        if (!stop) {
+            TG_HIP(hipMemcpyAsync(hflag, &st->iter, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+            TG_HIP(hipStreamSynchronize(s));
+        }
+    }
     while (!hflag[1] && launched < max_iter) {
         for (int it = 0; it < batch && launched < max_iter; ++it, ++launched) {
             rc = op_apply(Aop, p, u, k, &st->done, s);
@@ -393,6 +404,22 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
             rc = fail(TG_ERR_HIP, "cg: polling the convergence flag failed");
             break;
         }
+        if (hook) {
+            if (hipMemcpyAsync(&hs, st, sizeof(CGState), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess) {
+                rc = fail(TG_ERR_HIP, "cg: reading the solver state failed");
+                break;
+            }
+            bool stop = false;
+            rc = hook->fn(hs, r, z, x, partial, st, &stop);
+            if (rc || stop) break;
+            if (hipMemcpyAsync(hflag, &st->iter, 2 * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess) {
+                rc = fail(TG_ERR_HIP, "cg: polling the convergence flag failed");
+                break;
+            }
+            continue;
+        }
         if (!host_ops) batch = std::min(batch * 2, 64);
     }
     if (rc) return rc;
@@ -411,10 +438,10 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
 
 template int cg_solve_t<double>(const CsrView&, const OpRef&, int, const double*, const OpRef&, int32_t, int32_t,
                                 double, const double*, int64_t, double, double*, int64_t, double, int, void*, size_t,
-                                CGResult*, hipStream_t);
+                                CGResult*, hipStream_t, const CGHook*);
 template int cg_solve_t<float>(const CsrView&, const OpRef&, int, const float*, const OpRef&, int32_t, int32_t,
                                double, const float*, int64_t, double, float*, int64_t, double, int, void*, size_t,
-                               CGResult*, hipStream_t);
+                               CGResult*, hipStream_t, const CGHook*);
 
 }  // namespace tg
 

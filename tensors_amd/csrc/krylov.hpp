@@ -1,6 +1,7 @@
 // Internal interfaces shared by the SpMM and Krylov translation units.
 #pragma once
 
+#include <functional>
 #include <vector>
 
 #include "common.hpp"
@@ -100,9 +101,19 @@ int gmres_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d,
 
 size_t cg_workspace_bytes(int32_t n, int32_t k, size_t vs, int prec_kind);
 
+// Host hook of cg_solve_t: with a hook the host polls every `every` iterations and
+// calls fn with the device state copied to the host (hs), the device buffers of the
+// iteration (r, z, x, the workgroup partials, the device state).  fn may change
+// them (e.g. replace the residual) and sets *stop to end the solve; if the device
+// stopped (hs.done) and fn neither stops nor clears `done`, the solve ends.
+struct CGHook {
+    int every = 16;
+    std::function<int(const CGState& hs, void* r, void* z, void* x, double* partial, CGState* st, bool* stop)> fn;
+};
+
 template <typename T>
 int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, const OpRef& Pop, int32_t k, int32_t max_iter, double a, const T* Bm,
                int64_t ldB, double b, T* X, int64_t ldX, double tol, int use_guess, void* ws, size_t ws_bytes,
-               CGResult* res, hipStream_t s);
+               CGResult* res, hipStream_t s, const CGHook* hook = nullptr);
 
 }  // namespace tg

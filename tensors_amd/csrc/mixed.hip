@@ -1,21 +1,23 @@
-// Mixed-precision CG: FP32 inner CG sweeps inside an FP64 iterative-refinement
-// loop.
+// Mixed-precision CG: the ConjugateGradient.hpp recurrences run in FP32 (FP32
+// copy of A, FP32 Krylov vectors, FP64 reductions) on the correction equation,
+// with FP64 residual replacement (reliable updates: van der Vorst & Ye, "Residual
+// replacement strategies for Krylov subspace iterative methods for the
+// convergence of true residuals", SIAM J. Sci. Comput. 22 (2000); the mixed
+// precision form of Clark et al., Comput. Phys. Commun. 181 (2010)):
 //
-//   x = guess or 0
-//   loop:  r = b - A64 x                  (FP64 SpMM, FP64 residual)
+//   x64 = guess or 0;  r = float(b - A64 x64);  FP32 CG on A32 d = r, with d = 0
+//   every 16 iterations, when the FP32 residual estimate fell 10x since the last
+//   replacement (or the FP32 CG met its own test):
+//          x64 += d, d = 0                (FP64 solution, the FP32 part restarts at 0)
+//          r = float(b - A64 x64)         (FP64 SpMM and residual)
 //          stop when every column has ||r||_2 / ||b||_2 <= tol   (FP64 norms)
-//          r32 = float(r)                 (A32 and the Krylov vectors are FP32)
-//          A32 d = r32 by FP32 CG         (the ConjugateGradient.hpp recurrences,
-//                                          stopping at |r.z| <= tol_in^2 |b.Pb|)
-//          x += double(d)                 (FP64 solution update)
+//          rho = r.z for the replaced r   (the search direction p is kept)
 //   X <- a x + b X
 //
-// A32 is the FP32 rounding of the FP64 values on the same pattern (the
-// reference's external-values form of Dot, MatrixCSR.hpp:1026-1086).  The
-// stopping test of the refinement is the true FP64 residual, kept distinct
-// from the inner CG's recursive preconditioned residual.  Each inner solve is
-// asked for just the reduction still needed (never below inner_tol, the FP32
-// floor), so the last sweep does not over-solve.
+// Keeping p makes the FP32 iteration continue the same Krylov process instead
+// of restarting it for every refinement step, so the iteration count stays
+// close to FP64 CG's.  A32 is the FP32 rounding of the FP64 values on the same
+// pattern (the reference's external-values form of Dot, MatrixCSR.hpp:1026-1086).
 
 #include "common.hpp"
 #include "krylov.hpp"
@@ -108,6 +110,53 @@ __global__ __launch_bounds__(kBlock) void mx_scale_kernel(int n, int k, double b
     }
 }
 
+// x64 += d, d = 0
+__global__ __launch_bounds__(kBlock) void mx_flush_kernel(int64_t count, float* __restrict__ d, double* __restrict__ x) {
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < count; e += (int64_t)gridDim.x * kBlock) {
+        x[e] += (double)d[e];
+        d[e] = 0.0f;
+    }
+}
+
+// z = P r (if a diagonal is given); partial(r.z)
+__global__ __launch_bounds__(kBlock) void mx_rz_kernel(int n, int k, int kpad, const float* __restrict__ r,
+                                                       const float* __restrict__ diag, float* __restrict__ z,
+                                                       double* __restrict__ partial) {
+    int i0, i1;
+    block_row_range(n, i0, i1);
+    const Lanes L = lanes_of(kpad);
+    double acc = 0.0;
+    if (L.c < k)
+        for (int i = i0 + L.ro; i < i1; i += L.rows) {
+            const int64_t e = (int64_t)i * k + L.c;
+            const float rv = r[e];
+            float zv = rv;
+            if (diag) {
+                zv = diag[i] * rv;
+                z[e] = zv;
+            }
+            acc += (double)rv * (double)zv;
+        }
+    column_partial(acc, L, k, kpad, partial);
+}
+
+// rho = r.z of the replaced residual; with `scale` != 1 the device had stopped on
+// the FP32 estimate: tighten its tolerance and let it continue.
+__global__ __launch_bounds__(kBlock) void mx_replace_state_kernel(int k, int kpad, int B,
+                                                                  const double* __restrict__ partial, double scale,
+                                                                  CGState* __restrict__ st) {
+    const int c = threadIdx.x % kpad, j = threadIdx.x / kpad, J = kBlock / kpad;
+    const double tot = column_total(partial, B, k, kpad, c, j, J);
+    if (j == 0 && c < k) {
+        st->rho[c] = tot;
+        st->TOL[c] *= scale;
+    }
+    if (threadIdx.x == 0) {
+        st->done = 0;
+        st->succeeded = 0;
+    }
+}
+
 static int pow2ceil_m(int v) {
     int p = 1;
     while (p < v) p <<= 1;
@@ -116,6 +165,9 @@ static int pow2ceil_m(int v) {
 static int grid_m(int64_t count) {
     return (int)std::max<int64_t>(1, std::min<int64_t>((count + kBlock - 1) / kBlock, 8192));
 }
+
+constexpr int kReplaceCheck = 16;     // iterations between residual-replacement checks
+constexpr double kReplaceDrop = 0.1;  // default: replace when the FP32 residual estimate fell 10x
 
 struct MixedWs {
     double* partial;
@@ -163,7 +215,7 @@ struct MixedResult {
 };
 
 static int cg_mixed(const CsrView& A64, const float* a32_in, int prec_kind, const double* d64, int32_t k,
-                    int32_t max_outer, int32_t max_inner, double inner_tol, double a, const double* Bm, int64_t ldB,
+                    int32_t max_outer, int32_t max_inner, double replace_drop, double a, const double* Bm, int64_t ldB,
                     double b, double* X, int64_t ldX, double tol, int use_guess, void* ws, size_t ws_bytes,
                     MixedResult* res, hipStream_t s) {
     const int n = A64.n;
@@ -225,44 +277,89 @@ static int cg_mixed(const CsrView& A64, const float* a32_in, int prec_kind, cons
     }
     hipLaunchKernelGGL(mx_load_kernel, dim3(grid_m(N)), dim3(kBlock), 0, s, n, k, (const double*)X, ldX, w.x64,
                        use_guess ? 0 : 1);
-    bool first = !use_guess;
-    int outer = 0;
-    bool ok = false;
-    while (true) {
-        // r = b - A x  (FP64)
-        if (!first) {
+    // r = float(b - A64 x64)   (true residual norms in w.norms + k)
+    auto residual = [&](bool x_is_zero) -> int {
+        if (!x_is_zero) {
             int rc = csr_apply<double>(A64, w.x64, w.u64, k, s);
             if (rc) return rc;
         }
         hipLaunchKernelGGL(mx_residual_kernel, dim3(Bk), dim3(kBlock), 0, s, n, k, kpad, Bm, ldB,
-                           first ? (const double*)nullptr : (const double*)w.u64, w.r32, w.partial);
+                           x_is_zero ? (const double*)nullptr : (const double*)w.u64, w.r32, w.partial);
         hipLaunchKernelGGL(mx_total_kernel, dim3(1), dim3(kBlock), 0, s, k, kpad, Bk, (const double*)w.partial,
                            w.norms + k);
         TG_LAUNCH_CHECK();
         TG_HIP(hipMemcpyAsync(rn.data(), w.norms + k, k * 8, hipMemcpyDeviceToHost, s));
         TG_HIP(hipStreamSynchronize(s));
-        first = false;
+        return TG_OK;
+    };
+    auto worst_rel = [&]() {
         double worst = 0.0;
         for (int c = 0; c < k; ++c) {
             res->rel_res[c] = bn[c] > 0 ? rn[c] / bn[c] : 0.0;
-            worst = std::max(worst, res->rel_res[c]);
+            worst = std::max(worst, std::isnan(res->rel_res[c]) ? INFINITY : res->rel_res[c]);
         }
-        if (worst <= tol) {
-            ok = true;
-            break;
-        }
-        if (outer >= max_outer) break;
-        // inner FP32 CG on the correction: A32 d = r32
-        const double tin = std::min(0.5, std::max(inner_tol, 0.3 * tol / worst));
+        return worst;
+    };
+    int rc = residual(!use_guess);
+    if (rc) return rc;
+    bool ok = worst_rel() <= tol;
+    int replacements = 0;
+    if (!ok) {
+        double est_last = 1.0;
+        const double drop = (replace_drop > 0.0 && replace_drop < 1.0) ? replace_drop : kReplaceDrop;
+        CGHook hook;
+        hook.every = kReplaceCheck;
+        hook.fn = [&](const CGState& hs, void* r, void* z, void* d, double* partial, CGState* st, bool* stop) -> int {
+            double est = 0.0;
+            for (int c = 0; c < k; ++c) est = std::max(est, hs.bsq[c] > 0 ? std::sqrt(std::fabs(hs.rho[c]) / hs.bsq[c]) : 0.0);
+            const bool stopped = hs.done != 0;
+            if (!stopped && !(est < drop * est_last)) return TG_OK;
+            // fold the FP32 correction into x64, then the true residual
+            hipLaunchKernelGGL(mx_flush_kernel, dim3(grid_m(N)), dim3(kBlock), 0, s, N, (float*)d, w.x64);
+            int rc2 = residual(false);
+            if (rc2) return rc2;
+            ++replacements;
+            est_last = est;
+            const double worst = worst_rel();
+            if (worst <= tol) {
+                ok = true;
+                *stop = true;
+                return TG_OK;
+            }
+            if (std::isinf(worst) || replacements >= std::max(1, max_outer) || hs.iter >= max_inner) {
+                *stop = true;
+                return TG_OK;
+            }
+            // continue the FP32 process from the replaced residual (copy it into the
+            // iteration's r); a stop on the FP32 estimate tightens its tolerance
+            TG_HIP(hipMemcpyAsync(r, w.r32, (size_t)N * 4, hipMemcpyDeviceToDevice, s));
+            hipLaunchKernelGGL(mx_rz_kernel, dim3(Bk), dim3(kBlock), 0, s, n, k, kpad, (const float*)r,
+                               (const float*)w.diag32, (float*)z, partial);
+            double scale = 1.0;
+            if (stopped) {
+                const double f = tol / worst;
+                scale = std::min(0.25, std::max(1e-4, 0.25 * f * f));
+            }
+            hipLaunchKernelGGL(mx_replace_state_kernel, dim3(1), dim3(kBlock), 0, s, k, kpad, Bk,
+                               (const double*)partial, scale, st);
+            TG_LAUNCH_CHECK();
+            return TG_OK;
+        };
         CGResult cr{};
-        int rc = cg_solve_t<float>(A32, A32op, prec_kind, w.diag32, none, k, max_inner, 1.0, w.r32, k, 0.0, w.d32, k,
-                                   tin, 0, w.inner, w.inner_bytes, &cr, s);
+        rc = cg_solve_t<float>(A32, A32op, prec_kind, w.diag32, none, k, max_inner, 1.0, w.r32, k, 0.0, w.d32, k, tol,
+                               0, w.inner, w.inner_bytes, &cr, s, &hook);
         if (rc) return rc;
-        res->inner_total += cr.iterations;
+        res->inner_total = cr.iterations;
+        // the FP32 part left after the last replacement (zero if the solve stopped there)
         hipLaunchKernelGGL(mx_update_kernel, dim3(grid_m(N)), dim3(kBlock), 0, s, N, (const float*)w.d32, w.x64);
         TG_LAUNCH_CHECK();
-        ++outer;
+        if (!ok) {
+            rc = residual(false);
+            if (rc) return rc;
+            ok = worst_rel() <= tol;
+        }
     }
+    const int outer = replacements;
     const int b_mode = (b == 0.0) ? 0 : (b == 1.0) ? 1 : 2;
     hipLaunchKernelGGL(mx_write_kernel, dim3(grid_m(N)), dim3(kBlock), 0, s, n, k, a, (const double*)w.x64, b, X, ldX,
                        b_mode);
@@ -285,7 +382,7 @@ extern "C" size_t tg_cg_mixed_workspace_bytes(int32_t n, int64_t nnz, int32_t nr
 
 extern "C" int tg_cg_solve_mixed(int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, const int32_t* ci,
                                  const double* a64, const float* a32, int32_t prec_kind, const double* prec_diag,
-                                 int32_t nrhs, int32_t max_outer, int32_t max_inner, double inner_tol, double a_coef,
+                                 int32_t nrhs, int32_t max_outer, int32_t max_inner, double replace_drop, double a_coef,
                                  const double* B, int64_t ldB, double b_coef, double* X, int64_t ldX, double tol,
                                  int32_t use_guess, void* workspace, size_t workspace_bytes, int32_t* succeeded,
                                  int32_t* outer_iterations, int32_t* inner_iterations, double* rel_residuals,
@@ -308,7 +405,7 @@ extern "C" int tg_cg_solve_mixed(int32_t n, int64_t nnz, const void* rp, tg_dtyp
     A.val_type = TG_F64;
     if (spmm_may_slab(A, nrhs)) A.band = csr_band(A, as_stream(stream));
     MixedResult res{};
-    const int rc = cg_mixed(A, a32, prec_kind, prec_diag, nrhs, max_outer, max_inner, inner_tol, a_coef, B, ldB,
+    const int rc = cg_mixed(A, a32, prec_kind, prec_diag, nrhs, max_outer, max_inner, replace_drop, a_coef, B, ldB,
                             b_coef, X, ldX, tol, use_guess, workspace, workspace_bytes, &res, as_stream(stream));
     if (rc) return rc;
     if (succeeded) *succeeded = res.succeeded;

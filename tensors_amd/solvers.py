@@ -149,10 +149,10 @@ def gmres_solve_op(n, nrhs, max_iter, apply_A, apply_P, a, B, b, X, tol, max_res
     return bool(ok.value), it.value, rs.value, rr
 
 
-def cg_solve_mixed(A: MatrixCSR, B, X, tol=1e-10, max_outer=20, max_inner=10000, inner_tol=1e-5, a=1.0, b=0.0,
+def cg_solve_mixed(A: MatrixCSR, B, X, tol=1e-10, max_outer=100, max_inner=20000, replace_drop=0.1, a=1.0, b=0.0,
                    P=None, use_initial_guess=False, a32=None, stream=None):
-    """tg_cg_solve_mixed: FP32 inner CG + FP64 refinement to ||B - A X|| / ||B|| <= tol.
-    Returns (succeeded, outer, inner_total, rel_res).  a32: optional FP32 values (CUDA)."""
+    """tg_cg_solve_mixed: FP32 CG with FP64 residual replacement to ||B - A X|| / ||B|| <= tol.
+    Returns (succeeded, replacements, fp32_iterations, rel_res).  a32: optional FP32 values (CUDA)."""
     n, k = X.shape[0], (X.shape[1] if X.dim() == 2 else 1)
     rp, ci, vals = A.device_arrays(torch.float64)
     if a32 is None:
@@ -164,7 +164,7 @@ def cg_solve_mixed(A: MatrixCSR, B, X, tol=1e-10, max_outer=20, max_inner=10000,
     capi.call("tg_cg_solve_mixed", n, A.NonzeroCount(), rp.data_ptr(),
               capi.I64 if rp.dtype == torch.int64 else capi.I32, ci.data_ptr(), vals.data_ptr(), a32.data_ptr(),
               0 if P is None else 1, P.data_ptr() if P is not None else None, k, max_outer, max_inner,
-              float(inner_tol), float(a), B.data_ptr(), B.stride(0) if B.dim() == 2 else 1, float(b),
+              float(replace_drop), float(a), B.data_ptr(), B.stride(0) if B.dim() == 2 else 1, float(b),
               X.data_ptr(), X.stride(0) if X.dim() == 2 else 1, float(tol), int(use_initial_guess),
               ws.data_ptr(), nb, C.byref(ok), C.byref(outer), C.byref(inner), rr.ctypes.data,
               stream_handle(stream))

@@ -56,3 +56,27 @@ def test_mixed_alpha_beta_and_special_cases(cuda):
     Xd = torch.full((n, k), 3.0, dtype=torch.float64, device=cuda)
     ok, _, _, _ = cg_solve_mixed(A, Bn, Xd)
     assert not ok and torch.all(Xd == 3.0)
+
+
+def test_replacement_keeps_the_krylov_process(cuda):
+    """Residual replacement keeps p, so the FP32 iteration count stays near FP64 CG's
+    (a restart per refinement step would cost roughly one extra half-solve each)."""
+    from tensors_amd.solvers import ConjugateGradient
+
+    g, k = 40, 4
+    rp, ci, val = gen.laplacian_3d(g, 1e-2)
+    n = g ** 3
+    A = MatrixCSR(rp.astype(np.int32), ci, val, n, n)
+    B = torch.from_numpy(gen.uniform(3003, n * k).reshape(n, k)).to(cuda)
+    X64 = torch.zeros(n, k, dtype=torch.float64, device=cuda)
+    cg = ConjugateGradient(n, 5000, k)
+    assert cg(A, None, 1.0, B, 0.0, X64, 1e-10)
+    it64 = cg.IterationCount()
+    Xm = torch.zeros(n, k, dtype=torch.float64, device=cuda)
+    ok, reps, it32, rr = cg_solve_mixed(A, B, Xm, tol=1e-10)
+    assert ok and np.all(rr <= 1e-10) and reps >= 2
+    assert it32 <= 1.15 * it64 + 16, (it32, it64)
+    # an initial guess near the solution needs only a few iterations
+    Xg = Xm.clone() * (1 + 1e-6)
+    ok, _, it_g, rr = cg_solve_mixed(A, B, Xg, tol=1e-10, use_initial_guess=True)
+    assert ok and it_g < it32 and np.all(rr <= 1e-10)
