@@ -59,6 +59,7 @@ def parse():
     ap.add_argument("--batch-systems", type=int, default=2000)
     ap.add_argument("--batch-iters", type=int, default=16, help="CG iterations enqueued per poll")
     ap.add_argument("--batch-tile-rows", type=int, default=0, help="rows per workgroup tile (0 = library default)")
+    ap.add_argument("--batch-stream", type=int, default=-1, help="batched SpMM kernel: 1 streaming, 0 row per lane")
     ap.add_argument("--batch-seq-sample", type=int, default=32,
                     help="systems solved one at a time by the drop-in FP64 CG for comparison")
     ap.add_argument("--slices", type=int, default=-1,
@@ -254,10 +255,12 @@ def batch_section(args, dev):
 
     specs = batch_specs(args.batch_systems)
     tol = 1e-10
-    if args.batch_tile_rows:
-        from tensors_amd import _capi as capi
+    from tensors_amd import _capi as capi
 
+    if args.batch_tile_rows:
         capi.call("tg_tune_set", b"batch_tile_rows", args.batch_tile_rows)
+    if args.batch_stream >= 0:
+        capi.call("tg_tune_set", b"batch_stream", args.batch_stream)
 
     def make(i):
         d, side, k = specs[i]

@@ -41,6 +41,7 @@ constexpr int kBatchMaxK = 8;
 constexpr int kTileRows = 1024;  // default rows per tile (tg_tune_set "batch_tile_rows")
 constexpr int kMinTileRows = 256;
 static int g_batch_tile_rows = kTileRows;
+static int g_batch_stream = 1;  // SpMM kernel of the engine: 1 = streaming chunks, 0 = a row per lane
 constexpr int kGather = 8;     // nonzeros per gather chunk in the SpMM kernel
 constexpr int kRowUnroll = 4;  // rows per step in the update kernel
 
@@ -167,10 +168,117 @@ __global__ __launch_bounds__(kBlock) void bk_setup_kernel(const int* __restrict_
     }
 }
 
+// ---- K1 body, streaming form ------------------------------------------------------
+// The tile's rows are taken in chunks whose nonzeros fit kSlots * kBlock / kpad; a
+// lane fills kSlots (nonzero, column) slots of the chunk -- first all column
+// indices and values (consecutive nonzeros on consecutive lane groups, so the
+// loads coalesce), then all neighbour directions d_j r_j + beta p_j -- and leaves
+// the products in LDS; then the (row, column) lanes add their row's products in
+// nonzero order, form their own row's direction and write p and u.  Compared with
+// a row per lane, every lane has kSlots independent gathers in flight and the
+// index/value loads are coalesced.  A row longer than a chunk is done by its own
+// lanes.  All threads of the workgroup must call it (it synchronises).
+constexpr int kSlots = 4;
+constexpr int kRowSeg = 1024;  // rows of the tile whose row pointers sit in LDS at a time
+
+template <typename T>
+__device__ double k1_stream(const SysDev<T>& S, const Tile& tl, int c, int ro, int rows, bool first, T be,
+                            const T* __restrict__ pold, T* __restrict__ pnew) {
+    constexpr int CHUNK = kBlock * kSlots;
+    __shared__ T prod[CHUNK];
+    __shared__ int srp[kRowSeg + 1];
+    __shared__ int sb;
+    const int k = S.k, kpad = S.kpad, cap = CHUNK / kpad;
+    const int32_t* __restrict__ rp = S.rp;
+    const int32_t* __restrict__ ci = S.ci;
+    const T* __restrict__ val = S.val;
+    const T* __restrict__ r = S.r;
+    const T* __restrict__ d = S.d;
+    const bool col = c < k;
+    double acc = 0.0;
+    for (int g0 = tl.r0; g0 < tl.r1; g0 += kRowSeg) {
+        const int nr = min(tl.r1 - g0, kRowSeg);
+        for (int i = threadIdx.x; i <= nr; i += kBlock) srp[i] = rp[g0 + i];
+        __syncthreads();
+        for (int a = 0; a < nr;) {
+            if (threadIdx.x == 0) {  // last row b with srp[b] - srp[a] <= cap (b > a)
+                const int base = srp[a];
+                int lo = a + 1, hi = nr;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (srp[mid] - base <= cap)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                sb = lo;
+            }
+            __syncthreads();
+            const int b = sb;
+            const int L0 = srp[a], cnt = srp[b] - L0;
+            if (cnt <= cap) {
+                int32_t jn[kSlots];
+                T v[kSlots], pj[kSlots];
+#pragma unroll
+                for (int q = 0; q < kSlots; ++q) {
+                    const int e = ro + q * rows;
+                    const bool ok = col && e < cnt;
+                    jn[q] = ok ? ci[L0 + e] : 0;
+                    v[q] = ok ? val[L0 + e] : T(0);
+                }
+#pragma unroll
+                for (int q = 0; q < kSlots; ++q) {
+                    const int e = ro + q * rows;
+                    if (col && e < cnt) {
+                        const int64_t f = (int64_t)jn[q] * k + c;
+                        const T zj = d ? d[jn[q]] * r[f] : r[f];
+                        pj[q] = first ? zj : fma_t(be, pold[f], zj);
+                    } else {
+                        pj[q] = T(0);
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < kSlots; ++q) prod[threadIdx.x + q * kBlock] = v[q] * pj[q];
+                __syncthreads();
+                if (col)
+                    for (int i = a + ro; i < b; i += rows) {
+                        T sum = T(0);
+                        for (int l = srp[i]; l < srp[i + 1]; ++l) sum += prod[(l - L0) * kpad + c];
+                        const int gi = g0 + i;
+                        const int64_t e = (int64_t)gi * k + c;
+                        const T zi = d ? d[gi] * r[e] : r[e];
+                        const T pi = first ? zi : fma_t(be, pold[e], zi);
+                        pnew[e] = pi;
+                        S.u[e] = sum;
+                        acc += (double)pi * (double)sum;
+                    }
+            } else if (col && ro == 0) {  // one row longer than a chunk (b == a + 1)
+                const int gi = g0 + a;
+                T sum = T(0);
+                for (int l = srp[a]; l < srp[a + 1]; ++l) {
+                    const int32_t j = ci[l];
+                    const int64_t f = (int64_t)j * k + c;
+                    const T zj = d ? d[j] * r[f] : r[f];
+                    sum += val[l] * (first ? zj : fma_t(be, pold[f], zj));
+                }
+                const int64_t e = (int64_t)gi * k + c;
+                const T zi = d ? d[gi] * r[e] : r[e];
+                const T pi = first ? zi : fma_t(be, pold[e], zi);
+                pnew[e] = pi;
+                S.u[e] = sum;
+                acc += (double)pi * (double)sum;
+            }
+            __syncthreads();  // prod and sb are reused by the next chunk
+            a = b;
+        }
+    }
+    return acc;
+}
+
 // ---- K1: direction + SpMM + partial p.u -----------------------------------------
 template <typename T>
 __global__ __launch_bounds__(kBlock) void bk_dir_spmm_kernel(const Tile* __restrict__ tiles, const SysDev<T>* __restrict__ sys,
-                                                             SysState* __restrict__ st) {
+                                                             SysState* __restrict__ st, int streaming) {
     const Tile tl = tiles[xcd_swizzle(blockIdx.x, gridDim.x)];
     SysState& ss = st[tl.slot];
     const SysDev<T>& S = sys[tl.slot];
@@ -198,7 +306,9 @@ __global__ __launch_bounds__(kBlock) void bk_dir_spmm_kernel(const Tile* __restr
     }
     if (tl.t == 0 && threadIdx.x == 0) ss.it_a = it;
     double acc = 0.0;
-    if (c < S.k) {
+    if (streaming) {
+        acc = k1_stream<T>(S, tl, c, ro, rows, it == 0, (T)(rz / rz_prev), pick(S.p, it + 1), pick(S.p, it));
+    } else if (c < S.k) {
         const bool first = it == 0;
         const T be = (T)(rz / rz_prev);
         const T* __restrict__ pold = pick(S.p, it + 1);
@@ -431,7 +541,8 @@ template <typename T> class CGBatch {
         }
         if (ntiles_ == 0) return TG_OK;
         for (int it = 0; it < iters; ++it) {
-            hipLaunchKernelGGL((bk_dir_spmm_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s_, dtiles_, dsys_, dst_);
+            hipLaunchKernelGGL((bk_dir_spmm_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s_, dtiles_, dsys_, dst_,
+                               g_batch_stream);
             hipLaunchKernelGGL((bk_update_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s_, dtiles_, dsys_, dst_);
         }
         TG_LAUNCH_CHECK();
@@ -977,6 +1088,12 @@ static int batch_advance(Batch& B, int iters) {
 
 int batch_tune_set(const char* key, int32_t value, bool* known) {
     *known = false;
+    if (std::string(key) == "batch_stream") {
+        *known = true;
+        if (value != 0 && value != 1) return fail(TG_ERR_INVALID, "tg_tune_set: batch_stream must be 0 or 1");
+        g_batch_stream = value;
+        return TG_OK;
+    }
     if (std::string(key) != "batch_tile_rows") return TG_OK;
     *known = true;
     if (value < kMinTileRows || value > 8192 || (value & (value - 1)))
@@ -985,7 +1102,10 @@ int batch_tune_set(const char* key, int32_t value, bool* known) {
     return TG_OK;
 }
 
-int32_t batch_tune_get(const char* key) { return std::string(key) == "batch_tile_rows" ? g_batch_tile_rows : -1; }
+int32_t batch_tune_get(const char* key) {
+    if (std::string(key) == "batch_stream") return g_batch_stream;
+    return std::string(key) == "batch_tile_rows" ? g_batch_tile_rows : -1;
+}
 
 }  // namespace tg
 

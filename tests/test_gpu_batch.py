@@ -89,3 +89,22 @@ def test_batch_alone_equals_mix_and_midflight(cuda, mixed):
         Xalone, _ = run_batch([sysl[j]], cuda, mixed, 1e-10)
         assert np.array_equal(Xalone[0], Xmix[j])
         assert np.array_equal(Xstaged[j], Xmix[j])
+
+
+def test_batch_spmm_kernels_agree(cuda):
+    """The streaming and the row-per-lane SpMM kernels of the engine run the same
+    recurrences: same iteration counts (within one) and the same solutions to rounding."""
+    from tensors_amd import _capi as capi
+
+    sysl = systems(6, seed=99)
+    before = capi.lib().tg_tune_get(b"batch_stream")
+    try:
+        capi.call("tg_tune_set", b"batch_stream", 1)
+        Xs, rs = run_batch(sysl, cuda, False, 1e-10)
+        capi.call("tg_tune_set", b"batch_stream", 0)
+        Xr, rr = run_batch(sysl, cuda, False, 1e-10)
+    finally:
+        capi.call("tg_tune_set", b"batch_stream", before)
+    for i in range(len(sysl)):
+        assert abs(rs[i]["iterations"] - rr[i]["iterations"]) <= 1
+        assert rel_err(Xs[i], Xr[i]) < 1e-8
