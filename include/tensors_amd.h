@@ -122,8 +122,8 @@ int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, doubl
  *                  1 = never (default), 2 / 4 / 8 = always that many (nrhs <= 4).
  *   "batch_tile_rows": rows per workgroup tile of systems added to a batched
  *                  engine from now on, a power of two in [256, 8192] (1024).
- *   "batch_stream": SpMM kernel of the batched engine, 1 = streaming chunks
- *                  (default), 0 = a row per lane.
+ *   "batch_stream": SpMM kernel of the batched engine, 0 = a row per lane
+ *                  (default), 1 = streaming chunks (measured slower so far).
  * Process-wide; not thread-safe against concurrent SpMM calls. */
 int tg_tune_set(const char* key, int32_t value);
 int32_t tg_tune_get(const char* key);

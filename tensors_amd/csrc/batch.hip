@@ -41,7 +41,7 @@ constexpr int kBatchMaxK = 8;
 constexpr int kTileRows = 1024;  // default rows per tile (tg_tune_set "batch_tile_rows")
 constexpr int kMinTileRows = 256;
 static int g_batch_tile_rows = kTileRows;
-static int g_batch_stream = 1;  // SpMM kernel of the engine: 1 = streaming chunks, 0 = a row per lane
+static int g_batch_stream = 0;  // SpMM kernel of the engine: 0 = a row per lane (measured faster), 1 = streaming chunks
 constexpr int kGather = 8;     // nonzeros per gather chunk in the SpMM kernel
 constexpr int kRowUnroll = 4;  // rows per step in the update kernel
 
@@ -275,6 +275,67 @@ __device__ double k1_stream(const SysDev<T>& S, const Tile& tl, int c, int ro, i
     return acc;
 }
 
+// ---- K1 body, a row per (row, column) lane -------------------------------------------
+// A row's nonzeros in chunks of kGather.  The loop body has no branches: the
+// index/value loads are made at clamped (always valid) positions and masked
+// afterwards, and the uniform choices (Jacobi or not, first iteration or not) are
+// template arguments, so the compiler issues all loads of a chunk before the first
+// wait (branches inside the unrolled body split it into blocks that each wait for
+// their loads).  The sum runs in nonzero order.
+template <typename T, bool HAS_D, bool FIRST>
+__device__ __forceinline__ double k1_rows(const SysDev<T>& S, const Tile& tl, int c, int ro, int rows, T be,
+                                          const T* __restrict__ pold, T* __restrict__ pnew) {
+    const T* __restrict__ r = S.r;
+    const T* __restrict__ d = S.d;
+    const int32_t* __restrict__ rp = S.rp;
+    const int32_t* __restrict__ ci = S.ci;
+    const T* __restrict__ val = S.val;
+    T* __restrict__ u = S.u;
+    const int k = S.k;
+    double acc = 0.0;
+    for (int i = tl.r0 + ro; i < tl.r1; i += rows) {
+        const int lb = rp[i], le = rp[i + 1];
+        T sum = T(0);
+        for (int l0 = lb; l0 < le; l0 += kGather) {
+            int32_t jj[kGather];
+            T vv[kGather], pj[kGather];
+#pragma unroll
+            for (int q = 0; q < kGather; ++q) {
+                const int lq = min(l0 + q, le - 1);
+                const int32_t cq = ci[lq];
+                const T vq = val[lq];
+                const bool ok = l0 + q < le;
+                jj[q] = ok ? cq : i;
+                vv[q] = ok ? vq : T(0);
+            }
+#pragma unroll
+            for (int q = 0; q < kGather; ++q) {
+                const int64_t f = (int64_t)jj[q] * k + c;
+                T zj = r[f];
+                if constexpr (HAS_D) zj = d[jj[q]] * zj;
+                if constexpr (FIRST)
+                    pj[q] = zj;
+                else
+                    pj[q] = fma_t(be, pold[f], zj);
+            }
+#pragma unroll
+            for (int q = 0; q < kGather; ++q) sum += (l0 + q < le) ? vv[q] * pj[q] : T(0);
+        }
+        const int64_t e = (int64_t)i * k + c;
+        T zi = r[e];
+        if constexpr (HAS_D) zi = d[i] * zi;
+        T pi;
+        if constexpr (FIRST)
+            pi = zi;
+        else
+            pi = fma_t(be, pold[e], zi);
+        pnew[e] = pi;
+        u[e] = sum;
+        acc += (double)pi * (double)sum;
+    }
+    return acc;
+}
+
 // ---- K1: direction + SpMM + partial p.u -----------------------------------------
 template <typename T>
 __global__ __launch_bounds__(kBlock) void bk_dir_spmm_kernel(const Tile* __restrict__ tiles, const SysDev<T>* __restrict__ sys,
@@ -309,50 +370,56 @@ __global__ __launch_bounds__(kBlock) void bk_dir_spmm_kernel(const Tile* __restr
     if (streaming) {
         acc = k1_stream<T>(S, tl, c, ro, rows, it == 0, (T)(rz / rz_prev), pick(S.p, it + 1), pick(S.p, it));
     } else if (c < S.k) {
-        const bool first = it == 0;
         const T be = (T)(rz / rz_prev);
-        const T* __restrict__ pold = pick(S.p, it + 1);
-        T* __restrict__ pnew = pick(S.p, it);
-        const T* __restrict__ r = S.r;
-        const T* __restrict__ d = S.d;
-        const int32_t* __restrict__ rp = S.rp;
-        const int32_t* __restrict__ ci = S.ci;
-        const T* __restrict__ val = S.val;
-        const int k = S.k;
-        for (int i = tl.r0 + ro; i < tl.r1; i += rows) {
-            // A row's nonzeros in chunks of kGather: all index/value loads of a chunk,
-            // then all its gathers, are issued before any is used (memory-level
-            // parallelism); the sum still runs in nonzero order.
-            const int lb = rp[i], le = rp[i + 1];
-            T s = T(0);
-            for (int l0 = lb; l0 < le; l0 += kGather) {
-                int32_t jj[kGather];
-                T vv[kGather], pj[kGather];
-#pragma unroll
-                for (int q = 0; q < kGather; ++q) {
-                    const bool ok = l0 + q < le;
-                    jj[q] = ok ? ci[l0 + q] : i;
-                    vv[q] = ok ? val[l0 + q] : T(0);
-                }
-#pragma unroll
-                for (int q = 0; q < kGather; ++q) {
-                    const int64_t f = (int64_t)jj[q] * k + c;
-                    const T zj = d ? d[jj[q]] * r[f] : r[f];
-                    pj[q] = first ? zj : fma_t(be, pold[f], zj);
-                }
-#pragma unroll
-                for (int q = 0; q < kGather; ++q)
-                    if (l0 + q < le) s += vv[q] * pj[q];
-            }
-            const int64_t e = (int64_t)i * k + c;
-            const T zi = d ? d[i] * r[e] : r[e];
-            const T pi = first ? zi : fma_t(be, pold[e], zi);
-            pnew[e] = pi;
-            S.u[e] = s;
-            acc += (double)pi * (double)s;
-        }
+        const T* pold = pick(S.p, it + 1);
+        T* pnew = pick(S.p, it);
+        if (S.d)
+            acc = it == 0 ? k1_rows<T, true, true>(S, tl, c, ro, rows, be, pold, pnew)
+                          : k1_rows<T, true, false>(S, tl, c, ro, rows, be, pold, pnew);
+        else
+            acc = it == 0 ? k1_rows<T, false, true>(S, tl, c, ro, rows, be, pold, pnew)
+                          : k1_rows<T, false, false>(S, tl, c, ro, rows, be, pold, pnew);
     }
     tile_partial(acc, S.k, S.kpad, S.ppu, tl.t);
+}
+
+// ---- K2 body: kRowUnroll rows per step, all loads first, then the stores ----------
+template <typename T, bool HAS_D>
+__device__ __forceinline__ double k2_rows(const SysDev<T>& S, const Tile& tl, int c, int ro, int rows, T al,
+                                          const T* __restrict__ p) {
+    const T* __restrict__ u = S.u;
+    const T* __restrict__ d = S.d;
+    T* __restrict__ x = S.x;
+    T* __restrict__ r = S.r;
+    const int k = S.k;
+    double acc = 0.0;
+    for (int i0 = tl.r0 + ro; i0 < tl.r1; i0 += kRowUnroll * rows) {
+        T pv[kRowUnroll], xv[kRowUnroll], rv[kRowUnroll], uv[kRowUnroll], dv[kRowUnroll];
+#pragma unroll
+        for (int q = 0; q < kRowUnroll; ++q) {
+            const int i = min(i0 + q * rows, tl.r1 - 1);  // clamped: always a valid row
+            const int64_t e = (int64_t)i * k + c;
+            pv[q] = p[e];
+            xv[q] = x[e];
+            rv[q] = r[e];
+            uv[q] = u[e];
+            if constexpr (HAS_D) dv[q] = d[i];
+        }
+#pragma unroll
+        for (int q = 0; q < kRowUnroll; ++q) {
+            const int i = i0 + q * rows;
+            if (i < tl.r1) {
+                const int64_t e = (int64_t)i * k + c;
+                x[e] = xv[q] + al * pv[q];
+                const T rn = rv[q] - al * uv[q];
+                r[e] = rn;
+                T zv = rn;
+                if constexpr (HAS_D) zv = dv[q] * rn;
+                acc += (double)rn * (double)zv;
+            }
+        }
+    }
+    return acc;
 }
 
 // ---- K2: x, r, z update + partial r.z (next parity) -------------------------------
@@ -370,40 +437,8 @@ __global__ __launch_bounds__(kBlock) void bk_update_kernel(const Tile* __restric
     double acc = 0.0;
     if (c < S.k) {
         const T al = (T)(rz / pu);
-        const T* __restrict__ p = pick(S.p, it);
-        const T* __restrict__ u = S.u;
-        const T* __restrict__ d = S.d;
-        T* __restrict__ x = S.x;
-        T* __restrict__ r = S.r;
-        const int k = S.k;
-        // kRowUnroll rows per step: all loads first, then the stores (the arrays are
-        // distinct, so nothing a step stores is read by it)
-        for (int i0 = tl.r0 + ro; i0 < tl.r1; i0 += kRowUnroll * rows) {
-            T pv[kRowUnroll], xv[kRowUnroll], rv[kRowUnroll], uv[kRowUnroll], dv[kRowUnroll];
-#pragma unroll
-            for (int q = 0; q < kRowUnroll; ++q) {
-                const int i = i0 + q * rows;
-                const bool ok = i < tl.r1;
-                const int64_t e = (int64_t)(ok ? i : i0) * k + c;
-                pv[q] = p[e];
-                xv[q] = x[e];
-                rv[q] = r[e];
-                uv[q] = u[e];
-                dv[q] = d ? d[ok ? i : i0] : T(1);
-            }
-#pragma unroll
-            for (int q = 0; q < kRowUnroll; ++q) {
-                const int i = i0 + q * rows;
-                if (i < tl.r1) {
-                    const int64_t e = (int64_t)i * k + c;
-                    x[e] = xv[q] + al * pv[q];
-                    const T rn = rv[q] - al * uv[q];
-                    r[e] = rn;
-                    const T zv = d ? dv[q] * rn : rn;
-                    acc += (double)rn * (double)zv;
-                }
-            }
-        }
+        acc = S.d ? k2_rows<T, true>(S, tl, c, ro, rows, al, pick(S.p, it))
+                  : k2_rows<T, false>(S, tl, c, ro, rows, al, pick(S.p, it));
     }
     tile_partial(acc, S.k, S.kpad, pick(S.prz, it + 1), tl.t);
     if (tl.t == 0 && threadIdx.x == 0) ss.it_b = it + 1;
@@ -743,10 +778,13 @@ __global__ __launch_bounds__(kBlock) void bm_replace_many_kernel(const ReplaceTi
                 int32_t jj[kGather];
                 double vv[kGather], xv[kGather];
 #pragma unroll
-                for (int q = 0; q < kGather; ++q) {
+                for (int q = 0; q < kGather; ++q) {  // clamped loads, masked values: no branches
+                    const int lq = min(l0 + q, le - 1);
+                    const int32_t cq = ci[lq];
+                    const double vq = val[lq];
                     const bool ok = l0 + q < le;
-                    jj[q] = ok ? ci[l0 + q] : i;
-                    vv[q] = ok ? val[l0 + q] : 0.0;
+                    jj[q] = ok ? cq : i;
+                    vv[q] = ok ? vq : 0.0;
                 }
 #pragma unroll
                 for (int q = 0; q < kGather; ++q) {
@@ -754,8 +792,7 @@ __global__ __launch_bounds__(kBlock) void bm_replace_many_kernel(const ReplaceTi
                     xv[q] = x64[f] + (double)x32[f];
                 }
 #pragma unroll
-                for (int q = 0; q < kGather; ++q)
-                    if (l0 + q < le) ax += vv[q] * xv[q];
+                for (int q = 0; q < kGather; ++q) ax += (l0 + q < le) ? vv[q] * xv[q] : 0.0;
             }
             const int64_t e = (int64_t)i * k + c;
             const double r = D.b[e] - ax;

@@ -41,15 +41,25 @@ __device__ __forceinline__ void column_partial(double v, const Lanes& L, int k, 
     if (L.ro == 0 && L.c < k) partial[(int64_t)blockIdx.x * k + L.c] = sh[L.c];
 }
 
-// Sum of the B workgroup partials of each column in index order, by one
-// workgroup: thread (c, j) adds partials j, j+J, ...; then a fixed tree over j.
+// Sum of the B workgroup partials of each column in a fixed order, by one
+// workgroup: thread (c, j) adds partials j, j+J, ... into four running sums
+// (j, j+4J, ... / j+J, j+5J, ... / ...), then a fixed tree over j.
 __device__ __forceinline__ double column_total(const double* partial, int B, int k, int kpad, int c, int j,
                                                int J) {
     __shared__ double sh[kBlock];
-    double s = 0.0;
-    if (c < k)
-        for (int b = j; b < B; b += J) s += partial[(int64_t)b * k + c];
-    sh[threadIdx.x] = s;
+    // four independent running sums (loads in flight together), combined in a fixed order
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    if (c < k) {
+        int b = j;
+        for (; b + 3 * J < B; b += 4 * J) {
+            s0 += partial[(int64_t)b * k + c];
+            s1 += partial[(int64_t)(b + J) * k + c];
+            s2 += partial[(int64_t)(b + 2 * J) * k + c];
+            s3 += partial[(int64_t)(b + 3 * J) * k + c];
+        }
+        for (; b < B; b += J) s0 += partial[(int64_t)b * k + c];
+    }
+    sh[threadIdx.x] = (s0 + s1) + (s2 + s3);
     __syncthreads();
     for (int h = J / 2; h > 0; h >>= 1) {
         if (j < h) sh[threadIdx.x] += sh[threadIdx.x + h * kpad];

@@ -189,10 +189,15 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(
                     av[j] = a ? __builtin_nontemporal_load(a + cs + e) : T(1);
                 }
             }
+            // gathers unconditionally (c = 0 for the lanes past the chunk, a valid
+            // entry) so that all ELEMS are in flight together; only the store is masked
+            T xv[ELEMS];
+#pragma unroll
+            for (int j = 0; j < ELEMS; ++j) xv[j] = x[(int64_t)c[j] * ldx];
 #pragma unroll
             for (int j = 0; j < ELEMS; ++j) {
                 const int e = threadIdx.x + j * kBlock;
-                if (e < cnt) prod[e] = av[j] * x[(int64_t)c[j] * ldx];
+                if (e < cnt) prod[e] = av[j] * xv[j];
             }
             __syncthreads();
             for (int i = team; i < nr; i += TEAMS) {
@@ -771,14 +776,17 @@ __global__ __launch_bounds__(kBlock) void spmm_slice_stream_kernel(int m, int S,
                     av[j] = a ? __builtin_nontemporal_load(a + l) : T(1);
                 }
             }
+            T xv[ELEMS][K];  // gathered unconditionally (c = 0 past the chunk) so they overlap
+#pragma unroll
+            for (int j = 0; j < ELEMS; ++j)
+#pragma unroll
+                for (int q = 0; q < K; ++q) xv[j][q] = X[(int64_t)c[j] * ldX + q];
 #pragma unroll
             for (int j = 0; j < ELEMS; ++j) {
                 const int e = threadIdx.x + j * kBlock;
-                if (e < cnt) {
-                    const T* xr = X + (int64_t)c[j] * ldX;
+                if (e < cnt)
 #pragma unroll
-                    for (int q = 0; q < K; ++q) prod[e * K + q] = av[j] * xr[q];
-                }
+                    for (int q = 0; q < K; ++q) prod[e * K + q] = av[j] * xv[j][q];
             }
             __syncthreads();
             for (int i = team; i < nr; i += TEAMS) {
