@@ -74,6 +74,9 @@ struct Tile {
     int slot, r0, r1, t;
 };
 
+// Pointer into the global address space.
+template <typename T> using gptr = __attribute__((address_space(1))) T*;
+
 // Parity selection without indexing a local array (which would live in scratch).
 template <typename P> __host__ __device__ __forceinline__ P pick(P const (&a)[2], int it) {
     const P a0 = a[0], a1 = a[1];
@@ -284,13 +287,17 @@ __device__ double k1_stream(const SysDev<T>& S, const Tile& tl, int c, int ro, i
 // their loads).  The sum runs in nonzero order.
 template <typename T, bool HAS_D, bool FIRST>
 __device__ __forceinline__ double k1_rows(const SysDev<T>& S, const Tile& tl, int c, int ro, int rows, T be,
-                                          const T* __restrict__ pold, T* __restrict__ pnew) {
-    const T* __restrict__ r = S.r;
-    const T* __restrict__ d = S.d;
-    const int32_t* __restrict__ rp = S.rp;
-    const int32_t* __restrict__ ci = S.ci;
-    const T* __restrict__ val = S.val;
-    T* __restrict__ u = S.u;
+                                          const T* __restrict__ pold_, T* __restrict__ pnew_) {
+    // global-address-space views: the pointers come from a table in memory, so the
+    // compiler cannot tell them from LDS pointers and would use flat accesses
+    const gptr<const T> pold = (gptr<const T>)pold_;
+    const gptr<T> pnew = (gptr<T>)pnew_;
+    const gptr<const T> r = (gptr<const T>)S.r;
+    const gptr<const T> d = (gptr<const T>)S.d;
+    const gptr<const int32_t> rp = (gptr<const int32_t>)S.rp;
+    const gptr<const int32_t> ci = (gptr<const int32_t>)S.ci;
+    const gptr<const T> val = (gptr<const T>)S.val;
+    const gptr<T> u = (gptr<T>)S.u;
     const int k = S.k;
     double acc = 0.0;
     for (int i = tl.r0 + ro; i < tl.r1; i += rows) {
@@ -386,11 +393,12 @@ __global__ __launch_bounds__(kBlock) void bk_dir_spmm_kernel(const Tile* __restr
 // ---- K2 body: kRowUnroll rows per step, all loads first, then the stores ----------
 template <typename T, bool HAS_D>
 __device__ __forceinline__ double k2_rows(const SysDev<T>& S, const Tile& tl, int c, int ro, int rows, T al,
-                                          const T* __restrict__ p) {
-    const T* __restrict__ u = S.u;
-    const T* __restrict__ d = S.d;
-    T* __restrict__ x = S.x;
-    T* __restrict__ r = S.r;
+                                          const T* __restrict__ p_) {
+    const gptr<const T> p = (gptr<const T>)p_;
+    const gptr<const T> u = (gptr<const T>)S.u;
+    const gptr<const T> d = (gptr<const T>)S.d;
+    const gptr<T> x = (gptr<T>)S.x;
+    const gptr<T> r = (gptr<T>)S.r;
     const int k = S.k;
     double acc = 0.0;
     for (int i0 = tl.r0 + ro; i0 < tl.r1; i0 += kRowUnroll * rows) {
