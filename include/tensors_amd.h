@@ -118,12 +118,18 @@ int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, doubl
  *   "spmm_rows":   nrhs >= 2 kernel, 0 = from the mean row length (up to 16:
  *                  row per lane group), 1 = the wave's lane groups split one row,
  *                  2 = each lane group owns a row.
- *   "spmm_slices": column slices of tg_csr_spmm_plan, 0 = from the size of X,
- *                  1 = never (default), 2 / 4 / 8 = always that many (nrhs <= 4).
+ *   "spmm_slices": column slices of tg_csr_spmm_plan, 0 = from the size of X
+ *                  (default: 2 or 4 slices for X blocks of 6-32 MB, else none),
+ *                  1 = never, 2 / 4 / 8 = always that many (nrhs <= 4).
  *   "batch_tile_rows": rows per workgroup tile of systems added to a batched
  *                  engine from now on, a power of two in [256, 8192] (1024).
  *   "batch_stream": SpMM kernel of the batched engine, 0 = a row per lane
  *                  (default), 1 = streaming chunks (measured slower so far).
+ *   "cg_fused":    iteration of tg_cg_solve / tg_cg_solve_mixed for CSR operators
+ *                  with P = I or Jacobi and mean row length <= 16: 0 = SpMM then
+ *                  four vector kernels, 1 = p.u in the SpMM epilogue, 2 = p
+ *                  update, SpMM and p.u in one kernel plus one vector kernel,
+ *                  3 = default (form 1).  Results agree to rounding.
  * Process-wide; not thread-safe against concurrent SpMM calls. */
 int tg_tune_set(const char* key, int32_t value);
 int32_t tg_tune_get(const char* key);
@@ -371,6 +377,31 @@ int64_t tg_gen_weighted_grid(int32_t dims, int32_t s0, int32_t s1, int32_t s2, d
                              int64_t* rp, int32_t* ci, double* val);
 int64_t tg_gen_convection_diffusion_2d(int32_t g, double peclet, double theta, int64_t* rp, int32_t* ci,
                                        double* val);
+
+/* ------------------------------------------------------------------------
+ * MatrixMarket coordinate files (HOST buffers; no device work).  Replace
+ * MatrixCSR::LoadFromMatrixMarket / WriteToMatrixMarket
+ * (MatrixCSR/MatrixMarket.hpp:3-306); loading = tg_mm_info + tg_mm_read, then
+ * tg_csr_from_triples with compress = 1 and symmetrize = the file's symmetric
+ * flag (the reference's MatrixCSR(nnz, i, j, a, m, n, threads, true,
+ * symmetrizeQ), :190-196).
+ *
+ * tg_mm_info: banner and size line.  Field "complex" -> TG_ERR_DTYPE;
+ * symmetry other than general / symmetric -> TG_ERR_INVALID.
+ * tg_mm_read: the first `entries` (= the announced count) entry lines, in
+ * file order, as zero-based I, J (int32) and V (double; 1 for "pattern"
+ * files; V may be NULL).  A malformed line or an index outside [1, m] x
+ * [1, n] -> TG_ERR_INVALID naming the entry.  threads <= 0: all cores.
+ * tg_mm_write: "coordinate real general" (or "pattern" when vals is NULL),
+ * one "i j value" line per nonzero in row order, %.17E for F64, %.8E for F32
+ * (enough digits that every float reads back exactly).  rp / ci / vals are
+ * host arrays.
+ * ------------------------------------------------------------------------ */
+typedef enum tg_mm_field { TG_MM_REAL = 0, TG_MM_INTEGER = 1, TG_MM_PATTERN = 2 } tg_mm_field;
+int tg_mm_info(const char* path, int32_t* m, int32_t* n, int64_t* entries, int32_t* field, int32_t* symmetric);
+int tg_mm_read(const char* path, int64_t entries, int32_t* I, int32_t* J, double* V, int32_t threads);
+int tg_mm_write(const char* path, int32_t m, int32_t n, const void* rp, tg_dtype rp_type, const int32_t* ci,
+                const void* vals, tg_dtype val_type, int32_t threads);
 
 #ifdef __cplusplus
 }

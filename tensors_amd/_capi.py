@@ -49,6 +49,9 @@ SIGNATURES = {
                                  vp, C.c_size_t, vp, vp, vp, vp]),
     "tg_gmres_solve_op": (C.c_int, [i32, vp, vp, vp, vp, C.c_int, i32, i32, i32, dbl, vp, i64, dbl, vp, i64,
                                     dbl, i32, i32, vp, C.c_size_t, vp, vp, vp, vp, vp]),
+    "tg_mm_info": (C.c_int, [C.c_char_p, vp, vp, vp, vp, vp]),
+    "tg_mm_read": (C.c_int, [C.c_char_p, i64, vp, vp, vp, i32]),
+    "tg_mm_write": (C.c_int, [C.c_char_p, i32, i32, vp, C.c_int, vp, vp, C.c_int, i32]),
     "tg_csr_from_triples_workspace_bytes": (C.c_size_t, [i64, i32, i32]),
     "tg_csr_from_triples": (C.c_int, [i64, vp, vp, vp, C.c_int, i32, i32, i32, i32, vp, C.c_int, vp, vp, i64,
                                       vp, C.c_size_t, vp, vp]),

@@ -20,6 +20,8 @@
 // once, so the host enqueues iterations in batches and polls the flag between
 // batches instead of synchronising every iteration.
 
+#include <string>
+
 #include "common.hpp"
 #include "krylov.hpp"
 #include "reduce.hpp"
@@ -79,10 +81,10 @@ __global__ __launch_bounds__(kBlock) void cg_guess_kernel(int n, int k, int kpad
 
 // mode 0: rho = sum, bsq = |rho|, TOL = bsq * tol^2, nan/zero flags (:171-211)
 // mode 1: rho = sum (after the initial-guess residual)
-__global__ __launch_bounds__(kBlock) void cg_setup_finalize_kernel(int k, int kpad, int B, int mode, double tol2,
+__global__ __launch_bounds__(kFinalThreads) void cg_setup_finalize_kernel(int k, int kpad, int B, int mode, double tol2,
                                                                    const double* __restrict__ partial,
                                                                    CGState* __restrict__ st) {
-    const int c = threadIdx.x % kpad, j = threadIdx.x / kpad, J = kBlock / kpad;
+    const int c = threadIdx.x % kpad, j = threadIdx.x / kpad, J = kFinalThreads / kpad;
     const double tot = column_total(partial, B, k, kpad, c, j, J);
     if (j == 0 && c < k) {
         st->rho[c] = tot;
@@ -139,10 +141,10 @@ __global__ __launch_bounds__(kBlock) void cg_dot_pu_kernel(int n, int k, int kpa
 }
 
 // alpha = rho / (p.u)   (:255-260)
-__global__ __launch_bounds__(kBlock) void cg_alpha_kernel(int k, int kpad, int B, const double* __restrict__ partial,
+__global__ __launch_bounds__(kFinalThreads) void cg_alpha_kernel(int k, int kpad, int B, const double* __restrict__ partial,
                                                           CGState* __restrict__ st) {
     if (st->done) return;
-    const int c = threadIdx.x % kpad, j = threadIdx.x / kpad, J = kBlock / kpad;
+    const int c = threadIdx.x % kpad, j = threadIdx.x / kpad, J = kFinalThreads / kpad;
     const double pu = column_total(partial, B, k, kpad, c, j, J);
     if (j == 0 && c < k) st->alpha[c] = st->rho[c] / pu;
 }
@@ -180,10 +182,10 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(int n, int k, int kpa
 }
 
 // rho_old = rho; rho = r.z; beta = rho / rho_old; convergence; ++iter  (:309-338)
-__global__ __launch_bounds__(kBlock) void cg_rho_kernel(int k, int kpad, int B, const double* __restrict__ partial,
+__global__ __launch_bounds__(kFinalThreads) void cg_rho_kernel(int k, int kpad, int B, const double* __restrict__ partial,
                                                         CGState* __restrict__ st) {
     if (st->done) return;
-    const int c = threadIdx.x % kpad, j = threadIdx.x / kpad, J = kBlock / kpad;
+    const int c = threadIdx.x % kpad, j = threadIdx.x / kpad, J = kFinalThreads / kpad;
     const double rz = column_total(partial, B, k, kpad, c, j, J);
     __shared__ int conv;
     if (threadIdx.x == 0) conv = 1;
@@ -218,6 +220,190 @@ __global__ __launch_bounds__(kBlock) void cg_p_kernel(int n, int k, int kpad, co
             p[e] = z[e] + be * p[e];
         }
     }
+}
+
+// --- fused iteration forms (CSR operator, P = I or Jacobi) ------------------
+// Form 1 keeps the four-kernel order but computes partial(p.u) in the SpMM
+// epilogue (cg_dir_spmm_kernel with FORM = false), saving the separate p.u pass.
+// Form 2 has two vector kernels per iteration instead of four:
+//   K1 (cg_dir_spmm_kernel):  p = z + beta p_prev;  u = A p;  partial(p.u)
+//   K2 (cg_step_kernel):      x += alpha p;  r -= alpha u;  partial(r.z)
+// with z = r (P = I) or z = d r (Jacobi) formed where it is needed and never
+// stored.  p is double-buffered by the parity of the iteration count: K1 of
+// iteration t writes p[t & 1] and reads p[(t - 1) & 1]; p[1] is zero before the
+// first iteration, where beta = 0, so p = z exactly as after the reference's
+// copy (ConjugateGradient.hpp:247-253).  A row needs p of the rows its columns
+// name, which other workgroups write in the same launch; it recomputes them from
+// z and p_prev with the same fused multiply-add, so the values are bitwise those
+// the owners store and no workgroup reads what another writes.
+
+template <typename T>
+__device__ __forceinline__ T fma_t(T a, T b, T c) {
+    if constexpr (sizeof(T) == 8)
+        return __builtin_fma(a, b, c);
+    else
+        return __builtin_fmaf(a, b, c);
+}
+
+// One row per lane group of G lanes; each lane holds V consecutive columns of a
+// G*V-column tile (k > G*V only for G = 64: further tiles).  The workgroup's p.u
+// partial is summed per (wave, lane group, column) in row order in LDS, then by a
+// fixed tree over the lane groups: deterministic.
+// FORM = false is the plain "u = A p, partial(p.u)" step of the three-pass
+// iteration (p = p0 is formed by cg_p_kernel; r, d, p1 unused).
+template <typename T, typename RP, int G, int V, bool FORM, bool HAS_D>
+__global__ __launch_bounds__(kBlock) void cg_dir_spmm_kernel(int n, int k, const RP* __restrict__ rp,
+                                                             const int32_t* __restrict__ ci, const T* __restrict__ a,
+                                                             const T* __restrict__ r, const T* __restrict__ d,
+                                                             T* __restrict__ p0, T* __restrict__ p1,
+                                                             T* __restrict__ u, double* __restrict__ partial,
+                                                             const CGState* __restrict__ st) {
+    if (st->done) return;
+    constexpr int S = kWave / G;  // rows in flight per wave
+    constexpr int KT = G * V;     // columns per tile
+    constexpr int WAVES = kBlock / kWave;
+    constexpr int U = 4;
+    __shared__ double sh[WAVES * kWave * 4];  // WAVES * S * k <= 1024 (k <= G V, or G = 64 and k <= 256)
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    const int s = lane / G, g = lane % G;
+    const int slot = wave * S + s;
+    const int slots = WAVES * S;
+    for (int i = threadIdx.x; i < slots * k; i += kBlock) sh[i] = 0.0;
+    __syncthreads();
+    const int B = gridDim.x;
+    const int lb = xcd_swizzle(blockIdx.x, B);
+    const int i0 = (int)((int64_t)n * lb / B), i1 = (int)((int64_t)n * (lb + 1) / B);
+    const bool odd = FORM && (st->iter & 1) != 0;
+    const T* __restrict__ pp = (odd || !FORM) ? p0 : p1;  // FORM: written by the previous iteration
+    T* __restrict__ pn = odd ? p1 : p0;
+    for (int rbase = i0 + wave * S; rbase < i1; rbase += WAVES * S) {
+        const int row = rbase + s;
+        const bool rok = row < i1;
+        const RP lo = rok ? rp[row] : RP(0), hi = rok ? rp[row + 1] : RP(0);
+        const int len = (int)(hi - lo);
+        int wmax = len;  // longest row of the wave (uniform loop bound)
+#pragma unroll
+        for (int off = G; off < kWave; off <<= 1) wmax = max(wmax, __shfl_xor(wmax, off, kWave));
+        const T di = (HAS_D && rok) ? d[row] : T(1);
+        for (int c0 = 0; c0 < k; c0 += KT) {
+            const int col = c0 + g * V;
+            const bool colok = col < k;
+            const int colc = colok ? col : 0;
+            T be[V];
+#pragma unroll
+            for (int i = 0; i < V; ++i) be[i] = FORM ? (T)st->beta[colc + i] : T(0);
+            T acc[V];
+#pragma unroll
+            for (int i = 0; i < V; ++i) acc[i] = T(0);
+            for (int base = 0; base < wmax; base += G) {
+                const int e = base + g;
+                int32_t myc = 0;
+                T mya = T(0);
+                if (e < len) {
+                    myc = __builtin_nontemporal_load(ci + lo + e);
+                    mya = a ? __builtin_nontemporal_load(a + lo + e) : T(1);
+                }
+                const int cnt = min(G, len - base);  // this group's pairs in the chunk
+                const int wcnt = min(G, wmax - base);
+                for (int u0 = 0; u0 < wcnt; u0 += U) {
+                    // U (column, value) pairs from the group's lanes, then all their
+                    // gathers in flight together, then the arithmetic
+                    int cj[U];
+                    T av[U];
+                    bool live[U];
+#pragma unroll
+                    for (int uu = 0; uu < U; ++uu) {
+                        const int j = u0 + uu;
+                        const int src = s * G + (j < G ? j : 0);
+                        const int32_t c = __shfl(myc, src, kWave);
+                        av[uu] = __shfl(mya, src, kWave);
+                        live[uu] = (j < cnt) && colok;
+                        cj[uu] = live[uu] ? c : 0;
+                    }
+                    T rv[U][V], pv[U][V], dj[U];
+#pragma unroll
+                    for (int uu = 0; uu < U; ++uu) {
+                        const int64_t o = (int64_t)cj[uu] * k + colc;
+                        if constexpr (FORM) load_vec<T, V>(r + o, rv[uu]);
+                        load_vec<T, V>(pp + o, pv[uu]);
+                        dj[uu] = HAS_D ? d[cj[uu]] : T(1);
+                    }
+#pragma unroll
+                    for (int uu = 0; uu < U; ++uu) {
+#pragma unroll
+                        for (int i = 0; i < V; ++i) {
+                            T pj = pv[uu][i];
+                            if constexpr (FORM) {
+                                const T z = HAS_D ? dj[uu] * rv[uu][i] : rv[uu][i];
+                                pj = fma_t(be[i], pv[uu][i], z);
+                            }
+                            acc[i] += live[uu] ? av[uu] * pj : T(0);
+                        }
+                    }
+                }
+            }
+            if (rok && colok) {
+                const int64_t o = (int64_t)row * k + col;
+                T rv[V], pv[V], pi[V];
+                if constexpr (FORM) load_vec<T, V>(r + o, rv);
+                load_vec<T, V>(pp + o, pv);
+                double* shs = sh + (int64_t)slot * k + col;
+#pragma unroll
+                for (int i = 0; i < V; ++i) {
+                    pi[i] = pv[i];
+                    if constexpr (FORM) {
+                        const T z = HAS_D ? di * rv[i] : rv[i];
+                        pi[i] = fma_t(be[i], pv[i], z);
+                    }
+                    shs[i] += (double)pi[i] * (double)acc[i];
+                }
+                if constexpr (FORM) store_vec<T, V>(pn + o, pi);
+                store_vec<T, V>(u + o, acc);
+            }
+        }
+    }
+    __syncthreads();
+    // fixed tree over the lane-group slots (slots is a power of two)
+    for (int h = slots / 2; h > 0; h >>= 1) {
+        for (int t = threadIdx.x; t < h * k; t += kBlock) sh[t] += sh[t + h * k];
+        __syncthreads();
+    }
+    for (int c = threadIdx.x; c < k; c += kBlock) partial[(int64_t)lb * k + c] = sh[c];
+}
+
+// x = (first ? 0 : x) + alpha p;  r -= alpha u;  partial(r.z), z = r or d r
+template <typename T, bool HAS_D>
+__global__ __launch_bounds__(kBlock) void cg_step_kernel(int n, int k, int kpad, const T* __restrict__ p0,
+                                                         const T* __restrict__ p1, const T* __restrict__ u,
+                                                         const T* __restrict__ d, T* __restrict__ x,
+                                                         T* __restrict__ r, double* __restrict__ partial,
+                                                         const CGState* __restrict__ st) {
+    if (st->done) return;
+    int i0, i1;
+    block_row_range(n, i0, i1);
+    const Lanes L = lanes_of(kpad);
+    const T* __restrict__ p = (st->iter & 1) ? p1 : p0;
+    const bool first = (st->iter == 0) && !st->use_guess;
+    double acc = 0.0;
+    if (L.c < k) {
+        const T al = (T)st->alpha[L.c];
+        for (int i = i0 + L.ro; i < i1; i += L.rows) {
+            const int64_t e = (int64_t)i * k + L.c;
+            const T pv = p[e];
+            x[e] = first ? al * pv : x[e] + al * pv;
+            const T rv = r[e] - al * u[e];
+            r[e] = rv;
+            const T zv = HAS_D ? d[i] * rv : rv;
+            acc += (double)rv * (double)zv;
+        }
+    }
+    column_partial(acc, L, k, kpad, partial);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void cg_zero_kernel(int64_t count, T* __restrict__ v) {
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < count; e += (int64_t)gridDim.x * kBlock)
+        v[e] = T(0);
 }
 
 // X <- a x + b X  with combine_matrices_auto's shortcuts (:342-346)
@@ -256,10 +442,92 @@ static int pow2ceil_i(int v) {
     return p;
 }
 
+// Workgroup partials of the dots: kSolverBlocks per column for the vector
+// kernels, kFusedBlocks for the fused direction + SpMM kernel.
+constexpr int kPartialBlocks = kFusedBlocks > kSolverBlocks ? kFusedBlocks : kSolverBlocks;
+
 size_t cg_workspace_bytes(int32_t n, int32_t k, size_t vs, int prec_kind) {
     const size_t vec = align256((size_t)n * k * vs);
-    const size_t nvec = prec_kind ? 5 : 4;  // r, u, p, x (+ z)
-    return align256(sizeof(CGState)) + align256((size_t)kSolverBlocks * k * sizeof(double)) + nvec * vec;
+    const size_t nvec = prec_kind ? 6 : 5;  // r, u, p, x (+ z), second p of the fused iteration
+    return align256(sizeof(CGState)) + align256((size_t)kPartialBlocks * k * sizeof(double)) + nvec * vec;
+}
+
+// Iteration form for CSR operators with P = I or Jacobi, short rows (mean length
+// <= 16) and no column slabs ("cg_fused"): 0 = four vector kernels after the
+// SpMM; 1 = p.u in the SpMM epilogue (kSpmmDot); 2 = direction, SpMM and p.u in
+// one kernel (kFormDir); 3 = default (kSpmmDot).  On the 128^3 Laplacian with 32
+// right-hand sides form 1 measured fastest in FP64 and FP32: gathering both z and
+// p_prev per nonzero (form 2) cost more than streaming p once.  Other operators
+// always take form 0.
+enum CgForm : int { kFourKernels = 0, kSpmmDot = 1, kFormDir = 2, kFormAuto = 3 };
+static int g_cg_fused = kFormAuto;
+
+int cg_tune_set(const char* key, int32_t value, bool* known) {
+    *known = false;
+    if (std::string(key) != "cg_fused") return TG_OK;
+    *known = true;
+    if (value < 0 || value > 3) return fail(TG_ERR_INVALID, "tg_tune_set: cg_fused must be 0, 1, 2 or 3 (auto)");
+    g_cg_fused = value;
+    return TG_OK;
+}
+
+int32_t cg_tune_get(const char* key) { return std::string(key) == "cg_fused" ? g_cg_fused : -1; }
+
+template <typename T, typename RP, int V>
+static int launch_dir_v(int G, int form, bool has_d, hipStream_t s, int n, int k, const RP* rp, const int32_t* ci,
+                        const T* a, const T* r, const T* d, T* p0, T* p1, T* u, double* partial, const CGState* st) {
+#define TG_DIRK(GG, FORM, HD)                                                                                  \
+    hipLaunchKernelGGL((cg_dir_spmm_kernel<T, RP, GG, V, FORM, HD>), dim3(kFusedBlocks), dim3(kBlock), 0, s, n, k, \
+                       rp, ci, a, r, d, p0, p1, u, partial, st)
+#define TG_DIR(GG)                     \
+    case GG:                           \
+        if (form != kFormDir)          \
+            TG_DIRK(GG, false, false); \
+        else if (has_d)                \
+            TG_DIRK(GG, true, true);   \
+        else                           \
+            TG_DIRK(GG, true, false);  \
+        break;
+    switch (G) {
+        TG_DIR(1)
+        TG_DIR(2)
+        TG_DIR(4)
+        TG_DIR(8)
+        TG_DIR(16)
+        TG_DIR(32)
+        TG_DIR(64)
+        default:
+            return fail(TG_ERR_INVALID, "internal: bad lane group %d", G);
+    }
+#undef TG_DIR
+#undef TG_DIRK
+    return TG_OK;
+}
+
+// The SpMM kernel of forms 1 and 2: V columns per lane (16-byte loads where k
+// allows), G = enough lanes for all k columns (at most 64; then k / (64 V) tiles).
+template <typename T, typename RP>
+static int launch_dir(int form, hipStream_t s, int n, int k, const RP* rp, const int32_t* ci, const T* a,
+                      const T* r, const T* d, T* p0, T* p1, T* u, double* partial, const CGState* st) {
+    int V = 1;
+    if (sizeof(T) == 8) {
+        if (k % 2 == 0) V = 2;
+    } else {
+        V = (k % 4 == 0) ? 4 : (k % 2 == 0) ? 2 : 1;
+    }
+    int G = 1;
+    while (G < 64 && G * V < k) G *= 2;
+    const bool has_d = d != nullptr;
+    switch (V) {
+        case 1:
+            return launch_dir_v<T, RP, 1>(G, form, has_d, s, n, k, rp, ci, a, r, d, p0, p1, u, partial, st);
+        case 2:
+            return launch_dir_v<T, RP, 2>(G, form, has_d, s, n, k, rp, ci, a, r, d, p0, p1, u, partial, st);
+        default:
+            if constexpr (sizeof(T) == 4)
+                return launch_dir_v<T, RP, 4>(G, form, has_d, s, n, k, rp, ci, a, r, d, p0, p1, u, partial, st);
+            return fail(TG_ERR_INVALID, "internal: bad vector width");
+    }
 }
 
 template <typename T>
@@ -292,18 +560,34 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
     CGState* st = (CGState*)w;
     w += align256(sizeof(CGState));
     double* partial = (double*)w;
-    w += align256((size_t)kSolverBlocks * k * sizeof(double));
+    w += align256((size_t)kPartialBlocks * k * sizeof(double));
     const size_t vec = align256((size_t)N * sizeof(T));
     T* r = (T*)w;
     T* u = (T*)(w + vec);
     T* p = (T*)(w + 2 * vec);
     T* x = (T*)(w + 3 * vec);
     T* z = prec_kind ? (T*)(w + 4 * vec) : r;
+    T* p1 = (T*)(w + (prec_kind ? 5 : 4) * vec);
     const T* dd = prec_kind == 1 ? d : nullptr;
     const int zmode = prec_kind;  // 0: z is r, 1: z = d r in the kernels, 2: z = P(r) by callback
     const int kpad = pow2ceil_i(k);
     const int B = kSolverBlocks;
     const bool host_ops = Aop.is_callback() || prec_kind == 2;
+    // forms 1 / 2: CSR operator, P = I or Jacobi, short rows, no column slabs
+    const CsrView& Ac = Aop.csr ? *Aop.csr : A;
+    const bool csr_ok = !host_ops && Aop.csr && n > 0 && Ac.m == n && Ac.n == n && Ac.nnz <= 16 * (int64_t)n &&
+                        !spmm_uses_slabs(Ac, k) && Ac.rp && (Ac.nnz == 0 || Ac.ci) &&
+                        Ac.val_type == (sizeof(T) == 8 ? TG_F64 : TG_F32);
+    int form = g_cg_fused == kFormAuto ? kSpmmDot : g_cg_fused;
+    if (!csr_ok) form = kFourKernels;
+    const bool fused = form == kFormDir;
+    auto dir_spmm = [&]() -> int {
+        if (Ac.rp_type == TG_I64)
+            return launch_dir<T, int64_t>(form, s, n, k, (const int64_t*)Ac.rp, Ac.ci, (const T*)Ac.val, r, dd, p,
+                                          p1, u, partial, st);
+        return launch_dir<T, int32_t>(form, s, n, k, (const int32_t*)Ac.rp, Ac.ci, (const T*)Ac.val, r, dd, p, p1,
+                                      u, partial, st);
+    };
 
     // z = P r by callback, then partial(r.z) -- the external-preconditioner step
     auto apply_P = [&]() -> int {
@@ -324,7 +608,7 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
         int rc = apply_P();
         if (rc) return rc;
     }
-    hipLaunchKernelGGL(cg_setup_finalize_kernel, dim3(1), dim3(kBlock), 0, s, k, kpad, B, 0, tol * tol, partial, st);
+    hipLaunchKernelGGL(cg_setup_finalize_kernel, dim3(1), dim3(kFinalThreads), 0, s, k, kpad, B, 0, tol * tol, partial, st);
     TG_LAUNCH_CHECK();
     CGState hs;
     TG_HIP(hipMemcpyAsync(&hs, st, sizeof(CGState), hipMemcpyDeviceToHost, s));
@@ -352,9 +636,12 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
             rc = apply_P();
             if (rc) return rc;
         }
-        hipLaunchKernelGGL(cg_setup_finalize_kernel, dim3(1), dim3(kBlock), 0, s, k, kpad, B, 1, 0.0, partial, st);
+        hipLaunchKernelGGL(cg_setup_finalize_kernel, dim3(1), dim3(kFinalThreads), 0, s, k, kpad, B, 1, 0.0, partial, st);
     }
-    hipLaunchKernelGGL((cg_copy_kernel<T>), dim3(grid_for<T>(N)), dim3(kBlock), 0, s, N, (const T*)z, p);
+    if (fused)  // p_prev of the first iteration (beta = 0 there)
+        hipLaunchKernelGGL((cg_zero_kernel<T>), dim3(grid_for<T>(N)), dim3(kBlock), 0, s, N, p1);
+    else
+        hipLaunchKernelGGL((cg_copy_kernel<T>), dim3(grid_for<T>(N)), dim3(kBlock), 0, s, N, (const T*)z, p);
     hipLaunchKernelGGL(cg_check_kernel, dim3(1), dim3(64), 0, s, k, st);
     TG_LAUNCH_CHECK();
 
@@ -378,19 +665,40 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
         }
     }
     while (!hflag[1] && launched < max_iter) {
-        for (int it = 0; it < batch && launched < max_iter; ++it, ++launched) {
-            rc = op_apply(Aop, p, u, k, &st->done, s);
+        for (int it = 0; it < batch && launched < max_iter && fused; ++it, ++launched) {
+            rc = dir_spmm();
             if (rc) break;
-            hipLaunchKernelGGL((cg_dot_pu_kernel<T>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, (const T*)p,
-                               (const T*)u, partial, (const CGState*)st);
-            hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kBlock), 0, s, k, kpad, B, (const double*)partial, st);
+            hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kFinalThreads), 0, s, k, kpad, kFusedBlocks,
+                               (const double*)partial, st);
+            if (dd)
+                hipLaunchKernelGGL((cg_step_kernel<T, true>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, (const T*)p,
+                                   (const T*)p1, (const T*)u, dd, x, r, partial, (const CGState*)st);
+            else
+                hipLaunchKernelGGL((cg_step_kernel<T, false>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, (const T*)p,
+                                   (const T*)p1, (const T*)u, dd, x, r, partial, (const CGState*)st);
+            hipLaunchKernelGGL(cg_rho_kernel, dim3(1), dim3(kFinalThreads), 0, s, k, kpad, B, (const double*)partial, st);
+        }
+        for (int it = 0; it < batch && launched < max_iter && !fused; ++it, ++launched) {
+            if (form == kSpmmDot) {  // u = A p and partial(p.u) in one kernel
+                rc = dir_spmm();
+                if (rc) break;
+                hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kFinalThreads), 0, s, k, kpad, kFusedBlocks,
+                                   (const double*)partial, st);
+            } else {
+                rc = op_apply(Aop, p, u, k, &st->done, s);
+                if (rc) break;
+                hipLaunchKernelGGL((cg_dot_pu_kernel<T>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, (const T*)p,
+                                   (const T*)u, partial, (const CGState*)st);
+                hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kFinalThreads), 0, s, k, kpad, B,
+                                   (const double*)partial, st);
+            }
             hipLaunchKernelGGL((cg_update_kernel<T>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, (const T*)p,
                                (const T*)u, dd, x, r, z, zmode, partial, (const CGState*)st);
             if (zmode == 2) {
                 rc = apply_P();
                 if (rc) break;
             }
-            hipLaunchKernelGGL(cg_rho_kernel, dim3(1), dim3(kBlock), 0, s, k, kpad, B, (const double*)partial, st);
+            hipLaunchKernelGGL(cg_rho_kernel, dim3(1), dim3(kFinalThreads), 0, s, k, kpad, B, (const double*)partial, st);
             hipLaunchKernelGGL((cg_p_kernel<T>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, (const T*)z, p,
                                (const CGState*)st);
         }

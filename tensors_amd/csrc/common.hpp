@@ -32,6 +32,38 @@ constexpr int kWave = 64;      // CDNA wavefront width
 constexpr int kBlock = 256;    // threads per workgroup of the streaming kernels
 constexpr int kXcds = 8;       // MI355X: 8 XCDs, each with its own L2
 
+// V consecutive elements as one vector load / store (V = 2, 4: 8- or 16-byte
+// aligned addresses).
+template <typename T, int V>
+struct VecOf {
+    typedef T type __attribute__((ext_vector_type(V)));
+};
+
+template <typename T, int V>
+__device__ __forceinline__ void load_vec(const T* __restrict__ p, T (&out)[V]) {
+    if constexpr (V == 1) {
+        out[0] = *p;
+    } else {
+        using VT = typename VecOf<T, V>::type;
+        const VT v = *reinterpret_cast<const VT*>(p);
+#pragma unroll
+        for (int i = 0; i < V; ++i) out[i] = v[i];
+    }
+}
+
+template <typename T, int V>
+__device__ __forceinline__ void store_vec(T* __restrict__ p, const T (&in)[V]) {
+    if constexpr (V == 1) {
+        *p = in[0];
+    } else {
+        using VT = typename VecOf<T, V>::type;
+        VT v;
+#pragma unroll
+        for (int i = 0; i < V; ++i) v[i] = in[i];
+        *reinterpret_cast<VT*>(p) = v;
+    }
+}
+
 // Workgroups are dispatched round-robin over the XCDs.  Give XCD x the
 // contiguous logical range [x*q + min(x,r), ...) so neighbouring row blocks
 // (which touch neighbouring X rows for banded matrices) share one L2.

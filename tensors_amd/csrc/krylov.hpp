@@ -27,9 +27,15 @@ int64_t csr_band(const CsrView& A, hipStream_t s);
 // knowing the band can matter.
 bool spmm_may_slab(const CsrView& A, int32_t nrhs);
 
-// Tuning keys of the batched engine, forwarded by tg_tune_set / tg_tune_get.
+// Whether spmm_dispatch cuts X into column slabs (uses A.band when known).
+bool spmm_uses_slabs(const CsrView& A, int32_t nrhs);
+
+// Tuning keys of the batched engine and of the CG driver, forwarded by
+// tg_tune_set / tg_tune_get.
 int batch_tune_set(const char* key, int32_t value, bool* known);
 int32_t batch_tune_get(const char* key);
+int cg_tune_set(const char* key, int32_t value, bool* known);
+int32_t cg_tune_get(const char* key);
 
 // Y <- alpha A X + beta Y with the SpMM kernels; when `skip` is non-null and
 // *skip != 0 at kernel start, the kernels return without touching Y.
@@ -58,6 +64,7 @@ inline int csr_apply_guarded(const CsrView& A, const T* X, T* Y, int k, const in
 }
 
 constexpr int kSolverBlocks = 1024;  // workgroups of the column-blocked vector kernels
+constexpr int kFusedBlocks = 2048;   // workgroups of the fused CG direction + SpMM kernel
 constexpr int kMaxCols = 256;        // nrhs limit of the solvers
 
 inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }

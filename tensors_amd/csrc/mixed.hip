@@ -58,9 +58,9 @@ __global__ __launch_bounds__(kBlock) void mx_residual_kernel(int n, int k, int k
     column_partial(acc, L, k, kpad, partial);
 }
 
-__global__ __launch_bounds__(kBlock) void mx_total_kernel(int k, int kpad, int B, const double* __restrict__ partial,
+__global__ __launch_bounds__(kFinalThreads) void mx_total_kernel(int k, int kpad, int B, const double* __restrict__ partial,
                                                           double* __restrict__ out) {
-    const int c = threadIdx.x % kpad, j = threadIdx.x / kpad, J = kBlock / kpad;
+    const int c = threadIdx.x % kpad, j = threadIdx.x / kpad, J = kFinalThreads / kpad;
     const double t = column_total(partial, B, k, kpad, c, j, J);
     if (j == 0 && c < k) out[c] = sqrt(t);
 }
@@ -140,15 +140,17 @@ __global__ __launch_bounds__(kBlock) void mx_rz_kernel(int n, int k, int kpad, c
     column_partial(acc, L, k, kpad, partial);
 }
 
-// rho = r.z of the replaced residual; with `scale` != 1 the device had stopped on
-// the FP32 estimate: tighten its tolerance and let it continue.
-__global__ __launch_bounds__(kBlock) void mx_replace_state_kernel(int k, int kpad, int B,
+// rho = r.z of the replaced residual, beta = rho / rho_old for the next direction
+// p = z + beta p (iterations after the first); with `scale` != 1 the device had
+// stopped on the FP32 estimate: tighten its tolerance and let it continue.
+__global__ __launch_bounds__(kFinalThreads) void mx_replace_state_kernel(int k, int kpad, int B,
                                                                   const double* __restrict__ partial, double scale,
                                                                   CGState* __restrict__ st) {
-    const int c = threadIdx.x % kpad, j = threadIdx.x / kpad, J = kBlock / kpad;
+    const int c = threadIdx.x % kpad, j = threadIdx.x / kpad, J = kFinalThreads / kpad;
     const double tot = column_total(partial, B, k, kpad, c, j, J);
     if (j == 0 && c < k) {
         st->rho[c] = tot;
+        if (st->iter > 0) st->beta[c] = tot / st->rho_old[c];
         st->TOL[c] *= scale;
     }
     if (threadIdx.x == 0) {
@@ -256,7 +258,7 @@ static int cg_mixed(const CsrView& A64, const float* a32_in, int prec_kind, cons
 
     // ||b||
     hipLaunchKernelGGL(mx_bnorm_kernel, dim3(Bk), dim3(kBlock), 0, s, n, k, kpad, Bm, ldB, w.partial);
-    hipLaunchKernelGGL(mx_total_kernel, dim3(1), dim3(kBlock), 0, s, k, kpad, Bk, (const double*)w.partial, w.norms);
+    hipLaunchKernelGGL(mx_total_kernel, dim3(1), dim3(kFinalThreads), 0, s, k, kpad, Bk, (const double*)w.partial, w.norms);
     TG_LAUNCH_CHECK();
     std::vector<double> bn(k), rn(k);
     TG_HIP(hipMemcpyAsync(bn.data(), w.norms, k * 8, hipMemcpyDeviceToHost, s));
@@ -285,7 +287,7 @@ static int cg_mixed(const CsrView& A64, const float* a32_in, int prec_kind, cons
         }
         hipLaunchKernelGGL(mx_residual_kernel, dim3(Bk), dim3(kBlock), 0, s, n, k, kpad, Bm, ldB,
                            x_is_zero ? (const double*)nullptr : (const double*)w.u64, w.r32, w.partial);
-        hipLaunchKernelGGL(mx_total_kernel, dim3(1), dim3(kBlock), 0, s, k, kpad, Bk, (const double*)w.partial,
+        hipLaunchKernelGGL(mx_total_kernel, dim3(1), dim3(kFinalThreads), 0, s, k, kpad, Bk, (const double*)w.partial,
                            w.norms + k);
         TG_LAUNCH_CHECK();
         TG_HIP(hipMemcpyAsync(rn.data(), w.norms + k, k * 8, hipMemcpyDeviceToHost, s));
@@ -340,7 +342,7 @@ static int cg_mixed(const CsrView& A64, const float* a32_in, int prec_kind, cons
                 const double f = tol / worst;
                 scale = std::min(0.25, std::max(1e-4, 0.25 * f * f));
             }
-            hipLaunchKernelGGL(mx_replace_state_kernel, dim3(1), dim3(kBlock), 0, s, k, kpad, Bk,
+            hipLaunchKernelGGL(mx_replace_state_kernel, dim3(1), dim3(kFinalThreads), 0, s, k, kpad, Bk,
                                (const double*)partial, scale, st);
             TG_LAUNCH_CHECK();
             return TG_OK;

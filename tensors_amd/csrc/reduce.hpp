@@ -41,25 +41,32 @@ __device__ __forceinline__ void column_partial(double v, const Lanes& L, int k, 
     if (L.ro == 0 && L.c < k) partial[(int64_t)blockIdx.x * k + L.c] = sh[L.c];
 }
 
+// Threads of the one-workgroup kernels that finish a reduction.
+constexpr int kFinalThreads = 1024;
+
 // Sum of the B workgroup partials of each column in a fixed order, by one
-// workgroup: thread (c, j) adds partials j, j+J, ... into four running sums
-// (j, j+4J, ... / j+J, j+5J, ... / ...), then a fixed tree over j.
+// workgroup of kFinalThreads threads: thread (c, j), j < J = kFinalThreads / kpad,
+// adds partials j, j+J, ... into eight running sums (partial j + qJ + 8mJ goes to
+// sum q), combines them in a fixed tree, then a fixed tree over j.  Eight loads
+// per thread are in flight at a time, so B = 1024-2048 partials of 32 columns
+// take a few memory round trips.
 __device__ __forceinline__ double column_total(const double* partial, int B, int k, int kpad, int c, int j,
                                                int J) {
-    __shared__ double sh[kBlock];
-    // four independent running sums (loads in flight together), combined in a fixed order
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    __shared__ double sh[kFinalThreads];
+    double s[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s[q] = 0.0;
     if (c < k) {
         int b = j;
-        for (; b + 3 * J < B; b += 4 * J) {
-            s0 += partial[(int64_t)b * k + c];
-            s1 += partial[(int64_t)(b + J) * k + c];
-            s2 += partial[(int64_t)(b + 2 * J) * k + c];
-            s3 += partial[(int64_t)(b + 3 * J) * k + c];
+        for (; b + 7 * J < B; b += 8 * J) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) s[q] += partial[(int64_t)(b + q * J) * k + c];
         }
-        for (; b < B; b += J) s0 += partial[(int64_t)b * k + c];
+#pragma unroll
+        for (int q = 0; q < 7; ++q)  // fewer than 8 left
+            if (b + q * J < B) s[q] += partial[(int64_t)(b + q * J) * k + c];
     }
-    sh[threadIdx.x] = (s0 + s1) + (s2 + s3);
+    sh[threadIdx.x] = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
     __syncthreads();
     for (int h = J / 2; h > 0; h >>= 1) {
         if (j < h) sh[threadIdx.x] += sh[threadIdx.x + h * kpad];

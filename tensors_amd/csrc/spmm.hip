@@ -32,36 +32,6 @@ namespace tg {
 
 enum BetaMode : int { kBetaZero = 0, kBetaOne = 1, kBetaGeneric = 2 };
 
-template <typename T, int V>
-struct VecOf {
-    typedef T type __attribute__((ext_vector_type(V)));
-};
-
-template <typename T, int V>
-__device__ __forceinline__ void load_vec(const T* __restrict__ p, T (&out)[V]) {
-    if constexpr (V == 1) {
-        out[0] = *p;
-    } else {
-        using VT = typename VecOf<T, V>::type;
-        const VT v = *reinterpret_cast<const VT*>(p);
-#pragma unroll
-        for (int i = 0; i < V; ++i) out[i] = v[i];
-    }
-}
-
-template <typename T, int V>
-__device__ __forceinline__ void store_vec(T* __restrict__ p, const T (&in)[V]) {
-    if constexpr (V == 1) {
-        *p = in[0];
-    } else {
-        using VT = typename VecOf<T, V>::type;
-        VT v;
-#pragma unroll
-        for (int i = 0; i < V; ++i) v[i] = in[i];
-        *reinterpret_cast<VT*>(p) = v;
-    }
-}
-
 // Row epilogue for V consecutive entries of one Y row: the combine_scalars /
 // combine_buffers contract y = (alpha * z) + (beta * y) (SpMV.hpp:224-225,
 // SpMM.hpp:433-438) and the empty-row rules (SpMM.hpp:440-455).
@@ -441,9 +411,9 @@ constexpr int kRowsPerTeam = 8;
 static int g_spmv_team = 0;
 static int g_spmm_group = 0;
 static int g_spmm_rows = 0;  // nrhs >= 2 kernel: 0 = by mean row length, 1 = row per wave, 2 = row per lane group
-// planned SpMM: 0 = by X size, 1 = never sliced (default: the sliced kernel measured
-// slower than the plain ones on the 1e6-row random matrix), 2/4/8 = that many slices
-static int g_spmm_slices = 1;
+// planned SpMM: 0 = by X size (slice_count, the default), 1 = never sliced,
+// 2/4/8 = that many slices
+static int g_spmm_slices = 0;
 
 static int spmv_team_width(int32_t m, int64_t nnz) {
     if (g_spmv_team) return g_spmv_team;
@@ -834,11 +804,18 @@ __global__ __launch_bounds__(kBlock) void slice_reduce_kernel(int m, int S, cons
 
 // Slices for an X of `xbytes`: 0 = not worth it (X fits an L2 already, or is so
 // large that the partial blocks cost more than the misses they save).
+// Slices for an X block of xbytes: none up to 6 MB (X mostly stays in the L2s
+// anyway) or above 32 MB (not measured to pay); otherwise 2 or 4 slices of at
+// most 4 MB each where possible.  Measured on a random 1e6-row CSR with 32
+// nonzeros per row: 2 slices for the 8 MB FP64 SpMV, 4 for the 16 MB FP32 and
+// 32 MB FP64 nrhs = 4 blocks were the fastest counts; 8 slices were slower.
 static int slice_count(int64_t xbytes) {
-    constexpr int64_t kL2Share = 2ll << 20;  // X bytes per XCD slice that stay L2-resident
-    if (xbytes < 6 * kL2Share / 2 || xbytes > 16 * kL2Share) return 0;
+    constexpr int64_t kMB = 1ll << 20;
+    constexpr int64_t kSliceBytes = 4 * kMB;
+    constexpr int kMaxSlices = 4;
+    if (xbytes <= 6 * kMB || xbytes > 32 * kMB) return 0;
     int S = 2;
-    while (S < kXcds && xbytes / S > kL2Share) S *= 2;
+    while (S < kMaxSlices && xbytes / S > kSliceBytes) S *= 2;
     return S;
 }
 
@@ -937,7 +914,9 @@ extern "C" int tg_tune_set(const char* key, int32_t value) {
         return TG_OK;
     }
     bool known = false;
-    const int rc = batch_tune_set(key, value, &known);
+    int rc = batch_tune_set(key, value, &known);
+    if (known) return rc;
+    rc = cg_tune_set(key, value, &known);
     if (known) return rc;
     return fail(TG_ERR_INVALID, "tg_tune_set: unknown key '%s'", key);
 }
@@ -950,6 +929,7 @@ extern "C" int32_t tg_tune_get(const char* key) {
     if (k == "spmm_group") return g_spmm_group;
     if (k == "spmm_slices") return g_spmm_slices;
     if (k == "spmm_rows") return g_spmm_rows;
+    if (k == "cg_fused") return cg_tune_get(key);
     return batch_tune_get(key);
 }
 
@@ -1037,6 +1017,15 @@ bool spmm_may_slab(const CsrView& A, int32_t nrhs) {
     return (int64_t)A.n * nrhs * vs > kXTileBytes && nrhs > 8;
 }
 
+// Column slabs keep a random-column X resident in the 256 MB Infinity Cache; a
+// banded matrix (known band) only touches a window of X around the current
+// rows, so there they would just re-read A once per slab.
+bool spmm_uses_slabs(const CsrView& A, int32_t nrhs) {
+    const int64_t vs = A.val_type == TG_F64 ? 8 : 4;
+    const bool banded = A.band >= 0 && (2 * A.band + 1) * (int64_t)nrhs * vs <= kXTileBytes / 2;
+    return spmm_may_slab(A, nrhs) && !banded;
+}
+
 int spmm_dispatch(const CsrView& A, double alpha, const void* X, int64_t ldX, double beta, void* Y, int64_t ldY,
                   int32_t nrhs, const int32_t* job_ptr, int32_t blocks, const int* skip, hipStream_t st) {
     const int32_t m = A.m, n = A.n;
@@ -1056,11 +1045,7 @@ int spmm_dispatch(const CsrView& A, double alpha, const void* X, int64_t ldX, do
     if (A.nnz > 0 && !A.ci) return fail(TG_ERR_INVALID, "spmm: ci is null");
     const size_t vs = f64 ? 8 : 4;
     int32_t kt = nrhs;
-    // Column slabs keep a random-column X resident in the 256 MB Infinity Cache; a
-    // banded matrix (known band) only touches a window of X around the current
-    // rows, so there they would just re-read A once per slab.
-    const bool banded = A.band >= 0 && (2 * A.band + 1) * (int64_t)nrhs * (int64_t)vs <= kXTileBytes / 2;
-    if ((int64_t)n * nrhs * (int64_t)vs > kXTileBytes && nrhs > 8 && !banded) {
+    if (spmm_uses_slabs(A, nrhs)) {
         kt = 8;
         while ((int64_t)n * (2 * kt) * (int64_t)vs <= kXTileBytes && 2 * kt < nrhs) kt *= 2;
     }

@@ -123,11 +123,11 @@ struct TriplesWs {
 static size_t triples_ws(int64_t count, int32_t m, int symmetrize, char* base, TriplesWs* out) {
     const int64_t total = symmetrize ? 2 * count : count;
     size_t sort_tmp = 0, scan_tmp = 0, scan_tmp2 = 0;
-    rocprim::radix_sort_pairs(nullptr, sort_tmp, (uint64_t*)nullptr, (uint64_t*)nullptr, (uint32_t*)nullptr,
+    (void)rocprim::radix_sort_pairs(nullptr, sort_tmp, (uint64_t*)nullptr, (uint64_t*)nullptr, (uint32_t*)nullptr,
                               (uint32_t*)nullptr, (size_t)std::max<int64_t>(total, 1), 0, 64);
-    rocprim::exclusive_scan(nullptr, scan_tmp, (const int64_t*)nullptr, (int64_t*)nullptr, (int64_t)0,
+    (void)rocprim::exclusive_scan(nullptr, scan_tmp, (const int64_t*)nullptr, (int64_t*)nullptr, (int64_t)0,
                             (size_t)std::max<int64_t>(total, 1), rocprim::plus<int64_t>());
-    rocprim::exclusive_scan(nullptr, scan_tmp2, (const int64_t*)nullptr, (int64_t*)nullptr, (int64_t)0,
+    (void)rocprim::exclusive_scan(nullptr, scan_tmp2, (const int64_t*)nullptr, (int64_t*)nullptr, (int64_t)0,
                             (size_t)m + 1, rocprim::plus<int64_t>());
     const size_t tmp = std::max(sort_tmp, std::max(scan_tmp, scan_tmp2));
     size_t off = 0;
@@ -316,7 +316,7 @@ extern "C" int tg_csr_spgemm_expanded_count(int32_t m, const void* Arp, const in
                                             size_t workspace_bytes, void* stream) {
     if (m < 0 || !Arp || !Brp || !count_out) return fail(TG_ERR_INVALID, "tg_csr_spgemm_expanded_count: bad argument");
     size_t scan_tmp = 0;
-    rocprim::exclusive_scan(nullptr, scan_tmp, (const int64_t*)nullptr, (int64_t*)nullptr, (int64_t)0, (size_t)m + 1,
+    (void)rocprim::exclusive_scan(nullptr, scan_tmp, (const int64_t*)nullptr, (int64_t*)nullptr, (int64_t)0, (size_t)m + 1,
                             rocprim::plus<int64_t>());
     const size_t need = 2 * align256(8 * ((size_t)m + 1)) + align256(scan_tmp);
     if (!workspace || workspace_bytes < need)
@@ -342,7 +342,7 @@ extern "C" int tg_csr_spgemm_expanded_count(int32_t m, const void* Arp, const in
 
 extern "C" size_t tg_csr_spgemm_workspace_bytes(int32_t m, int32_t n, int64_t expanded, tg_dtype val_type) {
     size_t scan_tmp = 0;
-    rocprim::exclusive_scan(nullptr, scan_tmp, (const int64_t*)nullptr, (int64_t*)nullptr, (int64_t)0, (size_t)m + 1,
+    (void)rocprim::exclusive_scan(nullptr, scan_tmp, (const int64_t*)nullptr, (int64_t*)nullptr, (int64_t)0, (size_t)m + 1,
                             rocprim::plus<int64_t>());
     const size_t vs = val_type == TG_F32 ? 4 : 8;
     return 2 * align256(8 * ((size_t)m + 1)) + align256(scan_tmp) + 2 * align256(4 * (size_t)std::max<int64_t>(expanded, 1)) +
@@ -362,7 +362,7 @@ extern "C" int tg_csr_spgemm(int32_t m, int32_t k, int32_t n, const void* Arp, c
         return fail(TG_ERR_WORKSPACE, "tg_csr_spgemm: workspace missing or too small");
     hipStream_t s = as_stream(stream);
     size_t scan_tmp = 0;
-    rocprim::exclusive_scan(nullptr, scan_tmp, (const int64_t*)nullptr, (int64_t*)nullptr, (int64_t)0, (size_t)m + 1,
+    (void)rocprim::exclusive_scan(nullptr, scan_tmp, (const int64_t*)nullptr, (int64_t*)nullptr, (int64_t)0, (size_t)m + 1,
                             rocprim::plus<int64_t>());
     char* w = (char*)workspace;
     int64_t* cnt = (int64_t*)w;

@@ -8,6 +8,8 @@ given by their row stride).  torch only supplies device memory and streams here.
 """
 from __future__ import annotations
 
+import ctypes as C
+import os
 import sys
 
 import numpy as np
@@ -47,8 +49,6 @@ class SlicePlan:
     """tg_csr_plan_t of one device CSR pattern (column-sliced SpMM for small X)."""
 
     def __init__(self, m, n, nnz, rp, ci, stream=None):
-        import ctypes as C
-
         self.h = C.c_void_p()
         self._keep = (rp, ci)
         capi.call("tg_csr_plan_create", m, n, nnz, rp.data_ptr(), capi.I64 if rp.dtype == torch.int64 else capi.I32,
@@ -101,6 +101,35 @@ class MatrixCSR:
                 device=rp.device)
         A._dev = {"rp": rp, "ci": ci, "vals": {} if vals is None else {vals.dtype: vals}}
         return A
+
+    # -- MatrixMarket files (MatrixCSR/MatrixMarket.hpp:3-306) ----------------------
+    @classmethod
+    def from_matrix_market(cls, path, thread_count=1, dtype=np.float64, threads=0):
+        """LoadFromMatrixMarket: the file's entries are parsed on the host (tg_mm_read) and
+        assembled on the GPU (tg_csr_from_triples: duplicates summed, off-diagonal entries
+        mirrored for "symmetric" files).  Values of "pattern" files are 1."""
+        from . import structure
+        m, n, cnt, field, symm = C.c_int32(0), C.c_int32(0), C.c_int64(0), C.c_int32(0), C.c_int32(0)
+        p = os.fsencode(path)
+        capi.call("tg_mm_info", p, C.byref(m), C.byref(n), C.byref(cnt), C.byref(field), C.byref(symm))
+        k = cnt.value
+        I = np.empty(max(k, 1), np.int32)
+        J = np.empty(max(k, 1), np.int32)
+        V = np.empty(max(k, 1), np.float64)
+        capi.call("tg_mm_read", p, k, I.ctypes.data, J.ctypes.data, V.ctypes.data, int(threads))
+        rp, ci, val = structure.from_triples(I[:k], J[:k], V[:k], m.value, n.value, compress=True,
+                                             symmetrize=bool(symm.value))
+        return cls(rp, ci, val.astype(dtype), m.value, n.value, thread_count)
+
+    def write_matrix_market(self, path, threads=0):
+        """WriteToMatrixMarket: "coordinate real general" ("pattern" without values), one
+        line per nonzero in row order."""
+        rp = np.ascontiguousarray(self.outer)
+        vals = None if self.values is None else np.ascontiguousarray(self.values)
+        capi.call("tg_mm_write", os.fsencode(path), self.m, self.n, rp.ctypes.data,
+                  capi.I64 if rp.dtype == np.int64 else capi.I32, self.inner.ctypes.data if self.inner.size else None,
+                  None if vals is None else vals.ctypes.data,
+                  capi.F32 if vals is not None and vals.dtype == np.float32 else capi.F64, int(threads))
 
     # -- accessors (PatternCSR.hpp / MatrixCSR.hpp) --------------------------------
     def RowCount(self): return self.m

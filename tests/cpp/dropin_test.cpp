@@ -190,11 +190,14 @@ int main(int argc, char** argv) {
         auto A_host = [&](const Real* x, Real* y) { L.Dot(Real(1), x, Real(0), y, k); };
         Tensor2<Real, Int> X2(n, k, Real(0));
         const bool ok2 = cg(A_host, P, Real(1), B.data(), k, Real(0), X2.data(), k, Real(1e-10));
-        check(ok2 && X2 == X, "CG(host lambda) == CG(MatrixCSR) bitwise");
+        // a MatrixCSR takes the fused iteration (p.u summed over another workgroup
+        // partition), a lambda the four-kernel one: equal up to rounding
+        const double e2 = relerr(X2.data(), X.data(), (size_t)n * k);
+        check(ok2 && e2 < 1e-9, "CG(host lambda) matches CG(MatrixCSR)", e2);
         auto A_dev = DeviceOperator([&](const Real* x, Real* y) { L.Dot(Real(1), x, Real(0), y, k); });
         Tensor2<Real, Int> X3(n, k, Real(0));
         const bool ok3 = cg(A_dev, P, Real(1), B.data(), k, Real(0), X3.data(), k, Real(1e-10));
-        check(ok3 && X3 == X, "CG(device lambda) == CG(MatrixCSR) bitwise");
+        check(ok3 && X3 == X2, "CG(device lambda) == CG(host lambda) bitwise");
         Tensor1<Real, Int> diag(n);
         for (Int i = 0; i < n; ++i)
             for (LInt l = L.Outer()[i]; l < L.Outer()[i + 1]; ++l)

@@ -111,9 +111,25 @@ def spgemm(m, k, n, A, B, threads=3, values=True):
     return Crp, Cci[:nnz].copy(), (Cval[:nnz].copy() if values else None)
 
 
+def _solver_arrays(rp, ci, val, B, prec_diag):
+    """The solvers' C signatures take int64 rp, int32 ci and float64 values / B / diagonal."""
+    rp = np.ascontiguousarray(rp, np.int64)
+    ci = np.ascontiguousarray(ci, np.int32)
+    val = None if val is None else np.ascontiguousarray(val, np.float64)
+    B = np.asarray(B, np.float64)
+    if B.ndim == 2 and B.strides[1] != 8:
+        B = np.ascontiguousarray(B)
+    prec_diag = None if prec_diag is None else np.ascontiguousarray(prec_diag, np.float64)
+    return rp, ci, val, B, prec_diag
+
+
 def cg(n, rp, ci, val, B, X, nrhs, max_iter, tol, a=1.0, b=0.0, use_guess=False, prec_diag=None, threads=4,
        fast=False):
-    """Oracle CG; X (n x nrhs, C order) updated in place.  Returns (succeeded, iters, rel_res)."""
+    """Oracle CG; X (n x nrhs, C order, float64) updated in place.  Returns (succeeded, iters,
+    rel_res).  rp, ci, val, B and prec_diag are converted to the C types (int64, int32,
+    float64) first."""
+    rp, ci, val, B, prec_diag = _solver_arrays(rp, ci, val, B, prec_diag)
+    assert X.dtype == np.float64 and X.strides[1] == 8
     it = np.zeros(1, np.int32)
     rr = np.zeros(nrhs, np.float64)
     ok = lib(fast).orc_cg(n, p(rp), p(ci), p(val), 1 if prec_diag is not None else 0, p(prec_diag), nrhs,
@@ -124,6 +140,9 @@ def cg(n, rp, ci, val, B, X, nrhs, max_iter, tol, a=1.0, b=0.0, use_guess=False,
 
 def gmres(n, rp, ci, val, B, X, nrhs, max_iter, tol, max_restarts, a=1.0, b=0.0, use_guess=False,
           prec_diag=None, side=0, threads=4, fast=False):
+    """Oracle GMRES; X updated in place.  Returns (succeeded, iters, restarts, rel_res)."""
+    rp, ci, val, B, prec_diag = _solver_arrays(rp, ci, val, B, prec_diag)
+    assert X.dtype == np.float64 and X.strides[1] == 8
     it = np.zeros(1, np.int32)
     rs = np.zeros(1, np.int32)
     rr = np.zeros(nrhs, np.float64)

@@ -7,6 +7,7 @@ import torch
 
 import oracle
 from helpers import rel_err
+from tensors_amd import _capi as capi
 from tensors_amd import generators as gen
 from tensors_amd.solvers import ConjugateGradient
 from tensors_amd.sparse import MatrixCSR
@@ -21,6 +22,14 @@ def dev(a, d, dtype=torch.float64):
 def grid_system(g, mass=1.0):
     rp, ci, val = oracle.grid_laplacian(g, mass)
     return rp, ci, val, g * g
+
+
+@pytest.fixture
+def fused_knob():
+    """Restores the process-wide cg_fused knob after the test."""
+    before = int(capi.lib().tg_tune_get(b"cg_fused"))
+    yield lambda v: capi.call("tg_tune_set", b"cg_fused", int(v))
+    capi.call("tg_tune_set", b"cg_fused", before)
 
 
 @pytest.mark.parametrize("k", [1, 2, 5, 16, 32])
@@ -126,10 +135,13 @@ def test_cg_3d_laplacian_medium(cuda):
 
 
 @pytest.mark.parametrize("prec", [False, True])
-def test_cg_callback_operators_match_builtin(cuda, prec):
+def test_cg_callback_operators_match_builtin(cuda, fused_knob, prec):
     """tg_cg_solve_op with host callbacks (the reference's generic A / P callables) gives
-    bit-identical results to the built-in CSR / diagonal path."""
+    bit-identical results to the built-in CSR / diagonal path in its four-kernel form
+    (the fused form sums p.u over another workgroup partition: same result to rounding,
+    covered below)."""
     from tensors_amd.solvers import cg_solve_op, csr_callback
+    fused_knob(0)
     rp, ci, val, n = grid_system(16, 0.2)
     k = 3
     B = gen.uniform(77, n * k).reshape(n, k)
@@ -145,3 +157,38 @@ def test_cg_callback_operators_match_builtin(cuda, prec):
     ok, it, rr = cg_solve_op(n, k, 500, fA, fP, 1.0, dev(B, cuda), 0.0, X2, 1e-10)
     assert ok and it == cg.IterationCount()
     assert torch.equal(X1, X2)
+
+
+@pytest.mark.parametrize("form", [1, 2])
+@pytest.mark.parametrize("k", [1, 2, 3, 8, 33, 64, 130, 256])
+@pytest.mark.parametrize("prec", [False, True])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_cg_fused_iteration_matches_four_kernel_path(cuda, fused_knob, form, k, prec, dtype):
+    """The fused CG iteration forms against the four-kernel iteration and the oracle:
+    form 1 computes p.u in the SpMM epilogue, form 2 forms p = z + beta p, u = A p and p.u
+    in one kernel and x, r and r.z in a second.  Every lane-group width and vector width of
+    the fused kernel (k = 1 ... 256), FP64 / FP32, P = I / Jacobi, int64 and int32 row
+    pointers."""
+    rp, ci, val, n = grid_system(20, 0.1)
+    if k % 2:
+        rp = rp.astype(np.int32)
+    B = gen.uniform(911 + k, n * k).reshape(n, k)
+    d = 1.0 / sp.csr_matrix((val, ci, rp), shape=(n, n)).diagonal() if prec else None
+    tol = 1e-10 if dtype == torch.float64 else 1e-5
+    A = MatrixCSR(rp, ci, val, n, n)
+    out = {}
+    for fused in (0, form):
+        fused_knob(fused)
+        cg = ConjugateGradient(n, 1000, k, dtype=dtype)
+        Xd = torch.zeros(n, k, dtype=dtype, device=cuda)
+        assert cg(A, None if d is None else dev(d, cuda, dtype), 1.0, dev(B, cuda, dtype), 0.0, Xd, tol)
+        out[fused] = (Xd.double().cpu().numpy(), cg.IterationCount(), cg.RelativeResiduals())
+    (X0, it0, rr0), (X1, it1, rr1) = out[0], out[form]
+    assert abs(it0 - it1) <= 1
+    assert np.all(rr1 <= tol)
+    assert rel_err(X1, X0) < (1e-9 if dtype == torch.float64 else 2e-4)
+    if dtype == torch.float64:
+        Xo = np.zeros((n, k))
+        ok_o, it_o, _ = oracle.cg(n, rp, ci, val, B, Xo, k, 1000, tol, prec_diag=d)
+        assert ok_o == 1 and abs(it1 - it_o) <= 1
+        assert rel_err(X1, Xo) < 1e-8

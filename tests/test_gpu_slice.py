@@ -98,7 +98,7 @@ def test_unsorted_rows_fall_back(cuda, knobs):
 @pytest.mark.parametrize("k,dtype", [(1, torch.float64), (4, torch.float32)])
 def test_auto_slices_large_matrix_dot(cuda, knobs, k, dtype):
     """MatrixCSR.Dot on an 800k x 800k matrix (X of 6.4 / 12.8 MB): slices chosen by size."""
-    autotune.set_knobs({"spmm_slices": 0})  # let the plan choose (the default is off)
+    autotune.set_knobs({"spmm_slices": 0})  # let the plan choose (the default)
     m = n = 800_000
     rng = np.random.default_rng(11)
     lens = rng.integers(1, 12, m)
@@ -112,7 +112,7 @@ def test_auto_slices_large_matrix_dot(cuda, knobs, k, dtype):
     A = MatrixCSR(rp.astype(np.int32), ci, val, m, n)
     X = torch.rand(n, k, dtype=dtype, device=cuda) * 2 - 1
     Y = torch.zeros(m, k, dtype=dtype, device=cuda)
-    assert A.Plan().slices(dtype, k) in (4, 8)
+    assert A.Plan().slices(dtype, k) in (2, 4)
     assert A.Dot(1.0, X, 0.0, Y) == 0
     Xh = X.double().cpu().numpy()
     Yo = np.zeros((m, k))

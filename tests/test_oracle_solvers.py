@@ -158,3 +158,19 @@ def test_trsv_upper_matches_numpy():
         x = b.copy()
         oracle.lib().orc_trsv_upper(k, oracle.p(H), k, oracle.p(x))
         assert rel_err(x, np.linalg.solve(H, b)) < 1e-13
+
+
+def test_solver_wrappers_accept_int32_row_pointers():
+    """The CG / GMRES wrappers convert rp to the C signature's int64 (an int32 array used to
+    be read as int64 out of bounds)."""
+    rp, ci, val = oracle.grid_laplacian(12, 0.3)
+    n, k = 144, 3
+    B = np.random.default_rng(5).uniform(-1, 1, (n, k))
+    X64, X32 = np.zeros((n, k)), np.zeros((n, k))
+    r64 = oracle.cg(n, rp, ci, val, B, X64, k, 500, 1e-10)
+    r32 = oracle.cg(n, rp.astype(np.int32), ci, val, B, X32, k, 500, 1e-10)
+    assert r64[:2] == r32[:2] and np.array_equal(X64, X32)
+    G64, G32 = np.zeros((n, k)), np.zeros((n, k))
+    g64 = oracle.gmres(n, rp, ci, val, B, G64, k, 30, 1e-10, 50)
+    g32 = oracle.gmres(n, rp.astype(np.int32), ci, val, B, G32, k, 30, 1e-10, 50)
+    assert g64[:3] == g32[:3] and np.array_equal(G64, G32)
