@@ -17,6 +17,9 @@
 // complex scalars and base-1 indices are not on this path.
 #pragma once
 
+#include <filesystem>
+#include <vector>
+
 #include "../Tensor.hpp"
 
 namespace Tensors {
@@ -113,6 +116,48 @@ template <typename Scal_, typename Int_, typename LInt_, Parallel_T parQ_ = Para
             total += (int64_t)t.Size();
         }
         FromTriples<ExtInt, ExtScal>(total, I, J, A, c, compressQ, symmetrizeQ);
+    }
+
+    // ---- MatrixMarket files (MatrixCSR/MatrixMarket.hpp:3-306) ------------------
+    // The entries are parsed on the host (tg_mm_info / tg_mm_read) and assembled on
+    // the GPU like the triples constructor: duplicates summed, off-diagonal entries
+    // mirrored for "symmetric" files.  A file the reference refuses (missing, not
+    // MatrixMarket, dense, complex, skew-symmetric, hermitian) or a malformed entry
+    // prints the reason and leaves the matrix unchanged.
+    void LoadFromMatrixMarket(const std::filesystem::path& file, Int thread_count_) {
+        const std::string where = MethodName("LoadFromMatrixMarket");
+        int32_t mm = 0, nn = 0, field = 0, symm = 0;
+        int64_t count = 0;
+        if (tg_mm_info(file.c_str(), &mm, &nn, &count, &field, &symm) != TG_OK) {
+            eprint(where + ": " + tg_last_error() + ". Doing nothing.");
+            return;
+        }
+        std::vector<int32_t> I((Size_T)std::max<int64_t>(count, 1)), J(I.size());
+        std::vector<double> V(I.size());
+        if (tg_mm_read(file.c_str(), count, I.data(), J.data(), V.data(), 0) != TG_OK) {
+            eprint(where + ": " + tg_last_error() + ". Doing nothing.");
+            return;
+        }
+        MatrixCSR A((LInt)count, I.data(), J.data(), V.data(), (Int)mm, (Int)nn, thread_count_, true, symm);
+        *this = std::move(A);
+    }
+
+    // "coordinate real general", one line per nonzero in row order (every row; the
+    // reference's writer keeps only two of its four thread buffers, :283-295).
+    void WriteToMatrixMarket(const std::filesystem::path& file) const {
+        const std::string where = MethodName("WriteToMatrixMarket");
+        if (!WellFormedQ()) {
+            eprint(where + ": Matrix is not well-formed. Doing nothing.");
+            return;
+        }
+        const int64_t nnz = (int64_t)NonzeroCount();
+        std::vector<int64_t> r(outer.Size());
+        for (Int i = 0; i < outer.Size(); ++i) r[(Size_T)i] = (int64_t)outer[i];
+        std::vector<int32_t> c((Size_T)nnz);
+        for (int64_t e = 0; e < nnz; ++e) c[(Size_T)e] = (int32_t)inner[(LInt)e];
+        if (tg_mm_write(file.c_str(), (int32_t)m, (int32_t)n, r.data(), TG_I64, c.data(), values.data(),
+                        Device::DType<Scal>(), 0) != TG_OK)
+            eprint(where + ": " + tg_last_error());
     }
 
     MatrixCSR(const MatrixCSR& o)

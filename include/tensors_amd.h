@@ -379,6 +379,31 @@ int64_t tg_gen_convection_diffusion_2d(int32_t g, double peclet, double theta, i
                                        double* val);
 
 /* ------------------------------------------------------------------------
+ * Block-CSR SpMM: replaces Sparse::KernelMatrixCSR<Kernel_T>::Dot
+ * (KernelMatrixCSR.hpp:153-227) with the dense block kernels
+ * (BlockKernels/DenseBlockKernel_Tiny.hpp:160-180, BlockKernel_Tiny.hpp:156-196).
+ * A pattern of mb x nb blocks (rp[mb+1], ci[nnzb]) with one R x C block per
+ * nonzero (a + k*R*C; row-major if a_row_major, else column-major).  For every
+ * block row i:  Y_i <- alpha * sum_k A_k X_{ci[k]} + beta * Y_i, where X_j is
+ * the C x nrhs chunk at X + j*C*nrhs and Y_i the R x nrhs chunk at
+ * Y + i*R*nrhs, each row-major (x_row_major / y_row_major: entry (c, h) at
+ * c*nrhs + h) or transposed (h*C + c).  Empty block rows: Y_i <- beta Y_i.
+ * alpha == 0 or nnzb == 0: Y <- beta Y (beta == 0 writes zeros).  X == Y ->
+ * TG_ERR_ALIAS.  Block sizes 1..64.  Device pointers; stream-ordered.
+ *
+ * tg_bsr_fill_lower: FillLowerTriangleFromUpperTriangle (KernelMatrixCSR.hpp:
+ * 71-128) for square R x R blocks and sorted block columns: every block
+ * (i, j), j < i, becomes the transpose of block (j, i) (left as it is where
+ * the pattern has no (j, i)).
+ * ------------------------------------------------------------------------ */
+int tg_bsr_spmm(int32_t mb, int32_t nb, int64_t nnzb, int32_t R, int32_t C, const void* rp, tg_dtype rp_type,
+                const int32_t* ci, const void* a, tg_dtype val_type, int32_t a_row_major, double alpha,
+                const void* X, int32_t x_row_major, double beta, void* Y, int32_t y_row_major, int32_t nrhs,
+                void* stream);
+int tg_bsr_fill_lower(int32_t mb, const void* rp, tg_dtype rp_type, const int32_t* ci, int32_t R, void* a,
+                      tg_dtype val_type, void* stream);
+
+/* ------------------------------------------------------------------------
  * MatrixMarket coordinate files (HOST buffers; no device work).  Replace
  * MatrixCSR::LoadFromMatrixMarket / WriteToMatrixMarket
  * (MatrixCSR/MatrixMarket.hpp:3-306); loading = tg_mm_info + tg_mm_read, then

@@ -26,6 +26,9 @@
 //                                            termination on diagonal systems
 //   gmres (left/right, restarts)             dense solve; GMRES(n) exactness
 //   upper back substitution                  numpy triangular solve
+//   block CSR product (all storage orders)   the scalar CSR of the expanded
+//                                            blocks (spmm above), dense numpy
+//   fill lower blocks from upper             dense symmetric closed form
 // Bit-level float parity with the reference is unpinned everywhere (Tools'
 // reduction splits, Sort stability and -ffast-math evaluation order are not
 // available); structure (rp, ci) is pinned bit-exact.
@@ -490,6 +493,85 @@ static CSR spgemm(const CSR& A, const CSR& B, i32 T, bool values) {
 }
 
 // ---------------------------------------------------------------------------
+// Block CSR: KernelMatrixCSR<DenseBlockKernel_Tiny<ROWS, COLS, NRHS, ...>>::Dot
+// (KernelMatrixCSR.hpp:153-227).  Per block row i: the kernel's y = 0
+// (CleanseY, BlockKernel_Tiny.hpp:170-174), for each block k in row order
+// ReadX(ci[k]) and ReadA(k) then y += a x (DenseBlockKernel_Tiny.hpp:150-180:
+// y(r,h) += sum_c a(r,c) x(c,h)), then WriteY: Y_i = alpha y + beta Y_i
+// (BlockKernel_Tiny.hpp:176-183, flags as combine_buffers); an empty row
+// gets WriteYZero (:185-190): Y_i = beta Y_i.  alpha == 0 or no nonzeros:
+// Scale (KernelMatrixCSR.hpp:137-151).  Storage orders: a_RM (a(r,c) at
+// r*COLS + c, else c*ROWS + r), x_RM (x(c,h) at c*NRHS + h, else h*COLS + c),
+// y_RM likewise with ROWS.
+// ---------------------------------------------------------------------------
+template <typename T>
+void bsr_spmm(i64 mb, const i64* rp, const i32* ci, int R, int C, const T* A, bool a_rm, T alpha, const T* X,
+              bool x_rm, T beta, T* Y, bool y_rm, int k) {
+    const i64 E = (i64)R * k;
+    if (alpha == T(0) || rp[mb] - rp[0] <= 0) {
+        for (i64 e = 0; e < mb * E; ++e) Y[e] = (beta == T(0)) ? T(0) : beta * Y[e];
+        return;
+    }
+    std::vector<T> y((size_t)E);
+    for (i64 i = 0; i < mb; ++i) {
+        T* Yi = Y + i * E;
+        if (rp[i + 1] == rp[i]) {
+            for (i64 e = 0; e < E; ++e) Yi[e] = (beta == T(0)) ? T(0) : (beta == T(1)) ? Yi[e] : beta * Yi[e];
+            continue;
+        }
+        std::fill(y.begin(), y.end(), T(0));
+        for (i64 kb = rp[i]; kb < rp[i + 1]; ++kb) {
+            const T* a = A + kb * R * C;
+            const T* x = X + (i64)ci[kb] * C * k;
+            for (int r = 0; r < R; ++r)
+                for (int h = 0; h < k; ++h) {
+                    T z = T(0);
+                    for (int c = 0; c < C; ++c)
+                        z += a[a_rm ? r * C + c : c * R + r] * x[x_rm ? c * k + h : h * C + c];
+                    y[(size_t)r * k + h] += z;
+                }
+        }
+        for (int r = 0; r < R; ++r)
+            for (int h = 0; h < k; ++h) {
+                T& out = Yi[y_rm ? r * k + h : h * R + r];
+                const T v = (alpha == T(1)) ? y[(size_t)r * k + h] : alpha * y[(size_t)r * k + h];
+                out = (beta == T(0)) ? v : (beta == T(1)) ? v + out : v + beta * out;
+            }
+    }
+}
+
+// FillLowerTriangleFromUpperTriangle (KernelMatrixCSR.hpp:71-128) with the
+// diagonal positions of PatternCSR::RequireDiag (PatternCSR.hpp:408-445: first
+// position with column >= row) and TransposeBlock (DenseBlockKernel_Tiny.hpp:
+// 112-148).  A lower block whose mirror (j, i) is not in the pattern is left
+// unchanged (the reference's search then lands on another block).
+template <typename T>
+void bsr_fill_lower(i64 mb, const i64* rp, const i32* ci, int R, T* A) {
+    std::vector<i64> diag((size_t)mb);
+    for (i64 i = 0; i < mb; ++i) {
+        i64 kk = rp[i];
+        while (kk < rp[i + 1] && ci[kk] < i) ++kk;
+        diag[(size_t)i] = kk;
+    }
+    const i64 RR = (i64)R * R;
+    for (i64 i = 0; i < mb; ++i)
+        for (i64 kb = rp[i]; kb < diag[(size_t)i]; ++kb) {
+            const i64 j = ci[kb];
+            i64 L = diag[(size_t)j], Rr = rp[j + 1] - 1;
+            while (L < Rr) {
+                const i64 M = Rr - (Rr - L) / 2;
+                if (ci[M] > i)
+                    Rr = M - 1;
+                else
+                    L = M;
+            }
+            if (L >= rp[j + 1] || ci[L] != i) continue;
+            for (int a = 0; a < R; ++a)
+                for (int b = 0; b < R; ++b) A[kb * RR + (i64)a * R + b] = A[L * RR + (i64)b * R + a];
+        }
+}
+
+// ---------------------------------------------------------------------------
 // Krylov solvers.  The operator is y = A x on n x nrhs row-major blocks.
 // Preconditioner kinds: 0 = identity (copy), 1 = row scaling z_i = d_i * r_i
 // (the lumped-mass-inverse / Jacobi diagonal the reference's CG tests use,
@@ -868,6 +950,24 @@ int64_t orc_job_pointer(int64_t n, int64_t T, int64_t t) { return orc::job_point
 void orc_job_pointers_costs(int64_t m, const int64_t* acc, int64_t T, int64_t* out) {
     auto jp = orc::job_pointers_costs(m, acc, T);
     std::copy(jp.begin(), jp.end(), out);
+}
+
+int orc_bsr_spmm_f64(int64_t mb, const int64_t* rp, const int32_t* ci, int32_t R, int32_t C, const double* A,
+                     int32_t a_rm, double alpha, const double* X, int32_t x_rm, double beta, double* Y,
+                     int32_t y_rm, int32_t k) {
+    orc::bsr_spmm<double>(mb, rp, ci, R, C, A, a_rm != 0, alpha, X, x_rm != 0, beta, Y, y_rm != 0, k);
+    return 0;
+}
+
+int orc_bsr_spmm_f32(int64_t mb, const int64_t* rp, const int32_t* ci, int32_t R, int32_t C, const float* A,
+                     int32_t a_rm, float alpha, const float* X, int32_t x_rm, float beta, float* Y, int32_t y_rm,
+                     int32_t k) {
+    orc::bsr_spmm<float>(mb, rp, ci, R, C, A, a_rm != 0, alpha, X, x_rm != 0, beta, Y, y_rm != 0, k);
+    return 0;
+}
+
+void orc_bsr_fill_lower(int64_t mb, const int64_t* rp, const int32_t* ci, int32_t R, double* A) {
+    orc::bsr_fill_lower<double>(mb, rp, ci, R, A);
 }
 
 int orc_multiply_dense_f64(const int64_t* rp, const int32_t* ci, const double* a, int32_t m, int32_t n,

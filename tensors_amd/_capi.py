@@ -49,6 +49,9 @@ SIGNATURES = {
                                  vp, C.c_size_t, vp, vp, vp, vp]),
     "tg_gmres_solve_op": (C.c_int, [i32, vp, vp, vp, vp, C.c_int, i32, i32, i32, dbl, vp, i64, dbl, vp, i64,
                                     dbl, i32, i32, vp, C.c_size_t, vp, vp, vp, vp, vp]),
+    "tg_bsr_spmm": (C.c_int, [i32, i32, i64, i32, i32, vp, C.c_int, vp, vp, C.c_int, i32, dbl, vp, i32, dbl, vp,
+                              i32, i32, vp]),
+    "tg_bsr_fill_lower": (C.c_int, [i32, vp, C.c_int, vp, i32, vp, C.c_int, vp]),
     "tg_mm_info": (C.c_int, [C.c_char_p, vp, vp, vp, vp, vp]),
     "tg_mm_read": (C.c_int, [C.c_char_p, i64, vp, vp, vp, i32]),
     "tg_mm_write": (C.c_int, [C.c_char_p, i32, i32, vp, C.c_int, vp, vp, C.c_int, i32]),

@@ -225,3 +225,59 @@ class MatrixCSR:
         vt = "Real64" if self.values is None or self.values.dtype == np.float64 else "Real32"
         it = "Int64" if self.outer.dtype == np.int64 else "Int32"
         return f"Sparse::MatrixCSR<{vt},Int32,{it}>::{method}"
+
+
+class KernelMatrixCSR:
+    """Sparse::KernelMatrixCSR<Kernel_T> (KernelMatrixCSR.hpp:7-227) with the dense block kernel
+    DenseBlockKernel_Tiny<ROWS, COLS, NRHS, ..., a_RM, ..., x_RM, ..., y_RM>
+    (BlockKernels/DenseBlockKernel_Tiny.hpp): a pattern of mb x nb blocks (outer, inner) whose
+    nonzeros are ROWS x COLS dense blocks.  The block values are passed to Dot, as in the
+    reference (Dot(A, alpha, X, beta, Y, nrhs)); X holds nb chunks of COLS x nrhs, Y mb chunks
+    of ROWS x nrhs, each row-major (x_RM / y_RM) or transposed.  The product runs on the GPU
+    (tg_bsr_spmm)."""
+
+    def __init__(self, outer, inner, mb, nb, rows, cols, a_row_major=True, x_row_major=True, y_row_major=True,
+                 thread_count=1, device="cuda"):
+        self.mb, self.nb = int(mb), int(nb)
+        self.ROWS, self.COLS = int(rows), int(cols)
+        self.a_rm, self.x_rm, self.y_rm = bool(a_row_major), bool(x_row_major), bool(y_row_major)
+        self.outer = np.ascontiguousarray(outer)
+        self.inner = np.ascontiguousarray(inner, dtype=np.int32)
+        self.thread_count = int(thread_count)
+        self.rp = torch.from_numpy(self.outer).to(device)
+        self.ci = torch.from_numpy(self.inner).to(device)
+
+    def RowCount(self): return self.mb * self.ROWS
+    def ColCount(self): return self.nb * self.COLS
+    def BlockNonzeroCount(self): return int(self.outer[self.mb])
+    def NonzeroCount(self): return self.BlockNonzeroCount() * self.ROWS * self.COLS
+
+    def Dot(self, A, alpha, X, beta, Y, nrhs, stream=None):
+        """Y <- alpha * A X + beta * Y for block values A (CUDA tensor, nnzb*ROWS*COLS) and CUDA
+        X (nb*COLS*nrhs entries), Y (mb*ROWS*nrhs entries) of A's dtype."""
+        if X.numel() != self.nb * self.COLS * nrhs or Y.numel() != self.mb * self.ROWS * nrhs:
+            _eprint(self._name("Dot") + ": shapes of matrix, input, and output do not match.")
+            return capi.ERR_SHAPE
+        st = capi.lib().tg_bsr_spmm(
+            self.mb, self.nb, self.BlockNonzeroCount(), self.ROWS, self.COLS, self.rp.data_ptr(),
+            capi.I64 if self.rp.dtype == torch.int64 else capi.I32,
+            self.ci.data_ptr() if self.ci.numel() else None, A.data_ptr() if A.numel() else None,
+            _TORCH_TO_TG[Y.dtype], int(self.a_rm), float(alpha), X.data_ptr(), int(self.x_rm), float(beta),
+            Y.data_ptr(), int(self.y_rm), int(nrhs), stream_handle(stream))
+        if st == capi.ERR_ALIAS:
+            _eprint(self._name("Dot") + ": Input and output pointer coincide. Doing nothing.")
+            return st
+        return capi.check(st, "tg_bsr_spmm")
+
+    def FillLowerTriangleFromUpperTriangle(self, A, stream=None):
+        """Every block (i, j), j < i, becomes the transpose of block (j, i) (square blocks,
+        sorted block columns; KernelMatrixCSR.hpp:71-128)."""
+        if self.ROWS != self.COLS:
+            raise ValueError("FillLowerTriangleFromUpperTriangle needs square blocks")
+        capi.call("tg_bsr_fill_lower", self.mb, self.rp.data_ptr(),
+                  capi.I64 if self.rp.dtype == torch.int64 else capi.I32,
+                  self.ci.data_ptr() if self.ci.numel() else None, self.ROWS, A.data_ptr(), _TORCH_TO_TG[A.dtype],
+                  stream_handle(stream))
+
+    def _name(self, method):
+        return f"Sparse::KernelMatrixCSR<DenseBlockKernel_Tiny<{self.ROWS},{self.COLS},...>>::{method}"

@@ -5,6 +5,7 @@
 // operators, host lambdas and device lambdas.  Prints one line per check and
 // exits non-zero on the first failure.  argv[1] (optional): file that receives
 // the GridLaplacian(16) structure for the pytest to compare with the oracle.
+#include <unistd.h>
 #include <cstdio>
 #include <cstdlib>
 #include <map>
@@ -75,6 +76,61 @@ int main(int argc, char** argv) {
         std::fwrite(L.Inner().data(), sizeof(Int), nnz, f);
         std::fwrite(L.Values().data(), sizeof(Real), nnz, f);
         std::fclose(f);
+    }
+
+    // --- MatrixMarket round trip (MatrixCSR/MatrixMarket.hpp:3-306) ---------------
+    {
+        const auto path = std::filesystem::temp_directory_path() / ("tensors_amd_dropin_" +
+                                                                    std::to_string((long)getpid()) + ".mtx");
+        L.WriteToMatrixMarket(path);
+        CSR M;
+        M.LoadFromMatrixMarket(path, 2);
+        check(M.RowCount() == L.RowCount() && M.Outer() == L.Outer() && M.Inner() == L.Inner() &&
+                  M.Values() == L.Values(),
+              "WriteToMatrixMarket -> LoadFromMatrixMarket reproduces the matrix");
+        CSR keep = M;
+        M.LoadFromMatrixMarket(path.string() + ".missing", 2);  // prints, leaves M unchanged
+        check(M.Outer() == keep.Outer() && M.Values() == keep.Values(), "LoadFromMatrixMarket(missing) is a no-op");
+        std::filesystem::remove(path);
+    }
+
+    // --- KernelMatrixCSR with dense 3 x 3 blocks (KernelMatrixCSR.hpp:153-227) ----
+    {
+        using Ker = DenseBlockKernel_Tiny<3, 3, 2, Real, Real, Real, Int, LInt, Scalar::Flag::Generic,
+                                          Scalar::Flag::Generic, true, false, true, true, true, true>;
+        Sparse::KernelMatrixCSR<Ker> K(L);  // the grid pattern, one 3 x 3 block per nonzero
+        const Int mb = L.RowCount();
+        const LInt nnzb = L.NonzeroCount();
+        check(K.RowCount() == 3 * mb && K.NonzeroCount() == 9 * nnzb, "KernelMatrixCSR sizes");
+        Tensor1<Real, LInt> A(9 * nnzb);
+        Tensor2<Real, Int> X(3 * mb, 2), Y(3 * mb, 2), Yr(3 * mb, 2);
+        A.Random();
+        X.Random();
+        Y.Random();
+        Yr = Y;
+        K.Dot(A.data(), Real(2), X.data(), Real(-0.5), Y.data(), 2);
+        for (Int i = 0; i < mb; ++i)
+            for (int r = 0; r < 3; ++r)
+                for (int h = 0; h < 2; ++h) {
+                    double z = 0;
+                    for (LInt l = L.Outer()[i]; l < L.Outer()[i + 1]; ++l)
+                        for (int c = 0; c < 3; ++c) z += A[9 * l + 3 * r + c] * X(3 * L.Inner()[l] + c, h);
+                    Yr(3 * i + r, h) = 2.0 * z - 0.5 * Yr(3 * i + r, h);
+                }
+        const double e = relerr(Y.data(), Yr.data(), (size_t)6 * mb);
+        check(e < 1e-13, "KernelMatrixCSR::Dot (host pointers) matches the block product", e);
+        K.FillLowerTriangleFromUpperTriangle(A.data());
+        bool sym = true;
+        for (Int i = 0; i < mb; ++i)
+            for (LInt l = L.Outer()[i]; l < L.Outer()[i + 1]; ++l) {
+                const Int j = L.Inner()[l];
+                if (j >= i) continue;
+                for (LInt u = L.Outer()[j]; u < L.Outer()[j + 1]; ++u)
+                    if (L.Inner()[u] == i)
+                        for (int a = 0; a < 3; ++a)
+                            for (int b = 0; b < 3; ++b) sym &= A[9 * l + 3 * a + b] == A[9 * u + 3 * b + a];
+            }
+        check(sym, "KernelMatrixCSR::FillLowerTriangleFromUpperTriangle mirrors the upper blocks");
     }
 
     // --- Dot overloads (MatrixCSR.hpp:965-1086) --------------------------------

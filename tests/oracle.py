@@ -17,6 +17,9 @@ _SIG = {
     "orc_multiply_dense_f64": (C.c_int, [vp, vp, vp, i32, i32, dbl, vp, i64, dbl, vp, i64, i32, i32, i32]),
     "orc_multiply_dense_f32": (C.c_int, [vp, vp, vp, i32, i32, C.c_float, vp, i64, C.c_float, vp, i64, i32,
                                          i32, i32]),
+    "orc_bsr_spmm_f64": (C.c_int, [i64, vp, vp, i32, i32, vp, i32, dbl, vp, i32, dbl, vp, i32, i32]),
+    "orc_bsr_spmm_f32": (C.c_int, [i64, vp, vp, i32, i32, vp, i32, C.c_float, vp, i32, C.c_float, vp, i32, i32]),
+    "orc_bsr_fill_lower": (None, [i64, vp, vp, i32, vp]),
     "orc_from_triples": (i64, [i64, vp, vp, vp, i32, i64, i64, C.c_int, C.c_int, vp, vp, vp, i64]),
     "orc_grid_laplacian": (i64, [i32, dbl, i32, vp, vp, vp, i64]),
     "orc_transpose": (C.c_int, [i64, i64, vp, vp, vp, i32, vp, vp, vp]),
@@ -60,6 +63,24 @@ def spmm(rp, ci, a, m, n, alpha, X, ldX, beta, Y, ldY, nrhs, NRHS=0, threads=4, 
                                                 nrhs, NRHS, threads)
     return lib(fast).orc_multiply_dense_f32(p(rp), p(ci), p(a), m, n, alpha, p(X), ldX, beta, p(Y), ldY, nrhs,
                                             NRHS, threads)
+
+
+def bsr_spmm(rp, ci, R, C_, A, alpha, X, beta, Y, nrhs, a_rm=True, x_rm=True, y_rm=True):
+    """Oracle KernelMatrixCSR<DenseBlockKernel>::Dot; Y (contiguous, A's dtype) updated in place."""
+    rp = np.ascontiguousarray(rp, np.int64)
+    ci = np.ascontiguousarray(ci, np.int32)
+    mb = len(rp) - 1
+    assert A.flags.c_contiguous and X.flags.c_contiguous and Y.flags.c_contiguous
+    f = lib().orc_bsr_spmm_f64 if Y.dtype == np.float64 else lib().orc_bsr_spmm_f32
+    return f(mb, p(rp), p(ci), R, C_, p(A), int(a_rm), alpha, p(X), int(x_rm), beta, p(Y), int(y_rm), nrhs)
+
+
+def bsr_fill_lower(rp, ci, R, A):
+    """Oracle FillLowerTriangleFromUpperTriangle on float64 block values (in place)."""
+    rp = np.ascontiguousarray(rp, np.int64)
+    ci = np.ascontiguousarray(ci, np.int32)
+    assert A.dtype == np.float64 and A.flags.c_contiguous
+    lib().orc_bsr_fill_lower(len(rp) - 1, p(rp), p(ci), R, p(A))
 
 
 def grid_laplacian(g, mass, parallel_threads=0):

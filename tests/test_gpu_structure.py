@@ -174,7 +174,7 @@ def test_matrix_market_load_matches_oracle(cuda, tmp_path, symmetric):
         fh.write("% generated\n" + f"{m} {n} {k}\n")
         np.savetxt(fh, np.column_stack([I, J, V]), fmt=["%d", "%d", "%.17g"])
     A = MatrixCSR.from_matrix_market(f)
-    T = np.loadtxt(f, comments="%", skiprows=2)
+    T = np.loadtxt(f, comments="%", skiprows=3)  # banner, comment, size line
     orp, oci, oval = oracle.from_triples(T[:, 0].astype(np.int32) - 1, T[:, 1].astype(np.int32) - 1, T[:, 2], m, n,
                                          compress=True, symmetrize=symmetric)
     assert np.array_equal(A.Outer(), orp) and np.array_equal(A.Inner(), oci)
