@@ -43,6 +43,13 @@ constexpr Parallel_T Parallel = Parallel_T::Parallel;
 
 enum class Side : int { Left = 0, Right = 1 };
 
+// Operator tags of MatrixCSR::Op and Sparse::Dot (Tools::Op in the reference).
+enum class Op : int { Id = 0, Conj, Trans, ConjTrans, Re, ReTrans, Im, ImTrans };
+constexpr bool TransposedQ(Op op) {
+    return op == Op::Trans || op == Op::ConjTrans || op == Op::ReTrans || op == Op::ImTrans;
+}
+constexpr bool NotTransposedQ(Op op) { return !TransposedQ(op); }
+
 template <typename I> concept IntQ = std::is_integral_v<I>;
 
 inline void eprint(const std::string& s) { std::fprintf(stderr, "ERROR: %s\n", s.c_str()); }

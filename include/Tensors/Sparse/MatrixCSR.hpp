@@ -294,6 +294,18 @@ template <typename Scal_, typename Int_, typename LInt_, Parallel_T parQ_ = Para
         return FromDevice(n, m, orp, ci.data(), va.data(), nnz);
     }
 
+    // MatrixCSR::Op<op> (MatrixCSR.hpp:389-511) for real scalars: Id / Conj / Re copy,
+    // Trans / ConjTrans / ReTrans transpose (on the GPU), Im / ImTrans are empty.
+    template <Tensors::Op op> MatrixCSR Op() const {
+        if constexpr (op == Tensors::Op::Im) return MatrixCSR(m, n, 0, 1);
+        else if constexpr (op == Tensors::Op::ImTrans) return MatrixCSR(n, m, 0, 1);
+        else if constexpr (Tensors::TransposedQ(op)) return Transpose();
+        else {
+            if (!WellFormedQ()) return MatrixCSR(m, n, 0, 1);
+            return MatrixCSR(*this);
+        }
+    }
+
     // Sparse x sparse (MatrixCSR.hpp:835-921), on the GPU.
     MatrixCSR Dot(const MatrixCSR& B) const {
         if (!WellFormedQ() || !B.WellFormedQ()) {

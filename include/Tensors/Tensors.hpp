@@ -7,6 +7,7 @@
 #include "Tensor.hpp"
 #include "Sparse/MatrixCSR.hpp"
 #include "Sparse/GridLaplacian.hpp"
+#include "Sparse/Dot.hpp"
 #include "BlockKernels.hpp"
 #include "Sparse/PatternCSR.hpp"
 #include "Sparse/KernelMatrixCSR.hpp"

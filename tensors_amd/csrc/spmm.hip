@@ -929,7 +929,7 @@ extern "C" int32_t tg_tune_get(const char* key) {
     if (k == "spmm_group") return g_spmm_group;
     if (k == "spmm_slices") return g_spmm_slices;
     if (k == "spmm_rows") return g_spmm_rows;
-    if (k == "cg_fused") return cg_tune_get(key);
+    if (k == "cg_fused" || k == "cg_unroll") return cg_tune_get(key);
     return batch_tune_get(key);
 }
 

@@ -228,6 +228,11 @@ int main(int argc, char** argv) {
             for (LInt l = C.Outer()[i]; l < C.Outer()[i + 1]; ++l)
                 cerr = std::max(cerr, std::abs(cd[{i, C.Inner()[l]}] - C.Values()[l]));
         check(cok && cerr <= 1e-12, "sparse x sparse A * A^T", cerr);
+        CSR C2 = Sparse::Dot<Op::Id, Op::Trans>(A, A);  // Sparse/Dot.hpp:280-323
+        check(C2.Outer() == C.Outer() && C2.Inner() == C.Inner() && C2.Values() == C.Values(),
+              "Sparse::Dot<Id, Trans>(A, A) == A.Dot(A.Transpose())");
+        CSR At2 = A.Op<Op::Trans>();
+        check(At2.Outer() == At.Outer() && At2.Inner() == At.Inner(), "MatrixCSR::Op<Trans> == Transpose");
     }
 
     // --- ConjugateGradient (ConjugateGradient.hpp:105-353) ---------------------
