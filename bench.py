@@ -58,8 +58,6 @@ def parse():
     ap.add_argument("--cg-fused", type=int, default=-1,
                     help="CG iteration form (tg_tune_set cg_fused): 0 four kernels, 1 p.u in the SpMM, "
                          "2 fused direction, -1 library default")
-    ap.add_argument("--cg-unroll", type=int, default=-1,
-                    help="gathers per round of the CG SpMM (tg_tune_set cg_unroll: 4 or 8; -1 library default)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven CG section")
     ap.add_argument("--batch-systems", type=int, default=2000)
     ap.add_argument("--batch-iters", type=int, default=16, help="CG iterations enqueued per poll")
@@ -71,6 +69,8 @@ def parse():
                     help="column slices of the planned SpMM (tg_tune_set spmm_slices; -1 = library default)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop section")
     ap.add_argument("--no-bsr", action="store_true", help="skip the block-CSR SpMM section")
+    ap.add_argument("--bsr-vector", type=int, default=-1,
+                    help="block-CSR lane mapping (tg_tune_set bsr_vector: 1 vector, 0 scalar; -1 default)")
     ap.add_argument("--no-mm", action="store_true", help="skip the MatrixMarket I/O section")
     ap.add_argument("--serve-requests", type=int, default=160)
     ap.add_argument("--serve-rates", type=str, default="0.25,0.5,1,2",
@@ -164,8 +164,6 @@ def cg_section(args, dev):
 
     if args.cg_fused >= 0:
         capi.call("tg_tune_set", b"cg_fused", args.cg_fused)
-    if args.cg_unroll > 0:
-        capi.call("tg_tune_set", b"cg_unroll", args.cg_unroll)
     g, k, tol = args.cg_grid, 32, 1e-10
     rp, ci, val = gen.laplacian_3d(g, 1e-2)
     n = g ** 3
@@ -521,6 +519,10 @@ def bsr_section(args, dev, read_gbps):
     from tensors_amd import generators as gen
     from tensors_amd.sparse import KernelMatrixCSR
 
+    from tensors_amd import _capi as capi
+
+    if args.bsr_vector >= 0:
+        capi.call("tg_tune_set", b"bsr_vector", args.bsr_vector)
     stream = torch.cuda.current_stream()
     res = []
     for g, R, k in ((96, 3, 1), (96, 3, 4), (64, 4, 4), (48, 8, 4)):

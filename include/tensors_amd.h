@@ -130,6 +130,10 @@ int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, doubl
  *                  four vector kernels, 1 = p.u in the SpMM epilogue, 2 = p
  *                  update, SpMM and p.u in one kernel plus one vector kernel,
  *                  3 = default (form 1).  Results agree to rounding.
+ *   "bsr_vector":  tg_bsr_spmm with row-major X / Y chunks, 1 = a lane per
+ *                  (block row, 1-4 right-hand sides) with vector loads (default),
+ *                  0 = a lane per output entry (the mapping of the other
+ *                  storage orders).
  * Process-wide; not thread-safe against concurrent SpMM calls. */
 int tg_tune_set(const char* key, int32_t value);
 int32_t tg_tune_get(const char* key);

@@ -21,11 +21,27 @@
 // Memory-bound at every block size the reference instantiates (R, C <= 8:
 // 2RCk flops per RCs + Cks bytes stays below 2 flop/byte), so no MFMA.
 
+#include <string>
+
 #include "common.hpp"
+#include "krylov.hpp"
 
 namespace tg {
 
 enum BsrBeta : int { kBsrBetaZero = 0, kBsrBetaOne = 1, kBsrBetaGeneric = 2 };
+
+// 1: row-major X / Y chunks use bsr_spmm_vec_kernel (lane = block row x V
+// right-hand sides); 0: every storage order uses bsr_spmm_kernel (lane = entry).
+static int g_bsr_vector = 1;
+
+int bsr_tune_set(const char* key, int32_t value, bool* known) {
+    *known = std::string(key) == "bsr_vector";
+    if (!*known) return TG_OK;
+    if (value != 0 && value != 1) return fail(TG_ERR_INVALID, "tg_tune_set: bsr_vector must be 0 or 1");
+    g_bsr_vector = value;
+    return TG_OK;
+}
+int32_t bsr_tune_get(const char* key) { return std::string(key) == "bsr_vector" ? g_bsr_vector : -1; }
 
 template <typename T>
 __device__ __forceinline__ T bsr_out(T acc, const T* yp, T alpha, int alpha_one, T beta, int beta_mode) {
@@ -94,6 +110,91 @@ __global__ __launch_bounds__(kBlock) void bsr_spmm_kernel(int mb, int R, int Cr,
             acc += two ? s1 : T(0);
         }
         *yp = bsr_out(acc, yp, alpha, alpha_one, beta, beta_mode);
+    }
+}
+
+// Row-major X and Y chunks (the default storage): lane t of a group owns block
+// row r and V consecutive right-hand sides h0 .. h0+V-1 (entries t, t+L, ... of
+// the R x k/V grid of such tiles), so each block costs it C loads of A and C
+// vector loads of V X entries.  k % V == 0 and V-aligned X / Y are checked by the
+// host.
+template <typename T, typename RP, int CT, int V, bool A_RM>
+__global__ __launch_bounds__(kBlock) void bsr_spmm_vec_kernel(int mb, int R, int Cr, int k, const RP* __restrict__ rp,
+                                                              const int32_t* __restrict__ ci, const T* __restrict__ A,
+                                                              T alpha, const T* __restrict__ X, T beta,
+                                                              T* __restrict__ Y, int L, int alpha_one, int beta_mode) {
+    const int C = CT > 0 ? CT : Cr;
+    const int groups = kBlock / L;
+    const int t = threadIdx.x % L;
+    const int i = blockIdx.x * groups + threadIdx.x / L;
+    if (i >= mb) return;
+    const RP lo = rp[i], hi = rp[i + 1];
+    const int kv = k / V;
+    const int E = R * kv;  // V-wide tiles of the output chunk
+    const int64_t RC = (int64_t)R * C, CK = (int64_t)C * k;
+    T* __restrict__ Yi = Y + (int64_t)i * R * k;
+    for (int e = t; e < E; e += L) {
+        const int r = e / kv, h0 = (e % kv) * V;
+        T* yp = Yi + (int64_t)r * k + h0;
+        T acc[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[v] = T(0);
+        if (hi == lo) {  // WriteYZero: Y_i <- beta Y_i
+            T y[V];
+            if (beta_mode != kBsrBetaZero) load_vec<T, V>(yp, y);
+#pragma unroll
+            for (int v = 0; v < V; ++v)
+                y[v] = beta_mode == kBsrBetaZero ? T(0) : beta_mode == kBsrBetaOne ? y[v] : beta * y[v];
+            store_vec<T, V>(yp, y);
+            continue;
+        }
+        for (RP kb = lo; kb < hi; kb += 2) {
+            const bool two = kb + 1 < hi;
+            const RP kb1 = two ? kb + 1 : kb;
+            const int64_t j0 = ci[kb], j1 = ci[kb1];
+            const T* a0 = A + (int64_t)kb * RC;
+            const T* a1 = A + (int64_t)kb1 * RC;
+            const T* x0 = X + j0 * CK + h0;
+            const T* x1 = X + j1 * CK + h0;
+            T s0[V], s1[V];
+#pragma unroll
+            for (int v = 0; v < V; ++v) s0[v] = s1[v] = T(0);
+            auto step = [&](int c) {
+                const int ia = A_RM ? r * C + c : c * R + r;
+                const T av0 = a0[ia], av1 = a1[ia];
+                T xv0[V], xv1[V];
+                load_vec<T, V>(x0 + (int64_t)c * k, xv0);
+                load_vec<T, V>(x1 + (int64_t)c * k, xv1);
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    s0[v] += av0 * xv0[v];
+                    s1[v] += av1 * xv1[v];
+                }
+            };
+            if constexpr (CT > 0) {
+#pragma unroll
+                for (int c = 0; c < CT; ++c) step(c);
+            } else {
+                for (int c = 0; c < C; ++c) step(c);
+            }
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                acc[v] += s0[v];
+                acc[v] += two ? s1[v] : T(0);
+            }
+        }
+        T y[V];
+        if (beta_mode != kBsrBetaZero) load_vec<T, V>(yp, y);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            T out = alpha_one ? acc[v] : alpha * acc[v];
+            if (beta_mode == kBsrBetaOne)
+                out = out + y[v];
+            else if (beta_mode == kBsrBetaGeneric)
+                out = out + beta * y[v];
+            y[v] = out;
+        }
+        store_vec<T, V>(yp, y);
     }
 }
 
@@ -172,11 +273,64 @@ static int bsr_typed(int32_t mb, int64_t nnzb, int32_t R, int32_t C, const RP* r
         TG_LAUNCH_CHECK();
         return TG_OK;
     }
+    const int alpha_one = alpha_d == 1.0;
+    if (x_rm && y_rm && g_bsr_vector) {  // lane = (block row r, V right-hand sides)
+        const auto aligned = [&](int bytes) {
+            return (uintptr_t)X % bytes == 0 && (uintptr_t)Y % bytes == 0;
+        };
+        // V * C X values of two blocks stay in registers: V * C <= 16 (FP64) / 32 (FP32)
+        const int cap = (int)(128 / sizeof(T));
+        int V = 1;
+        if (k % 4 == 0 && aligned(4 * (int)sizeof(T)) && 4 * C <= cap)
+            V = 4;
+        else if (k % 2 == 0 && aligned(2 * (int)sizeof(T)) && 2 * C <= cap)
+            V = 2;
+        int L = 1;
+        while (L < R * (k / V) && L < kWave) L *= 2;
+        const int groups = kBlock / L;
+        const dim3 grid((unsigned)((mb + groups - 1) / groups));
+#define TG_BSRV(CC, VV)                                                                                            \
+    do {                                                                                                           \
+        if (a_rm)                                                                                                  \
+            hipLaunchKernelGGL((bsr_spmm_vec_kernel<T, RP, CC, VV, true>), grid, dim3(kBlock), 0, s, mb, R, C, k, rp, \
+                               ci, A, alpha, X, beta, Y, L, alpha_one, beta_mode);                                 \
+        else                                                                                                       \
+            hipLaunchKernelGGL((bsr_spmm_vec_kernel<T, RP, CC, VV, false>), grid, dim3(kBlock), 0, s, mb, R, C, k,     \
+                               rp, ci, A, alpha, X, beta, Y, L, alpha_one, beta_mode);                             \
+    } while (0)
+#define TG_BSRVC(CC)             \
+    case CC:                     \
+        if (V == 4)              \
+            TG_BSRV(CC, 4);      \
+        else if (V == 2)         \
+            TG_BSRV(CC, 2);      \
+        else                     \
+            TG_BSRV(CC, 1);      \
+        break;
+        switch (C) {
+            TG_BSRVC(1)
+            TG_BSRVC(2)
+            TG_BSRVC(3)
+            TG_BSRVC(4)
+            TG_BSRVC(6)
+            TG_BSRVC(8)
+            default:
+                if (V == 4)
+                    TG_BSRV(0, 4);
+                else if (V == 2)
+                    TG_BSRV(0, 2);
+                else
+                    TG_BSRV(0, 1);
+        }
+#undef TG_BSRVC
+#undef TG_BSRV
+        TG_LAUNCH_CHECK();
+        return TG_OK;
+    }
     int L = 1;
     while (L < R * k && L < kWave) L *= 2;
     const int groups = kBlock / L;
     const dim3 grid((unsigned)((mb + groups - 1) / groups));
-    const int alpha_one = alpha_d == 1.0;
     switch (C) {
 #define TG_BSRC(CC)                                                                                                 \
     case CC:                                                                                                        \

@@ -251,7 +251,7 @@ __device__ __forceinline__ T fma_t(T a, T b, T c) {
 // fixed tree over the lane groups: deterministic.
 // FORM = false is the plain "u = A p, partial(p.u)" step of the three-pass
 // iteration (p = p0 is formed by cg_p_kernel; r, d, p1 unused).
-template <typename T, typename RP, int G, int V, bool FORM, bool HAS_D, int UN>
+template <typename T, typename RP, int G, int V, bool FORM, bool HAS_D>
 __global__ __launch_bounds__(kBlock) void cg_dir_spmm_kernel(int n, int k, const RP* __restrict__ rp,
                                                              const int32_t* __restrict__ ci, const T* __restrict__ a,
                                                              const T* __restrict__ r, const T* __restrict__ d,
@@ -262,9 +262,9 @@ __global__ __launch_bounds__(kBlock) void cg_dir_spmm_kernel(int n, int k, const
     constexpr int S = kWave / G;  // rows in flight per wave
     constexpr int KT = G * V;     // columns per tile
     constexpr int WAVES = kBlock / kWave;
-    // gathers in flight per lane and round (UN = 8: rows of up to 8 nonzeros,
-    // grid operators, take one round; 4 keeps more waves resident)
-    constexpr int U = UN;
+    // gathers in flight per lane and round (8 measured slower on the 128^3
+    // Laplacian: 86 VGPRs and 5 waves per SIMD instead of 56 and 8)
+    constexpr int U = 4;
     __shared__ double sh[WAVES * kWave * 4];  // WAVES * S * k <= 1024 (k <= G V, or G = 64 and k <= 256)
     const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
     const int s = lane / G, g = lane % G;
@@ -467,45 +467,31 @@ size_t cg_workspace_bytes(int32_t n, int32_t k, size_t vs, int prec_kind) {
 // always take form 0.
 enum CgForm : int { kFourKernels = 0, kSpmmDot = 1, kFormDir = 2, kFormAuto = 3 };
 static int g_cg_fused = kFormAuto;
-// gathers per round of the SpMM+dot kernel for lane groups of >= 8 lanes (4 or 8)
-static int g_cg_unroll = 8;
 
 int cg_tune_set(const char* key, int32_t value, bool* known) {
-    *known = false;
-    const std::string k(key);
-    if (k == "cg_fused") {
-        *known = true;
-        if (value < 0 || value > 3) return fail(TG_ERR_INVALID, "tg_tune_set: cg_fused must be 0, 1, 2 or 3 (auto)");
-        g_cg_fused = value;
-    } else if (k == "cg_unroll") {
-        *known = true;
-        if (value != 4 && value != 8) return fail(TG_ERR_INVALID, "tg_tune_set: cg_unroll must be 4 or 8");
-        g_cg_unroll = value;
-    }
+    *known = std::string(key) == "cg_fused";
+    if (!*known) return TG_OK;
+    if (value < 0 || value > 3) return fail(TG_ERR_INVALID, "tg_tune_set: cg_fused must be 0, 1, 2 or 3 (auto)");
+    g_cg_fused = value;
     return TG_OK;
 }
 
-int32_t cg_tune_get(const char* key) {
-    const std::string k(key);
-    return k == "cg_fused" ? g_cg_fused : k == "cg_unroll" ? g_cg_unroll : -1;
-}
+int32_t cg_tune_get(const char* key) { return std::string(key) == "cg_fused" ? g_cg_fused : -1; }
 
 template <typename T, typename RP, int V>
 static int launch_dir_v(int G, int form, bool has_d, hipStream_t s, int n, int k, const RP* rp, const int32_t* ci,
                         const T* a, const T* r, const T* d, T* p0, T* p1, T* u, double* partial, const CGState* st) {
-#define TG_DIRK(GG, FORM, HD, UU)                                                                         \
-    hipLaunchKernelGGL((cg_dir_spmm_kernel<T, RP, GG, V, FORM, HD, UU>), dim3(kFusedBlocks), dim3(kBlock), 0, s, \
-                       n, k, rp, ci, a, r, d, p0, p1, u, partial, st)
-#define TG_DIR(GG)                                  \
-    case GG:                                        \
-        if (form != kFormDir && GG >= 8 && g_cg_unroll == 8) \
-            TG_DIRK(GG, false, false, (GG >= 8 ? 8 : 4)); \
-        else if (form != kFormDir)                  \
-            TG_DIRK(GG, false, false, 4);           \
-        else if (has_d)                             \
-            TG_DIRK(GG, true, true, 4);             \
-        else                                        \
-            TG_DIRK(GG, true, false, 4);            \
+#define TG_DIRK(GG, FORM, HD)                                                                                  \
+    hipLaunchKernelGGL((cg_dir_spmm_kernel<T, RP, GG, V, FORM, HD>), dim3(kFusedBlocks), dim3(kBlock), 0, s, n, k, \
+                       rp, ci, a, r, d, p0, p1, u, partial, st)
+#define TG_DIR(GG)                     \
+    case GG:                           \
+        if (form != kFormDir)          \
+            TG_DIRK(GG, false, false); \
+        else if (has_d)                \
+            TG_DIRK(GG, true, true);   \
+        else                           \
+            TG_DIRK(GG, true, false);  \
         break;
     switch (G) {
         TG_DIR(1)

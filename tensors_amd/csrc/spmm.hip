@@ -918,6 +918,8 @@ extern "C" int tg_tune_set(const char* key, int32_t value) {
     if (known) return rc;
     rc = cg_tune_set(key, value, &known);
     if (known) return rc;
+    rc = bsr_tune_set(key, value, &known);
+    if (known) return rc;
     return fail(TG_ERR_INVALID, "tg_tune_set: unknown key '%s'", key);
 }
 
@@ -929,7 +931,8 @@ extern "C" int32_t tg_tune_get(const char* key) {
     if (k == "spmm_group") return g_spmm_group;
     if (k == "spmm_slices") return g_spmm_slices;
     if (k == "spmm_rows") return g_spmm_rows;
-    if (k == "cg_fused" || k == "cg_unroll") return cg_tune_get(key);
+    if (k == "cg_fused") return cg_tune_get(key);
+    if (k == "bsr_vector") return bsr_tune_get(key);
     return batch_tune_get(key);
 }
 

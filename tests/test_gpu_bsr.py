@@ -13,13 +13,23 @@ from tensors_amd.sparse import KernelMatrixCSR
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(params=[1, 0], ids=["vector", "scalar"])
+def mapping(request):
+    """Both lane mappings of tg_bsr_spmm (knob bsr_vector), restored afterwards."""
+    from tensors_amd import _capi as capi
+    before = int(capi.lib().tg_tune_get(b"bsr_vector"))
+    capi.call("tg_tune_set", b"bsr_vector", request.param)
+    yield request.param
+    capi.call("tg_tune_set", b"bsr_vector", before)
+
+
 def dev(a, cuda, dtype):
     return torch.from_numpy(np.ascontiguousarray(a)).to(cuda).to(dtype)
 
 
 @pytest.mark.parametrize("R,C", [(1, 1), (2, 2), (3, 3), (4, 4), (6, 6), (8, 8), (2, 5), (5, 3), (12, 12)])
 @pytest.mark.parametrize("k", [1, 3, 8, 20])
-def test_bsr_integer_data_bit_exact(cuda, R, C, k):
+def test_bsr_integer_data_bit_exact(cuda, mapping, R, C, k):
     mb, nb = 57, 49
     rp, ci = block_pattern(mb, nb, 0.15, R * 31 + C)
     rng = np.random.default_rng(R * C + k)
@@ -37,7 +47,7 @@ def test_bsr_integer_data_bit_exact(cuda, R, C, k):
 
 
 @pytest.mark.parametrize("a_rm,x_rm,y_rm", [(a, x, y) for a in (0, 1) for x in (0, 1) for y in (0, 1)])
-def test_bsr_storage_orders(cuda, a_rm, x_rm, y_rm):
+def test_bsr_storage_orders(cuda, mapping, a_rm, x_rm, y_rm):
     R, C, k, mb, nb = 3, 4, 5, 64, 40
     rp, ci = block_pattern(mb, nb, 0.2, 17)
     rng = np.random.default_rng(2)
@@ -54,7 +64,7 @@ def test_bsr_storage_orders(cuda, a_rm, x_rm, y_rm):
     assert np.abs(Yg - Yo).max() <= 1e-13 * max(1.0, np.abs(Yo).max())
 
 
-def test_bsr_special_cases(cuda):
+def test_bsr_special_cases(cuda, mapping):
     R, C, k, mb, nb = 2, 3, 4, 30, 20
     rp, ci = block_pattern(mb, nb, 0.3, 4)
     assert np.any(np.diff(rp) == 0)
@@ -86,7 +96,7 @@ def test_bsr_alias_is_a_no_op(cuda):
     assert torch.equal(Y, before)
 
 
-def test_bsr_large_against_expanded_csr_spmm(cuda):
+def test_bsr_large_against_expanded_csr_spmm(cuda, mapping):
     """A 3 x 3 block 7-point grid pattern (40^3 block rows, 120 x 40^3 scalar unknowns):
     the block product equals the scalar CSR SpMM of the expanded matrix (tg_csr_spmm)."""
     import scipy.sparse as sp

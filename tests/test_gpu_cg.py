@@ -159,11 +159,11 @@ def test_cg_callback_operators_match_builtin(cuda, fused_knob, prec):
     assert torch.equal(X1, X2)
 
 
-@pytest.mark.parametrize("form,unroll", [(1, 8), (1, 4), (2, 4)])
+@pytest.mark.parametrize("form", [1, 2])
 @pytest.mark.parametrize("k", [1, 2, 3, 8, 33, 64, 130, 256])
 @pytest.mark.parametrize("prec", [False, True])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
-def test_cg_fused_iteration_matches_four_kernel_path(cuda, fused_knob, form, unroll, k, prec, dtype):
+def test_cg_fused_iteration_matches_four_kernel_path(cuda, fused_knob, form, k, prec, dtype):
     """The fused CG iteration forms against the four-kernel iteration and the oracle:
     form 1 computes p.u in the SpMM epilogue, form 2 forms p = z + beta p, u = A p and p.u
     in one kernel and x, r and r.z in a second.  Every lane-group width and vector width of
@@ -177,15 +177,12 @@ def test_cg_fused_iteration_matches_four_kernel_path(cuda, fused_knob, form, unr
     tol = 1e-10 if dtype == torch.float64 else 1e-5
     A = MatrixCSR(rp, ci, val, n, n)
     out = {}
-    before = int(capi.lib().tg_tune_get(b"cg_unroll"))
-    capi.call("tg_tune_set", b"cg_unroll", unroll)
     for fused in (0, form):
         fused_knob(fused)
         cg = ConjugateGradient(n, 1000, k, dtype=dtype)
         Xd = torch.zeros(n, k, dtype=dtype, device=cuda)
         assert cg(A, None if d is None else dev(d, cuda, dtype), 1.0, dev(B, cuda, dtype), 0.0, Xd, tol)
         out[fused] = (Xd.double().cpu().numpy(), cg.IterationCount(), cg.RelativeResiduals())
-    capi.call("tg_tune_set", b"cg_unroll", before)
     (X0, it0, rr0), (X1, it1, rr1) = out[0], out[form]
     assert abs(it0 - it1) <= 1
     assert np.all(rr1 <= tol)
