@@ -63,6 +63,8 @@ def parse():
     ap.add_argument("--batch-iters", type=int, default=16, help="CG iterations enqueued per poll")
     ap.add_argument("--batch-tile-rows", type=int, default=0, help="rows per workgroup tile (0 = library default)")
     ap.add_argument("--batch-stream", type=int, default=-1, help="batched SpMM kernel: 1 streaming, 0 row per lane")
+    ap.add_argument("--batch-split", type=int, default=-1,
+                    help="batched iteration: 1 direction pass + SpMM of p, 0 direction rebuilt in the SpMM")
     ap.add_argument("--batch-seq-sample", type=int, default=32,
                     help="systems solved one at a time by the drop-in FP64 CG for comparison")
     ap.add_argument("--slices", type=int, default=-1,
@@ -270,6 +272,8 @@ def batch_section(args, dev):
         capi.call("tg_tune_set", b"batch_tile_rows", args.batch_tile_rows)
     if args.batch_stream >= 0:
         capi.call("tg_tune_set", b"batch_stream", args.batch_stream)
+    if args.batch_split >= 0:
+        capi.call("tg_tune_set", b"batch_split", args.batch_split)
 
     def make(i):
         d, side, k = specs[i]

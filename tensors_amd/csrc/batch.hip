@@ -42,6 +42,7 @@ constexpr int kTileRows = 1024;  // default rows per tile (tg_tune_set "batch_ti
 constexpr int kMinTileRows = 256;
 static int g_batch_tile_rows = kTileRows;
 static int g_batch_stream = 0;  // SpMM kernel of the engine: 0 = a row per lane (measured faster), 1 = streaming chunks
+static int g_batch_split = 1;   // 1: direction pass + SpMM of p (three launches per iteration); 0: fused K1
 constexpr int kGather = 8;     // nonzeros per gather chunk in the SpMM kernel
 constexpr int kRowUnroll = 4;  // rows per step in the update kernel
 
@@ -390,6 +391,148 @@ __global__ __launch_bounds__(kBlock) void bk_dir_spmm_kernel(const Tile* __restr
     tile_partial(acc, S.k, S.kpad, S.ppu, tl.t);
 }
 
+// ---- split form: Kdir (direction) + Kspmm (u = A p, partial p.u) ----------------
+// The fused K1 gathers r, p_prev (and d) for every nonzero to rebuild p of the
+// neighbour rows; the split form writes p for the system's own rows in a
+// streaming pass and gathers only p in the SpMM, which halves the gathered bytes
+// for one more launch per iteration.  The arithmetic of p, u and the partials is
+// the same as the fused form's (p = fma(beta, p_prev, z) per entry; the row sums
+// in nonzero order), so both forms give bitwise the same iterates.
+template <typename T, bool HAS_D, bool FIRST>
+__device__ __forceinline__ void dir_rows(const SysDev<T>& S, const Tile& tl, int c, int ro, int rows, T be,
+                                         const T* __restrict__ pold_, T* __restrict__ pnew_) {
+    const gptr<const T> pold = (gptr<const T>)pold_;
+    const gptr<T> pnew = (gptr<T>)pnew_;
+    const gptr<const T> r = (gptr<const T>)S.r;
+    const gptr<const T> d = (gptr<const T>)S.d;
+    const int k = S.k;
+    for (int i0 = tl.r0 + ro; i0 < tl.r1; i0 += kRowUnroll * rows) {
+        T rv[kRowUnroll], pv[kRowUnroll], dv[kRowUnroll];
+#pragma unroll
+        for (int q = 0; q < kRowUnroll; ++q) {
+            const int i = min(i0 + q * rows, tl.r1 - 1);  // clamped: always a valid row
+            const int64_t e = (int64_t)i * k + c;
+            rv[q] = r[e];
+            if constexpr (!FIRST) pv[q] = pold[e];
+            if constexpr (HAS_D) dv[q] = d[i];
+        }
+#pragma unroll
+        for (int q = 0; q < kRowUnroll; ++q) {
+            const int i = i0 + q * rows;
+            if (i < tl.r1) {
+                T z = rv[q];
+                if constexpr (HAS_D) z = dv[q] * z;
+                T pi;
+                if constexpr (FIRST)
+                    pi = z;
+                else
+                    pi = fma_t(be, pv[q], z);
+                pnew[(int64_t)i * k + c] = pi;
+            }
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ double spmm_dot_rows(const SysDev<T>& S, const Tile& tl, int c, int ro, int rows,
+                                                const T* __restrict__ p_) {
+    const gptr<const T> p = (gptr<const T>)p_;
+    const gptr<const int32_t> rp = (gptr<const int32_t>)S.rp;
+    const gptr<const int32_t> ci = (gptr<const int32_t>)S.ci;
+    const gptr<const T> val = (gptr<const T>)S.val;
+    const gptr<T> u = (gptr<T>)S.u;
+    const int k = S.k;
+    double acc = 0.0;
+    for (int i = tl.r0 + ro; i < tl.r1; i += rows) {
+        const int lb = rp[i], le = rp[i + 1];
+        const int64_t e = (int64_t)i * k + c;
+        const T pi = p[e];  // issued with the index loads
+        T sum = T(0);
+        for (int l0 = lb; l0 < le; l0 += kGather) {
+            int32_t jj[kGather];
+            T vv[kGather], pj[kGather];
+#pragma unroll
+            for (int q = 0; q < kGather; ++q) {
+                const int lq = min(l0 + q, le - 1);
+                const int32_t cq = ci[lq];
+                const T vq = val[lq];
+                const bool ok = l0 + q < le;
+                jj[q] = ok ? cq : i;
+                vv[q] = ok ? vq : T(0);
+            }
+#pragma unroll
+            for (int q = 0; q < kGather; ++q) pj[q] = p[(int64_t)jj[q] * k + c];
+#pragma unroll
+            for (int q = 0; q < kGather; ++q) sum += (l0 + q < le) ? vv[q] * pj[q] : T(0);
+        }
+        u[e] = sum;
+        acc += (double)pi * (double)sum;
+    }
+    return acc;
+}
+
+// Kdir: beta, the stop test and the checkpoint pause as in K1; then p for the own rows.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bk_dir_kernel(const Tile* __restrict__ tiles, const SysDev<T>* __restrict__ sys,
+                                                        SysState* __restrict__ st) {
+    const Tile tl = tiles[xcd_swizzle(blockIdx.x, gridDim.x)];
+    SysState& ss = st[tl.slot];
+    const SysDev<T>& S = sys[tl.slot];
+    const int it = ss.it_b;
+    const int c = threadIdx.x % S.kpad, ro = threadIdx.x / S.kpad, rows = kBlock / S.kpad;
+    const double rz = sys_total(pick(S.prz, it), S.tiles, S.k, S.kpad, c, ro, rows);
+    const double rz_prev = it > 0 ? sys_total(pick(S.prz, it + 1), S.tiles, S.k, S.kpad, c, ro, rows) : 1.0;
+    __shared__ int conv;
+    if (threadIdx.x == 0) conv = 1;
+    __syncthreads();
+    if (ro == 0 && c < S.k && !(fabs(rz) <= ss.TOL[c])) conv = 0;
+    __syncthreads();
+    if (tl.t == 0 && ro == 0 && c < S.k) ss.rho[c] = rz;
+    const bool pause = S.check_every > 0 && it > 0 && it % S.check_every == 0 && it != ss.resumed;
+    if (conv || it >= ss.max_iter || pause) {
+        if (tl.t == 0 && threadIdx.x == 0) {
+            ss.iter = it;
+            ss.succ = conv;
+            ss.paused = (!conv && it < ss.max_iter) ? 1 : 0;
+            ss.done = 1;
+        }
+        return;
+    }
+    if (tl.t == 0 && threadIdx.x == 0) ss.it_a = it;
+    if (c < S.k) {
+        const T be = (T)(rz / rz_prev);
+        const T* pold = pick(S.p, it + 1);
+        T* pnew = pick(S.p, it);
+        if (S.d) {
+            if (it == 0)
+                dir_rows<T, true, true>(S, tl, c, ro, rows, be, pold, pnew);
+            else
+                dir_rows<T, true, false>(S, tl, c, ro, rows, be, pold, pnew);
+        } else {
+            if (it == 0)
+                dir_rows<T, false, true>(S, tl, c, ro, rows, be, pold, pnew);
+            else
+                dir_rows<T, false, false>(S, tl, c, ro, rows, be, pold, pnew);
+        }
+    }
+}
+
+// Kspmm: u = A p and the tile partial of p.u (systems stopped by Kdir skip).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bk_spmm_dot_kernel(const Tile* __restrict__ tiles,
+                                                             const SysDev<T>* __restrict__ sys,
+                                                             SysState* __restrict__ st) {
+    const Tile tl = tiles[xcd_swizzle(blockIdx.x, gridDim.x)];
+    SysState& ss = st[tl.slot];
+    if (ss.done) return;
+    const SysDev<T>& S = sys[tl.slot];
+    const int it = ss.it_a;
+    const int c = threadIdx.x % S.kpad, ro = threadIdx.x / S.kpad, rows = kBlock / S.kpad;
+    double acc = 0.0;
+    if (c < S.k) acc = spmm_dot_rows<T>(S, tl, c, ro, rows, pick(S.p, it));
+    tile_partial(acc, S.k, S.kpad, S.ppu, tl.t);
+}
+
 // ---- K2 body: kRowUnroll rows per step, all loads first, then the stores ----------
 template <typename T, bool HAS_D>
 __device__ __forceinline__ double k2_rows(const SysDev<T>& S, const Tile& tl, int c, int ro, int rows, T al,
@@ -584,8 +727,14 @@ template <typename T> class CGBatch {
         }
         if (ntiles_ == 0) return TG_OK;
         for (int it = 0; it < iters; ++it) {
-            hipLaunchKernelGGL((bk_dir_spmm_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s_, dtiles_, dsys_, dst_,
-                               g_batch_stream);
+            if (g_batch_split) {
+                hipLaunchKernelGGL((bk_dir_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s_, dtiles_, dsys_, dst_);
+                hipLaunchKernelGGL((bk_spmm_dot_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s_, dtiles_, dsys_,
+                                   dst_);
+            } else {
+                hipLaunchKernelGGL((bk_dir_spmm_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s_, dtiles_, dsys_,
+                                   dst_, g_batch_stream);
+            }
             hipLaunchKernelGGL((bk_update_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s_, dtiles_, dsys_, dst_);
         }
         TG_LAUNCH_CHECK();
@@ -1133,6 +1282,12 @@ static int batch_advance(Batch& B, int iters) {
 
 int batch_tune_set(const char* key, int32_t value, bool* known) {
     *known = false;
+    if (std::string(key) == "batch_split") {
+        *known = true;
+        if (value != 0 && value != 1) return fail(TG_ERR_INVALID, "tg_tune_set: batch_split must be 0 or 1");
+        g_batch_split = value;
+        return TG_OK;
+    }
     if (std::string(key) == "batch_stream") {
         *known = true;
         if (value != 0 && value != 1) return fail(TG_ERR_INVALID, "tg_tune_set: batch_stream must be 0 or 1");
@@ -1148,6 +1303,7 @@ int batch_tune_set(const char* key, int32_t value, bool* known) {
 }
 
 int32_t batch_tune_get(const char* key) {
+    if (std::string(key) == "batch_split") return g_batch_split;
     if (std::string(key) == "batch_stream") return g_batch_stream;
     return std::string(key) == "batch_tile_rows" ? g_batch_tile_rows : -1;
 }

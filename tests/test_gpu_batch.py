@@ -108,3 +108,26 @@ def test_batch_spmm_kernels_agree(cuda):
     for i in range(len(sysl)):
         assert abs(rs[i]["iterations"] - rr[i]["iterations"]) <= 1
         assert rel_err(Xs[i], Xr[i]) < 1e-8
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_split_and_fused_direction_forms_are_bitwise_equal(cuda, mixed):
+    """The split iteration (direction pass, SpMM of p, update) and the fused one (direction
+    rebuilt inside the SpMM) compute p, u and every partial with the same operations in the
+    same order: identical iterates, iteration counts and replacements."""
+    from tensors_amd import _capi as capi
+
+    sysl = systems(6, seed=123)
+    before = (capi.lib().tg_tune_get(b"batch_split"), capi.lib().tg_tune_get(b"batch_stream"))
+    try:
+        capi.call("tg_tune_set", b"batch_stream", 0)
+        capi.call("tg_tune_set", b"batch_split", 1)
+        Xs, rs = run_batch(sysl, cuda, mixed, 1e-10)
+        capi.call("tg_tune_set", b"batch_split", 0)
+        Xf, rf = run_batch(sysl, cuda, mixed, 1e-10)
+    finally:
+        capi.call("tg_tune_set", b"batch_split", before[0])
+        capi.call("tg_tune_set", b"batch_stream", before[1])
+    for i in range(len(sysl)):
+        assert rs[i]["iterations"] == rf[i]["iterations"]
+        assert np.array_equal(Xs[i], Xf[i])
