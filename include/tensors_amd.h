@@ -123,8 +123,15 @@ int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, doubl
  *                  1 = never, 2 / 4 / 8 = always that many (nrhs <= 4).
  *   "batch_tile_rows": rows per workgroup tile of systems added to a batched
  *                  engine from now on, a power of two in [256, 8192] (1024).
- *   "batch_stream": SpMM kernel of the batched engine, 0 = a row per lane
- *                  (default), 1 = streaming chunks (measured slower so far).
+ *   "batch_stream": SpMM kernel of the batched engine's fused form, 0 = a row
+ *                  per lane (default), 1 = streaming chunks (measured slower).
+ *   "batch_split": batched iteration, 1 = direction pass + SpMM of p + update
+ *                  (default), 0 = fused direction and SpMM (bitwise the same).
+ *   "batch_idx16": 1 = 16-bit column offsets (ci - row) in the batched SpMM for
+ *                  systems whose offsets fit (default), 0 = int32 indices.
+ *   "batch_replace_drop_pct", "batch_check_every": residual replacement of
+ *                  batched modes 1 / 2 when the estimate fell to this percent
+ *                  (10) at checkpoints every this many iterations (16).
  *   "cg_fused":    iteration of tg_cg_solve / tg_cg_solve_mixed for CSR operators
  *                  with P = I or Jacobi and mean row length <= 16: 0 = SpMM then
  *                  four vector kernels, 1 = p.u in the SpMM epilogue, 2 = p
@@ -288,14 +295,17 @@ int tg_csr_spgemm(int32_t m, int32_t k, int32_t n, const void* Arp, const int32_
  * Jacobi diagonal `diag` (z = diag * r), nrhs in 1..8, B and X n x nrhs
  * row-major device blocks (ld = nrhs), X <- A^{-1} B.
  *   mixed == 0: FP64 CG with the reference stopping rule |r.z| <= tol^2 |b.Pb|.
- *   mixed != 0: CG in FP32 (FP32 copy of A, FP32 vectors, FP64 reductions)
+ *   mixed == 1: CG in FP32 (FP32 copy of A, FP32 vectors, FP64 reductions)
  *               with FP64 residual replacement: every 16 iterations the host
  *               may replace the recursive residual by b - A x computed in FP64
  *               (x accumulated in FP64), keeping the search direction; done
  *               when the true ||B - A X|| / ||B|| <= tol in every column
  *               (max_iter bounds the total iterations).  `outer` in
  *               tg_batch_result counts the replacements.
- * All active systems advance together (two launches per CG iteration cover
+ *   mixed == 2: FP64 CG with the same residual replacement and the same
+ *               true-residual stop as mode 1 (the FP64 comparison at equal
+ *               accuracy).
+ * All active systems advance together (three launches per CG iteration cover
  * every system's row tiles); a system's arithmetic does not depend on the
  * others, so its result is bitwise the same alone and in any mix.  Systems may be added
  * between steps.  The caller's arrays must stay valid until the system has

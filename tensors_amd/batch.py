@@ -33,14 +33,18 @@ class BatchCG:
         except Exception:
             pass
 
-    def add(self, rp, ci, val, B, X, tol, max_iter, diag=None, mixed=False):
-        """rp/ci int32 CUDA tensors, val/diag/B/X float64 CUDA tensors (X written in place)."""
+    def add(self, rp, ci, val, B, X, tol, max_iter, diag=None, mixed=False, true_residual=False):
+        """rp/ci int32 CUDA tensors, val/diag/B/X float64 CUDA tensors (X written in place).
+        mixed: FP32 iteration with FP64 residual replacement, stopping on the true residual;
+        true_residual (FP64 only): the same replacement and stop in FP64 (mode 2) instead of
+        the reference's preconditioned test."""
         n = rp.numel() - 1
         k = B.shape[1] if B.dim() == 2 else 1
         sid = C.c_int64(0)
+        mode = 1 if mixed else (2 if true_residual else 0)
         capi.call("tg_batch_add_cg", self.h, n, ci.numel(), rp.data_ptr(), ci.data_ptr(), val.data_ptr(),
                   diag.data_ptr() if diag is not None else None, k, B.data_ptr(), X.data_ptr(), float(tol),
-                  int(max_iter), int(bool(mixed)), C.byref(sid))
+                  int(max_iter), mode, C.byref(sid))
         self._keep[sid.value] = (rp, ci, val, diag, B, X, k)
         return sid.value
 

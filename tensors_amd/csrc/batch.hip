@@ -43,12 +43,14 @@ constexpr int kMinTileRows = 256;
 static int g_batch_tile_rows = kTileRows;
 static int g_batch_stream = 0;  // SpMM kernel of the engine: 0 = a row per lane (measured faster), 1 = streaming chunks
 static int g_batch_split = 1;   // 1: direction pass + SpMM of p (three launches per iteration); 0: fused K1
+static int g_batch_idx16 = 1;   // 1: 16-bit column offsets in the split SpMM where they fit (read at add)
 constexpr int kGather = 8;     // nonzeros per gather chunk in the SpMM kernel
 constexpr int kRowUnroll = 4;  // rows per step in the update kernel
 
 template <typename T> struct SysDev {
     const int32_t* rp;
     const int32_t* ci;
+    const int16_t* cid;  // ci - row as 16 bits when every |ci - row| <= 32767 (else null)
     const T* val;
     const T* d;  // Jacobi diagonal (inverse) or null
     const T* b;
@@ -279,6 +281,28 @@ __device__ double k1_stream(const SysDev<T>& S, const Tile& tl, int c, int ro, i
     return acc;
 }
 
+// ---- 16-bit column offsets ---------------------------------------------------------
+// cid[l] = ci[l] - row for every nonzero; *ok is cleared if an offset does not fit
+// (the system then keeps its int32 indices).  Grid operators, the engine's
+// workload, have offsets up to a plane of the grid.
+__global__ __launch_bounds__(kBlock) void bk_delta16_kernel(int n, int64_t nnz, const int32_t* __restrict__ rp,
+                                                            const int32_t* __restrict__ ci, int16_t* __restrict__ cid,
+                                                            int* __restrict__ ok) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        for (int l = rp[i]; l < rp[i + 1]; ++l) {
+            if (l >= nnz) {  // rp does not match the nnz given: keep the int32 indices
+                atomicAnd(ok, 0);
+                break;
+            }
+            int dlt = ci[l] - i;
+            if (dlt < -32768 || dlt > 32767) {
+                atomicAnd(ok, 0);
+                dlt = 0;
+            }
+            cid[l] = (int16_t)dlt;
+        }
+}
+
 // ---- K1 body, a row per (row, column) lane -------------------------------------------
 // A row's nonzeros in chunks of kGather.  The loop body has no branches: the
 // index/value loads are made at clamped (always valid) positions and masked
@@ -433,12 +457,14 @@ __device__ __forceinline__ void dir_rows(const SysDev<T>& S, const Tile& tl, int
     }
 }
 
-template <typename T>
+// IDX16: column indices from the 16-bit offsets (2 bytes per nonzero instead of 4).
+template <typename T, bool IDX16>
 __device__ __forceinline__ double spmm_dot_rows(const SysDev<T>& S, const Tile& tl, int c, int ro, int rows,
                                                 const T* __restrict__ p_) {
     const gptr<const T> p = (gptr<const T>)p_;
     const gptr<const int32_t> rp = (gptr<const int32_t>)S.rp;
     const gptr<const int32_t> ci = (gptr<const int32_t>)S.ci;
+    const gptr<const int16_t> cid = (gptr<const int16_t>)S.cid;
     const gptr<const T> val = (gptr<const T>)S.val;
     const gptr<T> u = (gptr<T>)S.u;
     const int k = S.k;
@@ -454,7 +480,11 @@ __device__ __forceinline__ double spmm_dot_rows(const SysDev<T>& S, const Tile& 
 #pragma unroll
             for (int q = 0; q < kGather; ++q) {
                 const int lq = min(l0 + q, le - 1);
-                const int32_t cq = ci[lq];
+                int32_t cq;
+                if constexpr (IDX16)
+                    cq = i + (int32_t)cid[lq];
+                else
+                    cq = ci[lq];
                 const T vq = val[lq];
                 const bool ok = l0 + q < le;
                 jj[q] = ok ? cq : i;
@@ -529,7 +559,9 @@ __global__ __launch_bounds__(kBlock) void bk_spmm_dot_kernel(const Tile* __restr
     const int it = ss.it_a;
     const int c = threadIdx.x % S.kpad, ro = threadIdx.x / S.kpad, rows = kBlock / S.kpad;
     double acc = 0.0;
-    if (c < S.k) acc = spmm_dot_rows<T>(S, tl, c, ro, rows, pick(S.p, it));
+    if (c < S.k)
+        acc = S.cid ? spmm_dot_rows<T, true>(S, tl, c, ro, rows, pick(S.p, it))
+                    : spmm_dot_rows<T, false>(S, tl, c, ro, rows, pick(S.p, it));
     tile_partial(acc, S.k, S.kpad, S.ppu, tl.t);
 }
 
@@ -629,8 +661,8 @@ template <typename T> class CGBatch {
     ~CGBatch() { release(); }
 
     // Adds a system; its setup runs at once.  b, x: n x k device blocks (ld = k).
-    int add(int n, const int32_t* rp, const int32_t* ci, const T* val, const T* d, int k, const T* b, T* x,
-            double tol, int max_iter, int* slot_out, int check_every = 0) {
+    int add(int n, int64_t nnz, const int32_t* rp, const int32_t* ci, const T* val, const T* d, int k, const T* b,
+            T* x, double tol, int max_iter, int* slot_out, int check_every = 0) {
         if (n < 1 || k < 1 || k > kBatchMaxK) return fail(TG_ERR_INVALID, "batch: n >= 1 and 1 <= nrhs <= 8");
         int slot = -1;
         for (int i = 0; i < (int)used_.size(); ++i)
@@ -659,9 +691,12 @@ template <typename T> class CGBatch {
         const size_t N = (size_t)n * k;
         const size_t nvec = 4;  // r, u, p0, p1
         const size_t vb = align256(N * sizeof(T)), pb = align256((size_t)S.tiles * k * sizeof(double));
-        const size_t bytes = nvec * vb + 3 * pb;
+        const size_t ib = align256((size_t)std::max<int64_t>(nnz, 1) * sizeof(int16_t));
+        const size_t bytes = nvec * vb + 3 * pb + ib + 256;
         char* mem = nullptr;
         TG_HIP(hipMallocAsync((void**)&mem, bytes, s_));
+        int16_t* cid = (int16_t*)(mem + nvec * vb + 3 * pb);
+        int* dok = (int*)(mem + nvec * vb + 3 * pb + ib);
         S.r = (T*)mem;
         S.u = (T*)(mem + vb);
         S.p[0] = (T*)(mem + 2 * vb);
@@ -693,9 +728,22 @@ template <typename T> class CGBatch {
         hipLaunchKernelGGL((bk_init_kernel<T>), dim3((unsigned)tl.size()), dim3(kBlock), 0, s_, dsetup_tiles_, dsys_);
         hipLaunchKernelGGL((bk_setup_kernel<T>), dim3(1), dim3(kBlock), 0, s_, dsetup_slot_, dsys_, dst_,
                            (const double*)dtol2_);
+        // 16-bit column offsets for the SpMM of the split iteration
+        const int one = 1;
+        int ok16 = 0;
+        TG_HIP(hipMemcpyAsync(dok, &one, sizeof(int), hipMemcpyHostToDevice, s_));
+        hipLaunchKernelGGL(bk_delta16_kernel, dim3((unsigned)std::min(ceil_div(n, kBlock), 4096)), dim3(kBlock), 0,
+                           s_, n, nnz, rp, ci, cid, dok);
         TG_LAUNCH_CHECK();
+        TG_HIP(hipMemcpyAsync(&ok16, dok, sizeof(int), hipMemcpyDeviceToHost, s_));
         // the setup buffers are reused by the next add: finish this one first
         TG_HIP(hipStreamSynchronize(s_));
+        if (ok16 && g_batch_idx16) {
+            S.cid = cid;
+            host_[slot] = S;
+            TG_HIP(hipMemcpyAsync(dsys_ + slot, &S, sizeof(S), hipMemcpyHostToDevice, s_));
+            TG_HIP(hipStreamSynchronize(s_));  // S is a stack copy
+        }
         dirty_ = true;
         *slot_out = slot;
         return TG_OK;
@@ -904,7 +952,9 @@ struct ReplaceDesc {
     double* x64;
     float* x32;
     const float* d32;
-    float* r32;      // the FP32 engine's r
+    float* r32;      // the FP32 engine's r (mixed systems)
+    double* r64;     // the FP64 engine's r (FP64 systems with the true-residual stop)
+    const double* d64;
     double* prz;     // the engine's r.z partials of the current parity
     double* pnorm;   // ||r||^2 partials
     double* norms;   // k true-residual norms out
@@ -915,6 +965,9 @@ struct ReplaceTile {
     int desc, r0, r1, t;
 };
 
+// F32: mixed systems (x = x64 + x32, the replaced residual goes to the FP32
+// engine's r); otherwise FP64 systems (x = x64 is the engine's x, r in FP64).
+template <bool F32>
 __global__ __launch_bounds__(kBlock) void bm_replace_many_kernel(const ReplaceTile* __restrict__ tiles,
                                                                  const ReplaceDesc* __restrict__ descs) {
     const ReplaceTile tl = tiles[xcd_swizzle(blockIdx.x, gridDim.x)];
@@ -946,17 +999,23 @@ __global__ __launch_bounds__(kBlock) void bm_replace_many_kernel(const ReplaceTi
 #pragma unroll
                 for (int q = 0; q < kGather; ++q) {
                     const int64_t f = (int64_t)jj[q] * k + c;
-                    xv[q] = x64[f] + (double)x32[f];
+                    xv[q] = F32 ? x64[f] + (double)x32[f] : x64[f];
                 }
 #pragma unroll
                 for (int q = 0; q < kGather; ++q) ax += (l0 + q < le) ? vv[q] * xv[q] : 0.0;
             }
             const int64_t e = (int64_t)i * k + c;
             const double r = D.b[e] - ax;
-            const float rf = (float)r;
-            D.r32[e] = rf;
-            const float zf = D.d32 ? D.d32[i] * rf : rf;
-            acc_rz += (double)rf * (double)zf;
+            if constexpr (F32) {
+                const float rf = (float)r;
+                D.r32[e] = rf;
+                const float zf = D.d32 ? D.d32[i] * rf : rf;
+                acc_rz += (double)rf * (double)zf;
+            } else {
+                D.r64[e] = r;
+                const double z = D.d64 ? D.d64[i] * r : r;
+                acc_rz += r * z;
+            }
             acc_n += r * r;
         }
     }
@@ -1017,10 +1076,16 @@ __global__ __launch_bounds__(kBlock) void bm_tofloat_kernel(int64_t count, const
         b[e] = (float)a[e];
 }
 
+constexpr double kReplaceDrop = 0.1;  // replace when the residual estimate fell by this factor
+constexpr int kCheckEvery = 16;       // mixed systems pause every 16 iterations for that test
+// knobs "batch_replace_drop_pct" / "batch_check_every": both are read when a system is added
+static double g_replace_drop = kReplaceDrop;
+static int g_check_every = kCheckEvery;
+
 // ---- the batch object behind the C ABI ------------------------------------------
 struct BatchSystem {
     int64_t id;
-    int mixed;
+    int mixed;  // 0: FP64, the reference's stop test; 1: mixed; 2: FP64, true-residual stop
     int n, k;
     int64_t nnz;
     const int32_t* rp;
@@ -1035,6 +1100,8 @@ struct BatchSystem {
     // mixed
     int replacements = 0;
     double est_last = 1.0;   // preconditioned residual estimate at the last replacement
+    double replace_drop = kReplaceDrop;
+    int check_every = kCheckEvery;
     char* mem = nullptr;     // a32, d32, b32, x32, norm partials, norms
     size_t mem_bytes = 0;
     float* a32 = nullptr;
@@ -1053,32 +1120,46 @@ struct BatchSystem {
 // (replacement, fold of x32 into x64, norms) and reads every norm back in one copy.
 struct ReplaceBatch {
     std::vector<ReplaceDesc> descs;
-    std::vector<ReplaceTile> tiles;
+    std::vector<ReplaceTile> tiles;  // tiles of the mixed systems, then of the FP64 ones
+    size_t tiles32 = 0;              // how many of them are mixed
+    std::vector<ReplaceTile> tiles64;
     std::vector<BatchSystem*> sys;
     ReplaceDesc* ddesc = nullptr;
     ReplaceTile* dtile = nullptr;
     size_t desc_cap = 0, tile_cap = 0;
 
+    // a mixed system (FP32 engine)
     void add(const SysDev<float>& E, double* norms_out, BatchSystem& S, int it) {
         ReplaceDesc D{};
+        D.x32 = S.x32;
+        D.d32 = S.d32;
+        D.r32 = E.r;
+        add_common(D, E.prz, E.tiles, E.tile_rows, norms_out, S, it, tiles);
+    }
+    // an FP64 system with the true-residual stop (FP64 engine)
+    void add(const SysDev<double>& E, double* norms_out, BatchSystem& S, int it) {
+        ReplaceDesc D{};
+        D.r64 = E.r;
+        D.d64 = S.d64;
+        add_common(D, E.prz, E.tiles, E.tile_rows, norms_out, S, it, tiles64);
+    }
+    void add_common(ReplaceDesc& D, double* const (&prz)[2], int etiles, int tile_rows, double* norms_out,
+                    BatchSystem& S, int it, std::vector<ReplaceTile>& out) {
         D.rp = S.rp;
         D.ci = S.ci;
         D.val = S.val64;
         D.b = S.b64;
         D.x64 = S.x64;
-        D.x32 = S.x32;
-        D.d32 = S.d32;
-        D.r32 = E.r;
-        D.prz = pick(E.prz, it);
+        D.prz = pick(prz, it);
         D.pnorm = S.pnorm;
         D.norms = norms_out;
         D.n = S.n;
         D.k = S.k;
         D.kpad = pow2_at_least(S.k);
-        D.tiles = E.tiles;
+        D.tiles = etiles;
         const int di = (int)descs.size();
-        for (int t = 0; t < E.tiles; ++t)
-            tiles.push_back(ReplaceTile{di, t * E.tile_rows, std::min(S.n, (t + 1) * E.tile_rows), t});
+        for (int t = 0; t < etiles; ++t)
+            out.push_back(ReplaceTile{di, t * tile_rows, std::min(S.n, (t + 1) * tile_rows), t});
         descs.push_back(D);
         sys.push_back(&S);
         ++S.replacements;
@@ -1086,6 +1167,8 @@ struct ReplaceBatch {
 
     int run(hipStream_t s) {
         if (descs.empty()) return TG_OK;
+        tiles32 = tiles.size();
+        tiles.insert(tiles.end(), tiles64.begin(), tiles64.end());
         if (descs.size() > desc_cap) {
             if (ddesc) TG_HIP(hipFreeAsync(ddesc, s));
             desc_cap = std::max(descs.size(), 2 * desc_cap);
@@ -1098,10 +1181,15 @@ struct ReplaceBatch {
         }
         TG_HIP(hipMemcpyAsync(ddesc, descs.data(), descs.size() * sizeof(ReplaceDesc), hipMemcpyHostToDevice, s));
         TG_HIP(hipMemcpyAsync(dtile, tiles.data(), tiles.size() * sizeof(ReplaceTile), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(bm_replace_many_kernel, dim3((unsigned)tiles.size()), dim3(kBlock), 0, s,
-                           (const ReplaceTile*)dtile, (const ReplaceDesc*)ddesc);
-        hipLaunchKernelGGL(bm_flush_many_kernel, dim3((unsigned)tiles.size()), dim3(kBlock), 0, s,
-                           (const ReplaceTile*)dtile, (const ReplaceDesc*)ddesc);
+        if (tiles32) {
+            hipLaunchKernelGGL(bm_replace_many_kernel<true>, dim3((unsigned)tiles32), dim3(kBlock), 0, s,
+                               (const ReplaceTile*)dtile, (const ReplaceDesc*)ddesc);
+            hipLaunchKernelGGL(bm_flush_many_kernel, dim3((unsigned)tiles32), dim3(kBlock), 0, s,
+                               (const ReplaceTile*)dtile, (const ReplaceDesc*)ddesc);
+        }
+        if (tiles.size() > tiles32)
+            hipLaunchKernelGGL(bm_replace_many_kernel<false>, dim3((unsigned)(tiles.size() - tiles32)), dim3(kBlock),
+                               0, s, (const ReplaceTile*)(dtile + tiles32), (const ReplaceDesc*)ddesc);
         hipLaunchKernelGGL(bm_norm_many_kernel, dim3((unsigned)descs.size()), dim3(kBlock), 0, s,
                            (const ReplaceDesc*)ddesc);
         TG_LAUNCH_CHECK();
@@ -1111,6 +1199,8 @@ struct ReplaceBatch {
     void clear() {
         descs.clear();
         tiles.clear();
+        tiles64.clear();
+        tiles32 = 0;
         sys.clear();
     }
 
@@ -1132,16 +1222,19 @@ struct Batch {
     double* dnorms = nullptr;  // kBatchMaxK true-residual norms per FP32-engine slot
     int dnorms_cap = 0;
     std::vector<double> hnorms;
+    double* dnorms64 = nullptr;  // the same per FP64-engine slot (true-residual FP64 systems)
+    int dnorms64_cap = 0;
+    std::vector<double> hnorms64;
     ReplaceBatch rb;
     explicit Batch(hipStream_t st) : s(st), b64(st), b32(st) {}
     ~Batch() {
         if (dnorms) (void)hipFreeAsync(dnorms, s);
+        if (dnorms64) (void)hipFreeAsync(dnorms64, s);
         rb.release(s);
     }
 };
 
-constexpr double kReplaceDrop = 0.1;  // replace when the residual estimate fell by this factor
-constexpr int kCheckEvery = 16;       // mixed systems pause every 16 iterations for that test
+
 constexpr int kMaxReplacements = 200;
 
 static int finish(Batch& B, BatchSystem& S, int succeeded, int iterations) {
@@ -1192,7 +1285,33 @@ static int mixed_start(Batch& B, BatchSystem& S) {
         if (std::isnan(S.bnorm[c])) return finish(B, S, 0, 0);
     // The engine's own test (preconditioned FP32 estimate <= tol) only pauses the
     // system; the host then decides on the FP64 true residual.
-    return B.b32.add(S.n, S.rp, S.ci, S.a32, S.d32, S.k, S.b32, S.x32, S.tol, S.max_iter, &S.slot, kCheckEvery);
+    return B.b32.add(S.n, S.nnz, S.rp, S.ci, S.a32, S.d32, S.k, S.b32, S.x32, S.tol, S.max_iter, &S.slot,
+                     S.check_every);
+}
+
+// FP64 system with the true-residual stop: the FP64 engine iterates on the caller's
+// arrays with checkpoints; the host replaces the residual there (FP64) and stops
+// on ||b - A x|| / ||b|| <= tol, exactly as for mixed systems.
+static int true_start(Batch& B, BatchSystem& S) {
+    const int tiles = ceil_div(S.n, kTileRows);
+    const int ptiles = ceil_div(S.n, kMinTileRows);
+    const size_t bytes = align256((size_t)ptiles * S.k * 8) + align256((size_t)S.k * 8);
+    TG_HIP(hipMallocAsync((void**)&S.mem, bytes, B.s));
+    S.mem_bytes = bytes;
+    S.pnorm = (double*)S.mem;
+    S.norms = (double*)(S.mem + align256((size_t)ptiles * S.k * 8));
+    const int kpad = pow2_at_least(S.k);
+    hipLaunchKernelGGL(bm_bnorm_kernel, dim3(tiles), dim3(kBlock), 0, B.s, S.n, S.k, kpad, S.b64, S.pnorm);
+    hipLaunchKernelGGL(bm_norm_total_kernel, dim3(1), dim3(kBlock), 0, B.s, tiles, S.k, kpad, (const double*)S.pnorm,
+                       S.norms);
+    TG_LAUNCH_CHECK();
+    S.bnorm.assign(S.k, 0.0);
+    TG_HIP(hipMemcpyAsync(S.bnorm.data(), S.norms, S.k * 8, hipMemcpyDeviceToHost, B.s));
+    TG_HIP(hipStreamSynchronize(B.s));
+    for (int c = 0; c < S.k; ++c)
+        if (std::isnan(S.bnorm[c])) return finish(B, S, 0, 0);
+    return B.b64.add(S.n, S.nnz, S.rp, S.ci, S.val64, S.d64, S.k, S.b64, S.x64, S.tol, S.max_iter, &S.slot,
+                     S.check_every);
 }
 
 static int batch_advance(Batch& B, int iters) {
@@ -1210,10 +1329,28 @@ static int batch_advance(Batch& B, int iters) {
         B.dnorms_cap = std::max(B.b32.capacity(), 2 * B.dnorms_cap);
         TG_HIP(hipMallocAsync((void**)&B.dnorms, (size_t)B.dnorms_cap * kBatchMaxK * sizeof(double), B.s));
     }
+    if (B.dnorms64_cap < B.b64.capacity()) {
+        if (B.dnorms64) TG_HIP(hipFreeAsync(B.dnorms64, B.s));
+        B.dnorms64_cap = std::max(B.b64.capacity(), 2 * B.dnorms64_cap);
+        TG_HIP(hipMallocAsync((void**)&B.dnorms64, (size_t)B.dnorms64_cap * kBatchMaxK * sizeof(double), B.s));
+    }
+    // park / resume in the engine the system runs in
+    const auto park = [&](BatchSystem& S) {
+        if (S.mixed == 1)
+            B.b32.park(S.slot);
+        else
+            B.b64.park(S.slot);
+    };
+    const auto resume = [&](BatchSystem& S, int it, double scale) {
+        if (S.mixed == 1)
+            B.b32.resume(S.slot, it, scale);
+        else
+            B.b64.resume(S.slot, it, scale);
+    };
     for (auto& kv : B.sys) {
         BatchSystem& S = *kv.second;
         if (S.finished || S.slot < 0) continue;
-        if (!S.mixed) {
+        if (S.mixed == 0) {
             const SysState& st = s64[S.slot];
             if (st.done) {
                 for (int c = 0; c < S.k; ++c) S.rel[c] = st.bsq[c] > 0 ? std::sqrt(std::fabs(st.rho[c]) / st.bsq[c]) : 0.0;
@@ -1223,26 +1360,29 @@ static int batch_advance(Batch& B, int iters) {
             }
             continue;
         }
-        const SysState& st = s32[S.slot];
+        const SysState& st = S.mixed == 1 ? s32[S.slot] : s64[S.slot];
         if (st.nan) {
-            B.b32.park(S.slot);
+            park(S);
             rc = finish(B, S, 0, st.iter);
             if (rc) return rc;
             continue;
         }
-        // Decisions only at the system's own checkpoints (every kCheckEvery iterations:
+        // Decisions only at the system's own checkpoints (every check_every iterations:
         // paused there by the device, or standing exactly there at this poll) and stops.
         const bool stopped = st.done && !st.paused;
         const bool at_check = st.done ? (bool)st.paused
-                                      : (st.it_b > 0 && st.it_b % kCheckEvery == 0 && st.it_b != st.resumed);
+                                      : (st.it_b > 0 && st.it_b % S.check_every == 0 && st.it_b != st.resumed);
         if (!at_check && !stopped) continue;
         double est = 0.0;
         for (int c = 0; c < S.k; ++c) est = std::max(est, st.bsq[c] > 0 ? std::sqrt(std::fabs(st.rho[c]) / st.bsq[c]) : 0.0);
-        if (!stopped && !(est < kReplaceDrop * S.est_last)) {
-            B.b32.resume(S.slot, st.it_b, 1.0);
+        if (!stopped && !(est < S.replace_drop * S.est_last)) {
+            resume(S, st.it_b, 1.0);
             continue;
         }
-        B.rb.add(B.b32.desc(S.slot), B.dnorms + (size_t)S.slot * kBatchMaxK, S, st.it_b);
+        if (S.mixed == 1)
+            B.rb.add(B.b32.desc(S.slot), B.dnorms + (size_t)S.slot * kBatchMaxK, S, st.it_b);
+        else
+            B.rb.add(B.b64.desc(S.slot), B.dnorms64 + (size_t)S.slot * kBatchMaxK, S, st.it_b);
         S.est_last = est;
     }
     if (B.rb.sys.empty()) return TG_OK;
@@ -1250,30 +1390,34 @@ static int batch_advance(Batch& B, int iters) {
     if (rc) return rc;
     B.hnorms.resize((size_t)B.dnorms_cap * kBatchMaxK);
     TG_HIP(hipMemcpyAsync(B.hnorms.data(), B.dnorms, B.hnorms.size() * sizeof(double), hipMemcpyDeviceToHost, B.s));
+    B.hnorms64.resize((size_t)B.dnorms64_cap * kBatchMaxK);
+    TG_HIP(hipMemcpyAsync(B.hnorms64.data(), B.dnorms64, B.hnorms64.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          B.s));
     TG_HIP(hipStreamSynchronize(B.s));
     std::vector<BatchSystem*> replaced = B.rb.sys;
     B.rb.clear();
     for (BatchSystem* Sp : replaced) {
         BatchSystem& S = *Sp;
-        const SysState& st = s32[S.slot];
+        const SysState& st = S.mixed == 1 ? s32[S.slot] : s64[S.slot];
+        const std::vector<double>& hn = S.mixed == 1 ? B.hnorms : B.hnorms64;
         double worst = 0.0;
         for (int c = 0; c < S.k; ++c) {
-            const double rn = B.hnorms[(size_t)S.slot * kBatchMaxK + c];
+            const double rn = hn[(size_t)S.slot * kBatchMaxK + c];
             S.rel[c] = S.bnorm[c] > 0 ? rn / S.bnorm[c] : 0.0;
             worst = std::max(worst, S.rel[c]);
         }
         if (worst <= S.tol) {
-            B.b32.park(S.slot);
+            park(S);
             rc = finish(B, S, 1, st.it_b);
         } else if (st.it_b >= S.max_iter || S.replacements >= kMaxReplacements || std::isnan(worst)) {
-            B.b32.park(S.slot);
+            park(S);
             rc = finish(B, S, 0, st.it_b);
         } else if (st.done && !st.paused) {
-            // the FP32 estimate met the tolerance but the true residual did not: tighten it
+            // the engine's estimate met the tolerance but the true residual did not: tighten it
             const double f = S.tol / worst;
-            B.b32.resume(S.slot, st.it_b, std::min(0.25, std::max(1e-4, 0.25 * f * f)));
+            resume(S, st.it_b, std::min(0.25, std::max(1e-4, 0.25 * f * f)));
         } else {
-            B.b32.resume(S.slot, st.it_b, 1.0);
+            resume(S, st.it_b, 1.0);
         }
         if (rc) return rc;
     }
@@ -1282,6 +1426,24 @@ static int batch_advance(Batch& B, int iters) {
 
 int batch_tune_set(const char* key, int32_t value, bool* known) {
     *known = false;
+    if (std::string(key) == "batch_replace_drop_pct") {
+        *known = true;
+        if (value < 1 || value > 99) return fail(TG_ERR_INVALID, "tg_tune_set: batch_replace_drop_pct in [1, 99]");
+        g_replace_drop = value / 100.0;
+        return TG_OK;
+    }
+    if (std::string(key) == "batch_check_every") {
+        *known = true;
+        if (value < 1 || value > 1024) return fail(TG_ERR_INVALID, "tg_tune_set: batch_check_every in [1, 1024]");
+        g_check_every = value;
+        return TG_OK;
+    }
+    if (std::string(key) == "batch_idx16") {
+        *known = true;
+        if (value != 0 && value != 1) return fail(TG_ERR_INVALID, "tg_tune_set: batch_idx16 must be 0 or 1");
+        g_batch_idx16 = value;
+        return TG_OK;
+    }
     if (std::string(key) == "batch_split") {
         *known = true;
         if (value != 0 && value != 1) return fail(TG_ERR_INVALID, "tg_tune_set: batch_split must be 0 or 1");
@@ -1303,7 +1465,10 @@ int batch_tune_set(const char* key, int32_t value, bool* known) {
 }
 
 int32_t batch_tune_get(const char* key) {
+    if (std::string(key) == "batch_replace_drop_pct") return (int32_t)std::lround(g_replace_drop * 100.0);
+    if (std::string(key) == "batch_check_every") return g_check_every;
     if (std::string(key) == "batch_split") return g_batch_split;
+    if (std::string(key) == "batch_idx16") return g_batch_idx16;
     if (std::string(key) == "batch_stream") return g_batch_stream;
     return std::string(key) == "batch_tile_rows" ? g_batch_tile_rows : -1;
 }
@@ -1334,12 +1499,15 @@ extern "C" int tg_batch_destroy(tg_batch_t h) {
 extern "C" int tg_batch_add_cg(tg_batch_t h, int32_t n, int64_t nnz, const int32_t* rp, const int32_t* ci,
                                const double* val, const double* diag, int32_t nrhs, const double* B, double* X,
                                double tol, int32_t max_iter, int32_t mixed, int64_t* id_out) {
-    if (!h || !id_out || n < 1 || nrhs < 1 || nrhs > kBatchMaxK || !rp || !ci || !val || !B || !X)
-        return fail(TG_ERR_INVALID, "tg_batch_add_cg: bad argument (1 <= nrhs <= 8, device pointers)");
+    if (!h || !id_out || n < 1 || nrhs < 1 || nrhs > kBatchMaxK || !rp || !ci || !val || !B || !X || mixed < 0 ||
+        mixed > 2)
+        return fail(TG_ERR_INVALID, "tg_batch_add_cg: bad argument (1 <= nrhs <= 8, mode 0 / 1 / 2, device pointers)");
     Batch& Bt = *h->b;
     auto S = std::make_unique<BatchSystem>();
     S->id = Bt.next_id++;
-    S->mixed = mixed ? 1 : 0;
+    S->mixed = mixed;
+    S->replace_drop = g_replace_drop;
+    S->check_every = g_check_every;
     S->n = n;
     S->k = nrhs;
     S->nnz = nnz;
@@ -1355,7 +1523,8 @@ extern "C" int tg_batch_add_cg(tg_batch_t h, int32_t n, int64_t nnz, const int32
     BatchSystem& ref = *S;
     Bt.sys[S->id] = std::move(S);
     *id_out = ref.id;
-    if (!ref.mixed) return Bt.b64.add(n, rp, ci, val, diag, nrhs, B, X, tol, max_iter, &ref.slot);
+    if (ref.mixed == 0) return Bt.b64.add(n, nnz, rp, ci, val, diag, nrhs, B, X, tol, max_iter, &ref.slot);
+    if (ref.mixed == 2) return true_start(Bt, ref);
     return mixed_start(Bt, ref);
 }
 
@@ -1398,7 +1567,7 @@ extern "C" int tg_batch_release(tg_batch_t h, int64_t id) {
     BatchSystem& S = *it->second;
     if (!S.finished) return fail(TG_ERR_INVALID, "tg_batch_release: system %lld still running", (long long)id);
     if (S.slot >= 0) {
-        int rc = S.mixed ? Bt.b32.retire(S.slot) : Bt.b64.retire(S.slot);
+        int rc = S.mixed == 1 ? Bt.b32.retire(S.slot) : Bt.b64.retire(S.slot);
         if (rc) return rc;
     }
     if (S.mem) TG_HIP(hipFreeAsync(S.mem, Bt.s));
@@ -1413,8 +1582,12 @@ extern "C" size_t tg_batch_request_bytes(int32_t n, int64_t nnz, int32_t nrhs, i
     const size_t N = (size_t)n * nrhs, tiles = (size_t)ceil_div(n, g_batch_tile_rows);
     const size_t pb = align256(tiles * nrhs * sizeof(double));
     const size_t nvec = 4;
-    if (!mixed) return nvec * align256(N * sizeof(double)) + 3 * pb;
-    return nvec * align256(N * sizeof(float)) + 3 * pb + align256((size_t)nnz * 4) + (has_diag ? align256((size_t)n * 4) : 0) +
+    const size_t ib = align256((size_t)std::max<int64_t>(nnz, 1) * 2) + 256;  // 16-bit column offsets, flag
+    if (mixed == 0) return nvec * align256(N * sizeof(double)) + 3 * pb + ib;
+    if (mixed == 2)  // + the ||r|| partials and norms of the true-residual stop
+        return nvec * align256(N * sizeof(double)) + 3 * pb + ib +
+               align256((size_t)ceil_div(n, kMinTileRows) * nrhs * 8) + align256((size_t)nrhs * 8);
+    return nvec * align256(N * sizeof(float)) + 3 * pb + ib + align256((size_t)nnz * 4) + (has_diag ? align256((size_t)n * 4) : 0) +
            2 * align256(N * 4) + align256((size_t)ceil_div(n, kMinTileRows) * nrhs * 8) + align256((size_t)2 * nrhs * 8);
 }
 

@@ -393,7 +393,7 @@ extern "C" int tg_server_submit_cg(tg_server_t h, int32_t n, int64_t nnz, const 
     r->X = X;
     r->tol = tol;
     r->max_iter = max_iter;
-    r->mixed = mixed ? 1 : 0;
+    r->mixed = mixed == 2 ? 2 : mixed ? 1 : 0;  // the batched engine's modes (tg_batch_add_cg)
     r->bytes = tg_server_request_bytes(kCG, n, nnz, nrhs, inv_diag != nullptr, r->mixed, 0);
     return submit(h, std::move(r), id_out);
 }

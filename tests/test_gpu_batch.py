@@ -46,7 +46,8 @@ def run_batch(sysl, dev, mixed, tol, order=None, staged=False):
     half = len(sysl) // 2
     for j, i in enumerate(idx):
         rp, ci, val, d, B, X = devs[i]
-        ids.append((i, bt.add(rp, ci, val, B, X, tol, 20000, diag=d, mixed=mixed)))
+        mode = mixed[i] if isinstance(mixed, (list, tuple)) else int(mixed)
+        ids.append((i, bt.add(rp, ci, val, B, X, tol, 20000, diag=d, mixed=mode == 1, true_residual=mode == 2)))
         if staged and j == half:
             bt.step(3)  # admit the rest while the first half is running
     bt.run(8)
@@ -56,7 +57,7 @@ def run_batch(sysl, dev, mixed, tol, order=None, staged=False):
     return X, res
 
 
-@pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("mixed", [0, 1, 2], ids=["fp64", "mixed", "fp64_true_residual"])
 def test_batch_matches_oracle(cuda, mixed):
     sysl = systems(10)
     X, res = run_batch(sysl, cuda, mixed, 1e-10)
@@ -65,7 +66,9 @@ def test_batch_matches_oracle(cuda, mixed):
         A = sp.csr_matrix((val, ci, rp), shape=(n, n))
         true = np.linalg.norm(B - A @ X[i], axis=0) / np.linalg.norm(B, axis=0)
         if mixed:
-            # mixed systems stop on the FP64 true residual
+            # mixed systems and FP64 systems with the true-residual stop stop on the
+            # FP64 true residual
+            assert np.all(res[i]["rel_res"] <= 1e-10), (i, res[i])
             assert np.all(true <= 1.5e-10), (i, true)
             Xs = np.linalg.solve(A.toarray(), B) if n <= 3000 else None
             if Xs is not None:
@@ -131,3 +134,53 @@ def test_split_and_fused_direction_forms_are_bitwise_equal(cuda, mixed):
     for i in range(len(sysl)):
         assert rs[i]["iterations"] == rf[i]["iterations"]
         assert np.array_equal(Xs[i], Xf[i])
+
+
+def test_batch_modes_in_one_mix_equal_alone(cuda):
+    """FP64 (reference stop), mixed and FP64-true-residual systems in one batch: each one's
+    result is bitwise what it gives alone in its own mode."""
+    sysl = systems(6, seed=31)
+    modes = [0, 1, 2, 2, 1, 0]
+    Xmix, rmix = run_batch(sysl, cuda, modes, 1e-10)
+    for j in range(len(sysl)):
+        Xa, ra = run_batch([sysl[j]], cuda, modes[j], 1e-10)
+        assert np.array_equal(Xa[0], Xmix[j]), (j, modes[j])
+        assert ra[0]["iterations"] == rmix[j]["iterations"]
+
+
+def wide_system(n=70000, k=2, seed=9):
+    """A tridiagonal SPD system plus a symmetric coupling of the first and the last row:
+    a column offset of n - 1 > 32767, so the engine keeps its int32 column indices."""
+    main = np.full(n, 2.5)
+    off = np.full(n - 1, -1.0)
+    A = sp.diags([off, main, off], [-1, 0, 1], format="lil")
+    A[0, n - 1] = -0.25
+    A[n - 1, 0] = -0.25
+    A = A.tocsr()
+    A.sort_indices()
+    B = gen.uniform(seed, n * k).reshape(n, k)
+    return (A.indptr.astype(np.int64), A.indices.astype(np.int32), A.data, 1.0 / A.diagonal(), B, n, k)
+
+
+@pytest.mark.parametrize("mixed", [0, 1, 2])
+def test_idx16_offsets_are_bitwise_neutral(cuda, mixed):
+    """16-bit column offsets only change how the SpMM reads the pattern: results are bitwise
+    those with int32 indices, and a system whose offsets do not fit falls back to int32."""
+    from tensors_amd import _capi as capi
+
+    sysl = systems(4, seed=55) + [wide_system()]
+    before = capi.lib().tg_tune_get(b"batch_idx16")
+    try:
+        capi.call("tg_tune_set", b"batch_idx16", 1)
+        X16, r16 = run_batch(sysl, cuda, mixed, 1e-10)
+        capi.call("tg_tune_set", b"batch_idx16", 0)
+        X32, r32 = run_batch(sysl, cuda, mixed, 1e-10)
+    finally:
+        capi.call("tg_tune_set", b"batch_idx16", before)
+    for i in range(len(sysl)):
+        assert r16[i]["succeeded"] and r16[i]["iterations"] == r32[i]["iterations"]
+        assert np.array_equal(X16[i], X32[i])
+    rp, ci, val, d, B, n, k = sysl[-1]
+    A = sp.csr_matrix((val, ci, rp), shape=(n, n))
+    Xw = X16[len(sysl) - 1]
+    assert np.max(np.linalg.norm(B - A @ Xw, axis=0) / np.linalg.norm(B, axis=0)) < 1e-8
