@@ -67,6 +67,8 @@ def parse():
                     help="residual replacement when the estimate fell to this percent (default 10)")
     ap.add_argument("--batch-check-every", type=int, default=-1,
                     help="iterations between residual-replacement checkpoints (default 16)")
+    ap.add_argument("--batch-graph", type=int, default=-1,
+                    help="batched CG steps as captured hipGraphs (1 default, 0 eager launches)")
     ap.add_argument("--batch-idx16", type=int, default=-1,
                     help="16-bit column offsets in the batched SpMM (1 default, 0 int32)")
     ap.add_argument("--batch-split", type=int, default=-1,
@@ -282,6 +284,8 @@ def batch_section(args, dev):
         capi.call("tg_tune_set", b"batch_split", args.batch_split)
     if args.batch_idx16 >= 0:
         capi.call("tg_tune_set", b"batch_idx16", args.batch_idx16)
+    if args.batch_graph >= 0:
+        capi.call("tg_tune_set", b"batch_graph", args.batch_graph)
     if args.batch_replace_drop_pct > 0:
         capi.call("tg_tune_set", b"batch_replace_drop_pct", args.batch_replace_drop_pct)
     if args.batch_check_every > 0:

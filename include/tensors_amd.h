@@ -127,6 +127,9 @@ int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, doubl
  *                  per lane (default), 1 = streaming chunks (measured slower).
  *   "batch_split": batched iteration, 1 = direction pass + SpMM of p + update
  *                  (default), 0 = fused direction and SpMM (bitwise the same).
+ *   "batch_graph": 1 = each tg_batch_step runs its iterations as one captured
+ *                  hipGraph (re-captured when the set of active systems changes;
+ *                  default), 0 = eager launches.  Bitwise the same results.
  *   "batch_idx16": 1 = 16-bit column offsets (ci - row) in the batched SpMM for
  *                  systems whose offsets fit (default), 0 = int32 indices.
  *   "batch_replace_drop_pct", "batch_check_every": residual replacement of

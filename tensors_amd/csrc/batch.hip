@@ -44,6 +44,7 @@ static int g_batch_tile_rows = kTileRows;
 static int g_batch_stream = 0;  // SpMM kernel of the engine: 0 = a row per lane (measured faster), 1 = streaming chunks
 static int g_batch_split = 1;   // 1: direction pass + SpMM of p (three launches per iteration); 0: fused K1
 static int g_batch_idx16 = 1;   // 1: 16-bit column offsets in the split SpMM where they fit (read at add)
+static int g_batch_graph = 1;   // 1: the iterations of a step run as one captured hipGraph
 constexpr int kGather = 8;     // nonzeros per gather chunk in the SpMM kernel
 constexpr int kRowUnroll = 4;  // rows per step in the update kernel
 
@@ -774,19 +775,47 @@ template <typename T> class CGBatch {
             if (rc) return rc;
         }
         if (ntiles_ == 0) return TG_OK;
+        if (!g_batch_graph) {
+            launch_iters(s_, iters);
+            TG_LAUNCH_CHECK();
+            return TG_OK;
+        }
+        // The iterations as one hipGraph: captured on a private stream (capture is
+        // not allowed on the legacy default stream a caller may pass), launched on
+        // s_, captured again when the tile list, the buffers or the shape change.
+        const GraphKey key{dtiles_, dsys_, dst_, ntiles_, iters, g_batch_split, g_batch_stream};
+        if (!gexec_ || !(key == gkey_)) {
+            if (gexec_) (void)hipGraphExecDestroy(gexec_);
+            gexec_ = nullptr;
+            if (!cap_s_) TG_HIP(hipStreamCreateWithFlags(&cap_s_, hipStreamNonBlocking));
+            TG_HIP(hipStreamBeginCapture(cap_s_, hipStreamCaptureModeThreadLocal));
+            launch_iters(cap_s_, iters);
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(cap_s_, &g);
+            if (ec != hipSuccess) return fail(TG_ERR_HIP, "batch: graph capture failed: %s", hipGetErrorString(ec));
+            const hipError_t ei = hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) {
+                gexec_ = nullptr;
+                return fail(TG_ERR_HIP, "batch: graph instantiation failed: %s", hipGetErrorString(ei));
+            }
+            gkey_ = key;
+        }
+        TG_HIP(hipGraphLaunch(gexec_, s_));
+        return TG_OK;
+    }
+
+    void launch_iters(hipStream_t s, int iters) {
         for (int it = 0; it < iters; ++it) {
             if (g_batch_split) {
-                hipLaunchKernelGGL((bk_dir_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s_, dtiles_, dsys_, dst_);
-                hipLaunchKernelGGL((bk_spmm_dot_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s_, dtiles_, dsys_,
-                                   dst_);
+                hipLaunchKernelGGL((bk_dir_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s, dtiles_, dsys_, dst_);
+                hipLaunchKernelGGL((bk_spmm_dot_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s, dtiles_, dsys_, dst_);
             } else {
-                hipLaunchKernelGGL((bk_dir_spmm_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s_, dtiles_, dsys_,
-                                   dst_, g_batch_stream);
+                hipLaunchKernelGGL((bk_dir_spmm_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s, dtiles_, dsys_, dst_,
+                                   g_batch_stream);
             }
-            hipLaunchKernelGGL((bk_update_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s_, dtiles_, dsys_, dst_);
+            hipLaunchKernelGGL((bk_update_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s, dtiles_, dsys_, dst_);
         }
-        TG_LAUNCH_CHECK();
-        return TG_OK;
     }
 
     // Reads the states of all used slots (synchronises).
@@ -907,7 +936,25 @@ template <typename T> class CGBatch {
         if (dsetup_slot_) (void)hipFreeAsync(dsetup_slot_, s_);
         if (dresume_) (void)hipFreeAsync(dresume_, s_);
         (void)hipStreamSynchronize(s_);
+        if (gexec_) (void)hipGraphExecDestroy(gexec_);
+        gexec_ = nullptr;
+        if (cap_s_) (void)hipStreamDestroy(cap_s_);
+        cap_s_ = nullptr;
     }
+
+    struct GraphKey {
+        const void* tiles;
+        const void* sys;
+        const void* st;
+        int ntiles, iters, split, stream;
+        bool operator==(const GraphKey& o) const {
+            return tiles == o.tiles && sys == o.sys && st == o.st && ntiles == o.ntiles && iters == o.iters &&
+                   split == o.split && stream == o.stream;
+        }
+    };
+    hipStream_t cap_s_ = nullptr;
+    hipGraphExec_t gexec_ = nullptr;
+    GraphKey gkey_{};
 
     hipStream_t s_;
     size_t cap_ = 0;
@@ -1438,6 +1485,12 @@ int batch_tune_set(const char* key, int32_t value, bool* known) {
         g_check_every = value;
         return TG_OK;
     }
+    if (std::string(key) == "batch_graph") {
+        *known = true;
+        if (value != 0 && value != 1) return fail(TG_ERR_INVALID, "tg_tune_set: batch_graph must be 0 or 1");
+        g_batch_graph = value;
+        return TG_OK;
+    }
     if (std::string(key) == "batch_idx16") {
         *known = true;
         if (value != 0 && value != 1) return fail(TG_ERR_INVALID, "tg_tune_set: batch_idx16 must be 0 or 1");
@@ -1469,6 +1522,7 @@ int32_t batch_tune_get(const char* key) {
     if (std::string(key) == "batch_check_every") return g_check_every;
     if (std::string(key) == "batch_split") return g_batch_split;
     if (std::string(key) == "batch_idx16") return g_batch_idx16;
+    if (std::string(key) == "batch_graph") return g_batch_graph;
     if (std::string(key) == "batch_stream") return g_batch_stream;
     return std::string(key) == "batch_tile_rows" ? g_batch_tile_rows : -1;
 }

@@ -184,3 +184,24 @@ def test_idx16_offsets_are_bitwise_neutral(cuda, mixed):
     A = sp.csr_matrix((val, ci, rp), shape=(n, n))
     Xw = X16[len(sysl) - 1]
     assert np.max(np.linalg.norm(B - A @ Xw, axis=0) / np.linalg.norm(B, axis=0)) < 1e-8
+
+
+@pytest.mark.parametrize("mixed", [0, 1, 2])
+def test_graph_replayed_steps_equal_eager_steps(cuda, mixed):
+    """Steps run as captured hipGraphs (replayed every step, with the host's polls, parks and
+    residual replacements between the replays, and captured again whenever the tile list
+    changes) give bitwise the results of eager launches."""
+    from tensors_amd import _capi as capi
+
+    sysl = systems(6, seed=202)
+    before = capi.lib().tg_tune_get(b"batch_graph")
+    try:
+        capi.call("tg_tune_set", b"batch_graph", 1)
+        Xg, rg = run_batch(sysl, cuda, mixed, 1e-10, staged=True)
+        capi.call("tg_tune_set", b"batch_graph", 0)
+        Xe, re_ = run_batch(sysl, cuda, mixed, 1e-10, staged=True)
+    finally:
+        capi.call("tg_tune_set", b"batch_graph", before)
+    for i in range(len(sysl)):
+        assert rg[i]["iterations"] == re_[i]["iterations"] and rg[i]["outer"] == re_[i]["outer"]
+        assert np.array_equal(Xg[i], Xe[i])
