@@ -215,9 +215,10 @@ def cg_section(args, dev):
         log(f"CG mixed solve {latm[-1]:.3f}s")
     resm = true_res(Xm)
     err = ((Xm - X64).abs().max() / X64.abs().max()).item()
-    # algorithmic bytes of one FP64 CG iteration (P = I): SpMM + 11 vector passes
+    # bytes of one FP64 CG iteration as executed (P = I): SpMM (p read once, u written) +
+    # 8 vector passes (r update: r, u read, r written; p step: r, p, x read, p, x written)
     spmm_b = nnz * 12 + 4 * (n + 1) + 2 * n * k * 8
-    it_bytes = spmm_b + 11 * n * k * 8
+    it_bytes = spmm_b + 8 * n * k * 8
     t64 = float(np.median(lat64))
     tm = float(np.median(latm))
     return {
@@ -358,8 +359,8 @@ def batch_section(args, dev):
         tr = np.array([true_res(systems[i], Xs[i]) for sid, i in ids.items()])
         iters = np.array([res[sid]["iterations"] for sid in ids])
         # bytes of the executed schedule per iteration: nnz*(s_v + 4) + 4(n+1) + 2*n*s_v (Jacobi
-        # diagonal, read by the direction and the update pass) + 11*n*k*s (direction: r, p_old
-        # read, p written; SpMM: p read once, u written; update: p, u, x, r read, x, r written);
+        # diagonal, read by the direction and the update pass) + 10*n*k*s (direction: r, p_old,
+        # x read, p, x written; SpMM: p read once, u written; update: u, r read, r written);
         # a residual replacement adds an FP64 SpMV over x (x64 + x32 for mixed) plus b read and
         # r written (and the x fold for mixed)
         sv = 4 if mixed else 8
@@ -367,7 +368,7 @@ def batch_section(args, dev):
         for sid, i in ids.items():
             s_ = systems[i]
             n_, k_, z_ = s_["n"], s_["k"], s_["nnz"]
-            alg += res[sid]["iterations"] * (z_ * (sv + 4) + 4 * (n_ + 1) + 2 * n_ * sv + 11 * n_ * k_ * sv)
+            alg += res[sid]["iterations"] * (z_ * (sv + 4) + 4 * (n_ + 1) + 2 * n_ * sv + 10 * n_ * k_ * sv)
             if mixed:
                 alg += res[sid]["outer"] * (z_ * 12 + 4 * (n_ + 1) + n_ * k_ * (8 + 12 + 8 + 4 + 12))
             elif mode == 2:

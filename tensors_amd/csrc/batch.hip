@@ -61,6 +61,7 @@ template <typename T> struct SysDev {
     T* p[2];         // direction, by iteration parity (z = d * r is never stored)
     double* ppu;     // tiles * k partials of p.u
     double* prz[2];  // tiles * k partials of r.z, by parity
+    int* xit;        // per tile: the iteration whose deferred x update the tile has applied
     int n, k, kpad, tiles, tile_rows;
     int check_every;  // > 0: pause at every multiple of this many iterations for the host
 };
@@ -137,6 +138,7 @@ __global__ __launch_bounds__(kBlock) void bk_init_kernel(const Tile* __restrict_
             S.x[e] = T(0);
             acc += (double)rv * (double)zv;
         }
+    if (threadIdx.x == 0) S.xit[tl.t] = 0;
     tile_partial(acc, S.k, S.kpad, S.prz[0], tl.t);
 }
 
@@ -423,16 +425,19 @@ __global__ __launch_bounds__(kBlock) void bk_dir_spmm_kernel(const Tile* __restr
 // for one more launch per iteration.  The arithmetic of p, u and the partials is
 // the same as the fused form's (p = fma(beta, p_prev, z) per entry; the row sums
 // in nonzero order), so both forms give bitwise the same iterates.
-template <typename T, bool HAS_D, bool FIRST>
-__device__ __forceinline__ void dir_rows(const SysDev<T>& S, const Tile& tl, int c, int ro, int rows, T be,
+// XUPD: also x += alpha p_old, the x update of the previous iteration that the
+// update kernel leaves to this pass (which reads p_old anyway).
+template <typename T, bool HAS_D, bool FIRST, bool XUPD>
+__device__ __forceinline__ void dir_rows(const SysDev<T>& S, const Tile& tl, int c, int ro, int rows, T be, T al,
                                          const T* __restrict__ pold_, T* __restrict__ pnew_) {
     const gptr<const T> pold = (gptr<const T>)pold_;
     const gptr<T> pnew = (gptr<T>)pnew_;
     const gptr<const T> r = (gptr<const T>)S.r;
     const gptr<const T> d = (gptr<const T>)S.d;
+    const gptr<T> x = (gptr<T>)S.x;
     const int k = S.k;
     for (int i0 = tl.r0 + ro; i0 < tl.r1; i0 += kRowUnroll * rows) {
-        T rv[kRowUnroll], pv[kRowUnroll], dv[kRowUnroll];
+        T rv[kRowUnroll], pv[kRowUnroll], dv[kRowUnroll], xv[kRowUnroll];
 #pragma unroll
         for (int q = 0; q < kRowUnroll; ++q) {
             const int i = min(i0 + q * rows, tl.r1 - 1);  // clamped: always a valid row
@@ -440,11 +445,13 @@ __device__ __forceinline__ void dir_rows(const SysDev<T>& S, const Tile& tl, int
             rv[q] = r[e];
             if constexpr (!FIRST) pv[q] = pold[e];
             if constexpr (HAS_D) dv[q] = d[i];
+            if constexpr (XUPD) xv[q] = x[e];
         }
 #pragma unroll
         for (int q = 0; q < kRowUnroll; ++q) {
             const int i = i0 + q * rows;
             if (i < tl.r1) {
+                const int64_t e = (int64_t)i * k + c;
                 T z = rv[q];
                 if constexpr (HAS_D) z = dv[q] * z;
                 T pi;
@@ -452,9 +459,23 @@ __device__ __forceinline__ void dir_rows(const SysDev<T>& S, const Tile& tl, int
                     pi = z;
                 else
                     pi = fma_t(be, pv[q], z);
-                pnew[(int64_t)i * k + c] = pi;
+                pnew[e] = pi;
+                if constexpr (XUPD) x[e] = fma_t(al, pv[q], xv[q]);
             }
         }
+    }
+}
+
+// x += alpha p_old alone: the deferred x update of a system that stops or pauses here.
+template <typename T>
+__device__ __forceinline__ void x_rows(const SysDev<T>& S, const Tile& tl, int c, int ro, int rows, T al,
+                                       const T* __restrict__ pold_) {
+    const gptr<const T> pold = (gptr<const T>)pold_;
+    const gptr<T> x = (gptr<T>)S.x;
+    const int k = S.k;
+    for (int i = tl.r0 + ro; i < tl.r1; i += rows) {
+        const int64_t e = (int64_t)i * k + c;
+        x[e] = fma_t(al, pold[e], x[e]);
     }
 }
 
@@ -502,7 +523,11 @@ __device__ __forceinline__ double spmm_dot_rows(const SysDev<T>& S, const Tile& 
     return acc;
 }
 
-// Kdir: beta, the stop test and the checkpoint pause as in K1; then p for the own rows.
+// Kdir: beta, the stop test and the checkpoint pause as in K1; then p for the own
+// rows, together with the previous iteration's x += alpha p (deferred from the
+// update kernel).  A tile applies that x update once per iteration (xit), also
+// when the system stops or pauses here; before a checkpoint the update kernel has
+// done it already.  So x is whole whenever the host reads it.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void bk_dir_kernel(const Tile* __restrict__ tiles, const SysDev<T>* __restrict__ sys,
                                                         SysState* __restrict__ st) {
@@ -513,13 +538,21 @@ __global__ __launch_bounds__(kBlock) void bk_dir_kernel(const Tile* __restrict__
     const int c = threadIdx.x % S.kpad, ro = threadIdx.x / S.kpad, rows = kBlock / S.kpad;
     const double rz = sys_total(pick(S.prz, it), S.tiles, S.k, S.kpad, c, ro, rows);
     const double rz_prev = it > 0 ? sys_total(pick(S.prz, it + 1), S.tiles, S.k, S.kpad, c, ro, rows) : 1.0;
-    __shared__ int conv;
-    if (threadIdx.x == 0) conv = 1;
+    __shared__ int conv, xdo;
+    if (threadIdx.x == 0) {
+        conv = 1;
+        xdo = it > 0 && S.xit[tl.t] != it;
+    }
     __syncthreads();
+    if (threadIdx.x == 0 && xdo) S.xit[tl.t] = it;
+    // alpha of iteration it - 1 from the same partials and sums the update kernel used
+    const double pu = xdo ? sys_total(S.ppu, S.tiles, S.k, S.kpad, c, ro, rows) : 1.0;
+    const T al = (T)(rz_prev / pu);
     if (ro == 0 && c < S.k && !(fabs(rz) <= ss.TOL[c])) conv = 0;
     __syncthreads();
     if (tl.t == 0 && ro == 0 && c < S.k) ss.rho[c] = rz;
     const bool pause = S.check_every > 0 && it > 0 && it % S.check_every == 0 && it != ss.resumed;
+    const T* pold = pick(S.p, it + 1);
     if (conv || it >= ss.max_iter || pause) {
         if (tl.t == 0 && threadIdx.x == 0) {
             ss.iter = it;
@@ -527,23 +560,27 @@ __global__ __launch_bounds__(kBlock) void bk_dir_kernel(const Tile* __restrict__
             ss.paused = (!conv && it < ss.max_iter) ? 1 : 0;
             ss.done = 1;
         }
+        if (xdo && c < S.k) x_rows<T>(S, tl, c, ro, rows, al, pold);
         return;
     }
     if (tl.t == 0 && threadIdx.x == 0) ss.it_a = it;
     if (c < S.k) {
         const T be = (T)(rz / rz_prev);
-        const T* pold = pick(S.p, it + 1);
         T* pnew = pick(S.p, it);
         if (S.d) {
             if (it == 0)
-                dir_rows<T, true, true>(S, tl, c, ro, rows, be, pold, pnew);
+                dir_rows<T, true, true, false>(S, tl, c, ro, rows, be, al, pold, pnew);
+            else if (xdo)
+                dir_rows<T, true, false, true>(S, tl, c, ro, rows, be, al, pold, pnew);
             else
-                dir_rows<T, true, false>(S, tl, c, ro, rows, be, pold, pnew);
+                dir_rows<T, true, false, false>(S, tl, c, ro, rows, be, al, pold, pnew);
         } else {
             if (it == 0)
-                dir_rows<T, false, true>(S, tl, c, ro, rows, be, pold, pnew);
+                dir_rows<T, false, true, false>(S, tl, c, ro, rows, be, al, pold, pnew);
+            else if (xdo)
+                dir_rows<T, false, false, true>(S, tl, c, ro, rows, be, al, pold, pnew);
             else
-                dir_rows<T, false, false>(S, tl, c, ro, rows, be, pold, pnew);
+                dir_rows<T, false, false, false>(S, tl, c, ro, rows, be, al, pold, pnew);
         }
     }
 }
@@ -567,7 +604,8 @@ __global__ __launch_bounds__(kBlock) void bk_spmm_dot_kernel(const Tile* __restr
 }
 
 // ---- K2 body: kRowUnroll rows per step, all loads first, then the stores ----------
-template <typename T, bool HAS_D>
+// DEFER_X: x is left to the next direction pass (bk_dir_kernel): r and u only.
+template <typename T, bool HAS_D, bool DEFER_X>
 __device__ __forceinline__ double k2_rows(const SysDev<T>& S, const Tile& tl, int c, int ro, int rows, T al,
                                           const T* __restrict__ p_) {
     const gptr<const T> p = (gptr<const T>)p_;
@@ -583,8 +621,10 @@ __device__ __forceinline__ double k2_rows(const SysDev<T>& S, const Tile& tl, in
         for (int q = 0; q < kRowUnroll; ++q) {
             const int i = min(i0 + q * rows, tl.r1 - 1);  // clamped: always a valid row
             const int64_t e = (int64_t)i * k + c;
-            pv[q] = p[e];
-            xv[q] = x[e];
+            if constexpr (!DEFER_X) {
+                pv[q] = p[e];
+                xv[q] = x[e];
+            }
             rv[q] = r[e];
             uv[q] = u[e];
             if constexpr (HAS_D) dv[q] = d[i];
@@ -594,7 +634,7 @@ __device__ __forceinline__ double k2_rows(const SysDev<T>& S, const Tile& tl, in
             const int i = i0 + q * rows;
             if (i < tl.r1) {
                 const int64_t e = (int64_t)i * k + c;
-                x[e] = xv[q] + al * pv[q];
+                if constexpr (!DEFER_X) x[e] = fma_t(al, pv[q], xv[q]);
                 const T rn = rv[q] - al * uv[q];
                 r[e] = rn;
                 T zv = rn;
@@ -607,7 +647,7 @@ __device__ __forceinline__ double k2_rows(const SysDev<T>& S, const Tile& tl, in
 }
 
 // ---- K2: x, r, z update + partial r.z (next parity) -------------------------------
-template <typename T>
+template <typename T, bool DEFER_X>
 __global__ __launch_bounds__(kBlock) void bk_update_kernel(const Tile* __restrict__ tiles, const SysDev<T>* __restrict__ sys,
                                                            SysState* __restrict__ st) {
     const Tile tl = tiles[xcd_swizzle(blockIdx.x, gridDim.x)];
@@ -618,12 +658,20 @@ __global__ __launch_bounds__(kBlock) void bk_update_kernel(const Tile* __restric
     const int c = threadIdx.x % S.kpad, ro = threadIdx.x / S.kpad, rows = kBlock / S.kpad;
     const double rz = sys_total(pick(S.prz, it), S.tiles, S.k, S.kpad, c, ro, rows);
     const double pu = sys_total(S.ppu, S.tiles, S.k, S.kpad, c, ro, rows);
+    // Before a checkpoint the x update is done here: the host may read x there
+    // without the direction pass having run (a poll that lands on the checkpoint).
+    const bool xnow = !DEFER_X || (S.check_every > 0 && (it + 1) % S.check_every == 0);
     double acc = 0.0;
     if (c < S.k) {
         const T al = (T)(rz / pu);
-        acc = S.d ? k2_rows<T, true>(S, tl, c, ro, rows, al, pick(S.p, it))
-                  : k2_rows<T, false>(S, tl, c, ro, rows, al, pick(S.p, it));
+        if (xnow)
+            acc = S.d ? k2_rows<T, true, false>(S, tl, c, ro, rows, al, pick(S.p, it))
+                      : k2_rows<T, false, false>(S, tl, c, ro, rows, al, pick(S.p, it));
+        else
+            acc = S.d ? k2_rows<T, true, true>(S, tl, c, ro, rows, al, pick(S.p, it))
+                      : k2_rows<T, false, true>(S, tl, c, ro, rows, al, pick(S.p, it));
     }
+    if (DEFER_X && xnow && threadIdx.x == 0) S.xit[tl.t] = it + 1;  // the direction pass of it + 1 skips it
     tile_partial(acc, S.k, S.kpad, pick(S.prz, it + 1), tl.t);
     if (tl.t == 0 && threadIdx.x == 0) ss.it_b = it + 1;
 }
@@ -693,11 +741,13 @@ template <typename T> class CGBatch {
         const size_t nvec = 4;  // r, u, p0, p1
         const size_t vb = align256(N * sizeof(T)), pb = align256((size_t)S.tiles * k * sizeof(double));
         const size_t ib = align256((size_t)std::max<int64_t>(nnz, 1) * sizeof(int16_t));
-        const size_t bytes = nvec * vb + 3 * pb + ib + 256;
+        const size_t xb = align256((size_t)S.tiles * sizeof(int));
+        const size_t bytes = nvec * vb + 3 * pb + ib + 256 + xb;
         char* mem = nullptr;
         TG_HIP(hipMallocAsync((void**)&mem, bytes, s_));
         int16_t* cid = (int16_t*)(mem + nvec * vb + 3 * pb);
         int* dok = (int*)(mem + nvec * vb + 3 * pb + ib);
+        S.xit = (int*)(mem + nvec * vb + 3 * pb + ib + 256);
         S.r = (T*)mem;
         S.u = (T*)(mem + vb);
         S.p[0] = (T*)(mem + 2 * vb);
@@ -814,7 +864,10 @@ template <typename T> class CGBatch {
                 hipLaunchKernelGGL((bk_dir_spmm_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s, dtiles_, dsys_, dst_,
                                    g_batch_stream);
             }
-            hipLaunchKernelGGL((bk_update_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s, dtiles_, dsys_, dst_);
+            if (g_batch_split)
+                hipLaunchKernelGGL((bk_update_kernel<T, true>), dim3(ntiles_), dim3(kBlock), 0, s, dtiles_, dsys_, dst_);
+            else
+                hipLaunchKernelGGL((bk_update_kernel<T, false>), dim3(ntiles_), dim3(kBlock), 0, s, dtiles_, dsys_, dst_);
         }
     }
 
@@ -1636,7 +1689,8 @@ extern "C" size_t tg_batch_request_bytes(int32_t n, int64_t nnz, int32_t nrhs, i
     const size_t N = (size_t)n * nrhs, tiles = (size_t)ceil_div(n, g_batch_tile_rows);
     const size_t pb = align256(tiles * nrhs * sizeof(double));
     const size_t nvec = 4;
-    const size_t ib = align256((size_t)std::max<int64_t>(nnz, 1) * 2) + 256;  // 16-bit column offsets, flag
+    const size_t ib = align256((size_t)std::max<int64_t>(nnz, 1) * 2) + 256 +  // 16-bit column offsets, flag,
+                      align256(tiles * sizeof(int));                              // per-tile x-update marks
     if (mixed == 0) return nvec * align256(N * sizeof(double)) + 3 * pb + ib;
     if (mixed == 2)  // + the ||r|| partials and norms of the true-residual stop
         return nvec * align256(N * sizeof(double)) + 3 * pb + ib +

@@ -149,8 +149,24 @@ __global__ __launch_bounds__(kFinalThreads) void cg_alpha_kernel(int k, int kpad
     if (j == 0 && c < k) st->alpha[c] = st->rho[c] / pu;
 }
 
-// x = (first ? 0 : x) + alpha p;  r -= alpha u;  z = P r;  partial(r.z)   (:262-313)
 template <typename T>
+__device__ __forceinline__ T fma_t(T a, T b, T c) {
+    if constexpr (sizeof(T) == 8)
+        return __builtin_fma(a, b, c);
+    else
+        return __builtin_fmaf(a, b, c);
+}
+
+// x += alpha p with the first iteration's x = alpha p (x starts at zero, :262-266)
+template <typename T>
+__device__ __forceinline__ T x_step(bool first, T x, T al, T pv) {
+    return first ? al * pv : fma_t(al, pv, x);
+}
+
+// x = (first ? 0 : x) + alpha p;  r -= alpha u;  z = P r;  partial(r.z)   (:262-313)
+// DEFER_X: the x update is left to the p step (cg_p_kernel), which reads p anyway:
+// three passes over n x nrhs here instead of six.
+template <typename T, bool DEFER_X>
 __global__ __launch_bounds__(kBlock) void cg_update_kernel(int n, int k, int kpad, const T* __restrict__ p,
                                                            const T* __restrict__ u, const T* __restrict__ d,
                                                            T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
@@ -166,8 +182,7 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(int n, int k, int kpa
         const T al = (T)st->alpha[L.c];
         for (int i = i0 + L.ro; i < i1; i += L.rows) {
             const int64_t e = (int64_t)i * k + L.c;
-            const T pv = p[e];
-            x[e] = first ? al * pv : x[e] + al * pv;
+            if constexpr (!DEFER_X) x[e] = x_step(first, x[e], al, p[e]);
             const T rv = r[e] - al * u[e];
             r[e] = rv;
             T zv = rv;
@@ -202,25 +217,52 @@ __global__ __launch_bounds__(kFinalThreads) void cg_rho_kernel(int k, int kpad, 
         st->succeeded = conv;
         st->iter = st->iter + 1;
         st->done = (conv || st->iter >= st->max_iter) ? 1 : 0;
+        st->xpend = st->done;  // the p step (and its x update) will not run
     }
 }
 
-// p = z + beta p   (:321-335)
-template <typename T>
+// p = z + beta p   (:321-335); DEFER_X: first x += alpha p with the old p (the
+// update the x,r step left out; rho has advanced iter, so iteration 0 is iter 1)
+template <typename T, bool DEFER_X>
 __global__ __launch_bounds__(kBlock) void cg_p_kernel(int n, int k, int kpad, const T* __restrict__ z,
-                                                      T* __restrict__ p, const CGState* __restrict__ st) {
+                                                      T* __restrict__ p, T* __restrict__ x,
+                                                      const CGState* __restrict__ st) {
     if (st->done) return;
     int i0, i1;
     block_row_range(n, i0, i1);
     const Lanes L = lanes_of(kpad);
+    const bool first = (st->iter == 1) && !st->use_guess;
     if (L.c < k) {
         const T be = (T)st->beta[L.c];
+        const T al = (T)st->alpha[L.c];
         for (int i = i0 + L.ro; i < i1; i += L.rows) {
             const int64_t e = (int64_t)i * k + L.c;
-            p[e] = z[e] + be * p[e];
+            const T pv = p[e];
+            if constexpr (DEFER_X) x[e] = x_step(first, x[e], al, pv);
+            p[e] = z[e] + be * pv;
         }
     }
 }
+
+// The x update of an iteration after which the solve stopped (the p step did not run).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void cg_xfinish_kernel(int n, int k, int kpad, const T* __restrict__ p,
+                                                            T* __restrict__ x, const CGState* __restrict__ st) {
+    if (!st->xpend) return;
+    int i0, i1;
+    block_row_range(n, i0, i1);
+    const Lanes L = lanes_of(kpad);
+    const bool first = (st->iter == 1) && !st->use_guess;
+    if (L.c < k) {
+        const T al = (T)st->alpha[L.c];
+        for (int i = i0 + L.ro; i < i1; i += L.rows) {
+            const int64_t e = (int64_t)i * k + L.c;
+            x[e] = x_step(first, x[e], al, p[e]);
+        }
+    }
+}
+
+__global__ void cg_xclear_kernel(CGState* __restrict__ st) { st->xpend = 0; }
 
 // --- fused iteration forms (CSR operator, P = I or Jacobi) ------------------
 // Form 1 keeps the four-kernel order but computes partial(p.u) in the SpMM
@@ -236,14 +278,6 @@ __global__ __launch_bounds__(kBlock) void cg_p_kernel(int n, int k, int kpad, co
 // name, which other workgroups write in the same launch; it recomputes them from
 // z and p_prev with the same fused multiply-add, so the values are bitwise those
 // the owners store and no workgroup reads what another writes.
-
-template <typename T>
-__device__ __forceinline__ T fma_t(T a, T b, T c) {
-    if constexpr (sizeof(T) == 8)
-        return __builtin_fma(a, b, c);
-    else
-        return __builtin_fmaf(a, b, c);
-}
 
 // One row per lane group of G lanes; each lane holds V consecutive columns of a
 // G*V-column tile (k > G*V only for G = 64: further tiles).  The workgroup's p.u
@@ -395,8 +429,7 @@ __global__ __launch_bounds__(kBlock) void cg_step_kernel(int n, int k, int kpad,
         const T al = (T)st->alpha[L.c];
         for (int i = i0 + L.ro; i < i1; i += L.rows) {
             const int64_t e = (int64_t)i * k + L.c;
-            const T pv = p[e];
-            x[e] = first ? al * pv : x[e] + al * pv;
+            x[e] = x_step(first, x[e], al, p[e]);
             const T rv = r[e] - al * u[e];
             r[e] = rv;
             const T zv = HAS_D ? d[i] * rv : rv;
@@ -697,17 +730,22 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
                 hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kFinalThreads), 0, s, k, kpad, B,
                                    (const double*)partial, st);
             }
-            hipLaunchKernelGGL((cg_update_kernel<T>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, (const T*)p,
+            hipLaunchKernelGGL((cg_update_kernel<T, true>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, (const T*)p,
                                (const T*)u, dd, x, r, z, zmode, partial, (const CGState*)st);
             if (zmode == 2) {
                 rc = apply_P();
                 if (rc) break;
             }
             hipLaunchKernelGGL(cg_rho_kernel, dim3(1), dim3(kFinalThreads), 0, s, k, kpad, B, (const double*)partial, st);
-            hipLaunchKernelGGL((cg_p_kernel<T>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, (const T*)z, p,
+            hipLaunchKernelGGL((cg_p_kernel<T, true>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, (const T*)z, p, x,
                                (const CGState*)st);
         }
         if (rc) break;
+        if (!fused) {  // x complete before the host looks at the state
+            hipLaunchKernelGGL((cg_xfinish_kernel<T>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, (const T*)p, x,
+                               (const CGState*)st);
+            hipLaunchKernelGGL(cg_xclear_kernel, dim3(1), dim3(1), 0, s, st);
+        }
         if (hipGetLastError() != hipSuccess) {
             rc = fail(TG_ERR_HIP, "cg: kernel launch failed");
             break;

@@ -78,7 +78,8 @@ struct CGState {
     int succeeded;
     int max_iter;
     int use_guess;
-    int pad[11];
+    int xpend;  // the last iteration's x += alpha p is still to be applied (it stopped before the p step)
+    int pad[10];
     double rho[kMaxCols];
     double rho_old[kMaxCols];
     double alpha[kMaxCols];
