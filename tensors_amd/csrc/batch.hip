@@ -45,6 +45,9 @@ static int g_batch_stream = 0;  // SpMM kernel of the engine: 0 = a row per lane
 static int g_batch_split = 1;   // 1: direction pass + SpMM of p (three launches per iteration); 0: fused K1
 static int g_batch_idx16 = 1;   // 1: 16-bit column offsets in the split SpMM where they fit (read at add)
 static int g_batch_graph = 1;   // 1: the iterations of a step run as one captured hipGraph
+// SpMM kernel of the split iteration: 0 = row loop, 1 = software-pipelined row loop,
+// 2 = indices and values of a 256-row segment staged through LDS with coalesced loads
+static int g_batch_spmm = 2;
 constexpr int kGather = 8;     // nonzeros per gather chunk in the SpMM kernel
 constexpr int kRowUnroll = 4;  // rows per step in the update kernel
 
@@ -482,7 +485,7 @@ __device__ __forceinline__ void x_rows(const SysDev<T>& S, const Tile& tl, int c
 // IDX16: column indices from the 16-bit offsets (2 bytes per nonzero instead of 4).
 template <typename T, bool IDX16>
 __device__ __forceinline__ double spmm_dot_rows(const SysDev<T>& S, const Tile& tl, int c, int ro, int rows,
-                                                const T* __restrict__ p_) {
+                                                const T* __restrict__ p_, double acc = 0.0) {
     const gptr<const T> p = (gptr<const T>)p_;
     const gptr<const int32_t> rp = (gptr<const int32_t>)S.rp;
     const gptr<const int32_t> ci = (gptr<const int32_t>)S.ci;
@@ -490,7 +493,6 @@ __device__ __forceinline__ double spmm_dot_rows(const SysDev<T>& S, const Tile& 
     const gptr<const T> val = (gptr<const T>)S.val;
     const gptr<T> u = (gptr<T>)S.u;
     const int k = S.k;
-    double acc = 0.0;
     for (int i = tl.r0 + ro; i < tl.r1; i += rows) {
         const int lb = rp[i], le = rp[i + 1];
         const int64_t e = (int64_t)i * k + c;
@@ -508,6 +510,144 @@ __device__ __forceinline__ double spmm_dot_rows(const SysDev<T>& S, const Tile& 
                 else
                     cq = ci[lq];
                 const T vq = val[lq];
+                const bool ok = l0 + q < le;
+                jj[q] = ok ? cq : i;
+                vv[q] = ok ? vq : T(0);
+            }
+#pragma unroll
+            for (int q = 0; q < kGather; ++q) pj[q] = p[(int64_t)jj[q] * k + c];
+#pragma unroll
+            for (int q = 0; q < kGather; ++q) sum += (l0 + q < le) ? vv[q] * pj[q] : T(0);
+        }
+        u[e] = sum;
+        acc += (double)pi * (double)sum;
+    }
+    return acc;
+}
+
+// The software-pipelined form of spmm_dot_rows.  The row pointers of a segment of
+// the tile sit in LDS (rps, staged by the caller), and a lane issues the column
+// indices and values of its next row before the gathers of the current one, so a
+// row costs about one memory latency instead of three (row pointers, then
+// indices, then gathers).  Same operations in the same order as spmm_dot_rows:
+// bitwise the same u and partials.
+template <typename T, bool IDX16>
+__device__ __forceinline__ void row_chunk_indices(const SysDev<T>& S, int i, int lb, int le, int32_t (&jj)[kGather],
+                                                  T (&vv)[kGather]) {
+    const gptr<const int32_t> ci = (gptr<const int32_t>)S.ci;
+    const gptr<const int16_t> cid = (gptr<const int16_t>)S.cid;
+    const gptr<const T> val = (gptr<const T>)S.val;
+    if (le > lb) {
+#pragma unroll
+        for (int q = 0; q < kGather; ++q) {
+            const int lq = min(lb + q, le - 1);
+            int32_t cq;
+            if constexpr (IDX16)
+                cq = i + (int32_t)cid[lq];
+            else
+                cq = ci[lq];
+            const T vq = val[lq];
+            const bool ok = lb + q < le;
+            jj[q] = ok ? cq : i;
+            vv[q] = ok ? vq : T(0);
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < kGather; ++q) {
+            jj[q] = i;
+            vv[q] = T(0);
+        }
+    }
+}
+
+template <typename T, bool IDX16>
+__device__ __forceinline__ double spmm_dot_rows_pipe(const SysDev<T>& S, int s0, int s1, const int* rps, int c,
+                                                     int ro, int rows, const T* __restrict__ p_, double acc) {
+    const gptr<const T> p = (gptr<const T>)p_;
+    const gptr<const int32_t> ci = (gptr<const int32_t>)S.ci;
+    const gptr<const int16_t> cid = (gptr<const int16_t>)S.cid;
+    const gptr<const T> val = (gptr<const T>)S.val;
+    const gptr<T> u = (gptr<T>)S.u;
+    const int k = S.k;
+    int i = s0 + ro;
+    if (i >= s1) return acc;
+    int32_t jA[kGather];
+    T vA[kGather];
+    row_chunk_indices<T, IDX16>(S, i, rps[i - s0], rps[i - s0 + 1], jA, vA);
+    for (; i < s1; i += rows) {
+        const int in = i + rows;
+        int32_t jB[kGather];
+        T vB[kGather];
+        if (in < s1) row_chunk_indices<T, IDX16>(S, in, rps[in - s0], rps[in - s0 + 1], jB, vB);
+        const int lb = rps[i - s0], le = rps[i - s0 + 1];
+        const int64_t e = (int64_t)i * k + c;
+        const T pi = p[e];
+        T pj[kGather];
+#pragma unroll
+        for (int q = 0; q < kGather; ++q) pj[q] = p[(int64_t)jA[q] * k + c];
+        T sum = T(0);
+#pragma unroll
+        for (int q = 0; q < kGather; ++q) sum += (lb + q < le) ? vA[q] * pj[q] : T(0);
+        for (int l0 = lb + kGather; l0 < le; l0 += kGather) {  // rows longer than one chunk
+            int32_t jj[kGather];
+            T vv[kGather];
+#pragma unroll
+            for (int q = 0; q < kGather; ++q) {
+                const int lq = min(l0 + q, le - 1);
+                int32_t cq;
+                if constexpr (IDX16)
+                    cq = i + (int32_t)cid[lq];
+                else
+                    cq = ci[lq];
+                const T vq = val[lq];
+                const bool ok = l0 + q < le;
+                jj[q] = ok ? cq : i;
+                vv[q] = ok ? vq : T(0);
+            }
+#pragma unroll
+            for (int q = 0; q < kGather; ++q) pj[q] = p[(int64_t)jj[q] * k + c];
+#pragma unroll
+            for (int q = 0; q < kGather; ++q) sum += (l0 + q < le) ? vv[q] * pj[q] : T(0);
+        }
+        u[e] = sum;
+        acc += (double)pi * (double)sum;
+#pragma unroll
+        for (int q = 0; q < kGather; ++q) {
+            jA[q] = jB[q];
+            vA[q] = vB[q];
+        }
+    }
+    return acc;
+}
+
+constexpr int kPipeSeg = 2048;  // rows per LDS segment of row pointers in the pipelined SpMM
+constexpr int kLdsRows = 256;   // rows per segment of the LDS-staged SpMM (a multiple of every kBlock / kpad)
+constexpr int kLdsNnz = 1792;   // nonzeros of a segment that fit the staging buffers (7 per row)
+
+// The rows [s0, s1) of a segment whose 16-bit column offsets and values sit in LDS
+// from nonzero rps[0] on: the gathers and the arithmetic of spmm_dot_rows, the index
+// loads from LDS.  acc continues the lane's partial.
+template <typename T>
+__device__ __forceinline__ double spmm_dot_rows_lds(const SysDev<T>& S, int s0, int s1, const int* rps,
+                                                    const int16_t* cls, const T* vls, int c, int ro, int rows,
+                                                    const T* __restrict__ p_, double acc) {
+    const gptr<const T> p = (gptr<const T>)p_;
+    const gptr<T> u = (gptr<T>)S.u;
+    const int k = S.k;
+    const int base = rps[0];
+    for (int i = s0 + ro; i < s1; i += rows) {
+        const int lb = rps[i - s0] - base, le = rps[i - s0 + 1] - base;
+        const int64_t e = (int64_t)i * k + c;
+        const T pi = p[e];
+        T sum = T(0);
+        for (int l0 = lb; l0 < le; l0 += kGather) {
+            int32_t jj[kGather];
+            T vv[kGather], pj[kGather];
+#pragma unroll
+            for (int q = 0; q < kGather; ++q) {
+                const int lq = min(l0 + q, le - 1);
+                const int32_t cq = i + (int32_t)cls[lq];
+                const T vq = vls[lq];
                 const bool ok = l0 + q < le;
                 jj[q] = ok ? cq : i;
                 vv[q] = ok ? vq : T(0);
@@ -600,6 +740,89 @@ __global__ __launch_bounds__(kBlock) void bk_spmm_dot_kernel(const Tile* __restr
     if (c < S.k)
         acc = S.cid ? spmm_dot_rows<T, true>(S, tl, c, ro, rows, pick(S.p, it))
                     : spmm_dot_rows<T, false>(S, tl, c, ro, rows, pick(S.p, it));
+    tile_partial(acc, S.k, S.kpad, S.ppu, tl.t);
+}
+
+// Kspmm with the column indices and values staged through LDS.  A row per lane
+// group keeps the gathers of neighbouring rows in the same cache lines, but its
+// index and value loads are strided (each lane reads its own row's nonzeros), so
+// every load instruction touches many lines and each line is touched once per
+// slot; here the workgroup first copies a 256-row segment's nonzeros with
+// consecutive lanes on consecutive nonzeros, then the row loop reads them from LDS.
+// Systems with 16-bit column offsets only (the others, and a segment with more than
+// kLdsNnz nonzeros, take the global loads of spmm_dot_rows).  Rows, lanes and
+// arithmetic are those of spmm_dot_rows: bitwise the same u and partials.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bk_spmm_dot_lds_kernel(const Tile* __restrict__ tiles,
+                                                                 const SysDev<T>* __restrict__ sys,
+                                                                 SysState* __restrict__ st) {
+    __shared__ int rps[kLdsRows + 1];
+    __shared__ int16_t cls[kLdsNnz];
+    __shared__ T vls[kLdsNnz];
+    const Tile tl = tiles[xcd_swizzle(blockIdx.x, gridDim.x)];
+    SysState& ss = st[tl.slot];
+    if (ss.done) return;
+    const SysDev<T>& S = sys[tl.slot];
+    const int it = ss.it_a;
+    const int c = threadIdx.x % S.kpad, ro = threadIdx.x / S.kpad, rows = kBlock / S.kpad;
+    const gptr<const int32_t> rp = (gptr<const int32_t>)S.rp;
+    const gptr<const int16_t> cid = (gptr<const int16_t>)S.cid;
+    const gptr<const T> val = (gptr<const T>)S.val;
+    const T* p = pick(S.p, it);
+    double acc = 0.0;
+    if (!S.cid) {
+        if (c < S.k) acc = spmm_dot_rows<T, false>(S, tl, c, ro, rows, p);
+        tile_partial(acc, S.k, S.kpad, S.ppu, tl.t);
+        return;
+    }
+    for (int s0 = tl.r0; s0 < tl.r1; s0 += kLdsRows) {
+        const int s1 = min(s0 + kLdsRows, tl.r1);
+        for (int j = threadIdx.x; j <= s1 - s0; j += kBlock) rps[j] = rp[s0 + j];
+        __syncthreads();
+        const int base = rps[0], cnt = rps[s1 - s0] - base;
+        if (cnt <= kLdsNnz) {
+            for (int t = threadIdx.x; t < cnt; t += kBlock) {
+                cls[t] = cid[base + t];
+                vls[t] = val[base + t];
+            }
+            __syncthreads();
+            if (c < S.k) acc = spmm_dot_rows_lds<T>(S, s0, s1, rps, cls, vls, c, ro, rows, p, acc);
+        } else if (c < S.k) {
+            const Tile seg{tl.slot, s0, s1, tl.t};
+            acc = spmm_dot_rows<T, true>(S, seg, c, ro, rows, p, acc);
+        }
+        __syncthreads();
+    }
+    tile_partial(acc, S.k, S.kpad, S.ppu, tl.t);
+}
+
+// Kspmm, software-pipelined: the tile's row pointers go through LDS in segments.
+// A segment's rows are those of spmm_dot_rows in the same per-lane order (a lane
+// takes rows ro, ro + rows, ... of the tile; segments are multiples of rows long),
+// so each lane adds the same terms to its partial in the same order.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bk_spmm_dot_pipe_kernel(const Tile* __restrict__ tiles,
+                                                                  const SysDev<T>* __restrict__ sys,
+                                                                  SysState* __restrict__ st) {
+    __shared__ int rps[kPipeSeg + 1];
+    const Tile tl = tiles[xcd_swizzle(blockIdx.x, gridDim.x)];
+    SysState& ss = st[tl.slot];
+    if (ss.done) return;
+    const SysDev<T>& S = sys[tl.slot];
+    const int it = ss.it_a;
+    const int c = threadIdx.x % S.kpad, ro = threadIdx.x / S.kpad, rows = kBlock / S.kpad;
+    const gptr<const int32_t> rp = (gptr<const int32_t>)S.rp;
+    const T* p = pick(S.p, it);
+    double acc = 0.0;
+    for (int s0 = tl.r0; s0 < tl.r1; s0 += kPipeSeg) {
+        const int s1 = min(s0 + kPipeSeg, tl.r1);
+        for (int j = threadIdx.x; j <= s1 - s0; j += kBlock) rps[j] = rp[s0 + j];
+        __syncthreads();
+        if (c < S.k)  // the lane's partial continues across segments, as in one pass over the tile
+            acc = S.cid ? spmm_dot_rows_pipe<T, true>(S, s0, s1, rps, c, ro, rows, p, acc)
+                        : spmm_dot_rows_pipe<T, false>(S, s0, s1, rps, c, ro, rows, p, acc);
+        __syncthreads();
+    }
     tile_partial(acc, S.k, S.kpad, S.ppu, tl.t);
 }
 
@@ -833,7 +1056,7 @@ template <typename T> class CGBatch {
         // The iterations as one hipGraph: captured on a private stream (capture is
         // not allowed on the legacy default stream a caller may pass), launched on
         // s_, captured again when the tile list, the buffers or the shape change.
-        const GraphKey key{dtiles_, dsys_, dst_, ntiles_, iters, g_batch_split, g_batch_stream};
+        const GraphKey key{dtiles_, dsys_, dst_, ntiles_, iters, g_batch_split, g_batch_stream, g_batch_spmm};
         if (!gexec_ || !(key == gkey_)) {
             if (gexec_) (void)hipGraphExecDestroy(gexec_);
             gexec_ = nullptr;
@@ -859,7 +1082,15 @@ template <typename T> class CGBatch {
         for (int it = 0; it < iters; ++it) {
             if (g_batch_split) {
                 hipLaunchKernelGGL((bk_dir_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s, dtiles_, dsys_, dst_);
-                hipLaunchKernelGGL((bk_spmm_dot_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s, dtiles_, dsys_, dst_);
+                if (g_batch_spmm == 2)
+                    hipLaunchKernelGGL((bk_spmm_dot_lds_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s, dtiles_,
+                                       dsys_, dst_);
+                else if (g_batch_spmm == 1)
+                    hipLaunchKernelGGL((bk_spmm_dot_pipe_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s, dtiles_,
+                                       dsys_, dst_);
+                else
+                    hipLaunchKernelGGL((bk_spmm_dot_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s, dtiles_, dsys_,
+                                       dst_);
             } else {
                 hipLaunchKernelGGL((bk_dir_spmm_kernel<T>), dim3(ntiles_), dim3(kBlock), 0, s, dtiles_, dsys_, dst_,
                                    g_batch_stream);
@@ -999,10 +1230,10 @@ template <typename T> class CGBatch {
         const void* tiles;
         const void* sys;
         const void* st;
-        int ntiles, iters, split, stream;
+        int ntiles, iters, split, stream, pipe;
         bool operator==(const GraphKey& o) const {
             return tiles == o.tiles && sys == o.sys && st == o.st && ntiles == o.ntiles && iters == o.iters &&
-                   split == o.split && stream == o.stream;
+                   split == o.split && stream == o.stream && pipe == o.pipe;
         }
     };
     hipStream_t cap_s_ = nullptr;
@@ -1556,6 +1787,12 @@ int batch_tune_set(const char* key, int32_t value, bool* known) {
         g_batch_split = value;
         return TG_OK;
     }
+    if (std::string(key) == "batch_spmm") {
+        *known = true;
+        if (value < 0 || value > 2) return fail(TG_ERR_INVALID, "tg_tune_set: batch_spmm must be 0, 1 or 2");
+        g_batch_spmm = value;
+        return TG_OK;
+    }
     if (std::string(key) == "batch_stream") {
         *known = true;
         if (value != 0 && value != 1) return fail(TG_ERR_INVALID, "tg_tune_set: batch_stream must be 0 or 1");
@@ -1577,6 +1814,7 @@ int32_t batch_tune_get(const char* key) {
     if (std::string(key) == "batch_idx16") return g_batch_idx16;
     if (std::string(key) == "batch_graph") return g_batch_graph;
     if (std::string(key) == "batch_stream") return g_batch_stream;
+    if (std::string(key) == "batch_spmm") return g_batch_spmm;
     return std::string(key) == "batch_tile_rows" ? g_batch_tile_rows : -1;
 }
 

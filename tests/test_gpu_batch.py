@@ -205,3 +205,53 @@ def test_graph_replayed_steps_equal_eager_steps(cuda, mixed):
     for i in range(len(sysl)):
         assert rg[i]["iterations"] == re_[i]["iterations"] and rg[i]["outer"] == re_[i]["outer"]
         assert np.array_equal(Xg[i], Xe[i])
+
+
+def nine_point_system(side=60, k=2, seed=12):
+    """An SPD 9-point grid operator (3 x 3 neighbourhood, -1 off the diagonal, 8 + 1e-2 on
+    it): 9 nonzeros per interior row, more than the LDS-staged SpMM holds for a 256-row
+    segment, so those segments take its global-load path."""
+    n = side * side
+    A = sp.lil_matrix((n, n))
+    for a in range(side):
+        for b in range(side):
+            i = a * side + b
+            A[i, i] = 8.01
+            for da in (-1, 0, 1):
+                for db in (-1, 0, 1):
+                    if (da or db) and 0 <= a + da < side and 0 <= b + db < side:
+                        A[i, (a + da) * side + b + db] = -1.0
+    A = A.tocsr()
+    A.sort_indices()
+    B = gen.uniform(seed, n * k).reshape(n, k)
+    return (A.indptr.astype(np.int64), A.indices.astype(np.int32), A.data, 1.0 / A.diagonal(), B, n, k)
+
+
+@pytest.mark.parametrize("mixed", [0, 1, 2])
+@pytest.mark.parametrize("tile_rows", [1024, 4096])
+def test_spmm_kernel_forms_are_bitwise_equal(cuda, mixed, tile_rows):
+    """The three SpMM kernels of the split iteration -- the row loop, the software-pipelined
+    row loop (row pointers through LDS in segments of 2048 rows, the next row's indices
+    loaded during this row's gathers) and the LDS-staged one (a 256-row segment's offsets
+    and values copied with coalesced loads; segments with more nonzeros than it holds, and
+    systems with int32 indices, on global loads) -- give bitwise the same results, with
+    tiles of one pipelined segment and of two."""
+    from tensors_amd import _capi as capi
+
+    sysl = systems(5, seed=808) + [wide_system(n=9000, k=3), nine_point_system(), wide_system(n=40000, k=1)]
+    before = (capi.lib().tg_tune_get(b"batch_spmm"), capi.lib().tg_tune_get(b"batch_tile_rows"))
+    out = {}
+    try:
+        capi.call("tg_tune_set", b"batch_tile_rows", tile_rows)
+        for form in (0, 1, 2):
+            capi.call("tg_tune_set", b"batch_spmm", form)
+            out[form] = run_batch(sysl, cuda, mixed, 1e-10)
+    finally:
+        capi.call("tg_tune_set", b"batch_spmm", before[0])
+        capi.call("tg_tune_set", b"batch_tile_rows", before[1])
+    X0, r0 = out[0]
+    for form in (1, 2):
+        X, r = out[form]
+        for i in range(len(sysl)):
+            assert r[i]["succeeded"] and r[i]["iterations"] == r0[i]["iterations"], (form, i)
+            assert np.array_equal(X[i], X0[i]), (form, i)
