@@ -47,6 +47,7 @@ static int g_batch_idx16 = 1;   // 1: 16-bit column offsets in the split SpMM wh
 static int g_batch_graph = 1;   // 1: the iterations of a step run as one captured hipGraph
 // SpMM kernel of the split iteration: 0 = row loop, 1 = software-pipelined row loop,
 // 2 = indices and values of a 256-row segment staged through LDS with coalesced loads
+// (two rows per lane in flight measured slower: 102 VGPRs, 4 waves per SIMD)
 static int g_batch_spmm = 2;
 constexpr int kGather = 8;     // nonzeros per gather chunk in the SpMM kernel
 constexpr int kRowUnroll = 4;  // rows per step in the update kernel
@@ -96,8 +97,8 @@ __device__ __forceinline__ float fma_t(float a, float b, float c) { return __bui
 
 // ---- reductions ----------------------------------------------------------------
 // Block tree over the kBlock/kpad row groups; row group 0 column c writes partial[t*k + c].
-__device__ __forceinline__ void tile_partial(double v, int k, int kpad, double* partial, int t) {
-    __shared__ double sh[kBlock];
+// sh: kBlock doubles of LDS (a kernel's own buffer, or tile_partial's).
+__device__ __forceinline__ void tile_partial_in(double v, int k, int kpad, double* partial, int t, double* sh) {
     const int c = threadIdx.x % kpad, ro = threadIdx.x / kpad, rows = kBlock / kpad;
     sh[threadIdx.x] = v;
     __syncthreads();
@@ -106,6 +107,11 @@ __device__ __forceinline__ void tile_partial(double v, int k, int kpad, double* 
         __syncthreads();
     }
     if (ro == 0 && c < k) partial[(int64_t)t * k + c] = sh[c];
+}
+
+__device__ __forceinline__ void tile_partial(double v, int k, int kpad, double* partial, int t) {
+    __shared__ double sh[kBlock];
+    tile_partial_in(v, k, kpad, partial, t, sh);
 }
 
 // Fixed-order sum over a system's tile partials (column c); every thread gets its column's total.
@@ -641,21 +647,55 @@ __device__ __forceinline__ double spmm_dot_rows_lds(const SysDev<T>& S, int s0, 
         const T pi = p[e];
         T sum = T(0);
         for (int l0 = lb; l0 < le; l0 += kGather) {
-            int32_t jj[kGather];
-            T vv[kGather], pj[kGather];
+            T pj[kGather];
 #pragma unroll
             for (int q = 0; q < kGather; ++q) {
                 const int lq = min(l0 + q, le - 1);
-                const int32_t cq = i + (int32_t)cls[lq];
-                const T vq = vls[lq];
-                const bool ok = l0 + q < le;
-                jj[q] = ok ? cq : i;
-                vv[q] = ok ? vq : T(0);
+                const int32_t jj = l0 + q < le ? i + (int32_t)cls[lq] : i;
+                // 32-bit byte offset from the block-uniform base (n k sizeof(T) < 4 GiB is
+                // checked at add): one address register per gather instead of two
+                const uint32_t off = ((uint32_t)jj * (uint32_t)k + (uint32_t)c) * (uint32_t)sizeof(T);
+                pj[q] = *(gptr<const T>)((gptr<const char>)p + off);
             }
+            // the values only now, from LDS: fewer registers live across the gathers
 #pragma unroll
-            for (int q = 0; q < kGather; ++q) pj[q] = p[(int64_t)jj[q] * k + c];
-#pragma unroll
-            for (int q = 0; q < kGather; ++q) sum += (l0 + q < le) ? vv[q] * pj[q] : T(0);
+            for (int q = 0; q < kGather; ++q) {
+                const bool ok = l0 + q < le;
+                const T vv = ok ? vls[min(l0 + q, le - 1)] : T(0);
+                sum += ok ? vv * pj[q] : T(0);
+            }
+        }
+        u[e] = sum;
+        acc += (double)pi * (double)sum;
+    }
+    return acc;
+}
+
+// The rows of spmm_dot_rows one nonzero at a time, with few registers: the rare paths
+// of the LDS-staged kernel (systems with int32 indices, segments with more nonzeros
+// than the staging buffers) would otherwise set that kernel's register count.  The
+// same products and additions in the same order (unfused: the chunked loop adds a
+// masked product through a select, which the compiler does not contract); it only
+// skips the +0 that loop adds for masked slots, which changes no value but the sign
+// of a zero sum.
+template <typename T, bool IDX16>
+__device__ __attribute__((noinline)) double spmm_dot_rows_lean(const SysDev<T>& S, int r0, int r1, int c, int ro,
+                                                               int rows, const T* __restrict__ p_, double acc) {
+    const gptr<const T> p = (gptr<const T>)p_;
+    const gptr<const int32_t> rp = (gptr<const int32_t>)S.rp;
+    const gptr<const int32_t> ci = (gptr<const int32_t>)S.ci;
+    const gptr<const int16_t> cid = (gptr<const int16_t>)S.cid;
+    const gptr<const T> val = (gptr<const T>)S.val;
+    const gptr<T> u = (gptr<T>)S.u;
+    const int k = S.k;
+    for (int i = r0 + ro; i < r1; i += rows) {
+        const int64_t e = (int64_t)i * k + c;
+        const T pi = p[e];
+        T sum = T(0);
+        for (int l = rp[i], le = rp[i + 1]; l < le; ++l) {
+#pragma clang fp contract(off)
+            const int32_t j = IDX16 ? i + (int32_t)cid[l] : ci[l];
+            sum += val[l] * p[(int64_t)j * k + c];
         }
         u[e] = sum;
         acc += (double)pi * (double)sum;
@@ -750,7 +790,7 @@ __global__ __launch_bounds__(kBlock) void bk_spmm_dot_kernel(const Tile* __restr
 // slot; here the workgroup first copies a 256-row segment's nonzeros with
 // consecutive lanes on consecutive nonzeros, then the row loop reads them from LDS.
 // Systems with 16-bit column offsets only (the others, and a segment with more than
-// kLdsNnz nonzeros, take the global loads of spmm_dot_rows).  Rows, lanes and
+// kLdsNnz nonzeros, take the global loads of spmm_dot_rows_lean).  Rows, lanes and
 // arithmetic are those of spmm_dot_rows: bitwise the same u and partials.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void bk_spmm_dot_lds_kernel(const Tile* __restrict__ tiles,
@@ -758,7 +798,10 @@ __global__ __launch_bounds__(kBlock) void bk_spmm_dot_lds_kernel(const Tile* __r
                                                                  SysState* __restrict__ st) {
     __shared__ int rps[kLdsRows + 1];
     __shared__ int16_t cls[kLdsNnz];
-    __shared__ T vls[kLdsNnz];
+    // the staged values; after the last segment the tile partial's reduction buffer
+    __shared__ __align__(16) char vbuf[kLdsNnz * sizeof(T) > kBlock * sizeof(double) ? kLdsNnz * sizeof(T)
+                                                                                      : kBlock * sizeof(double)];
+    T* vls = (T*)vbuf;
     const Tile tl = tiles[xcd_swizzle(blockIdx.x, gridDim.x)];
     SysState& ss = st[tl.slot];
     if (ss.done) return;
@@ -771,8 +814,8 @@ __global__ __launch_bounds__(kBlock) void bk_spmm_dot_lds_kernel(const Tile* __r
     const T* p = pick(S.p, it);
     double acc = 0.0;
     if (!S.cid) {
-        if (c < S.k) acc = spmm_dot_rows<T, false>(S, tl, c, ro, rows, p);
-        tile_partial(acc, S.k, S.kpad, S.ppu, tl.t);
+        if (c < S.k) acc = spmm_dot_rows_lean<T, false>(S, tl.r0, tl.r1, c, ro, rows, p, acc);
+        tile_partial_in(acc, S.k, S.kpad, S.ppu, tl.t, (double*)vbuf);
         return;
     }
     for (int s0 = tl.r0; s0 < tl.r1; s0 += kLdsRows) {
@@ -788,12 +831,11 @@ __global__ __launch_bounds__(kBlock) void bk_spmm_dot_lds_kernel(const Tile* __r
             __syncthreads();
             if (c < S.k) acc = spmm_dot_rows_lds<T>(S, s0, s1, rps, cls, vls, c, ro, rows, p, acc);
         } else if (c < S.k) {
-            const Tile seg{tl.slot, s0, s1, tl.t};
-            acc = spmm_dot_rows<T, true>(S, seg, c, ro, rows, p, acc);
+            acc = spmm_dot_rows_lean<T, true>(S, s0, s1, c, ro, rows, p, acc);
         }
         __syncthreads();
     }
-    tile_partial(acc, S.k, S.kpad, S.ppu, tl.t);
+    tile_partial_in(acc, S.k, S.kpad, S.ppu, tl.t, (double*)vbuf);
 }
 
 // Kspmm, software-pipelined: the tile's row pointers go through LDS in segments.
@@ -936,6 +978,8 @@ template <typename T> class CGBatch {
     int add(int n, int64_t nnz, const int32_t* rp, const int32_t* ci, const T* val, const T* d, int k, const T* b,
             T* x, double tol, int max_iter, int* slot_out, int check_every = 0) {
         if (n < 1 || k < 1 || k > kBatchMaxK) return fail(TG_ERR_INVALID, "batch: n >= 1 and 1 <= nrhs <= 8");
+        if ((uint64_t)n * (uint64_t)k * sizeof(T) >= (1ull << 32))  // the SpMM addresses vectors by 32-bit byte offsets
+            return fail(TG_ERR_INVALID, "batch: n * nrhs * sizeof(scalar) must be below 4 GiB (use the single-system solver)");
         int slot = -1;
         for (int i = 0; i < (int)used_.size(); ++i)
             if (!used_[i]) {

@@ -230,12 +230,12 @@ def nine_point_system(side=60, k=2, seed=12):
 @pytest.mark.parametrize("mixed", [0, 1, 2])
 @pytest.mark.parametrize("tile_rows", [1024, 4096])
 def test_spmm_kernel_forms_are_bitwise_equal(cuda, mixed, tile_rows):
-    """The three SpMM kernels of the split iteration -- the row loop, the software-pipelined
-    row loop (row pointers through LDS in segments of 2048 rows, the next row's indices
-    loaded during this row's gathers) and the LDS-staged one (a 256-row segment's offsets
-    and values copied with coalesced loads; segments with more nonzeros than it holds, and
-    systems with int32 indices, on global loads) -- give bitwise the same results, with
-    tiles of one pipelined segment and of two."""
+    """The SpMM kernels of the split iteration -- the row loop, the software-pipelined row
+    loop (row pointers through LDS in segments of 2048 rows, the next row's indices loaded
+    during this row's gathers), the LDS-staged one (a 256-row segment's offsets and values
+    copied with coalesced loads; segments with more nonzeros than it holds, and systems with
+    int32 indices, on global loads) -- give bitwise the same results, with tiles of one
+    pipelined segment and of two."""
     from tensors_amd import _capi as capi
 
     sysl = systems(5, seed=808) + [wide_system(n=9000, k=3), nine_point_system(), wide_system(n=40000, k=1)]
