@@ -341,6 +341,7 @@ def batch_section(args, dev):
         for i, s in enumerate(systems):
             ids[bt.add(s["rp"], s["ci"], s["val"], s["B"], Xs[i], tol, 20000, diag=s["d"], mixed=mixed,
                        true_residual=mode == 2)] = i
+        t_add = time.perf_counter() - t0
         done_t = {}
         for sid in bt.finished():
             done_t[sid] = time.perf_counter() - t0
@@ -378,7 +379,7 @@ def batch_section(args, dev):
             elif mode == 2:
                 alg += res[sid]["outer"] * (z_ * 12 + 4 * (n_ + 1) + n_ * k_ * (8 + 8 + 8))
         return Xs, {
-            "solves_per_s": round(len(systems) / wall, 2), "wall_s": round(wall, 3),
+            "solves_per_s": round(len(systems) / wall, 2), "wall_s": round(wall, 3), "submit_s": round(t_add, 3),
             "alg_GB": round(alg / 1e9, 1), "alg_GBps": round(alg / wall / 1e9, 1),
             "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 1),
             "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 1),

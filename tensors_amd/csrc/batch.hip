@@ -132,10 +132,33 @@ __device__ __forceinline__ double sys_total(const double* partial, int tiles, in
 }
 
 // ---- setup: r = b, z = P r, x = 0, partial r.z (parity 0) -------------------------
+// A new system's descriptor, state and tol^2 into the slot tables (arguments by
+// value, so no host buffer has to outlive the launch); the 16-bit-offset flag starts
+// set (bk_delta16_kernel clears it).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void bk_init_kernel(const Tile* __restrict__ tiles, const SysDev<T>* __restrict__ sys) {
-    const Tile tl = tiles[xcd_swizzle(blockIdx.x, gridDim.x)];
-    const SysDev<T>& S = sys[tl.slot];
+__global__ void bk_install_kernel(SysDev<T> S, SysState z, double t2, int slot, SysDev<T>* __restrict__ sys,
+                                  SysState* __restrict__ st, double* __restrict__ tol2, int* __restrict__ ok16) {
+    if (threadIdx.x == 0) {
+        sys[slot] = S;
+        st[slot] = z;
+        tol2[slot] = t2;
+        *ok16 = 1;
+    }
+}
+
+// The 16-bit column offsets go live when every offset fitted.
+template <typename T>
+__global__ void bk_cid_commit_kernel(int slot, SysDev<T>* __restrict__ sys, const int* __restrict__ ok16,
+                                     const int16_t* cid) {
+    if (threadIdx.x == 0 && *ok16) sys[slot].cid = cid;
+}
+
+// One workgroup per row tile of the system in `slot` (tile blockIdx.x).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bk_init_kernel(int slot, const SysDev<T>* __restrict__ sys) {
+    const SysDev<T>& S = sys[slot];
+    const int t = blockIdx.x;
+    const Tile tl{slot, t * S.tile_rows, min(S.n, (t + 1) * S.tile_rows), t};
     const int c = threadIdx.x % S.kpad, ro = threadIdx.x / S.kpad, rows = kBlock / S.kpad;
     double acc = 0.0;
     if (c < S.k)
@@ -153,9 +176,8 @@ __global__ __launch_bounds__(kBlock) void bk_init_kernel(const Tile* __restrict_
 
 // Setup totals of one system (one workgroup): rho_0, TOL = tol^2 |r.z|, initial test.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void bk_setup_kernel(const int* __restrict__ slotp, const SysDev<T>* __restrict__ sys,
+__global__ __launch_bounds__(kBlock) void bk_setup_kernel(int slot, const SysDev<T>* __restrict__ sys,
                                                           SysState* __restrict__ st, const double* __restrict__ tol2) {
-    const int slot = *slotp;
     SysState& ss = st[slot];
     const SysDev<T>& S = sys[slot];
     const int c = threadIdx.x % S.kpad, j = threadIdx.x / S.kpad, J = kBlock / S.kpad;
@@ -1029,39 +1051,22 @@ template <typename T> class CGBatch {
         host_[slot] = S;
         SysState z{};
         z.max_iter = std::max(0, max_iter);
-        const double t2 = tol * tol;
-        TG_HIP(hipMemcpyAsync(dsys_ + slot, &S, sizeof(S), hipMemcpyHostToDevice, s_));
-        TG_HIP(hipMemcpyAsync(dst_ + slot, &z, sizeof(z), hipMemcpyHostToDevice, s_));
-        TG_HIP(hipMemcpyAsync(dtol2_ + slot, &t2, sizeof(double), hipMemcpyHostToDevice, s_));
-        // setup for this system alone
-        std::vector<Tile> tl;
-        tiles_of(slot, tl);
-        if (tl.size() > setup_cap_) {
-            if (dsetup_tiles_) TG_HIP(hipFreeAsync(dsetup_tiles_, s_));
-            setup_cap_ = std::max(tl.size(), 2 * setup_cap_);
-            TG_HIP(hipMallocAsync((void**)&dsetup_tiles_, setup_cap_ * sizeof(Tile), s_));
-        }
-        TG_HIP(hipMemcpyAsync(dsetup_tiles_, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice, s_));
-        TG_HIP(hipMemcpyAsync(dsetup_slot_, &slot, sizeof(int), hipMemcpyHostToDevice, s_));
-        hipLaunchKernelGGL((bk_init_kernel<T>), dim3((unsigned)tl.size()), dim3(kBlock), 0, s_, dsetup_tiles_, dsys_);
-        hipLaunchKernelGGL((bk_setup_kernel<T>), dim3(1), dim3(kBlock), 0, s_, dsetup_slot_, dsys_, dst_,
+        // Setup for this system alone, all stream-ordered with no host wait: the
+        // descriptor and state by kernel arguments, r = b, x = 0, the r.z partials
+        // and totals, then the 16-bit column offsets (used by the split iteration's
+        // SpMM when every offset fits; the device decides).
+        hipLaunchKernelGGL((bk_install_kernel<T>), dim3(1), dim3(kWave), 0, s_, S, z, tol * tol, slot, dsys_, dst_,
+                           dtol2_, dok);
+        hipLaunchKernelGGL((bk_init_kernel<T>), dim3((unsigned)S.tiles), dim3(kBlock), 0, s_, slot,
+                           (const SysDev<T>*)dsys_);
+        hipLaunchKernelGGL((bk_setup_kernel<T>), dim3(1), dim3(kBlock), 0, s_, slot, (const SysDev<T>*)dsys_, dst_,
                            (const double*)dtol2_);
-        // 16-bit column offsets for the SpMM of the split iteration
-        const int one = 1;
-        int ok16 = 0;
-        TG_HIP(hipMemcpyAsync(dok, &one, sizeof(int), hipMemcpyHostToDevice, s_));
         hipLaunchKernelGGL(bk_delta16_kernel, dim3((unsigned)std::min(ceil_div(n, kBlock), 4096)), dim3(kBlock), 0,
                            s_, n, nnz, rp, ci, cid, dok);
+        if (g_batch_idx16)
+            hipLaunchKernelGGL((bk_cid_commit_kernel<T>), dim3(1), dim3(kWave), 0, s_, slot, dsys_, (const int*)dok,
+                               (const int16_t*)cid);
         TG_LAUNCH_CHECK();
-        TG_HIP(hipMemcpyAsync(&ok16, dok, sizeof(int), hipMemcpyDeviceToHost, s_));
-        // the setup buffers are reused by the next add: finish this one first
-        TG_HIP(hipStreamSynchronize(s_));
-        if (ok16 && g_batch_idx16) {
-            S.cid = cid;
-            host_[slot] = S;
-            TG_HIP(hipMemcpyAsync(dsys_ + slot, &S, sizeof(S), hipMemcpyHostToDevice, s_));
-            TG_HIP(hipStreamSynchronize(s_));  // S is a stack copy
-        }
         dirty_ = true;
         *slot_out = slot;
         return TG_OK;
@@ -1229,7 +1234,6 @@ template <typename T> class CGBatch {
         mem_.resize(cap, nullptr);
         bytes_.resize(cap, 0);
         host_.resize(cap);
-        if (!dsetup_slot_) TG_HIP(hipMallocAsync((void**)&dsetup_slot_, sizeof(int), s_));
         return TG_OK;
     }
 
@@ -1260,8 +1264,6 @@ template <typename T> class CGBatch {
         if (dst_) (void)hipFreeAsync(dst_, s_);
         if (dtol2_) (void)hipFreeAsync(dtol2_, s_);
         if (dtiles_) (void)hipFreeAsync(dtiles_, s_);
-        if (dsetup_tiles_) (void)hipFreeAsync(dsetup_tiles_, s_);
-        if (dsetup_slot_) (void)hipFreeAsync(dsetup_slot_, s_);
         if (dresume_) (void)hipFreeAsync(dresume_, s_);
         (void)hipStreamSynchronize(s_);
         if (gexec_) (void)hipGraphExecDestroy(gexec_);
@@ -1298,9 +1300,6 @@ template <typename T> class CGBatch {
     double* dtol2_ = nullptr;
     Tile* dtiles_ = nullptr;
     size_t tiles_cap_ = 0;
-    Tile* dsetup_tiles_ = nullptr;
-    size_t setup_cap_ = 0;
-    int* dsetup_slot_ = nullptr;
     int ntiles_ = 0;
     bool dirty_ = true;
 };
@@ -1486,6 +1485,7 @@ struct BatchSystem {
     double* pnorm = nullptr;
     double* norms = nullptr;  // k
     std::vector<double> bnorm;
+    bool bnorm_pending = false;  // ||b|| computed on the device, not read yet
     // results
     int finished = 0, succeeded = 0, iterations = 0;
     std::vector<double> rel;
@@ -1654,10 +1654,7 @@ static int mixed_start(Batch& B, BatchSystem& S) {
                        S.norms);
     TG_LAUNCH_CHECK();
     S.bnorm.assign(S.k, 0.0);
-    TG_HIP(hipMemcpyAsync(S.bnorm.data(), S.norms, S.k * 8, hipMemcpyDeviceToHost, B.s));
-    TG_HIP(hipStreamSynchronize(B.s));
-    for (int c = 0; c < S.k; ++c)
-        if (std::isnan(S.bnorm[c])) return finish(B, S, 0, 0);
+    S.bnorm_pending = true;  // read at the next poll (a NaN in b stops the engine's setup itself)
     // The engine's own test (preconditioned FP32 estimate <= tol) only pauses the
     // system; the host then decides on the FP64 true residual.
     return B.b32.add(S.n, S.nnz, S.rp, S.ci, S.a32, S.d32, S.k, S.b32, S.x32, S.tol, S.max_iter, &S.slot,
@@ -1681,10 +1678,7 @@ static int true_start(Batch& B, BatchSystem& S) {
                        S.norms);
     TG_LAUNCH_CHECK();
     S.bnorm.assign(S.k, 0.0);
-    TG_HIP(hipMemcpyAsync(S.bnorm.data(), S.norms, S.k * 8, hipMemcpyDeviceToHost, B.s));
-    TG_HIP(hipStreamSynchronize(B.s));
-    for (int c = 0; c < S.k; ++c)
-        if (std::isnan(S.bnorm[c])) return finish(B, S, 0, 0);
+    S.bnorm_pending = true;
     return B.b64.add(S.n, S.nnz, S.rp, S.ci, S.val64, S.d64, S.k, S.b64, S.x64, S.tol, S.max_iter, &S.slot,
                      S.check_every);
 }
@@ -1699,6 +1693,18 @@ static int batch_advance(Batch& B, int iters) {
     if (rc) return rc;
     rc = B.b32.poll(s32);
     if (rc) return rc;
+    // ||b|| of the systems added since the last poll (before any replacement needs it)
+    bool pend = false;
+    for (auto& kv : B.sys) {
+        BatchSystem& S = *kv.second;
+        if (!S.bnorm_pending || S.finished) continue;
+        TG_HIP(hipMemcpyAsync(S.bnorm.data(), S.norms, S.k * 8, hipMemcpyDeviceToHost, B.s));
+        pend = true;
+    }
+    if (pend) {
+        TG_HIP(hipStreamSynchronize(B.s));
+        for (auto& kv : B.sys) kv.second->bnorm_pending = false;
+    }
     if (B.dnorms_cap < B.b32.capacity()) {  // sized before any replacement of this poll writes it
         if (B.dnorms) TG_HIP(hipFreeAsync(B.dnorms, B.s));
         B.dnorms_cap = std::max(B.b32.capacity(), 2 * B.dnorms_cap);
