@@ -132,6 +132,11 @@ int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, doubl
  *                  default), 0 = eager launches.  Bitwise the same results.
  *   "batch_idx16": 1 = 16-bit column offsets (ci - row) in the batched SpMM for
  *                  systems whose offsets fit (default), 0 = int32 indices.
+ *   "batch_spmm":  SpMM kernel of the split batched iteration, 2 = the offsets and
+ *                  values of 256-row segments staged through LDS with coalesced
+ *                  loads (default), 1 = row loop with the next row's indices
+ *                  loaded during this row's gathers, 0 = row loop.  Bitwise the
+ *                  same results.
  *   "batch_replace_drop_pct", "batch_check_every": residual replacement of
  *                  batched modes 1 / 2 when the estimate fell to this percent
  *                  (10) at checkpoints every this many iterations (16).

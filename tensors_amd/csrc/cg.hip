@@ -163,35 +163,66 @@ __device__ __forceinline__ T x_step(bool first, T x, T al, T pv) {
     return first ? al * pv : fma_t(al, pv, x);
 }
 
-// x = (first ? 0 : x) + alpha p;  r -= alpha u;  z = P r;  partial(r.z)   (:262-313)
-// DEFER_X: the x update is left to the p step (cg_p_kernel), which reads p anyway:
-// three passes over n x nrhs here instead of six.
-template <typename T, bool DEFER_X>
-__global__ __launch_bounds__(kBlock) void cg_update_kernel(int n, int k, int kpad, const T* __restrict__ p,
-                                                           const T* __restrict__ u, const T* __restrict__ d,
-                                                           T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
-                                                           int zmode, double* __restrict__ partial,
+constexpr int kRowsInFlight = 4;  // rows per thread whose loads are issued together
+
+// The thread's rows of the r update, kRowsInFlight at a time: all loads first (at
+// clamped, always valid rows), then the arithmetic and stores in row order, so the
+// loads of four rows are in flight together and the partial adds its terms in the
+// same order as a plain row loop.
+template <typename T, bool HAS_D, bool ZDOT>
+__device__ __forceinline__ double update_rows(int i0, int i1, const Lanes& L, int k, T al, const T* __restrict__ u,
+                                              const T* __restrict__ d, T* __restrict__ r, T* __restrict__ z) {
+    double acc = 0.0;
+    for (int ib = i0 + L.ro; ib < i1; ib += kRowsInFlight * L.rows) {
+        T rv[kRowsInFlight], uv[kRowsInFlight], dv[kRowsInFlight];
+#pragma unroll
+        for (int q = 0; q < kRowsInFlight; ++q) {
+            const int i = min(ib + q * L.rows, i1 - 1);
+            const int64_t e = (int64_t)i * k + L.c;
+            rv[q] = r[e];
+            uv[q] = u[e];
+            if constexpr (HAS_D) dv[q] = d[i];
+        }
+#pragma unroll
+        for (int q = 0; q < kRowsInFlight; ++q) {
+            const int i = ib + q * L.rows;
+            if (i < i1) {
+                const int64_t e = (int64_t)i * k + L.c;
+                const T rn = rv[q] - al * uv[q];
+                r[e] = rn;
+                T zv = rn;
+                if constexpr (HAS_D) {
+                    zv = dv[q] * rn;
+                    z[e] = zv;
+                }
+                if constexpr (ZDOT) acc += (double)rn * (double)zv;
+            }
+        }
+    }
+    return acc;
+}
+
+// r -= alpha u;  z = P r;  partial(r.z)   (:262-313).  The reference updates x here as
+// well; x += alpha p is left to the p step (cg_p_kernel), which reads p anyway: three
+// passes over n x nrhs here instead of six.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void cg_update_kernel(int n, int k, int kpad, const T* __restrict__ u,
+                                                           const T* __restrict__ d, T* __restrict__ r,
+                                                           T* __restrict__ z, int zmode, double* __restrict__ partial,
                                                            const CGState* __restrict__ st) {
     if (st->done) return;
     int i0, i1;
     block_row_range(n, i0, i1);
     const Lanes L = lanes_of(kpad);
-    const bool first = (st->iter == 0) && !st->use_guess;
     double acc = 0.0;
     if (L.c < k) {
         const T al = (T)st->alpha[L.c];
-        for (int i = i0 + L.ro; i < i1; i += L.rows) {
-            const int64_t e = (int64_t)i * k + L.c;
-            if constexpr (!DEFER_X) x[e] = x_step(first, x[e], al, p[e]);
-            const T rv = r[e] - al * u[e];
-            r[e] = rv;
-            T zv = rv;
-            if (d) {
-                zv = d[i] * rv;
-                z[e] = zv;
-            }
-            if (zmode != 2) acc += (double)rv * (double)zv;
-        }
+        if (d)
+            acc = zmode != 2 ? update_rows<T, true, true>(i0, i1, L, k, al, u, d, r, z)
+                             : update_rows<T, true, false>(i0, i1, L, k, al, u, d, r, z);
+        else
+            acc = zmode != 2 ? update_rows<T, false, true>(i0, i1, L, k, al, u, d, r, z)
+                             : update_rows<T, false, false>(i0, i1, L, k, al, u, d, r, z);
     }
     column_partial(acc, L, k, kpad, partial);
 }
@@ -221,9 +252,9 @@ __global__ __launch_bounds__(kFinalThreads) void cg_rho_kernel(int k, int kpad, 
     }
 }
 
-// p = z + beta p   (:321-335); DEFER_X: first x += alpha p with the old p (the
-// update the x,r step left out; rho has advanced iter, so iteration 0 is iter 1)
-template <typename T, bool DEFER_X>
+// p = z + beta p   (:321-335), and first x += alpha p with the old p (the update the
+// r step leaves out; rho has advanced iter, so iteration 0 is iter 1)
+template <typename T>
 __global__ __launch_bounds__(kBlock) void cg_p_kernel(int n, int k, int kpad, const T* __restrict__ z,
                                                       T* __restrict__ p, T* __restrict__ x,
                                                       const CGState* __restrict__ st) {
@@ -235,11 +266,24 @@ __global__ __launch_bounds__(kBlock) void cg_p_kernel(int n, int k, int kpad, co
     if (L.c < k) {
         const T be = (T)st->beta[L.c];
         const T al = (T)st->alpha[L.c];
-        for (int i = i0 + L.ro; i < i1; i += L.rows) {
-            const int64_t e = (int64_t)i * k + L.c;
-            const T pv = p[e];
-            if constexpr (DEFER_X) x[e] = x_step(first, x[e], al, pv);
-            p[e] = z[e] + be * pv;
+        for (int ib = i0 + L.ro; ib < i1; ib += kRowsInFlight * L.rows) {
+            T pv[kRowsInFlight], zv[kRowsInFlight], xv[kRowsInFlight];
+#pragma unroll
+            for (int q = 0; q < kRowsInFlight; ++q) {  // loads of four rows first (clamped rows)
+                const int64_t e = (int64_t)min(ib + q * L.rows, i1 - 1) * k + L.c;
+                pv[q] = p[e];
+                zv[q] = z[e];
+                xv[q] = x[e];
+            }
+#pragma unroll
+            for (int q = 0; q < kRowsInFlight; ++q) {
+                const int i = ib + q * L.rows;
+                if (i < i1) {
+                    const int64_t e = (int64_t)i * k + L.c;
+                    x[e] = x_step(first, xv[q], al, pv[q]);
+                    p[e] = zv[q] + be * pv[q];
+                }
+            }
         }
     }
 }
@@ -730,14 +774,14 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
                 hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kFinalThreads), 0, s, k, kpad, B,
                                    (const double*)partial, st);
             }
-            hipLaunchKernelGGL((cg_update_kernel<T, true>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, (const T*)p,
-                               (const T*)u, dd, x, r, z, zmode, partial, (const CGState*)st);
+            hipLaunchKernelGGL((cg_update_kernel<T>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, (const T*)u, dd, r,
+                               z, zmode, partial, (const CGState*)st);
             if (zmode == 2) {
                 rc = apply_P();
                 if (rc) break;
             }
             hipLaunchKernelGGL(cg_rho_kernel, dim3(1), dim3(kFinalThreads), 0, s, k, kpad, B, (const double*)partial, st);
-            hipLaunchKernelGGL((cg_p_kernel<T, true>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, (const T*)z, p, x,
+            hipLaunchKernelGGL((cg_p_kernel<T>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, (const T*)z, p, x,
                                (const CGState*)st);
         }
         if (rc) break;
