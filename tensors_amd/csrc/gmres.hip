@@ -139,6 +139,11 @@ __global__ __launch_bounds__(kBlock) void gm_dots1_kernel(int n, int k, int kpad
     if (c < k)
         for (int r = i0 + ro; r < i1; r += rows) {
             const int64_t e = (int64_t)r * k + c;
+            // all basis loads first, at clamped (valid) vectors: a condition around a
+            // load would split the unrolled loop into one memory round trip per vector
+            T qi[NQMAX];
+#pragma unroll
+            for (int i = 0; i < NQMAX; ++i) qi[i] = Q[(int64_t)min(i, nq - 1) * qstride + e];
             T qv;
             if (d) {
                 qv = d[r] * z[e];
@@ -148,7 +153,7 @@ __global__ __launch_bounds__(kBlock) void gm_dots1_kernel(int n, int k, int kpad
             }
 #pragma unroll
             for (int i = 0; i < NQMAX; ++i)
-                if (i < nq) v[i] += (double)Q[i * qstride + e] * (double)qv;
+                if (i < nq) v[i] += (double)qi[i] * (double)qv;
         }
     multi_partial<NQMAX>(v, nq, k, kpad, partial);
 }
@@ -176,11 +181,10 @@ __global__ __launch_bounds__(kBlock) void gm_update_kernel(int n, int k, int kpa
             T qi[NQMAX];
             T qv = q[e];
 #pragma unroll
+            for (int i = 0; i < NQMAX; ++i) qi[i] = Q[(int64_t)min(i, nq - 1) * qstride + e];  // loads first
+#pragma unroll
             for (int i = 0; i < NQMAX; ++i)
-                if (i < nq) {
-                    qi[i] = Q[i * qstride + e];
-                    qv -= hc[i] * qi[i];
-                }
+                if (i < nq) qv -= hc[i] * qi[i];
             q[e] = qv;
             if (norm_only) {
                 v[0] += (double)qv * (double)qv;
