@@ -84,6 +84,7 @@ def parse():
     ap.add_argument("--bsr-vector", type=int, default=-1,
                     help="block-CSR lane mapping (tg_tune_set bsr_vector: 1 vector, 0 scalar; -1 default)")
     ap.add_argument("--no-mm", action="store_true", help="skip the MatrixMarket I/O section")
+    ap.add_argument("--no-gmres", action="store_true", help="skip the single-system GMRES section")
     ap.add_argument("--serve-requests", type=int, default=160)
     ap.add_argument("--serve-rates", type=str, default="0.25,0.5,1,2",
                     help="arrival rates as multiples of the measured saturated service rate")
@@ -622,6 +623,39 @@ def mm_section(args):
             "load_MBps": round(size / tl / 1e6, 1), "round_trip_exact": same, "host_threads": os.cpu_count()}
 
 
+def gmres_section(args, dev):
+    """GMRES(30) to 1e-8 (P = I, 100 restarts) on single upwind convection-diffusion systems of the
+    serving mix's range (tg_gmres_solve): one warm-up and one timed solve each; ms, Arnoldi steps and
+    microseconds per Arnoldi step (restarts x 30 + the last cycle's iterations)."""
+    import torch
+
+    from tensors_amd import generators as gen
+    from tensors_amd.solvers import GMRES
+    from tensors_amd.sparse import MatrixCSR
+
+    res = []
+    for g, pe in ((1000, 10.0), (300, 30.0), (100, 3.0)):
+        rp, ci, val = gen.convection_diffusion_2d(g, pe)
+        n = g * g
+        A = MatrixCSR(rp, ci, val, n, n)
+        B = torch.from_numpy(gen.uniform(7, n)).to(dev).reshape(n, 1)
+        t = 0.0
+        for rep_ in range(2):
+            X = torch.zeros(n, 1, dtype=torch.float64, device=dev)
+            gm = GMRES(n, 30, 1)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ok = gm(A, None, 1.0, B, 0.0, X, 1e-8, 100)
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+        steps = gm.restarts * 30 + gm.iter
+        res.append({"grid": g, "peclet": pe, "n": n, "succeeded": bool(ok), "restarts": gm.restarts,
+                    "arnoldi_steps": steps, "ms": round(t * 1e3, 1), "us_per_step": round(t * 1e6 / max(steps, 1), 1)})
+        log(f"GMRES {res[-1]}")
+    return {"workload": "GMRES(30), 1e-8, P = I, 100 restarts, 2D upwind convection-diffusion, 1 RHS "
+                        "uniform[-1,1] (tg_gmres_solve)", "cases": res}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -755,6 +789,10 @@ def main():
     if not args.no_cg:
         log("CG section")
         out["cg"] = cg_section(args, dev)
+        torch.cuda.empty_cache()
+    if not args.no_gmres:
+        log("GMRES section")
+        out["gmres"] = gmres_section(args, dev)
         torch.cuda.empty_cache()
     if not args.no_batch:
         log("batched CG section")
