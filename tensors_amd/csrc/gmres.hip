@@ -16,12 +16,15 @@
 // vectors live in the vector type T (FP64 or FP32); the Hessenberg matrix,
 // rotations, beta, y and all reductions are FP64.
 //
-// Kernels of Arnoldi step j (q = Q_{j+1}):
+// Kernels of Arnoldi step j (q = Q_{j+1}), eight launches:
 //   spmm (left: z = A Q_j; right: z = P Q_j then q = A z)
 //   dots1:  q = P z (left), h1_i = Q_i.q             i <= j
+//   total:  h1 from the workgroup partials, H[.][j] += h1
 //   upd1:   q -= sum h1_i Q_i,  h2_i = Q_i.q
+//   total:  h2, H[.][j] += h2
 //   upd2:   q -= sum h2_i Q_i,  |q|^2
-//   givens: H[j+1][j] = |q|, rotations, beta, convergence (one workgroup)
+//   givens: |q| from the partials, H[j+1][j] = |q|, rotations, beta, convergence
+//           (one workgroup)
 //   scale:  q /= |q|
 // Every kernel returns at once when the device `done` flag is set, so the host
 // enqueues steps in batches and polls between batches.
@@ -123,6 +126,29 @@ __global__ __launch_bounds__(kBlock) void gm_total_kernel(int nq, int k, int B, 
     if (threadIdx.x == 0) out[ic] = sh[0];
 }
 
+// gm_total_kernel followed by H[i][j] += out[i*k + c] (each Gram-Schmidt pass adds its
+// coefficients, GMRES.hpp:564-568): one workgroup per (i, c), so each H entry has one writer.
+__global__ __launch_bounds__(kBlock) void gm_total_addh_kernel(int nq, int k, int B, int j, int m,
+                                                               const double* __restrict__ partial,
+                                                               double* __restrict__ out, double* __restrict__ H,
+                                                               const GmresHdr* __restrict__ hd) {
+    if (hd->done) return;
+    const int ic = blockIdx.x, i = ic / k, c = ic - i * k;
+    __shared__ double sh[kBlock];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < B; b += kBlock) s += partial[((int64_t)b * nq + i) * k + c];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[ic] = sh[0];
+        H[((int64_t)i * m + j) * k + c] += sh[0];
+    }
+}
+
 // dots1: q = P z (left, diag) or q already holds A Q_j; h1_i = Q_i . q
 template <typename T, int NQMAX>
 __global__ __launch_bounds__(kBlock) void gm_dots1_kernel(int n, int k, int kpad, int nq, const T* __restrict__ Q,
@@ -198,21 +224,29 @@ __global__ __launch_bounds__(kBlock) void gm_update_kernel(int n, int k, int kpa
     multi_partial<NQMAX>(v, norm_only ? 1 : nq, k, kpad, partial);
 }
 
-// H[0..nq)[j] += h  (each Gram-Schmidt pass adds its coefficients, GMRES.hpp:564-568)
-__global__ void gm_addh_kernel(int j, int nq, int m, int k, const double* __restrict__ h, double* __restrict__ H,
-                               const GmresHdr* __restrict__ hd) {
-    if (hd->done) return;
-    for (int t = threadIdx.x; t < nq * k; t += blockDim.x) {
-        const int i = t / k, c = t - i * k;
-        H[((int64_t)i * m + j) * k + c] += h[t];
-    }
-}
 
-// H[j+1][j] = |q|; Givens (GMRES.hpp:603-658); beta; convergence; ++iter.
-__global__ void gm_givens_kernel(int j, int m, int k, const double* __restrict__ qn2, double* __restrict__ H,
-                                 double* __restrict__ cs, double* __restrict__ sn, double* __restrict__ beta,
-                                 const double* __restrict__ TOL, double* __restrict__ inv, GmresHdr* __restrict__ hd) {
+// H[j+1][j] = |q|; Givens (GMRES.hpp:603-658); beta; convergence; ++iter.  |q|^2 per
+// column is first summed from the B workgroup partials in gm_total_kernel's order (one
+// workgroup of kBlock threads, one column at a time).
+__global__ __launch_bounds__(kBlock) void gm_givens_kernel(int j, int m, int k, int B, const double* __restrict__ partial,
+                                                           double* __restrict__ qn2, double* __restrict__ H,
+                                                           double* __restrict__ cs, double* __restrict__ sn,
+                                                           double* __restrict__ beta, const double* __restrict__ TOL,
+                                                           double* __restrict__ inv, GmresHdr* __restrict__ hd) {
     if (hd->done) return;
+    __shared__ double sh[kBlock];
+    for (int c = 0; c < k; ++c) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < B; b += kBlock) s += partial[(int64_t)b * k + c];
+        sh[threadIdx.x] = s;
+        __syncthreads();
+        for (int h = kBlock / 2; h > 0; h >>= 1) {
+            if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) qn2[c] = sh[0];
+        __syncthreads();
+    }
     __shared__ int conv;
     if (threadIdx.x == 0) conv = 1;
     __syncthreads();
@@ -629,23 +663,17 @@ int gmres_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d,
                 if (rc) return rc;
                 dots1<T>(s, n, k, kpad, nq, (const T*)Q, qstride, (const T*)z, side == 0 ? dd : nullptr, q, w.partial,
                          (const GmresHdr*)w.hd);
-                hipLaunchKernelGGL(gm_total_kernel, dim3(nq * k), dim3(kBlock), 0, s, nq, k, kGB,
-                                   (const double*)w.partial, w.tot, (const GmresHdr*)w.hd);
-                hipLaunchKernelGGL(gm_addh_kernel, dim3(1), dim3(kBlock), 0, s, j, nq, mm, k, (const double*)w.tot,
-                                   w.small + L.H, (const GmresHdr*)w.hd);
+                hipLaunchKernelGGL(gm_total_addh_kernel, dim3(nq * k), dim3(kBlock), 0, s, nq, k, kGB, j, mm,
+                                   (const double*)w.partial, w.tot, w.small + L.H, (const GmresHdr*)w.hd);
                 update<T>(s, n, k, kpad, nq, (const T*)Q, qstride, q, (const double*)w.tot, 0, w.partial,
                           (const GmresHdr*)w.hd);
-                hipLaunchKernelGGL(gm_total_kernel, dim3(nq * k), dim3(kBlock), 0, s, nq, k, kGB,
-                                   (const double*)w.partial, w.tot, (const GmresHdr*)w.hd);
-                hipLaunchKernelGGL(gm_addh_kernel, dim3(1), dim3(kBlock), 0, s, j, nq, mm, k, (const double*)w.tot,
-                                   w.small + L.H, (const GmresHdr*)w.hd);
+                hipLaunchKernelGGL(gm_total_addh_kernel, dim3(nq * k), dim3(kBlock), 0, s, nq, k, kGB, j, mm,
+                                   (const double*)w.partial, w.tot, w.small + L.H, (const GmresHdr*)w.hd);
                 update<T>(s, n, k, kpad, nq, (const T*)Q, qstride, q, (const double*)w.tot, 1, w.partial,
                           (const GmresHdr*)w.hd);
-                hipLaunchKernelGGL(gm_total_kernel, dim3(k), dim3(kBlock), 0, s, 1, k, kGB, (const double*)w.partial,
-                                   w.tot, (const GmresHdr*)w.hd);
-                hipLaunchKernelGGL(gm_givens_kernel, dim3(1), dim3(64), 0, s, j, mm, k, (const double*)w.tot,
-                                   w.small + L.H, w.small + L.cs, w.small + L.sn, w.small + L.beta,
-                                   (const double*)(w.small + L.TOL), w.small + L.inv, w.hd);
+                hipLaunchKernelGGL(gm_givens_kernel, dim3(1), dim3(kBlock), 0, s, j, mm, k, kGB,
+                                   (const double*)w.partial, w.tot, w.small + L.H, w.small + L.cs, w.small + L.sn,
+                                   w.small + L.beta, (const double*)(w.small + L.TOL), w.small + L.inv, w.hd);
                 hipLaunchKernelGGL((gm_scale_kernel<T>), dim3(ge), dim3(kBlock), 0, s, n, k, q,
                                    (const double*)(w.small + L.inv), (const int*)&w.hd->done);
             }
