@@ -29,6 +29,8 @@
 // Every kernel returns at once when the device `done` flag is set, so the host
 // enqueues steps in batches and polls between batches.
 
+#include <string>
+
 #include "common.hpp"
 #include "krylov.hpp"
 
@@ -37,6 +39,7 @@ namespace tg {
 constexpr int kGmresMaxCols = 32;  // nrhs limit (LDS of the multi-dot)
 constexpr int kGmresMaxM = 63;     // restart length limit (m + 1 basis vectors <= 64)
 constexpr int kGB = 512;           // workgroups of the vector kernels
+static int g_gmres_graph = 1;      // 1: cycles after the first run as one captured hipGraph
 
 struct GmresHdr {
     int iter;
@@ -562,12 +565,13 @@ int gmres_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d,
     GmresLayout L(mm, k);
     const T* dd = prec_kind == 1 ? d : nullptr;
     // v = P u: device scaling (identity / diagonal) or the host callback
-    auto applyP = [&](const T* u_, T* v_, const int* skip) -> int {
-        if (prec_kind == 2) return op_apply(Pop, u_, v_, k, skip, s);
-        hipLaunchKernelGGL((gm_prec_kernel<T>), dim3(grid_elems((int64_t)n * k)), dim3(kBlock), 0, s, n, k, u_, dd, v_,
-                           skip);
+    auto applyP_on = [&](hipStream_t st, const T* u_, T* v_, const int* skip) -> int {
+        if (prec_kind == 2) return op_apply(Pop, u_, v_, k, skip, st);
+        hipLaunchKernelGGL((gm_prec_kernel<T>), dim3(grid_elems((int64_t)n * k)), dim3(kBlock), 0, st, n, k, u_, dd,
+                           v_, skip);
         return TG_OK;
     };
+    auto applyP = [&](const T* u_, T* v_, const int* skip) -> int { return applyP_on(s, u_, v_, skip); };
     const int64_t N = (int64_t)n * k;
     const int64_t qstride = N;
     T* Q = (T*)w.Q;
@@ -616,6 +620,49 @@ int gmres_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d,
     if (use_guess) hipLaunchKernelGGL((gm_load_kernel<T>), dim3(ge), dim3(kBlock), 0, s, n, k, (const T*)X, ldX, sol);
     else hipLaunchKernelGGL((gm_zero_kernel<T>), dim3(ge), dim3(kBlock), 0, s, N, sol);
 
+    // Arnoldi step j on stream st (GMRES.hpp:526-601): eight launches, each returning at
+    // once when the device flag says the cycle is done.
+    auto arnoldi_step = [&](hipStream_t st, int j) -> int {
+        const int nq = j + 1;
+        T* Qj = Q + j * qstride;
+        T* q = Q + (int64_t)(j + 1) * qstride;
+        int rc;
+        if (side == 0) {
+            // z = A Q_j; q = P z (diagonal P is folded into dots1)
+            rc = op_apply(Aop, (const T*)Qj, (dd || prec_kind == 2) ? z : q, k, &w.hd->done, st);
+            if (!rc && prec_kind == 2) rc = applyP_on(st, (const T*)z, q, &w.hd->done);
+        } else {
+            rc = applyP_on(st, (const T*)Qj, z, &w.hd->done);
+            if (!rc) rc = op_apply(Aop, (const T*)z, q, k, &w.hd->done, st);
+        }
+        if (rc) return rc;
+        dots1<T>(st, n, k, kpad, nq, (const T*)Q, qstride, (const T*)z, side == 0 ? dd : nullptr, q, w.partial,
+                 (const GmresHdr*)w.hd);
+        hipLaunchKernelGGL(gm_total_addh_kernel, dim3(nq * k), dim3(kBlock), 0, st, nq, k, kGB, j, mm,
+                           (const double*)w.partial, w.tot, w.small + L.H, (const GmresHdr*)w.hd);
+        update<T>(st, n, k, kpad, nq, (const T*)Q, qstride, q, (const double*)w.tot, 0, w.partial,
+                  (const GmresHdr*)w.hd);
+        hipLaunchKernelGGL(gm_total_addh_kernel, dim3(nq * k), dim3(kBlock), 0, st, nq, k, kGB, j, mm,
+                           (const double*)w.partial, w.tot, w.small + L.H, (const GmresHdr*)w.hd);
+        update<T>(st, n, k, kpad, nq, (const T*)Q, qstride, q, (const double*)w.tot, 1, w.partial,
+                  (const GmresHdr*)w.hd);
+        hipLaunchKernelGGL(gm_givens_kernel, dim3(1), dim3(kBlock), 0, st, j, mm, k, kGB, (const double*)w.partial,
+                           w.tot, w.small + L.H, w.small + L.cs, w.small + L.sn, w.small + L.beta,
+                           (const double*)(w.small + L.TOL), w.small + L.inv, w.hd);
+        hipLaunchKernelGGL((gm_scale_kernel<T>), dim3(ge), dim3(kBlock), 0, st, n, k, q,
+                           (const double*)(w.small + L.inv), (const int*)&w.hd->done);
+        return TG_OK;
+    };
+    struct CycleGraph {
+        hipStream_t cs = nullptr;
+        hipGraphExec_t ex = nullptr;
+        ~CycleGraph() {
+            if (ex) (void)hipGraphExecDestroy(ex);
+            if (cs) (void)hipStreamDestroy(cs);
+        }
+    } cyc;
+    int cycles = 0;
+
     auto solve = [&](bool implicit_zero, int& succ_out) -> int {
         // residual: z = A x - b (or -b), x = solution   (GMRES.hpp:282-350)
         T* target = (side == 0) ? z : Q;
@@ -644,45 +691,46 @@ int gmres_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d,
         hipLaunchKernelGGL((gm_scale_kernel<T>), dim3(ge), dim3(kBlock), 0, s, n, k, Q,
                            (const double*)(w.small + L.inv), (const int*)nullptr);
         TG_LAUNCH_CHECK();
-        int j = 0, batch = host_ops ? 1 : 4;
         GmresHdr hh{};
-        while (j < m) {
-            for (int t = 0; t < batch && j < m; ++t, ++j) {
-                const int nq = j + 1;
-                T* Qj = Q + j * qstride;
-                T* q = Q + (int64_t)(j + 1) * qstride;
-                int rc;
-                if (side == 0) {
-                    // z = A Q_j; q = P z (diagonal P is folded into dots1)
-                    rc = op_apply(Aop, (const T*)Qj, (dd || prec_kind == 2) ? z : q, k, &w.hd->done, s);
-                    if (!rc && prec_kind == 2) rc = applyP((const T*)z, q, &w.hd->done);
-                } else {
-                    rc = applyP((const T*)Qj, z, &w.hd->done);
-                    if (!rc) rc = op_apply(Aop, (const T*)z, q, k, &w.hd->done, s);
+        if (!host_ops && g_gmres_graph && cycles > 0) {
+            // every cycle after the first: its m steps as one captured graph (captured
+            // at the second cycle, replayed after that; steps after convergence return
+            // at once on the device flag)
+            if (!cyc.ex) {
+                if (!cyc.cs) TG_HIP(hipStreamCreateWithFlags(&cyc.cs, hipStreamNonBlocking));
+                TG_HIP(hipStreamBeginCapture(cyc.cs, hipStreamCaptureModeThreadLocal));
+                int rc = TG_OK;
+                for (int j = 0; j < m && rc == TG_OK; ++j) rc = arnoldi_step(cyc.cs, j);
+                hipGraph_t g = nullptr;
+                const hipError_t ec = hipStreamEndCapture(cyc.cs, &g);
+                if (rc) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc;
                 }
-                if (rc) return rc;
-                dots1<T>(s, n, k, kpad, nq, (const T*)Q, qstride, (const T*)z, side == 0 ? dd : nullptr, q, w.partial,
-                         (const GmresHdr*)w.hd);
-                hipLaunchKernelGGL(gm_total_addh_kernel, dim3(nq * k), dim3(kBlock), 0, s, nq, k, kGB, j, mm,
-                                   (const double*)w.partial, w.tot, w.small + L.H, (const GmresHdr*)w.hd);
-                update<T>(s, n, k, kpad, nq, (const T*)Q, qstride, q, (const double*)w.tot, 0, w.partial,
-                          (const GmresHdr*)w.hd);
-                hipLaunchKernelGGL(gm_total_addh_kernel, dim3(nq * k), dim3(kBlock), 0, s, nq, k, kGB, j, mm,
-                                   (const double*)w.partial, w.tot, w.small + L.H, (const GmresHdr*)w.hd);
-                update<T>(s, n, k, kpad, nq, (const T*)Q, qstride, q, (const double*)w.tot, 1, w.partial,
-                          (const GmresHdr*)w.hd);
-                hipLaunchKernelGGL(gm_givens_kernel, dim3(1), dim3(kBlock), 0, s, j, mm, k, kGB,
-                                   (const double*)w.partial, w.tot, w.small + L.H, w.small + L.cs, w.small + L.sn,
-                                   w.small + L.beta, (const double*)(w.small + L.TOL), w.small + L.inv, w.hd);
-                hipLaunchKernelGGL((gm_scale_kernel<T>), dim3(ge), dim3(kBlock), 0, s, n, k, q,
-                                   (const double*)(w.small + L.inv), (const int*)&w.hd->done);
+                if (ec != hipSuccess) return fail(TG_ERR_HIP, "gmres: graph capture failed: %s", hipGetErrorString(ec));
+                const hipError_t ei = hipGraphInstantiate(&cyc.ex, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (ei != hipSuccess) {
+                    cyc.ex = nullptr;
+                    return fail(TG_ERR_HIP, "gmres: graph instantiation failed: %s", hipGetErrorString(ei));
+                }
             }
-            TG_LAUNCH_CHECK();
-            TG_HIP(hipMemcpyAsync(&hh, w.hd, sizeof(hh), hipMemcpyDeviceToHost, s));
-            TG_HIP(hipStreamSynchronize(s));
-            if (hh.done) break;
-            if (!host_ops) batch = std::min(batch * 2, 16);
+            TG_HIP(hipGraphLaunch(cyc.ex, s));
+        } else {
+            int j = 0, batch = host_ops ? 1 : 4;
+            while (j < m) {
+                for (int t = 0; t < batch && j < m; ++t, ++j) {
+                    int rc = arnoldi_step(s, j);
+                    if (rc) return rc;
+                }
+                TG_LAUNCH_CHECK();
+                TG_HIP(hipMemcpyAsync(&hh, w.hd, sizeof(hh), hipMemcpyDeviceToHost, s));
+                TG_HIP(hipStreamSynchronize(s));
+                if (hh.done) break;
+                if (!host_ops) batch = std::min(batch * 2, 16);
+            }
         }
+        ++cycles;
         TG_HIP(hipMemcpyAsync(&hh, w.hd, sizeof(hh), hipMemcpyDeviceToHost, s));
         TG_HIP(hipStreamSynchronize(s));
         succ_out = hh.succ;
@@ -736,6 +784,16 @@ template int gmres_solve_t<double>(const CsrView&, const OpRef&, int, const doub
 template int gmres_solve_t<float>(const CsrView&, const OpRef&, int, const float*, const OpRef&, int, int32_t,
                                   int32_t, double, const float*, int64_t, double, float*, int64_t, double, int32_t, int,
                                   void*, size_t, GmresResult*, hipStream_t);
+
+int gmres_tune_set(const char* key, int32_t value, bool* known) {
+    *known = std::string(key) == "gmres_graph";
+    if (!*known) return TG_OK;
+    if (value != 0 && value != 1) return fail(TG_ERR_INVALID, "tg_tune_set: gmres_graph must be 0 or 1");
+    g_gmres_graph = value;
+    return TG_OK;
+}
+
+int32_t gmres_tune_get(const char* key) { return std::string(key) == "gmres_graph" ? g_gmres_graph : -1; }
 
 }  // namespace tg
 

@@ -920,6 +920,8 @@ extern "C" int tg_tune_set(const char* key, int32_t value) {
     if (known) return rc;
     rc = bsr_tune_set(key, value, &known);
     if (known) return rc;
+    rc = gmres_tune_set(key, value, &known);
+    if (known) return rc;
     return fail(TG_ERR_INVALID, "tg_tune_set: unknown key '%s'", key);
 }
 
@@ -933,6 +935,7 @@ extern "C" int32_t tg_tune_get(const char* key) {
     if (k == "spmm_rows") return g_spmm_rows;
     if (k == "cg_fused") return cg_tune_get(key);
     if (k == "bsr_vector") return bsr_tune_get(key);
+    if (k == "gmres_graph") return gmres_tune_get(key);
     return batch_tune_get(key);
 }
 

@@ -132,3 +132,34 @@ def test_gmres_callback_operators_match_builtin(cuda, side, prec):
                                     dev(B, cuda), 0.0, X2, 1e-11, 50, side=side)
     assert ok and rs == g.RestartCount() and it == g.IterationCount()
     assert rel_err(X2.cpu().numpy(), X1.cpu().numpy()) < 1e-12
+
+
+@pytest.mark.parametrize("side", [0, 1])
+@pytest.mark.parametrize("prec", [False, True])
+@pytest.mark.parametrize("k", [1, 3])
+def test_gmres_cycle_graph_equals_eager(cuda, side, prec, k):
+    """Cycles after the first run as one captured hipGraph (knob gmres_graph): the same
+    kernels with the same arguments, so the solution, iteration and restart counts are
+    bitwise those of eager launches, over many restarts."""
+    from tensors_amd import _capi as capi
+
+    rp, ci, val, n = cd_system(60, 20.0)
+    B = gen.uniform(77 + k, n * k).reshape(n, k)
+    d = 1.0 / sp.csr_matrix((val, ci, rp), shape=(n, n)).diagonal() if prec else None
+    A = MatrixCSR(rp, ci, val, n, n)
+    out = {}
+    before = capi.lib().tg_tune_get(b"gmres_graph")
+    try:
+        for mode in (1, 0):
+            capi.call("tg_tune_set", b"gmres_graph", mode)
+            g = GMRES(n, 10, k, side=side)
+            X = torch.zeros(n, k, dtype=torch.float64, device=cuda)
+            ok = g(A, None if d is None else dev(d, cuda), 1.0, dev(B, cuda), 0.0, X, 1e-9, 400)
+            out[mode] = (ok, g.iter, g.restarts, X.cpu().numpy())
+    finally:
+        capi.call("tg_tune_set", b"gmres_graph", before)
+    assert out[1][0] and out[1][2] >= 3, out[1][:3]
+    assert out[1][:3] == out[0][:3]
+    assert np.array_equal(out[1][3], out[0][3])
+    S = sp.csr_matrix((val, ci, rp), shape=(n, n))
+    assert np.max(np.linalg.norm(B - S @ out[1][3], axis=0) / np.linalg.norm(B, axis=0)) < 1e-7
