@@ -38,7 +38,7 @@ namespace tg {
 
 constexpr int kGmresMaxCols = 32;  // nrhs limit (LDS of the multi-dot)
 constexpr int kGmresMaxM = 63;     // restart length limit (m + 1 basis vectors <= 64)
-constexpr int kGB = 512;           // workgroups of the vector kernels
+constexpr int kGB = 512;           // workgroups of the vector kernels (at most; fewer for small n)
 static int g_gmres_graph = 1;      // 1: cycles after the first run as one captured hipGraph
 
 struct GmresHdr {
@@ -468,34 +468,34 @@ static int pow2ceil_g(int v) {
 }
 
 template <typename T, int NQMAX>
-static void launch_dots1(hipStream_t s, int n, int k, int kpad, int nq, const T* Q, int64_t qs, const T* z,
+static void launch_dots1(hipStream_t s, int gb, int n, int k, int kpad, int nq, const T* Q, int64_t qs, const T* z,
                          const T* d, T* q, double* partial, const GmresHdr* hd) {
-    hipLaunchKernelGGL((gm_dots1_kernel<T, NQMAX>), dim3(kGB), dim3(kBlock), multi_lds_bytes(nq, k), s, n, k, kpad,
+    hipLaunchKernelGGL((gm_dots1_kernel<T, NQMAX>), dim3(gb), dim3(kBlock), multi_lds_bytes(nq, k), s, n, k, kpad,
                        nq, Q, qs, z, d, q, partial, hd);
 }
 template <typename T, int NQMAX>
-static void launch_update(hipStream_t s, int n, int k, int kpad, int nq, const T* Q, int64_t qs, T* q,
+static void launch_update(hipStream_t s, int gb, int n, int k, int kpad, int nq, const T* Q, int64_t qs, T* q,
                           const double* h, int norm_only, double* partial, const GmresHdr* hd) {
-    hipLaunchKernelGGL((gm_update_kernel<T, NQMAX>), dim3(kGB), dim3(kBlock), multi_lds_bytes(nq, k), s, n, k, kpad,
+    hipLaunchKernelGGL((gm_update_kernel<T, NQMAX>), dim3(gb), dim3(kBlock), multi_lds_bytes(nq, k), s, n, k, kpad,
                        nq, Q, qs, q, h, norm_only, partial, hd);
 }
 template <typename T>
-static void dots1(hipStream_t s, int n, int k, int kpad, int nq, const T* Q, int64_t qs, const T* z, const T* d,
+static void dots1(hipStream_t s, int gb, int n, int k, int kpad, int nq, const T* Q, int64_t qs, const T* z, const T* d,
                   T* q, double* partial, const GmresHdr* hd) {
-    if (nq <= 4) launch_dots1<T, 4>(s, n, k, kpad, nq, Q, qs, z, d, q, partial, hd);
-    else if (nq <= 8) launch_dots1<T, 8>(s, n, k, kpad, nq, Q, qs, z, d, q, partial, hd);
-    else if (nq <= 16) launch_dots1<T, 16>(s, n, k, kpad, nq, Q, qs, z, d, q, partial, hd);
-    else if (nq <= 32) launch_dots1<T, 32>(s, n, k, kpad, nq, Q, qs, z, d, q, partial, hd);
-    else launch_dots1<T, 64>(s, n, k, kpad, nq, Q, qs, z, d, q, partial, hd);
+    if (nq <= 4) launch_dots1<T, 4>(s, gb, n, k, kpad, nq, Q, qs, z, d, q, partial, hd);
+    else if (nq <= 8) launch_dots1<T, 8>(s, gb, n, k, kpad, nq, Q, qs, z, d, q, partial, hd);
+    else if (nq <= 16) launch_dots1<T, 16>(s, gb, n, k, kpad, nq, Q, qs, z, d, q, partial, hd);
+    else if (nq <= 32) launch_dots1<T, 32>(s, gb, n, k, kpad, nq, Q, qs, z, d, q, partial, hd);
+    else launch_dots1<T, 64>(s, gb, n, k, kpad, nq, Q, qs, z, d, q, partial, hd);
 }
 template <typename T>
-static void update(hipStream_t s, int n, int k, int kpad, int nq, const T* Q, int64_t qs, T* q, const double* h,
+static void update(hipStream_t s, int gb, int n, int k, int kpad, int nq, const T* Q, int64_t qs, T* q, const double* h,
                    int norm_only, double* partial, const GmresHdr* hd) {
-    if (nq <= 4) launch_update<T, 4>(s, n, k, kpad, nq, Q, qs, q, h, norm_only, partial, hd);
-    else if (nq <= 8) launch_update<T, 8>(s, n, k, kpad, nq, Q, qs, q, h, norm_only, partial, hd);
-    else if (nq <= 16) launch_update<T, 16>(s, n, k, kpad, nq, Q, qs, q, h, norm_only, partial, hd);
-    else if (nq <= 32) launch_update<T, 32>(s, n, k, kpad, nq, Q, qs, q, h, norm_only, partial, hd);
-    else launch_update<T, 64>(s, n, k, kpad, nq, Q, qs, q, h, norm_only, partial, hd);
+    if (nq <= 4) launch_update<T, 4>(s, gb, n, k, kpad, nq, Q, qs, q, h, norm_only, partial, hd);
+    else if (nq <= 8) launch_update<T, 8>(s, gb, n, k, kpad, nq, Q, qs, q, h, norm_only, partial, hd);
+    else if (nq <= 16) launch_update<T, 16>(s, gb, n, k, kpad, nq, Q, qs, q, h, norm_only, partial, hd);
+    else if (nq <= 32) launch_update<T, 32>(s, gb, n, k, kpad, nq, Q, qs, q, h, norm_only, partial, hd);
+    else launch_update<T, 64>(s, gb, n, k, kpad, nq, Q, qs, q, h, norm_only, partial, hd);
 }
 
 static int grid_elems(int64_t count) {
@@ -580,6 +580,9 @@ int gmres_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d,
     T* sol = (T*)w.sol;
     const int kpad = pow2ceil_g(k);
     const int ge = grid_elems(N);
+    // workgroups of the reduction kernels: enough for one row per thread, at most kGB
+    // (a 1e4-row system gets 40: fewer idle lanes and 40 partials per total, not 512)
+    const int gb = (int)std::max<int64_t>(1, std::min<int64_t>(kGB, ((int64_t)n + kBlock / kpad - 1) / (kBlock / kpad)));
 
     // b norms: x = B; z = P x (left) or z = B (right)  (GMRES.hpp:168-185)
     GmresHdr h0{};
@@ -593,9 +596,9 @@ int gmres_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d,
     } else {
         hipLaunchKernelGGL((gm_copy_kernel<T>), dim3(ge), dim3(kBlock), 0, s, N, (const T*)x, z);
     }
-    hipLaunchKernelGGL((gm_norm_kernel<T>), dim3(kGB), dim3(kBlock), multi_lds_bytes(1, k), s, n, k, kpad, (const T*)z,
+    hipLaunchKernelGGL((gm_norm_kernel<T>), dim3(gb), dim3(kBlock), multi_lds_bytes(1, k), s, n, k, kpad, (const T*)z,
                        w.partial);
-    hipLaunchKernelGGL(gm_total_kernel, dim3(k), dim3(kBlock), 0, s, 1, k, kGB, (const double*)w.partial, w.tot,
+    hipLaunchKernelGGL(gm_total_kernel, dim3(k), dim3(kBlock), 0, s, 1, k, gb, (const double*)w.partial, w.tot,
                        (const GmresHdr*)w.hd);
     hipLaunchKernelGGL(gm_start_kernel, dim3(1), dim3(kBlock), 0, s, mm, k, (const double*)w.tot, w.small, L, tol, 1,
                        w.hd);
@@ -636,17 +639,17 @@ int gmres_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d,
             if (!rc) rc = op_apply(Aop, (const T*)z, q, k, &w.hd->done, st);
         }
         if (rc) return rc;
-        dots1<T>(st, n, k, kpad, nq, (const T*)Q, qstride, (const T*)z, side == 0 ? dd : nullptr, q, w.partial,
+        dots1<T>(st, gb, n, k, kpad, nq, (const T*)Q, qstride, (const T*)z, side == 0 ? dd : nullptr, q, w.partial,
                  (const GmresHdr*)w.hd);
-        hipLaunchKernelGGL(gm_total_addh_kernel, dim3(nq * k), dim3(kBlock), 0, st, nq, k, kGB, j, mm,
+        hipLaunchKernelGGL(gm_total_addh_kernel, dim3(nq * k), dim3(kBlock), 0, st, nq, k, gb, j, mm,
                            (const double*)w.partial, w.tot, w.small + L.H, (const GmresHdr*)w.hd);
-        update<T>(st, n, k, kpad, nq, (const T*)Q, qstride, q, (const double*)w.tot, 0, w.partial,
+        update<T>(st, gb, n, k, kpad, nq, (const T*)Q, qstride, q, (const double*)w.tot, 0, w.partial,
                   (const GmresHdr*)w.hd);
-        hipLaunchKernelGGL(gm_total_addh_kernel, dim3(nq * k), dim3(kBlock), 0, st, nq, k, kGB, j, mm,
+        hipLaunchKernelGGL(gm_total_addh_kernel, dim3(nq * k), dim3(kBlock), 0, st, nq, k, gb, j, mm,
                            (const double*)w.partial, w.tot, w.small + L.H, (const GmresHdr*)w.hd);
-        update<T>(st, n, k, kpad, nq, (const T*)Q, qstride, q, (const double*)w.tot, 1, w.partial,
+        update<T>(st, gb, n, k, kpad, nq, (const T*)Q, qstride, q, (const double*)w.tot, 1, w.partial,
                   (const GmresHdr*)w.hd);
-        hipLaunchKernelGGL(gm_givens_kernel, dim3(1), dim3(kBlock), 0, st, j, mm, k, kGB, (const double*)w.partial,
+        hipLaunchKernelGGL(gm_givens_kernel, dim3(1), dim3(kBlock), 0, st, j, mm, k, gb, (const double*)w.partial,
                            w.tot, w.small + L.H, w.small + L.cs, w.small + L.sn, w.small + L.beta,
                            (const double*)(w.small + L.TOL), w.small + L.inv, w.hd);
         hipLaunchKernelGGL((gm_scale_kernel<T>), dim3(ge), dim3(kBlock), 0, st, n, k, q,
@@ -677,14 +680,14 @@ int gmres_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d,
             int rcp = applyP((const T*)z, Q, nullptr);
             if (rcp) return rcp;
         }
-        hipLaunchKernelGGL((gm_norm_kernel<T>), dim3(kGB), dim3(kBlock), multi_lds_bytes(1, k), s, n, k, kpad,
+        hipLaunchKernelGGL((gm_norm_kernel<T>), dim3(gb), dim3(kBlock), multi_lds_bytes(1, k), s, n, k, kpad,
                            (const T*)Q, w.partial);
         TG_LAUNCH_CHECK();
         // the totals kernel returns early while `done` is set: clear it first
         GmresHdr clr{};
         clr.m = m;
         TG_HIP(hipMemcpyAsync(w.hd, &clr, sizeof(clr), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(gm_total_kernel, dim3(k), dim3(kBlock), 0, s, 1, k, kGB, (const double*)w.partial, w.tot,
+        hipLaunchKernelGGL(gm_total_kernel, dim3(k), dim3(kBlock), 0, s, 1, k, gb, (const double*)w.partial, w.tot,
                            (const GmresHdr*)w.hd);
         hipLaunchKernelGGL(gm_start_kernel, dim3(1), dim3(kBlock), 0, s, mm, k, (const double*)w.tot, w.small, L, tol,
                            0, w.hd);
