@@ -145,6 +145,9 @@ int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, doubl
  *                  four vector kernels, 1 = p.u in the SpMM epilogue, 2 = p
  *                  update, SpMM and p.u in one kernel plus one vector kernel,
  *                  3 = default (form 1).  Results agree to rounding.
+ *   "gmres_graph": tg_gmres_solve with device operators, 1 = every restart cycle
+ *                  after the first runs as one captured hipGraph (default), 0 =
+ *                  eager launches.  Bitwise the same results.
  *   "bsr_vector":  tg_bsr_spmm with row-major X / Y chunks, 1 = a lane per
  *                  (block row, 1-4 right-hand sides) with vector loads (default),
  *                  0 = a lane per output entry (the mapping of the other
