@@ -85,6 +85,8 @@ def parse():
                     help="block-CSR lane mapping (tg_tune_set bsr_vector: 1 vector, 0 scalar; -1 default)")
     ap.add_argument("--no-mm", action="store_true", help="skip the MatrixMarket I/O section")
     ap.add_argument("--no-gmres", action="store_true", help="skip the single-system GMRES section")
+    ap.add_argument("--serve-gmres-workers", type=int, default=6,
+                    help="GMRES worker threads (own streams) of the serving loop")
     ap.add_argument("--serve-requests", type=int, default=160)
     ap.add_argument("--serve-rates", type=str, default="0.25,0.5,1,2",
                     help="arrival rates as multiples of the measured saturated service rate")
@@ -488,7 +490,7 @@ def serve_section(args, dev):
         return srv.submit_gmres(rp, ci, val, B, tol=1e-8, restart=30, max_restarts=100)
 
     def run(rate, seed):
-        srv = Server(budget, 3, args.batch_iters)
+        srv = Server(budget, args.serve_gmres_workers, args.batch_iters)
         rng = np.random.default_rng(seed)
         gaps = rng.exponential(1.0 / rate, len(reqs)) if rate > 0 else np.zeros(len(reqs))
         arrivals = np.cumsum(gaps) - gaps[0]
@@ -544,7 +546,8 @@ def serve_section(args, dev):
         "workload": f"{len(reqs)} requests: {n_cg} SPD CG (weighted 2D/3D grids, n log-uniform [1e3, 2e5], "
                     f"1-8 RHS, Jacobi, FP32 inner + FP64 refinement to 1e-10) and {len(reqs) - n_cg} GMRES(30) "
                     "(2D upwind convection-diffusion, n log-uniform [1e4, 1e6], Peclet log-uniform [1, 100], "
-                    "P = I, 1e-8, 100 restarts); host arrays in/out; 200 GB admission budget; 3 GMRES streams",
+                    f"P = I, 1e-8, 100 restarts); host arrays in/out; 200 GB admission budget; "
+                    f"{args.serve_gmres_workers} GMRES streams",
         "generation_s": round(t_gen, 2),
         "saturated": sat,
         "poisson_sweep": sweep,
