@@ -127,6 +127,8 @@ int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, doubl
  *                  per lane (default), 1 = streaming chunks (measured slower).
  *   "batch_split": batched iteration, 1 = direction pass + SpMM of p + update
  *                  (default), 0 = fused direction and SpMM (bitwise the same).
+ *                  The split form defers x += alpha p to the next direction pass:
+ *                  change this knob only while no batch has systems in flight.
  *   "batch_graph": 1 = each tg_batch_step runs its iterations as one captured
  *                  hipGraph (re-captured when the set of active systems changes;
  *                  default), 0 = eager launches.  Bitwise the same results.
