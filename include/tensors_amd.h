@@ -321,9 +321,11 @@ int tg_csr_spgemm(int32_t m, int32_t k, int32_t n, const void* Arp, const int32_
  * All active systems advance together (three launches per CG iteration cover
  * every system's row tiles); a system's arithmetic does not depend on the
  * others, so its result is bitwise the same alone and in any mix.  Systems may be added
- * between steps.  The caller's arrays must stay valid until the system has
- * finished; tg_batch_release frees the system's workspace.  Calls synchronise
- * the batch's stream.
+ * between steps; tg_batch_add_cg only enqueues the system's setup (it does not wait
+ * for the GPU), and n * nrhs * 8 bytes must be below 4 GiB (the engine addresses its
+ * vectors by 32-bit offsets; larger systems belong to tg_cg_solve).  The caller's
+ * arrays must stay valid until the system has finished; tg_batch_release frees the
+ * system's workspace.  tg_batch_step synchronises the batch's stream.
  * ------------------------------------------------------------------------ */
 typedef struct tg_batch_s* tg_batch_t;
 int tg_batch_create(tg_batch_t* out, void* stream);
