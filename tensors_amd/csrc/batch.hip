@@ -77,6 +77,7 @@ struct SysState {
     int it_b;     // completed iterations, written by K2 (tile 0), read by K1
     int paused;   // done because of a checkpoint (written by K1 tile 0, read by the host)
     int resumed;  // iteration at which the host last resumed the system (no pause there)
+    int restart;  // iteration at which the direction restarts (p = z, as at iteration 0), or -1
 };
 
 struct Tile {
@@ -202,6 +203,7 @@ __global__ __launch_bounds__(kBlock) void bk_setup_kernel(int slot, const SysDev
         ss.it_b = 0;
         ss.paused = 0;
         ss.resumed = 0;
+        ss.restart = -1;
         ss.nan = nanf;
         ss.succ = nanf ? 0 : flag;
         ss.done = (nanf || flag || ss.max_iter <= 0) ? 1 : 0;
@@ -434,16 +436,17 @@ __global__ __launch_bounds__(kBlock) void bk_dir_spmm_kernel(const Tile* __restr
     if (tl.t == 0 && threadIdx.x == 0) ss.it_a = it;
     double acc = 0.0;
     if (streaming) {
-        acc = k1_stream<T>(S, tl, c, ro, rows, it == 0, (T)(rz / rz_prev), pick(S.p, it + 1), pick(S.p, it));
+        acc = k1_stream<T>(S, tl, c, ro, rows, it == 0 || it == ss.restart, (T)(rz / rz_prev), pick(S.p, it + 1),
+                           pick(S.p, it));
     } else if (c < S.k) {
         const T be = (T)(rz / rz_prev);
         const T* pold = pick(S.p, it + 1);
         T* pnew = pick(S.p, it);
         if (S.d)
-            acc = it == 0 ? k1_rows<T, true, true>(S, tl, c, ro, rows, be, pold, pnew)
+            acc = (it == 0 || it == ss.restart) ? k1_rows<T, true, true>(S, tl, c, ro, rows, be, pold, pnew)
                           : k1_rows<T, true, false>(S, tl, c, ro, rows, be, pold, pnew);
         else
-            acc = it == 0 ? k1_rows<T, false, true>(S, tl, c, ro, rows, be, pold, pnew)
+            acc = (it == 0 || it == ss.restart) ? k1_rows<T, false, true>(S, tl, c, ro, rows, be, pold, pnew)
                           : k1_rows<T, false, false>(S, tl, c, ro, rows, be, pold, pnew);
     }
     tile_partial(acc, S.k, S.kpad, S.ppu, tl.t);
@@ -766,18 +769,21 @@ __global__ __launch_bounds__(kBlock) void bk_dir_kernel(const Tile* __restrict__
         return;
     }
     if (tl.t == 0 && threadIdx.x == 0) ss.it_a = it;
+    // p = z at iteration 0 and where the host restarted the direction (after a pause,
+    // so the deferred x update is done: xdo is false there)
+    const bool first = it == 0 || (it == ss.restart && !xdo);
     if (c < S.k) {
         const T be = (T)(rz / rz_prev);
         T* pnew = pick(S.p, it);
         if (S.d) {
-            if (it == 0)
+            if (first)
                 dir_rows<T, true, true, false>(S, tl, c, ro, rows, be, al, pold, pnew);
             else if (xdo)
                 dir_rows<T, true, false, true>(S, tl, c, ro, rows, be, al, pold, pnew);
             else
                 dir_rows<T, true, false, false>(S, tl, c, ro, rows, be, al, pold, pnew);
         } else {
-            if (it == 0)
+            if (first)
                 dir_rows<T, false, true, false>(S, tl, c, ro, rows, be, al, pold, pnew);
             else if (xdo)
                 dir_rows<T, false, false, true>(S, tl, c, ro, rows, be, al, pold, pnew);
@@ -967,6 +973,7 @@ struct Resume {
     int slot;
     int it;  // the system's completed iterations: no checkpoint pause there
     double scale;
+    int restart;  // 1: the direction restarts there (p = z)
 };
 
 // One workgroup per resumed system: tolerance scaled, pause cleared, iterating again.
@@ -976,6 +983,7 @@ __global__ void bk_resume_kernel(const Resume* __restrict__ list, SysState* __re
     if (threadIdx.x < kBatchMaxK) ss.TOL[threadIdx.x] *= rs.scale;
     if (threadIdx.x == 0) {
         ss.resumed = rs.it;
+        if (rs.restart) ss.restart = rs.it;
         ss.paused = 0;
         ss.done = 0;
         ss.succ = 0;
@@ -1194,7 +1202,9 @@ template <typename T> class CGBatch {
 
     // Continues a system the device marked done, with its tolerance scaled by `scale`.
     // Queued; applied by one launch before the next iterations (or poll).
-    void resume(int slot, int it, double scale) { resumes_.push_back(Resume{slot, it, scale}); }
+    void resume(int slot, int it, double scale, bool restart = false) {
+        resumes_.push_back(Resume{slot, it, scale, restart ? 1 : 0});
+    }
 
     void mark_done_host(int slot) {
         if (slot >= 0 && slot < (int)parked_.size()) parked_[slot] = 1;
@@ -1451,6 +1461,7 @@ __global__ __launch_bounds__(kBlock) void bm_tofloat_kernel(int64_t count, const
 }
 
 constexpr double kReplaceDrop = 0.1;  // replace when the residual estimate fell by this factor
+constexpr int kFlatChecks = 8;        // ... or did not fall at this many checkpoints in a row
 constexpr int kCheckEvery = 16;       // mixed systems pause every 16 iterations for that test
 // knobs "batch_replace_drop_pct" / "batch_check_every": both are read when a system is added
 static double g_replace_drop = kReplaceDrop;
@@ -1474,6 +1485,9 @@ struct BatchSystem {
     // mixed
     int replacements = 0;
     double est_last = 1.0;   // preconditioned residual estimate at the last replacement
+    double est_check = 0.0;  // the estimate at the previous checkpoint
+    int flat = 0;            // consecutive checkpoints at which the estimate did not fall
+    bool restart_next = false;  // the pending replacement was forced by stagnation: restart p
     double replace_drop = kReplaceDrop;
     int check_every = kCheckEvery;
     char* mem = nullptr;     // a32, d32, b32, x32, norm partials, norms
@@ -1722,11 +1736,11 @@ static int batch_advance(Batch& B, int iters) {
         else
             B.b64.park(S.slot);
     };
-    const auto resume = [&](BatchSystem& S, int it, double scale) {
+    const auto resume = [&](BatchSystem& S, int it, double scale, bool restart) {
         if (S.mixed == 1)
-            B.b32.resume(S.slot, it, scale);
+            B.b32.resume(S.slot, it, scale, restart);
         else
-            B.b64.resume(S.slot, it, scale);
+            B.b64.resume(S.slot, it, scale, restart);
     };
     for (auto& kv : B.sys) {
         BatchSystem& S = *kv.second;
@@ -1756,8 +1770,15 @@ static int batch_advance(Batch& B, int iters) {
         if (!at_check && !stopped) continue;
         double est = 0.0;
         for (int c = 0; c < S.k; ++c) est = std::max(est, st.bsq[c] > 0 ? std::sqrt(std::fabs(st.rho[c]) / st.bsq[c]) : 0.0);
-        if (!stopped && !(est < S.replace_drop * S.est_last)) {
-            resume(S, st.it_b, 1.0);
+        // a replacement when the estimate fell replace_drop-fold since the last one, or
+        // when it stagnated (no decrease at kFlatChecks checkpoints in a row: the FP32
+        // recursion at its rounding floor, which only a fresh true residual moves on)
+        const bool flat_now = !stopped && S.est_check > 0.0 && !(est < S.est_check);
+        S.flat = flat_now ? S.flat + 1 : 0;
+        S.est_check = est;
+        S.restart_next = !stopped && !(est < S.replace_drop * S.est_last);  // forced by stagnation
+        if (S.restart_next && S.flat < kFlatChecks) {
+            resume(S, st.it_b, 1.0, false);
             continue;
         }
         if (S.mixed == 1)
@@ -1765,6 +1786,8 @@ static int batch_advance(Batch& B, int iters) {
         else
             B.rb.add(B.b64.desc(S.slot), B.dnorms64 + (size_t)S.slot * kBatchMaxK, S, st.it_b);
         S.est_last = est;
+        S.flat = 0;
+        S.est_check = 0.0;  // the replaced residual's estimate is not known here: start over
     }
     if (B.rb.sys.empty()) return TG_OK;
     rc = B.rb.run(B.s);
@@ -1796,9 +1819,11 @@ static int batch_advance(Batch& B, int iters) {
         } else if (st.done && !st.paused) {
             // the engine's estimate met the tolerance but the true residual did not: tighten it
             const double f = S.tol / worst;
-            resume(S, st.it_b, std::min(0.25, std::max(1e-4, 0.25 * f * f)));
+            resume(S, st.it_b, std::min(0.25, std::max(1e-4, 0.25 * f * f)), false);
         } else {
-            resume(S, st.it_b, 1.0);
+            // a replacement forced by stagnation also restarts the direction (p = z): the
+            // old p no longer spans anything new (a small system's Krylov space is used up)
+            resume(S, st.it_b, 1.0, S.restart_next);
         }
         if (rc) return rc;
     }

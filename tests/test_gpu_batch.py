@@ -255,3 +255,44 @@ def test_spmm_kernel_forms_are_bitwise_equal(cuda, mixed, tile_rows):
         for i in range(len(sysl)):
             assert r[i]["succeeded"] and r[i]["iterations"] == r0[i]["iterations"], (form, i)
             assert np.array_equal(X[i], X0[i]), (form, i)
+
+
+def tiny_system(n, k, seed):
+    """A small SPD tridiagonal system (n = 1 is the 1 x 1 matrix [2])."""
+    A = sp.diags([np.full(n - 1, -1.0), np.full(n, 2.0 + 0.1 * seed), np.full(n - 1, -1.0)], [-1, 0, 1],
+                 format="csr") if n > 1 else sp.csr_matrix(np.array([[2.0]]))
+    A.sort_indices()
+    B = gen.uniform(seed, n * k).reshape(n, k)
+    return (A.indptr.astype(np.int64), A.indices.astype(np.int32), A.data, 1.0 / A.diagonal(), B, n, k)
+
+
+@pytest.mark.parametrize("mixed", [0, 1, 2])
+def test_batch_edge_cases(cuda, mixed):
+    """Tiny systems (n = 1, 2, 3; 1 and 8 right-hand sides), a zero right-hand side (done at
+    setup, X = 0), a right-hand side with a NaN (reported as failed, nothing iterated) and
+    max_iter = 0, all in one batch, next to an ordinary system."""
+    sysl = [tiny_system(1, 1, 1), tiny_system(2, 8, 2), tiny_system(3, 3, 3)] + systems(1, seed=5)
+    zero = list(tiny_system(40, 2, 4))
+    zero[4] = np.zeros_like(zero[4])
+    nanb = list(tiny_system(40, 2, 5))
+    nanb[4] = nanb[4].copy()
+    nanb[4][7, 1] = np.nan
+    capped = tiny_system(40, 2, 6)
+    bt = BatchCG()
+    devs = [to_dev(s, cuda) for s in sysl + [tuple(zero), tuple(nanb), capped]]
+    ids = []
+    for j, (rp, ci, val, d, B, X) in enumerate(devs):
+        ids.append(bt.add(rp, ci, val, B, X, 1e-10, 0 if j == len(devs) - 1 else 20000, diag=d, mixed=mixed == 1,
+                          true_residual=mixed == 2))
+    bt.run(8)
+    res = [bt.result(sid) for sid in ids]
+    Xs = [dv[5].cpu().numpy() for dv in devs]
+    bt.close()
+    for j, (rp, ci, val, d, B, n, k) in enumerate(sysl):
+        assert res[j]["succeeded"], (j, res[j])
+        A = sp.csr_matrix((val, ci, rp), shape=(n, n))
+        assert np.max(np.linalg.norm(B - A @ Xs[j], axis=0) / np.linalg.norm(B, axis=0)) < 1.5e-10
+    z, nb, cap = len(sysl), len(sysl) + 1, len(sysl) + 2
+    assert res[z]["succeeded"] and res[z]["iterations"] == 0 and not np.any(Xs[z])
+    assert not res[nb]["succeeded"] and res[nb]["iterations"] == 0
+    assert not res[cap]["succeeded"] and res[cap]["iterations"] == 0
