@@ -1817,9 +1817,11 @@ static int batch_advance(Batch& B, int iters) {
             park(S);
             rc = finish(B, S, 0, st.it_b);
         } else if (st.done && !st.paused) {
-            // the engine's estimate met the tolerance but the true residual did not: tighten it
+            // the engine's estimate met the tolerance but the true residual did not: tighten
+            // it and restart the direction (the old p has vanished; the replaced residual's
+            // larger r.z over its tiny p.u would throw x far off)
             const double f = S.tol / worst;
-            resume(S, st.it_b, std::min(0.25, std::max(1e-4, 0.25 * f * f)), false);
+            resume(S, st.it_b, std::min(0.25, std::max(1e-4, 0.25 * f * f)), true);
         } else {
             // a replacement forced by stagnation also restarts the direction (p = z): the
             // old p no longer spans anything new (a small system's Krylov space is used up)

@@ -146,7 +146,9 @@ __global__ __launch_bounds__(kFinalThreads) void cg_alpha_kernel(int k, int kpad
     if (st->done) return;
     const int c = threadIdx.x % kpad, j = threadIdx.x / kpad, J = kFinalThreads / kpad;
     const double pu = column_total(partial, B, k, kpad, c, j, J);
-    if (j == 0 && c < k) st->alpha[c] = st->rho[c] / pu;
+    // guard: a column whose direction has vanished (p.u <= 0, an FP32 iteration past
+    // the point its Krylov space is used up) stops moving instead of producing NaN
+    if (j == 0 && c < k) st->alpha[c] = (st->guard && !(pu > 0.0)) ? 0.0 : st->rho[c] / pu;
 }
 
 template <typename T>
@@ -240,7 +242,7 @@ __global__ __launch_bounds__(kFinalThreads) void cg_rho_kernel(int k, int kpad, 
         const double old = st->rho[c];
         st->rho_old[c] = old;
         st->rho[c] = rz;
-        st->beta[c] = rz / old;
+        st->beta[c] = (st->guard && !(old != 0.0)) ? 0.0 : rz / old;  // guard: a column whose r vanished
         if (!(fabs(rz) <= st->TOL[c])) atomicAnd(&conv, 0);
     }
     __syncthreads();
@@ -683,6 +685,7 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
     CGState h{};
     h.max_iter = max_iter;
     h.use_guess = use_guess ? 1 : 0;
+    h.guard = hook ? hook->guard : 0;
     TG_HIP(hipMemcpyAsync(st, &h, sizeof(CGState), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL((cg_init_kernel<T>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, Bm, ldB, dd, r, z, x,
                        use_guess ? 0 : 1, zmode, partial);
@@ -739,7 +742,7 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
         TG_HIP(hipMemcpyAsync(&hs, st, sizeof(CGState), hipMemcpyDeviceToHost, s));
         TG_HIP(hipStreamSynchronize(s));
         bool stop = false;
-        rc = hook->fn(hs, r, z, x, partial, st, &stop);
+        rc = hook->fn(hs, r, z, x, fused ? nullptr : p, partial, st, &stop);
         if (rc) return rc;
         if (!stop) {
             TG_HIP(hipMemcpyAsync(hflag, &st->iter, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -806,7 +809,7 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
                 break;
             }
             bool stop = false;
-            rc = hook->fn(hs, r, z, x, partial, st, &stop);
+            rc = hook->fn(hs, r, z, x, fused ? nullptr : p, partial, st, &stop);
             if (rc || stop) break;
             if (hipMemcpyAsync(hflag, &st->iter, 2 * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
                 hipStreamSynchronize(s) != hipSuccess) {

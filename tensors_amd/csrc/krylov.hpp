@@ -81,7 +81,8 @@ struct CGState {
     int max_iter;
     int use_guess;
     int xpend;  // the last iteration's x += alpha p is still to be applied (it stopped before the p step)
-    int pad[10];
+    int guard;  // alpha = 0 where p.u <= 0 (set by the mixed solver for its FP32 iteration)
+    int pad[9];
     double rho[kMaxCols];
     double rho_old[kMaxCols];
     double alpha[kMaxCols];
@@ -120,7 +121,12 @@ size_t cg_workspace_bytes(int32_t n, int32_t k, size_t vs, int prec_kind);
 // stopped (hs.done) and fn neither stops nor clears `done`, the solve ends.
 struct CGHook {
     int every = 16;
-    std::function<int(const CGState& hs, void* r, void* z, void* x, double* partial, CGState* st, bool* stop)> fn;
+    int guard = 0;  // 1: alpha = 0 for a column whose p.u is not positive (its p has vanished)
+    // fn(state, r, z, x, p, partials, device state, stop): p is null where the direction
+    // is double-buffered (form 2)
+    std::function<int(const CGState& hs, void* r, void* z, void* x, void* p, double* partial, CGState* st,
+                      bool* stop)>
+        fn;
 };
 
 template <typename T>

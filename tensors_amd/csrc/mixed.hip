@@ -150,7 +150,7 @@ __global__ __launch_bounds__(kFinalThreads) void mx_replace_state_kernel(int k, 
     const double tot = column_total(partial, B, k, kpad, c, j, J);
     if (j == 0 && c < k) {
         st->rho[c] = tot;
-        if (st->iter > 0) st->beta[c] = tot / st->rho_old[c];
+        if (st->iter > 0) st->beta[c] = st->rho_old[c] != 0.0 ? tot / st->rho_old[c] : 0.0;
         st->TOL[c] *= scale;
     }
     if (threadIdx.x == 0) {
@@ -170,6 +170,7 @@ static int grid_m(int64_t count) {
 
 constexpr int kReplaceCheck = 16;     // iterations between residual-replacement checks
 constexpr double kReplaceDrop = 0.1;  // default: replace when the FP32 residual estimate fell 10x
+constexpr int kFlatChecks = 8;        // ... or when it did not fall at this many checks in a row
 
 struct MixedWs {
     double* partial;
@@ -307,15 +308,27 @@ static int cg_mixed(const CsrView& A64, const float* a32_in, int prec_kind, cons
     bool ok = worst_rel() <= tol;
     int replacements = 0;
     if (!ok) {
-        double est_last = 1.0;
+        double est_last = 1.0, est_check = 0.0;
+        int flat = 0;  // consecutive checks at which the estimate did not fall
         const double drop = (replace_drop > 0.0 && replace_drop < 1.0) ? replace_drop : kReplaceDrop;
         CGHook hook;
         hook.every = kReplaceCheck;
-        hook.fn = [&](const CGState& hs, void* r, void* z, void* d, double* partial, CGState* st, bool* stop) -> int {
+        hook.guard = 1;  // a column whose FP32 direction vanished stops instead of turning NaN
+        hook.fn = [&](const CGState& hs, void* r, void* z, void* d, void* p, double* partial, CGState* st,
+                      bool* stop) -> int {
             double est = 0.0;
             for (int c = 0; c < k; ++c) est = std::max(est, hs.bsq[c] > 0 ? std::sqrt(std::fabs(hs.rho[c]) / hs.bsq[c]) : 0.0);
             const bool stopped = hs.done != 0;
-            if (!stopped && !(est < drop * est_last)) return TG_OK;
+            // a replacement when the estimate fell drop-fold since the last one, when the
+            // FP32 iteration stopped, or when the estimate did not fall at kFlatChecks checks
+            // in a row (the FP32 recursion at its rounding floor, or a small system's
+            // Krylov space used up); the last one also restarts the direction
+            flat = (!stopped && est_check > 0.0 && !(est < est_check)) ? flat + 1 : 0;
+            est_check = est;
+            const bool forced = !stopped && !(est < drop * est_last);
+            if (forced && flat < kFlatChecks) return TG_OK;
+            flat = 0;
+            est_check = 0.0;
             // fold the FP32 correction into x64, then the true residual
             hipLaunchKernelGGL(mx_flush_kernel, dim3(grid_m(N)), dim3(kBlock), 0, s, N, (float*)d, w.x64);
             int rc2 = residual(false);
@@ -337,6 +350,11 @@ static int cg_mixed(const CsrView& A64, const float* a32_in, int prec_kind, cons
             TG_HIP(hipMemcpyAsync(r, w.r32, (size_t)N * 4, hipMemcpyDeviceToDevice, s));
             hipLaunchKernelGGL(mx_rz_kernel, dim3(Bk), dim3(kBlock), 0, s, n, k, kpad, (const float*)r,
                                (const float*)w.diag32, (float*)z, partial);
+            // restart (p = z of the replaced residual) after a stop or a stagnation: there
+            // the old direction has vanished, and the replaced residual's larger r.z over
+            // its tiny p.u would throw x far off (p is null in form 2)
+            if ((forced || stopped) && p)
+                TG_HIP(hipMemcpyAsync(p, z, (size_t)N * 4, hipMemcpyDeviceToDevice, s));
             double scale = 1.0;
             if (stopped) {
                 const double f = tol / worst;

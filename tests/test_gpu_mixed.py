@@ -80,3 +80,21 @@ def test_replacement_keeps_the_krylov_process(cuda):
     Xg = Xm.clone() * (1 + 1e-6)
     ok, _, it_g, rr = cg_solve_mixed(A, B, Xg, tol=1e-10, use_initial_guess=True)
     assert ok and it_g < it32 and np.all(rr <= 1e-10)
+
+
+@pytest.mark.parametrize("n,k", [(1, 1), (2, 8), (3, 3), (5, 32)])
+@pytest.mark.parametrize("prec", [False, True])
+def test_mixed_tiny_systems(cuda, n, k, prec):
+    """Systems smaller than a handful of CG iterations: the FP32 iteration uses up its Krylov
+    space at once, and the solve must still reach the FP64 true residual."""
+    A = sp.diags([np.full(n - 1, -1.0), np.full(n, 2.3), np.full(n - 1, -1.0)], [-1, 0, 1], format="csr") \
+        if n > 1 else sp.csr_matrix(np.array([[2.0]]))
+    A.sort_indices()
+    B = gen.uniform(31 + n, n * k).reshape(n, k)
+    M = MatrixCSR(A.indptr.astype(np.int64), A.indices.astype(np.int32), A.data, n, n)
+    P = torch.from_numpy(1.0 / A.diagonal()).to(cuda) if prec else None
+    X = torch.zeros(n, k, dtype=torch.float64, device=cuda)
+    ok, outer, inner, rr = cg_solve_mixed(M, torch.from_numpy(B).to(cuda), X, tol=1e-10, P=P)
+    assert ok, (outer, inner, rr)
+    Xh = X.cpu().numpy()
+    assert np.max(np.linalg.norm(B - A @ Xh, axis=0) / np.linalg.norm(B, axis=0)) <= 1.5e-10
