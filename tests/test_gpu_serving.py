@@ -98,3 +98,42 @@ def test_serving_refuses_request_over_budget(cuda):
     r = srv.drain([sid], 60)[sid]
     srv.close()
     assert r["status"] == capi.ERR_WORKSPACE
+
+
+def test_serving_edge_requests(cuda):
+    """Tiny CG requests (n = 1 and 2, up to 8 right-hand sides), a tiny GMRES request (3 x 3
+    grid), and CG / GMRES requests whose right-hand side holds a NaN, in one server: the tiny
+    ones reach their tolerance, the NaN ones come back as not succeeded with status 0, and
+    nothing stays admitted."""
+    def tri(n):
+        A = sp.diags([np.full(n - 1, -1.0), np.full(n, 2.2), np.full(n - 1, -1.0)], [-1, 0, 1], format="csr") \
+            if n > 1 else sp.csr_matrix(np.array([[2.0]]))
+        A.sort_indices()
+        return A.indptr.astype(np.int64), A.indices.astype(np.int32), A.data, 1.0 / A.diagonal()
+
+    reqs = []
+    for n, k in ((1, 1), (2, 8), (3, 3)):
+        rp, ci, val, d = tri(n)
+        reqs.append(("cg", rp, ci, val, gen.uniform(70 + n, n * k).reshape(n, k), d))
+    rp, ci, val = gen.convection_diffusion_2d(3, 5.0)
+    reqs.append(("gmres", rp, ci, val, gen.uniform(80, 9).reshape(9, 1), None))
+    rp, ci, val, d = tri(50)
+    Bn = gen.uniform(81, 100).reshape(50, 2)
+    Bn[7, 1] = np.nan
+    reqs.append(("cg", rp, ci, val, Bn, d))
+    rp, ci, val = gen.convection_diffusion_2d(10, 5.0)
+    Bg = gen.uniform(82, 100).reshape(100, 1)
+    Bg[3, 0] = np.nan
+    reqs.append(("gmres", rp, ci, val, Bg, None))
+    res, stats = solve_all(reqs, workers=2)
+    assert stats["bytes_in_use"] == 0 and stats["running"] == 0
+    for j, ((kind, rp, ci, val, B, d), r) in enumerate(zip(reqs, res)):
+        assert r["status"] == 0, (j, r["status"])
+        if j < 4:
+            n = len(rp) - 1
+            A = sp.csr_matrix((val, ci, rp), shape=(n, n))
+            true = np.linalg.norm(B - A @ r["X"], axis=0) / np.linalg.norm(B, axis=0)
+            assert r["succeeded"], (j, r)
+            assert np.all(true <= (1.5e-10 if kind == "cg" else 1.5e-8)), (j, true)
+        else:
+            assert not r["succeeded"], (j, r)
