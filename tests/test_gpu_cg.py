@@ -192,3 +192,24 @@ def test_cg_fused_iteration_matches_four_kernel_path(cuda, fused_knob, form, k, 
         ok_o, it_o, _ = oracle.cg(n, rp, ci, val, B, Xo, k, 1000, tol, prec_diag=d)
         assert ok_o == 1 and abs(it1 - it_o) <= 1
         assert rel_err(X1, Xo) < 1e-8
+
+
+@pytest.mark.parametrize("n,k", [(1, 1), (1, 4), (2, 3), (3, 8)])
+@pytest.mark.parametrize("prec", [False, True])
+def test_cg_tiny_systems(cuda, n, k, prec):
+    """Systems of 1-3 rows (the Krylov space is used up after n iterations): the drop-in CG
+    and the oracle take the same iterations to the same solution."""
+    A = sp.diags([np.full(n - 1, -1.0), np.full(n, 2.4), np.full(n - 1, -1.0)], [-1, 0, 1], format="csr") \
+        if n > 1 else sp.csr_matrix(np.array([[2.0]]))
+    A.sort_indices()
+    rp, ci, val = A.indptr.astype(np.int64), A.indices.astype(np.int32), A.data
+    B = gen.uniform(17 + n + k, n * k).reshape(n, k)
+    d = 1.0 / A.diagonal() if prec else None
+    cg = ConjugateGradient(n, 50, k)
+    X = torch.zeros(n, k, dtype=torch.float64, device=cuda)
+    ok = cg(MatrixCSR(rp, ci, val, n, n), None if d is None else dev(d, cuda, torch.float64), 1.0,
+            dev(B, cuda, torch.float64), 0.0, X, 1e-10)
+    Xo = np.zeros((n, k))
+    ok_o, it_o, _ = oracle.cg(n, rp, ci, val, B, Xo, k, 50, 1e-10, prec_diag=d)
+    assert bool(ok) == bool(ok_o) and abs(cg.IterationCount() - it_o) <= 1
+    assert rel_err(X.cpu().numpy(), np.linalg.solve(A.toarray(), B)) < 1e-9

@@ -163,3 +163,20 @@ def test_gmres_cycle_graph_equals_eager(cuda, side, prec, k):
     assert np.array_equal(out[1][3], out[0][3])
     S = sp.csr_matrix((val, ci, rp), shape=(n, n))
     assert np.max(np.linalg.norm(B - S @ out[1][3], axis=0) / np.linalg.norm(B, axis=0)) < 1e-7
+
+
+@pytest.mark.parametrize("n", [1, 2, 4])
+@pytest.mark.parametrize("side", [0, 1])
+def test_gmres_tiny_systems(cuda, n, side):
+    """GMRES(30) on systems of 1-4 rows (the Krylov space is used up after n steps, before the
+    restart length): the dense solution, in at most n steps, without restarts."""
+    A = sp.diags([np.full(n - 1, -1.3), np.full(n, 3.0), np.full(n - 1, -0.7)], [-1, 0, 1], format="csr") \
+        if n > 1 else sp.csr_matrix(np.array([[3.0]]))
+    A.sort_indices()
+    rp, ci, val = A.indptr.astype(np.int64), A.indices.astype(np.int32), A.data
+    B = gen.uniform(23 + n, n * 2).reshape(n, 2)
+    g = GMRES(n, 30, 2, side=side)
+    X = torch.zeros(n, 2, dtype=torch.float64, device=cuda)
+    assert g(MatrixCSR(rp, ci, val, n, n), None, 1.0, dev(B, cuda), 0.0, X, 1e-10, 5)
+    assert g.RestartCount() == 0 and g.IterationCount() <= n
+    assert rel_err(X.cpu().numpy(), np.linalg.solve(A.toarray(), B)) < 1e-9
