@@ -14,6 +14,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -138,6 +139,20 @@ template <typename T> class Buffer {
   private:
     T* p_ = nullptr;
     Size_T n_ = 0;
+};
+
+// A mutex owned by one object: a copied or moved object gets a fresh, unlocked
+// one.  It guards a matrix's lazily built device mirror and its staging buffers,
+// so that const calls on one matrix from several host threads (the first of
+// which uploads the mirror) run one after another instead of racing.
+struct OwnMutex {
+    OwnMutex() = default;
+    OwnMutex(const OwnMutex&) {}
+    OwnMutex& operator=(const OwnMutex&) { return *this; }
+    std::unique_lock<std::recursive_mutex> Lock() const { return std::unique_lock<std::recursive_mutex>(m); }
+
+  private:
+    mutable std::recursive_mutex m;
 };
 
 // Copies a rows x cols block with leading dimensions between host and device.

@@ -202,6 +202,7 @@ template <typename Scal_, typename Int_, typename LInt_, Parallel_T parQ_ = Para
     // ---- device mirror ---------------------------------------------------------
     static constexpr bool kRp64 = sizeof(LInt) > 4;
     void RequireDevice() const {
+        const auto lk = mirror_mutex.Lock();
         if (dev_valid) return;
         plan.Reset();  // it points at the device arrays about to be replaced
         const int64_t nnz = (int64_t)NonzeroCount();
@@ -223,6 +224,7 @@ template <typename Scal_, typename Int_, typename LInt_, Parallel_T parQ_ = Para
     // Column-slice plan of the device pattern (tg_csr_plan_create), built on first
     // use like the reference's lazily built job pointers (PatternCSR.hpp:389-398).
     tg_csr_plan_t DevicePlan() const {
+        const auto lk = mirror_mutex.Lock();
         RequireDevice();
         if (!plan.p)
             Device::Status(tg_csr_plan_create((int32_t)m, (int32_t)n, (int64_t)NonzeroCount(), DeviceRowPointers(),
@@ -360,6 +362,8 @@ template <typename Scal_, typename Int_, typename LInt_, Parallel_T parQ_ = Para
             eprint(MethodName("Dot") + ": Input and output pointer coincide. This is not safe. Aborting.");
             return;
         }
+        // the staging buffers and the plan's partial blocks belong to the matrix
+        const auto lk = mirror_mutex.Lock();
         RequireDevice();
         const int64_t nnz = (int64_t)NonzeroCount();
         const Scal* vals = d_val.data();
@@ -496,6 +500,7 @@ template <typename Scal_, typename Int_, typename LInt_, Parallel_T parQ_ = Para
         }
     };
 
+    Device::OwnMutex mirror_mutex;
     mutable bool dev_valid = false;
     mutable PlanHolder plan;
     mutable Device::Buffer<int64_t> d_rp64;

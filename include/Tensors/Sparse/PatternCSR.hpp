@@ -65,6 +65,7 @@ template <typename Int_, typename LInt_, Parallel_T parQ_ = Parallel> class Patt
 
   private:
     void RequireDevice() const {
+        const auto lk = mirror_mutex.Lock();
         if (dev_valid) return;
         if (kRp64) {
             std::vector<int64_t> r(outer.Size());
@@ -86,6 +87,7 @@ template <typename Int_, typename LInt_, Parallel_T parQ_ = Parallel> class Patt
     Int m = 0;
     Int n = 0;
     Int thread_count = 1;
+    Device::OwnMutex mirror_mutex;
     mutable bool dev_valid = false;
     mutable Device::Buffer<int64_t> d_rp64;
     mutable Device::Buffer<int32_t> d_rp32;

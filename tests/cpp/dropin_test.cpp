@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <map>
+#include <thread>
 
 #include "Tensors/Tensors.hpp"
 
@@ -175,6 +176,31 @@ int main(int argc, char** argv) {
         brute(L, 1, X.data(), k, 0, Yr.data(), k, k);
         const double e = relerr(Yh.data(), Yr.data(), (size_t)n * k);
         check(e <= 1e-12, "Dot on device pointers", e);
+    }
+    {
+        // Dot from several host threads on one fresh matrix: the first calls race
+        // for the lazy device upload and all share the staging buffers
+        CSR F = L;
+        const int T = 6;
+        std::vector<Tensor2<Real, Int>> Xs, Ys, Yr;
+        for (int t = 0; t < T; ++t) {
+            const Int k = 1 + t % 4;
+            Xs.emplace_back(n, k);
+            Ys.emplace_back(n, k);
+            Yr.emplace_back(n, k);
+            Xs[t].Random();
+            brute(L, 1, Xs[t].data(), k, 0, Yr[t].data(), k, k);
+        }
+        std::vector<std::thread> th;
+        for (int t = 0; t < T; ++t)
+            th.emplace_back([&, t] {
+                for (int rep = 0; rep < 4; ++rep) F.Dot(Real(1), Xs[t], Real(0), Ys[t]);
+            });
+        for (auto& x : th) x.join();
+        double e = 0;
+        for (int t = 0; t < T; ++t)
+            e = std::max(e, relerr(Ys[t].data(), Yr[t].data(), (size_t)n * (1 + t % 4)));
+        check(e <= 1e-12, "Dot from 6 host threads on one matrix (first use inside the race)", e);
     }
     {
         Tensor2<Real, Int> X(n, 2), snap(n, 2);
