@@ -97,7 +97,10 @@ int tg_csr_spmm(int32_t m, int32_t n, int64_t nnz, const void* rp, tg_dtype rp_t
  * the rp/ci pointers (they must stay valid) and 7 int32 per row; it needs
  * sorted columns within rows (checked when built, else the plain kernels
  * run).  tg_csr_spmm_plan has tg_csr_spmm's semantics and falls back to it
- * whenever slicing does not pay; calls on one plan must not overlap.
+ * whenever slicing does not pay.  Calls on one plan may come from several host
+ * threads and streams: they share the plan's partial blocks, so each sliced call
+ * first waits for the previous one on the device (outside stream capture; under
+ * capture the caller orders them).
  * tg_csr_plan_slices returns the slice count it would use (0 = plain). */
 typedef struct tg_csr_plan_s* tg_csr_plan_t;
 int tg_csr_plan_create(int32_t m, int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, const int32_t* ci,

@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <mutex>
 #include <string>
 
 #include "common.hpp"
@@ -1102,6 +1103,12 @@ struct tg_csr_plan_s {
     int64_t band = -1;        // max |column - row|
     void* part = nullptr;     // partial blocks of the sliced path (grown on demand)
     size_t part_bytes = 0;
+    // The partial blocks are shared by every sliced call on the plan: a call on another
+    // stream than the previous one first waits for it (event `used`), and host threads
+    // take turns.
+    hipEvent_t used = nullptr;
+    hipStream_t used_on = nullptr;
+    std::mutex mu;
 };
 
 extern "C" int tg_csr_plan_create(int32_t m, int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type,
@@ -1159,6 +1166,7 @@ extern "C" int tg_csr_plan_destroy(tg_csr_plan_t plan) {
     if (!plan) return TG_OK;
     if (plan->cuts) (void)hipFree(plan->cuts);
     if (plan->part) (void)hipFree(plan->part);
+    if (plan->used) (void)hipEventDestroy(plan->used);
     delete plan;
     return TG_OK;
 }
@@ -1194,6 +1202,13 @@ extern "C" int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_
         return spmm_dispatch(A, alpha, X, ldX, beta, Y, ldY, nrhs, nullptr, 0, nullptr, st);
     const size_t vs = val_type == TG_F64 ? 8 : 4;
     const size_t need = (size_t)S * plan->m * nrhs * vs;
+    std::lock_guard<std::mutex> lk(plan->mu);
+    // under stream capture the caller orders the uses of one plan; otherwise a call on
+    // another stream waits for the previous user of the partial blocks
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    TG_HIP(hipStreamIsCapturing(st, &cap));
+    const bool track = cap == hipStreamCaptureStatusNone;
+    if (track && plan->used && plan->used_on != st) TG_HIP(hipStreamWaitEvent(st, plan->used, 0));
     if (plan->part_bytes < need) {
         if (plan->part) TG_HIP(hipFreeAsync(plan->part, st));
         plan->part = nullptr;
@@ -1202,22 +1217,32 @@ extern "C" int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_
         plan->part_bytes = need;
     }
     const int32_t m = plan->m;
+    int rc;
     if (val_type == TG_F64) {
         if (plan->rp_type == TG_I32)
-            return spmm_sliced_typed<double, int32_t>(S, m, plan->nnz, (const int32_t*)plan->rp, plan->cuts, plan->ci,
-                                                      (const double*)a, alpha, (const double*)X, ldX, beta, (double*)Y,
-                                                      ldY, nrhs, (double*)plan->part, st);
-        return spmm_sliced_typed<double, int64_t>(S, m, plan->nnz, (const int64_t*)plan->rp, plan->cuts, plan->ci,
-                                                  (const double*)a, alpha, (const double*)X, ldX, beta, (double*)Y,
-                                                  ldY, nrhs, (double*)plan->part, st);
+            rc = spmm_sliced_typed<double, int32_t>(S, m, plan->nnz, (const int32_t*)plan->rp, plan->cuts, plan->ci,
+                                                    (const double*)a, alpha, (const double*)X, ldX, beta, (double*)Y,
+                                                    ldY, nrhs, (double*)plan->part, st);
+        else
+            rc = spmm_sliced_typed<double, int64_t>(S, m, plan->nnz, (const int64_t*)plan->rp, plan->cuts, plan->ci,
+                                                    (const double*)a, alpha, (const double*)X, ldX, beta, (double*)Y,
+                                                    ldY, nrhs, (double*)plan->part, st);
+    } else {
+        if (plan->rp_type == TG_I32)
+            rc = spmm_sliced_typed<float, int32_t>(S, m, plan->nnz, (const int32_t*)plan->rp, plan->cuts, plan->ci,
+                                                   (const float*)a, alpha, (const float*)X, ldX, beta, (float*)Y, ldY,
+                                                   nrhs, (float*)plan->part, st);
+        else
+            rc = spmm_sliced_typed<float, int64_t>(S, m, plan->nnz, (const int64_t*)plan->rp, plan->cuts, plan->ci,
+                                                   (const float*)a, alpha, (const float*)X, ldX, beta, (float*)Y, ldY,
+                                                   nrhs, (float*)plan->part, st);
     }
-    if (plan->rp_type == TG_I32)
-        return spmm_sliced_typed<float, int32_t>(S, m, plan->nnz, (const int32_t*)plan->rp, plan->cuts, plan->ci,
-                                                 (const float*)a, alpha, (const float*)X, ldX, beta, (float*)Y, ldY,
-                                                 nrhs, (float*)plan->part, st);
-    return spmm_sliced_typed<float, int64_t>(S, m, plan->nnz, (const int64_t*)plan->rp, plan->cuts, plan->ci,
-                                             (const float*)a, alpha, (const float*)X, ldX, beta, (float*)Y, ldY, nrhs,
-                                             (float*)plan->part, st);
+    if (track) {
+        if (!plan->used) TG_HIP(hipEventCreateWithFlags(&plan->used, hipEventDisableTiming));
+        TG_HIP(hipEventRecord(plan->used, st));
+        plan->used_on = st;
+    }
+    return rc;
 }
 
 extern "C" int tg_csr_spmm(int32_t m, int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, const int32_t* ci,

@@ -120,3 +120,52 @@ def test_auto_slices_large_matrix_dot(cuda, knobs, k, dtype):
                 Xh, k, 0.0, Yo, k, k)
     tol = 1e-12 if dtype == torch.float64 else 2e-6
     assert rel_err(Y.double().cpu().numpy(), Yo) <= tol
+
+
+def test_one_plan_on_two_streams_and_threads(cuda, knobs):
+    """Sliced calls on one plan from two host threads, each on its own stream, with the
+    partial blocks regrown in between (nrhs 1, then 4): every result is bitwise the one
+    the same call gives alone."""
+    import threading
+    m, n = 40000, 40000
+    rp, ci, val = random_csr_np(m, n, 2e-4, seed=77)
+    rpd, cid, vd = dev_arrays(rp, ci, val, np.float64, cuda)
+    plan = SlicePlan(m, n, len(ci), rpd, cid)
+    autotune.set_knobs({"spmm_slices": 4})
+    gen = torch.Generator(device=cuda).manual_seed(5)
+    Xs = {(t, k): torch.rand(n, k, generator=gen, device=cuda, dtype=torch.float64) for t in (0, 1) for k in (1, 4)}
+    alone = {}
+    for key, X in Xs.items():
+        Y = torch.zeros(m, key[1], dtype=torch.float64, device=cuda)
+        assert plan.spmm(vd, 1.0, X, key[1], 0.0, Y, key[1], key[1]) == 0
+        alone[key] = Y
+    torch.cuda.synchronize()
+    plan2 = SlicePlan(m, n, len(ci), rpd, cid)  # fresh: its partial blocks grow inside the race
+    streams = [torch.cuda.Stream(device=cuda) for _ in range(2)]
+    outs = {key: [torch.full((m, key[1]), float("nan"), dtype=torch.float64, device=cuda) for _ in range(6)]
+            for key in Xs}
+    torch.cuda.synchronize()
+    errors = []
+
+    def work(t):
+        try:
+            s = streams[t]
+            for rep in range(6):
+                for k in (1, 4):
+                    Y = outs[(t, k)][rep]
+                    st = plan2.spmm(vd, 1.0, Xs[(t, k)], k, 0.0, Y, k, k, stream=s)
+                    if st != 0:
+                        errors.append(st)
+        except Exception as e:  # reported below, on the test's thread
+            errors.append(e)
+
+    th = [threading.Thread(target=work, args=(t,)) for t in (0, 1)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    torch.cuda.synchronize()
+    assert not errors
+    for key, ys in outs.items():
+        for Y in ys:
+            assert torch.equal(Y, alone[key]), key
