@@ -155,6 +155,13 @@ struct OwnMutex {
     mutable std::recursive_mutex m;
 };
 
+// Copies `count` contiguous elements between host and device (synchronous).
+template <typename T>
+inline void Copy(T* dst, const T* src, Size_T count, hipMemcpyKind kind) {
+    if (count == 0) return;
+    Check(hipMemcpy(dst, src, count * sizeof(T), kind), "hipMemcpy");
+}
+
 // Copies a rows x cols block with leading dimensions between host and device.
 template <typename T>
 inline void Copy2D(T* dst, Size_T ldd, const T* src, Size_T lds, Size_T rows, Size_T cols, hipMemcpyKind kind) {

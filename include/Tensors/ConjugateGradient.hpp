@@ -27,7 +27,7 @@ class ConjugateGradient final {
     ConjugateGradient(const Int n_, const Int max_iter_, const Size_T eq_count_ = NRHS_,
                       const Size_T thread_count_ = 1)
         : n(n_), max_iter(std::max<Int>(0, max_iter_)), nrhs(NRHS > 0 ? NRHS : (Int)eq_count_),
-          thread_count((Int)thread_count_), rel64(nrhs) {
+          thread_count((Int)thread_count_), rel64(nrhs), res64(nrhs) {
         ws.Resize(tg_cg_workspace_bytes((int32_t)n, (int32_t)nrhs, Device::DType<Scal>(), 1));
     }
 
@@ -71,6 +71,11 @@ class ConjugateGradient final {
                                 rel64.data(), nullptr);
         }
         Device::Status(st, ClassName() + "::operator()");
+        // sqrt|r.z| per column for Residuals() (the solver's final rho); with a == 0 the
+        // reference returns before touching rho (:147-163), so the last values stay
+        if ((double)a != 0.0)
+            Device::Status(tg_cg_residuals(ws.data(), ws.size(), (int32_t)nrhs, res64.data(), nullptr, nullptr),
+                           ClassName() + "::Residuals");
         if (Xd.host) Device::Copy2D(X_inout, (Size_T)ldX, Xd.ptr, (Size_T)nrhs, (Size_T)n, (Size_T)nrhs,
                                     hipMemcpyDeviceToHost);
         iter = it;
@@ -79,6 +84,12 @@ class ConjugateGradient final {
     }
 
     Int IterationCount() const { return iter; }
+    // sqrt|rho_k| per column (ConjugateGradient.hpp:447-456).
+    RealVector_T Residuals() const {
+        RealVector_T r(nrhs);
+        for (Int k = 0; k < nrhs; ++k) r[k] = (Real)res64[k];
+        return r;
+    }
     RealVector_T RelativeResiduals() const {
         RealVector_T r(nrhs);
         for (Int k = 0; k < nrhs; ++k) r[k] = (Real)rel64[k];
@@ -104,6 +115,7 @@ class ConjugateGradient final {
     const Int nrhs;
     const Int thread_count;
     Tensor1<double, Int> rel64;
+    Tensor1<double, Int> res64;
     Device::Buffer<char> ws;
     Int iter = 0;
     bool succeeded = false;

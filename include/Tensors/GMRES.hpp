@@ -26,7 +26,7 @@ class GMRES final {
 
     GMRES(const Int n_, const Int max_iter_, const Size_T nrhs_ = NRHS_, const Size_T thread_count_ = 1)
         : n(n_), max_iter(std::min<Int>(max_iter_, n_ + 1)), nrhs(NRHS > 0 ? NRHS : (Int)nrhs_),
-          thread_count((Int)thread_count_), rel64(nrhs) {
+          thread_count((Int)thread_count_), rel64(nrhs), res64(nrhs) {
         ws.Resize(tg_gmres_workspace_bytes((int32_t)n, (int32_t)max_iter, (int32_t)nrhs, Device::DType<Scal>()));
     }
 
@@ -78,16 +78,57 @@ class GMRES final {
         iter = it;
         restarts = rs;
         succeeded = ok != 0;
+        // |beta[iter][k]| for Residuals() (GMRES.hpp:779-789)
+        double* beta_d = nullptr;
+        Device::Status(tg_gmres_workspace_parts(ws.data(), ws.size(), (int32_t)n, (int32_t)max_iter, (int32_t)nrhs,
+                                                Device::DType<Scal>(), nullptr, nullptr, &beta_d, nullptr),
+                       ClassName() + "::Residuals");
+        Device::Copy(res64.data(), beta_d + (Size_T)iter * (Size_T)nrhs, (Size_T)nrhs, hipMemcpyDeviceToHost);
+        for (Int k = 0; k < nrhs; ++k) res64[k] = std::abs(res64[k]);
         return succeeded;
     }
 
     Int ThreadCount() const { return thread_count; }
     Int IterationCount() const { return iter; }
     Int RestartCount() const { return restarts; }
+    // |beta[iter][k]| per column (GMRES.hpp:779-789).
+    RealVector_T Residuals() const {
+        RealVector_T r(nrhs);
+        for (Int k = 0; k < nrhs; ++k) r[k] = (Real)res64[k];
+        return r;
+    }
     RealVector_T RelativeResiduals() const {
         RealVector_T r(nrhs);
         for (Int k = 0; k < nrhs; ++k) r[k] = (Real)rel64[k];
         return r;
+    }
+
+    // Host copies of the device workspace (GMRES.hpp:820-833): the basis Q of the last
+    // cycle ((m+1) x n x nrhs), the Hessenberg matrix H after the Givens rotations
+    // ((m+1) x m x nrhs) and the solution before the a/b combination (n x nrhs).
+    const Tensor3<Scal, Int>& GetBasis() const {
+        void* q = nullptr;
+        Parts(&q, nullptr, nullptr);
+        Q_host = Tensor3<Scal, Int>(max_iter_ws() + 1, n, nrhs);
+        Device::Copy(Q_host.data(), (const Scal*)q, (Size_T)Q_host.Size(), hipMemcpyDeviceToHost);
+        return Q_host;
+    }
+    const Tensor3<Scal, Int>& GetHessenbergMatrix() const {
+        double* h = nullptr;
+        Parts(nullptr, &h, nullptr);
+        const Int m = max_iter_ws();
+        std::vector<double> tmp((Size_T)(m + 1) * (Size_T)m * (Size_T)nrhs);
+        Device::Copy(tmp.data(), (const double*)h, tmp.size(), hipMemcpyDeviceToHost);
+        H_host = Tensor3<Scal, Int>(m + 1, m, nrhs);
+        H_host.Read(tmp.data());
+        return H_host;
+    }
+    const Tensor2<Scal, Int>& Solution() const {
+        void* x = nullptr;
+        Parts(nullptr, nullptr, &x);
+        sol_host = Tensor2<Scal, Int>(n, nrhs);
+        Device::Copy(sol_host.data(), (const Scal*)x, (Size_T)sol_host.Size(), hipMemcpyDeviceToHost);
+        return sol_host;
     }
 
     std::string Stats() const {
@@ -105,11 +146,23 @@ class GMRES final {
     }
 
   private:
+    // restart length of the workspace: m = min(max_iter, n + 1), at least 1
+    Int max_iter_ws() const { return std::max<Int>(Int(1), max_iter); }
+    void Parts(void** q, double** h, void** x) const {
+        Device::Status(tg_gmres_workspace_parts((void*)ws.data(), ws.size(), (int32_t)n, (int32_t)max_iter,
+                                                (int32_t)nrhs, Device::DType<Scal>(), q, h, nullptr, x),
+                       ClassName() + "::workspace");
+    }
+
     const Int n;
     const Int max_iter;
     const Int nrhs;
     const Int thread_count;
     Tensor1<double, Int> rel64;
+    Tensor1<double, Int> res64;
+    mutable Tensor3<Scal, Int> Q_host;
+    mutable Tensor3<Scal, Int> H_host;
+    mutable Tensor2<Scal, Int> sol_host;
     Device::Buffer<char> ws;
     Int iter = 0;
     Int restarts = 0;

@@ -125,4 +125,38 @@ template <typename Scal_, typename Int_ = Int64> class Tensor2 {
     std::vector<Scal> a_;
 };
 
+// Three-index row-major container: entry (i, j, k) at (i*d1 + j)*d2 + k
+// (src/Tensor3.hpp:116-165).  In the drop-in it carries host copies of the
+// GMRES basis and Hessenberg matrix.
+template <typename Scal_, typename Int_ = Int64> class Tensor3 {
+  public:
+    using Scal = Scal_;
+    using Int = Int_;
+
+    Tensor3() = default;
+    template <IntQ I0, IntQ I1, IntQ I2>
+    Tensor3(I0 d0, I1 d1, I2 d2) : d_{(Int)d0, (Int)d1, (Int)d2}, a_((Size_T)d0 * (Size_T)d1 * (Size_T)d2) {}
+
+    Scal* data() { return a_.data(); }
+    const Scal* data() const { return a_.data(); }
+    Scal* data(Int i) { return a_.data() + (Size_T)i * (Size_T)d_[1] * (Size_T)d_[2]; }
+    const Scal* data(Int i) const { return a_.data() + (Size_T)i * (Size_T)d_[1] * (Size_T)d_[2]; }
+    Scal* data(Int i, Int j) { return data(i) + (Size_T)j * (Size_T)d_[2]; }
+    const Scal* data(Int i, Int j) const { return data(i) + (Size_T)j * (Size_T)d_[2]; }
+    Int Dim(int i) const { return (i >= 0 && i < 3) ? d_[i] : 0; }
+    Int Dimension(int i) const { return Dim(i); }
+    Int Size() const { return (Int)a_.size(); }
+    Scal& operator()(Int i, Int j, Int k) { return data(i, j)[k]; }
+    const Scal& operator()(Int i, Int j, Int k) const { return data(i, j)[k]; }
+
+    void Fill(const Scal& v) { std::fill(a_.begin(), a_.end(), v); }
+    void SetZero() { Fill(Scal(0)); }
+    template <typename S> void Read(const S* p) { for (Size_T i = 0; i < a_.size(); ++i) a_[i] = (Scal)p[i]; }
+    template <typename S> void Write(S* p) const { for (Size_T i = 0; i < a_.size(); ++i) p[i] = (S)a_[i]; }
+
+  private:
+    Int d_[3] = {0, 0, 0};
+    std::vector<Scal> a_;
+};
+
 }  // namespace Tensors

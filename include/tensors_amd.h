@@ -37,7 +37,7 @@ typedef int (*tg_apply_fn)(void* ctx, const void* x, void* y, int32_t nrhs, void
 typedef enum tg_status {
     TG_OK = 0,
     TG_ERR_INVALID = -1,  /* bad argument (negative size, null pointer, ld < nrhs, ...) */
-    TG_ERR_ALIAS = -2,    /* X == Y: the reference prints and does nothing (SpMM.hpp:236-243) */
+    TG_ERR_ALIAS = -2,    /* X == Y: the reference prints and does nothing (SpMM.hpp:191-198) */
     TG_ERR_NRHS = -3,     /* fixed NRHS != nrhs: reference no-op (Multiply_DenseMatrix.hpp:19-26) */
     TG_ERR_DTYPE = -4,    /* unsupported index / value type */
     TG_ERR_HIP = -5,      /* a HIP runtime call failed */
@@ -100,7 +100,10 @@ int tg_csr_spmm(int32_t m, int32_t n, int64_t nnz, const void* rp, tg_dtype rp_t
  * whenever slicing does not pay.  Calls on one plan may come from several host
  * threads and streams: they share the plan's partial blocks, so each sliced call
  * first waits for the previous one on the device (outside stream capture; under
- * capture the caller orders them).
+ * capture the caller orders them).  The partial blocks grow on the first call of
+ * a larger shape; a captured call never allocates: it returns TG_ERR_WORKSPACE
+ * (nothing enqueued) unless one call of that shape (same nrhs and value type, or
+ * a larger one) ran on the plan outside capture first.
  * tg_csr_plan_slices returns the slice count it would use (0 = plain). */
 typedef struct tg_csr_plan_s* tg_csr_plan_t;
 int tg_csr_plan_create(int32_t m, int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, const int32_t* ci,
@@ -202,6 +205,13 @@ int tg_cg_solve(int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, const 
                 double a_coef, const void* B, int64_t ldB, double b_coef, void* X, int64_t ldX, double tol,
                 int32_t use_guess, void* workspace, size_t workspace_bytes, int32_t* succeeded,
                 int32_t* iterations, double* rel_residuals, void* stream);
+/* After a tg_cg_solve / tg_cg_solve_op on `workspace` that reached its
+ * iteration loop: host copies of sqrt|r.z| per column -- replaces
+ * ConjugateGradient::Residuals (ConjugateGradient.hpp:447-456) -- and of
+ * |b.Pb| (the member b_squared_norms, :45) into residuals[nrhs] /
+ * b_squared_norms[nrhs] (either may be NULL).  Synchronises `stream`. */
+int tg_cg_residuals(const void* workspace, size_t workspace_bytes, int32_t nrhs, double* residuals,
+                    double* b_squared_norms, void* stream);
 
 /* ------------------------------------------------------------------------
  * Mixed-precision CG: the recurrences of tg_cg_solve in FP32 (FP32 values a32
@@ -263,6 +273,17 @@ int tg_gmres_solve(int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, con
                    int64_t ldX, double tol, int32_t max_restarts, int32_t use_guess, void* workspace,
                    size_t workspace_bytes, int32_t* succeeded, int32_t* iterations, int32_t* restarts,
                    double* rel_residuals, void* stream);
+/* Device views into a GMRES workspace after a solve with the same n, max_iter,
+ * nrhs and val_type -- the members behind GMRES::GetBasis, GetHessenbergMatrix,
+ * Solution and Residuals (GMRES.hpp:779-789, 820-833).  m = min(max_iter, n+1)
+ * (at least 1).  *Q: (m+1) x n x nrhs basis of the last cycle (val_type);
+ * *H: (m+1) x m x nrhs FP64, the Hessenberg matrix after the Givens rotations,
+ * entry (i, j, k) at (i*m + j)*nrhs + k, as the reference's Tensor3 H;
+ * *beta: (m+1) x nrhs FP64 (|beta[iter][k]| = GMRES::Residuals()[k]);
+ * *solution: n x nrhs (val_type) before the a/b combination.  Any output may
+ * be NULL.  Host arithmetic only. */
+int tg_gmres_workspace_parts(void* workspace, size_t workspace_bytes, int32_t n, int32_t max_iter, int32_t nrhs,
+                             tg_dtype val_type, void** Q, double** H, double** beta, void** solution);
 
 /* ------------------------------------------------------------------------
  * Structure work on the GPU (integer results are exact and deterministic).

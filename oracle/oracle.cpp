@@ -88,7 +88,7 @@ static std::vector<i64> job_pointers_costs(i64 m, const i64* acc, i64 T) {
 }
 
 // ---------------------------------------------------------------------------
-// alpha/beta flag semantics (Scalar::Flag, SpMM.hpp:83-175): Zero means the
+// alpha/beta flag semantics (Scalar::Flag, SpMM.hpp:28-131): Zero means the
 // term is dropped without reading memory; Plus means multiplication by one is
 // skipped; Generic multiplies.
 // ---------------------------------------------------------------------------
@@ -131,15 +131,15 @@ static int spmv(const i64* rp, const i32* ci, const T* a, i32 m, T alpha, const 
     return OK;
 }
 
-// SparseBLAS::SpMM (SpMM.hpp:3-176) + SpMM_impl (SpMM.hpp:181-460).  K > 0 is
-// the fixed-NRHS instantiation that SpMM_vec (SpMM.hpp:462-613) provides, with
+// SparseBLAS::SpMM (SpMM.hpp:3-131) + SpMM_impl (SpMM.hpp:134-415).  K > 0 is
+// the fixed-NRHS instantiation that SpMM_vec (SpMM.hpp:419-569) provides, with
 // the same summation order; K == 0 is the runtime-nrhs path.
 template <class T, int K>
 static int spmm(const i64* rp, const i32* ci, const T* a, i32 m, T alpha, const T* X, i64 ldX, T beta,
                 T* Y, i64 ldY, i32 nrhs_rt, const std::vector<i64>& jp) {
     const int threads = (int)jp.size() - 1;
     const i32 nrhs = K > 0 ? K : nrhs_rt;
-    if (alpha == T(0)) {  // modify_matrix_auto, SpMM.hpp:20-26
+    if (alpha == T(0)) {  // modify_matrix_auto, SpMM.hpp:18-25
         if (beta == T(0)) {
             for (i32 i = 0; i < m; ++i) std::fill(Y + ldY * i, Y + ldY * i + nrhs, T(0));
         } else if (beta != T(1)) {
@@ -148,7 +148,7 @@ static int spmm(const i64* rp, const i32* ci, const T* a, i32 m, T alpha, const 
         }
         return OK;
     }
-    if ((const void*)X == (const void*)Y) return ERR_ALIAS;  // SpMM.hpp:236-243
+    if ((const void*)X == (const void*)Y) return ERR_ALIAS;  // SpMM.hpp:191-198
     const Flag af = (alpha == T(1)) ? Plus : Generic;
     const Flag bf = (beta == T(0)) ? Zero : (beta == T(1)) ? Plus : Generic;
     parallel_do(threads, [&](int t) {
@@ -159,18 +159,18 @@ static int spmm(const i64* rp, const i32* ci, const T* a, i32 m, T alpha, const 
             T* y = Y + ldY * i;
             const i64 lb = rp[i], le = rp[i + 1];
             if (le > lb) {
-                {  // first entry overwrites z (SpMM.hpp:366-397)
+                {  // first entry overwrites z (SpMM.hpp:322-352)
                     const T av = a ? a[lb] : T(1);
                     const T* x = X + ldX * (i64)ci[lb];
                     for (i32 k = 0; k < nrhs; ++k) z[k] = av * x[k];
                 }
-                for (i64 l = lb + 1; l < le; ++l) {  // add-in (SpMM.hpp:399-431)
+                for (i64 l = lb + 1; l < le; ++l) {  // add-in (SpMM.hpp:355-386)
                     const T av = a ? a[l] : T(1);
                     const T* x = X + ldX * (i64)ci[l];
                     for (i32 k = 0; k < nrhs; ++k) z[k] += av * x[k];
                 }
                 for (i32 k = 0; k < nrhs; ++k) y[k] = combine(af, alpha, z[k], bf, beta, y[k]);
-            } else {  // empty row (SpMM.hpp:440-455)
+            } else {  // empty row (SpMM.hpp:395-410)
                 if (bf == Zero)
                     for (i32 k = 0; k < nrhs; ++k) y[k] = T(0);
                 else if (bf == Generic)
@@ -639,7 +639,7 @@ struct CGResult {
 // runs without an initial guess (:214-250, 342-346); here x starts at 0.
 static CGResult conjugate_gradient(const Op& A, const Prec& P, i32 n, i32 nrhs, i32 max_iter, i32 T,
                                    double a, const double* B, i64 ldB, double b, double* X, i64 ldX,
-                                   double tol, bool use_guess, double* rel_res) {
+                                   double tol, bool use_guess, double* rel_res, double* abs_res = nullptr) {
     CGResult res{0, 0};
     max_iter = std::max(0, max_iter);
     const i64 N = (i64)n * nrhs;
@@ -711,6 +711,8 @@ static CGResult conjugate_gradient(const Op& A, const Prec& P, i32 n, i32 nrhs, 
     write_solution(a, x.data(), b, X, ldX, n, nrhs);
     if (rel_res)
         for (i32 k = 0; k < nrhs; ++k) rel_res[k] = std::sqrt(std::fabs(rho[k]) / bsq[k]);
+    if (abs_res)  // ConjugateGradient::Residuals (ConjugateGradient.hpp:447-456)
+        for (i32 k = 0; k < nrhs; ++k) abs_res[k] = std::sqrt(std::fabs(rho[k]));
     res.succeeded = succeeded ? 1 : 0;
     res.iter = iter;
     return res;
@@ -876,9 +878,18 @@ struct GMRES {
     }
 };
 
+// Optional state outputs: the members behind GetBasis / GetHessenbergMatrix /
+// Solution / Residuals (GMRES.hpp:779-789, 820-833) after the last cycle.
+struct GMRESState {
+    double* Q = nullptr;         // (m+1) x n x nrhs
+    double* H = nullptr;         // (m+1) x m x nrhs
+    double* beta = nullptr;      // (m+1) x nrhs
+    double* solution = nullptr;  // n x nrhs
+};
+
 static GMRESResult gmres(const Op& A, const Prec& P, i32 n, i32 nrhs, i32 max_iter, i32 T, i32 side,
                          double a, const double* B, i64 ldB, double b, double* X, i64 ldX, double tol,
-                         i32 max_restarts, bool use_guess, double* rel_res) {
+                         i32 max_restarts, bool use_guess, double* rel_res, const GMRESState* out = nullptr) {
     GMRESResult res{0, 0, 0};
     GMRES g(A, P, n, max_iter, nrhs, T, side);
     const i64 N = (i64)n * nrhs;
@@ -916,6 +927,12 @@ static GMRESResult gmres(const Op& A, const Prec& P, i32 n, i32 nrhs, i32 max_it
     if (rel_res)
         for (i32 k = 0; k < nrhs; ++k)
             rel_res[k] = std::fabs(g.beta[(i64)g.iter * nrhs + k]) / g.b_norms[k];
+    if (out) {
+        if (out->Q) std::copy(g.Q.begin(), g.Q.end(), out->Q);
+        if (out->H) std::copy(g.H.begin(), g.H.end(), out->H);
+        if (out->beta) std::copy(g.beta.begin(), g.beta.end(), out->beta);
+        if (out->solution) std::copy(g.solution.begin(), g.solution.end(), out->solution);
+    }
     res.succeeded = succeeded ? 1 : 0;
     res.iter = g.iter;
     res.restarts = restarts;
@@ -1060,6 +1077,19 @@ int orc_cg(int32_t n, const int64_t* rp, const int32_t* ci, const double* val, i
     return r.succeeded;
 }
 
+// orc_cg plus abs_res[nrhs] = sqrt|rho| (ConjugateGradient::Residuals).
+int orc_cg_ex(int32_t n, const int64_t* rp, const int32_t* ci, const double* val, int32_t prec_kind,
+              const double* prec_diag, int32_t nrhs, int32_t max_iter, int32_t threads, double a, const double* B,
+              int64_t ldB, double b, double* X, int64_t ldX, double tol, int32_t use_guess, int32_t* iter,
+              double* rel_res, double* abs_res) {
+    orc::Op A{rp, ci, val, n, threads};
+    orc::Prec P{prec_kind, prec_diag};
+    auto r = orc::conjugate_gradient(A, P, n, nrhs, max_iter, threads, a, B, ldB, b, X, ldX, tol,
+                                     use_guess != 0, rel_res, abs_res);
+    if (iter) *iter = r.iter;
+    return r.succeeded;
+}
+
 int orc_gmres(int32_t n, const int64_t* rp, const int32_t* ci, const double* val, int32_t prec_kind,
               const double* prec_diag, int32_t side, int32_t nrhs, int32_t max_iter, int32_t threads, double a,
               const double* B, int64_t ldB, double b, double* X, int64_t ldX, double tol, int32_t max_restarts,
@@ -1068,6 +1098,23 @@ int orc_gmres(int32_t n, const int64_t* rp, const int32_t* ci, const double* val
     orc::Prec P{prec_kind, prec_diag};
     auto r = orc::gmres(A, P, n, nrhs, max_iter, threads, side, a, B, ldB, b, X, ldX, tol, max_restarts,
                         use_guess != 0, rel_res);
+    if (iter) *iter = r.iter;
+    if (restarts) *restarts = r.restarts;
+    return r.succeeded;
+}
+
+// orc_gmres plus the final-cycle state: Q ((m+1) x n x nrhs), H ((m+1) x m x nrhs),
+// beta ((m+1) x nrhs), solution (n x nrhs) with m = min(max_iter, n + 1); any may be null.
+int orc_gmres_ex(int32_t n, const int64_t* rp, const int32_t* ci, const double* val, int32_t prec_kind,
+                 const double* prec_diag, int32_t side, int32_t nrhs, int32_t max_iter, int32_t threads, double a,
+                 const double* B, int64_t ldB, double b, double* X, int64_t ldX, double tol, int32_t max_restarts,
+                 int32_t use_guess, int32_t* iter, int32_t* restarts, double* rel_res, double* Q, double* H,
+                 double* beta, double* solution) {
+    orc::Op A{rp, ci, val, n, threads};
+    orc::Prec P{prec_kind, prec_diag};
+    orc::GMRESState st{Q, H, beta, solution};
+    auto r = orc::gmres(A, P, n, nrhs, max_iter, threads, side, a, B, ldB, b, X, ldX, tol, max_restarts,
+                        use_guess != 0, rel_res, &st);
     if (iter) *iter = r.iter;
     if (restarts) *restarts = r.restarts;
     return r.succeeded;

@@ -928,3 +928,21 @@ extern "C" int tg_cg_solve_op(int32_t n, tg_apply_fn apply_A, void* ctx_A, tg_ap
                                use_guess, workspace, workspace_bytes, succeeded, iterations, rel_residuals, s);
     return fail(TG_ERR_DTYPE, "tg_cg_solve_op: val_type must be F64/F32");
 }
+
+// Residuals of the last solve on `workspace`: sqrt|r.z| per column
+// (ConjugateGradient::Residuals, ConjugateGradient.hpp:447-456) and |b.Pb|
+// (b_squared_norms), read from the device state at the head of the workspace.
+extern "C" int tg_cg_residuals(const void* workspace, size_t workspace_bytes, int32_t nrhs, double* residuals,
+                               double* b_squared_norms, void* stream) {
+    if (!workspace || nrhs < 1 || nrhs > kMaxCols || workspace_bytes < sizeof(CGState))
+        return fail(TG_ERR_INVALID, "tg_cg_residuals: bad argument");
+    hipStream_t s = as_stream(stream);
+    CGState hs;
+    TG_HIP(hipMemcpyAsync(&hs, workspace, sizeof(CGState), hipMemcpyDeviceToHost, s));
+    TG_HIP(hipStreamSynchronize(s));
+    for (int c = 0; c < nrhs; ++c) {
+        if (residuals) residuals[c] = std::sqrt(std::fabs(hs.rho[c]));
+        if (b_squared_norms) b_squared_norms[c] = hs.bsq[c];
+    }
+    return TG_OK;
+}

@@ -897,3 +897,26 @@ extern "C" int tg_gmres_solve_op(int32_t n, tg_apply_fn apply_A, void* ctx_A, tg
                                   restarts, rel_residuals, s);
     return fail(TG_ERR_DTYPE, "tg_gmres_solve_op: val_type must be F64/F32");
 }
+
+// Device views into a GMRES workspace (the members GMRES::GetBasis,
+// GetHessenbergMatrix, Solution and beta of GMRES.hpp:51-69, 820-833), valid
+// after a solve with the same n, max_iter, nrhs and value type.  m = min(max_iter,
+// n + 1) (at least 1): Q is (m+1) x n x nrhs in val_type, H (m+1) x m x nrhs and
+// beta (m+1) x nrhs in FP64 (H holds the Givens-rotated Hessenberg matrix, as the
+// reference's H after ApplyGivensRotations), solution n x nrhs in val_type.
+extern "C" int tg_gmres_workspace_parts(void* workspace, size_t workspace_bytes, int32_t n, int32_t max_iter,
+                                        int32_t nrhs, tg_dtype val_type, void** Q, double** H, double** beta,
+                                        void** solution) {
+    if (!workspace || n < 1 || nrhs < 1 || (val_type != TG_F64 && val_type != TG_F32))
+        return fail(TG_ERR_INVALID, "tg_gmres_workspace_parts: bad argument");
+    const int m = std::max(1, std::min(max_iter, n + 1));
+    GmresWs w;
+    const size_t need = gmres_layout(n, m, nrhs, val_type == TG_F64 ? 8 : 4, &w, (char*)workspace);
+    if (workspace_bytes < need) return fail(TG_ERR_WORKSPACE, "tg_gmres_workspace_parts: workspace too small");
+    const GmresLayout L(m, nrhs);
+    if (Q) *Q = w.Q;
+    if (H) *H = w.small + L.H;
+    if (beta) *beta = w.small + L.beta;
+    if (solution) *solution = w.sol;
+    return TG_OK;
+}

@@ -3,7 +3,7 @@
 // Reference semantics: SparseBLAS::Multiply_DenseMatrix routing
 // (src/SparseBLAS/Multiply_DenseMatrix.hpp:9-45), SpMV (src/SparseBLAS/SpMV.hpp:
 // 3-288), SpMM (src/SparseBLAS/SpMM.hpp:3-569).  The reference's compile-time
-// flag dispatch (a_flag x alpha_flag x beta_flag, SpMM.hpp:83-175) becomes
+// flag dispatch (a_flag x alpha_flag x beta_flag, SpMM.hpp:28-131) becomes
 // wave-uniform runtime branches here: they cost nothing in a bandwidth-bound
 // kernel and keep the instantiation count down.
 //
@@ -35,7 +35,7 @@ enum BetaMode : int { kBetaZero = 0, kBetaOne = 1, kBetaGeneric = 2 };
 
 // Row epilogue for V consecutive entries of one Y row: the combine_scalars /
 // combine_buffers contract y = (alpha * z) + (beta * y) (SpMV.hpp:224-225,
-// SpMM.hpp:433-438) and the empty-row rules (SpMM.hpp:440-455).
+// SpMM.hpp:388-393) and the empty-row rules (SpMM.hpp:395-410).
 template <typename T, int V>
 __device__ __forceinline__ void epilogue(T* __restrict__ yp, const T (&z)[V], bool empty, T alpha,
                                          int alpha_one, T beta, int beta_mode) {
@@ -346,7 +346,7 @@ __global__ __launch_bounds__(kBlock) void spmm_rows_kernel(
     }
 }
 
-// Y <- beta * Y over m x nrhs (alpha == 0 path of SpMM.hpp:65-71 / SpMV.hpp:15-30)
+// Y <- beta * Y over m x nrhs (alpha == 0 path of SpMM.hpp:18-25 / SpMV.hpp:14-29)
 // and the general axpby used by the solvers.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void axpby_kernel(int m, int nrhs, T alpha, const T* __restrict__ X,
@@ -1043,7 +1043,7 @@ int spmm_dispatch(const CsrView& A, double alpha, const void* X, int64_t ldX, do
     if (!Y || ldY < nrhs) return fail(TG_ERR_INVALID, "spmm: Y is null or ldY < nrhs");
     if (job_ptr && blocks < 1) return fail(TG_ERR_INVALID, "spmm: job_ptr given with blocks < 1");
     const bool f64 = A.val_type == TG_F64;
-    if (alpha == 0.0) {  // SpMM.hpp:65-71, SpMV.hpp:15-30
+    if (alpha == 0.0) {  // SpMM.hpp:18-25, SpMV.hpp:14-29
         if (f64) return axpby_typed<double>(m, nrhs, 0.0, nullptr, 0, beta, (double*)Y, ldY, st);
         return axpby_typed<float>(m, nrhs, 0.0, nullptr, 0, beta, (float*)Y, ldY, st);
     }
@@ -1209,6 +1209,13 @@ extern "C" int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_
     TG_HIP(hipStreamIsCapturing(st, &cap));
     const bool track = cap == hipStreamCaptureStatusNone;
     if (track && plan->used && plan->used_on != st) TG_HIP(hipStreamWaitEvent(st, plan->used, 0));
+    if (plan->part_bytes < need && !track)
+        // growing the blocks under capture would record an allocation node whose memory
+        // belongs to the graph; the plan must be sized by an eager call of this shape first
+        return fail(TG_ERR_WORKSPACE,
+                    "tg_csr_spmm_plan: the plan's partial blocks (%zu bytes) must be sized by one call of this shape "
+                    "outside stream capture before the call is captured",
+                    need);
     if (plan->part_bytes < need) {
         if (plan->part) TG_HIP(hipFreeAsync(plan->part, st));
         plan->part = nullptr;

@@ -58,6 +58,12 @@ class ConjugateGradient:
     def RelativeResiduals(self):
         return self.rel_res
 
+    def Residuals(self):
+        """sqrt|rho_k| of the last solve (ConjugateGradient.hpp:447-456), via tg_cg_residuals."""
+        out = np.zeros(self.nrhs)
+        capi.call("tg_cg_residuals", self._ws.data_ptr(), self._ws.numel(), self.nrhs, out.ctypes.data, None, None)
+        return out
+
 
 class GMRES:
     """GMRES<VarSize,Scal,Int,Side>(n, max_iter, nrhs) (GMRES.hpp:82-108); side 0 = Left, 1 = Right."""
@@ -104,6 +110,40 @@ class GMRES:
 
     def RelativeResiduals(self):
         return self.rel_res
+
+    def _parts(self):
+        """Device views (Q, H, beta, solution) of the workspace (tg_gmres_workspace_parts)."""
+        q, h, bt, x = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+        capi.call("tg_gmres_workspace_parts", self._ws.data_ptr(), self._ws.numel(), self.n, self.max_iter,
+                  self.nrhs, _TG[self.dtype], C.byref(q), C.byref(h), C.byref(bt), C.byref(x))
+        m = max(1, min(self.max_iter, self.n + 1))
+        vs = 8 if self.dtype == torch.float64 else 4
+        base = self._ws.data_ptr()
+
+        def view(ptr, shape, dtype, esize):
+            off = ptr.value - base
+            count = int(np.prod(shape))
+            return self._ws[off:off + count * esize].view(dtype).view(*shape)
+        return (view(q, (m + 1, self.n, self.nrhs), self.dtype, vs),
+                view(h, (m + 1, m, self.nrhs), torch.float64, 8),
+                view(bt, (m + 1, self.nrhs), torch.float64, 8),
+                view(x, (self.n, self.nrhs), self.dtype, vs))
+
+    def GetBasis(self):
+        """Basis Q of the last cycle, (m+1) x n x nrhs (GMRES.hpp:820-823)."""
+        return self._parts()[0].clone()
+
+    def GetHessenbergMatrix(self):
+        """H after the Givens rotations, (m+1) x m x nrhs (GMRES.hpp:825-828)."""
+        return self._parts()[1].clone()
+
+    def Solution(self):
+        """Solution before the a/b combination, n x nrhs (GMRES.hpp:830-833)."""
+        return self._parts()[3].clone()
+
+    def Residuals(self):
+        """|beta[iter][k]| (GMRES.hpp:779-789)."""
+        return self._parts()[2][self.iter].abs().cpu().numpy()
 
 
 def csr_callback(A: MatrixCSR, dtype=torch.float64):

@@ -4,7 +4,8 @@
 // error cases, Transpose, sparse x sparse, ConjugateGradient and GMRES with CSR
 // operators, host lambdas and device lambdas.  Prints one line per check and
 // exits non-zero on the first failure.  argv[1] (optional): file that receives
-// the GridLaplacian(16) structure for the pytest to compare with the oracle.
+// the GridLaplacian(16) structure for the pytest to compare with the oracle;
+// argv[2] (optional): directory that receives the solver accessors' values.
 #include <unistd.h>
 #include <cstdio>
 #include <cstdlib>
@@ -313,6 +314,65 @@ int main(int argc, char** argv) {
         const bool ok2 = gm(A_host, P, Real(1), B.data(), k, Real(0), X2.data(), k, Real(1e-10), 50);
         const double e2 = relerr(X2.data(), X.data(), n);
         check(ok2 && e2 < 1e-12, "GMRES(host lambda) matches", e2);
+    }
+
+    // --- Solver accessors at a fixed iteration count -----------------------------
+    // Residuals (ConjugateGradient.hpp:447-456; GMRES.hpp:779-789), GetBasis,
+    // GetHessenbergMatrix and Solution (GMRES.hpp:820-833) after tol = 0 runs; argv[2]
+    // names a directory that receives them (raw float64) for the pytest to compare with
+    // the oracle's values on the same inputs.
+    {
+        const Int k = 2, steps = 20, m = 10;
+        Tensor2<Real, Int> B(n, k), X(n, k, Real(0));
+        B.Random();
+        IdentityPreconditioner P;
+        ConjugateGradient<0, Real, Int> cg(n, steps, k, 4);
+        const bool ok = cg(L, P, Real(1), B.data(), k, Real(0), X.data(), k, Real(0));
+        const auto res = cg.Residuals();
+        const auto rel = cg.RelativeResiduals();
+        bool pos = true;
+        for (Int c = 0; c < k; ++c) pos &= res[c] > 0 && rel[c] > 0 && res[c] > rel[c];
+        check(!ok && cg.IterationCount() == steps && pos, "CG Residuals() after 20 steps");
+        GMRES<0, Real, Int, Side::Left> gm(n, m, k, 4);
+        Tensor2<Real, Int> Xg(n, k, Real(0));
+        const bool okg = gm(L, P, Real(1), B.data(), k, Real(0), Xg.data(), k, Real(0), 1);
+        const auto& Q = gm.GetBasis();
+        const auto& H = gm.GetHessenbergMatrix();
+        const auto& S = gm.Solution();
+        double orth = 0;  // columns of the last cycle's basis are orthonormal
+        for (Int c = 0; c < k; ++c)
+            for (Int i = 0; i <= m; ++i)
+                for (Int j = 0; j <= m; ++j) {
+                    double d = 0;
+                    for (Int r = 0; r < n; ++r) d += Q(i, r, c) * Q(j, r, c);
+                    orth = std::max(orth, std::abs(d - (i == j ? 1.0 : 0.0)));
+                }
+        check(!okg && gm.IterationCount() == m && gm.RestartCount() == 1 && Q.Dim(0) == m + 1 && Q.Dim(1) == n &&
+                  H.Dim(0) == m + 1 && H.Dim(1) == m && orth < 1e-12,
+              "GMRES GetBasis / GetHessenbergMatrix", orth);
+        check(relerr(S.data(), Xg.data(), (size_t)n * k) == 0.0, "GMRES Solution() == X (a = 1, b = 0)");
+        if (argc > 2) {
+            const std::string dir = argv[2];
+            auto dump = [&](const char* name, const double* p, size_t count) {
+                FILE* f = std::fopen((dir + "/" + name).c_str(), "wb");
+                if (f) {
+                    std::fwrite(p, sizeof(double), count, f);
+                    std::fclose(f);
+                }
+            };
+            const auto gres = gm.Residuals();
+            const auto grel = gm.RelativeResiduals();
+            dump("B.bin", B.data(), (size_t)n * k);
+            dump("cg_X.bin", X.data(), (size_t)n * k);
+            dump("cg_res.bin", res.data(), (size_t)k);
+            dump("cg_rel.bin", rel.data(), (size_t)k);
+            dump("gm_X.bin", Xg.data(), (size_t)n * k);
+            dump("gm_res.bin", gres.data(), (size_t)k);
+            dump("gm_rel.bin", grel.data(), (size_t)k);
+            dump("gm_Q.bin", Q.data(), (size_t)Q.Size());
+            dump("gm_H.bin", H.data(), (size_t)H.Size());
+            dump("gm_sol.bin", S.data(), (size_t)S.Size());
+        }
     }
 
     std::printf(failures ? "FAILED %d\n" : "ALL OK\n", failures);

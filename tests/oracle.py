@@ -30,6 +30,10 @@ _SIG = {
                          vp]),
     "orc_gmres": (C.c_int, [i32, vp, vp, vp, i32, vp, i32, i32, i32, i32, dbl, vp, i64, dbl, vp, i64, dbl, i32,
                             i32, vp, vp, vp]),
+    "orc_cg_ex": (C.c_int, [i32, vp, vp, vp, i32, vp, i32, i32, i32, dbl, vp, i64, dbl, vp, i64, dbl, i32, vp,
+                            vp, vp]),
+    "orc_gmres_ex": (C.c_int, [i32, vp, vp, vp, i32, vp, i32, i32, i32, i32, dbl, vp, i64, dbl, vp, i64, dbl, i32,
+                               i32, vp, vp, vp, vp, vp, vp, vp]),
 }
 
 _libs = {}
@@ -171,3 +175,36 @@ def gmres(n, rp, ci, val, B, X, nrhs, max_iter, tol, max_restarts, a=1.0, b=0.0,
                              nrhs, max_iter, threads, a, p(B), B.strides[0] // 8, b, p(X), X.strides[0] // 8,
                              tol, max_restarts, int(use_guess), p(it), p(rs), p(rr))
     return ok, int(it[0]), int(rs[0]), rr
+
+
+def cg_ex(n, rp, ci, val, B, X, nrhs, max_iter, tol, a=1.0, b=0.0, use_guess=False, prec_diag=None, threads=4):
+    """oracle.cg plus ConjugateGradient::Residuals: returns (succeeded, iters, rel_res, abs_res)."""
+    rp, ci, val, B, prec_diag = _solver_arrays(rp, ci, val, B, prec_diag)
+    assert X.dtype == np.float64 and X.strides[1] == 8
+    it = np.zeros(1, np.int32)
+    rr = np.zeros(nrhs, np.float64)
+    ar = np.zeros(nrhs, np.float64)
+    ok = lib().orc_cg_ex(n, p(rp), p(ci), p(val), 1 if prec_diag is not None else 0, p(prec_diag), nrhs,
+                         max_iter, threads, a, p(B), B.strides[0] // 8, b, p(X), X.strides[0] // 8, tol,
+                         int(use_guess), p(it), p(rr), p(ar))
+    return ok, int(it[0]), rr, ar
+
+
+def gmres_ex(n, rp, ci, val, B, X, nrhs, max_iter, tol, max_restarts, a=1.0, b=0.0, use_guess=False,
+             prec_diag=None, side=0, threads=4):
+    """oracle.gmres plus the final-cycle state of GetBasis / GetHessenbergMatrix / Solution / beta.
+    Returns (succeeded, iters, restarts, rel_res, Q, H, beta, solution) with m = min(max_iter, n+1)."""
+    rp, ci, val, B, prec_diag = _solver_arrays(rp, ci, val, B, prec_diag)
+    assert X.dtype == np.float64 and X.strides[1] == 8
+    m = min(max_iter, n + 1)
+    it = np.zeros(1, np.int32)
+    rs = np.zeros(1, np.int32)
+    rr = np.zeros(nrhs, np.float64)
+    Q = np.zeros((m + 1, n, nrhs))
+    H = np.zeros((m + 1, m, nrhs))
+    beta = np.zeros((m + 1, nrhs))
+    sol = np.zeros((n, nrhs))
+    ok = lib().orc_gmres_ex(n, p(rp), p(ci), p(val), 1 if prec_diag is not None else 0, p(prec_diag), side,
+                            nrhs, max_iter, threads, a, p(B), B.strides[0] // 8, b, p(X), X.strides[0] // 8,
+                            tol, max_restarts, int(use_guess), p(it), p(rs), p(rr), p(Q), p(H), p(beta), p(sol))
+    return ok, int(it[0]), int(rs[0]), rr, Q, H, beta, sol

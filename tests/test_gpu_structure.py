@@ -10,6 +10,7 @@ import pytest
 import scipy.sparse as sp
 
 import oracle
+from helpers import rel_err
 from tensors_amd import structure as st
 
 pytestmark = pytest.mark.gpu
@@ -135,7 +136,9 @@ def test_cpp_dropin_program(cuda, tmp_path):
     exe = os.path.join(ROOT, "tests", "cpp", "dropin_test")
     assert os.path.exists(exe), "build tests/cpp/dropin_test first (make -C tests/cpp)"
     dump = tmp_path / "grid16.bin"
-    r = subprocess.run([exe, str(dump)], capture_output=True, text=True, timeout=300)
+    sdir = tmp_path / "solvers"
+    sdir.mkdir()
+    r = subprocess.run([exe, str(dump), str(sdir)], capture_output=True, text=True, timeout=300)
     print(r.stdout)
     print(r.stderr[-2000:])
     assert r.returncode == 0 and "ALL OK" in r.stdout
@@ -147,6 +150,24 @@ def test_cpp_dropin_program(cuda, tmp_path):
     val = np.frombuffer(raw[off:off + 8 * nnz], np.float64)
     orp, oci, oval = oracle.grid_laplacian(16, 1.0)
     assert np.array_equal(rp, orp) and np.array_equal(ci, oci) and np.array_equal(val, oval)
+    # the drop-in's solver accessors after fixed-iteration runs (CG 20 steps; GMRES(10),
+    # two cycles) against the oracle's values on the same right-hand sides, at 1e-12
+    n, k, m = 256, 2, 10
+    ld = lambda name, *shape: np.fromfile(sdir / name, np.float64).reshape(shape)
+    B = ld("B.bin", n, k)
+    Xo = np.zeros((n, k))
+    _, _, rr_o, ar_o = oracle.cg_ex(n, orp, oci, oval, B, Xo, k, 20, 0.0)
+    assert rel_err(ld("cg_X.bin", n, k), Xo) <= 1e-12
+    assert np.max(np.abs(ld("cg_res.bin", k) - ar_o) / ar_o) <= 1e-12
+    assert np.max(np.abs(ld("cg_rel.bin", k) - rr_o) / rr_o) <= 1e-12
+    Xg = np.zeros((n, k))
+    _, it_o, _, grr_o, Q_o, H_o, beta_o, sol_o = oracle.gmres_ex(n, orp, oci, oval, B, Xg, k, m, 0.0, 1)
+    assert rel_err(ld("gm_X.bin", n, k), Xg) <= 1e-12
+    assert np.max(np.abs(ld("gm_res.bin", k) - np.abs(beta_o[it_o])) / np.abs(beta_o[it_o])) <= 1e-12
+    assert np.max(np.abs(ld("gm_rel.bin", k) - grr_o) / grr_o) <= 1e-12
+    assert rel_err(ld("gm_Q.bin", m + 1, n, k), Q_o) <= 1e-12
+    assert rel_err(ld("gm_H.bin", m + 1, m, k), H_o) <= 1e-12
+    assert rel_err(ld("gm_sol.bin", n, k), sol_o) <= 1e-12
 
 
 @pytest.mark.parametrize("symmetric", [False, True])

@@ -1,7 +1,7 @@
 """Pins the oracle's SpMV/SpMM (oracle.cpp: spmv/spmm/multiply_dense) against what the
 mathematics fixes: dense brute force (the Dot_True triple loop of
 Unit_Test_SpMM/main.cpp:19-60, done in numpy), exact integer arithmetic, the alpha/beta
-flag table of SpMM.hpp:83-175 / SpMV.hpp:15-30, empty rows (SpMM.hpp:440-455), the
+flag table of SpMM.hpp:28-131 / SpMV.hpp:14-29, empty rows (SpMM.hpp:395-410), the
 binary a == nullptr case and the reference's no-op error cases."""
 import itertools
 
