@@ -44,17 +44,25 @@ def alg_bytes(nnz, m, n, k, sv):
     return nnz * (sv + 4) + 4 * (m + 1) + n * k * sv + m * k * sv
 
 
+def line_fill_bytes(nnz, m, k, sv, line=128):
+    """Traffic when no X line is reused from cache: every nonzero fills the whole 128-byte
+    lines of its k-value X row, plus A and Y once.  For the random-column matrix with X far
+    above the L2s and the Infinity Cache (nrhs 16-64) this is the floor the gathers set."""
+    return nnz * (-(-k * sv // line)) * line + nnz * (sv + 4) + 4 * (m + 1) + m * k * sv
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-reps", type=int, default=20, help="timed calls per CPU baseline case (median)")
+    ap.add_argument("--cpu-warmup", type=int, default=3, help="untimed calls per CPU baseline case")
     ap.add_argument("--no-cg", action="store_true", help="skip the CG (configs 3) section")
     ap.add_argument("--cg-grid", type=int, default=128)
     ap.add_argument("--cg-reps", type=int, default=2)
-    ap.add_argument("--cpu-cg-iters", type=int, default=2, help="oracle CG iterations timed on the host")
+    ap.add_argument("--cpu-cg-iters", type=int, default=20, help="oracle CG iterations timed on the host")
     ap.add_argument("--cg-fused", type=int, default=-1,
                     help="CG iteration form (tg_tune_set cg_fused): 0 four kernels, 1 p.u in the SpMM, "
                          "2 fused direction, -1 library default")
@@ -77,6 +85,10 @@ def parse():
                     help="batched iteration: 1 direction pass + SpMM of p, 0 direction rebuilt in the SpMM")
     ap.add_argument("--batch-seq-sample", type=int, default=32,
                     help="systems solved one at a time by the drop-in FP64 CG for comparison")
+    ap.add_argument("--no-split", action="store_true",
+                    help="do not register the values with the plan (no re-laid slice layout)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuners (library default SpMM shapes and batch tiles)")
     ap.add_argument("--slices", type=int, default=-1,
                     help="column slices of the planned SpMM (tg_tune_set spmm_slices; -1 = library default)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop section")
@@ -87,6 +99,8 @@ def parse():
     ap.add_argument("--no-gmres", action="store_true", help="skip the single-system GMRES section")
     ap.add_argument("--serve-gmres-workers", type=int, default=6,
                     help="GMRES worker threads (own streams) of the serving loop")
+    ap.add_argument("--serve-gmres-batch", type=int, default=1,
+                    help="1: GMRES requests join the batched GMRES engine; 0: one worker stream per request")
     ap.add_argument("--serve-requests", type=int, default=160)
     ap.add_argument("--serve-rates", type=str, default="0.25,0.5,1,2",
                     help="arrival rates as multiples of the measured saturated service rate")
@@ -108,60 +122,100 @@ def read_profile_traffic():
     return vals.get("traffic_bytes_per_launch")
 
 
-def cpu_baseline(rp, ci, v64, v32, X64, X32, budget_s):
-    """The oracle (same source, reference release flags) on the host cores, on the same
-    matrix; each case repeated until the time budget is spent."""
+def host_cpus():
+    """Threads of the CPU baseline and a description of the host: the process's CPU share
+    (its affinity mask, capped by OMP_NUM_THREADS where the box sets it) and what lscpu
+    says about sockets / cores / threads."""
+    try:
+        mask = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        mask = os.cpu_count() or 1
+    share = mask
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        share = min(share, int(env))
+    desc = {"nproc": os.cpu_count(), "affinity_cpus": mask, "threads_used": share}
+    try:
+        import subprocess
+
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            key, _, val = line.partition(":")
+            key = key.strip()
+            if key in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                desc[key] = val.strip()
+    except Exception:
+        pass
+    return share, desc
+
+
+def cpu_baseline(rp, ci, v64, v32, X64, X32, reps, warmup):
+    """The oracle (same source, the reference's release flags) on the host's CPU share,
+    threads pinned, on the same matrix: per case `warmup` untimed calls, then the median
+    of `reps` timed calls (steady clock)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle  # test-side checker; here only as the timed CPU baseline
 
-    threads = min(16, os.cpu_count() or 1)
+    threads, desc = host_cpus()
+    oracle.lib(True).orc_set_pinning(1)
     nnz = int(rp[-1])
-    total_bytes, total_time, reps_done = 0.0, 0.0, 0
-    per_case_budget = budget_s / (2 * len(KS))
+    total_bytes, total_time = 0.0, 0.0
+    cases = []
     for dtype, vals, Xall in ((np.float64, v64, X64), (np.float32, v32, X32)):
         sv = 8 if dtype == np.float64 else 4
         for k in KS:
             X = np.ascontiguousarray(Xall[: N * k].reshape(N, k))
             Y = np.zeros((M, k), dtype=dtype)
-            oracle.spmm(rp, ci, vals, M, N, 1.0, X, k, 0.0, Y, k, k, NRHS=k, threads=threads, fast=True)
-            t0 = time.perf_counter()
-            reps = 0
-            while True:
+            for _ in range(warmup):
                 oracle.spmm(rp, ci, vals, M, N, 1.0, X, k, 0.0, Y, k, k, NRHS=k, threads=threads, fast=True)
-                reps += 1
-                if time.perf_counter() - t0 > per_case_budget:
-                    break
-            dt = (time.perf_counter() - t0) / reps
-            total_bytes += alg_bytes(nnz, M, N, k, sv)
+            times = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                oracle.spmm(rp, ci, vals, M, N, 1.0, X, k, 0.0, Y, k, k, NRHS=k, threads=threads, fast=True)
+                times.append(time.perf_counter() - t0)
+            dt = float(np.median(times))
+            b = alg_bytes(nnz, M, N, k, sv)
+            total_bytes += b
             total_time += dt
-            reps_done += reps
+            cases.append({"dtype": "f64" if sv == 8 else "f32", "nrhs": k, "median_ms": round(dt * 1e3, 2),
+                          "GBps": round(b / dt / 1e9, 2)})
+            log(f"CPU baseline {cases[-1]}")
+    oracle.lib(True).orc_set_pinning(0)
     return {
         "value": round(total_bytes / total_time / 1e9, 3),
         "unit": "GB/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"the same 8 SpMMs (FP64/FP32 x nrhs 1,4,16,64) on the full n=1e6 matrix, "
-                  f"{reps_done} timed calls in ~{budget_s:.0f}s, fixed-NRHS path, -O3 -ffast-math",
+        "sample": f"the same 8 SpMMs (FP64/FP32 x nrhs 1,4,16,64) on the full n=1e6 matrix; per case {warmup} "
+                  f"warm-up calls, then the median of {reps} timed calls; {threads} pinned threads (the "
+                  f"process's CPU share); fixed-NRHS path, -O3 -ffast-math",
+        "host": desc,
+        "cases": cases,
     }
 
 
 def cpu_cg_sample(g, k, iters):
-    """The oracle CG (reference release flags, all host cores) on the CG section's system
-    for a fixed number of iterations; returns seconds per iteration."""
+    """The oracle CG (reference release flags, the host's CPU share, threads pinned, the
+    vector updates threaded as ConjugateGradient.hpp:262-335 does) on the CG section's
+    system: a 3-iteration warm-up call, then one timed call of `iters` iterations;
+    returns (seconds per iteration, threads)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle  # test-side checker; here only as the timed CPU baseline
 
     from tensors_amd import generators as gen
 
-    threads = min(16, os.cpu_count() or 1)
+    threads, _ = host_cpus()
+    oracle.lib(True).orc_set_pinning(1)
     rp, ci, val = gen.laplacian_3d(g, 1e-2)
     n = g ** 3
     B = gen.uniform(3003, n * k).reshape(n, k)
     X = np.zeros((n, k))
-    oracle.cg(n, rp, ci, val, B, X, k, 1, 1e-30, threads=threads, fast=True)  # warm-up
+    oracle.cg(n, rp, ci, val, B, X, k, 3, 1e-30, threads=threads, fast=True)  # warm-up
     t0 = time.perf_counter()
     oracle.cg(n, rp, ci, val, B, X, k, iters, 1e-30, threads=threads, fast=True)
-    return (time.perf_counter() - t0) / iters, threads
+    dt = (time.perf_counter() - t0) / iters
+    oracle.lib(True).orc_set_pinning(0)
+    return dt, threads
 
 
 def cg_section(args, dev):
@@ -247,8 +301,10 @@ def cpu_cg_part(args, g, k, it64):
     per_it, threads = cpu_cg_sample(g, k, args.cpu_cg_iters)
     return {"seconds_per_iteration": round(per_it, 4), "cores": threads, "kind": "port",
             "est_solve_s": round(per_it * it64, 2), "est_solves_per_s": round(1.0 / (per_it * it64), 4),
-            "sample": f"oracle CG (FP64, P = I, -O3 -ffast-math) on the same system, {args.cpu_cg_iters} iterations "
-                      f"timed, solve time extrapolated to the GPU's {it64} iterations"}
+            "sample": f"oracle CG (FP64, P = I, -O3 -ffast-math, SpMM and vector updates threaded over "
+                      f"{threads} pinned threads) on the same system: a 3-iteration warm-up call, then "
+                      f"{args.cpu_cg_iters} iterations timed in one call; solve time extrapolated to the GPU's "
+                      f"{it64} iterations"}
 
 
 def batch_specs(count, seed=4004):
@@ -393,6 +449,18 @@ def batch_section(args, dev):
             "frac_true_rel_residual_le_1e-10": round(float((tr <= 1e-10).mean()), 4)}
 
     log(f"batch: {len(systems)} systems resident, {total_rows} rows, {total_nnz} nnz")
+    tuned = None
+    if not args.no_tune and not args.batch_tile_rows and args.batch_spmm < 0:
+        # the project's autotuner picks the engine's tile height and SpMM form on a sample
+        # of the workload (every 20th system, mixed mode, a fixed number of iterations)
+        from tensors_amd import autotune
+
+        sample = [(s["rp"], s["ci"], s["val"], s["d"], s["B"]) for s in systems[::20]]
+        t_tune = time.perf_counter()
+        best, table = autotune.tune_batch(sample, iters=16, warm=2, mixed=1)
+        tuned = {"chosen": best, "sample_systems": len(sample), "tune_s": round(time.perf_counter() - t_tune, 2),
+                 "ms_per_iteration": [dict(c, ms=round(t, 3)) for c, t in table]}
+        log(f"batch: tuned {best} in {tuned['tune_s']}s")
     X64, r64 = run(0)
     log(f"batch fp64: {r64}")
     Xt, rt = run(2)
@@ -432,6 +500,7 @@ def batch_section(args, dev):
                 "fp64_true_residual_batched and mixed_batched stop when every column has the true "
                 "||b - A x|| / ||b|| <= 1e-10",
         "fp64_dropin_sequential_sample": seq,
+        "autotuned_engine_shape": tuned,
     }
 
 
@@ -475,8 +544,10 @@ def serve_section(args, dev):
     admission budget, host arrays in and out, so latency includes the PCIe transfers).
     First every request at once (saturated service rate), then Poisson arrivals at the
     requested multiples of that rate; p50/p99 latency = completion - arrival."""
+    from tensors_amd import _capi as capi
     from tensors_amd.serving import Server
 
+    capi.call("tg_tune_set", b"serve_gmres_batch", int(args.serve_gmres_batch))
     t_gen = time.perf_counter()
     reqs = serve_requests(args.serve_requests)
     t_gen = time.perf_counter() - t_gen
@@ -546,8 +617,9 @@ def serve_section(args, dev):
         "workload": f"{len(reqs)} requests: {n_cg} SPD CG (weighted 2D/3D grids, n log-uniform [1e3, 2e5], "
                     f"1-8 RHS, Jacobi, FP32 inner + FP64 refinement to 1e-10) and {len(reqs) - n_cg} GMRES(30) "
                     "(2D upwind convection-diffusion, n log-uniform [1e4, 1e6], Peclet log-uniform [1, 100], "
-                    f"P = I, 1e-8, 100 restarts); host arrays in/out; 200 GB admission budget; "
-                    f"{args.serve_gmres_workers} GMRES streams",
+                    f"P = I, 1e-8, 100 restarts); host arrays in/out; 200 GB admission budget; GMRES "
+                    + ("requests in the batched GMRES engine" if args.serve_gmres_batch else
+                       f"requests on {args.serve_gmres_workers} worker streams"),
         "generation_s": round(t_gen, 2),
         "saturated": sat,
         "poisson_sweep": sweep,
@@ -633,7 +705,8 @@ def gmres_section(args, dev):
     import torch
 
     from tensors_amd import generators as gen
-    from tensors_amd.solvers import GMRES
+    from tensors_amd.batch import BatchGMRES
+    from tensors_amd.solvers import GMRES, gmres_solve_mixed
     from tensors_amd.sparse import MatrixCSR
 
     res = []
@@ -652,11 +725,54 @@ def gmres_section(args, dev):
             torch.cuda.synchronize()
             t = time.perf_counter() - t0
         steps = gm.restarts * 30 + gm.iter
+        # the mixed-precision form (FP32 Arnoldi cycles, FP64 residual / update / stop test)
+        tm, okm = 0.0, False
+        for rep_ in range(2):
+            Xm = torch.zeros(n, 1, dtype=torch.float64, device=dev)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            okm, rsm, stepsm, rrm = gmres_solve_mixed(A, B, Xm, tol=1e-8, restart=30, max_restarts=100)
+            torch.cuda.synchronize()
+            tm = time.perf_counter() - t0
         res.append({"grid": g, "peclet": pe, "n": n, "succeeded": bool(ok), "restarts": gm.restarts,
-                    "arnoldi_steps": steps, "ms": round(t * 1e3, 1), "us_per_step": round(t * 1e6 / max(steps, 1), 1)})
+                    "arnoldi_steps": steps, "ms": round(t * 1e3, 1), "us_per_step": round(t * 1e6 / max(steps, 1), 1),
+                    "mixed": {"succeeded": bool(okm), "restarts": rsm, "fp32_arnoldi_steps": stepsm,
+                              "ms": round(tm * 1e3, 1), "true_rel_residual": float(rrm[0]),
+                              "speedup_over_fp64": round(t / tm, 3) if tm > 0 else None}})
         log(f"GMRES {res[-1]}")
+    # many small systems at once: the batched engine against one tg_gmres_solve after another
+    count, gsz = 64, 100
+    systems = []
+    for i in range(count):
+        rp, ci, val = gen.convection_diffusion_2d(gsz, 1.0 + 99.0 * i / (count - 1))
+        n = gsz * gsz
+        systems.append((torch.from_numpy(rp.astype(np.int32)).to(dev), torch.from_numpy(ci).to(dev),
+                        torch.from_numpy(val).to(dev), torch.from_numpy(gen.uniform(70 + i, n)).to(dev).reshape(n, 1),
+                        MatrixCSR(rp, ci, val, n, n)))
+    Xs = [torch.zeros(gsz * gsz, 1, dtype=torch.float64, device=dev) for _ in systems]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for (rpd, cid, vd, B, A), X in zip(systems, Xs):
+        GMRES(gsz * gsz, 30, 1)(A, None, 1.0, B, 0.0, X, 1e-8, 100)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    bt = BatchGMRES()
+    Xb = [torch.zeros(gsz * gsz, 1, dtype=torch.float64, device=dev) for _ in systems]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ids = [bt.add(rpd, cid, vd, B, X, 1e-8, 30, 100) for (rpd, cid, vd, B, A), X in zip(systems, Xb)]
+    bt.run(1)
+    torch.cuda.synchronize()
+    t_bat = time.perf_counter() - t0
+    okb = sum(bt.result(i)["succeeded"] for i in ids)
+    bt.close()
+    batched = {"systems": count, "grid": gsz, "peclet": "1..100", "sequential_solves_per_s": round(count / t_seq, 2),
+               "batched_solves_per_s": round(count / t_bat, 2), "batched_succeeded": int(okb),
+               "speedup": round(t_seq / t_bat, 3)}
+    log(f"GMRES batched {batched}")
     return {"workload": "GMRES(30), 1e-8, P = I, 100 restarts, 2D upwind convection-diffusion, 1 RHS "
-                        "uniform[-1,1] (tg_gmres_solve)", "cases": res}
+                        "uniform[-1,1] (tg_gmres_solve; mixed: tg_gmres_solve_mixed)", "cases": res,
+            "batched_small_systems": batched}
 
 
 def main():
@@ -694,16 +810,44 @@ def main():
     if args.slices >= 0:
         capi.call("tg_tune_set", b"spmm_slices", args.slices)
     plan = SlicePlan(M, N, nnz, rp32, cid)
+    if not args.no_split:
+        # the matrix's values registered with the plan, as MatrixCSR.Dot does: the re-laid
+        # 8-slice layout serves the nrhs <= 4 cases with X of 2-64 MB
+        plan.set_values(vals[np.float64])
+        plan.set_values(vals[np.float32])
     for c in cases:
-        c["slices"] = plan.slices(c["Y"].dtype, c["k"])
+        c["slices"] = plan.slices(c["Y"].dtype, c["k"], a=c["A"])
 
     def launch(c):
         return plan.spmm(c["A"], 1.0, c["X"], c["k"], 0.0, c["Y"], c["k"], c["k"])
 
+    # per-case kernel shapes (lanes per row / RHS tile, rows per lane group, column
+    # slices) from the project's autotuner, set before each case's launch
+    from tensors_amd import autotune
+
+    t_tune = time.perf_counter()
+    for c in cases:
+        if args.no_tune:
+            c["cfg"] = autotune.get_knobs()
+        else:
+            c["cfg"], _ = autotune.tune_shapes(lambda c=c: launch(c), c["k"], reps=2, keep=False)
+    t_tune = time.perf_counter() - t_tune
+    if not args.no_tune:
+        log(f"SpMM shapes tuned in {t_tune:.1f}s")
+
+    def launch_tuned(c):
+        autotune.set_knobs(c["cfg"])
+        return launch(c)
+
+    for c in cases:
+        autotune.set_knobs(c["cfg"])
+        c["slices"] = plan.slices(c["Y"].dtype, c["k"], a=c["A"])
+        c["split"] = plan.uses_split(c["A"], c["k"])
+
     log(f"SpMM suite: nnz={nnz}, {len(cases)} launches per step")
     for _ in range(args.warmup):
         for c in cases:
-            launch(c)
+            launch_tuned(c)
     torch.cuda.synchronize()
     # timed region: events around every launch on the launch stream
     ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in cases]
@@ -712,6 +856,7 @@ def main():
     t0 = time.perf_counter()
     for s in range(args.steps):
         for i, c in enumerate(cases):
+            autotune.set_knobs(c["cfg"])
             ev[s][i][0].record(stream)
             launch(c)
             ev[s][i][1].record(stream)
@@ -750,10 +895,15 @@ def main():
     dc = cases[dom]
     achieved = dc["bytes"] / (per_case_ms[dom] * 1e-3) / 1e9
     traffic = read_profile_traffic()
-    breakdown = [{"dtype": c["dtype"], "nrhs": c["k"], "slices": c["slices"], "ms": round(t, 4),
+    for c in cases:
+        c["lf_bytes"] = line_fill_bytes(nnz, M, c["k"], 8 if c["dtype"] == "f64" else 4)
+    breakdown = [{"dtype": c["dtype"], "nrhs": c["k"], "slices": c["slices"], "relaid": c["split"], "shape": c["cfg"],
+                  "ms": round(t, 4),
                   "GBps": round(c["bytes"] / (t * 1e-3) / 1e9, 1),
                   "frac_peak": round(c["bytes"] / (t * 1e-3) / 1e9 / PEAK_HBM_GBPS, 4),
-                  "frac_measured_read": round(c["bytes"] / (t * 1e-3) / 1e9 / read_gbps, 4)}
+                  "frac_measured_read": round(c["bytes"] / (t * 1e-3) / 1e9 / read_gbps, 4),
+                  "line_fill_bytes": c["lf_bytes"],
+                  "frac_peak_line_fill": round(c["lf_bytes"] / (t * 1e-3) / 1e9 / PEAK_HBM_GBPS, 4)}
                  for c, t in zip(cases, per_case_ms)]
     out = {
         "metric": METRIC,
@@ -775,7 +925,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": traffic,
                      "kernel": f"spmm {dc['dtype']} nrhs={dc['k']}",
-                     "alg_bytes_per_launch": dc["bytes"]},
+                     "alg_bytes_per_launch": dc["bytes"],
+                     "line_fill_bytes_per_launch": dc["lf_bytes"],
+                     "achieved_line_fill": round(dc["lf_bytes"] / (per_case_ms[dom] * 1e-3) / 1e9, 1),
+                     "frac_line_fill": round(dc["lf_bytes"] / (per_case_ms[dom] * 1e-3) / 1e9 / PEAK_HBM_GBPS, 4)},
         "hbm_measured": {"copy_GBps": round(copy_gbps, 1), "read_GBps": round(read_gbps, 1)},
         "breakdown": breakdown,
     }
@@ -806,7 +959,7 @@ def main():
         out["serve"] = serve_section(args, dev)
     if not args.no_cpu:
         log("CPU baseline")
-        out["cpu_baseline"] = cpu_baseline(rp, ci, v64, v32, X64h, X32h, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(rp, ci, v64, v32, X64h, X32h, args.cpu_reps, args.cpu_warmup)
     print(json.dumps(out), flush=True)
 
 

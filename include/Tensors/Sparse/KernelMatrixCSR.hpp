@@ -90,7 +90,7 @@ template <typename Kernel_T> class KernelMatrixCSR final {
         if (!yd)
             dY.Download(Y, nY);
         else
-            Device::Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+            Device::Check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
     }
 
     // FillLowerTriangleFromUpperTriangle (KernelMatrixCSR.hpp:71-128): every block
@@ -113,7 +113,7 @@ template <typename Kernel_T> class KernelMatrixCSR final {
         if (!vd)
             dA.Download(values, nA);
         else
-            Device::Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+            Device::Check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
     }
 
     std::string ClassName() const { return "Sparse::KernelMatrixCSR<" + Kernel_T::ClassName() + ">"; }

@@ -5,13 +5,18 @@
 // Host arrays (outer, inner, values) stay authoritative, as in the reference.
 // A device mirror is uploaded on first use and reused -- the same lazy-cache
 // pattern as PatternCSR::RequireJobPtr (src/Sparse/PatternCSR.hpp:389-398);
-// the non-const accessors Outer()/Inner()/Values() drop the mirror.  Like the
-// reference's lazy caches, the first call on one matrix from two threads at
-// once is not safe.
+// the non-const accessors Outer()/Inner()/Values() drop the mirror.  Const calls
+// on one matrix from several host threads (Dot, the first of which uploads the
+// mirror) are serialised by a per-matrix mutex (mirror_mutex).  Writing through
+// a reference that Values(), Inner() or Outer() returned earlier while another
+// thread calls Dot is still a data race, as it is in the reference.
 //
 // Dot accepts host or device pointers for X and Y (told apart with
 // hipPointerGetAttributes).  Device pointers are used in place; host blocks
-// are staged through device buffers the matrix keeps.  Error behaviour is the
+// are staged through device buffers the matrix keeps.  Like the reference's
+// Dot, a call returns with Y complete: it waits for the null stream it ran on
+// (not for the whole device, so work on the caller's other non-blocking streams
+// keeps running).  Error behaviour is the
 // reference's: shape mismatch, X == Y and NRHS != nrhs print a message and do
 // nothing.  Supported: Scal = double or float with X, Y of the same type;
 // complex scalars and base-1 indices are not on this path.
@@ -377,11 +382,18 @@ template <typename Scal_, typename Int_, typename LInt_, Parallel_T parQ_ = Para
         }
         const bool xd = X == nullptr || Device::IsDevicePointer(X), yd = Device::IsDevicePointer(Y);
         const std::string where = MethodName("Dot");
+        // the matrix's own values get the plan's re-laid copy on the first product whose
+        // shape uses it (nrhs <= 4, X of 2-64 MB); changing the values drops the plan
+        const double xbytes = (double)n * (double)nrhs * (double)sizeof(Scal);
+        if (!ext_values && !plan.split && nrhs <= 4 && xbytes > 2.0 * (1 << 20) && xbytes <= 64.0 * (1 << 20)) {
+            Device::Status(tg_csr_plan_set_values(DevicePlan(), d_val.data(), Device::DType<Scal>(), nullptr), where);
+            plan.split = true;
+        }
         if (xd && yd) {
             Device::Status(tg_csr_spmm_plan(DevicePlan(), vals, Device::DType<Scal>(), (double)alpha, X, (int64_t)ldX,
                                             (double)beta, Y, (int64_t)ldY, (int32_t)nrhs, (int32_t)NRHS, nullptr),
                            where);
-            Device::Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+            Device::Check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
             return;
         }
         if (NRHS > 0 && nrhs != NRHS) {
@@ -410,7 +422,7 @@ template <typename Scal_, typename Int_, typename LInt_, Parallel_T parQ_ = Para
         if (st == TG_OK && !yd)
             Device::Copy2D(Y, (Size_T)ldY, Yd, (Size_T)nrhs, (Size_T)m, (Size_T)nrhs, hipMemcpyDeviceToHost);
         else
-            Device::Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+            Device::Check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
     }
 
     template <typename ExtInt, typename ExtScal>
@@ -478,9 +490,13 @@ template <typename Scal_, typename Int_, typename LInt_, Parallel_T parQ_ = Para
     // the device arrays of the matrix that built it), a moved one takes it along.
     struct PlanHolder {
         mutable tg_csr_plan_t p = nullptr;
+        mutable bool split = false;  // the own values are registered (tg_csr_plan_set_values)
         PlanHolder() = default;
         PlanHolder(const PlanHolder&) {}
-        PlanHolder(PlanHolder&& o) noexcept : p(o.p) { o.p = nullptr; }
+        PlanHolder(PlanHolder&& o) noexcept : p(o.p), split(o.split) {
+            o.p = nullptr;
+            o.split = false;
+        }
         PlanHolder& operator=(const PlanHolder&) {
             Reset();
             return *this;
@@ -489,7 +505,9 @@ template <typename Scal_, typename Int_, typename LInt_, Parallel_T parQ_ = Para
             if (this != &o) {
                 Reset();
                 p = o.p;
+                split = o.split;
                 o.p = nullptr;
+                o.split = false;
             }
             return *this;
         }
@@ -497,6 +515,7 @@ template <typename Scal_, typename Int_, typename LInt_, Parallel_T parQ_ = Para
         void Reset() const {
             if (p) tg_csr_plan_destroy(p);
             p = nullptr;
+            split = false;
         }
     };
 

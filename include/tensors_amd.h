@@ -104,12 +104,29 @@ int tg_csr_spmm(int32_t m, int32_t n, int64_t nnz, const void* rp, tg_dtype rp_t
  * a larger shape; a captured call never allocates: it returns TG_ERR_WORKSPACE
  * (nothing enqueued) unless one call of that shape (same nrhs and value type, or
  * a larger one) ran on the plan outside capture first.
- * tg_csr_plan_slices returns the slice count it would use (0 = plain). */
+ * tg_csr_plan_slices returns the slice count it would use (0 = plain) for values
+ * that are not registered (below).
+ *
+ * tg_csr_plan_set_values registers the value array `a` of one value type (NULL
+ * drops the registration): the plan then keeps a re-laid copy of the pattern and
+ * of those values in which each of 8 column slices (one per XCD) is a CSR of its
+ * own, stored slice after slice, so the workgroups of one XCD stream only their
+ * slice's nonzeros instead of touching every cache line of A for a short run per
+ * row.  Calls of tg_csr_spmm_plan with exactly that pointer and an X block of
+ * 2-64 MB (nrhs <= 4) use it (8 slices; the same run sums in the same order as
+ * the 8-slice cut path); any other pointer takes the paths above.  The plan
+ * copies the values when the call is made: if the values at `a` change, call
+ * tg_csr_plan_set_values again (the drop-in MatrixCSR does, since changing its
+ * values drops its plan).  Extra device memory: 4 (8m + 1) + 4 nnz bytes for
+ * the pattern and nnz * sizeof(value) per registered type.  Synchronises
+ * `stream`.  tg_csr_plan_uses_split tells whether a call with `a` would use it. */
 typedef struct tg_csr_plan_s* tg_csr_plan_t;
 int tg_csr_plan_create(int32_t m, int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, const int32_t* ci,
                        tg_csr_plan_t* out, void* stream);
 int tg_csr_plan_destroy(tg_csr_plan_t plan);
 int32_t tg_csr_plan_slices(tg_csr_plan_t plan, tg_dtype val_type, int32_t nrhs);
+int tg_csr_plan_set_values(tg_csr_plan_t plan, const void* a, tg_dtype val_type, void* stream);
+int32_t tg_csr_plan_uses_split(tg_csr_plan_t plan, const void* a, tg_dtype val_type, int32_t nrhs);
 int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, double alpha, const void* X, int64_t ldX,
                      double beta, void* Y, int64_t ldY, int32_t nrhs, int32_t NRHS, void* stream);
 
@@ -125,8 +142,11 @@ int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, doubl
  *                  row per lane group), 1 = the wave's lane groups split one row,
  *                  2 = each lane group owns a row.
  *   "spmm_slices": column slices of tg_csr_spmm_plan, 0 = from the size of X
- *                  (default: 2 or 4 slices for X blocks of 6-32 MB, else none),
+ *                  (default: the re-laid 8 slices for registered values and X of
+ *                  2-64 MB, else 2 or 4 slices for X blocks of 6-32 MB, else none),
  *                  1 = never, 2 / 4 / 8 = always that many (nrhs <= 4).
+ *   "spmm_split":  1 = registered values use the re-laid layout (default),
+ *                  0 = never.
  *   "batch_tile_rows": rows per workgroup tile of systems added to a batched
  *                  engine from now on, a power of two in [256, 8192] (1024).
  *   "batch_stream": SpMM kernel of the batched engine's fused form, 0 = a row
@@ -239,6 +259,36 @@ int tg_cg_solve_mixed(int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, 
                       const double* B, int64_t ldB, double b_coef, double* X, int64_t ldX, double tol,
                       int32_t use_guess, void* workspace, size_t workspace_bytes, int32_t* succeeded,
                       int32_t* outer_iterations, int32_t* inner_iterations, double* rel_residuals, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Mixed-precision restarted GMRES(m): every cycle's Arnoldi process runs in
+ * FP32 (FP32 values a32 on the same pattern, FP32 basis and Krylov vectors)
+ * while the Hessenberg matrix, Givens rotations, beta and least squares of the
+ * cycle are FP64, and every restart recomputes the residual b - A x in FP64
+ * with the solution x kept and updated in FP64 (iterative refinement, one
+ * FP32 GMRES(m) cycle per refinement step).  No reference counterpart (the
+ * reference GMRES is precision-agnostic); it solves what tg_gmres_solve solves
+ * (GMRES.hpp:127-260):
+ *
+ *   X <- a_coef * A^{-1} B + b_coef * X      (FP64 in and out)
+ *
+ * Stops when every column has ||P (B - A x)|| <= tol ||P B|| (P = the diagonal
+ * for side 0 with prec_kind 1, else the identity; the reference's test on the
+ * true FP64 residual), or after max_restarts + 1 cycles.  a32 may be NULL (the
+ * FP64 values are rounded into the workspace; pass with_a32 = 0 to the
+ * workspace query then).  m = min(max_iter, n + 1) <= 63, nrhs <= 32.
+ * *restarts counts the cycles after the first, *inner_iterations the FP32
+ * Arnoldi steps over all cycles, rel_residuals[nrhs] the final true relative
+ * residuals.  Synchronises `stream`.
+ * ------------------------------------------------------------------------ */
+size_t tg_gmres_mixed_workspace_bytes(int32_t n, int64_t nnz, int32_t max_iter, int32_t nrhs, int32_t prec_kind,
+                                      int32_t with_a32);
+int tg_gmres_solve_mixed(int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, const int32_t* ci,
+                         const double* a64, const float* a32, int32_t prec_kind, const double* prec_diag,
+                         int32_t side, int32_t nrhs, int32_t max_iter, int32_t max_restarts, double a_coef,
+                         const double* B, int64_t ldB, double b_coef, double* X, int64_t ldX, double tol,
+                         int32_t use_guess, void* workspace, size_t workspace_bytes, int32_t* succeeded,
+                         int32_t* restarts, int32_t* inner_iterations, double* rel_residuals, void* stream);
 
 /* ------------------------------------------------------------------------
  * Restarted GMRES(m) -- replaces GMRES<NRHS,Scal,Int,Side>::operator()(A, P,
@@ -367,6 +417,42 @@ int tg_batch_release(tg_batch_t batch, int64_t id);
 int tg_batch_active(tg_batch_t batch, int32_t* systems, int64_t* bytes_in_use);
 /* Device bytes the engine itself allocates for one such system. */
 size_t tg_batch_request_bytes(int32_t n, int64_t nnz, int32_t nrhs, int32_t has_diag, int32_t mixed);
+
+/* ------------------------------------------------------------------------
+ * Batched, continuously fed GMRES(m) (no single reference counterpart: it runs
+ * many GMRES::operator() solves, GMRES.hpp:127-260, at once, with a = 1, b = 0,
+ * no initial guess, Side::Left).  Each system: n x n CSR (int32 rp/ci, FP64
+ * values, device), optional left diagonal preconditioner `inv_diag` (z = d r),
+ * nrhs in 1..8, restart length m = min(restart, n + 1) <= 63, B and X n x nrhs
+ * row-major device blocks (ld = nrhs); X <- A^{-1} B.  The same recurrences,
+ * stop test (|beta_iter|_k <= tol ||P b||_k) and restart rule as tg_gmres_solve
+ * (classical Gram-Schmidt applied twice; Hessenberg, Givens and back
+ * substitution in FP64 on the device).
+ * All active systems run restart cycles in lock step: a cycle is 4 launches per
+ * Arnoldi step plus 5 for ALL systems (captured as one hipGraph), and a
+ * system's arithmetic does not depend on the others, so its result is bitwise
+ * the same alone and in any mix.  tg_gbatch_add only enqueues the system's setup
+ * (no host wait); the system joins at the next cycle.  tg_gbatch_step runs up to
+ * `cycles` cycles and synchronises the batch's stream.  Results: succeeded,
+ * iterations of the last cycle, restarts (cycles after the first) and
+ * rel_residuals[nrhs] = |beta_iter| / ||P b||.  A NaN right-hand side finishes
+ * at once with nothing written; an all-zero one with X = 0, succeeded.  The
+ * caller's arrays must stay valid until the system has finished.
+ * ------------------------------------------------------------------------ */
+typedef struct tg_gbatch_s* tg_gbatch_t;
+int tg_gbatch_create(tg_gbatch_t* out, void* stream);
+int tg_gbatch_destroy(tg_gbatch_t batch);
+int tg_gbatch_add(tg_gbatch_t batch, int32_t n, int64_t nnz, const int32_t* rp, const int32_t* ci, const double* val,
+                  const double* inv_diag, int32_t nrhs, const double* B, double* X, double tol, int32_t restart,
+                  int32_t max_restarts, int64_t* id_out);
+int tg_gbatch_step(tg_gbatch_t batch, int32_t cycles);
+int tg_gbatch_finished(tg_gbatch_t batch, int64_t* ids, int32_t cap, int32_t* count);
+int tg_gbatch_result(tg_gbatch_t batch, int64_t id, int32_t* finished, int32_t* succeeded, int32_t* iterations,
+                     int32_t* restarts, double* rel_residuals);
+int tg_gbatch_release(tg_gbatch_t batch, int64_t id);
+int tg_gbatch_active(tg_gbatch_t batch, int32_t* systems, int64_t* bytes_in_use);
+/* Device bytes the engine allocates for one such system. */
+size_t tg_gbatch_request_bytes(int32_t n, int32_t nrhs, int32_t restart);
 
 /* ------------------------------------------------------------------------
  * Serving loop (no reference counterpart).  Requests carry HOST arrays (int32

@@ -34,7 +34,11 @@
 // available); structure (rp, ci) is pinned bit-exact.
 // =============================================================================
 
+#include <pthread.h>
+#include <sched.h>
+
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -52,15 +56,41 @@ using i64 = int64_t;
 // Fork-join over `threads` workers: the role of Tools' ParallelDo
 // (used e.g. at src/SparseBLAS/SpMM.hpp:286-414).
 // ---------------------------------------------------------------------------
+// With pinning on (orc_set_pinning, used by the timed CPU baseline) worker t runs
+// on the t-th CPU of the process's affinity mask; the results do not depend on it.
+static std::atomic<int> g_pin{0};
+
+static void pin_to(int t) {
+    cpu_set_t mask;
+    if (sched_getaffinity(0, sizeof(mask), &mask) != 0) return;
+    const int cpus = CPU_COUNT(&mask);
+    if (cpus <= 0) return;
+    int want = t % cpus, seen = 0;
+    for (int c = 0; c < CPU_SETSIZE; ++c)
+        if (CPU_ISSET(c, &mask) && seen++ == want) {
+            cpu_set_t one;
+            CPU_ZERO(&one);
+            CPU_SET(c, &one);
+            (void)pthread_setaffinity_np(pthread_self(), sizeof(one), &one);
+            return;
+        }
+}
+
 template <class F>
 static void parallel_do(int threads, F&& f) {
     if (threads <= 1) {
         f(0);
         return;
     }
+    const bool pin = g_pin.load() != 0;
     std::vector<std::thread> pool;
     pool.reserve(threads - 1);
-    for (int t = 1; t < threads; ++t) pool.emplace_back([&f, t] { f(t); });
+    for (int t = 1; t < threads; ++t)
+        pool.emplace_back([&f, t, pin] {
+            if (pin) pin_to(t);
+            f(t);
+        });
+    if (pin) pin_to(0);
     f(0);
     for (auto& th : pool) th.join();
 }
@@ -587,16 +617,22 @@ struct Op {
         multiply_dense<double>(rp, ci, a, n, n, 1.0, x, nrhs, 0.0, y, nrhs, nrhs, 0, threads);
     }
 };
+// z = P r, rows split over `threads` uniform blocks (elementwise: the split does not
+// change the result).
 struct Prec {
     int kind;
     const double* d;
+    i32 threads = 1;
     void apply(const double* r, double* z, i64 n, i32 nrhs) const {
-        if (kind == 1) {
-            for (i64 i = 0; i < n; ++i)
-                for (i32 k = 0; k < nrhs; ++k) z[i * nrhs + k] = d[i] * r[i * nrhs + k];
-        } else {
-            std::memcpy(z, r, sizeof(double) * n * nrhs);
-        }
+        parallel_do(threads, [&](int t) {
+            const i64 i0 = job_pointer(n, threads, t), i1 = job_pointer(n, threads, t + 1);
+            if (kind == 1) {
+                for (i64 i = i0; i < i1; ++i)
+                    for (i32 k = 0; k < nrhs; ++k) z[i * nrhs + k] = d[i] * r[i * nrhs + k];
+            } else {
+                std::memcpy(z + i0 * nrhs, r + i0 * nrhs, sizeof(double) * (i1 - i0) * nrhs);
+            }
+        });
     }
 };
 
@@ -690,21 +726,27 @@ static CGResult conjugate_gradient(const Op& A, const Prec& P, i32 n, i32 nrhs, 
         dots(p.data(), u.data(), n, nrhs, T, alpha.data());
         for (i32 k = 0; k < nrhs; ++k) alpha[k] = rho[k] / alpha[k];
         const bool first = (iter == 0) && !use_guess;
-        for (i64 i = 0; i < n; ++i)
-            for (i32 k = 0; k < nrhs; ++k) {
-                const i64 e = i * nrhs + k;
-                x[e] = first ? alpha[k] * p[e] : x[e] + alpha[k] * p[e];
-                r[e] -= alpha[k] * u[e];
-            }
+        // x and r updates over the threads' row blocks (ConjugateGradient.hpp:262-303)
+        parallel_do(T, [&](int t) {
+            for (i64 i = job_pointer(n, T, t); i < job_pointer(n, T, t + 1); ++i)
+                for (i32 k = 0; k < nrhs; ++k) {
+                    const i64 e = i * nrhs + k;
+                    x[e] = first ? alpha[k] * p[e] : x[e] + alpha[k] * p[e];
+                    r[e] -= alpha[k] * u[e];
+                }
+        });
         P.apply(r.data(), z.data(), n, nrhs);
         std::swap(rho_old, rho);
         dots(r.data(), z.data(), n, nrhs, T, rho.data());
         for (i32 k = 0; k < nrhs; ++k) beta[k] = rho[k] / rho_old[k];
-        for (i64 i = 0; i < n; ++i)
-            for (i32 k = 0; k < nrhs; ++k) {
-                const i64 e = i * nrhs + k;
-                p[e] = z[e] + beta[k] * p[e];
-            }
+        // p = z + beta p (:323-335)
+        parallel_do(T, [&](int t) {
+            for (i64 i = job_pointer(n, T, t); i < job_pointer(n, T, t + 1); ++i)
+                for (i32 k = 0; k < nrhs; ++k) {
+                    const i64 e = i * nrhs + k;
+                    p[e] = z[e] + beta[k] * p[e];
+                }
+        });
         succeeded = check();
         ++iter;
     }
@@ -962,6 +1004,9 @@ static CSR make_csr(i64 m, i64 n, const i64* rp, const i32* ci, const double* va
 // =============================================================================
 extern "C" {
 
+// Pins the worker threads of parallel regions to CPUs (1) or not (0, the default).
+void orc_set_pinning(int32_t on) { orc::g_pin.store(on ? 1 : 0); }
+
 int64_t orc_job_pointer(int64_t n, int64_t T, int64_t t) { return orc::job_pointer(n, T, t); }
 
 void orc_job_pointers_costs(int64_t m, const int64_t* acc, int64_t T, int64_t* out) {
@@ -1070,7 +1115,7 @@ int orc_cg(int32_t n, const int64_t* rp, const int32_t* ci, const double* val, i
            int64_t ldB, double b, double* X, int64_t ldX, double tol, int32_t use_guess, int32_t* iter,
            double* rel_res) {
     orc::Op A{rp, ci, val, n, threads};
-    orc::Prec P{prec_kind, prec_diag};
+    orc::Prec P{prec_kind, prec_diag, threads};
     auto r = orc::conjugate_gradient(A, P, n, nrhs, max_iter, threads, a, B, ldB, b, X, ldX, tol,
                                      use_guess != 0, rel_res);
     if (iter) *iter = r.iter;
@@ -1083,7 +1128,7 @@ int orc_cg_ex(int32_t n, const int64_t* rp, const int32_t* ci, const double* val
               int64_t ldB, double b, double* X, int64_t ldX, double tol, int32_t use_guess, int32_t* iter,
               double* rel_res, double* abs_res) {
     orc::Op A{rp, ci, val, n, threads};
-    orc::Prec P{prec_kind, prec_diag};
+    orc::Prec P{prec_kind, prec_diag, threads};
     auto r = orc::conjugate_gradient(A, P, n, nrhs, max_iter, threads, a, B, ldB, b, X, ldX, tol,
                                      use_guess != 0, rel_res, abs_res);
     if (iter) *iter = r.iter;

@@ -35,6 +35,8 @@ SIGNATURES = {
     "tg_csr_plan_create": (C.c_int, [i32, i32, i64, vp, C.c_int, vp, vp, vp]),
     "tg_csr_plan_destroy": (C.c_int, [vp]),
     "tg_csr_plan_slices": (i32, [vp, C.c_int, i32]),
+    "tg_csr_plan_set_values": (C.c_int, [vp, vp, C.c_int, vp]),
+    "tg_csr_plan_uses_split": (i32, [vp, vp, C.c_int, i32]),
     "tg_csr_spmm_plan": (C.c_int, [vp, vp, C.c_int, dbl, vp, i64, dbl, vp, i64, i32, i32, vp]),
     "tg_tune_set": (C.c_int, [C.c_char_p, i32]),
     "tg_tune_get": (i32, [C.c_char_p]),
@@ -69,6 +71,18 @@ SIGNATURES = {
     "tg_cg_mixed_workspace_bytes": (C.c_size_t, [i32, i64, i32, i32, i32]),
     "tg_cg_solve_mixed": (C.c_int, [i32, i64, vp, C.c_int, vp, vp, vp, i32, vp, i32, i32, i32, dbl, dbl, vp, i64,
                                     dbl, vp, i64, dbl, i32, vp, C.c_size_t, vp, vp, vp, vp, vp]),
+    "tg_gmres_mixed_workspace_bytes": (C.c_size_t, [i32, i64, i32, i32, i32, i32]),
+    "tg_gmres_solve_mixed": (C.c_int, [i32, i64, vp, C.c_int, vp, vp, vp, i32, vp, i32, i32, i32, i32, dbl, vp,
+                                       i64, dbl, vp, i64, dbl, i32, vp, C.c_size_t, vp, vp, vp, vp, vp]),
+    "tg_gbatch_create": (C.c_int, [vp, vp]),
+    "tg_gbatch_destroy": (C.c_int, [vp]),
+    "tg_gbatch_add": (C.c_int, [vp, i32, i64, vp, vp, vp, vp, i32, vp, vp, dbl, i32, i32, vp]),
+    "tg_gbatch_step": (C.c_int, [vp, i32]),
+    "tg_gbatch_finished": (C.c_int, [vp, vp, i32, vp]),
+    "tg_gbatch_result": (C.c_int, [vp, i64, vp, vp, vp, vp, vp]),
+    "tg_gbatch_release": (C.c_int, [vp, i64]),
+    "tg_gbatch_active": (C.c_int, [vp, vp, vp]),
+    "tg_gbatch_request_bytes": (C.c_size_t, [i32, i32, i32]),
     "tg_batch_create": (C.c_int, [vp, vp]),
     "tg_batch_destroy": (C.c_int, [vp]),
     "tg_batch_add_cg": (C.c_int, [vp, i32, i64, vp, vp, vp, vp, i32, vp, vp, dbl, i32, i32, vp]),

@@ -22,7 +22,7 @@ import torch
 from . import _capi as capi
 from .sparse import MatrixCSR
 
-KNOBS = ("spmv_kernel", "spmv_team", "spmm_group", "spmm_rows", "spmm_slices")
+KNOBS = ("spmv_kernel", "spmv_team", "spmm_group", "spmm_rows", "spmm_slices", "spmm_split")
 
 
 def get_knobs():
@@ -51,8 +51,46 @@ def candidates(nrhs):
 
 
 def slice_candidates(nrhs):
-    """Column-slice counts of the planned path (nrhs <= 4 only)."""
-    return [{"spmm_slices": s} for s in ((1, 2, 4, 8) if nrhs <= 4 else (1,))]
+    """Column-slice choices of the planned path (nrhs <= 4 only): none, the 2/4/8-slice
+    cut of the original layout, and the re-laid 8-slice layout (used where values are
+    registered with the plan and X is 2-64 MB)."""
+    if nrhs > 4:
+        return [{"spmm_slices": 1, "spmm_split": 0}]
+    return [{"spmm_slices": s, "spmm_split": 0} for s in (1, 2, 4, 8)] + [{"spmm_slices": 0, "spmm_split": 1}]
+
+
+def tune_shapes(launch, nrhs, reps=5, keep=True):
+    """Times launch() -- one product enqueued on the current stream -- for every kernel
+    shape with column slicing off, then every slice count with the best shape; returns
+    (best_cfg, table of (cfg, ms)) and leaves the best set (keep) or restores the
+    previous knobs."""
+    stream = torch.cuda.current_stream()
+    before = get_knobs()
+    table = []
+
+    def timed(cfg):
+        set_knobs(cfg)
+        launch()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record(stream)
+        for _ in range(reps):
+            launch()
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        table.append((dict(cfg), ev[0].elapsed_time(ev[1]) / reps))
+        return table[-1][1]
+
+    try:
+        shapes = [dict(c, spmm_slices=1, spmm_split=0) for c in candidates(nrhs)]
+        best = min(shapes, key=timed)
+        for sc in slice_candidates(nrhs)[1:]:
+            timed(dict(best, **sc))
+    finally:
+        set_knobs(before)
+    best = min(table, key=lambda t: t[1])[0]
+    if keep:
+        set_knobs(best)
+    return best, table
 
 
 def tune_spmm(A: MatrixCSR, X: torch.Tensor, Y: torch.Tensor, reps: int = 5, keep: bool = True):
@@ -65,27 +103,53 @@ def tune_spmm(A: MatrixCSR, X: torch.Tensor, Y: torch.Tensor, reps: int = 5, kee
     plan = A.Plan()
     ld_x = 1 if X.dim() == 1 else X.stride(0)
     ld_y = 1 if Y.dim() == 1 else Y.stride(0)
-    stream = torch.cuda.current_stream()
-    before = get_knobs()
+    return tune_shapes(lambda: plan.spmm(vals, 1.0, X, ld_x, 0.0, Y, ld_y, nrhs), nrhs, reps, keep)
+
+
+BATCH_KNOBS = ("batch_tile_rows", "batch_spmm")
+
+
+def get_batch_knobs():
+    return {k: int(capi.lib().tg_tune_get(k.encode())) for k in BATCH_KNOBS}
+
+
+def batch_candidates(tile_rows=(256, 512, 1024, 2048), spmm_forms=(2, 1, 0)):
+    """Rows per workgroup tile x SpMM kernel of the batched CG engine.  Every SpMM form
+    gives bitwise the same iterates; the tile height changes the order in which a
+    system's tile partials are added (rounding only)."""
+    return [{"batch_tile_rows": t, "batch_spmm": f} for t in tile_rows for f in spmm_forms]
+
+
+def tune_batch(systems, iters=24, warm=4, mixed=1, candidates=None, keep=True):
+    """Times the batched CG engine on the caller's systems for every candidate shape and
+    keeps the fastest.  systems: list of (rp, ci, val, diag, B) CUDA tensors (int32 rp/ci,
+    float64 values / diagonal / right-hand sides); each candidate runs a fresh batch that
+    holds all of them for `warm` untimed and `iters` timed iterations (tol = 0, so no
+    system leaves early) in mode `mixed` (0 FP64, 1 mixed).  Returns (best_cfg, table of
+    (cfg, ms per iteration)); the knobs apply to systems added after the call."""
+    import time
+
+    from .batch import BatchCG
+
+    cands = candidates if candidates is not None else batch_candidates()
+    before = get_batch_knobs()
     table = []
-
-    def timed(cfg):
-        set_knobs(cfg)
-        plan.spmm(vals, 1.0, X, ld_x, 0.0, Y, ld_y, nrhs)
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-        ev[0].record(stream)
-        for _ in range(reps):
-            plan.spmm(vals, 1.0, X, ld_x, 0.0, Y, ld_y, nrhs)
-        ev[1].record(stream)
-        torch.cuda.synchronize()
-        table.append((dict(cfg), ev[0].elapsed_time(ev[1]) / reps))
-        return table[-1][1]
-
     try:
-        shapes = [dict(c, spmm_slices=1) for c in candidates(nrhs)]
-        best = min(shapes, key=timed)
-        for sc in slice_candidates(nrhs)[1:]:
-            timed(dict(best, **sc))
+        for cfg in cands:
+            set_knobs(cfg)
+            bt = BatchCG()
+            Xs = []
+            for rp, ci, val, d, B in systems:
+                X = torch.zeros_like(B)
+                Xs.append(X)
+                bt.add(rp, ci, val, B, X, 0.0, warm + iters + 1, diag=d, mixed=mixed == 1)
+            bt.step(warm)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            bt.step(iters)  # synchronises the batch's stream
+            table.append((dict(cfg), (time.perf_counter() - t0) * 1e3 / iters))
+            bt.close()
+            del Xs
     finally:
         set_knobs(before)
     best = min(table, key=lambda t: t[1])[0]

@@ -303,6 +303,19 @@ __global__ __launch_bounds__(kBlock) void gm_scale_kernel(int n, int k, T* __res
         q[e] = q[e] * (T)inv[e % k];
 }
 
+// q *= inv after Arnoldi step j: runs when step j ran, including the step that ended
+// the cycle (the reference normalises every new basis vector, GMRES.hpp:589-598, so
+// GetBasis sees it normalised); returns at once for steps after the cycle ended.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void gm_scale_step_kernel(int n, int k, T* __restrict__ q,
+                                                               const double* __restrict__ inv,
+                                                               const GmresHdr* __restrict__ hd, int j) {
+    if (hd->done && hd->iter != j + 1) return;
+    const int64_t total = (int64_t)n * k;
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < total; e += (int64_t)gridDim.x * kBlock)
+        q[e] = q[e] * (T)inv[e % k];
+}
+
 // v = P u (diag) or copy
 template <typename T>
 __global__ __launch_bounds__(kBlock) void gm_prec_kernel(int n, int k, const T* __restrict__ u, const T* __restrict__ d,
@@ -652,8 +665,8 @@ int gmres_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d,
         hipLaunchKernelGGL(gm_givens_kernel, dim3(1), dim3(kBlock), 0, st, j, mm, k, gb, (const double*)w.partial,
                            w.tot, w.small + L.H, w.small + L.cs, w.small + L.sn, w.small + L.beta,
                            (const double*)(w.small + L.TOL), w.small + L.inv, w.hd);
-        hipLaunchKernelGGL((gm_scale_kernel<T>), dim3(ge), dim3(kBlock), 0, st, n, k, q,
-                           (const double*)(w.small + L.inv), (const int*)&w.hd->done);
+        hipLaunchKernelGGL((gm_scale_step_kernel<T>), dim3(ge), dim3(kBlock), 0, st, n, k, q,
+                           (const double*)(w.small + L.inv), (const GmresHdr*)w.hd, j);
         return TG_OK;
     };
     struct CycleGraph {

@@ -40,6 +40,8 @@ int bsr_tune_set(const char* key, int32_t value, bool* known);
 int32_t bsr_tune_get(const char* key);
 int gmres_tune_set(const char* key, int32_t value, bool* known);
 int32_t gmres_tune_get(const char* key);
+int serve_tune_set(const char* key, int32_t value, bool* known);
+int32_t serve_tune_get(const char* key);
 
 // Y <- alpha A X + beta Y with the SpMM kernels; when `skip` is non-null and
 // *skip != 0 at kernel start, the kernels return without touching Y.

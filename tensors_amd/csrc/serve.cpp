@@ -12,9 +12,11 @@
 // keeps every system's arithmetic to itself, and a GMRES solve runs alone on
 // its stream.  The loop is host code only; it launches no kernels itself.
 
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <string>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -25,6 +27,22 @@
 #include "krylov.hpp"
 
 namespace tg {
+
+// GMRES requests of servers created from now on: 1 = one batched GMRES engine
+// (tg_gbatch_*) advanced by a dedicated thread, 0 = each request on one of the
+// gmres_workers threads with tg_gmres_solve.
+static int g_serve_gmres_batch = 1;
+
+int serve_tune_set(const char* key, int32_t value, bool* known) {
+    *known = std::string(key) == "serve_gmres_batch";
+    if (!*known) return TG_OK;
+    if (value != 0 && value != 1) return fail(TG_ERR_INVALID, "tg_tune_set: serve_gmres_batch must be 0 or 1");
+    g_serve_gmres_batch = value;
+    return TG_OK;
+}
+
+int32_t serve_tune_get(const char* key) { return std::string(key) == "serve_gmres_batch" ? g_serve_gmres_batch : -1; }
+
 namespace {
 
 enum : int { kCG = 0, kGMRES = 1 };
@@ -83,6 +101,16 @@ struct Server {
     hipStream_t cg_stream = nullptr;
     tg_batch_t batch = nullptr;
     std::map<int64_t, Request*> by_batch_id;
+    // set when a batched step failed: the engine is not used again, CG requests are refused
+    std::atomic<bool> cg_failed{false};
+
+    // batched GMRES (gmres_batched): engine, stream and the requests in it, owned by the
+    // GMRES thread
+    bool gmres_batched = false;
+    bool gmres_enabled = false;
+    hipStream_t gm_stream = nullptr;
+    tg_gbatch_t gbatch = nullptr;
+    std::map<int64_t, Request*> by_gbatch_id;
 
     std::thread sched;
     std::vector<std::thread> workers;
@@ -182,6 +210,11 @@ struct Server {
     }
 
     void start_cg(Request* r) {
+        if (cg_failed) {
+            r->status = fail(TG_ERR_HIP, "tg_server: the batched CG engine failed earlier; CG requests are refused");
+            complete(r);
+            return;
+        }
         const bool wd = r->diag != nullptr;
         int rc = upload(r, cg_stream, wd);
         if (rc == TG_OK) {
@@ -244,13 +277,20 @@ struct Server {
             for (Request* r : add) start_cg(r);
             if (by_batch_id.empty()) continue;
             if (tg_batch_step(batch, cg_iters) != TG_OK) {
-                // the engine failed: finish every CG request with that error
+                // the engine failed: release every CG system and its device memory, finish
+                // the requests with that error, and refuse CG work from now on
                 const int rc = TG_ERR_HIP;
+                cg_failed = true;
                 for (auto& kv : by_batch_id) {
-                    kv.second->status = rc;
-                    complete(kv.second);
+                    Request* r = kv.second;
+                    (void)tg_batch_release(batch, kv.first);
+                    if (r->dmem) (void)hipFreeAsync(r->dmem, cg_stream);
+                    r->dmem = nullptr;
+                    r->status = rc;
+                    complete(r);
                 }
                 by_batch_id.clear();
+                (void)hipStreamSynchronize(cg_stream);
                 continue;
             }
             for (;;) {
@@ -277,6 +317,86 @@ struct Server {
             solve_gmres(r, s);
         }
         if (s) (void)hipStreamDestroy(s);
+    }
+
+    // The batched GMRES thread: admitted GMRES requests join the engine at the next
+    // restart cycle; one cycle of all of them per step.
+    void gmres_batch_loop() {
+        (void)hipSetDevice(device);
+        std::vector<Request*> add;
+        std::vector<int64_t> done(256);
+        for (;;) {
+            add.clear();
+            {
+                std::unique_lock<std::mutex> lk(m);
+                if (by_gbatch_id.empty())
+                    cv_work.wait(lk, [&] { return sched_done || !gmres_ready.empty(); });
+                if (gmres_ready.empty() && by_gbatch_id.empty()) break;  // sched_done, nothing left
+                while (!gmres_ready.empty()) {
+                    add.push_back(gmres_ready.front());
+                    gmres_ready.pop_front();
+                }
+            }
+            for (Request* r : add) start_gmres_batched(r);
+            if (by_gbatch_id.empty()) continue;
+            if (tg_gbatch_step(gbatch, 1) != TG_OK) {
+                // the engine failed: finish every GMRES request in it with that error
+                for (auto& kv : by_gbatch_id) {
+                    Request* r = kv.second;
+                    if (r->dmem) (void)hipFreeAsync(r->dmem, gm_stream);
+                    r->dmem = nullptr;
+                    r->status = TG_ERR_HIP;
+                    complete(r);
+                }
+                by_gbatch_id.clear();
+                (void)hipStreamSynchronize(gm_stream);
+                continue;
+            }
+            for (;;) {
+                int32_t c = 0;
+                if (tg_gbatch_finished(gbatch, done.data(), (int32_t)done.size(), &c) != TG_OK || c == 0) break;
+                for (int32_t i = 0; i < c; ++i) finish_gmres_batched(done[i]);
+            }
+        }
+    }
+
+    void start_gmres_batched(Request* r) {
+        int rc = upload(r, gm_stream, false);
+        if (rc == TG_OK) {
+            DevPtrs d = ptrs(r, false);
+            rc = tg_gbatch_add(gbatch, r->n, r->nnz, d.rp, d.ci, d.val, nullptr, r->k, d.B, d.X, r->tol, r->restart,
+                               r->max_restarts, &r->batch_id);
+        }
+        if (rc != TG_OK) {
+            r->status = rc;
+            if (r->dmem) (void)hipFreeAsync(r->dmem, gm_stream);
+            r->dmem = nullptr;
+            complete(r);
+            return;
+        }
+        by_gbatch_id[r->batch_id] = r;
+    }
+
+    void finish_gmres_batched(int64_t bid) {
+        auto it = by_gbatch_id.find(bid);
+        if (it == by_gbatch_id.end()) return;
+        Request* r = it->second;
+        by_gbatch_id.erase(it);
+        int32_t fin = 0;
+        r->rel.assign(r->k, 0.0);
+        int rc = tg_gbatch_result(gbatch, bid, &fin, &r->succeeded, &r->iterations, &r->restarts, r->rel.data());
+        if (rc == TG_OK) {
+            DevPtrs d = ptrs(r, false);
+            rc = hipMemcpyAsync(r->X, d.X, (size_t)r->n * r->k * 8, hipMemcpyDeviceToHost, gm_stream) == hipSuccess
+                     ? TG_OK
+                     : fail(TG_ERR_HIP, "tg_server: download failed");
+        }
+        if (rc == TG_OK && hipStreamSynchronize(gm_stream) != hipSuccess) rc = fail(TG_ERR_HIP, "tg_server: sync failed");
+        (void)tg_gbatch_release(gbatch, bid);
+        (void)hipFreeAsync(r->dmem, gm_stream);
+        r->dmem = nullptr;
+        r->status = rc;
+        complete(r);
     }
 
     void solve_gmres(Request* r, hipStream_t s) {
@@ -316,6 +436,7 @@ extern "C" size_t tg_server_request_bytes(int32_t kind, int32_t n, int64_t nnz, 
     if (n < 1 || nrhs < 1 || nnz < 0) return 0;
     const size_t in = input_bytes(n, nnz, nrhs, kind == kCG && has_diag);
     if (kind == kCG) return in + tg_batch_request_bytes(n, nnz, nrhs, has_diag, mixed);
+    if (g_serve_gmres_batch) return in + tg_gbatch_request_bytes(n, nrhs, restart);
     return in + tg_gmres_workspace_bytes(n, restart, nrhs, TG_F64);
 }
 
@@ -333,9 +454,25 @@ extern "C" int tg_server_create(int64_t budget_bytes, int32_t gmres_workers, int
         (void)hipStreamDestroy(S->cg_stream);
         return rc;
     }
+    S->gmres_enabled = gmres_workers > 0;
+    S->gmres_batched = g_serve_gmres_batch != 0 && gmres_workers > 0;
+    if (S->gmres_batched) {
+        rc = hipStreamCreateWithFlags(&S->gm_stream, hipStreamNonBlocking) == hipSuccess
+                 ? tg_gbatch_create(&S->gbatch, S->gm_stream)
+                 : fail(TG_ERR_HIP, "tg_server_create: stream creation failed");
+        if (rc != TG_OK) {
+            (void)tg_batch_destroy(S->batch);
+            if (S->gm_stream) (void)hipStreamDestroy(S->gm_stream);
+            (void)hipStreamDestroy(S->cg_stream);
+            return rc;
+        }
+    }
     Server* raw = S.get();
     S->sched = std::thread([raw] { raw->sched_loop(); });
-    for (int i = 0; i < gmres_workers; ++i) S->workers.emplace_back([raw] { raw->gmres_loop(); });
+    if (S->gmres_batched)
+        S->workers.emplace_back([raw] { raw->gmres_batch_loop(); });
+    else
+        for (int i = 0; i < gmres_workers; ++i) S->workers.emplace_back([raw] { raw->gmres_loop(); });
     *out = new tg_server_s{S.release()};
     return TG_OK;
 }
@@ -355,6 +492,11 @@ extern "C" int tg_server_destroy(tg_server_t h) {
     (void)tg_batch_destroy(S->batch);
     (void)hipStreamSynchronize(S->cg_stream);
     (void)hipStreamDestroy(S->cg_stream);
+    if (S->gbatch) (void)tg_gbatch_destroy(S->gbatch);
+    if (S->gm_stream) {
+        (void)hipStreamSynchronize(S->gm_stream);
+        (void)hipStreamDestroy(S->gm_stream);
+    }
     delete S;
     delete h;
     return TG_OK;
@@ -395,6 +537,8 @@ extern "C" int tg_server_submit_cg(tg_server_t h, int32_t n, int64_t nnz, const 
     r->max_iter = max_iter;
     r->mixed = mixed == 2 ? 2 : mixed ? 1 : 0;  // the batched engine's modes (tg_batch_add_cg)
     r->bytes = tg_server_request_bytes(kCG, n, nnz, nrhs, inv_diag != nullptr, r->mixed, 0);
+    if (h->s->cg_failed)
+        return fail(TG_ERR_HIP, "tg_server_submit_cg: the batched CG engine failed earlier; CG requests are refused");
     return submit(h, std::move(r), id_out);
 }
 
@@ -403,7 +547,7 @@ extern "C" int tg_server_submit_gmres(tg_server_t h, int32_t n, int64_t nnz, con
                                       int32_t restart, int32_t max_restarts, int64_t* id_out) {
     if (!h || !id_out || n < 1 || nnz < 0 || nrhs < 1 || restart < 1 || !rp || !ci || !val || !B || !X)
         return fail(TG_ERR_INVALID, "tg_server_submit_gmres: bad argument");
-    if (h->s->workers.empty()) return fail(TG_ERR_INVALID, "tg_server_submit_gmres: server has no GMRES workers");
+    if (!h->s->gmres_enabled) return fail(TG_ERR_INVALID, "tg_server_submit_gmres: server has no GMRES workers");
     auto r = std::make_unique<Request>();
     r->kind = kGMRES;
     r->n = n;
@@ -417,7 +561,8 @@ extern "C" int tg_server_submit_gmres(tg_server_t h, int32_t n, int64_t nnz, con
     r->tol = tol;
     r->restart = restart;
     r->max_restarts = max_restarts;
-    r->bytes = tg_server_request_bytes(kGMRES, n, nnz, nrhs, 0, 0, restart);
+    r->bytes = h->s->gmres_batched ? input_bytes(n, nnz, nrhs, false) + tg_gbatch_request_bytes(n, nrhs, restart)
+                                   : input_bytes(n, nnz, nrhs, false) + tg_gmres_workspace_bytes(n, restart, nrhs, TG_F64);
     return submit(h, std::move(r), id_out);
 }
 

@@ -26,6 +26,8 @@
 #include <mutex>
 #include <string>
 
+#include <rocprim/device/device_scan.hpp>
+
 #include "common.hpp"
 #include "krylov.hpp"
 
@@ -415,6 +417,9 @@ static int g_spmm_rows = 0;  // nrhs >= 2 kernel: 0 = by mean row length, 1 = ro
 // planned SpMM: 0 = by X size (slice_count, the default), 1 = never sliced,
 // 2/4/8 = that many slices
 static int g_spmm_slices = 0;
+// planned SpMM with values registered by tg_csr_plan_set_values: 1 = the re-laid
+// layout where it pays (default), 0 = never
+static int g_spmm_split = 1;
 
 static int spmv_team_width(int32_t m, int64_t nnz) {
     if (g_spmv_team) return g_spmv_team;
@@ -649,6 +654,42 @@ __global__ __launch_bounds__(kBlock) void slice_plan_kernel(int m, int n, const 
     }
 }
 
+// Run lengths of the re-laid layout: len[s * m + i] = nonzeros of row i in column
+// slice s of the 8-way cut (len[8m] = 0, so the exclusive scan ends with nnz).
+template <typename RP>
+__global__ __launch_bounds__(kBlock) void split_len_kernel(int m, const RP* __restrict__ rp,
+                                                           const int32_t* __restrict__ cuts, int32_t* __restrict__ len) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < m; i += gridDim.x * kBlock) {
+        const int32_t total = (int32_t)(rp[i + 1] - rp[i]);
+        int32_t prev = 0;
+        for (int s = 0; s < kXcds; ++s) {
+            const int32_t cut = s == kXcds - 1 ? total : cuts[(int64_t)s * m + i];
+            len[(int64_t)s * m + i] = cut - prev;
+            prev = cut;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) len[(int64_t)kXcds * m] = 0;
+}
+
+// dst[srp[s*m + i] + j] = src[rp[i] + (start of slice s in row i) + j]: ci or values
+// in the slice-major order of the re-laid layout (a row per thread; built once).
+template <typename RP, typename V>
+__global__ __launch_bounds__(kBlock) void split_scatter_kernel(int m, const RP* __restrict__ rp,
+                                                               const int32_t* __restrict__ cuts,
+                                                               const int32_t* __restrict__ srp,
+                                                               const V* __restrict__ src, V* __restrict__ dst) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < m; i += gridDim.x * kBlock) {
+        const RP base = rp[i];
+        int32_t from = 0;
+        for (int s = 0; s < kXcds; ++s) {
+            const int32_t to = s == kXcds - 1 ? (int32_t)(rp[i + 1] - base) : cuts[(int64_t)s * m + i];
+            const int32_t d0 = srp[(int64_t)s * m + i];
+            for (int32_t j = from; j < to; ++j) dst[d0 + (j - from)] = src[base + j];
+            from = to;
+        }
+    }
+}
+
 // Exclusive scan of v[0..n) in place into v[0..n] (v[n] = total), n <= 4 * kBlock,
 // by one workgroup: 4 entries per thread, a wave scan of the thread totals, then
 // the wave totals.  All threads must call it.
@@ -690,9 +731,14 @@ __device__ __forceinline__ void block_exclusive_scan(int* v, int n) {
 // so loads of a run coalesce), finds its row by binary search in LDS, gathers the
 // X row and leaves the K products in LDS; teams of TEAM lanes then add each row's
 // products (rows spanning chunks accumulate in LDS).
-template <typename T, typename RP, int ELEMS, int TEAM, int K>
+// SPLIT: the re-laid layout of the plan -- every slice's runs stored as a CSR of
+// their own, slice after slice (srp: the 8m + 1 row pointers of all slices, ci and a
+// the re-laid column indices and values), so a workgroup's runs are one contiguous
+// range and no XCD touches another slice's cache lines of A.  S is 8 then.
+template <typename T, typename RP, int ELEMS, int TEAM, int K, bool SPLIT>
 __global__ __launch_bounds__(kBlock) void spmm_slice_stream_kernel(int m, int S, const RP* __restrict__ rp,
                                                                    const int32_t* __restrict__ cuts,
+                                                                   const int32_t* __restrict__ srp,
                                                                    const int32_t* __restrict__ ci,
                                                                    const T* __restrict__ a, const T* __restrict__ X,
                                                                    int64_t ldX, T* __restrict__ part,
@@ -713,9 +759,15 @@ __global__ __launch_bounds__(kBlock) void spmm_slice_stream_kernel(int m, int S,
         const int nr = min(RMAX, r1 - rs);
         for (int i = threadIdx.x; i < nr; i += kBlock) {
             const int row = rs + i;
-            const RP base = rp[row];
-            const RP lb = s == 0 ? base : base + cuts[(int64_t)(s * step - 1) * m + row];
-            const RP le = s == S - 1 ? rp[row + 1] : base + cuts[(int64_t)((s + 1) * step - 1) * m + row];
+            RP lb, le;
+            if constexpr (SPLIT) {
+                lb = srp[(int64_t)s * m + row];
+                le = srp[(int64_t)s * m + row + 1];
+            } else {
+                const RP base = rp[row];
+                lb = s == 0 ? base : base + cuts[(int64_t)(s * step - 1) * m + row];
+                le = s == S - 1 ? rp[row + 1] : base + cuts[(int64_t)((s + 1) * step - 1) * m + row];
+            }
             slo[i] = lb;
             soff[i] = (int)(le - lb);
         }
@@ -810,6 +862,13 @@ __global__ __launch_bounds__(kBlock) void slice_reduce_kernel(int m, int S, cons
 // most 4 MB each where possible.  Measured on a random 1e6-row CSR with 32
 // nonzeros per row: 2 slices for the 8 MB FP64 SpMV, 4 for the 16 MB FP32 and
 // 32 MB FP64 nrhs = 4 blocks were the fastest counts; 8 slices were slower.
+// The re-laid layout (8 slices, one per XCD): X blocks of more than 2 MB and at
+// most 64 MB, i.e. slices of 0.25-8 MB.
+static bool split_pays(int64_t xbytes) {
+    constexpr int64_t kMB = 1ll << 20;
+    return xbytes > 2 * kMB && xbytes <= 64 * kMB;
+}
+
 static int slice_count(int64_t xbytes) {
     constexpr int64_t kMB = 1ll << 20;
     constexpr int64_t kSliceBytes = 4 * kMB;
@@ -820,15 +879,16 @@ static int slice_count(int64_t xbytes) {
     return S;
 }
 
-template <typename T, typename RP, int K>
-static void launch_slice_stream(int S, int team, int grid, int m, const RP* rp, const int32_t* cuts, const int32_t* ci,
-                                const T* a, const T* X, int64_t ldX, T* part, int rpb, hipStream_t st) {
+template <typename T, typename RP, int K, bool SPLIT>
+static void launch_slice_stream(int S, int team, int grid, int m, const RP* rp, const int32_t* cuts,
+                                const int32_t* srp, const int32_t* ci, const T* a, const T* X, int64_t ldX, T* part,
+                                int rpb, hipStream_t st) {
     // ELEMS keeps the product stage at 2048 entries (16 KB in FP64)
     constexpr int E = K == 1 ? 8 : K == 2 ? 4 : 2;
-#define TG_SLS(TM)                                                                                            \
-    case TM:                                                                                                  \
-        hipLaunchKernelGGL((spmm_slice_stream_kernel<T, RP, E, TM, K>), dim3(grid), dim3(kBlock), 0, st, m, S, rp, \
-                           cuts, ci, a, X, ldX, part, rpb);                                                   \
+#define TG_SLS(TM)                                                                                                \
+    case TM:                                                                                                      \
+        hipLaunchKernelGGL((spmm_slice_stream_kernel<T, RP, E, TM, K, SPLIT>), dim3(grid), dim3(kBlock), 0, st, m, \
+                           S, rp, cuts, srp, ci, a, X, ldX, part, rpb);                                           \
         break;
     switch (team) {
         TG_SLS(2)
@@ -841,9 +901,9 @@ static void launch_slice_stream(int S, int team, int grid, int m, const RP* rp, 
 }
 
 template <typename T, typename RP>
-static int spmm_sliced_typed(int S, int32_t m, int64_t nnz, const RP* rp, const int32_t* cuts, const int32_t* ci,
-                             const T* a, double alpha_d, const T* X, int64_t ldX, double beta_d, T* Y, int64_t ldY,
-                             int32_t K, T* part, hipStream_t st) {
+static int spmm_sliced_typed(int S, int32_t m, int64_t nnz, const RP* rp, const int32_t* cuts, const int32_t* srp,
+                             const int32_t* ci, const T* a, double alpha_d, const T* X, int64_t ldX, double beta_d,
+                             T* Y, int64_t ldY, int32_t K, T* part, hipStream_t st) {
     const double avg = m > 0 ? (double)nnz / m / S : 0.0;  // mean run length per row and slice
     const int team = std::min(16, std::max(2, pow2ceil((int)std::ceil(avg))));
     const int cap = kBlock * (K == 1 ? 8 : K == 2 ? 4 : 2);
@@ -852,12 +912,18 @@ static int spmm_sliced_typed(int S, int32_t m, int64_t nnz, const RP* rp, const 
     const int row_blocks = (m + rpb - 1) / rpb;
     const int per = kXcds / S;
     const int grid = kXcds * ((row_blocks + per - 1) / per);
+#define TG_SLK(KK)                                                                                              \
+    if (srp)                                                                                                    \
+        launch_slice_stream<T, RP, KK, true>(S, team, grid, m, rp, cuts, srp, ci, a, X, ldX, part, rpb, st);    \
+    else                                                                                                        \
+        launch_slice_stream<T, RP, KK, false>(S, team, grid, m, rp, cuts, srp, ci, a, X, ldX, part, rpb, st);
     switch (K) {
-        case 1: launch_slice_stream<T, RP, 1>(S, team, grid, m, rp, cuts, ci, a, X, ldX, part, rpb, st); break;
-        case 2: launch_slice_stream<T, RP, 2>(S, team, grid, m, rp, cuts, ci, a, X, ldX, part, rpb, st); break;
-        case 3: launch_slice_stream<T, RP, 3>(S, team, grid, m, rp, cuts, ci, a, X, ldX, part, rpb, st); break;
-        default: launch_slice_stream<T, RP, 4>(S, team, grid, m, rp, cuts, ci, a, X, ldX, part, rpb, st); break;
+        case 1: TG_SLK(1) break;
+        case 2: TG_SLK(2) break;
+        case 3: TG_SLK(3) break;
+        default: TG_SLK(4) break;
     }
+#undef TG_SLK
     const int alpha_one = (alpha_d == 1.0);
     const int beta_mode = (beta_d == 0.0) ? kBetaZero : (beta_d == 1.0) ? kBetaOne : kBetaGeneric;
     const int rgrid = (int)std::min<int64_t>((m + kBlock - 1) / kBlock, 65536);
@@ -908,6 +974,11 @@ extern "C" int tg_tune_set(const char* key, int32_t value) {
         g_spmm_rows = value;
         return TG_OK;
     }
+    if (k == "spmm_split") {
+        if (value != 0 && value != 1) return fail(TG_ERR_INVALID, "tg_tune_set: spmm_split must be 0 or 1");
+        g_spmm_split = value;
+        return TG_OK;
+    }
     if (k == "spmm_slices") {
         if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8)
             return fail(TG_ERR_INVALID, "tg_tune_set: spmm_slices must be 0 (auto), 1 (off), 2, 4 or 8");
@@ -923,6 +994,8 @@ extern "C" int tg_tune_set(const char* key, int32_t value) {
     if (known) return rc;
     rc = gmres_tune_set(key, value, &known);
     if (known) return rc;
+    rc = serve_tune_set(key, value, &known);
+    if (known) return rc;
     return fail(TG_ERR_INVALID, "tg_tune_set: unknown key '%s'", key);
 }
 
@@ -933,10 +1006,12 @@ extern "C" int32_t tg_tune_get(const char* key) {
     if (k == "spmv_team") return g_spmv_team;
     if (k == "spmm_group") return g_spmm_group;
     if (k == "spmm_slices") return g_spmm_slices;
+    if (k == "spmm_split") return g_spmm_split;
     if (k == "spmm_rows") return g_spmm_rows;
     if (k == "cg_fused") return cg_tune_get(key);
     if (k == "bsr_vector") return bsr_tune_get(key);
     if (k == "gmres_graph") return gmres_tune_get(key);
+    if (k == "serve_gmres_batch") return serve_tune_get(key);
     return batch_tune_get(key);
 }
 
@@ -1101,6 +1176,13 @@ struct tg_csr_plan_s {
     int32_t* cuts = nullptr;  // kCuts x m row offsets of the 8-way column cut
     int sorted = 0;
     int64_t band = -1;        // max |column - row|
+    // Re-laid layout (tg_csr_plan_set_values): the 8 slices' runs as CSRs of their own,
+    // slice after slice -- row pointers of all slices (8m + 1), column indices, and the
+    // values of one registered pointer per value type (0 FP64, 1 FP32).
+    int32_t* srp = nullptr;
+    int32_t* sci = nullptr;
+    void* sval[2] = {nullptr, nullptr};
+    const void* sval_src[2] = {nullptr, nullptr};
     void* part = nullptr;     // partial blocks of the sliced path (grown on demand)
     size_t part_bytes = 0;
     // The partial blocks are shared by every sliced call on the plan: a call on another
@@ -1164,6 +1246,10 @@ extern "C" int tg_csr_plan_create(int32_t m, int32_t n, int64_t nnz, const void*
 
 extern "C" int tg_csr_plan_destroy(tg_csr_plan_t plan) {
     if (!plan) return TG_OK;
+    if (plan->srp) (void)hipFree(plan->srp);
+    if (plan->sci) (void)hipFree(plan->sci);
+    for (void* v : plan->sval)
+        if (v) (void)hipFree(v);
     if (plan->cuts) (void)hipFree(plan->cuts);
     if (plan->part) (void)hipFree(plan->part);
     if (plan->used) (void)hipEventDestroy(plan->used);
@@ -1171,12 +1257,99 @@ extern "C" int tg_csr_plan_destroy(tg_csr_plan_t plan) {
     return TG_OK;
 }
 
-extern "C" int32_t tg_csr_plan_slices(tg_csr_plan_t plan, tg_dtype val_type, int32_t nrhs) {
+extern "C" int tg_csr_plan_set_values(tg_csr_plan_t plan, const void* a, tg_dtype val_type, void* stream) {
+    if (!plan || (val_type != TG_F64 && val_type != TG_F32))
+        return fail(TG_ERR_INVALID, "tg_csr_plan_set_values: bad argument");
+    const int vi = val_type == TG_F64 ? 0 : 1;
+    std::lock_guard<std::mutex> lk(plan->mu);
+    plan->sval_src[vi] = nullptr;
+    if (!a || !plan->sorted || plan->m == 0 || plan->n == 0 || plan->nnz == 0) return TG_OK;
+    if (plan->nnz >= (1ll << 31)) return TG_OK;  // int32 offsets of the layout
+    hipStream_t st = as_stream(stream);
+    const int32_t m = plan->m;
+    const int grid = (int)std::min<int64_t>((m + kBlock - 1) / kBlock, 65536);
+    if (!plan->srp) {  // the layout of the pattern, once
+        const size_t cnt = (size_t)kXcds * m + 1;
+        int32_t* len = nullptr;
+        size_t tmp_bytes = 0;
+        TG_HIP(rocprim::exclusive_scan(nullptr, tmp_bytes, (const int32_t*)nullptr, (int32_t*)nullptr, (int32_t)0, cnt,
+                                       rocprim::plus<int32_t>(), st));
+        void* tmp = nullptr;
+        TG_HIP(hipMallocAsync((void**)&len, cnt * sizeof(int32_t), st));
+        TG_HIP(hipMallocAsync(&tmp, std::max<size_t>(tmp_bytes, 8), st));
+        TG_HIP(hipMallocAsync((void**)&plan->srp, cnt * sizeof(int32_t), st));
+        TG_HIP(hipMallocAsync((void**)&plan->sci, (size_t)plan->nnz * sizeof(int32_t), st));
+        if (plan->rp_type == TG_I32)
+            hipLaunchKernelGGL((split_len_kernel<int32_t>), dim3(grid), dim3(kBlock), 0, st, m,
+                               (const int32_t*)plan->rp, (const int32_t*)plan->cuts, len);
+        else
+            hipLaunchKernelGGL((split_len_kernel<int64_t>), dim3(grid), dim3(kBlock), 0, st, m,
+                               (const int64_t*)plan->rp, (const int32_t*)plan->cuts, len);
+        TG_HIP(rocprim::exclusive_scan(tmp, tmp_bytes, (const int32_t*)len, plan->srp, (int32_t)0, cnt,
+                                       rocprim::plus<int32_t>(), st));
+        if (plan->rp_type == TG_I32)
+            hipLaunchKernelGGL((split_scatter_kernel<int32_t, int32_t>), dim3(grid), dim3(kBlock), 0, st, m,
+                               (const int32_t*)plan->rp, (const int32_t*)plan->cuts, (const int32_t*)plan->srp,
+                               plan->ci, plan->sci);
+        else
+            hipLaunchKernelGGL((split_scatter_kernel<int64_t, int32_t>), dim3(grid), dim3(kBlock), 0, st, m,
+                               (const int64_t*)plan->rp, (const int32_t*)plan->cuts, (const int32_t*)plan->srp,
+                               plan->ci, plan->sci);
+        TG_LAUNCH_CHECK();
+        TG_HIP(hipFreeAsync(len, st));
+        TG_HIP(hipFreeAsync(tmp, st));
+    }
+    const size_t vs = val_type == TG_F64 ? 8 : 4;
+    if (!plan->sval[vi]) TG_HIP(hipMallocAsync(&plan->sval[vi], (size_t)plan->nnz * vs, st));
+#define TG_SPLIT_VALS(RPT, VT)                                                                                   \
+    hipLaunchKernelGGL((split_scatter_kernel<RPT, VT>), dim3(grid), dim3(kBlock), 0, st, m, (const RPT*)plan->rp, \
+                       (const int32_t*)plan->cuts, (const int32_t*)plan->srp, (const VT*)a, (VT*)plan->sval[vi])
+    if (plan->rp_type == TG_I32) {
+        if (vi == 0)
+            TG_SPLIT_VALS(int32_t, double);
+        else
+            TG_SPLIT_VALS(int32_t, float);
+    } else {
+        if (vi == 0)
+            TG_SPLIT_VALS(int64_t, double);
+        else
+            TG_SPLIT_VALS(int64_t, float);
+    }
+#undef TG_SPLIT_VALS
+    TG_LAUNCH_CHECK();
+    TG_HIP(hipStreamSynchronize(st));
+    plan->sval_src[vi] = a;
+    return TG_OK;
+}
+
+// Slices of the planned product and whether the re-laid layout serves it (the values
+// `a` are the ones registered for the value type and X is a few L2s large).
+static int plan_slices(tg_csr_plan_t plan, tg_dtype val_type, int32_t nrhs, const void* a, bool* split) {
+    *split = false;
     if (!plan || !plan->sorted || nrhs < 1 || nrhs > 4 || plan->m == 0 || plan->n == 0) return 0;
     if (g_spmm_slices == 1) return 0;
-    if (g_spmm_slices > 1) return g_spmm_slices;
     const int64_t vs = val_type == TG_F64 ? 8 : 4;
-    return slice_count((int64_t)plan->n * nrhs * vs);
+    const int64_t xbytes = (int64_t)plan->n * nrhs * vs;
+    const int vi = val_type == TG_F64 ? 0 : 1;
+    if (g_spmm_split && a && plan->srp && plan->sval_src[vi] == a && (g_spmm_slices == 0 || g_spmm_slices == kXcds) &&
+        (g_spmm_slices == kXcds || split_pays(xbytes))) {
+        *split = true;
+        return kXcds;
+    }
+    if (g_spmm_slices > 1) return g_spmm_slices;
+    return slice_count(xbytes);
+}
+
+extern "C" int32_t tg_csr_plan_slices(tg_csr_plan_t plan, tg_dtype val_type, int32_t nrhs) {
+    bool split = false;
+    return plan_slices(plan, val_type, nrhs, nullptr, &split);
+}
+
+extern "C" int32_t tg_csr_plan_uses_split(tg_csr_plan_t plan, const void* a, tg_dtype val_type, int32_t nrhs) {
+    bool split = false;
+    if (!plan || (val_type != TG_F64 && val_type != TG_F32)) return 0;
+    (void)plan_slices(plan, val_type, nrhs, a, &split);
+    return split ? 1 : 0;
 }
 
 extern "C" int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, double alpha, const void* X,
@@ -1196,7 +1369,8 @@ extern "C" int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_
     A.val_type = val_type;
     A.band = plan->band;
     hipStream_t st = as_stream(stream);
-    const int S = (val_type == TG_F64 || val_type == TG_F32) ? tg_csr_plan_slices(plan, val_type, nrhs) : 0;
+    bool split = false;
+    const int S = (val_type == TG_F64 || val_type == TG_F32) ? plan_slices(plan, val_type, nrhs, a, &split) : 0;
     // the plain path covers every special case (alpha = 0, aliasing, bad arguments)
     if (S == 0 || alpha == 0.0 || X == Y || !X || !Y || ldX < nrhs || ldY < nrhs)
         return spmm_dispatch(A, alpha, X, ldX, beta, Y, ldY, nrhs, nullptr, 0, nullptr, st);
@@ -1224,25 +1398,30 @@ extern "C" int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_
         plan->part_bytes = need;
     }
     const int32_t m = plan->m;
+    const int vi = val_type == TG_F64 ? 0 : 1;
+    // the re-laid layout reads its own column indices and values (a binary matrix has none)
+    const int32_t* srp = split ? plan->srp : nullptr;
+    const int32_t* ci_use = split ? plan->sci : plan->ci;
+    const void* a_use = split ? plan->sval[vi] : a;
     int rc;
     if (val_type == TG_F64) {
         if (plan->rp_type == TG_I32)
-            rc = spmm_sliced_typed<double, int32_t>(S, m, plan->nnz, (const int32_t*)plan->rp, plan->cuts, plan->ci,
-                                                    (const double*)a, alpha, (const double*)X, ldX, beta, (double*)Y,
-                                                    ldY, nrhs, (double*)plan->part, st);
+            rc = spmm_sliced_typed<double, int32_t>(S, m, plan->nnz, (const int32_t*)plan->rp, plan->cuts, srp, ci_use,
+                                                    (const double*)a_use, alpha, (const double*)X, ldX, beta,
+                                                    (double*)Y, ldY, nrhs, (double*)plan->part, st);
         else
-            rc = spmm_sliced_typed<double, int64_t>(S, m, plan->nnz, (const int64_t*)plan->rp, plan->cuts, plan->ci,
-                                                    (const double*)a, alpha, (const double*)X, ldX, beta, (double*)Y,
-                                                    ldY, nrhs, (double*)plan->part, st);
+            rc = spmm_sliced_typed<double, int64_t>(S, m, plan->nnz, (const int64_t*)plan->rp, plan->cuts, srp, ci_use,
+                                                    (const double*)a_use, alpha, (const double*)X, ldX, beta,
+                                                    (double*)Y, ldY, nrhs, (double*)plan->part, st);
     } else {
         if (plan->rp_type == TG_I32)
-            rc = spmm_sliced_typed<float, int32_t>(S, m, plan->nnz, (const int32_t*)plan->rp, plan->cuts, plan->ci,
-                                                   (const float*)a, alpha, (const float*)X, ldX, beta, (float*)Y, ldY,
-                                                   nrhs, (float*)plan->part, st);
+            rc = spmm_sliced_typed<float, int32_t>(S, m, plan->nnz, (const int32_t*)plan->rp, plan->cuts, srp, ci_use,
+                                                   (const float*)a_use, alpha, (const float*)X, ldX, beta, (float*)Y,
+                                                   ldY, nrhs, (float*)plan->part, st);
         else
-            rc = spmm_sliced_typed<float, int64_t>(S, m, plan->nnz, (const int64_t*)plan->rp, plan->cuts, plan->ci,
-                                                   (const float*)a, alpha, (const float*)X, ldX, beta, (float*)Y, ldY,
-                                                   nrhs, (float*)plan->part, st);
+            rc = spmm_sliced_typed<float, int64_t>(S, m, plan->nnz, (const int64_t*)plan->rp, plan->cuts, srp, ci_use,
+                                                   (const float*)a_use, alpha, (const float*)X, ldX, beta, (float*)Y,
+                                                   ldY, nrhs, (float*)plan->part, st);
     }
     if (track) {
         if (!plan->used) TG_HIP(hipEventCreateWithFlags(&plan->used, hipEventDisableTiming));

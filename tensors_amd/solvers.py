@@ -209,3 +209,25 @@ def cg_solve_mixed(A: MatrixCSR, B, X, tol=1e-10, max_outer=100, max_inner=20000
               ws.data_ptr(), nb, C.byref(ok), C.byref(outer), C.byref(inner), rr.ctypes.data,
               stream_handle(stream))
     return bool(ok.value), outer.value, inner.value, rr
+
+
+def gmres_solve_mixed(A: MatrixCSR, B, X, tol=1e-8, restart=30, max_restarts=100, a=1.0, b=0.0, P=None, side=0,
+                      use_initial_guess=False, a32=None, stream=None):
+    """tg_gmres_solve_mixed: GMRES(restart) cycles in FP32 with the residual, solution and
+    stop test in FP64 at every restart.  Returns (succeeded, restarts, fp32_steps, rel_res).
+    P: None or a CUDA float64 diagonal (z = d * r); a32: optional FP32 values (CUDA)."""
+    n, k = X.shape[0], (X.shape[1] if X.dim() == 2 else 1)
+    rp, ci, vals = A.device_arrays(torch.float64)
+    if a32 is None:
+        _, _, a32 = A.device_arrays(torch.float32)
+    nb = capi.lib().tg_gmres_mixed_workspace_bytes(n, A.NonzeroCount(), restart, k, 0 if P is None else 1, 1)
+    ws = torch.empty(nb, dtype=torch.uint8, device=X.device)
+    ok, rs, inner = C.c_int32(0), C.c_int32(0), C.c_int32(0)
+    rr = np.zeros(k)
+    capi.call("tg_gmres_solve_mixed", n, A.NonzeroCount(), rp.data_ptr(),
+              capi.I64 if rp.dtype == torch.int64 else capi.I32, ci.data_ptr(), vals.data_ptr(), a32.data_ptr(),
+              0 if P is None else 1, P.data_ptr() if P is not None else None, int(side), k, int(restart),
+              int(max_restarts), float(a), B.data_ptr(), B.stride(0) if B.dim() == 2 else 1, float(b),
+              X.data_ptr(), X.stride(0) if X.dim() == 2 else 1, float(tol), int(use_initial_guess),
+              ws.data_ptr(), nb, C.byref(ok), C.byref(rs), C.byref(inner), rr.ctypes.data, stream_handle(stream))
+    return bool(ok.value), rs.value, inner.value, rr

@@ -54,8 +54,29 @@ class SlicePlan:
         capi.call("tg_csr_plan_create", m, n, nnz, rp.data_ptr(), capi.I64 if rp.dtype == torch.int64 else capi.I32,
                   ci.data_ptr() if ci.numel() else None, C.byref(self.h), stream_handle(stream))
 
-    def slices(self, dtype, nrhs):
+    def slices(self, dtype, nrhs, a=None):
+        """Slices a call with values `a` uses (8 on the re-laid layout)."""
+        if a is not None and self.uses_split(a, nrhs):
+            return 8
         return capi.lib().tg_csr_plan_slices(self.h, _TORCH_TO_TG[dtype], nrhs)
+
+    def set_values(self, a, stream=None):
+        """tg_csr_plan_set_values: keep a re-laid copy of the values `a` (a CUDA tensor that
+        must not change afterwards without another set_values); None drops it for a's type."""
+        self._vals = getattr(self, "_vals", {})
+        self._vals[a.dtype] = a
+        capi.call("tg_csr_plan_set_values", self.h, a.data_ptr(), _TORCH_TO_TG[a.dtype], stream_handle(stream))
+
+    def ensure_split(self, a, nrhs, n):
+        """Registers the matrix's own values `a` on the first call whose shape takes the
+        re-laid layout (nrhs <= 4, X of 2-64 MB), as the drop-in MatrixCSR does."""
+        xbytes = n * nrhs * a.element_size()
+        vals = getattr(self, "_vals", {})
+        if nrhs <= 4 and (2 << 20) < xbytes <= (64 << 20) and vals.get(a.dtype) is not a:
+            self.set_values(a)
+
+    def uses_split(self, a, nrhs):
+        return bool(capi.lib().tg_csr_plan_uses_split(self.h, a.data_ptr(), _TORCH_TO_TG[a.dtype], nrhs))
 
     def spmm(self, a, alpha, X, ldX, beta, Y, ldY, nrhs, NRHS=0, stream=None):
         """tg_csr_spmm_plan; returns the status like spmm_raw."""
@@ -214,7 +235,10 @@ class MatrixCSR:
             st = spmm_raw(self.m, self.n, self.NonzeroCount(), rp, ci, vals, alpha, X, ldX, beta, Y, ldY, k,
                           NRHS, jp, blocks, stream)
         else:
-            st = self.Plan().spmm(vals, alpha, X, ldX, beta, Y, ldY, k, NRHS, stream)
+            plan = self.Plan()
+            if ext_values is None and vals is not None:
+                plan.ensure_split(vals, k, self.n)
+            st = plan.spmm(vals, alpha, X, ldX, beta, Y, ldY, k, NRHS, stream)
         if st == capi.ERR_ALIAS:
             _eprint(self._name("Dot") + ": Input and output pointer coincide. This is not safe. Aborting.")
         elif st == capi.ERR_NRHS:

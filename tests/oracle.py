@@ -30,6 +30,7 @@ _SIG = {
                          vp]),
     "orc_gmres": (C.c_int, [i32, vp, vp, vp, i32, vp, i32, i32, i32, i32, dbl, vp, i64, dbl, vp, i64, dbl, i32,
                             i32, vp, vp, vp]),
+    "orc_set_pinning": (None, [i32]),
     "orc_cg_ex": (C.c_int, [i32, vp, vp, vp, i32, vp, i32, i32, i32, dbl, vp, i64, dbl, vp, i64, dbl, i32, vp,
                             vp, vp]),
     "orc_gmres_ex": (C.c_int, [i32, vp, vp, vp, i32, vp, i32, i32, i32, i32, dbl, vp, i64, dbl, vp, i64, dbl, i32,

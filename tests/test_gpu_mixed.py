@@ -7,6 +7,7 @@ import torch
 
 import oracle
 from helpers import rel_err
+from tensors_amd import _capi as capi
 from tensors_amd import generators as gen
 from tensors_amd.solvers import cg_solve_mixed
 from tensors_amd.sparse import MatrixCSR
@@ -82,11 +83,23 @@ def test_replacement_keeps_the_krylov_process(cuda):
     assert ok and it_g < it32 and np.all(rr <= 1e-10)
 
 
+@pytest.fixture
+def fused_knob():
+    """Restores the process-wide cg_fused knob after the test."""
+    before = int(capi.lib().tg_tune_get(b"cg_fused"))
+    yield lambda v: capi.call("tg_tune_set", b"cg_fused", int(v))
+    capi.call("tg_tune_set", b"cg_fused", before)
+
+
 @pytest.mark.parametrize("n,k", [(1, 1), (2, 8), (3, 3), (5, 32)])
 @pytest.mark.parametrize("prec", [False, True])
-def test_mixed_tiny_systems(cuda, n, k, prec):
+@pytest.mark.parametrize("form", [3, 2], ids=["default", "fused_direction"])
+def test_mixed_tiny_systems(cuda, fused_knob, n, k, prec, form):
     """Systems smaller than a handful of CG iterations: the FP32 iteration uses up its Krylov
-    space at once, and the solve must still reach the FP64 true residual."""
+    space at once, and the solve must still reach the FP64 true residual -- in the default
+    iteration and in the fused-direction form, whose restart after a replacement comes from
+    beta = 0 instead of a reset p."""
+    fused_knob(form)
     A = sp.diags([np.full(n - 1, -1.0), np.full(n, 2.3), np.full(n - 1, -1.0)], [-1, 0, 1], format="csr") \
         if n > 1 else sp.csr_matrix(np.array([[2.0]]))
     A.sort_indices()

@@ -45,6 +45,16 @@ def submit(srv, req):
     return srv.submit_gmres(rp, ci, val, B, tol=1e-8, restart=30, max_restarts=100)
 
 
+@pytest.fixture(params=[1, 0], ids=["gmres_batched", "gmres_workers"])
+def gmres_mode(request):
+    """Serving GMRES requests through the batched GMRES engine (1, the default) or on
+    per-request worker streams (0); restores the knob afterwards."""
+    before = int(capi.lib().tg_tune_get(b"serve_gmres_batch"))
+    capi.call("tg_tune_set", b"serve_gmres_batch", request.param)
+    yield request.param
+    capi.call("tg_tune_set", b"serve_gmres_batch", before)
+
+
 def solve_all(reqs, budget=8 << 30, workers=3):
     srv = Server(budget, workers)
     ids = [submit(srv, r) for r in reqs]
@@ -54,7 +64,7 @@ def solve_all(reqs, budget=8 << 30, workers=3):
     return [res[i] for i in ids], stats
 
 
-def test_serving_results_meet_tolerance_and_match_oracle(cuda):
+def test_serving_results_meet_tolerance_and_match_oracle(cuda, gmres_mode):
     reqs = requests()
     res, stats = solve_all(reqs)
     assert stats["bytes_in_use"] == 0 and stats["running"] == 0
@@ -76,7 +86,7 @@ def test_serving_results_meet_tolerance_and_match_oracle(cuda):
                 assert rel_err(X, Xo) < 1e-5
 
 
-def test_serving_alone_equals_mix_and_budget_holds(cuda):
+def test_serving_alone_equals_mix_and_budget_holds(cuda, gmres_mode):
     reqs = requests(seed=5105, count=9)
     mix, _ = solve_all(reqs)
     sizes = [Server.request_bytes(r[0], len(r[1]) - 1, len(r[2]), r[4].shape[1], r[5] is not None, True, 30)
@@ -100,7 +110,7 @@ def test_serving_refuses_request_over_budget(cuda):
     assert r["status"] == capi.ERR_WORKSPACE
 
 
-def test_serving_edge_requests(cuda):
+def test_serving_edge_requests(cuda, gmres_mode):
     """Tiny CG requests (n = 1 and 2, up to 8 right-hand sides), a tiny GMRES request (3 x 3
     grid), and CG / GMRES requests whose right-hand side holds a NaN, in one server: the tiny
     ones reach their tolerance, the NaN ones come back as not succeeded with status 0, and

@@ -169,3 +169,71 @@ def test_one_plan_on_two_streams_and_threads(cuda, knobs):
     for key, ys in outs.items():
         for Y in ys:
             assert torch.equal(Y, alone[key]), key
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("rp64", [False, True])
+def test_relaid_layout_equals_cut_path_bitwise(cuda, knobs, k, dtype, rp64):
+    """The re-laid 8-slice layout (values registered with tg_csr_plan_set_values) runs the
+    same runs in the same order as the 8-slice cut of the original layout: bitwise the same
+    Y for every alpha/beta, empty rows included, int32 and int64 row pointers; other value
+    pointers and the knob turned off take the cut path."""
+    m, n = 2600, 3300
+    rp, ci, val = random_csr_np(m, n, 0.012, seed=40 + k, empty_rows=0.05)
+    rng = np.random.default_rng(k + 9)
+    X = torch.from_numpy(rng.uniform(-1, 1, (n, k)).astype(dtype)).to(cuda)
+    Y0 = torch.from_numpy(rng.uniform(-1, 1, (m, k)).astype(dtype)).to(cuda)
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    rpd = torch.from_numpy(rp.astype(np.int64 if rp64 else np.int32)).to(cuda)
+    cid = torch.from_numpy(ci).to(cuda)
+    vd = torch.from_numpy(val.astype(dtype)).to(cuda)
+    plan = SlicePlan(m, n, len(ci), rpd, cid)
+    autotune.set_knobs({"spmm_slices": 8, "spmm_split": 1})
+    assert not plan.uses_split(vd, k)
+    plan.set_values(vd)
+    assert plan.uses_split(vd, k) and plan.slices(tdt, k, a=vd) == 8
+    other = vd.clone()
+    assert not plan.uses_split(other, k)
+    for alpha, beta in [(1.0, 0.0), (2.0, 1.0), (-1.0, 0.5), (0.0, 2.0)]:
+        Ys = Y0.clone()
+        assert plan.spmm(vd, alpha, X, k, beta, Ys, k, k) == 0
+        Yc = Y0.clone()
+        assert plan.spmm(other, alpha, X, k, beta, Yc, k, k) == 0  # same values, cut path
+        assert torch.equal(Ys, Yc), (alpha, beta)
+    autotune.set_knobs({"spmm_slices": 8, "spmm_split": 0})
+    assert not plan.uses_split(vd, k)
+    Yo = Y0.clone()
+    plan.spmm(vd, 1.0, X, k, 0.0, Yo, k, k)
+    autotune.set_knobs({"spmm_slices": 8, "spmm_split": 1})
+    Ys = Y0.clone()
+    plan.spmm(vd, 1.0, X, k, 0.0, Ys, k, k)
+    assert torch.equal(Ys, Yo)
+
+
+def test_relaid_layout_follows_set_values(cuda, knobs):
+    """After the registered values change, tg_csr_plan_set_values refreshes the copy; a
+    matrix of 3e5 rows with nrhs 1 (X = 2.4 MB in FP64) takes the layout by default."""
+    m = n = 300_000
+    rp, ci, val = random_csr_np(m, n, 20.0 / n, seed=8)
+    rpd, cid, vd = dev_arrays(rp, ci, val, np.float64, cuda)
+    X = torch.rand(n, 1, dtype=torch.float64, device=cuda)
+    plan = SlicePlan(m, n, len(ci), rpd, cid)
+    autotune.set_knobs({"spmm_slices": 0, "spmm_split": 1})
+    plan.set_values(vd)
+    assert plan.uses_split(vd, 1)
+    Y = torch.empty(m, 1, dtype=torch.float64, device=cuda)
+    plan.spmm(vd, 1.0, X, 1, 0.0, Y, 1, 1)
+    Yo = np.zeros((m, 1))
+    oracle.spmm(rp, ci, val, m, n, 1.0, X.cpu().numpy(), 1, 0.0, Yo, 1, 1)
+    assert rel_err(Y.cpu().numpy(), Yo) <= 1e-13
+    vd.mul_(2.0)
+    plan.set_values(vd)
+    plan.spmm(vd, 1.0, X, 1, 0.0, Y, 1, 1)
+    assert rel_err(Y.cpu().numpy(), 2.0 * Yo) <= 1e-13
+    # the drop-in's Python mirror registers its own values on the first qualifying Dot
+    A = MatrixCSR(rp.astype(np.int32), ci, val, m, n)
+    Y2 = torch.empty(m, 1, dtype=torch.float64, device=cuda)
+    A.Dot(1.0, X, 0.0, Y2)
+    assert A.Plan().uses_split(A.device_arrays(torch.float64)[2], 1)
+    assert rel_err(Y2.cpu().numpy(), Yo) <= 1e-13
