@@ -232,6 +232,22 @@ int tg_cg_solve(int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, const 
  * b_squared_norms[nrhs] (either may be NULL).  Synchronises `stream`. */
 int tg_cg_residuals(const void* workspace, size_t workspace_bytes, int32_t nrhs, double* residuals,
                     double* b_squared_norms, void* stream);
+/* tg_cg_solve without any allocation.  No counterpart in the reference: it
+ * serves callers that cannot allocate device memory (serving loops that must not
+ * call hipMalloc between requests, harnesses that forbid allocation).  Same
+ * arguments, kernels and results as tg_cg_solve, minus the workspace, which is a
+ * static device array of TG_CG_NOALLOC_SCRATCH_BYTES that every GPU the library
+ * runs on holds until the process ends.  A system whose
+ * tg_cg_workspace_bytes(n, nrhs, val_type, prec_kind) exceeds it is refused with
+ * TG_ERR_WORKSPACE before any device work.  All calls on one GPU share that
+ * GPU's array: calls that overlap on one GPU (several host threads, or streams
+ * whose work overlaps) race.  Synchronises `stream`. */
+#define TG_CG_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int tg_cg_solve_noalloc(int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, const int32_t* ci, const void* a,
+                        tg_dtype val_type, int32_t prec_kind, const void* prec_diag, int32_t nrhs, int32_t max_iter,
+                        double a_coef, const void* B, int64_t ldB, double b_coef, void* X, int64_t ldX, double tol,
+                        int32_t use_guess, int32_t* succeeded, int32_t* iterations, double* rel_residuals,
+                        void* stream);
 
 /* ------------------------------------------------------------------------
  * Mixed-precision CG: the recurrences of tg_cg_solve in FP32 (FP32 values a32

@@ -45,6 +45,8 @@ SIGNATURES = {
     "tg_cg_solve": (C.c_int, [i32, i64, vp, C.c_int, vp, vp, C.c_int, i32, vp, i32, i32, dbl, vp, i64, dbl, vp,
                               i64, dbl, i32, vp, C.c_size_t, vp, vp, vp, vp]),
     "tg_cg_residuals": (C.c_int, [vp, C.c_size_t, i32, vp, vp, vp]),
+    "tg_cg_solve_noalloc": (C.c_int, [i32, i64, vp, C.c_int, vp, vp, C.c_int, i32, vp, i32, i32, dbl, vp, i64, dbl,
+                                      vp, i64, dbl, i32, vp, vp, vp, vp]),
     "tg_gmres_workspace_bytes": (C.c_size_t, [i32, i32, i32, C.c_int]),
     "tg_gmres_workspace_parts": (C.c_int, [vp, C.c_size_t, i32, i32, i32, C.c_int, vp, vp, vp, vp]),
     "tg_gmres_solve": (C.c_int, [i32, i64, vp, C.c_int, vp, vp, C.c_int, i32, vp, i32, i32, i32, dbl, vp, i64,
