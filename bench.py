@@ -79,6 +79,8 @@ def parse():
                     help="batched CG steps as captured hipGraphs (1 default, 0 eager launches)")
     ap.add_argument("--batch-spmm", type=int, default=-1,
                     help="SpMM of the batched engine: 0 row loop, 1 pipelined row loop, 2 LDS-staged indices (default)")
+    ap.add_argument("--batch-xcd-order", type=int, default=-1,
+                    help="batched engine tile list: 0 round-robin over the XCDs (default), 1 contiguous runs per XCD")
     ap.add_argument("--batch-idx16", type=int, default=-1,
                     help="16-bit column offsets in the batched SpMM (1 default, 0 int32)")
     ap.add_argument("--batch-split", type=int, default=-1,
@@ -348,6 +350,8 @@ def batch_section(args, dev):
         capi.call("tg_tune_set", b"batch_idx16", args.batch_idx16)
     if args.batch_graph >= 0:
         capi.call("tg_tune_set", b"batch_graph", args.batch_graph)
+    if args.batch_xcd_order >= 0:
+        capi.call("tg_tune_set", b"batch_xcd_order", args.batch_xcd_order)
     if args.batch_spmm >= 0:
         capi.call("tg_tune_set", b"batch_spmm", args.batch_spmm)
     if args.batch_replace_drop_pct > 0:

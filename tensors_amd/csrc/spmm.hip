@@ -38,9 +38,21 @@ enum BetaMode : int { kBetaZero = 0, kBetaOne = 1, kBetaGeneric = 2 };
 // Row epilogue for V consecutive entries of one Y row: the combine_scalars /
 // combine_buffers contract y = (alpha * z) + (beta * y) (SpMV.hpp:224-225,
 // SpMM.hpp:388-393) and the empty-row rules (SpMM.hpp:395-410).
+// The rounding is fixed, not left to the compiler's contraction: alpha * z rounded
+// on its own, beta * y + (alpha * z) as one fused multiply-add, y + (alpha * z) as
+// an add; an empty row with a general beta gets beta * y rounded.
+template <typename T>
+__device__ __forceinline__ T fma_rn(T a, T b, T c) {
+    if constexpr (sizeof(T) == 8)
+        return __builtin_fma(a, b, c);
+    else
+        return __builtin_fmaf(a, b, c);
+}
+
 template <typename T, int V>
 __device__ __forceinline__ void epilogue(T* __restrict__ yp, const T (&z)[V], bool empty, T alpha,
                                          int alpha_one, T beta, int beta_mode) {
+#pragma clang fp contract(off)
     T out[V];
     if (empty) {
         if (beta_mode == kBetaOne) return;
@@ -65,7 +77,7 @@ __device__ __forceinline__ void epilogue(T* __restrict__ yp, const T (&z)[V], bo
             for (int i = 0; i < V; ++i) out[i] = out[i] + yv[i];
         } else {
 #pragma unroll
-            for (int i = 0; i < V; ++i) out[i] = out[i] + beta * yv[i];
+            for (int i = 0; i < V; ++i) out[i] = fma_rn(beta, yv[i], out[i]);
         }
     }
     store_vec<T, V>(yp, out);

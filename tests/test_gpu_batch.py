@@ -207,6 +207,27 @@ def test_graph_replayed_steps_equal_eager_steps(cuda, mixed):
         assert np.array_equal(Xg[i], Xe[i])
 
 
+@pytest.mark.parametrize("mixed", [0, 1, 2])
+def test_xcd_tile_order_is_bitwise_neutral(cuda, mixed):
+    """The tile list laid out so each XCD takes a contiguous run of tiles (knob
+    batch_xcd_order = 1) changes only which XCD runs a tile: every reduction adds the tile
+    partials by tile index, so the results are bitwise those of the default order."""
+    from tensors_amd import _capi as capi
+
+    sysl = systems(9, seed=303)
+    before = capi.lib().tg_tune_get(b"batch_xcd_order")
+    try:
+        capi.call("tg_tune_set", b"batch_xcd_order", 1)
+        Xa, ra = run_batch(sysl, cuda, mixed, 1e-10, staged=True)
+        capi.call("tg_tune_set", b"batch_xcd_order", 0)
+        Xb, rb = run_batch(sysl, cuda, mixed, 1e-10, staged=True)
+    finally:
+        capi.call("tg_tune_set", b"batch_xcd_order", before)
+    for i in range(len(sysl)):
+        assert ra[i]["iterations"] == rb[i]["iterations"] and ra[i]["outer"] == rb[i]["outer"]
+        assert np.array_equal(Xa[i], Xb[i])
+
+
 def nine_point_system(side=60, k=2, seed=12):
     """An SPD 9-point grid operator (3 x 3 neighbourhood, -1 off the diagonal, 8 + 1e-2 on
     it): 9 nonzeros per interior row, more than the LDS-staged SpMM holds for a 256-row
