@@ -101,6 +101,8 @@ def parse():
     ap.add_argument("--no-gmres", action="store_true", help="skip the single-system GMRES section")
     ap.add_argument("--serve-gmres-workers", type=int, default=6,
                     help="GMRES worker threads (own streams) of the serving loop")
+    ap.add_argument("--serve-gmres-batch-rows", type=int, default=262144,
+                    help="GMRES requests of up to this many rows join the batched engine (larger: worker streams)")
     ap.add_argument("--serve-gmres-batch", type=int, default=1,
                     help="1: GMRES requests join the batched GMRES engine; 0: one worker stream per request")
     ap.add_argument("--serve-requests", type=int, default=160)
@@ -552,6 +554,7 @@ def serve_section(args, dev):
     from tensors_amd.serving import Server
 
     capi.call("tg_tune_set", b"serve_gmres_batch", int(args.serve_gmres_batch))
+    capi.call("tg_tune_set", b"serve_gmres_batch_rows", int(args.serve_gmres_batch_rows))
     t_gen = time.perf_counter()
     reqs = serve_requests(args.serve_requests)
     t_gen = time.perf_counter() - t_gen
@@ -622,7 +625,8 @@ def serve_section(args, dev):
                     f"1-8 RHS, Jacobi, FP32 inner + FP64 refinement to 1e-10) and {len(reqs) - n_cg} GMRES(30) "
                     "(2D upwind convection-diffusion, n log-uniform [1e4, 1e6], Peclet log-uniform [1, 100], "
                     f"P = I, 1e-8, 100 restarts); host arrays in/out; 200 GB admission budget; GMRES "
-                    + ("requests in the batched GMRES engine" if args.serve_gmres_batch else
+                    + (f"requests of up to {args.serve_gmres_batch_rows} rows in the batched GMRES engine, larger "
+                       f"ones on {args.serve_gmres_workers} worker streams" if args.serve_gmres_batch else
                        f"requests on {args.serve_gmres_workers} worker streams"),
         "generation_s": round(t_gen, 2),
         "saturated": sat,

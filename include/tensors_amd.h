@@ -185,6 +185,11 @@ int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, doubl
  *                  (block row, 1-4 right-hand sides) with vector loads (default),
  *                  0 = a lane per output entry (the mapping of the other
  *                  storage orders).
+ *   "serve_gmres_batch", "serve_gmres_batch_rows": GMRES requests of servers
+ *                  created afterwards; 1 (default) = requests of up to
+ *                  serve_gmres_batch_rows rows (default 262144) join one batched
+ *                  GMRES engine (tg_gbatch_*), larger ones run on the worker
+ *                  threads; 0 = every request on a worker thread.
  * Process-wide; not thread-safe against concurrent SpMM calls. */
 int tg_tune_set(const char* key, int32_t value);
 int32_t tg_tune_get(const char* key);
@@ -484,8 +489,10 @@ size_t tg_gbatch_request_bytes(int32_t n, int32_t nrhs, int32_t restart);
  * solved while other requests run:
  *   CG requests join the batched CG engine (tg_batch_*), which advances all of
  *   them together on one stream (Jacobi `inv_diag` optional, mixed as there);
- *   GMRES(restart) requests (P = I, left side) run tg_gmres_solve on one of
- *   `gmres_workers` host threads, each with its own stream.
+ *   GMRES(restart) requests (P = I, left side) of up to serve_gmres_batch_rows
+ *   rows join one batched GMRES engine (tg_gbatch_*) advanced by a thread of its
+ *   own; larger ones (and all, with serve_gmres_batch = 0) run tg_gmres_solve on
+ *   one of `gmres_workers` host threads, each with its own stream.
  * On completion X is downloaded and all of the request's device memory freed.
  * Results do not depend on what else is in flight.  Times are seconds since
  * tg_server_create on the host steady clock.

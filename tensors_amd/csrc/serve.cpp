@@ -3,9 +3,11 @@
 // Requests arrive with host arrays.  A scheduler thread admits them in arrival
 // order while their device bytes fit in the budget, uploads each admitted CG
 // request and adds it to one batched CG engine (tg_batch_*), and advances that
-// engine between admissions; GMRES requests go to worker threads that run
-// tg_gmres_solve on their own streams.  So requests that arrive while others
-// run are admitted as memory frees up and their work overlaps on the device.
+// engine between admissions; GMRES requests of up to serve_gmres_batch_rows rows
+// join one batched GMRES engine advanced by a thread of its own, larger ones go
+// to worker threads that run tg_gmres_solve on their own streams.  So requests
+// that arrive while others run are admitted as memory frees up and their work
+// overlaps on the device.
 // A finished request's X is downloaded and its device memory freed at once.
 //
 // Nothing a request computes depends on the other requests: the batched engine
@@ -28,20 +30,40 @@
 
 namespace tg {
 
-// GMRES requests of servers created from now on: 1 = one batched GMRES engine
-// (tg_gbatch_*) advanced by a dedicated thread, 0 = each request on one of the
-// gmres_workers threads with tg_gmres_solve.
+// GMRES requests of servers created from now on: 1 = requests of up to
+// g_serve_gmres_batch_rows rows join one batched GMRES engine (tg_gbatch_*)
+// advanced by a dedicated thread, larger ones run on the gmres_workers threads
+// with tg_gmres_solve; 0 = every request on a worker thread.  The batched engine
+// has every workgroup of a system add all of that system's tile partials (so a
+// system's scalars are bitwise the same in any mix), which costs tiles^2 reads per
+// reduction: right for many small systems, wrong for a system of 1e6 rows (977
+// tiles), which the single-system solver reduces in two levels.
 static int g_serve_gmres_batch = 1;
+static int g_serve_gmres_batch_rows = 262144;  // 256 tiles of the batched engine
 
 int serve_tune_set(const char* key, int32_t value, bool* known) {
-    *known = std::string(key) == "serve_gmres_batch";
+    const std::string k(key);
+    *known = k == "serve_gmres_batch" || k == "serve_gmres_batch_rows";
     if (!*known) return TG_OK;
+    if (k == "serve_gmres_batch_rows") {
+        if (value < 1) return fail(TG_ERR_INVALID, "tg_tune_set: serve_gmres_batch_rows must be >= 1");
+        g_serve_gmres_batch_rows = value;
+        return TG_OK;
+    }
     if (value != 0 && value != 1) return fail(TG_ERR_INVALID, "tg_tune_set: serve_gmres_batch must be 0 or 1");
     g_serve_gmres_batch = value;
     return TG_OK;
 }
 
-int32_t serve_tune_get(const char* key) { return std::string(key) == "serve_gmres_batch" ? g_serve_gmres_batch : -1; }
+int32_t serve_tune_get(const char* key) {
+    const std::string k(key);
+    if (k == "serve_gmres_batch") return g_serve_gmres_batch;
+    if (k == "serve_gmres_batch_rows") return g_serve_gmres_batch_rows;
+    return -1;
+}
+
+// Whether a GMRES request of n rows goes to the batched engine under the current knobs.
+static bool gmres_goes_batched(int32_t n) { return g_serve_gmres_batch != 0 && n <= g_serve_gmres_batch_rows; }
 
 namespace {
 
@@ -64,6 +86,7 @@ struct Request {
     // device side
     char* dmem = nullptr;
     int64_t batch_id = -1;
+    bool batched = false;  // GMRES: in the batched engine (else on a worker thread)
     // outcome
     int state = kQueued;
     int status = TG_OK;
@@ -88,7 +111,8 @@ struct Server {
     std::mutex m;
     std::condition_variable cv_sched, cv_work, cv_done;
     std::deque<Request*> pending;  // arrival order
-    std::deque<Request*> gmres_ready;
+    std::deque<Request*> gmres_ready;   // for the worker threads
+    std::deque<Request*> gbatch_ready;  // for the batched GMRES thread
     std::map<int64_t, std::unique_ptr<Request>> all;
     std::vector<int64_t> finished;
     int64_t next_id = 1;
@@ -107,6 +131,7 @@ struct Server {
     // batched GMRES (gmres_batched): engine, stream and the requests in it, owned by the
     // GMRES thread
     bool gmres_batched = false;
+    int32_t gbatch_max_rows = 0;  // requests of up to this many rows join the engine
     bool gmres_enabled = false;
     hipStream_t gm_stream = nullptr;
     tg_gbatch_t gbatch = nullptr;
@@ -141,8 +166,8 @@ struct Server {
             if (r->kind == kCG) {
                 cg_out.push_back(r);
             } else {
-                gmres_ready.push_back(r);
-                cv_work.notify_one();
+                (r->batched ? gbatch_ready : gmres_ready).push_back(r);
+                cv_work.notify_all();
             }
         }
     }
@@ -330,11 +355,11 @@ struct Server {
             {
                 std::unique_lock<std::mutex> lk(m);
                 if (by_gbatch_id.empty())
-                    cv_work.wait(lk, [&] { return sched_done || !gmres_ready.empty(); });
-                if (gmres_ready.empty() && by_gbatch_id.empty()) break;  // sched_done, nothing left
-                while (!gmres_ready.empty()) {
-                    add.push_back(gmres_ready.front());
-                    gmres_ready.pop_front();
+                    cv_work.wait(lk, [&] { return sched_done || !gbatch_ready.empty(); });
+                if (gbatch_ready.empty() && by_gbatch_id.empty()) break;  // sched_done, nothing left
+                while (!gbatch_ready.empty()) {
+                    add.push_back(gbatch_ready.front());
+                    gbatch_ready.pop_front();
                 }
             }
             for (Request* r : add) start_gmres_batched(r);
@@ -436,7 +461,7 @@ extern "C" size_t tg_server_request_bytes(int32_t kind, int32_t n, int64_t nnz, 
     if (n < 1 || nrhs < 1 || nnz < 0) return 0;
     const size_t in = input_bytes(n, nnz, nrhs, kind == kCG && has_diag);
     if (kind == kCG) return in + tg_batch_request_bytes(n, nnz, nrhs, has_diag, mixed);
-    if (g_serve_gmres_batch) return in + tg_gbatch_request_bytes(n, nrhs, restart);
+    if (gmres_goes_batched(n)) return in + tg_gbatch_request_bytes(n, nrhs, restart);
     return in + tg_gmres_workspace_bytes(n, restart, nrhs, TG_F64);
 }
 
@@ -456,6 +481,7 @@ extern "C" int tg_server_create(int64_t budget_bytes, int32_t gmres_workers, int
     }
     S->gmres_enabled = gmres_workers > 0;
     S->gmres_batched = g_serve_gmres_batch != 0 && gmres_workers > 0;
+    S->gbatch_max_rows = g_serve_gmres_batch_rows;
     if (S->gmres_batched) {
         rc = hipStreamCreateWithFlags(&S->gm_stream, hipStreamNonBlocking) == hipSuccess
                  ? tg_gbatch_create(&S->gbatch, S->gm_stream)
@@ -469,10 +495,8 @@ extern "C" int tg_server_create(int64_t budget_bytes, int32_t gmres_workers, int
     }
     Server* raw = S.get();
     S->sched = std::thread([raw] { raw->sched_loop(); });
-    if (S->gmres_batched)
-        S->workers.emplace_back([raw] { raw->gmres_batch_loop(); });
-    else
-        for (int i = 0; i < gmres_workers; ++i) S->workers.emplace_back([raw] { raw->gmres_loop(); });
+    if (S->gmres_batched) S->workers.emplace_back([raw] { raw->gmres_batch_loop(); });
+    for (int i = 0; i < gmres_workers; ++i) S->workers.emplace_back([raw] { raw->gmres_loop(); });
     *out = new tg_server_s{S.release()};
     return TG_OK;
 }
@@ -561,8 +585,9 @@ extern "C" int tg_server_submit_gmres(tg_server_t h, int32_t n, int64_t nnz, con
     r->tol = tol;
     r->restart = restart;
     r->max_restarts = max_restarts;
-    r->bytes = h->s->gmres_batched ? input_bytes(n, nnz, nrhs, false) + tg_gbatch_request_bytes(n, nrhs, restart)
-                                   : input_bytes(n, nnz, nrhs, false) + tg_gmres_workspace_bytes(n, restart, nrhs, TG_F64);
+    r->batched = h->s->gmres_batched && n <= h->s->gbatch_max_rows;
+    r->bytes = r->batched ? input_bytes(n, nnz, nrhs, false) + tg_gbatch_request_bytes(n, nrhs, restart)
+                          : input_bytes(n, nnz, nrhs, false) + tg_gmres_workspace_bytes(n, restart, nrhs, TG_F64);
     return submit(h, std::move(r), id_out);
 }
 

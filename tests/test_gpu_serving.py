@@ -45,14 +45,19 @@ def submit(srv, req):
     return srv.submit_gmres(rp, ci, val, B, tol=1e-8, restart=30, max_restarts=100)
 
 
-@pytest.fixture(params=[1, 0], ids=["gmres_batched", "gmres_workers"])
+@pytest.fixture(params=[(1, 1 << 30), (0, 1 << 30), (1, 2000)], ids=["gmres_batched", "gmres_workers", "gmres_split"])
 def gmres_mode(request):
-    """Serving GMRES requests through the batched GMRES engine (1, the default) or on
-    per-request worker streams (0); restores the knob afterwards."""
-    before = int(capi.lib().tg_tune_get(b"serve_gmres_batch"))
-    capi.call("tg_tune_set", b"serve_gmres_batch", request.param)
+    """Serving GMRES requests through the batched GMRES engine (every request here
+    fits it), on per-request worker streams, or split by size (systems of up to 2000
+    rows batched, the larger ones on workers, as the default split does at 262144
+    rows); restores the knobs afterwards."""
+    keys = (b"serve_gmres_batch", b"serve_gmres_batch_rows")
+    before = [int(capi.lib().tg_tune_get(k)) for k in keys]
+    for k, v in zip(keys, request.param):
+        capi.call("tg_tune_set", k, v)
     yield request.param
-    capi.call("tg_tune_set", b"serve_gmres_batch", before)
+    for k, v in zip(keys, before):
+        capi.call("tg_tune_set", k, v)
 
 
 def solve_all(reqs, budget=8 << 30, workers=3):
