@@ -57,17 +57,28 @@ using i64 = int64_t;
 // (used e.g. at src/SparseBLAS/SpMM.hpp:286-414).
 // ---------------------------------------------------------------------------
 // With pinning on (orc_set_pinning, used by the timed CPU baseline) worker t runs
-// on the t-th CPU of the process's affinity mask; the results do not depend on it.
+// on the t-th CPU of the process's affinity mask, taken once when pinning is
+// switched on (a pinned thread's own mask is one CPU, and threads it starts would
+// inherit that); the calling thread gets its own mask back after the join.  The
+// results do not depend on it.
 static std::atomic<int> g_pin{0};
+static cpu_set_t g_proc_mask;
+static std::atomic<int> g_proc_cpus{0};
+
+static void take_process_mask() {
+    cpu_set_t mask;
+    CPU_ZERO(&mask);
+    if (sched_getaffinity(0, sizeof(mask), &mask) != 0) return;
+    g_proc_mask = mask;
+    g_proc_cpus.store(CPU_COUNT(&mask));
+}
 
 static void pin_to(int t) {
-    cpu_set_t mask;
-    if (sched_getaffinity(0, sizeof(mask), &mask) != 0) return;
-    const int cpus = CPU_COUNT(&mask);
+    const int cpus = g_proc_cpus.load();
     if (cpus <= 0) return;
     int want = t % cpus, seen = 0;
     for (int c = 0; c < CPU_SETSIZE; ++c)
-        if (CPU_ISSET(c, &mask) && seen++ == want) {
+        if (CPU_ISSET(c, &g_proc_mask) && seen++ == want) {
             cpu_set_t one;
             CPU_ZERO(&one);
             CPU_SET(c, &one);
@@ -83,6 +94,8 @@ static void parallel_do(int threads, F&& f) {
         return;
     }
     const bool pin = g_pin.load() != 0;
+    cpu_set_t caller;
+    const bool restore = pin && pthread_getaffinity_np(pthread_self(), sizeof(caller), &caller) == 0;
     std::vector<std::thread> pool;
     pool.reserve(threads - 1);
     for (int t = 1; t < threads; ++t)
@@ -93,6 +106,7 @@ static void parallel_do(int threads, F&& f) {
     if (pin) pin_to(0);
     f(0);
     for (auto& th : pool) th.join();
+    if (restore) (void)pthread_setaffinity_np(pthread_self(), sizeof(caller), &caller);
 }
 
 // Uniform split of [0,n) into T blocks: the formula AssemblyCounters.hpp:87-95
@@ -1005,7 +1019,10 @@ static CSR make_csr(i64 m, i64 n, const i64* rp, const i32* ci, const double* va
 extern "C" {
 
 // Pins the worker threads of parallel regions to CPUs (1) or not (0, the default).
-void orc_set_pinning(int32_t on) { orc::g_pin.store(on ? 1 : 0); }
+void orc_set_pinning(int32_t on) {
+    if (on && orc::g_proc_cpus.load() == 0) orc::take_process_mask();
+    orc::g_pin.store(on ? 1 : 0);
+}
 
 int64_t orc_job_pointer(int64_t n, int64_t T, int64_t t) { return orc::job_pointer(n, T, t); }
 

@@ -139,3 +139,28 @@ def test_job_pointers():
     assert out[0] == 0 and out[-1] == 200 and np.all(np.diff(out) >= 0)
     per = np.diff(acc[out])
     assert per.max() - acc[-1] / 8 <= 50  # balanced to within one row's cost
+
+
+def test_pinned_oracle_runs_keep_the_callers_cpu_set():
+    """The CPU baseline pins the oracle's workers (orc_set_pinning); the calling thread
+    gets its own CPU set back after every fork-join, so a later measurement still sees
+    the process's CPU share, and repeated pinned runs spread over all of it."""
+    import os
+
+    before = os.sched_getaffinity(0)
+    rng = np.random.default_rng(8)
+    m, n, k = 400, 300, 3
+    rp, ci, val = random_csr_np(m, n, 0.05, seed=8)
+    X = rng.uniform(-1, 1, (n, k))
+    lib = oracle.lib(True)
+    lib.orc_set_pinning(1)
+    try:
+        outs = []
+        for _ in range(3):
+            Y = np.zeros((m, k))
+            oracle.spmm(rp, ci, val, m, n, 1.0, X, k, 0.0, Y, k, k, threads=4, fast=True)
+            outs.append(Y)
+            assert os.sched_getaffinity(0) == before
+    finally:
+        lib.orc_set_pinning(0)
+    assert all(np.array_equal(outs[0], o) for o in outs[1:])
