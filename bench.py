@@ -66,6 +66,9 @@ def parse():
     ap.add_argument("--cg-fused", type=int, default=-1,
                     help="CG iteration form (tg_tune_set cg_fused): 0 four kernels, 1 p.u in the SpMM, "
                          "2 fused direction, -1 library default")
+    ap.add_argument("--cg-panel", type=int, default=-1,
+                    help="rows of the CG SpMM (tg_tune_set cg_panel): 1 panel order where the band allows, "
+                         "0 contiguous chunks, -1 library default")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven CG section")
     ap.add_argument("--batch-systems", type=int, default=2000)
     ap.add_argument("--batch-iters", type=int, default=16, help="CG iterations enqueued per poll")
@@ -237,6 +240,8 @@ def cg_section(args, dev):
 
     if args.cg_fused >= 0:
         capi.call("tg_tune_set", b"cg_fused", args.cg_fused)
+    if args.cg_panel >= 0:
+        capi.call("tg_tune_set", b"cg_panel", args.cg_panel)
     g, k, tol = args.cg_grid, 32, 1e-10
     rp, ci, val = gen.laplacian_3d(g, 1e-2)
     n = g ** 3

@@ -178,6 +178,11 @@ int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, doubl
  *                  four vector kernels, 1 = p.u in the SpMM epilogue, 2 = p
  *                  update, SpMM and p.u in one kernel plus one vector kernel,
  *                  3 = default (form 1).  Results agree to rounding.
+ *   "cg_panel":    rows of that iteration's SpMM kernel, 1 = panel order where
+ *                  the band allows (default: the rows cut into panels of one band,
+ *                  every workgroup taking the same offsets of each panel, so a row's
+ *                  neighbours one band away are reused from the L2), 0 = each
+ *                  workgroup a contiguous chunk.  Results differ by rounding only.
  *   "gmres_graph": tg_gmres_solve with device operators, 1 = every restart cycle
  *                  after the first runs as one captured hipGraph (default), 0 =
  *                  eager launches.  Bitwise the same results.

@@ -329,6 +329,13 @@ __global__ void cg_xclear_kernel(CGState* __restrict__ st) { st->xpend = 0; }
 // G*V-column tile (k > G*V only for G = 64: further tiles).  The workgroup's p.u
 // partial is summed per (wave, lane group, column) in row order in LDS, then by a
 // fixed tree over the lane groups: deterministic.
+// Rows of a workgroup: panel = 0, a contiguous 1/B of [0, n); panel = P > 0, the
+// panel order: [0, n) cut into panels of P rows and the workgroup taking the same
+// 1/B of the offsets [0, P) in every panel, panel after panel.  With P the matrix
+// band (the plane of a 3D grid), a row's neighbours lie at nearby offsets of the
+// panels before and after, which the workgroups of the same XCD (contiguous
+// offsets after xcd_swizzle) work on at about the same time, so p rows are reused
+// from the L2 instead of being fetched again for each neighbouring plane.
 // FORM = false is the plain "u = A p, partial(p.u)" step of the three-pass
 // iteration (p = p0 is formed by cg_p_kernel; r, d, p1 unused).
 template <typename T, typename RP, int G, int V, bool FORM, bool HAS_D>
@@ -337,7 +344,7 @@ __global__ __launch_bounds__(kBlock) void cg_dir_spmm_kernel(int n, int k, const
                                                              const T* __restrict__ r, const T* __restrict__ d,
                                                              T* __restrict__ p0, T* __restrict__ p1,
                                                              T* __restrict__ u, double* __restrict__ partial,
-                                                             const CGState* __restrict__ st) {
+                                                             const CGState* __restrict__ st, int panel) {
     if (st->done) return;
     constexpr int S = kWave / G;  // rows in flight per wave
     constexpr int KT = G * V;     // columns per tile
@@ -354,13 +361,26 @@ __global__ __launch_bounds__(kBlock) void cg_dir_spmm_kernel(int n, int k, const
     __syncthreads();
     const int B = gridDim.x;
     const int lb = xcd_swizzle(blockIdx.x, B);
-    const int i0 = (int)((int64_t)n * lb / B), i1 = (int)((int64_t)n * (lb + 1) / B);
+    // rows: q in [0, cnt) -> first + (q / sw) * P + q % sw (panel order) or first + q
+    int first, sw, cnt, P;
+    if (panel > 0) {
+        P = panel;
+        first = (int)((int64_t)P * lb / B);
+        sw = (int)((int64_t)P * (lb + 1) / B) - first;
+        cnt = sw * ((n + P - 1) / P);
+    } else {
+        P = 0;
+        first = (int)((int64_t)n * lb / B);
+        sw = 1;
+        cnt = (int)((int64_t)n * (lb + 1) / B) - first;
+    }
     const bool odd = FORM && (st->iter & 1) != 0;
     const T* __restrict__ pp = (odd || !FORM) ? p0 : p1;  // FORM: written by the previous iteration
     T* __restrict__ pn = odd ? p1 : p0;
-    for (int rbase = i0 + wave * S; rbase < i1; rbase += WAVES * S) {
-        const int row = rbase + s;
-        const bool rok = row < i1;
+    for (int qb = wave * S; qb < cnt; qb += WAVES * S) {
+        const int q = qb + s;
+        const int row = P > 0 ? first + (q / sw) * P + q % sw : first + q;
+        const bool rok = q < cnt && row < n;
         const RP lo = rok ? rp[row] : RP(0), hi = rok ? rp[row + 1] : RP(0);
         const int len = (int)(hi - lo);
         int wmax = len;  // longest row of the wave (uniform loop bound)
@@ -379,7 +399,7 @@ __global__ __launch_bounds__(kBlock) void cg_dir_spmm_kernel(int n, int k, const
             for (int i = 0; i < V; ++i) acc[i] = T(0);
             // the row's own r and p_prev (p for FORM = false), issued before the
             // gathers so they are in flight with them (clamped to a valid row)
-            const int64_t own = (int64_t)(rok ? row : i0) * k + colc;
+            const int64_t own = (int64_t)(rok ? row : first) * k + colc;
             T rv_own[V], pv_own[V];
             if constexpr (FORM) load_vec<T, V>(r + own, rv_own);
             load_vec<T, V>(pp + own, pv_own);
@@ -546,23 +566,40 @@ size_t cg_workspace_bytes(int32_t n, int32_t k, size_t vs, int prec_kind) {
 // always take form 0.
 enum CgForm : int { kFourKernels = 0, kSpmmDot = 1, kFormDir = 2, kFormAuto = 3 };
 static int g_cg_fused = kFormAuto;
+// rows of the forms' SpMM kernel: 1 = panel order where the band allows, 0 = contiguous chunks
+static int g_cg_panel = 1;
 
 int cg_tune_set(const char* key, int32_t value, bool* known) {
-    *known = std::string(key) == "cg_fused";
+    const std::string k(key);
+    *known = k == "cg_fused" || k == "cg_panel";
     if (!*known) return TG_OK;
+    if (k == "cg_panel") {
+        if (value != 0 && value != 1) return fail(TG_ERR_INVALID, "tg_tune_set: cg_panel must be 0 or 1");
+        g_cg_panel = value;
+        return TG_OK;
+    }
     if (value < 0 || value > 3) return fail(TG_ERR_INVALID, "tg_tune_set: cg_fused must be 0, 1, 2 or 3 (auto)");
     g_cg_fused = value;
     return TG_OK;
 }
 
-int32_t cg_tune_get(const char* key) { return std::string(key) == "cg_fused" ? g_cg_fused : -1; }
+// Whether the panel order could apply to an n-row system (so the caller measures the band).
+bool cg_panel_wants_band(int64_t n) { return g_cg_panel != 0 && n >= 16 * (int64_t)kFusedBlocks; }
+
+int32_t cg_tune_get(const char* key) {
+    const std::string k(key);
+    if (k == "cg_fused") return g_cg_fused;
+    if (k == "cg_panel") return g_cg_panel;
+    return -1;
+}
 
 template <typename T, typename RP, int V>
 static int launch_dir_v(int G, int form, bool has_d, hipStream_t s, int n, int k, const RP* rp, const int32_t* ci,
-                        const T* a, const T* r, const T* d, T* p0, T* p1, T* u, double* partial, const CGState* st) {
+                        const T* a, const T* r, const T* d, T* p0, T* p1, T* u, double* partial, const CGState* st,
+                        int panel) {
 #define TG_DIRK(GG, FORM, HD)                                                                                  \
     hipLaunchKernelGGL((cg_dir_spmm_kernel<T, RP, GG, V, FORM, HD>), dim3(kFusedBlocks), dim3(kBlock), 0, s, n, k, \
-                       rp, ci, a, r, d, p0, p1, u, partial, st)
+                       rp, ci, a, r, d, p0, p1, u, partial, st, panel)
 #define TG_DIR(GG)                     \
     case GG:                           \
         if (form != kFormDir)          \
@@ -592,7 +629,7 @@ static int launch_dir_v(int G, int form, bool has_d, hipStream_t s, int n, int k
 // allows), G = enough lanes for all k columns (at most 64; then k / (64 V) tiles).
 template <typename T, typename RP>
 static int launch_dir(int form, hipStream_t s, int n, int k, const RP* rp, const int32_t* ci, const T* a,
-                      const T* r, const T* d, T* p0, T* p1, T* u, double* partial, const CGState* st) {
+                      const T* r, const T* d, T* p0, T* p1, T* u, double* partial, const CGState* st, int panel) {
     int V = 1;
     if (sizeof(T) == 8) {
         V = (k % 2 == 0) ? 2 : 1;  // 4 doubles per lane measured slower on the 128^3 Laplacian
@@ -604,12 +641,15 @@ static int launch_dir(int form, hipStream_t s, int n, int k, const RP* rp, const
     const bool has_d = d != nullptr;
     switch (V) {
         case 1:
-            return launch_dir_v<T, RP, 1>(G, form, has_d, s, n, k, rp, ci, a, r, d, p0, p1, u, partial, st);
+            return launch_dir_v<T, RP, 1>(G, form, has_d, s, n, k, rp, ci, a, r, d, p0, p1, u, partial, st,
+                                                  panel);
         case 2:
-            return launch_dir_v<T, RP, 2>(G, form, has_d, s, n, k, rp, ci, a, r, d, p0, p1, u, partial, st);
+            return launch_dir_v<T, RP, 2>(G, form, has_d, s, n, k, rp, ci, a, r, d, p0, p1, u, partial, st,
+                                                  panel);
         default:
             if constexpr (sizeof(T) == 4)
-                return launch_dir_v<T, RP, 4>(G, form, has_d, s, n, k, rp, ci, a, r, d, p0, p1, u, partial, st);
+                return launch_dir_v<T, RP, 4>(G, form, has_d, s, n, k, rp, ci, a, r, d, p0, p1, u, partial, st,
+                                                  panel);
             return fail(TG_ERR_INVALID, "internal: bad vector width");
     }
 }
@@ -665,12 +705,18 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
     int form = g_cg_fused == kFormAuto ? kSpmmDot : g_cg_fused;
     if (!csr_ok) form = kFourKernels;
     const bool fused = form == kFormDir;
+    // panel order for the SpMM kernel of forms 1 / 2 (cg_dir_spmm_kernel): panels of
+    // one band, when that gives every workgroup at least 4 rows per panel and the
+    // matrix at least 4 panels (the band as the caller measured it; none measured: off)
+    int panel = 0;
+    if (form != kFourKernels && g_cg_panel && Ac.band >= 4 * (int64_t)kFusedBlocks && (int64_t)n >= 4 * Ac.band)
+        panel = (int)Ac.band;
     auto dir_spmm = [&]() -> int {
         if (Ac.rp_type == TG_I64)
             return launch_dir<T, int64_t>(form, s, n, k, (const int64_t*)Ac.rp, Ac.ci, (const T*)Ac.val, r, dd, p,
-                                          p1, u, partial, st);
+                                          p1, u, partial, st, panel);
         return launch_dir<T, int32_t>(form, s, n, k, (const int32_t*)Ac.rp, Ac.ci, (const T*)Ac.val, r, dd, p, p1,
-                                      u, partial, st);
+                                      u, partial, st, panel);
     };
 
     // z = P r by callback, then partial(r.z) -- the external-preconditioner step
@@ -889,7 +935,7 @@ extern "C" int tg_cg_solve(int32_t n, int64_t nnz, const void* rp, tg_dtype rp_t
     Aop.csr = &A;
     OpRef Pop;
     hipStream_t s = as_stream(stream);
-    if (spmm_may_slab(A, nrhs)) A.band = csr_band(A, s);
+    if (spmm_may_slab(A, nrhs) || cg_panel_wants_band(n)) A.band = csr_band(A, s);
     if (val_type == TG_F64)
         return cg_entry<double>(A, Aop, prec_kind, prec_diag, Pop, nrhs, max_iter, a_coef, B, ldB, b_coef, X, ldX, tol,
                                 use_guess, workspace, workspace_bytes, succeeded, iterations, rel_residuals, s);

@@ -23,6 +23,9 @@ struct CsrView {
 // Bandwidth max |ci - row| of a device CSR (one pass over ci; synchronises s).
 // Returns -1 if it could not be computed.
 int64_t csr_band(const CsrView& A, hipStream_t s);
+// Whether tg_cg_solve's panel-ordered SpMM could apply to an n-row system, so that
+// the caller measures the band (cg.hip).
+bool cg_panel_wants_band(int64_t n);
 // Whether spmm_dispatch would cut X into column slabs for this shape, so that
 // knowing the band can matter.
 bool spmm_may_slab(const CsrView& A, int32_t nrhs);

@@ -426,7 +426,7 @@ extern "C" int tg_cg_solve_mixed(int32_t n, int64_t nnz, const void* rp, tg_dtyp
     A.ci = ci;
     A.val = a64;
     A.val_type = TG_F64;
-    if (spmm_may_slab(A, nrhs)) A.band = csr_band(A, as_stream(stream));
+    if (spmm_may_slab(A, nrhs) || cg_panel_wants_band(n)) A.band = csr_band(A, as_stream(stream));
     MixedResult res{};
     const int rc = cg_mixed(A, a32, prec_kind, prec_diag, nrhs, max_outer, max_inner, replace_drop, a_coef, B, ldB,
                             b_coef, X, ldX, tol, use_guess, workspace, workspace_bytes, &res, as_stream(stream));

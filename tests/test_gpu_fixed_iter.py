@@ -90,6 +90,49 @@ def test_cg_fixed_iterations_3d_wide(cuda, steps):
     assert res_err(cg.Residuals(), ar_o, ar_o / rr_o) <= TOL
 
 
+@pytest.fixture
+def panel_knob():
+    before = int(capi.lib().tg_tune_get(b"cg_panel"))
+    yield lambda v: capi.call("tg_tune_set", b"cg_panel", int(v))
+    capi.call("tg_tune_set", b"cg_panel", before)
+
+
+@pytest.mark.parametrize("panel", [1, 0], ids=["panel_order", "contiguous"])
+@pytest.mark.parametrize("prec", [False, True], ids=["identity", "jacobi"])
+def test_cg_fixed_iterations_panel_order(cuda, panel_knob, panel, prec):
+    """The SpMM of the default iteration with its rows in panel order: the 7-point
+    Laplacian (+ 0.1) of a 128 x 64 x 8 grid, x fastest, has band 8192, so the rows are
+    cut into 8 panels of one plane and every workgroup takes 4 offsets of each (knob
+    cg_panel = 1, the default); the contiguous chunks (cg_panel = 0) for comparison.  Both
+    at 1e-12 from the oracle after 20 steps (a well-conditioned operator, so that 20 steps
+    do not amplify rounding past that bar)."""
+    panel_knob(panel)
+
+    def path(m):
+        return sp.diags([-np.ones(m - 1), 2 * np.ones(m), -np.ones(m - 1)], [-1, 0, 1])
+
+    nx, ny, nz = 128, 64, 8
+    A = (sp.kron(sp.identity(nz * ny), path(nx)) + sp.kron(sp.kron(sp.identity(nz), path(ny)), sp.identity(nx))
+         + sp.kron(path(nz), sp.identity(ny * nx)) + 0.1 * sp.identity(nx * ny * nz)).tocsr()
+    A.sort_indices()
+    rp, ci, val = A.indptr.astype(np.int64), A.indices.astype(np.int32), A.data.astype(np.float64)
+    assert int(np.max(np.abs(ci - np.repeat(np.arange(A.shape[0]), np.diff(rp))))) == nx * ny
+    n, k, steps = A.shape[0], 8, 20
+    d = 1.0 / sp.csr_matrix((val, ci, rp), shape=(n, n)).diagonal() if prec else None
+    B = gen.uniform(809, n * k).reshape(n, k)
+    Xo = np.zeros((n, k))
+    _, it_o, rr_o, ar_o = oracle.cg_ex(n, rp, ci, val, B, Xo, k, steps, 0.0, prec_diag=d, threads=8)
+    cg = ConjugateGradient(n, steps, k)
+    Xd = torch.zeros(n, k, dtype=torch.float64, device=cuda)
+    cg(MatrixCSR(rp.astype(np.int32), ci, val, n, n), None if d is None else dev(d, cuda), 1.0, dev(B, cuda), 0.0,
+       Xd, 0.0)
+    print(f"CG panel {panel} prec {prec}")
+    assert cg.IterationCount() == it_o == steps
+    assert rel_err(Xd.cpu().numpy(), Xo) <= TOL
+    assert res_err(cg.RelativeResiduals(), rr_o) <= TOL
+    assert res_err(cg.Residuals(), ar_o, ar_o / rr_o) <= TOL
+
+
 @pytest.mark.parametrize("m", [5, 12])
 @pytest.mark.parametrize("cycles", [1, 2])
 @pytest.mark.parametrize("side", [0, 1], ids=["left", "right"])
