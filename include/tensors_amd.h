@@ -133,8 +133,10 @@ int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, doubl
 /* Kernel-shape knobs read by tg_csr_spmm (no reference counterpart; the
  * reference picks its kernel by template arguments).  Keys:
  *   "spmv_kernel": nrhs == 1 kernel, 0 = streaming (default), 1 = team-per-row.
- *   "spmv_team":   lanes per row for nrhs == 1, 0 = from the mean row length,
- *                  else a power of two in [4, 64] (the streaming kernel caps it at 32).
+ *   "spmv_team":   lanes per row for nrhs == 1, 0 = from the mean row length
+ *                  (streaming: a quarter of it, 1..32; team-per-row: all of it,
+ *                  4..64), else a power of two in [1, 64] (the streaming kernel
+ *                  caps it at 32, the team-per-row kernel raises it to 4).
  *   "spmm_group":  lanes per RHS tile of the nrhs >= 2 kernel (a wave's 64/G lane
  *                  groups split each row's nonzeros), 0 = enough lanes for all
  *                  columns, else a power of two in [1, 64].

@@ -37,7 +37,7 @@ def set_knobs(cfg):
 def candidates(nrhs):
     if nrhs == 1:
         out = [{"spmv_kernel": 0, "spmv_team": 0}, {"spmv_kernel": 1, "spmv_team": 0}]
-        out += [{"spmv_kernel": 0, "spmv_team": t} for t in (4, 8, 16, 32)]
+        out += [{"spmv_kernel": 0, "spmv_team": t} for t in (1, 2, 4, 8, 16, 32)]
         out += [{"spmv_kernel": 1, "spmv_team": t} for t in (4, 8, 16, 32, 64)]
         return out
     gmax = 1
