@@ -253,8 +253,8 @@ int tg_cg_residuals(const void* workspace, size_t workspace_bytes, int32_t nrhs,
  * serves callers that cannot allocate device memory (serving loops that must not
  * call hipMalloc between requests, harnesses that forbid allocation).  Same
  * arguments, kernels and results as tg_cg_solve, minus the workspace, which is a
- * static device array of TG_CG_NOALLOC_SCRATCH_BYTES that every GPU the library
- * runs on holds until the process ends.  A system whose
+ * static device array of TG_CG_NOALLOC_SCRATCH_BYTES (plus 4 KiB for the band
+ * pass) that every GPU the library runs on holds until the process ends.  A system whose
  * tg_cg_workspace_bytes(n, nrhs, val_type, prec_kind) exceeds it is refused with
  * TG_ERR_WORKSPACE before any device work.  All calls on one GPU share that
  * GPU's array: calls that overlap on one GPU (several host threads, or streams

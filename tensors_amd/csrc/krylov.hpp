@@ -23,6 +23,9 @@ struct CsrView {
 // Bandwidth max |ci - row| of a device CSR (one pass over ci; synchronises s).
 // Returns -1 if it could not be computed.
 int64_t csr_band(const CsrView& A, hipStream_t s);
+// csr_band into a caller's device buffer of kBandInts int32 (allocates nothing).
+constexpr int kBandInts = 1024;
+int64_t csr_band_into(const CsrView& A, int32_t* buf, hipStream_t s);
 // Whether tg_cg_solve's panel-ordered SpMM could apply to an n-row system, so that
 // the caller measures the band (cg.hip).
 bool cg_panel_wants_band(int64_t n);

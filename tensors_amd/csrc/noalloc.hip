@@ -13,7 +13,11 @@
 // (two host threads, or two streams whose work overlaps) race on it.  The array
 // keeps the last call's contents; the solver initialises every part of its
 // workspace that it reads (state block by a copy, vectors and partial sums by
-// its setup kernels), so nothing has to start at zero.
+// its setup kernels), and the band pass writes every entry of its buffer that it
+// reads, so nothing has to start at zero.
+//
+// Layout: the solver's workspace (at most TG_CG_NOALLOC_SCRATCH_BYTES) at the
+// start, then the band pass's kBandInts int32 in the last 4 KiB.
 
 #include <mutex>
 
@@ -22,7 +26,10 @@
 
 namespace tg {
 
-constexpr size_t kScratchBytes = TG_CG_NOALLOC_SCRATCH_BYTES;
+constexpr size_t kWorkspaceBytes = TG_CG_NOALLOC_SCRATCH_BYTES;
+constexpr size_t kBandBytes = 4096;
+static_assert(kBandInts * sizeof(int32_t) <= kBandBytes, "band buffer");
+constexpr size_t kScratchBytes = kWorkspaceBytes + kBandBytes;
 constexpr int kMaxDevices = 64;
 
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
@@ -58,7 +65,7 @@ static int noalloc_entry(const CsrView& A, int32_t prec_kind, const void* prec_d
     OpRef Pop;
     CGResult res{};
     const int rc = cg_solve_t<T>(A, Aop, prec_kind, (const T*)prec_diag, Pop, nrhs, max_iter, a_coef, (const T*)B,
-                                 ldB, b_coef, (T*)X, ldX, tol, use_guess, ws, kScratchBytes, &res, s);
+                                 ldB, b_coef, (T*)X, ldX, tol, use_guess, ws, kWorkspaceBytes, &res, s);
     if (rc) return rc;
     if (succeeded) *succeeded = res.succeeded;
     if (iterations) *iterations = res.iterations;
@@ -83,11 +90,11 @@ extern "C" int tg_cg_solve_noalloc(int32_t n, int64_t nnz, const void* rp, tg_dt
     if (rp_type != TG_I32 && rp_type != TG_I64) return fail(TG_ERR_DTYPE, "tg_cg_solve_noalloc: rp_type must be I32/I64");
     if (val_type != TG_F64 && val_type != TG_F32) return fail(TG_ERR_DTYPE, "tg_cg_solve_noalloc: val_type must be F64/F32");
     const size_t need = cg_workspace_bytes(n, nrhs, val_type == TG_F32 ? 4 : 8, prec_kind ? 1 : 0);
-    if (need > kScratchBytes)
+    if (need > kWorkspaceBytes)
         return fail(TG_ERR_WORKSPACE,
                     "tg_cg_solve_noalloc: n = %d with %d right-hand sides needs %zu bytes of workspace; the static "
                     "scratch holds %zu (use tg_cg_solve with a workspace instead)",
-                    n, nrhs, need, kScratchBytes);
+                    n, nrhs, need, kWorkspaceBytes);
     void* ws = nullptr;
     const int rs = scratch_address(&ws);
     if (rs) return rs;
@@ -100,9 +107,13 @@ extern "C" int tg_cg_solve_noalloc(int32_t n, int64_t nnz, const void* rp, tg_dt
     A.ci = ci;
     A.val = a;
     A.val_type = val_type;
-    // The band is not measured (that pass allocates): it only matters for column slabs,
-    // which an X block this small never takes, so the kernels are tg_cg_solve's.
+    // The band decides the SpMM's row order as in tg_cg_solve (column slabs never apply:
+    // X is at most a fifth of the workspace); it is measured into the array's last 4 KiB.
     hipStream_t s = as_stream(stream);
+    if (cg_panel_wants_band(n)) {
+        A.band = csr_band_into(A, (int32_t*)((char*)ws + kWorkspaceBytes), s);
+        if (A.band < 0) return fail(TG_ERR_HIP, "tg_cg_solve_noalloc: band pass failed");
+    }
     if (val_type == TG_F64)
         return noalloc_entry<double>(A, prec_kind, prec_diag, nrhs, max_iter, a_coef, B, ldB, b_coef, X, ldX, tol,
                                      use_guess, ws, succeeded, iterations, rel_residuals, s);

@@ -1103,21 +1103,30 @@ __global__ __launch_bounds__(kBlock) void band_kernel(int m, const RP* __restric
     if (threadIdx.x == 0) block_max[blockIdx.x] = sh[0];
 }
 
-int64_t csr_band(const CsrView& A, hipStream_t s) {
+static int band_grid(const CsrView& A) { return (int)std::min<int64_t>((A.m + kBlock - 1) / kBlock, kBandInts); }
+
+// The band pass with a caller's device buffer of kBandInts int32 (no allocation).
+int64_t csr_band_into(const CsrView& A, int32_t* d, hipStream_t s) {
     if (A.m <= 0 || A.nnz <= 0 || !A.ci) return 0;
-    const int grid = (int)std::min<int64_t>((A.m + kBlock - 1) / kBlock, 1024);
-    int32_t* d = nullptr;
-    if (hipMallocAsync((void**)&d, grid * sizeof(int32_t), s) != hipSuccess) return -1;
+    const int grid = band_grid(A);
     if (A.rp_type == TG_I32)
         hipLaunchKernelGGL((band_kernel<int32_t>), dim3(grid), dim3(kBlock), 0, s, A.m, (const int32_t*)A.rp, A.ci, d);
     else
         hipLaunchKernelGGL((band_kernel<int64_t>), dim3(grid), dim3(kBlock), 0, s, A.m, (const int64_t*)A.rp, A.ci, d);
     std::vector<int32_t> h(grid, 0);
     const bool ok = hipMemcpyAsync(h.data(), d, grid * sizeof(int32_t), hipMemcpyDeviceToHost, s) == hipSuccess;
-    (void)hipFreeAsync(d, s);
     if (!ok || hipStreamSynchronize(s) != hipSuccess) return -1;
     int64_t b = 0;
     for (int32_t v : h) b = std::max<int64_t>(b, v);
+    return b;
+}
+
+int64_t csr_band(const CsrView& A, hipStream_t s) {
+    if (A.m <= 0 || A.nnz <= 0 || !A.ci) return 0;
+    int32_t* d = nullptr;
+    if (hipMallocAsync((void**)&d, band_grid(A) * sizeof(int32_t), s) != hipSuccess) return -1;
+    const int64_t b = csr_band_into(A, d, s);  // synchronises s
+    (void)hipFreeAsync(d, s);
     return b;
 }
 

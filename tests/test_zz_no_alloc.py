@@ -64,6 +64,47 @@ def test_noalloc_matches_allocated_entry(cuda, dtype, prec):
         assert np.array_equal(rr_a, rr_s)
 
 
+def lap3d_system(nx, ny, nz, k, dtype, dev, seed):
+    """7-point Laplacian (+ 0.1) of an nx x ny x nz grid, x fastest: band nx * ny."""
+    def path(m):
+        return sp.diags([-np.ones(m - 1), 2 * np.ones(m), -np.ones(m - 1)], [-1, 0, 1])
+
+    A = (sp.kron(sp.identity(nz * ny), path(nx)) + sp.kron(sp.kron(sp.identity(nz), path(ny)), sp.identity(nx))
+         + sp.kron(path(nz), sp.identity(ny * nx)) + 0.1 * sp.identity(nx * ny * nz)).tocsr()
+    A.sort_indices()
+    n = A.shape[0]
+    B = gen.uniform(seed, n * k).reshape(n, k)
+    T = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a)).to(dev).to(dt)
+    return (n, A.nnz, T(A.indptr, torch.int32), T(A.indices, torch.int32), T(A.data, dtype), None, T(B, dtype))
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32], ids=["f64", "f32"])
+def test_noalloc_matches_allocated_entry_in_panel_order(cuda, dtype):
+    """A system large enough for the band-dependent SpMM row order (65 536 rows, band 8192:
+    panels of one plane): both entries measure the band (the no-allocation one into the
+    array's last 4 KiB) and take the same order, so they still agree bit for bit; then a
+    smaller system.  In FP64 the first system's result does depend on that order (knob
+    cg_panel), which shows the case takes the band-dependent path; in FP32 the products
+    of two floats are exact in the double partial sums, so the order need not show."""
+    tol = 1e-10 if dtype == torch.float64 else 1e-5
+    for sysd, k, what in ((lap3d_system(128, 64, 8, 4, dtype, cuda, 71), 4, "128x64x8"),
+                          (system(40, 2, dtype, False, cuda, 72), 2, "grid 40")):
+        print(f"noalloc vs tg_cg_solve: {what} nrhs {k} {dtype}", flush=True)
+        ok_a, it_a, rr_a, X_a = solve("tg_cg_solve", sysd, k, dtype, tol, 3000, cuda)
+        ok_s, it_s, rr_s, X_s = solve("tg_cg_solve_noalloc", sysd, k, dtype, tol, 3000, cuda)
+        assert ok_a and ok_s and it_a == it_s > 0
+        assert torch.equal(X_a, X_s)
+        assert np.array_equal(rr_a, rr_s)
+        if what == "128x64x8" and dtype == torch.float64:
+            before = int(capi.lib().tg_tune_get(b"cg_panel"))
+            capi.call("tg_tune_set", b"cg_panel", 0)
+            try:
+                _, _, _, X_c = solve("tg_cg_solve_noalloc", sysd, k, dtype, tol, 3000, cuda)
+            finally:
+                capi.call("tg_tune_set", b"cg_panel", before)
+            assert not torch.equal(X_c, X_s)
+
+
 def test_noalloc_refuses_oversize_system(cuda):
     n, k = 1 << 20, 8  # 8 MiB per FP64 vector column block x 8 columns: far above the array
     tg = capi.F64
