@@ -97,6 +97,9 @@ def parse():
     ap.add_argument("--slices", type=int, default=-1,
                     help="column slices of the planned SpMM (tg_tune_set spmm_slices; -1 = library default)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop section")
+    ap.add_argument("--line-rate", action="store_true",
+                    help="add the line-rate probe: the FP64 16-column SpMM of the suite's row lengths with the "
+                         "columns limited to [0, W) for X of 3.8-122 MB")
     ap.add_argument("--no-bsr", action="store_true", help="skip the block-CSR SpMM section")
     ap.add_argument("--bsr-vector", type=int, default=-1,
                     help="block-CSR lane mapping (tg_tune_set bsr_vector: 1 vector, 0 scalar; -1 default)")
@@ -639,6 +642,53 @@ def serve_section(args, dev):
     }
 
 
+def line_rate_section(dev):
+    """What bounds the random-column SpMM: the suite's rows (Poisson(32), 1e6 rows) with the
+    columns drawn from [0, W), FP64, 16 right-hand sides (one 128-byte line of X per
+    nonzero), plain tg_csr_spmm.  With X small enough for the L2s every gather hits there,
+    so the time left is the rate at which the CUs issue random line requests; at the suite's
+    width X sits in the Infinity Cache.  Reports per W: time, line requests per second
+    (one per nonzero plus A and Y) and the rate in bytes."""
+    import torch
+
+    from tensors_amd import _capi as capi
+    from tensors_amd import generators as gen
+
+    s = torch.cuda.current_stream()
+    k, out = 16, []
+    for W in (31250, 125000, 500000, 1000000):
+        rp, ci, v64, _ = gen.random_csr(M, W, 32.0, 1, 512, SEED, want64=True, want32=False)
+        nnz = int(rp[-1])
+        rpd = torch.from_numpy(rp.astype(np.int32)).to(dev)
+        cid = torch.from_numpy(ci).to(dev)
+        vd = torch.from_numpy(v64).to(dev)
+        X = torch.from_numpy(gen.uniform(SEED + 7, W * k)).to(dev).view(W, k)
+        Y = torch.empty(M, k, dtype=torch.float64, device=dev)
+
+        def go():
+            capi.call("tg_csr_spmm", M, W, nnz, rpd.data_ptr(), capi.I32, cid.data_ptr(), vd.data_ptr(), capi.F64,
+                      1.0, X.data_ptr(), k, 0.0, Y.data_ptr(), k, k, k, None, 0, s.cuda_stream)
+
+        for _ in range(3):
+            go()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(s)
+        for _ in range(10):
+            go()
+        e1.record(s)
+        torch.cuda.synchronize()
+        sec = e0.elapsed_time(e1) * 1e-3 / 10
+        fill = line_fill_bytes(nnz, M, k, 8)
+        out.append({"W": W, "X_MB": round(W * k * 8 / 2 ** 20, 1), "nnz": nnz, "us": round(sec * 1e6, 1),
+                    "G_lines_per_s": round(fill / 128 / sec / 1e9, 1), "line_GBps": round(fill / sec / 1e9, 1)})
+        log(f"line rate: {out[-1]}")
+        del rpd, cid, vd, X, Y
+        torch.cuda.empty_cache()
+    return {"workload": "1e6 rows, Poisson(32) row lengths clipped [1,512], columns uniform in [0, W), FP64, "
+                        "16 right-hand sides (one 128 B line of X per nonzero), tg_csr_spmm", "cases": out}
+
+
 def bsr_section(args, dev, read_gbps):
     """Block-CSR SpMM (KernelMatrixCSR<DenseBlockKernel_Tiny>::Dot, tg_bsr_spmm): R x R FP64 blocks
     on the 3D 7-point grid pattern (block rows g^3), X / Y row-major chunks, alpha = 1, beta = 0.
@@ -948,6 +998,9 @@ def main():
     # free the SpMM suite's buffers before the solver section
     del cases, vals, Xs, rp32, cid, plan
     torch.cuda.empty_cache()
+    if args.line_rate:
+        log("line-rate probe")
+        out["line_rate"] = line_rate_section(dev)
     if not args.no_bsr:
         log("block-CSR section")
         out["bsr"] = bsr_section(args, dev, read_gbps)
