@@ -192,6 +192,10 @@ int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, doubl
  *                  (block row, 1-4 right-hand sides) with vector loads (default),
  *                  0 = a lane per output entry (the mapping of the other
  *                  storage orders).
+ *   "gbatch_big_tiles": systems added to a batched GMRES engine with more than
+ *                  this many 1024-row tiles (default 64) get each reduction's totals
+ *                  from one workgroup per system instead of every tile workgroup
+ *                  adding all tiles' partials.  Bitwise the same results.
  *   "serve_gmres_batch", "serve_gmres_batch_rows": GMRES requests of servers
  *                  created afterwards; 1 (default) = requests of up to
  *                  serve_gmres_batch_rows rows (default 262144) join one batched

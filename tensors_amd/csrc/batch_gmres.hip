@@ -65,7 +65,9 @@ struct GbSys {
     double* p1;    // tiles x (m+1) x k partials of the first Gram-Schmidt pass
     double* p2;    // ... of the second pass
     double* pn;    // tiles x k partials of a squared norm
+    double* tot;   // (kGbMaxM + 1) x kGbMaxK totals of the last reduction (big systems)
     int n, k, kpad, tiles, tile_rows, m;
+    int big;       // more than g_gb_big_tiles tiles: the totals come from gb_totals_kernel
 };
 
 struct GbState {
@@ -147,6 +149,20 @@ __device__ __forceinline__ void system_totals(const double* part, int tiles, int
         }
     }
     __syncthreads();
+}
+
+// The totals a consumer kernel needs: for a big system the ones gb_totals_kernel
+// published (the same system_totals, same block size: bitwise the same numbers),
+// otherwise summed here from the tile partials.
+__device__ __forceinline__ void gb_totals(const GbSys& S, const double* part, int stride, int nq, double* out,
+                                          double* sh) {
+    if (S.big) {
+        for (int p = threadIdx.x; p < nq * kGbMaxK; p += kBlock)
+            if (p % kGbMaxK < S.k) out[p] = S.tot[p];
+        __syncthreads();
+    } else {
+        system_totals(part, S.tiles, stride, nq, S.k, S.kpad, out, sh);
+    }
 }
 
 // ---- setup ----------------------------------------------------------------------
@@ -254,7 +270,7 @@ __global__ __launch_bounds__(kBlock) void gb_start2_kernel(const GbTile* __restr
     if (s->done) return;
     __shared__ double sh[kBlock];
     __shared__ double tot[kGbMaxK * kGbMaxK];
-    system_totals(S.pn, S.tiles, 1, 1, k, kpad, tot, sh);
+    gb_totals(S, S.pn, 1, 1, tot, sh);
     if (c < k) {
         const double inv = 1.0 / sqrt(tot[c]);
         for (int row = T.r0 + ro; row < T.r1; row += rows) S.Q[(int64_t)row * k + c] *= inv;
@@ -320,7 +336,7 @@ __global__ __launch_bounds__(kBlock) void gb_step23_kernel(const GbTile* __restr
     double* __restrict__ q = S.Q + (int64_t)(j + 1) * N;
     __shared__ double sh[kBlock];
     __shared__ double h[kGbMaxM + 1][kGbMaxK];
-    system_totals(PASS == 0 ? S.p1 : S.p2, S.tiles, S.m + 1, nq, k, kpad, &h[0][0], sh);
+    gb_totals(S, PASS == 0 ? S.p1 : S.p2, S.m + 1, nq, &h[0][0], sh);
     if (T.t == 0)
         for (int p = threadIdx.x; p < nq * k; p += kBlock) {
             const int i = p / k, cc = p % k;
@@ -369,7 +385,7 @@ __global__ __launch_bounds__(kBlock) void gb_step4_kernel(const GbTile* __restri
     double* __restrict__ q = S.Q + (int64_t)(j + 1) * N;
     __shared__ double sh[kBlock];
     __shared__ double tot[kGbMaxK * kGbMaxK];
-    system_totals(S.pn, S.tiles, 1, 1, k, kpad, tot, sh);
+    gb_totals(S, S.pn, 1, 1, tot, sh);
     if (c < k) {
         const double inv = 1.0 / sqrt(tot[c]);
         for (int row = T.r0 + ro; row < T.r1; row += rows) q[(int64_t)row * k + c] *= inv;
@@ -414,6 +430,29 @@ __global__ __launch_bounds__(kBlock) void gb_step4_kernel(const GbTile* __restri
         w.conv = conv;
         if (conv || j + 1 >= m) w.active = 0;
     }
+}
+
+// Totals of one reduction of each big system, one workgroup per system (grid: the
+// big active systems): phase 0 after start1 (|Q_0|^2), 1 after step1 (first-pass
+// dots), 2 after the first step23 (second-pass dots), 3 after the second (|q|^2).
+// Each big system's tile workgroups then read 1 + nq values instead of adding all
+// tiles' partials each (tiles^2 reads per reduction).
+__global__ __launch_bounds__(kBlock) void gb_totals_kernel(const int* __restrict__ bigs,
+                                                           const GbSys* __restrict__ sys,
+                                                           const GbState* __restrict__ st, int phase, int j) {
+    const int slot = bigs[blockIdx.x];
+    const GbSys S = sys[slot];
+    const GbState* __restrict__ s = st + slot;
+    if (s->done) return;
+    if (phase > 0 && (!s->active || j >= S.m)) return;
+    const int nq = (phase == 1 || phase == 2) ? j + 1 : 1;
+    const double* part = phase == 0 || phase == 3 ? S.pn : phase == 1 ? S.p1 : S.p2;
+    const int stride = (phase == 1 || phase == 2) ? S.m + 1 : 1;
+    __shared__ double sh[kBlock];
+    __shared__ double out[(kGbMaxM + 1) * kGbMaxK];
+    system_totals(part, S.tiles, stride, nq, S.k, S.kpad, out, sh);
+    for (int p = threadIdx.x; p < nq * kGbMaxK; p += kBlock)
+        if (p % kGbMaxK < S.k) S.tot[p] = out[p];
 }
 
 // tile 0: y = H^{-1} beta over the cycle's steps (BLAS::trsv, src/BLAS/trsv.hpp:7-48)
@@ -484,8 +523,23 @@ size_t gb_system_bytes(int n, int k, int m, int tile_rows) {
     const size_t N = (size_t)n * k;
     return align256((size_t)(m + 1) * N * 8) + align256((size_t)(m + 1) * m * k * 8) + 2 * align256((size_t)m * k * 8) +
            align256((size_t)(m + 1) * k * 8) + align256((size_t)m * k * 8) +
-           2 * align256((size_t)tiles * (m + 1) * k * 8) + align256((size_t)tiles * k * 8);
+           2 * align256((size_t)tiles * (m + 1) * k * 8) + align256((size_t)tiles * k * 8) +
+           align256((size_t)(kGbMaxM + 1) * kGbMaxK * 8);
 }
+
+// Systems of more than this many tiles get their totals from gb_totals_kernel
+// (tg_tune_set "gbatch_big_tiles", read when a system is added); bitwise the same.
+static int g_gb_big_tiles = 64;
+
+int gbatch_tune_set(const char* key, int32_t value, bool* known) {
+    *known = std::string(key) == "gbatch_big_tiles";
+    if (!*known) return TG_OK;
+    if (value < 0) return fail(TG_ERR_INVALID, "tg_tune_set: gbatch_big_tiles must be >= 0");
+    g_gb_big_tiles = value;
+    return TG_OK;
+}
+
+int32_t gbatch_tune_get(const char* key) { return std::string(key) == "gbatch_big_tiles" ? g_gb_big_tiles : -1; }
 
 class GmresBatch {
   public:
@@ -542,6 +596,8 @@ class GmresBatch {
         S.p1 = take((size_t)S.tiles * (m + 1) * k * 8);
         S.p2 = take((size_t)S.tiles * (m + 1) * k * 8);
         S.pn = take((size_t)S.tiles * k * 8);
+        S.tot = take((size_t)(kGbMaxM + 1) * kGbMaxK * 8);
+        S.big = S.tiles > g_gb_big_tiles ? 1 : 0;
         mem_[slot] = mem;
         bytes_[slot] = bytes;
         used_[slot] = 1;
@@ -569,7 +625,7 @@ class GmresBatch {
             if (rc) return rc;
         }
         if (ntiles_ == 0) return TG_OK;
-        const GraphKey key{dtiles_, dsys_, dst_, ntiles_, mmax_, cycles};
+        const GraphKey key{dtiles_, dsys_, dst_, dbig_, ntiles_, nbig_, mmax_, cycles};
         if (!gexec_ || !(key == gkey_)) {
             if (gexec_) (void)hipGraphExecDestroy(gexec_);
             gexec_ = nullptr;
@@ -623,14 +679,24 @@ class GmresBatch {
   private:
     void launch_cycle(hipStream_t s) {
         const dim3 g((unsigned)ntiles_), b(kBlock);
+        // totals of the big systems after each reduction (no launch without big systems)
+        auto totals = [&](int phase, int j) {
+            if (nbig_ > 0)
+                hipLaunchKernelGGL(gb_totals_kernel, dim3((unsigned)nbig_), b, 0, s, (const int*)dbig_, dsys_,
+                                   (const GbState*)dst_, phase, j);
+        };
         hipLaunchKernelGGL(gb_start1_kernel, g, b, 0, s, dtiles_, dsys_, dst_);
+        totals(0, 0);
         hipLaunchKernelGGL(gb_start2_kernel, g, b, 0, s, dtiles_, dsys_, dst_);
         for (int j = 0; j < mmax_; ++j) {
             const int nq = j + 1;
 #define TG_GB_STEPS(NQ)                                                                                 \
     hipLaunchKernelGGL((gb_step1_kernel<NQ>), g, b, 0, s, dtiles_, dsys_, dst_, j);                     \
+    totals(1, j);                                                                                       \
     hipLaunchKernelGGL((gb_step23_kernel<NQ, 0>), g, b, 0, s, dtiles_, dsys_, dst_, j);                 \
-    hipLaunchKernelGGL((gb_step23_kernel<NQ, 1>), g, b, 0, s, dtiles_, dsys_, dst_, j);
+    totals(2, j);                                                                                       \
+    hipLaunchKernelGGL((gb_step23_kernel<NQ, 1>), g, b, 0, s, dtiles_, dsys_, dst_, j);                 \
+    totals(3, j);
             if (nq <= 4) {
                 TG_GB_STEPS(4)
             } else if (nq <= 8) {
@@ -679,12 +745,14 @@ class GmresBatch {
         int rc = poll(states);
         if (rc) return rc;
         std::vector<GbTile> tl;
+        std::vector<int> bl;  // big active systems
         int mmax = 0;
         for (int i = 0; i < (int)used_.size(); ++i)
             if (used_[i] && !states[i].done) {
                 const GbSys& S = host_[i];
                 for (int t = 0; t < S.tiles; ++t)
                     tl.push_back(GbTile{i, t * S.tile_rows, std::min(S.n, (t + 1) * S.tile_rows), t});
+                if (S.big) bl.push_back(i);
                 mmax = std::max(mmax, S.m);
             }
         if (tl.size() > tiles_cap_) {
@@ -692,10 +760,17 @@ class GmresBatch {
             tiles_cap_ = std::max(tl.size(), 2 * tiles_cap_);
             TG_HIP(hipMallocAsync((void**)&dtiles_, tiles_cap_ * sizeof(GbTile), s_));
         }
+        if (bl.size() > big_cap_) {
+            if (dbig_) TG_HIP(hipFreeAsync(dbig_, s_));
+            big_cap_ = std::max(bl.size(), 2 * big_cap_);
+            TG_HIP(hipMallocAsync((void**)&dbig_, big_cap_ * sizeof(int), s_));
+        }
         if (!tl.empty())
             TG_HIP(hipMemcpyAsync(dtiles_, tl.data(), tl.size() * sizeof(GbTile), hipMemcpyHostToDevice, s_));
-        TG_HIP(hipStreamSynchronize(s_));  // the host list goes out of scope
+        if (!bl.empty()) TG_HIP(hipMemcpyAsync(dbig_, bl.data(), bl.size() * sizeof(int), hipMemcpyHostToDevice, s_));
+        TG_HIP(hipStreamSynchronize(s_));  // the host lists go out of scope
         ntiles_ = (int)tl.size();
+        nbig_ = (int)bl.size();
         mmax_ = mmax;
         dirty_ = false;
         return TG_OK;
@@ -707,6 +782,7 @@ class GmresBatch {
         if (dsys_) (void)hipFreeAsync(dsys_, s_);
         if (dst_) (void)hipFreeAsync(dst_, s_);
         if (dtiles_) (void)hipFreeAsync(dtiles_, s_);
+        if (dbig_) (void)hipFreeAsync(dbig_, s_);
         (void)hipStreamSynchronize(s_);
         if (gexec_) (void)hipGraphExecDestroy(gexec_);
         gexec_ = nullptr;
@@ -718,10 +794,11 @@ class GmresBatch {
         const void* tiles;
         const void* sys;
         const void* st;
-        int ntiles, mmax, cycles;
+        const void* big;
+        int ntiles, nbig, mmax, cycles;
         bool operator==(const GraphKey& o) const {
-            return tiles == o.tiles && sys == o.sys && st == o.st && ntiles == o.ntiles && mmax == o.mmax &&
-                   cycles == o.cycles;
+            return tiles == o.tiles && sys == o.sys && st == o.st && big == o.big && ntiles == o.ntiles &&
+                   nbig == o.nbig && mmax == o.mmax && cycles == o.cycles;
         }
     };
     hipStream_t cap_s_ = nullptr;
@@ -739,6 +816,9 @@ class GmresBatch {
     GbTile* dtiles_ = nullptr;
     size_t tiles_cap_ = 0;
     int ntiles_ = 0;
+    int* dbig_ = nullptr;  // slots of the big active systems
+    size_t big_cap_ = 0;
+    int nbig_ = 0;
     int mmax_ = 0;
     bool dirty_ = true;
 };

@@ -46,6 +46,8 @@ int bsr_tune_set(const char* key, int32_t value, bool* known);
 int32_t bsr_tune_get(const char* key);
 int gmres_tune_set(const char* key, int32_t value, bool* known);
 int32_t gmres_tune_get(const char* key);
+int gbatch_tune_set(const char* key, int32_t value, bool* known);
+int32_t gbatch_tune_get(const char* key);
 int serve_tune_set(const char* key, int32_t value, bool* known);
 int32_t serve_tune_get(const char* key);
 

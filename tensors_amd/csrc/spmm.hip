@@ -1021,6 +1021,8 @@ extern "C" int tg_tune_set(const char* key, int32_t value) {
     if (known) return rc;
     rc = gmres_tune_set(key, value, &known);
     if (known) return rc;
+    rc = gbatch_tune_set(key, value, &known);
+    if (known) return rc;
     rc = serve_tune_set(key, value, &known);
     if (known) return rc;
     return fail(TG_ERR_INVALID, "tg_tune_set: unknown key '%s'", key);
@@ -1038,6 +1040,7 @@ extern "C" int32_t tg_tune_get(const char* key) {
     if (k == "cg_fused" || k == "cg_panel") return cg_tune_get(key);
     if (k == "bsr_vector") return bsr_tune_get(key);
     if (k == "gmres_graph") return gmres_tune_get(key);
+    if (k == "gbatch_big_tiles") return gbatch_tune_get(key);
     if (k == "serve_gmres_batch" || k == "serve_gmres_batch_rows") return serve_tune_get(key);
     return batch_tune_get(key);
 }

@@ -130,3 +130,41 @@ def test_batch_gmres_special_cases(cuda):
     assert not rb["succeeded"] and torch.all(Xn == 3.0)  # NaN: nothing written
     assert rc["succeeded"] and abs(Xt.item() - Bt[0, 0].item() / 4.0) <= 1e-15
     bt.close()
+
+
+@pytest.fixture
+def big_tiles_knob():
+    from tensors_amd import _capi as capi
+
+    before = int(capi.lib().tg_tune_get(b"gbatch_big_tiles"))
+    yield lambda v: capi.call("tg_tune_set", b"gbatch_big_tiles", int(v))
+    capi.call("tg_tune_set", b"gbatch_big_tiles", before)
+
+
+def test_batch_gmres_big_system_totals_are_bitwise_neutral(cuda, big_tiles_knob):
+    """Systems of more than gbatch_big_tiles tiles get their reduction totals from one
+    workgroup per system (gb_totals_kernel) instead of every tile workgroup adding all
+    tiles' partials: the same sums in the same order, so the results are bitwise those of
+    the per-workgroup totals.  A mix with a 300^2 system (88 tiles) and small ones, run
+    with the default threshold (the large one big), with every system big (0) and with
+    none (2^30); the large system also against the host residual."""
+    sysl = systems(5, seed=7007, ms=(12, 30))
+    rp, ci, val = gen.convection_diffusion_2d(300, 20.0)
+    n = 300 * 300
+    sysl.append((rp.astype(np.int32), ci, val, None, gen.uniform(7100, n * 2).reshape(n, 2), n, 2, 30))
+    out = {}
+    for thr in (64, 0, 1 << 30):
+        big_tiles_knob(thr)
+        print(f"batched GMRES, big-system threshold {thr} tiles", flush=True)
+        out[thr] = run(sysl, cuda, 1e-9, 200)
+    for thr in (0, 1 << 30):
+        for i in range(len(sysl)):
+            assert np.array_equal(out[thr][0][i], out[64][0][i]), (thr, i)
+            assert out[thr][1][i]["iterations"] == out[64][1][i]["iterations"]
+            assert out[thr][1][i]["restarts"] == out[64][1][i]["restarts"]
+            assert np.array_equal(out[thr][1][i]["rel_res"], out[64][1][i]["rel_res"])
+    Xl, rl = out[64][0][len(sysl) - 1], out[64][1][len(sysl) - 1]
+    A = sp.csr_matrix((val, ci, rp), shape=(n, n))
+    B = sysl[-1][4]
+    assert rl["succeeded"]
+    assert np.max(np.linalg.norm(B - A @ Xl, axis=0) / np.linalg.norm(B, axis=0)) <= 1.5e-9
