@@ -119,7 +119,18 @@ int tg_csr_spmm(int32_t m, int32_t n, int64_t nnz, const void* rp, tg_dtype rp_t
  * tg_csr_plan_set_values again (the drop-in MatrixCSR does, since changing its
  * values drops its plan).  Extra device memory: 4 (8m + 1) + 4 nnz bytes for
  * the pattern and nnz * sizeof(value) per registered type.  Synchronises
- * `stream`.  tg_csr_plan_uses_split tells whether a call with `a` would use it. */
+ * `stream`.  tg_csr_plan_uses_split tells whether a call with `a` would use it.
+ *
+ * Long rows: a plan also finds the rows of more than spmm_long_row nonzeros
+ * (tg_tune_set knob, default 8192, 0 = never; read when the plan is made).  A
+ * single such row would keep one wave busy long after the rest of the matrix is
+ * done (one row of 1e6 nonzeros made an FP64 SpMV 150x slower).  tg_csr_spmm_plan
+ * then runs the row kernels with those rows seen as empty, adds each long row's
+ * nonzeros in segments of 4096 with a workgroup per segment, and combines each
+ * row's segment sums in order (deterministic; y = alpha z + beta y rounded as
+ * alpha z + (beta y)); slicing is not used for such a plan.  Extra device memory:
+ * m row ends of rp's type, 20 bytes per segment, and the segment sums (in the
+ * partial blocks).  tg_csr_plan_long_rows returns the number of long rows. */
 typedef struct tg_csr_plan_s* tg_csr_plan_t;
 int tg_csr_plan_create(int32_t m, int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type, const int32_t* ci,
                        tg_csr_plan_t* out, void* stream);
@@ -127,6 +138,7 @@ int tg_csr_plan_destroy(tg_csr_plan_t plan);
 int32_t tg_csr_plan_slices(tg_csr_plan_t plan, tg_dtype val_type, int32_t nrhs);
 int tg_csr_plan_set_values(tg_csr_plan_t plan, const void* a, tg_dtype val_type, void* stream);
 int32_t tg_csr_plan_uses_split(tg_csr_plan_t plan, const void* a, tg_dtype val_type, int32_t nrhs);
+int32_t tg_csr_plan_long_rows(tg_csr_plan_t plan);
 int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, double alpha, const void* X, int64_t ldX,
                      double beta, void* Y, int64_t ldY, int32_t nrhs, int32_t NRHS, void* stream);
 
@@ -147,6 +159,8 @@ int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, doubl
  *                  (default: the re-laid 8 slices for registered values and X of
  *                  2-64 MB, else 2 or 4 slices for X blocks of 6-32 MB, else none),
  *                  1 = never, 2 / 4 / 8 = always that many (nrhs <= 4).
+ *   "spmm_long_row": rows of more than this many nonzeros are split off by
+ *                  plans made afterwards (default 8192, 0 = never; see above).
  *   "spmm_split":  1 = registered values use the re-laid layout (default),
  *                  0 = never.
  *   "batch_tile_rows": rows per workgroup tile of systems added to a batched

@@ -37,6 +37,7 @@ SIGNATURES = {
     "tg_csr_plan_slices": (i32, [vp, C.c_int, i32]),
     "tg_csr_plan_set_values": (C.c_int, [vp, vp, C.c_int, vp]),
     "tg_csr_plan_uses_split": (i32, [vp, vp, C.c_int, i32]),
+    "tg_csr_plan_long_rows": (i32, [vp]),
     "tg_csr_spmm_plan": (C.c_int, [vp, vp, C.c_int, dbl, vp, i64, dbl, vp, i64, i32, i32, vp]),
     "tg_tune_set": (C.c_int, [C.c_char_p, i32]),
     "tg_tune_get": (i32, [C.c_char_p]),

@@ -9,6 +9,21 @@
 namespace tg {
 
 // Device CSR operand of the solvers (all pointers device-resident).
+// Long rows split off the per-row SpMM kernels (a plan's rows of more than
+// spmm_long_row nonzeros): the row kernels read each row's end from `rpe`, where a
+// long row is empty, and the long rows' nonzeros are added in segments of at most
+// kLongSeg by a workgroup each, then combined per row in segment order.
+constexpr int kLongSeg = 4096;
+struct LongRows {
+    int32_t nrows = 0, nseg = 0;
+    const void* rpe = nullptr;         // m row ends, the row pointers' type
+    const int32_t* rows = nullptr;     // [nrows] ascending
+    const int32_t* seg_ptr = nullptr;  // [nrows + 1] first segment of each long row
+    const int64_t* seg_lb = nullptr;   // [nseg] nonzero range of each segment
+    const int64_t* seg_le = nullptr;
+    void* part = nullptr;              // [nseg * nrhs] segment sums (sized by the caller)
+};
+
 struct CsrView {
     int32_t m = 0, n = 0;
     int64_t nnz = 0;
@@ -18,6 +33,7 @@ struct CsrView {
     const void* val = nullptr;  // null: binary matrix
     tg_dtype val_type = TG_F64;
     int64_t band = -1;          // max |column - row| if known (csr_band), else -1
+    const LongRows* longr = nullptr;  // split-off long rows (plan path), or none
 };
 
 // Bandwidth max |ci - row| of a device CSR (one pass over ci; synchronises s).

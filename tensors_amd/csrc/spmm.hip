@@ -100,7 +100,7 @@ __device__ __forceinline__ void block_rows(int m, const int32_t* __restrict__ jo
 // ---------------------------------------------------------------------------
 template <typename T, typename RP, int TEAM>
 __global__ __launch_bounds__(kBlock) void spmv_team_kernel(
-    int m, const RP* __restrict__ rp, const int32_t* __restrict__ ci, const T* __restrict__ a, T alpha,
+    int m, const RP* __restrict__ rp, const RP* __restrict__ rpe, const int32_t* __restrict__ ci, const T* __restrict__ a, T alpha,
     const T* __restrict__ x, int64_t ldx, T beta, T* __restrict__ y, int64_t ldy, int alpha_one,
     int beta_mode, const int32_t* __restrict__ job_ptr, int rows_per_block, const int* __restrict__ skip) {
     if (skip && *skip) return;
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(kBlock) void spmv_team_kernel(
     int r0, r1;
     block_rows(m, job_ptr, rows_per_block, r0, r1);
     for (int row = r0 + team; row < r1; row += TEAMS) {
-        const RP lb = rp[row], le = rp[row + 1];
+        const RP lb = rp[row], le = rpe[row];
         T z = T(0);
         if (a) {
             for (RP l = lb + lt; l < le; l += TEAM) {
@@ -227,7 +227,7 @@ __device__ __forceinline__ double lane_read_uniform(double v, int src) {
 
 template <typename T, typename RP, int G, int V>
 __global__ __launch_bounds__(kBlock) void spmm_wave_kernel(
-    int m, const RP* __restrict__ rp, const int32_t* __restrict__ ci, const T* __restrict__ a, T alpha,
+    int m, const RP* __restrict__ rp, const RP* __restrict__ rpe, const int32_t* __restrict__ ci, const T* __restrict__ a, T alpha,
     const T* __restrict__ X, int64_t ldX, T beta, T* __restrict__ Y, int64_t ldY, int nrhs, int alpha_one,
     int beta_mode, const int32_t* __restrict__ job_ptr, int rows_per_block, const int* __restrict__ skip) {
     if (skip && *skip) return;
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void spmm_wave_kernel(
     int r0, r1;
     block_rows(m, job_ptr, rows_per_block, r0, r1);
     for (int row = r0 + wave; row < r1; row += WAVES) {
-        const RP lb = rp[row], le = rp[row + 1];
+        const RP lb = rp[row], le = rpe[row];
         T* __restrict__ yrow = Y + (int64_t)row * ldY;
         for (int c0 = 0; c0 < nrhs; c0 += KT) {
             const int col = c0 + g * V;
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(kBlock) void spmm_wave_kernel(
 // nonzero order.
 template <typename T, typename RP, int G, int V>
 __global__ __launch_bounds__(kBlock) void spmm_rows_kernel(
-    int m, const RP* __restrict__ rp, const int32_t* __restrict__ ci, const T* __restrict__ a, T alpha,
+    int m, const RP* __restrict__ rp, const RP* __restrict__ rpe, const int32_t* __restrict__ ci, const T* __restrict__ a, T alpha,
     const T* __restrict__ X, int64_t ldX, T beta, T* __restrict__ Y, int64_t ldY, int nrhs, int alpha_one,
     int beta_mode, const int32_t* __restrict__ job_ptr, int rows_per_block, const int* __restrict__ skip) {
     if (skip && *skip) return;
@@ -319,7 +319,7 @@ __global__ __launch_bounds__(kBlock) void spmm_rows_kernel(
     for (int rbase = r0 + wave * S; rbase < r1; rbase += WAVES * S) {
         const int row = rbase + s;
         const bool rok = row < r1;
-        const RP lb = rok ? rp[row] : RP(0), le = rok ? rp[row + 1] : RP(0);
+        const RP lb = rok ? rp[row] : RP(0), le = rok ? rpe[row] : RP(0);
         const int len = (int)(le - lb);
         int wmax = len;  // longest row of the wave (uniform loop bound)
 #pragma unroll
@@ -432,6 +432,8 @@ static int g_spmm_slices = 0;
 // planned SpMM with values registered by tg_csr_plan_set_values: 1 = the re-laid
 // layout where it pays (default), 0 = never
 static int g_spmm_split = 1;
+// rows longer than this are split off by plans created afterwards (0: never)
+static int g_spmm_long_row = 8192;
 
 // Lanes per row of the nrhs == 1 kernels.  Team-per-row: the row length rounded up
 // (4..64).  Streaming (the products are formed with coalesced loads first; the team
@@ -483,11 +485,14 @@ static int stream_rows_per_block(int32_t m, int64_t nnz) {
 }
 
 template <typename T, typename RP>
-static int launch_spmv(int32_t m, int64_t nnz, const RP* rp, const int32_t* ci, const T* a, T alpha,
+static int launch_spmv(int32_t m, int64_t nnz, const RP* rp, const RP* rpe, const int32_t* ci, const T* a, T alpha,
                        const T* X, int64_t ldX, T beta, T* Y, int64_t ldY, int alpha_one, int beta_mode,
                        const int32_t* job_ptr, int32_t blocks, const int* skip, hipStream_t st) {
-    const int team = spmv_team_width(m, nnz, spmv_variant() == 0);
-    if (spmv_variant() == 0) {
+    // the streaming kernel walks whole nonzero ranges: with split-off long rows (row
+    // ends other than rp + 1) the team-per-row kernel, which reads each row's end
+    const bool stream = spmv_variant() == 0 && rpe == rp + 1;
+    const int team = spmv_team_width(m, nnz, stream);
+    if (stream) {
         const int rpb = stream_rows_per_block(m, nnz);
         const int grid = job_ptr ? blocks : (m + rpb - 1) / rpb;
         if (grid <= 0) return TG_OK;
@@ -516,7 +521,7 @@ static int launch_spmv(int32_t m, int64_t nnz, const RP* rp, const int32_t* ci, 
     if (grid <= 0) return TG_OK;
 #define TG_SPMV(TM)                                                                                     \
     case TM:                                                                                            \
-        hipLaunchKernelGGL((spmv_team_kernel<T, RP, TM>), dim3(grid), dim3(kBlock), 0, st, m, rp, ci, a, \
+        hipLaunchKernelGGL((spmv_team_kernel<T, RP, TM>), dim3(grid), dim3(kBlock), 0, st, m, rp, rpe, ci, a, \
                            alpha, X, ldX, beta, Y, ldY, alpha_one, beta_mode, job_ptr, rpb, skip);      \
         break;
     switch (tteam) {
@@ -533,7 +538,8 @@ static int launch_spmv(int32_t m, int64_t nnz, const RP* rp, const int32_t* ci, 
 }
 
 template <typename T, typename RP, int V>
-static int launch_spmm_v(int G, bool rows_mode, int32_t m, const RP* rp, const int32_t* ci, const T* a, T alpha,
+static int launch_spmm_v(int G, bool rows_mode, int32_t m, const RP* rp, const RP* rpe, const int32_t* ci, const T* a,
+                         T alpha,
                          const T* X, int64_t ldX, T beta, T* Y, int64_t ldY, int32_t nrhs, int alpha_one,
                          int beta_mode, const int32_t* job_ptr, int32_t blocks, const int* skip, hipStream_t st) {
     if (rows_mode && G < kWave) {
@@ -543,7 +549,7 @@ static int launch_spmm_v(int G, bool rows_mode, int32_t m, const RP* rp, const i
         if (grid <= 0) return TG_OK;
 #define TG_SPMMR(GG)                                                                                          \
     case GG:                                                                                                  \
-        hipLaunchKernelGGL((spmm_rows_kernel<T, RP, GG, V>), dim3(grid), dim3(kBlock), 0, st, m, rp, ci, a, alpha, \
+        hipLaunchKernelGGL((spmm_rows_kernel<T, RP, GG, V>), dim3(grid), dim3(kBlock), 0, st, m, rp, rpe, ci, a, alpha, \
                            X, ldX, beta, Y, ldY, nrhs, alpha_one, beta_mode, job_ptr, rpb, skip);             \
         break;
         switch (G) {
@@ -564,7 +570,7 @@ static int launch_spmm_v(int G, bool rows_mode, int32_t m, const RP* rp, const i
     if (grid <= 0) return TG_OK;
 #define TG_SPMM(GG)                                                                                      \
     case GG:                                                                                             \
-        hipLaunchKernelGGL((spmm_wave_kernel<T, RP, GG, V>), dim3(grid), dim3(kBlock), 0, st, m, rp, ci, a, \
+        hipLaunchKernelGGL((spmm_wave_kernel<T, RP, GG, V>), dim3(grid), dim3(kBlock), 0, st, m, rp, rpe, ci, a, \
                            alpha, X, ldX, beta, Y, ldY, nrhs, alpha_one, beta_mode, job_ptr, rpb, skip); \
         break;
     switch (G) {
@@ -585,13 +591,15 @@ static int launch_spmm_v(int G, bool rows_mode, int32_t m, const RP* rp, const i
 template <typename T, typename RP>
 static int spmm_typed(int32_t m, int64_t nnz, const RP* rp, const int32_t* ci, const T* a, double alpha_d,
                       const T* X, int64_t ldX, double beta_d, T* Y, int64_t ldY, int32_t nrhs,
-                      const int32_t* job_ptr, int32_t blocks, const int* skip, hipStream_t st) {
+                      const int32_t* job_ptr, int32_t blocks, const int* skip, hipStream_t st,
+                      const RP* rpe = nullptr) {
+    if (!rpe) rpe = rp + 1;  // row i ends at rpe[i]
     const T alpha = (T)alpha_d, beta = (T)beta_d;
     const int alpha_one = (alpha_d == 1.0);
     const int beta_mode = (beta_d == 0.0) ? kBetaZero : (beta_d == 1.0) ? kBetaOne : kBetaGeneric;
     if (nrhs == 1)
-        return launch_spmv<T, RP>(m, nnz, rp, ci, a, alpha, X, ldX, beta, Y, ldY, alpha_one, beta_mode, job_ptr,
-                                  blocks, skip, st);
+        return launch_spmv<T, RP>(m, nnz, rp, rpe, ci, a, alpha, X, ldX, beta, Y, ldY, alpha_one, beta_mode,
+                                  job_ptr, blocks, skip, st);
     // Vector width: 16-byte loads when X and Y rows allow it.
     const auto aligned = [&](int bytes) {
         return ((uintptr_t)X % bytes == 0) && ((uintptr_t)Y % bytes == 0) &&
@@ -612,14 +620,14 @@ static int spmm_typed(int32_t m, int64_t nnz, const RP* rp, const int32_t* ci, c
     const bool rows_mode = g_spmm_rows == 2 || (g_spmm_rows == 0 && avg <= 16.0);
     switch (V) {
         case 1:
-            return launch_spmm_v<T, RP, 1>(G, rows_mode, m, rp, ci, a, alpha, X, ldX, beta, Y, ldY, nrhs, alpha_one,
+            return launch_spmm_v<T, RP, 1>(G, rows_mode, m, rp, rpe, ci, a, alpha, X, ldX, beta, Y, ldY, nrhs, alpha_one,
                                            beta_mode, job_ptr, blocks, skip, st);
         case 2:
-            return launch_spmm_v<T, RP, 2>(G, rows_mode, m, rp, ci, a, alpha, X, ldX, beta, Y, ldY, nrhs, alpha_one,
+            return launch_spmm_v<T, RP, 2>(G, rows_mode, m, rp, rpe, ci, a, alpha, X, ldX, beta, Y, ldY, nrhs, alpha_one,
                                            beta_mode, job_ptr, blocks, skip, st);
         default:
             if constexpr (sizeof(T) == 4)
-                return launch_spmm_v<T, RP, 4>(G, rows_mode, m, rp, ci, a, alpha, X, ldX, beta, Y, ldY, nrhs, alpha_one,
+                return launch_spmm_v<T, RP, 4>(G, rows_mode, m, rp, rpe, ci, a, alpha, X, ldX, beta, Y, ldY, nrhs, alpha_one,
                                                beta_mode, job_ptr, blocks, skip, st);
             return fail(TG_ERR_INVALID, "internal: bad vector width");
     }
@@ -1001,6 +1009,11 @@ extern "C" int tg_tune_set(const char* key, int32_t value) {
         g_spmm_rows = value;
         return TG_OK;
     }
+    if (k == "spmm_long_row") {
+        if (value < 0) return fail(TG_ERR_INVALID, "tg_tune_set: spmm_long_row must be >= 0");
+        g_spmm_long_row = value;
+        return TG_OK;
+    }
     if (k == "spmm_split") {
         if (value != 0 && value != 1) return fail(TG_ERR_INVALID, "tg_tune_set: spmm_split must be 0 or 1");
         g_spmm_split = value;
@@ -1036,6 +1049,7 @@ extern "C" int32_t tg_tune_get(const char* key) {
     if (k == "spmm_group") return g_spmm_group;
     if (k == "spmm_slices") return g_spmm_slices;
     if (k == "spmm_split") return g_spmm_split;
+    if (k == "spmm_long_row") return g_spmm_long_row;
     if (k == "spmm_rows") return g_spmm_rows;
     if (k == "cg_fused" || k == "cg_panel") return cg_tune_get(key);
     if (k == "bsr_vector") return bsr_tune_get(key);
@@ -1133,6 +1147,79 @@ int64_t csr_band(const CsrView& A, hipStream_t s) {
     return b;
 }
 
+// ---------------------------------------------------------------------------
+// Long rows (plan path): a row of hundreds of thousands of nonzeros would keep one
+// wave or one workgroup busy long after the rest of the matrix is done.  The row
+// kernels see such rows as empty (row ends from LongRows::rpe) and apply the
+// empty-row rule; these two kernels then add the rows' products.
+// ---------------------------------------------------------------------------
+
+// One workgroup per segment of a long row: part[seg * k + c] = sum over the segment's
+// nonzeros of a_l X[ci_l, c].  Columns in tiles of CT lanes; the segment's nonzeros are
+// dealt over the kBlock / CT row lanes and summed by a fixed tree: deterministic.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void lr_segment_kernel(const int64_t* __restrict__ seg_lb,
+                                                            const int64_t* __restrict__ seg_le,
+                                                            const int32_t* __restrict__ ci, const T* __restrict__ a,
+                                                            const T* __restrict__ X, int64_t ldX, int k, int CT,
+                                                            T* __restrict__ part) {
+    __shared__ T sh[kBlock];
+    const int sg = blockIdx.x;
+    const int64_t lb = seg_lb[sg], le = seg_le[sg];
+    const int lc = threadIdx.x % CT, r = threadIdx.x / CT, R = kBlock / CT;
+    for (int c0 = 0; c0 < k; c0 += CT) {
+        const int c = c0 + lc;
+        T acc = T(0);
+        if (c < k)
+            for (int64_t l = lb + r; l < le; l += R) acc += (a ? a[l] : T(1)) * X[(int64_t)ci[l] * ldX + c];
+        sh[threadIdx.x] = acc;
+        __syncthreads();
+        for (int h = R / 2; h > 0; h >>= 1) {
+            if (r < h) sh[threadIdx.x] += sh[threadIdx.x + h * CT];
+            __syncthreads();
+        }
+        if (r == 0 && c < k) part[(int64_t)sg * k + c] = sh[threadIdx.x];
+        __syncthreads();
+    }
+}
+
+// One workgroup per long row: z = the row's segment sums in segment order, then
+// y = alpha z + y over the value the row kernels left there for an empty row (0 for
+// beta = 0, y for beta = 1, beta y otherwise).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void lr_combine_kernel(const int32_t* __restrict__ rows,
+                                                            const int32_t* __restrict__ seg_ptr,
+                                                            const T* __restrict__ part, int k, T alpha, int alpha_one,
+                                                            int beta_mode, T* __restrict__ Y, int64_t ldY) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x;
+    const int64_t row = rows[i];
+    const int s0 = seg_ptr[i], s1 = seg_ptr[i + 1];
+    for (int c = threadIdx.x; c < k; c += kBlock) {
+        T z = T(0);
+        for (int sg = s0; sg < s1; ++sg) z += part[(int64_t)sg * k + c];
+        const T az = alpha_one ? z : alpha * z;
+        T* y = Y + row * ldY + c;
+        *y = beta_mode == kBetaZero ? az : az + *y;
+    }
+}
+
+template <typename T>
+static int long_rows_apply(const CsrView& A, double alpha_d, const T* X, int64_t ldX, double beta_d, T* Y,
+                           int64_t ldY, int32_t nrhs, hipStream_t st) {
+    const LongRows& L = *A.longr;
+    if (L.nrows == 0) return TG_OK;
+    int CT = 1;
+    while (CT < nrhs && CT < 64) CT *= 2;
+    const int beta_mode = (beta_d == 0.0) ? kBetaZero : (beta_d == 1.0) ? kBetaOne : kBetaGeneric;
+    hipLaunchKernelGGL((lr_segment_kernel<T>), dim3(L.nseg), dim3(kBlock), 0, st, L.seg_lb, L.seg_le, A.ci,
+                       (const T*)A.val, X, ldX, nrhs, CT, (T*)L.part);
+    hipLaunchKernelGGL((lr_combine_kernel<T>), dim3(L.nrows), dim3(kBlock), 0, st, L.rows, L.seg_ptr,
+                       (const T*)L.part, nrhs, (T)alpha_d, alpha_d == 1.0 ? 1 : 0, beta_mode, Y, ldY);
+    TG_LAUNCH_CHECK();
+    return TG_OK;
+}
+
 bool spmm_may_slab(const CsrView& A, int32_t nrhs) {
     const int64_t vs = A.val_type == TG_F64 ? 8 : 4;
     return (int64_t)A.n * nrhs * vs > kXTileBytes && nrhs > 8;
@@ -1165,6 +1252,8 @@ int spmm_dispatch(const CsrView& A, double alpha, const void* X, int64_t ldX, do
     if (!A.rp || !X || ldX < nrhs) return fail(TG_ERR_INVALID, "spmm: rp/X null or ldX < nrhs");
     if (A.nnz > 0 && !A.ci) return fail(TG_ERR_INVALID, "spmm: ci is null");
     const size_t vs = f64 ? 8 : 4;
+    const bool long_rows = A.longr && A.longr->nrows > 0 && !job_ptr && !skip;
+    const void* rpe = long_rows ? A.longr->rpe : nullptr;
     int32_t kt = nrhs;
     if (spmm_uses_slabs(A, nrhs)) {
         kt = 8;
@@ -1178,21 +1267,29 @@ int spmm_dispatch(const CsrView& A, double alpha, const void* X, int64_t ldX, do
             double* Yc = (double*)Y + c0;
             if (A.rp_type == TG_I32)
                 rc = spmm_typed<double, int32_t>(m, A.nnz, (const int32_t*)A.rp, A.ci, (const double*)A.val, alpha, Xc,
-                                                 ldX, beta, Yc, ldY, kc, job_ptr, blocks, skip, st);
+                                                 ldX, beta, Yc, ldY, kc, job_ptr, blocks, skip, st,
+                                                 (const int32_t*)rpe);
             else
                 rc = spmm_typed<double, int64_t>(m, A.nnz, (const int64_t*)A.rp, A.ci, (const double*)A.val, alpha, Xc,
-                                                 ldX, beta, Yc, ldY, kc, job_ptr, blocks, skip, st);
+                                                 ldX, beta, Yc, ldY, kc, job_ptr, blocks, skip, st,
+                                                 (const int64_t*)rpe);
         } else {
             const float* Xc = (const float*)X + c0;
             float* Yc = (float*)Y + c0;
             if (A.rp_type == TG_I32)
                 rc = spmm_typed<float, int32_t>(m, A.nnz, (const int32_t*)A.rp, A.ci, (const float*)A.val, alpha, Xc,
-                                                ldX, beta, Yc, ldY, kc, job_ptr, blocks, skip, st);
+                                                ldX, beta, Yc, ldY, kc, job_ptr, blocks, skip, st,
+                                                (const int32_t*)rpe);
             else
                 rc = spmm_typed<float, int64_t>(m, A.nnz, (const int64_t*)A.rp, A.ci, (const float*)A.val, alpha, Xc,
-                                                ldX, beta, Yc, ldY, kc, job_ptr, blocks, skip, st);
+                                                ldX, beta, Yc, ldY, kc, job_ptr, blocks, skip, st,
+                                                (const int64_t*)rpe);
         }
         if (rc) return rc;
+    }
+    if (long_rows) {
+        if (f64) return long_rows_apply<double>(A, alpha, (const double*)X, ldX, beta, (double*)Y, ldY, nrhs, st);
+        return long_rows_apply<float>(A, alpha, (const float*)X, ldX, beta, (float*)Y, ldY, nrhs, st);
     }
     return TG_OK;
 }
@@ -1224,6 +1321,14 @@ struct tg_csr_plan_s {
     const void* sval_src[2] = {nullptr, nullptr};
     void* part = nullptr;     // partial blocks of the sliced path (grown on demand)
     size_t part_bytes = 0;
+    // long rows (more than spmm_long_row nonzeros when the plan was made): row ends
+    // with those rows empty, the rows (ascending), their segments' first index and
+    // the segments' nonzero ranges (lb then le); the segment sums use `part`
+    void* rpe = nullptr;
+    int32_t* lrows = nullptr;
+    int32_t* lseg_ptr = nullptr;
+    int64_t* lseg = nullptr;
+    int32_t nlong = 0, nseg = 0;
     // The partial blocks are shared by every sliced call on the plan: a call on another
     // stream than the previous one first waits for it (event `used`), and host threads
     // take turns.
@@ -1231,6 +1336,93 @@ struct tg_csr_plan_s {
     hipStream_t used_on = nullptr;
     std::mutex mu;
 };
+
+// Row ends with the long rows (more than `thr` nonzeros) empty, and the long rows'
+// ids appended to `list` (in no particular order; the host sorts them).
+template <typename RP>
+__global__ __launch_bounds__(kBlock) void long_rows_kernel(int m, const RP* __restrict__ rp, int64_t thr,
+                                                           RP* __restrict__ rpe, int32_t* __restrict__ list,
+                                                           int* __restrict__ count, int cap) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < m; i += gridDim.x * kBlock) {
+        const RP lb = rp[i], le = rp[i + 1];
+        const bool lng = (int64_t)(le - lb) > thr;
+        rpe[i] = lng ? lb : le;
+        if (lng) {
+            const int p = atomicAdd(count, 1);
+            if (p < cap) list[p] = i;
+        }
+    }
+}
+
+extern "C" int tg_csr_plan_destroy(tg_csr_plan_t plan);
+
+// Finds the plan's long rows and lays out their segments (once, at plan creation).
+static int plan_long_rows(tg_csr_plan_t P, hipStream_t st) {
+    const int64_t thr = g_spmm_long_row;
+    if (thr <= 0 || P->m == 0 || P->nnz <= thr) return TG_OK;
+    const size_t rs = P->rp_type == TG_I32 ? 4 : 8;
+    const int cap = (int)std::min<int64_t>(P->nnz / (thr + 1) + 1, P->m);
+    int32_t* list = nullptr;
+    int* count = nullptr;
+    TG_HIP(hipMallocAsync(&P->rpe, (size_t)P->m * rs, st));
+    TG_HIP(hipMallocAsync((void**)&list, (size_t)cap * sizeof(int32_t), st));
+    TG_HIP(hipMallocAsync((void**)&count, sizeof(int), st));
+    TG_HIP(hipMemsetAsync(count, 0, sizeof(int), st));
+    const int grid = (int)std::min<int64_t>((P->m + kBlock - 1) / kBlock, 65536);
+    if (P->rp_type == TG_I32)
+        hipLaunchKernelGGL((long_rows_kernel<int32_t>), dim3(grid), dim3(kBlock), 0, st, P->m, (const int32_t*)P->rp,
+                           thr, (int32_t*)P->rpe, list, count, cap);
+    else
+        hipLaunchKernelGGL((long_rows_kernel<int64_t>), dim3(grid), dim3(kBlock), 0, st, P->m, (const int64_t*)P->rp,
+                           thr, (int64_t*)P->rpe, list, count, cap);
+    TG_LAUNCH_CHECK();
+    int nl = 0;
+    TG_HIP(hipMemcpyAsync(&nl, count, sizeof(int), hipMemcpyDeviceToHost, st));
+    TG_HIP(hipStreamSynchronize(st));
+    nl = std::min(nl, cap);
+    std::vector<int32_t> rows(nl);
+    if (nl > 0) TG_HIP(hipMemcpyAsync(rows.data(), list, (size_t)nl * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    TG_HIP(hipFreeAsync(list, st));
+    TG_HIP(hipFreeAsync(count, st));
+    if (nl == 0) {
+        TG_HIP(hipFreeAsync(P->rpe, st));
+        P->rpe = nullptr;
+        TG_HIP(hipStreamSynchronize(st));
+        return TG_OK;
+    }
+    std::vector<int64_t> rph((size_t)P->m + 1);
+    if (P->rp_type == TG_I32) {
+        std::vector<int32_t> r32((size_t)P->m + 1);
+        TG_HIP(hipMemcpyAsync(r32.data(), P->rp, r32.size() * 4, hipMemcpyDeviceToHost, st));
+        TG_HIP(hipStreamSynchronize(st));
+        for (size_t i = 0; i < r32.size(); ++i) rph[i] = r32[i];
+    } else {
+        TG_HIP(hipMemcpyAsync(rph.data(), P->rp, rph.size() * 8, hipMemcpyDeviceToHost, st));
+        TG_HIP(hipStreamSynchronize(st));
+    }
+    std::sort(rows.begin(), rows.end());
+    std::vector<int32_t> sptr(1, 0);
+    std::vector<int64_t> lb, le;
+    for (int32_t r : rows) {
+        for (int64_t l = rph[r]; l < rph[r + 1]; l += kLongSeg) {
+            lb.push_back(l);
+            le.push_back(std::min<int64_t>(rph[r + 1], l + kLongSeg));
+        }
+        sptr.push_back((int32_t)lb.size());
+    }
+    const size_t ns = lb.size();
+    TG_HIP(hipMallocAsync((void**)&P->lrows, rows.size() * sizeof(int32_t), st));
+    TG_HIP(hipMallocAsync((void**)&P->lseg_ptr, sptr.size() * sizeof(int32_t), st));
+    TG_HIP(hipMallocAsync((void**)&P->lseg, 2 * ns * sizeof(int64_t), st));
+    TG_HIP(hipMemcpyAsync(P->lrows, rows.data(), rows.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    TG_HIP(hipMemcpyAsync(P->lseg_ptr, sptr.data(), sptr.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    TG_HIP(hipMemcpyAsync(P->lseg, lb.data(), ns * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    TG_HIP(hipMemcpyAsync(P->lseg + ns, le.data(), ns * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    TG_HIP(hipStreamSynchronize(st));  // the host vectors go out of scope
+    P->nlong = (int32_t)rows.size();
+    P->nseg = (int32_t)ns;
+    return TG_OK;
+}
 
 extern "C" int tg_csr_plan_create(int32_t m, int32_t n, int64_t nnz, const void* rp, tg_dtype rp_type,
                                   const int32_t* ci, tg_csr_plan_t* out, void* stream) {
@@ -1278,6 +1470,11 @@ extern "C" int tg_csr_plan_create(int32_t m, int32_t n, int64_t nnz, const void*
         V.rp_type = rp_type;
         V.ci = ci;
         P->band = csr_band(V, st);
+        const int rc = plan_long_rows(P, st);
+        if (rc) {
+            (void)tg_csr_plan_destroy(P);
+            return rc;
+        }
     }
     *out = P;
     return TG_OK;
@@ -1291,6 +1488,10 @@ extern "C" int tg_csr_plan_destroy(tg_csr_plan_t plan) {
         if (v) (void)hipFree(v);
     if (plan->cuts) (void)hipFree(plan->cuts);
     if (plan->part) (void)hipFree(plan->part);
+    if (plan->rpe) (void)hipFree(plan->rpe);
+    if (plan->lrows) (void)hipFree(plan->lrows);
+    if (plan->lseg_ptr) (void)hipFree(plan->lseg_ptr);
+    if (plan->lseg) (void)hipFree(plan->lseg);
     if (plan->used) (void)hipEventDestroy(plan->used);
     delete plan;
     return TG_OK;
@@ -1384,6 +1585,8 @@ extern "C" int32_t tg_csr_plan_slices(tg_csr_plan_t plan, tg_dtype val_type, int
     return plan_slices(plan, val_type, nrhs, nullptr, &split);
 }
 
+extern "C" int32_t tg_csr_plan_long_rows(tg_csr_plan_t plan) { return plan ? plan->nlong : 0; }
+
 extern "C" int32_t tg_csr_plan_uses_split(tg_csr_plan_t plan, const void* a, tg_dtype val_type, int32_t nrhs) {
     bool split = false;
     if (!plan || (val_type != TG_F64 && val_type != TG_F32)) return 0;
@@ -1408,6 +1611,45 @@ extern "C" int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_
     A.val_type = val_type;
     A.band = plan->band;
     hipStream_t st = as_stream(stream);
+    if (plan->nlong > 0 && (val_type == TG_F64 || val_type == TG_F32) && alpha != 0.0 && X && Y && X != Y &&
+        ldX >= nrhs && ldY >= nrhs && nrhs > 0) {
+        // long rows split off: the row kernels on the rest, then the rows' segments
+        const size_t need = (size_t)plan->nseg * nrhs * (val_type == TG_F64 ? 8 : 4);
+        std::lock_guard<std::mutex> lk(plan->mu);
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        TG_HIP(hipStreamIsCapturing(st, &cap));
+        const bool track = cap == hipStreamCaptureStatusNone;
+        if (track && plan->used && plan->used_on != st) TG_HIP(hipStreamWaitEvent(st, plan->used, 0));
+        if (plan->part_bytes < need && !track)
+            return fail(TG_ERR_WORKSPACE,
+                        "tg_csr_spmm_plan: the plan's partial blocks (%zu bytes) must be sized by one call of this shape "
+                        "outside stream capture before the call is captured",
+                        need);
+        if (plan->part_bytes < need) {
+            if (plan->part) TG_HIP(hipFreeAsync(plan->part, st));
+            plan->part = nullptr;
+            plan->part_bytes = 0;
+            TG_HIP(hipMallocAsync(&plan->part, need, st));
+            plan->part_bytes = need;
+        }
+        LongRows L;
+        L.nrows = plan->nlong;
+        L.nseg = plan->nseg;
+        L.rpe = plan->rpe;
+        L.rows = plan->lrows;
+        L.seg_ptr = plan->lseg_ptr;
+        L.seg_lb = plan->lseg;
+        L.seg_le = plan->lseg + plan->nseg;
+        L.part = plan->part;
+        A.longr = &L;
+        const int rc = spmm_dispatch(A, alpha, X, ldX, beta, Y, ldY, nrhs, nullptr, 0, nullptr, st);
+        if (track) {
+            if (!plan->used) TG_HIP(hipEventCreateWithFlags(&plan->used, hipEventDisableTiming));
+            TG_HIP(hipEventRecord(plan->used, st));
+            plan->used_on = st;
+        }
+        return rc;
+    }
     bool split = false;
     const int S = (val_type == TG_F64 || val_type == TG_F32) ? plan_slices(plan, val_type, nrhs, a, &split) : 0;
     // the plain path covers every special case (alpha = 0, aliasing, bad arguments)

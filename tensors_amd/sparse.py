@@ -75,6 +75,10 @@ class SlicePlan:
         if nrhs <= 4 and (2 << 20) < xbytes <= (64 << 20) and vals.get(a.dtype) is not a:
             self.set_values(a)
 
+    def long_rows(self):
+        """Rows split off the row kernels (more than the spmm_long_row knob's nonzeros)."""
+        return int(capi.lib().tg_csr_plan_long_rows(self.h))
+
     def uses_split(self, a, nrhs):
         return bool(capi.lib().tg_csr_plan_uses_split(self.h, a.data_ptr(), _TORCH_TO_TG[a.dtype], nrhs))
 
