@@ -100,6 +100,9 @@ def parse():
     ap.add_argument("--line-rate", action="store_true",
                     help="add the line-rate probe: the FP64 16-column SpMM of the suite's row lengths with the "
                          "columns limited to [0, W) for X of 3.8-122 MB")
+    ap.add_argument("--long-rows", action="store_true",
+                    help="add the long-row probe: planned FP64 SpMM on 12 M nonzeros spread evenly or with 2 rows of "
+                         "1e6 nonzeros, the long-row split on and off")
     ap.add_argument("--no-bsr", action="store_true", help="skip the block-CSR SpMM section")
     ap.add_argument("--bsr-vector", type=int, default=-1,
                     help="block-CSR lane mapping (tg_tune_set bsr_vector: 1 vector, 0 scalar; -1 default)")
@@ -689,6 +692,57 @@ def line_rate_section(dev):
                         "16 right-hand sides (one 128 B line of X per nonzero), tg_csr_spmm", "cases": out}
 
 
+def long_rows_section(dev):
+    """Planned FP64 SpMM (MatrixCSR.Dot's path) on 1e6 x 1e6 matrices of about 12 M
+    nonzeros: row lengths Poisson(12) clipped to [1, 64], or Poisson(10) plus 2 rows of
+    1e6 nonzeros; columns uniform; nrhs 1, 4, 16; the plan's long-row split (rows of more
+    than 8192 nonzeros) on and off (knob spmm_long_row)."""
+    import torch
+
+    from tensors_amd import _capi as capi
+    from tensors_amd.sparse import SlicePlan
+
+    s = torch.cuda.current_stream()
+    rng = np.random.default_rng(SEED + 11)
+    before = int(capi.lib().tg_tune_get(b"spmm_long_row"))
+    out = []
+    try:
+        for name, n_long, base in (("uniform", 0, 12.0), ("2 rows of 1e6", 2, 10.0)):
+            lens = rng.poisson(base, M).clip(1, 64)
+            lens[rng.choice(M, n_long, replace=False)] = 1_000_000
+            rp = np.r_[0, np.cumsum(lens)].astype(np.int32)
+            nnz = int(rp[-1])
+            rpd = torch.from_numpy(rp).to(dev)
+            cid = torch.from_numpy(rng.integers(0, N, nnz).astype(np.int32)).to(dev)
+            vd = torch.from_numpy(rng.normal(size=nnz)).to(dev)
+            for split in (1, 0):
+                capi.call("tg_tune_set", b"spmm_long_row", 8192 if split else 0)
+                plan = SlicePlan(M, N, nnz, rpd, cid)
+                for k in (1, 4, 16):
+                    X = torch.rand(N, k, dtype=torch.float64, device=dev)
+                    Y = torch.empty(M, k, dtype=torch.float64, device=dev)
+                    plan.spmm(vd, 1.0, X, k, 0.0, Y, k, k)
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    torch.cuda.synchronize()
+                    e0.record(s)
+                    reps = 5 if (split or n_long == 0) else 1
+                    for _ in range(reps):
+                        plan.spmm(vd, 1.0, X, k, 0.0, Y, k, k)
+                    e1.record(s)
+                    torch.cuda.synchronize()
+                    out.append({"matrix": name, "nnz": nnz, "split": bool(split), "long_rows": plan.long_rows(),
+                                "nrhs": k, "us": round(e0.elapsed_time(e1) * 1e3 / reps, 1)})
+                    log(f"long rows: {out[-1]}")
+                    del X, Y
+                del plan
+            del rpd, cid, vd
+            torch.cuda.empty_cache()
+    finally:
+        capi.call("tg_tune_set", b"spmm_long_row", before)
+    return {"workload": "1e6 x 1e6, ~12 M nonzeros, uniform columns, FP64, planned SpMM (tg_csr_spmm_plan)",
+            "cases": out}
+
+
 def bsr_section(args, dev, read_gbps):
     """Block-CSR SpMM (KernelMatrixCSR<DenseBlockKernel_Tiny>::Dot, tg_bsr_spmm): R x R FP64 blocks
     on the 3D 7-point grid pattern (block rows g^3), X / Y row-major chunks, alpha = 1, beta = 0.
@@ -1001,6 +1055,9 @@ def main():
     if args.line_rate:
         log("line-rate probe")
         out["line_rate"] = line_rate_section(dev)
+    if args.long_rows:
+        log("long-row probe")
+        out["long_rows"] = long_rows_section(dev)
     if not args.no_bsr:
         log("block-CSR section")
         out["bsr"] = bsr_section(args, dev, read_gbps)
