@@ -66,6 +66,9 @@ def parse():
     ap.add_argument("--cg-fused", type=int, default=-1,
                     help="CG iteration form (tg_tune_set cg_fused): 0 four kernels, 1 p.u in the SpMM, "
                          "2 fused direction, -1 library default")
+    ap.add_argument("--gmres-delayed", type=int, default=-1,
+                    help="GMRES step (tg_tune_set gmres_delayed): 1 second Gram-Schmidt pass delayed to the next "
+                         "step (two basis passes), 0 plain CGS2 (three)")
     ap.add_argument("--cg-panel", type=int, default=-1,
                     help="rows of the CG SpMM (tg_tune_set cg_panel): 1 panel order where the band allows, "
                          "0 contiguous chunks, -1 library default")
@@ -821,11 +824,14 @@ def gmres_section(args, dev):
     microseconds per Arnoldi step (restarts x 30 + the last cycle's iterations)."""
     import torch
 
+    from tensors_amd import _capi as capi
     from tensors_amd import generators as gen
     from tensors_amd.batch import BatchGMRES
     from tensors_amd.solvers import GMRES, gmres_solve_mixed
     from tensors_amd.sparse import MatrixCSR
 
+    if args.gmres_delayed >= 0:
+        capi.call("tg_tune_set", b"gmres_delayed", args.gmres_delayed)
     res = []
     for g, pe in ((1000, 10.0), (300, 30.0), (100, 3.0)):
         rp, ci, val = gen.convection_diffusion_2d(g, pe)

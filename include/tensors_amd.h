@@ -202,6 +202,12 @@ int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, doubl
  *   "gmres_graph": tg_gmres_solve with device operators, 1 = every restart cycle
  *                  after the first runs as one captured hipGraph (default), 0 =
  *                  eager launches.  Bitwise the same results.
+ *   "gmres_delayed": Gram-Schmidt of tg_gmres_solve (and of the FP32 cycles of
+ *                  tg_gmres_solve_mixed), 1 = CGS2 with each vector's second pass
+ *                  delayed to the next step and fused with its projection (default
+ *                  for restart lengths up to 32 and (2m + 2) * nrhs <= 2048: two
+ *                  passes over the basis and five launches per step), 0 = CGS2
+ *                  (three passes, eight launches).  Results agree to rounding.
  *   "bsr_vector":  tg_bsr_spmm with row-major X / Y chunks, 1 = a lane per
  *                  (block row, 1-4 right-hand sides) with vector loads (default),
  *                  0 = a lane per output entry (the mapping of the other

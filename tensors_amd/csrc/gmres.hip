@@ -16,7 +16,8 @@
 // vectors live in the vector type T (FP64 or FP32); the Hessenberg matrix,
 // rotations, beta, y and all reductions are FP64.
 //
-// Kernels of Arnoldi step j (q = Q_{j+1}), eight launches:
+// Kernels of Arnoldi step j (q = Q_{j+1}) in the plain CGS2 form (knob
+// gmres_delayed = 0, and where the delayed form does not fit), eight launches:
 //   spmm (left: z = A Q_j; right: z = P Q_j then q = A z)
 //   dots1:  q = P z (left), h1_i = Q_i.q             i <= j
 //   total:  h1 from the workgroup partials, H[.][j] += h1
@@ -40,6 +41,7 @@ constexpr int kGmresMaxCols = 32;  // nrhs limit (LDS of the multi-dot)
 constexpr int kGmresMaxM = 63;     // restart length limit (m + 1 basis vectors <= 64)
 constexpr int kGB = 512;           // workgroups of the vector kernels (at most; fewer for small n)
 static int g_gmres_graph = 1;      // 1: cycles after the first run as one captured hipGraph
+static int g_gmres_delayed = 1;    // 1: CGS2 with the second pass delayed to the next step (two basis passes per step)
 
 struct GmresHdr {
     int iter;
@@ -52,7 +54,7 @@ struct GmresHdr {
 // Small FP64 arrays (offsets in doubles from the start of the small region).
 struct GmresLayout {
     int m, k;
-    size_t H, cs, sn, beta, y, TOL, bnorm, rnorm, h, inv, total;
+    size_t H, cs, sn, beta, y, TOL, bnorm, rnorm, h, inv, Hraw, dA, dE, dsc, total;
     GmresLayout(int m_, int k_) : m(m_), k(k_) {
         size_t o = 0;
         H = o;
@@ -75,6 +77,14 @@ struct GmresLayout {
         o += (size_t)(m + 1) * k;
         inv = o;
         o += k;
+        Hraw = o;  // delayed form: H before the Givens rotations (the Arnoldi relation)
+        o += (size_t)(m + 1) * m * k;
+        dA = o;  // delayed form: coefficients of the update pass
+        o += (size_t)(m + 1) * k;
+        dE = o;
+        o += (size_t)(m + 1) * k;
+        dsc = o;  // per column: 1/rho, t_j/rho^2
+        o += (size_t)2 * k;
         total = o;
     }
 };
@@ -227,6 +237,270 @@ __global__ __launch_bounds__(kBlock) void gm_update_kernel(int n, int k, int kpa
     multi_partial<NQMAX>(v, norm_only ? 1 : nq, k, kpad, partial);
 }
 
+
+// ---------------------------------------------------------------------------
+// Delayed second pass (knob gmres_delayed, the default where it fits): classical
+// Gram-Schmidt applied twice, the second pass of step j's new vector done in step
+// j + 1 together with that step's projection (the delayed CGS2 of Swirydowicz et al.,
+// "Low synchronization Gram-Schmidt and GMRES algorithms", 2020).  Step j starts with
+// q_0..q_{j-1} final and u in slot j: the new vector of step j - 1 after its first
+// pass (for j = 0, q_0 itself).
+//   spmm:    v = A u into slot j + 1 (left Jacobi: z = A u, v = P z in the dots pass)
+//   ddots:   per column a_i = q_i.u, b_i = q_i.v (i < j), alpha = u.u, beta = u.v
+//   total:   the 2j + 2 totals
+//   dscalar: rho = sqrt(alpha - |a|^2) = |u - Q a| (Q orthonormal); column j - 1 of H
+//            gets += a (its second pass) and H[j][j-1] = rho, then its Givens
+//            rotation, beta and the stop test (iter = j).  Unless the cycle stops:
+//            with t = Q_{<=j}^T v (t_i = b_i, t_j = (beta - a.b) / rho) and
+//            A q_j = (v - Q Hraw a) / rho (the Arnoldi relation of the finished
+//            columns), column j of H gets its first pass g = (t - Hraw a) / rho and
+//            the next vector is u' = A q_j - Q g = (v - Q t) / rho.
+//   dupdate: q_j = (u - Q a) / rho and u' in one pass over the basis.
+// A cycle of m steps ends with a finishing step (ddots, total, dscalar, dupdate
+// without a new vector) that completes column m - 1 and q_m.  In exact arithmetic H
+// and the basis are those of CGS2; the basis is read twice per step instead of three
+// times and a step is five launches instead of eight.
+
+// Workgroup partials of the 2j + 2 values [a_0..a_{j-1}, b_0..b_{j-1}, alpha, beta]
+// held in v[i] (a_i), v[NQMAX + i] (b_i), v[2 NQMAX], v[2 NQMAX + 1].
+template <int NQMAX>
+__device__ __forceinline__ void dd_partial(double (&v)[2 * NQMAX + 2], int j, int k, int kpad,
+                                           double* __restrict__ partial) {
+    extern __shared__ double gm_sh[];
+    const int nv = 2 * j + 2;
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+#pragma unroll
+    for (int r = 0; r < 2 * NQMAX + 2; ++r) {
+        const int t = r < NQMAX ? (r < j ? r : -1)
+                                : r < 2 * NQMAX ? (r - NQMAX < j ? j + r - NQMAX : -1) : 2 * j + (r - 2 * NQMAX);
+        if (t >= 0) {
+            double x = v[r];
+            for (int off = kpad; off < kWave; off <<= 1) x += shfl_xor(x, off);
+            if (lane < kpad && lane < k) gm_sh[(wave * nv + t) * k + lane] = x;
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < nv * k; t += kBlock) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < kBlock / kWave; ++w) s += gm_sh[w * nv * k + t];
+        partial[(int64_t)blockIdx.x * nv * k + t] = s;
+    }
+}
+
+template <typename T, int NQMAX>
+__global__ __launch_bounds__(kBlock) void gm_ddots_kernel(int n, int k, int kpad, int j, int finish,
+                                                          const T* __restrict__ Q, int64_t qstride,
+                                                          const T* __restrict__ z, const T* __restrict__ d,
+                                                          T* __restrict__ vslot, double* __restrict__ partial,
+                                                          const GmresHdr* __restrict__ hd) {
+    if (hd->done) return;
+    int i0, i1;
+    grow_range(n, i0, i1);
+    const int c = threadIdx.x % kpad, ro = threadIdx.x / kpad, rows = kBlock / kpad;
+    double acc[2 * NQMAX + 2];
+#pragma unroll
+    for (int r = 0; r < 2 * NQMAX + 2; ++r) acc[r] = 0.0;
+    const T* U = Q + (int64_t)j * qstride;
+    if (c < k)
+        for (int r = i0 + ro; r < i1; r += rows) {
+            const int64_t e = (int64_t)r * k + c;
+            T qi[NQMAX];
+#pragma unroll
+            for (int i = 0; i < NQMAX; ++i) qi[i] = Q[(int64_t)max(0, min(i, j - 1)) * qstride + e];  // loads first
+            const T uv = U[e];
+            T vv;
+            if (finish) {
+                vv = uv;
+            } else if (d) {
+                vv = d[r] * z[e];
+                vslot[e] = vv;
+            } else {
+                vv = vslot[e];
+            }
+#pragma unroll
+            for (int i = 0; i < NQMAX; ++i)
+                if (i < j) {
+                    acc[i] += (double)qi[i] * (double)uv;
+                    acc[NQMAX + i] += (double)qi[i] * (double)vv;
+                }
+            acc[2 * NQMAX] += (double)uv * (double)uv;
+            acc[2 * NQMAX + 1] += (double)uv * (double)vv;
+        }
+    dd_partial<NQMAX>(acc, j, k, kpad, partial);
+}
+
+// One workgroup: the scalar work of delayed step j (finish: the cycle's last column).
+// The totals, column j - 1 of H and the rotations are staged in LDS; the per-column
+// serial part (|a|^2, a.b, the Givens chain) reads only LDS, everything else is
+// spread over (row, column) pairs.
+constexpr int kDdMaxM = 32;  // restart length limit of the delayed form
+__global__ __launch_bounds__(kBlock) void gm_dscalar_kernel(int j, int m, int k, int finish,
+                                                            const double* __restrict__ tot, double* __restrict__ small,
+                                                            GmresLayout L, GmresHdr* __restrict__ hd) {
+    if (hd->done) return;
+    __shared__ double sv[(2 * kDdMaxM + 2) * kGmresMaxCols];    // totals [value][column]
+    __shared__ double col[(kDdMaxM + 1) * kGmresMaxCols];       // column j - 1, then the new column j
+    __shared__ double rot[2 * kDdMaxM * kGmresMaxCols];         // cs, sn of rotations 0..j-2
+    __shared__ double sc[4 * kGmresMaxCols];                    // rho, 1/rho, a.b, t_j / rho^2
+    __shared__ int conv;
+    const int nv = 2 * j + 2, jj = j - 1;
+    double* H = small + L.H;
+    double* Hr = small + L.Hraw;
+    auto at = [&](int r, int cc) -> int64_t { return (int64_t)r * m * k + cc; };  // + col * k
+    for (int t = threadIdx.x; t < nv * k; t += kBlock) sv[t] = tot[t];
+    if (j > 0) {
+        for (int t = threadIdx.x; t < j * k; t += kBlock) {  // column jj (first pass) rows 0..jj
+            const int i = t / k, c = t - i * k;
+            col[t] = Hr[at(i, c) + (int64_t)jj * k];
+        }
+        for (int t = threadIdx.x; t < jj * k; t += kBlock) {
+            rot[t] = small[L.cs + t];
+            rot[jj * k + t] = small[L.sn + t];
+        }
+    }
+    if (threadIdx.x == 0) conv = 1;
+    __syncthreads();
+    // per column: rho, a.b, column jj completed and rotated, beta, stop test
+    for (int c = threadIdx.x; c < k; c += blockDim.x) {
+        double rho = 1.0, ab = 0.0;
+        if (j > 0) {
+            const double alpha = sv[2 * j * k + c];
+            double aa = 0.0;
+            for (int i = 0; i < j; ++i) {
+                const double ai = sv[i * k + c];
+                aa += ai * ai;
+                if (!finish) ab += ai * sv[(j + i) * k + c];
+            }
+            // |u - Q a|; at a breakdown (u in the span of Q to rounding) at least 2^-52 |u|
+            rho = sqrt(fmax(alpha - aa, 0x1p-104 * alpha));
+            for (int i = 0; i < j; ++i) col[i * k + c] += sv[i * k + c];  // second pass of column jj
+            col[j * k + c] = rho;
+        }
+        sc[c] = rho;
+        sc[k + c] = 1.0 / rho;
+        sc[2 * k + c] = ab;
+    }
+    __syncthreads();
+    if (j > 0) {
+        // Hraw column jj (unrotated) to memory; then the Givens chain on a copy
+        for (int t = threadIdx.x; t < (j + 1) * k; t += kBlock) {
+            const int i = t / k, c = t - i * k;
+            Hr[at(i, c) + (int64_t)jj * k] = col[t];
+        }
+        __syncthreads();
+        for (int c = threadIdx.x; c < k; c += blockDim.x) {
+            // Givens on column jj (GMRES.hpp:603-658)
+            for (int i = 0; i < jj; ++i) {
+                const double xi = col[i * k + c], eta = col[(i + 1) * k + c];
+                const double co = rot[i * k + c], si = rot[jj * k + i * k + c];
+                col[i * k + c] = co * xi + si * eta;
+                col[(i + 1) * k + c] = -si * xi + co * eta;
+            }
+            const double xi = col[jj * k + c], eta = col[j * k + c];
+            const double r = sqrt(xi * xi + eta * eta);
+            double co, si;
+            if (fabs(xi) <= 2.220446049250313e-16 * r) {
+                co = 0.0;
+                si = 1.0;
+            } else {
+                const double rinv = 1.0 / r;
+                co = fabs(xi) * rinv;
+                si = (xi / fabs(xi)) * eta * rinv;
+            }
+            small[L.cs + jj * k + c] = co;
+            small[L.sn + jj * k + c] = si;
+            col[jj * k + c] = co * xi + si * eta;
+            col[j * k + c] = 0.0;
+            const double bj = small[L.beta + jj * k + c];
+            small[L.beta + j * k + c] = -si * bj;
+            small[L.beta + jj * k + c] = co * bj;
+            if (!(fabs(small[L.beta + j * k + c]) <= small[L.TOL + c])) atomicAnd(&conv, 0);
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < (j + 1) * k; t += kBlock) {
+            const int i = t / k, c = t - i * k;
+            H[at(i, c) + (int64_t)jj * k] = col[t];
+        }
+    }
+    // coefficients of q_j = u / rho - sum (a_i / rho) q_i
+    for (int t = threadIdx.x; t < j * k; t += kBlock) small[L.dA + t] = sv[t] * sc[k + t % k];
+    for (int c = threadIdx.x; c < k; c += blockDim.x) small[L.dsc + c] = sc[k + c];
+    __syncthreads();
+    const bool stop = j > 0 && (conv || j >= hd->m);
+    if (!finish && !stop) {
+        // column j, first pass: g_l = (t_l - (Hraw a)_l) / rho, t_l = b_l (l < j),
+        // t_j = (beta - a.b) / rho; Hraw[l][i] = 0 for l > i + 1
+        for (int t = threadIdx.x; t < (j + 1) * k; t += kBlock) {
+            const int l = t / k, c = t - l * k;
+            const double irho = sc[k + c];
+            double ha = 0.0;
+#pragma unroll 4
+            for (int i = l > 0 ? l - 1 : 0; i < j; ++i) ha += Hr[at(l, c) + (int64_t)i * k] * sv[i * k + c];
+            const double tl = l < j ? sv[(j + l) * k + c] : (j > 0 ? (sv[(2 * j + 1) * k + c] - sc[2 * k + c]) * irho
+                                                                    : sv[(2 * j + 1) * k + c]);
+            const double g = (tl - ha) * irho;
+            Hr[at(l, c) + (int64_t)j * k] = g;
+            H[at(l, c) + (int64_t)j * k] = g;
+            if (l == j) sc[3 * k + c] = tl * irho * irho;
+        }
+        __syncthreads();
+        // u' = v / rho - (t_j / rho^2) u - sum (t_i / rho - t_j a_i / rho^2) q_i
+        for (int t = threadIdx.x; t < j * k; t += kBlock) {
+            const int c = t % k;
+            small[L.dE + t] = sv[j * k + t] * sc[k + c] - sc[3 * k + c] * sv[t];
+        }
+        for (int c = threadIdx.x; c < k; c += blockDim.x) small[L.dsc + k + c] = sc[3 * k + c];
+    }
+    if (threadIdx.x == 0 && j > 0) {
+        hd->iter = j;
+        hd->succ = conv;
+        hd->done = stop ? 1 : 0;
+    }
+}
+
+// q_j = u / rho - sum a_i / rho q_i (slot j, in place) and, while the cycle goes on,
+// u' = v / rho - (t_j / rho^2) u - sum (t_i / rho - t_j a_i / rho^2) q_i (slot j + 1,
+// in place of v).  Runs for the step whose scalar kernel ended the cycle (q_j only).
+template <typename T, int NQMAX>
+__global__ __launch_bounds__(kBlock) void gm_dupdate_kernel(int n, int k, int kpad, int j, T* __restrict__ Q,
+                                                            int64_t qstride, const double* __restrict__ small,
+                                                            GmresLayout L, const GmresHdr* __restrict__ hd) {
+    if (hd->done && hd->iter != j) return;
+    const bool next = !hd->done;
+    int i0, i1;
+    grow_range(n, i0, i1);
+    const int c = threadIdx.x % kpad, ro = threadIdx.x / kpad, rows = kBlock / kpad;
+    if (c >= k) return;
+    T ca[NQMAX], ce[NQMAX];
+#pragma unroll
+    for (int i = 0; i < NQMAX; ++i) {
+        ca[i] = i < j ? (T)small[L.dA + i * k + c] : T(0);
+        ce[i] = (next && i < j) ? (T)small[L.dE + i * k + c] : T(0);
+    }
+    const T ir = (T)small[L.dsc + c];
+    const T cj = next ? (T)small[L.dsc + k + c] : T(0);
+    T* U = Q + (int64_t)j * qstride;
+    T* V = Q + (int64_t)(j + 1) * qstride;
+    for (int r = i0 + ro; r < i1; r += rows) {
+        const int64_t e = (int64_t)r * k + c;
+        T qi[NQMAX];
+#pragma unroll
+        for (int i = 0; i < NQMAX; ++i) qi[i] = Q[(int64_t)max(0, min(i, j - 1)) * qstride + e];  // loads first
+        const T uv = U[e];
+        const T vv = next ? V[e] : T(0);
+        T qv = uv * ir;
+        T nv = vv * ir - cj * uv;
+#pragma unroll
+        for (int i = 0; i < NQMAX; ++i)
+            if (i < j) {
+                qv -= ca[i] * qi[i];
+                nv -= ce[i] * qi[i];
+            }
+        if (j > 0) U[e] = qv;
+        if (next) V[e] = nv;
+    }
+}
 
 // H[j+1][j] = |q|; Givens (GMRES.hpp:603-658); beta; convergence; ++iter.  |q|^2 per
 // column is first summed from the B workgroup partials in gm_total_kernel's order (one
@@ -511,6 +785,36 @@ static void update(hipStream_t s, int gb, int n, int k, int kpad, int nq, const 
     else launch_update<T, 64>(s, gb, n, k, kpad, nq, Q, qs, q, h, norm_only, partial, hd);
 }
 
+template <typename T, int NQMAX>
+static void launch_ddots(hipStream_t s, int gb, int n, int k, int kpad, int j, int finish, const T* Q, int64_t qs,
+                         const T* z, const T* d, T* v, double* partial, const GmresHdr* hd) {
+    const size_t lds = (size_t)(kBlock / kWave) * (2 * j + 2) * k * sizeof(double);
+    hipLaunchKernelGGL((gm_ddots_kernel<T, NQMAX>), dim3(gb), dim3(kBlock), lds, s, n, k, kpad, j, finish, Q, qs, z,
+                       d, v, partial, hd);
+}
+template <typename T>
+static void ddots(hipStream_t s, int gb, int n, int k, int kpad, int j, int finish, const T* Q, int64_t qs,
+                  const T* z, const T* d, T* v, double* partial, const GmresHdr* hd) {
+    if (j <= 4) launch_ddots<T, 4>(s, gb, n, k, kpad, j, finish, Q, qs, z, d, v, partial, hd);
+    else if (j <= 8) launch_ddots<T, 8>(s, gb, n, k, kpad, j, finish, Q, qs, z, d, v, partial, hd);
+    else if (j <= 16) launch_ddots<T, 16>(s, gb, n, k, kpad, j, finish, Q, qs, z, d, v, partial, hd);
+    else launch_ddots<T, 32>(s, gb, n, k, kpad, j, finish, Q, qs, z, d, v, partial, hd);
+}
+template <typename T>
+static void dupdate(hipStream_t s, int gb, int n, int k, int kpad, int j, T* Q, int64_t qs, const double* small,
+                    const GmresLayout& L, const GmresHdr* hd) {
+    if (j <= 4)
+        hipLaunchKernelGGL((gm_dupdate_kernel<T, 4>), dim3(gb), dim3(kBlock), 0, s, n, k, kpad, j, Q, qs, small, L, hd);
+    else if (j <= 8)
+        hipLaunchKernelGGL((gm_dupdate_kernel<T, 8>), dim3(gb), dim3(kBlock), 0, s, n, k, kpad, j, Q, qs, small, L, hd);
+    else if (j <= 16)
+        hipLaunchKernelGGL((gm_dupdate_kernel<T, 16>), dim3(gb), dim3(kBlock), 0, s, n, k, kpad, j, Q, qs, small, L,
+                           hd);
+    else
+        hipLaunchKernelGGL((gm_dupdate_kernel<T, 32>), dim3(gb), dim3(kBlock), 0, s, n, k, kpad, j, Q, qs, small, L,
+                           hd);
+}
+
 static int grid_elems(int64_t count) {
     return (int)std::max<int64_t>(1, std::min<int64_t>((count + kBlock - 1) / kBlock, 8192));
 }
@@ -519,7 +823,7 @@ struct GmresWs {
     GmresHdr* hd;
     double* small;
     double* partial;
-    double* tot;  // (m+1) * k totals
+    double* tot;  // (2m+2) * k totals
     void* Q;
     void* x;
     void* z;
@@ -538,8 +842,8 @@ static size_t gmres_layout(int32_t n, int32_t m, int32_t k, size_t vs, GmresWs* 
     GmresWs t;
     t.hd = (GmresHdr*)take(sizeof(GmresHdr));
     t.small = (double*)take(L.total * sizeof(double));
-    t.partial = (double*)take((size_t)kGB * (m + 1) * k * sizeof(double));
-    t.tot = (double*)take((size_t)(m + 1) * k * sizeof(double));
+    t.partial = (double*)take((size_t)kGB * (2 * m + 2) * k * sizeof(double));  // delayed form: 2m + 2 values
+    t.tot = (double*)take((size_t)(2 * m + 2) * k * sizeof(double));
     t.Q = take((size_t)(m + 1) * n * k * vs);
     t.x = take((size_t)n * k * vs);
     t.z = take((size_t)n * k * vs);
@@ -669,6 +973,37 @@ int gmres_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d,
                            (const double*)(w.small + L.inv), (const GmresHdr*)w.hd, j);
         return TG_OK;
     };
+    // Delayed second pass: two basis passes and five launches per step, and a finishing
+    // step per cycle.  Used where its partial sums fit the LDS (2m + 2 values of k columns)
+    // and its registers (m <= 32).
+    const bool delayed = g_gmres_delayed && mm <= kDdMaxM && (2 * mm + 2) * k <= 2048;
+    auto delayed_step = [&](hipStream_t st, int j, bool finish) -> int {
+        T* U = Q + (int64_t)j * qstride;
+        T* V = finish ? U : Q + (int64_t)(j + 1) * qstride;
+        if (!finish) {
+            int rc;
+            if (side == 0) {
+                // v = A u (diagonal P folded into the dots pass)
+                rc = op_apply(Aop, (const T*)U, (dd || prec_kind == 2) ? z : V, k, &w.hd->done, st);
+                if (!rc && prec_kind == 2) rc = applyP_on(st, (const T*)z, V, &w.hd->done);
+            } else {
+                rc = applyP_on(st, (const T*)U, z, &w.hd->done);
+                if (!rc) rc = op_apply(Aop, (const T*)z, V, k, &w.hd->done, st);
+            }
+            if (rc) return rc;
+        }
+        const int nv = 2 * j + 2;
+        ddots<T>(st, gb, n, k, kpad, j, finish ? 1 : 0, (const T*)Q, qstride, (const T*)z,
+                 (side == 0 && !finish) ? dd : nullptr, V, w.partial, (const GmresHdr*)w.hd);
+        hipLaunchKernelGGL(gm_total_kernel, dim3(nv * k), dim3(kBlock), 0, st, nv, k, gb, (const double*)w.partial,
+                           w.tot, (const GmresHdr*)w.hd);
+        hipLaunchKernelGGL(gm_dscalar_kernel, dim3(1), dim3(kBlock), 0, st, j, mm, k, finish ? 1 : 0,
+                           (const double*)w.tot, w.small, L, w.hd);
+        dupdate<T>(st, gb, n, k, kpad, j, Q, qstride, (const double*)w.small, L, (const GmresHdr*)w.hd);
+        return TG_OK;
+    };
+    auto step = [&](hipStream_t st, int j) -> int { return delayed ? delayed_step(st, j, false) : arnoldi_step(st, j); };
+    auto finish_cycle = [&](hipStream_t st) -> int { return delayed ? delayed_step(st, m, true) : TG_OK; };
     struct CycleGraph {
         hipStream_t cs = nullptr;
         hipGraphExec_t ex = nullptr;
@@ -716,7 +1051,8 @@ int gmres_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d,
                 if (!cyc.cs) TG_HIP(hipStreamCreateWithFlags(&cyc.cs, hipStreamNonBlocking));
                 TG_HIP(hipStreamBeginCapture(cyc.cs, hipStreamCaptureModeThreadLocal));
                 int rc = TG_OK;
-                for (int j = 0; j < m && rc == TG_OK; ++j) rc = arnoldi_step(cyc.cs, j);
+                for (int j = 0; j < m && rc == TG_OK; ++j) rc = step(cyc.cs, j);
+                if (rc == TG_OK) rc = finish_cycle(cyc.cs);
                 hipGraph_t g = nullptr;
                 const hipError_t ec = hipStreamEndCapture(cyc.cs, &g);
                 if (rc) {
@@ -736,7 +1072,7 @@ int gmres_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d,
             int j = 0, batch = host_ops ? 1 : 4;
             while (j < m) {
                 for (int t = 0; t < batch && j < m; ++t, ++j) {
-                    int rc = arnoldi_step(s, j);
+                    int rc = step(s, j);
                     if (rc) return rc;
                 }
                 TG_LAUNCH_CHECK();
@@ -745,6 +1081,9 @@ int gmres_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d,
                 if (hh.done) break;
                 if (!host_ops) batch = std::min(batch * 2, 16);
             }
+            int rcf = finish_cycle(s);  // returns at once on the device when the cycle stopped early
+            if (rcf) return rcf;
+            TG_LAUNCH_CHECK();
         }
         ++cycles;
         TG_HIP(hipMemcpyAsync(&hh, w.hd, sizeof(hh), hipMemcpyDeviceToHost, s));
@@ -802,14 +1141,18 @@ template int gmres_solve_t<float>(const CsrView&, const OpRef&, int, const float
                                   void*, size_t, GmresResult*, hipStream_t);
 
 int gmres_tune_set(const char* key, int32_t value, bool* known) {
-    *known = std::string(key) == "gmres_graph";
+    const std::string k = key;
+    *known = k == "gmres_graph" || k == "gmres_delayed";
     if (!*known) return TG_OK;
-    if (value != 0 && value != 1) return fail(TG_ERR_INVALID, "tg_tune_set: gmres_graph must be 0 or 1");
-    g_gmres_graph = value;
+    if (value != 0 && value != 1) return fail(TG_ERR_INVALID, "tg_tune_set: %s must be 0 or 1", key);
+    (k == "gmres_graph" ? g_gmres_graph : g_gmres_delayed) = value;
     return TG_OK;
 }
 
-int32_t gmres_tune_get(const char* key) { return std::string(key) == "gmres_graph" ? g_gmres_graph : -1; }
+int32_t gmres_tune_get(const char* key) {
+    const std::string k = key;
+    return k == "gmres_graph" ? g_gmres_graph : k == "gmres_delayed" ? g_gmres_delayed : -1;
+}
 
 }  // namespace tg
 

@@ -133,12 +133,23 @@ def test_cg_fixed_iterations_panel_order(cuda, panel_knob, panel, prec):
     assert res_err(cg.Residuals(), ar_o, ar_o / rr_o) <= TOL
 
 
+@pytest.fixture(params=[1, 0], ids=["delayed", "cgs2"])
+def gmres_delayed(request):
+    """Both Gram-Schmidt forms of the GMRES step (knob gmres_delayed)."""
+    from tensors_amd import _capi as capi
+
+    before = int(capi.lib().tg_tune_get(b"gmres_delayed"))
+    capi.call("tg_tune_set", b"gmres_delayed", request.param)
+    yield request.param
+    capi.call("tg_tune_set", b"gmres_delayed", before)
+
+
 @pytest.mark.parametrize("m", [5, 12])
 @pytest.mark.parametrize("cycles", [1, 2])
 @pytest.mark.parametrize("side", [0, 1], ids=["left", "right"])
 @pytest.mark.parametrize("prec", [False, True], ids=["identity", "jacobi"])
 @pytest.mark.parametrize("k", [1, 3])
-def test_gmres_fixed_cycles_match_oracle(cuda, m, cycles, side, prec, k):
+def test_gmres_fixed_cycles_match_oracle(cuda, gmres_delayed, m, cycles, side, prec, k):
     rp, ci, val = gen.convection_diffusion_2d(24, 10.0)
     n = len(rp) - 1
     d = 1.0 / sp.csr_matrix((val, ci, rp), shape=(n, n)).diagonal() if prec else None
@@ -150,7 +161,7 @@ def test_gmres_fixed_cycles_match_oracle(cuda, m, cycles, side, prec, k):
     Xd = torch.zeros(n, k, dtype=torch.float64, device=cuda)
     ok = gm(MatrixCSR(rp.astype(np.int32), ci, val, n, n), None if d is None else dev(d, cuda), 1.0,
             dev(B, cuda), 0.0, Xd, 0.0, cycles - 1)
-    print(f"GMRES m {m} cycles {cycles} side {side} prec {prec} k {k}")
+    print(f"GMRES delayed {gmres_delayed} m {m} cycles {cycles} side {side} prec {prec} k {k}")
     assert not ok and ok_o == 0
     assert gm.IterationCount() == it_o == m and gm.RestartCount() == rs_o == cycles - 1
     assert rel_err(Xd.cpu().numpy(), Xo) <= TOL
