@@ -99,9 +99,11 @@ template <int NQ>
 __device__ __forceinline__ void multi_partial(double (&acc)[NQ], int nq, int k, int kpad, double* part, int t,
                                               int stride, double* ws) {
     const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave, c = threadIdx.x % kpad;
+    // offset by offset over all values, so the NQ butterfly chains overlap
+    for (int off = kpad; off < kWave; off <<= 1) {
 #pragma unroll
-    for (int i = 0; i < NQ; ++i)
-        for (int off = kpad; off < kWave; off <<= 1) acc[i] += shfl_xor(acc[i], off);
+        for (int i = 0; i < NQ; ++i) acc[i] += shfl_xor(acc[i], off);
+    }
     if (lane < kpad)
 #pragma unroll
         for (int i = 0; i < NQ; ++i) ws[(w * NQ + i) * kGbMaxK + c] = acc[i];

@@ -103,13 +103,16 @@ __device__ __forceinline__ void multi_partial(double (&v)[NQMAX], int nq, int k,
                                               double* __restrict__ partial) {
     extern __shared__ double gm_sh[];
     const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    // butterflies of all registers, offset by offset, so the chains overlap (registers
+    // past nq hold zeros)
+    for (int off = kpad; off < kWave; off <<= 1) {
 #pragma unroll
-    for (int i = 0; i < NQMAX; ++i) {
-        if (i < nq) {
-            double t = v[i];
-            for (int off = kpad; off < kWave; off <<= 1) t += shfl_xor(t, off);
-            if (lane < kpad && lane < k) gm_sh[(wave * nq + i) * k + lane] = t;
-        }
+        for (int i = 0; i < NQMAX; ++i) v[i] += shfl_xor(v[i], off);
+    }
+    if (lane < kpad && lane < k) {
+#pragma unroll
+        for (int i = 0; i < NQMAX; ++i)
+            if (i < nq) gm_sh[(wave * nq + i) * k + lane] = v[i];
     }
     __syncthreads();
     for (int t = threadIdx.x; t < nq * k; t += kBlock) {
@@ -269,14 +272,18 @@ __device__ __forceinline__ void dd_partial(double (&v)[2 * NQMAX + 2], int j, in
     extern __shared__ double gm_sh[];
     const int nv = 2 * j + 2;
     const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    // butterflies of all registers, offset by offset: independent chains that overlap
+    // (the unused registers hold zeros); a condition per value would serialise them
+    for (int off = kpad; off < kWave; off <<= 1) {
 #pragma unroll
-    for (int r = 0; r < 2 * NQMAX + 2; ++r) {
-        const int t = r < NQMAX ? (r < j ? r : -1)
-                                : r < 2 * NQMAX ? (r - NQMAX < j ? j + r - NQMAX : -1) : 2 * j + (r - 2 * NQMAX);
-        if (t >= 0) {
-            double x = v[r];
-            for (int off = kpad; off < kWave; off <<= 1) x += shfl_xor(x, off);
-            if (lane < kpad && lane < k) gm_sh[(wave * nv + t) * k + lane] = x;
+        for (int r = 0; r < 2 * NQMAX + 2; ++r) v[r] += shfl_xor(v[r], off);
+    }
+    if (lane < kpad && lane < k) {
+#pragma unroll
+        for (int r = 0; r < 2 * NQMAX + 2; ++r) {
+            const int t = r < NQMAX ? (r < j ? r : -1)
+                                    : r < 2 * NQMAX ? (r - NQMAX < j ? j + r - NQMAX : -1) : 2 * j + (r - 2 * NQMAX);
+            if (t >= 0) gm_sh[(wave * nv + t) * k + lane] = v[r];
         }
     }
     __syncthreads();
