@@ -216,6 +216,13 @@ int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, doubl
  *                  this many 1024-row tiles (default 64) get each reduction's totals
  *                  from one workgroup per system instead of every tile workgroup
  *                  adding all tiles' partials.  Bitwise the same results.
+ *   "gbatch_delayed": batched GMRES engines created afterwards, 1 = the second
+ *                  Gram-Schmidt pass delayed to the next step (default: two
+ *                  launches and two passes over the basis per step, every
+ *                  workgroup computing the system's scalars from the totals), 0 =
+ *                  CGS2 (four launches, three passes).  Results agree to rounding;
+ *                  within one engine a system's result is bitwise the same alone
+ *                  and in any mix in both forms.
  *   "serve_gmres_batch", "serve_gmres_batch_rows": GMRES requests of servers
  *                  created afterwards; 1 (default) = requests of up to
  *                  serve_gmres_batch_rows rows (default 262144) join one batched

@@ -27,6 +27,21 @@
 //   lsq      tile 0: y = H^{-1} beta (upper back substitution)
 //   update   x -= Q y on the tile's rows
 //   end      tile 0: cycle count, finished / succeeded
+// With the delayed second pass (knob gbatch_delayed = 1 when the engine is
+// created, the default) a step is two launches and two passes over the basis,
+// as the single-system solver's delayed form: step j (j = 0 .. m; j = m is a
+// system's finishing step) runs
+//   dstep1   v = P A u on the tile's rows (u = Q_j, the step j - 1 vector after one
+//            pass; no product on the finishing step), tile partials of
+//            a_i = Q_i.u, b_i = Q_i.v (i < j), u.u, u.v
+//   dstep2   every workgroup adds the partials and computes the same scalars:
+//            rho = sqrt(u.u - |a|^2), column j - 1 of H completed (first pass G
+//            plus a, H[j][j-1] = rho), its Givens rotation and the stop test, and
+//            unless the cycle stops, column j's first pass G = (t - Hc a) / rho
+//            (t = Q^T v from the totals, Hc the unrotated columns); tile 0 writes
+//            them (Hc, H, cs, sn, the carried right-hand side beta, G, the state)
+//            into arrays no workgroup reads in the same launch; then q_j =
+//            (u - Q a) / rho and the next vector (v - Q t) / rho on the tile's rows.
 // Every workgroup of a system sums that system's tile partials itself, in one
 // fixed order, so all of them see bitwise the same scalars, and no workgroup
 // reads what another writes in the same launch.  A system's arithmetic depends
@@ -65,9 +80,12 @@ struct GbSys {
     double* p1;    // tiles x (m+1) x k partials of the first Gram-Schmidt pass
     double* p2;    // ... of the second pass
     double* pn;    // tiles x k partials of a squared norm
-    double* tot;   // (kGbMaxM + 1) x kGbMaxK totals of the last reduction (big systems)
+    double* tot;   // (2 kGbMaxM + 2) x kGbMaxK totals of the last reduction (big systems)
+    double* G;     // delayed form: (m+1) x m x k first-pass coefficients, column j from step j
+    double* Hc;    // delayed form: (m+1) x m x k completed, unrotated columns (the Arnoldi relation)
     int n, k, kpad, tiles, tile_rows, m;
     int big;       // more than g_gb_big_tiles tiles: the totals come from gb_totals_kernel
+    int delayed;   // delayed second pass (p1 holds 2m + 2 values per tile, beta the carried rhs)
 };
 
 struct GbState {
@@ -99,14 +117,11 @@ template <int NQ>
 __device__ __forceinline__ void multi_partial(double (&acc)[NQ], int nq, int k, int kpad, double* part, int t,
                                               int stride, double* ws) {
     const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave, c = threadIdx.x % kpad;
-    // offset by offset over all values, so the NQ butterfly chains overlap
-    for (int off = kpad; off < kWave; off <<= 1) {
+    const ScatterOut o = wave_scatter_sum<NQ>(acc, kpad);  // kpad <= kGbMaxK = 8: C values per lane
+    if ((lane & 7) < kpad)
 #pragma unroll
-        for (int i = 0; i < NQ; ++i) acc[i] += shfl_xor(acc[i], off);
-    }
-    if (lane < kpad)
-#pragma unroll
-        for (int i = 0; i < NQ; ++i) ws[(w * NQ + i) * kGbMaxK + c] = acc[i];
+        for (int i = 0; i < ScatterSizes<NQ>::C; ++i)
+            if (i < o.count) ws[(w * NQ + o.base + i) * kGbMaxK + c] = acc[i];
     __syncthreads();
     for (int p = threadIdx.x; p < nq * kpad; p += kBlock) {
         const int i = p / kpad, cc = p % kpad;
@@ -434,9 +449,225 @@ __global__ __launch_bounds__(kBlock) void gb_step4_kernel(const GbTile* __restri
     }
 }
 
+// ---- delayed second pass ----------------------------------------------------------
+// Tile partials of the 2j + 2 values [a_0..a_{j-1}, b_0..b_{j-1}, u.u, u.v] held in
+// acc[i] (a_i), acc[NQ + i] (b_i), acc[2 NQ], acc[2 NQ + 1]; the same fixed order as
+// multi_partial.  ws: 4 * (2 NQ + 2) * kGbMaxK doubles of LDS.
+template <int NQ>
+__device__ __forceinline__ void dd_partial(double (&acc)[2 * NQ + 2], int j, int k, int kpad, double* part, int t,
+                                           int stride, double* ws) {
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave, c = threadIdx.x % kpad;
+    constexpr int R = 2 * NQ + 2;
+    const ScatterOut o = wave_scatter_sum<R>(acc, kpad);  // kpad <= kGbMaxK = 8: C values per lane
+    if ((lane & 7) < kpad)
+#pragma unroll
+        for (int i = 0; i < ScatterSizes<R>::C; ++i)
+            if (i < o.count) ws[(w * R + o.base + i) * kGbMaxK + c] = acc[i];
+    __syncthreads();
+    const int nv = 2 * j + 2;
+    for (int p = threadIdx.x; p < nv * kpad; p += kBlock) {
+        const int i = p / kpad, cc = p % kpad;
+        const int r = i < j ? i : i < 2 * j ? NQ + (i - j) : 2 * NQ + (i - 2 * j);
+        if (cc < k) {
+            double v = ws[(0 * (2 * NQ + 2) + r) * kGbMaxK + cc];
+            for (int ww = 1; ww < kBlock / kWave; ++ww) v += ws[(ww * (2 * NQ + 2) + r) * kGbMaxK + cc];
+            part[((int64_t)t * stride + i) * k + cc] = v;
+        }
+    }
+    __syncthreads();
+}
+
+// v = P A u on the tile's rows (u = Q_j) into Q_{j+1}, partials of the 2j + 2 dots.
+// On a system's finishing step (j == m) there is no product: v = u, nothing written.
+template <int NQ>
+__global__ __launch_bounds__(kBlock, 2) void gb_dstep1_kernel(const GbTile* __restrict__ tiles,
+                                                           const GbSys* __restrict__ sys,
+                                                           const GbState* __restrict__ st, int j) {
+    GB_TILE_PROLOGUE
+    if (s->done || !s->active || j > S.m) return;
+    const bool fin = j == S.m;
+    const int64_t N = (int64_t)S.n * k;
+    const double* __restrict__ U = S.Q + j * N;
+    double* __restrict__ V = fin ? nullptr : S.Q + (int64_t)(j + 1) * N;
+    double acc[2 * NQ + 2];
+#pragma unroll
+    for (int r = 0; r < 2 * NQ + 2; ++r) acc[r] = 0.0;
+    if (c < k)
+        for (int row = T.r0 + ro; row < T.r1; row += rows) {
+            const int64_t e = (int64_t)row * k + c;
+            double qv[NQ];
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) qv[i] = S.Q[(int64_t)max(0, min(i, j - 1)) * N + e];
+            const double uv = U[e];
+            double vv = uv;
+            if (!fin) {
+                double w = 0.0;
+                for (int32_t l = S.rp[row]; l < S.rp[row + 1]; ++l) w += S.val[l] * U[(int64_t)S.ci[l] * k + c];
+                if (S.d) w = S.d[row] * w;
+                V[e] = w;
+                vv = w;
+            }
+#pragma unroll
+            for (int i = 0; i < NQ; ++i)
+                if (i < j) {
+                    acc[i] += qv[i] * uv;
+                    acc[NQ + i] += qv[i] * vv;
+                }
+            acc[2 * NQ] += uv * uv;
+            acc[2 * NQ + 1] += uv * vv;
+        }
+    __shared__ double ws[4 * (2 * NQ + 2) * kGbMaxK];
+    dd_partial<NQ>(acc, j, k, kpad, S.p1, T.t, 2 * S.m + 2, ws);
+}
+
+// The scalars of step j from the totals (the same in every workgroup of the system),
+// tile 0's writes, then q_j and the next vector on the tile's rows.
+template <int NQ>
+__global__ __launch_bounds__(kBlock) void gb_dstep2_kernel(const GbTile* __restrict__ tiles,
+                                                           const GbSys* __restrict__ sys, GbState* __restrict__ st,
+                                                           int j) {
+    GB_TILE_PROLOGUE
+    if (s->done || !s->active || j > S.m) return;
+    const int m = S.m, nv = 2 * j + 2, jj = j - 1;
+    const bool fin = j == m;
+    const int64_t N = (int64_t)S.n * k;
+    __shared__ double sh[kBlock];
+    __shared__ double tv[(2 * kGbMaxM + 2) * kGbMaxK];  // totals [value][column]
+    __shared__ double colc[(kGbMaxM + 1) * kGbMaxK];    // column j - 1 completed, unrotated
+    __shared__ double colr[(kGbMaxM + 1) * kGbMaxK];    // ... rotated
+    __shared__ double dA[kGbMaxM * kGbMaxK], dE[kGbMaxM * kGbMaxK];
+    __shared__ double rho_s[kGbMaxK], irho_s[kGbMaxK], ab_s[kGbMaxK], cj_s[kGbMaxK];
+    __shared__ double rcs[kGbMaxM * kGbMaxK], rsn[kGbMaxM * kGbMaxK];  // rotations of earlier columns
+    __shared__ int conv_s;
+    for (int p = threadIdx.x; p < jj * k; p += kBlock) {
+        const int i = p / k, cc = p % k;
+        rcs[i * kGbMaxK + cc] = S.cs[p];
+        rsn[i * kGbMaxK + cc] = S.sn[p];
+    }
+    gb_totals(S, S.p1, 2 * m + 2, nv, tv, sh);
+    auto a = [&](int i, int cc) { return tv[i * kGbMaxK + cc]; };
+    auto b = [&](int i, int cc) { return tv[(j + i) * kGbMaxK + cc]; };
+    if (threadIdx.x == 0) conv_s = 1;
+    if (threadIdx.x < k) {
+        const int cc = threadIdx.x;
+        double rho = 1.0, ab = 0.0;
+        if (j > 0) {
+            const double alpha = tv[2 * j * kGbMaxK + cc];
+            double aa = 0.0;
+            for (int i = 0; i < j; ++i) {
+                aa += a(i, cc) * a(i, cc);
+                ab += a(i, cc) * b(i, cc);
+            }
+            // |u - Q a|; at a breakdown (u in the span of Q to rounding) at least 2^-52 |u|
+            rho = sqrt(fmax(alpha - aa, 0x1p-104 * alpha));
+            for (int l = 0; l < j; ++l) colc[l * kGbMaxK + cc] = S.G[((int64_t)l * m + jj) * k + cc] + a(l, cc);
+            colc[j * kGbMaxK + cc] = rho;
+        }
+        rho_s[cc] = rho;
+        irho_s[cc] = 1.0 / rho;
+        ab_s[cc] = ab;
+    }
+    __syncthreads();
+    if (j > 0 && threadIdx.x < k) {
+        // Givens on column j - 1 (GMRES.hpp:603-658) with the rotations of earlier launches
+        const int cc = threadIdx.x;
+        for (int l = 0; l <= j; ++l) colr[l * kGbMaxK + cc] = colc[l * kGbMaxK + cc];
+        for (int i = 0; i < jj; ++i) {
+            const double xi = colr[i * kGbMaxK + cc], eta = colr[(i + 1) * kGbMaxK + cc];
+            const double co = rcs[i * kGbMaxK + cc], si = rsn[i * kGbMaxK + cc];
+            colr[i * kGbMaxK + cc] = co * xi + si * eta;
+            colr[(i + 1) * kGbMaxK + cc] = -si * xi + co * eta;
+        }
+        const double xi = colr[jj * kGbMaxK + cc], eta = colr[j * kGbMaxK + cc];
+        const double r = sqrt(xi * xi + eta * eta);
+        double co, si;
+        if (fabs(xi) <= 2.220446049250313e-16 * r) {
+            co = 0.0;
+            si = 1.0;
+        } else {
+            const double rinv = 1.0 / r;
+            co = fabs(xi) * rinv;
+            si = (xi / fabs(xi)) * eta * rinv;
+        }
+        colr[jj * kGbMaxK + cc] = co * xi + si * eta;
+        colr[j * kGbMaxK + cc] = 0.0;
+        const double carry = -si * S.beta[jj * k + cc];  // beta holds the carried right-hand side
+        if (!(fabs(carry) <= s->TOL[cc])) atomicAnd(&conv_s, 0);
+        if (T.t == 0) {
+            S.cs[jj * k + cc] = co;
+            S.sn[jj * k + cc] = si;
+            S.beta[j * k + cc] = carry;
+        }
+    }
+    __syncthreads();
+    const bool stop = fin || (j > 0 && conv_s);
+    if (T.t == 0 && j > 0) {
+        for (int p = threadIdx.x; p < (j + 1) * k; p += kBlock) {
+            const int l = p / k, cc = p % k;
+            S.Hc[((int64_t)l * m + jj) * k + cc] = colc[l * kGbMaxK + cc];
+            S.H[((int64_t)l * m + jj) * k + cc] = colr[l * kGbMaxK + cc];
+        }
+        if (threadIdx.x == 0) {
+            GbState& w = st[T.slot];
+            w.iters = j;
+            w.conv = conv_s;
+            if (stop) w.active = 0;
+        }
+    }
+    if (stop) return;  // block-uniform; the solution update uses q_0 .. q_{j-1}
+    // column j, first pass: g_l = (t_l - (Hc a)_l) / rho, t_l = b_l (l < j),
+    // t_j = (u.v - a.b) / rho; Hc[l][i] = 0 for l > i + 1, column j - 1 from colc
+    for (int p = threadIdx.x; p < (j + 1) * k; p += kBlock) {
+        const int l = p / k, cc = p % k;
+        const double irho = irho_s[cc];
+        double ha = 0.0;
+#pragma unroll 8
+        for (int i = l > 0 ? l - 1 : 0; i < jj; ++i) ha += S.Hc[((int64_t)l * m + i) * k + cc] * a(i, cc);
+        if (j > 0) ha += colc[l * kGbMaxK + cc] * a(jj, cc);
+        const double uv = tv[(2 * j + 1) * kGbMaxK + cc];
+        const double tl = l < j ? b(l, cc) : (j > 0 ? (uv - ab_s[cc]) * irho : uv);
+        if (T.t == 0) S.G[((int64_t)l * m + j) * k + cc] = (tl - ha) * irho;
+        if (l == j) cj_s[cc] = tl * irho * irho;
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < j * k; p += kBlock) {
+        const int i = p / k, cc = p % k;
+        dA[i * kGbMaxK + cc] = a(i, cc) * irho_s[cc];
+        dE[i * kGbMaxK + cc] = b(i, cc) * irho_s[cc] - cj_s[cc] * a(i, cc);
+    }
+    __syncthreads();
+    if (c >= k) return;
+    double ca[NQ], ce[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+        ca[i] = i < j ? dA[i * kGbMaxK + c] : 0.0;
+        ce[i] = i < j ? dE[i * kGbMaxK + c] : 0.0;
+    }
+    const double ir = irho_s[c], cj = cj_s[c];
+    double* __restrict__ U = S.Q + j * N;
+    double* __restrict__ V = S.Q + (int64_t)(j + 1) * N;
+    for (int row = T.r0 + ro; row < T.r1; row += rows) {
+        const int64_t e = (int64_t)row * k + c;
+        double qv[NQ];
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) qv[i] = S.Q[(int64_t)max(0, min(i, j - 1)) * N + e];
+        const double uv = U[e], vv = V[e];
+        double qj = uv * ir, nx = vv * ir - cj * uv;
+#pragma unroll
+        for (int i = 0; i < NQ; ++i)
+            if (i < j) {
+                qj -= ca[i] * qv[i];
+                nx -= ce[i] * qv[i];
+            }
+        if (j > 0) U[e] = qj;
+        V[e] = nx;
+    }
+}
+
 // Totals of one reduction of each big system, one workgroup per system (grid: the
 // big active systems): phase 0 after start1 (|Q_0|^2), 1 after step1 (first-pass
-// dots), 2 after the first step23 (second-pass dots), 3 after the second (|q|^2).
+// dots), 2 after the first step23 (second-pass dots), 3 after the second (|q|^2),
+// 4 after dstep1 (the delayed form's 2j + 2 dots).
 // Each big system's tile workgroups then read 1 + nq values instead of adding all
 // tiles' partials each (tiles^2 reads per reduction).
 __global__ __launch_bounds__(kBlock) void gb_totals_kernel(const int* __restrict__ bigs,
@@ -446,12 +677,12 @@ __global__ __launch_bounds__(kBlock) void gb_totals_kernel(const int* __restrict
     const GbSys S = sys[slot];
     const GbState* __restrict__ s = st + slot;
     if (s->done) return;
-    if (phase > 0 && (!s->active || j >= S.m)) return;
-    const int nq = (phase == 1 || phase == 2) ? j + 1 : 1;
-    const double* part = phase == 0 || phase == 3 ? S.pn : phase == 1 ? S.p1 : S.p2;
-    const int stride = (phase == 1 || phase == 2) ? S.m + 1 : 1;
+    if (phase > 0 && (!s->active || (phase == 4 ? j > S.m : j >= S.m))) return;
+    const int nq = phase == 4 ? 2 * j + 2 : (phase == 1 || phase == 2) ? j + 1 : 1;
+    const double* part = phase == 0 || phase == 3 ? S.pn : phase == 1 || phase == 4 ? S.p1 : S.p2;
+    const int stride = phase == 4 ? 2 * S.m + 2 : (phase == 1 || phase == 2) ? S.m + 1 : 1;
     __shared__ double sh[kBlock];
-    __shared__ double out[(kGbMaxM + 1) * kGbMaxK];
+    __shared__ double out[(2 * kGbMaxM + 2) * kGbMaxK];
     system_totals(part, S.tiles, stride, nq, S.k, S.kpad, out, sh);
     for (int p = threadIdx.x; p < nq * kGbMaxK; p += kBlock)
         if (p % kGbMaxK < S.k) S.tot[p] = out[p];
@@ -469,7 +700,8 @@ __global__ __launch_bounds__(kBlock) void gb_lsq_kernel(const GbTile* __restrict
     const int it = s->iters, m = S.m, k = S.k;
     for (int cc = threadIdx.x; cc < k; cc += kBlock)
         for (int i = it - 1; i >= 0; --i) {
-            double v = S.beta[i * k + cc];
+            // the delayed form keeps beta as the carried right-hand side: rotate it here
+            double v = S.delayed ? S.cs[i * k + cc] * S.beta[i * k + cc] : S.beta[i * k + cc];
             for (int jj = i + 1; jj < it; ++jj) v -= S.H[((int64_t)i * m + jj) * k + cc] * S.y[jj * k + cc];
             S.y[i * k + cc] = v / S.H[((int64_t)i * m + i) * k + cc];
         }
@@ -525,27 +757,40 @@ size_t gb_system_bytes(int n, int k, int m, int tile_rows) {
     const size_t N = (size_t)n * k;
     return align256((size_t)(m + 1) * N * 8) + align256((size_t)(m + 1) * m * k * 8) + 2 * align256((size_t)m * k * 8) +
            align256((size_t)(m + 1) * k * 8) + align256((size_t)m * k * 8) +
-           2 * align256((size_t)tiles * (m + 1) * k * 8) + align256((size_t)tiles * k * 8) +
-           align256((size_t)(kGbMaxM + 1) * kGbMaxK * 8);
+           align256((size_t)tiles * (2 * m + 2) * k * 8) + align256((size_t)tiles * (m + 1) * k * 8) +
+           align256((size_t)tiles * k * 8) + align256((size_t)(2 * kGbMaxM + 2) * kGbMaxK * 8) +
+           2 * align256((size_t)(m + 1) * m * k * 8);
 }
 
 // Systems of more than this many tiles get their totals from gb_totals_kernel
 // (tg_tune_set "gbatch_big_tiles", read when a system is added); bitwise the same.
 static int g_gb_big_tiles = 64;
+// 1: the delayed second Gram-Schmidt pass (two launches per step); read when an
+// engine is created (tg_tune_set "gbatch_delayed")
+static int g_gb_delayed = 1;
 
 int gbatch_tune_set(const char* key, int32_t value, bool* known) {
-    *known = std::string(key) == "gbatch_big_tiles";
+    const std::string kk = key;
+    *known = kk == "gbatch_big_tiles" || kk == "gbatch_delayed";
     if (!*known) return TG_OK;
+    if (kk == "gbatch_delayed") {
+        if (value != 0 && value != 1) return fail(TG_ERR_INVALID, "tg_tune_set: gbatch_delayed must be 0 or 1");
+        g_gb_delayed = value;
+        return TG_OK;
+    }
     if (value < 0) return fail(TG_ERR_INVALID, "tg_tune_set: gbatch_big_tiles must be >= 0");
     g_gb_big_tiles = value;
     return TG_OK;
 }
 
-int32_t gbatch_tune_get(const char* key) { return std::string(key) == "gbatch_big_tiles" ? g_gb_big_tiles : -1; }
+int32_t gbatch_tune_get(const char* key) {
+    const std::string kk = key;
+    return kk == "gbatch_big_tiles" ? g_gb_big_tiles : kk == "gbatch_delayed" ? g_gb_delayed : -1;
+}
 
 class GmresBatch {
   public:
-    explicit GmresBatch(hipStream_t s) : s_(s) {}
+    explicit GmresBatch(hipStream_t s) : s_(s), delayed_(g_gb_delayed) {}
     ~GmresBatch() { release(); }
 
     int add(int n, int64_t nnz, const int32_t* rp, const int32_t* ci, const double* val, const double* d, int k,
@@ -595,11 +840,14 @@ class GmresBatch {
         S.sn = take((size_t)m * k * 8);
         S.beta = take((size_t)(m + 1) * k * 8);
         S.y = take((size_t)m * k * 8);
-        S.p1 = take((size_t)S.tiles * (m + 1) * k * 8);
+        S.p1 = take((size_t)S.tiles * (2 * m + 2) * k * 8);
         S.p2 = take((size_t)S.tiles * (m + 1) * k * 8);
         S.pn = take((size_t)S.tiles * k * 8);
-        S.tot = take((size_t)(kGbMaxM + 1) * kGbMaxK * 8);
+        S.tot = take((size_t)(2 * kGbMaxM + 2) * kGbMaxK * 8);
+        S.G = take((size_t)(m + 1) * m * k * 8);
+        S.Hc = take((size_t)(m + 1) * m * k * 8);
         S.big = S.tiles > g_gb_big_tiles ? 1 : 0;
+        S.delayed = delayed_;
         mem_[slot] = mem;
         bytes_[slot] = bytes;
         used_[slot] = 1;
@@ -690,7 +938,26 @@ class GmresBatch {
         hipLaunchKernelGGL(gb_start1_kernel, g, b, 0, s, dtiles_, dsys_, dst_);
         totals(0, 0);
         hipLaunchKernelGGL(gb_start2_kernel, g, b, 0, s, dtiles_, dsys_, dst_);
-        for (int j = 0; j < mmax_; ++j) {
+        // delayed form: steps 0 .. m (m: the finishing step), two launches each
+        for (int j = 0; delayed_ && j <= mmax_; ++j) {
+#define TG_GB_DSTEPS(NQ)                                                                     \
+    hipLaunchKernelGGL((gb_dstep1_kernel<NQ>), g, b, 0, s, dtiles_, dsys_, dst_, j);       \
+    totals(4, j);                                                                          \
+    hipLaunchKernelGGL((gb_dstep2_kernel<NQ>), g, b, 0, s, dtiles_, dsys_, dst_, j);
+            if (j <= 4) {
+                TG_GB_DSTEPS(4)
+            } else if (j <= 8) {
+                TG_GB_DSTEPS(8)
+            } else if (j <= 16) {
+                TG_GB_DSTEPS(16)
+            } else if (j <= 32) {
+                TG_GB_DSTEPS(32)
+            } else {
+                TG_GB_DSTEPS(64)
+            }
+#undef TG_GB_DSTEPS
+        }
+        for (int j = 0; !delayed_ && j < mmax_; ++j) {
             const int nq = j + 1;
 #define TG_GB_STEPS(NQ)                                                                                 \
     hipLaunchKernelGGL((gb_step1_kernel<NQ>), g, b, 0, s, dtiles_, dsys_, dst_, j);                     \
@@ -791,6 +1058,8 @@ class GmresBatch {
         if (cap_s_) (void)hipStreamDestroy(cap_s_);
         cap_s_ = nullptr;
     }
+
+    const int delayed_;  // the form of every system of this engine (knob gbatch_delayed at creation)
 
     struct GraphKey {
         const void* tiles;

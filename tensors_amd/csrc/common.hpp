@@ -78,6 +78,65 @@ __device__ __forceinline__ T shfl_xor(T v, int mask) {
     return __shfl_xor(v, mask, kWave);
 }
 
+// Sums R values per lane over the lanes of a wave that share c = lane % kpad
+// (kpad a power of 2 <= 32), in place: a reduce-scatter over lane bits 5, 4 and 3
+// as far as they lie above the column bits (each level keeps half of the values and
+// adds the partner's copy of that half), then butterflies over the remaining bits
+// down to kpad.  Afterwards the lanes with (lane % low) < kpad, low = the lowest
+// scattered bit (32, 16 or 8), hold in v[0 .. count) the sums of registers base ..
+// base + count - 1 (those < R) for column lane % kpad; count = ceil(R / 2), ceil(R / 4)
+// or ceil(R / 8).  About R + 3 count shuffles instead of log2(64 / kpad) R, in a
+// fixed order.  A lane's registers are those of its segment: each level halves the
+// segment (the lower part ceil(size / 2)), so count = min(ceil, what remains) >= 0.
+template <int R>
+struct ScatterSizes {
+    static constexpr int A = (R + 1) / 2, B = (A + 1) / 2, C = (B + 1) / 2;
+};
+
+struct ScatterOut {
+    int base, count, low;
+};
+
+template <int R>
+__device__ __forceinline__ ScatterOut wave_scatter_sum(double (&v)[R], int kpad) {
+    constexpr int A = ScatterSizes<R>::A, B = ScatterSizes<R>::B, C = ScatterSizes<R>::C;
+    const int lane = threadIdx.x % kWave;
+    const bool b5 = (lane & 32) != 0, b4 = (lane & 16) != 0, b3 = (lane & 8) != 0;
+    ScatterOut o{b5 ? A : 0, 0, 32};
+    int end = b5 ? R : (A < R ? A : R);  // one past the lane's segment
+#pragma unroll
+    for (int i = 0; i < A; ++i) {
+        const double lo = v[i], hi = i + A < R ? v[i + A] : 0.0;
+        v[i] = (b5 ? hi : lo) + shfl_xor(b5 ? lo : hi, 32);
+    }
+    if (kpad <= 16) {
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            const double lo = v[i], hi = i + B < A ? v[i + B] : 0.0;
+            v[i] = (b4 ? hi : lo) + shfl_xor(b4 ? lo : hi, 16);
+        }
+        o.base += b4 ? B : 0;
+        if (!b4) end = o.base + B < end ? o.base + B : end;
+        o.low = 16;
+    }
+    if (kpad <= 8) {
+#pragma unroll
+        for (int i = 0; i < C; ++i) {
+            const double lo = v[i], hi = i + C < B ? v[i + C] : 0.0;
+            v[i] = (b3 ? hi : lo) + shfl_xor(b3 ? lo : hi, 8);
+        }
+        o.base += b3 ? C : 0;
+        if (!b3) end = o.base + C < end ? o.base + C : end;
+        o.low = 8;
+    }
+    for (int off = 4; off >= kpad; off >>= 1) {
+#pragma unroll
+        for (int i = 0; i < C; ++i) v[i] += shfl_xor(v[i], off);
+    }
+    o.count = end > o.base ? end - o.base : 0;
+    return o;
+}
+
 // Sum over the `width` lanes of an aligned lane group (width a power of 2).
 template <int WIDTH, typename T>
 __device__ __forceinline__ T group_sum(T v) {

@@ -103,16 +103,14 @@ __device__ __forceinline__ void multi_partial(double (&v)[NQMAX], int nq, int k,
                                               double* __restrict__ partial) {
     extern __shared__ double gm_sh[];
     const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
-    // butterflies of all registers, offset by offset, so the chains overlap (registers
-    // past nq hold zeros)
-    for (int off = kpad; off < kWave; off <<= 1) {
+    {
+        // reduce-scatter over the row lanes (registers past nq hold zeros)
+        const ScatterOut o = wave_scatter_sum<NQMAX>(v, kpad);
+        const int c = lane % kpad;
+        if ((lane % o.low) < kpad && c < k)
 #pragma unroll
-        for (int i = 0; i < NQMAX; ++i) v[i] += shfl_xor(v[i], off);
-    }
-    if (lane < kpad && lane < k) {
-#pragma unroll
-        for (int i = 0; i < NQMAX; ++i)
-            if (i < nq) gm_sh[(wave * nq + i) * k + lane] = v[i];
+            for (int i = 0; i < ScatterSizes<NQMAX>::A; ++i)
+                if (i < o.count && o.base + i < nq) gm_sh[(wave * nq + o.base + i) * k + c] = v[i];
     }
     __syncthreads();
     for (int t = threadIdx.x; t < nq * k; t += kBlock) {
@@ -272,19 +270,21 @@ __device__ __forceinline__ void dd_partial(double (&v)[2 * NQMAX + 2], int j, in
     extern __shared__ double gm_sh[];
     const int nv = 2 * j + 2;
     const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
-    // butterflies of all registers, offset by offset: independent chains that overlap
-    // (the unused registers hold zeros); a condition per value would serialise them
-    for (int off = kpad; off < kWave; off <<= 1) {
+    // compact index of register r (-1: an unused register)
+    auto compact = [&](int r) {
+        return r < NQMAX ? (r < j ? r : -1) : r < 2 * NQMAX ? (r - NQMAX < j ? j + r - NQMAX : -1) : 2 * j + (r - 2 * NQMAX);
+    };
+    constexpr int R = 2 * NQMAX + 2;
+    {
+        // reduce-scatter over the row lanes (the unused registers hold zeros)
+        const ScatterOut o = wave_scatter_sum<R>(v, kpad);
+        const int c = lane % kpad;
+        if ((lane % o.low) < kpad && c < k)
 #pragma unroll
-        for (int r = 0; r < 2 * NQMAX + 2; ++r) v[r] += shfl_xor(v[r], off);
-    }
-    if (lane < kpad && lane < k) {
-#pragma unroll
-        for (int r = 0; r < 2 * NQMAX + 2; ++r) {
-            const int t = r < NQMAX ? (r < j ? r : -1)
-                                    : r < 2 * NQMAX ? (r - NQMAX < j ? j + r - NQMAX : -1) : 2 * j + (r - 2 * NQMAX);
-            if (t >= 0) gm_sh[(wave * nv + t) * k + lane] = v[r];
-        }
+            for (int i = 0; i < ScatterSizes<R>::A; ++i) {
+                const int t = i < o.count ? compact(o.base + i) : -1;
+                if (t >= 0) gm_sh[(wave * nv + t) * k + c] = v[i];
+            }
     }
     __syncthreads();
     for (int t = threadIdx.x; t < nv * k; t += kBlock) {

@@ -1054,7 +1054,7 @@ extern "C" int32_t tg_tune_get(const char* key) {
     if (k == "cg_fused" || k == "cg_panel") return cg_tune_get(key);
     if (k == "bsr_vector") return bsr_tune_get(key);
     if (k == "gmres_graph" || k == "gmres_delayed") return gmres_tune_get(key);
-    if (k == "gbatch_big_tiles") return gbatch_tune_get(key);
+    if (k == "gbatch_big_tiles" || k == "gbatch_delayed") return gbatch_tune_get(key);
     if (k == "serve_gmres_batch" || k == "serve_gmres_batch_rows") return serve_tune_get(key);
     return batch_tune_get(key);
 }

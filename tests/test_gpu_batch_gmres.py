@@ -15,6 +15,18 @@ from tensors_amd.batch import BatchGMRES
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(params=[1, 0], ids=["delayed", "cgs2"], autouse=True)
+def gbatch_form(request):
+    """Every test with both step forms of the engine (knob gbatch_delayed, read when an
+    engine is created): the delayed second Gram-Schmidt pass and plain CGS2."""
+    from tensors_amd import _capi as capi
+
+    before = int(capi.lib().tg_tune_get(b"gbatch_delayed"))
+    capi.call("tg_tune_set", b"gbatch_delayed", request.param)
+    yield request.param
+    capi.call("tg_tune_set", b"gbatch_delayed", before)
+
+
 def systems(count, seed=5005, gmax=70, ms=(5, 12, 30)):
     rng = np.random.default_rng(seed)
     out = []
