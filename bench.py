@@ -113,7 +113,7 @@ def parse():
     ap.add_argument("--no-gmres", action="store_true", help="skip the single-system GMRES section")
     ap.add_argument("--serve-gmres-workers", type=int, default=6,
                     help="GMRES worker threads (own streams) of the serving loop")
-    ap.add_argument("--serve-gmres-batch-rows", type=int, default=262144,
+    ap.add_argument("--serve-gmres-batch-rows", type=int, default=131072,
                     help="GMRES requests of up to this many rows join the batched engine (larger: worker streams)")
     ap.add_argument("--serve-gmres-batch", type=int, default=1,
                     help="1: GMRES requests join the batched GMRES engine; 0: one worker stream per request")

@@ -225,7 +225,7 @@ int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, doubl
  *                  and in any mix in both forms.
  *   "serve_gmres_batch", "serve_gmres_batch_rows": GMRES requests of servers
  *                  created afterwards; 1 (default) = requests of up to
- *                  serve_gmres_batch_rows rows (default 262144) join one batched
+ *                  serve_gmres_batch_rows rows (default 131072) join one batched
  *                  GMRES engine (tg_gbatch_*), larger ones run on the worker
  *                  threads; 0 = every request on a worker thread.
  * Process-wide; not thread-safe against concurrent SpMM calls. */

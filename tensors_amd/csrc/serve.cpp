@@ -39,7 +39,7 @@ namespace tg {
 // reduction: right for many small systems, wrong for a system of 1e6 rows (977
 // tiles), which the single-system solver reduces in two levels.
 static int g_serve_gmres_batch = 1;
-static int g_serve_gmres_batch_rows = 262144;  // 256 tiles of the batched engine
+static int g_serve_gmres_batch_rows = 131072;  // 128 tiles of the batched engine
 
 int serve_tune_set(const char* key, int32_t value, bool* known) {
     const std::string k(key);

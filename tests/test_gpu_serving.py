@@ -49,7 +49,7 @@ def submit(srv, req):
 def gmres_mode(request):
     """Serving GMRES requests through the batched GMRES engine (every request here
     fits it), on per-request worker streams, or split by size (systems of up to 2000
-    rows batched, the larger ones on workers, as the default split does at 262144
+    rows batched, the larger ones on workers, as the default split does at 131072
     rows); restores the knobs afterwards."""
     keys = (b"serve_gmres_batch", b"serve_gmres_batch_rows")
     before = [int(capi.lib().tg_tune_get(k)) for k in keys]
