@@ -71,7 +71,9 @@ def run(sysl, dev, tol, max_restarts, order=None, staged=False):
 
 @pytest.mark.parametrize("cycles", [1, 2])
 def test_batch_gmres_fixed_cycles_match_oracle(cuda, cycles):
-    sysl = systems(9, gmax=40)
+    # restart lengths 5, 12 and 30, and two systems of 40 and 63 (the largest basis
+    # bucket of the step kernels) in the same batch
+    sysl = systems(9, gmax=40) + systems(2, seed=5105, gmax=40, ms=(40, 63))
     X, res = run(sysl, cuda, 0.0, cycles - 1)
     for i, (rp, ci, val, d, B, n, k, m) in enumerate(sysl):
         Xo = np.zeros((n, k))

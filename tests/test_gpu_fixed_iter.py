@@ -172,6 +172,33 @@ def test_gmres_fixed_cycles_match_oracle(cuda, gmres_delayed, m, cycles, side, p
     assert rel_err(gm.GetHessenbergMatrix().cpu().numpy(), H_o) <= TOL
 
 
+@pytest.mark.parametrize("side", [0, 1], ids=["left", "right"])
+def test_gmres_long_restart_falls_back_to_cgs2(cuda, side):
+    """Restart lengths above 32 take the CGS2 step whatever knob gmres_delayed says: the
+    two settings give bitwise the same solution, basis and Hessenberg matrix, and the
+    solution after two cycles of 40 steps matches the oracle at 1e-12."""
+    rp, ci, val = gen.convection_diffusion_2d(24, 10.0)
+    n, m, k = len(rp) - 1, 40, 2
+    B = gen.uniform(5300 + side, n * k).reshape(n, k)
+    Xo = np.zeros((n, k))
+    oracle.gmres_ex(n, rp, ci, val, B, Xo, k, m, 0.0, 1, side=side)
+    before = int(capi.lib().tg_tune_get(b"gmres_delayed"))
+    out = {}
+    try:
+        for mode in (1, 0):
+            capi.call("tg_tune_set", b"gmres_delayed", mode)
+            gm = GMRES(n, m, k, side=side)
+            Xd = torch.zeros(n, k, dtype=torch.float64, device=cuda)
+            gm(MatrixCSR(rp.astype(np.int32), ci, val, n, n), None, 1.0, dev(B, cuda), 0.0, Xd, 0.0, 1)
+            out[mode] = (Xd.cpu(), gm.GetBasis().cpu(), gm.GetHessenbergMatrix().cpu(), gm.IterationCount())
+    finally:
+        capi.call("tg_tune_set", b"gmres_delayed", before)
+    assert out[1][3] == out[0][3] == m
+    for a, b in zip(out[1][:3], out[0][:3]):
+        assert torch.equal(a, b)
+    assert rel_err(out[1][0].numpy(), Xo) <= TOL
+
+
 @pytest.mark.parametrize("steps", [1, 7, 25])
 def test_batch_cg_fixed_iterations_match_oracle(cuda, steps):
     """The batched engine's FP64 mode (the reference recurrences, Jacobi) after a fixed
