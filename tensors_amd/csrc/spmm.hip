@@ -1167,11 +1167,27 @@ __global__ __launch_bounds__(kBlock) void lr_segment_kernel(const int64_t* __res
     const int sg = blockIdx.x;
     const int64_t lb = seg_lb[sg], le = seg_le[sg];
     const int lc = threadIdx.x % CT, r = threadIdx.x / CT, R = kBlock / CT;
+    constexpr int U = 8;  // nonzeros per thread in flight: index loads first, then the gathers
     for (int c0 = 0; c0 < k; c0 += CT) {
         const int c = c0 + lc;
+        const int cs = c < k ? c : 0;
         T acc = T(0);
-        if (c < k)
-            for (int64_t l = lb + r; l < le; l += R) acc += (a ? a[l] : T(1)) * X[(int64_t)ci[l] * ldX + c];
+        for (int64_t l0 = lb + r; l0 < le; l0 += (int64_t)R * U) {
+            int32_t col[U];
+            T av[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int64_t l = l0 + (int64_t)u * R;
+                const int64_t ls = l < le ? l : lb;  // a valid entry; masked below
+                col[u] = ci[ls];
+                av[u] = (l < le && c < k) ? (a ? a[ls] : T(1)) : T(0);
+            }
+            T xv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) xv[u] = X[(int64_t)col[u] * ldX + cs];
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc += av[u] * xv[u];  // nonzero order, as one at a time
+        }
         sh[threadIdx.x] = acc;
         __syncthreads();
         for (int h = R / 2; h > 0; h >>= 1) {
