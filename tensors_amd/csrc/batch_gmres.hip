@@ -698,13 +698,36 @@ __global__ __launch_bounds__(kBlock) void gb_lsq_kernel(const GbTile* __restrict
     const GbState* s = st + T.slot;
     if (s->done || s->iters == 0) return;
     const int it = s->iters, m = S.m, k = S.k;
-    for (int cc = threadIdx.x; cc < k; cc += kBlock)
-        for (int i = it - 1; i >= 0; --i) {
-            // the delayed form keeps beta as the carried right-hand side: rotate it here
-            double v = S.delayed ? S.cs[i * k + cc] * S.beta[i * k + cc] : S.beta[i * k + cc];
-            for (int jj = i + 1; jj < it; ++jj) v -= S.H[((int64_t)i * m + jj) * k + cc] * S.y[jj * k + cc];
-            S.y[i * k + cc] = v / S.H[((int64_t)i * m + i) * k + cc];
+    // the triangle of H and the right-hand side staged in LDS (chunks of columns), then
+    // the serial substitution per column from LDS; plain row-oriented order
+    constexpr int kTri = 6144;
+    __shared__ double tri[kTri];
+    __shared__ double sb[(kGbMaxM + 1) * kGbMaxK];
+    const int ntri = it * (it + 1) / 2;
+    const int kc = max(1, min(k, kTri / ntri));
+    auto tix = [&](int i, int j) { return i * it - i * (i - 1) / 2 + (j - i); };  // i <= j
+    for (int c0 = 0; c0 < k; c0 += kc) {
+        const int nc = min(kc, k - c0);
+        for (int t = threadIdx.x; t < it * it * nc; t += blockDim.x) {
+            const int cc = t % nc, ij = t / nc, i = ij / it, j = ij % it;
+            if (j >= i) tri[tix(i, j) * nc + cc] = S.H[((int64_t)i * m + j) * k + c0 + cc];
         }
+        for (int t = threadIdx.x; t < it * nc; t += blockDim.x) {
+            const int i = t / nc, c = c0 + t % nc;
+            // the delayed form keeps beta as the carried right-hand side: rotate it here
+            sb[i * kGbMaxK + t % nc] = S.delayed ? S.cs[i * k + c] * S.beta[i * k + c] : S.beta[i * k + c];
+        }
+        __syncthreads();
+        for (int cc = threadIdx.x; cc < nc; cc += blockDim.x)
+            for (int i = it - 1; i >= 0; --i) {
+                double v = sb[i * kGbMaxK + cc];
+                for (int jj = i + 1; jj < it; ++jj) v -= tri[tix(i, jj) * nc + cc] * sb[jj * kGbMaxK + cc];
+                sb[i * kGbMaxK + cc] = v / tri[tix(i, i) * nc + cc];
+            }
+        __syncthreads();
+        for (int t = threadIdx.x; t < it * nc; t += blockDim.x) S.y[(t / nc) * k + c0 + t % nc] = sb[(t / nc) * kGbMaxK + t % nc];
+        __syncthreads();
+    }
 }
 
 // x -= Q y on the tile's rows (x = -Q y on a cycle that started from x = 0)
@@ -980,7 +1003,7 @@ class GmresBatch {
 #undef TG_GB_STEPS
             hipLaunchKernelGGL(gb_step4_kernel, g, b, 0, s, dtiles_, dsys_, dst_, j);
         }
-        hipLaunchKernelGGL(gb_lsq_kernel, g, dim3(kWave), 0, s, dtiles_, dsys_, dst_);
+        hipLaunchKernelGGL(gb_lsq_kernel, g, b, 0, s, dtiles_, dsys_, dst_);
         hipLaunchKernelGGL(gb_update_kernel, g, b, 0, s, dtiles_, dsys_, dst_);
         hipLaunchKernelGGL(gb_end_kernel, g, dim3(kWave), 0, s, dtiles_, dsys_, dst_);
     }

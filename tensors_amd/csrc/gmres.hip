@@ -672,15 +672,39 @@ __global__ void gm_start_kernel(int m, int k, const double* __restrict__ nn2, do
     }
 }
 
-// y = H^{-1} beta per column (upper back substitution, iter x iter)
-__global__ void gm_lsq_kernel(int m, int k, double* __restrict__ small, GmresLayout L, const GmresHdr* __restrict__ hd) {
+// y = H^{-1} beta per column (upper back substitution, iter x iter).  The triangle of
+// H and beta of a chunk of columns are first staged in LDS by the whole workgroup, so
+// the serial substitution of each column reads LDS only; the order of the
+// subtractions is the plain row-oriented one.
+constexpr int kLsqLds = 6144;  // doubles of staged triangle per chunk of columns
+__global__ __launch_bounds__(kBlock) void gm_lsq_kernel(int m, int k, double* __restrict__ small, GmresLayout L,
+                                                        const GmresHdr* __restrict__ hd) {
     const int it = hd->iter;
-    for (int c = threadIdx.x; c < k; c += blockDim.x) {
-        for (int i = it - 1; i >= 0; --i) {
-            double s = small[L.beta + i * k + c];
-            for (int j = i + 1; j < it; ++j) s -= small[L.H + ((int64_t)i * m + j) * k + c] * small[L.y + j * k + c];
-            small[L.y + i * k + c] = s / small[L.H + ((int64_t)i * m + i) * k + c];
+    if (it <= 0) return;
+    __shared__ double tri[kLsqLds];
+    __shared__ double sb[(kGmresMaxM + 1) * kGmresMaxCols];  // beta, then y, [row][chunk column]
+    const int ntri = it * (it + 1) / 2;
+    const int kc = max(1, min(k, kLsqLds / ntri));
+    auto tix = [&](int i, int j) { return i * it - i * (i - 1) / 2 + (j - i); };  // i <= j
+    for (int c0 = 0; c0 < k; c0 += kc) {
+        const int nc = min(kc, k - c0);
+        for (int t = threadIdx.x; t < it * it * nc; t += blockDim.x) {
+            const int cc = t % nc, ij = t / nc, i = ij / it, j = ij % it;
+            if (j >= i) tri[tix(i, j) * nc + cc] = small[L.H + ((int64_t)i * m + j) * k + c0 + cc];
         }
+        for (int t = threadIdx.x; t < it * nc; t += blockDim.x)
+            sb[(t / nc) * kGmresMaxCols + t % nc] = small[L.beta + (t / nc) * k + c0 + t % nc];
+        __syncthreads();
+        for (int cc = threadIdx.x; cc < nc; cc += blockDim.x)
+            for (int i = it - 1; i >= 0; --i) {
+                double s = sb[i * kGmresMaxCols + cc];
+                for (int j = i + 1; j < it; ++j) s -= tri[tix(i, j) * nc + cc] * sb[j * kGmresMaxCols + cc];
+                sb[i * kGmresMaxCols + cc] = s / tri[tix(i, i) * nc + cc];
+            }
+        __syncthreads();
+        for (int t = threadIdx.x; t < it * nc; t += blockDim.x)
+            small[L.y + (t / nc) * k + c0 + t % nc] = sb[(t / nc) * kGmresMaxCols + t % nc];
+        __syncthreads();
     }
 }
 
@@ -1098,7 +1122,7 @@ int gmres_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d,
         succ_out = hh.succ;
         res->iterations = hh.iter;
         if (hh.succ && hh.iter == 0) return TG_OK;  // converged before any step: solution untouched (:371-375)
-        hipLaunchKernelGGL(gm_lsq_kernel, dim3(1), dim3(64), 0, s, mm, k, w.small, L, (const GmresHdr*)w.hd);
+        hipLaunchKernelGGL(gm_lsq_kernel, dim3(1), dim3(kBlock), 0, s, mm, k, w.small, L, (const GmresHdr*)w.hd);
         if (side == 1 && prec_kind == 2) {
             // z = Q y;  x = P z;  solution -= x   (GMRES.hpp:416-480)
             hipLaunchKernelGGL((gm_synth_kernel<T>), dim3(ge), dim3(kBlock), 0, s, n, k, (const T*)Q, qstride,
