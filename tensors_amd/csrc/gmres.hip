@@ -309,30 +309,47 @@ __global__ __launch_bounds__(kBlock) void gm_ddots_kernel(int n, int k, int kpad
 #pragma unroll
     for (int r = 0; r < 2 * NQMAX + 2; ++r) acc[r] = 0.0;
     const T* U = Q + (int64_t)j * qstride;
+    // FP32: two rows per iteration, so the bytes in flight match FP64's (the FP64
+    // accumulators bound the waves per SIMD); the second row is clamped to a valid one
+    // and masked at the end of the range
+    constexpr int RU = sizeof(T) == 4 ? 2 : 1;
     if (c < k)
-        for (int r = i0 + ro; r < i1; r += rows) {
-            const int64_t e = (int64_t)r * k + c;
-            T qi[NQMAX];
+        for (int r = i0 + ro; r < i1; r += rows * RU) {
+            T qi[RU][NQMAX], uv[RU], vv[RU];
+            int rr[RU];
 #pragma unroll
-            for (int i = 0; i < NQMAX; ++i) qi[i] = Q[(int64_t)max(0, min(i, j - 1)) * qstride + e];  // loads first
-            const T uv = U[e];
-            T vv;
-            if (finish) {
-                vv = uv;
-            } else if (d) {
-                vv = d[r] * z[e];
-                vslot[e] = vv;
-            } else {
-                vv = vslot[e];
+            for (int u = 0; u < RU; ++u) {
+                rr[u] = r + u * rows < i1 ? r + u * rows : r;
+                const int64_t e = (int64_t)rr[u] * k + c;
+#pragma unroll
+                for (int i = 0; i < NQMAX; ++i) qi[u][i] = Q[(int64_t)max(0, min(i, j - 1)) * qstride + e];  // loads first
+                uv[u] = U[e];
             }
 #pragma unroll
-            for (int i = 0; i < NQMAX; ++i)
-                if (i < j) {
-                    acc[i] += (double)qi[i] * (double)uv;
-                    acc[NQMAX + i] += (double)qi[i] * (double)vv;
+            for (int u = 0; u < RU; ++u) {
+                const int64_t e = (int64_t)rr[u] * k + c;
+                const bool ok = r + u * rows < i1;
+                if (finish) {
+                    vv[u] = uv[u];
+                } else if (d) {
+                    vv[u] = d[rr[u]] * z[e];
+                    if (ok) vslot[e] = vv[u];
+                } else {
+                    vv[u] = vslot[e];
                 }
-            acc[2 * NQMAX] += (double)uv * (double)uv;
-            acc[2 * NQMAX + 1] += (double)uv * (double)vv;
+                if (!ok) uv[u] = vv[u] = T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < RU; ++u) {
+#pragma unroll
+                for (int i = 0; i < NQMAX; ++i)
+                    if (i < j) {
+                        acc[i] += (double)qi[u][i] * (double)uv[u];
+                        acc[NQMAX + i] += (double)qi[u][i] * (double)vv[u];
+                    }
+                acc[2 * NQMAX] += (double)uv[u] * (double)uv[u];
+                acc[2 * NQMAX + 1] += (double)uv[u] * (double)vv[u];
+            }
         }
     dd_partial<NQMAX>(acc, j, k, kpad, partial);
 }
