@@ -409,7 +409,7 @@ __global__ __launch_bounds__(kBlock) void gb_step4_kernel(const GbTile* __restri
     }
     if (T.t != 0) return;  // block-uniform
     const int m = S.m;
-    if (threadIdx.x < k) {
+    if ((int)threadIdx.x < k) {
         const int cc = threadIdx.x;
         auto Hx = [&](int r) -> double& { return S.H[((int64_t)r * m + j) * k + cc]; };
         Hx(j + 1) = sqrt(tot[cc]);
@@ -548,7 +548,7 @@ __global__ __launch_bounds__(kBlock) void gb_dstep2_kernel(const GbTile* __restr
     auto a = [&](int i, int cc) { return tv[i * kGbMaxK + cc]; };
     auto b = [&](int i, int cc) { return tv[(j + i) * kGbMaxK + cc]; };
     if (threadIdx.x == 0) conv_s = 1;
-    if (threadIdx.x < k) {
+    if ((int)threadIdx.x < k) {
         const int cc = threadIdx.x;
         double rho = 1.0, ab = 0.0;
         if (j > 0) {
@@ -568,7 +568,7 @@ __global__ __launch_bounds__(kBlock) void gb_dstep2_kernel(const GbTile* __restr
         ab_s[cc] = ab;
     }
     __syncthreads();
-    if (j > 0 && threadIdx.x < k) {
+    if (j > 0 && (int)threadIdx.x < k) {
         // Givens on column j - 1 (GMRES.hpp:603-658) with the rotations of earlier launches
         const int cc = threadIdx.x;
         for (int l = 0; l <= j; ++l) colr[l * kGbMaxK + cc] = colc[l * kGbMaxK + cc];
@@ -813,7 +813,7 @@ int32_t gbatch_tune_get(const char* key) {
 
 class GmresBatch {
   public:
-    explicit GmresBatch(hipStream_t s) : s_(s), delayed_(g_gb_delayed) {}
+    explicit GmresBatch(hipStream_t s) : delayed_(g_gb_delayed), s_(s) {}
     ~GmresBatch() { release(); }
 
     int add(int n, int64_t nnz, const int32_t* rp, const int32_t* ci, const double* val, const double* d, int k,
