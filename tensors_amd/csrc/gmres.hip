@@ -910,7 +910,7 @@ template <typename T>
 int gmres_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, const OpRef& Pop, int side,
                   int32_t k, int32_t max_iter, double a, const T* Bm, int64_t ldB, double b, T* X, int64_t ldX,
                   double tol, int32_t max_restarts, int use_guess, void* ws, size_t ws_bytes, GmresResult* res,
-                  hipStream_t s) {
+                  hipStream_t s, GmresReuse* reuse) {
     const int n = A.n;
     res->succeeded = 0;
     res->iterations = 0;
@@ -969,21 +969,23 @@ int gmres_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d,
                        w.hd);
     TG_LAUNCH_CHECK();
     std::vector<double> bn(k);
-    TG_HIP(hipMemcpyAsync(bn.data(), w.small + L.bnorm, k * sizeof(double), hipMemcpyDeviceToHost, s));
-    TG_HIP(hipStreamSynchronize(s));
-    double bmax = 0.0;
-    for (int c = 0; c < k; ++c) {
-        if (std::isnan(bn[c])) {  // GMRES.hpp:193-204
-            fail(TG_OK, "gmres: right-hand side contains NaNs. Doing nothing.");
+    if (!(reuse && reuse->checked)) {
+        TG_HIP(hipMemcpyAsync(bn.data(), w.small + L.bnorm, k * sizeof(double), hipMemcpyDeviceToHost, s));
+        TG_HIP(hipStreamSynchronize(s));
+        double bmax = 0.0;
+        for (int c = 0; c < k; ++c) {
+            if (std::isnan(bn[c])) {  // GMRES.hpp:193-204
+                fail(TG_OK, "gmres: right-hand side contains NaNs. Doing nothing.");
+                return TG_OK;
+            }
+            bmax = std::max(bmax, bn[c]);
+        }
+        if (!(bmax > 0.0)) {  // :206-229
+            hipLaunchKernelGGL((gm_zero_or_scale_kernel<T>), dim3(ge), dim3(kBlock), 0, s, n, k, (T)b, X, ldX);
+            TG_LAUNCH_CHECK();
+            res->succeeded = 1;
             return TG_OK;
         }
-        bmax = std::max(bmax, bn[c]);
-    }
-    if (!(bmax > 0.0)) {  // :206-229
-        hipLaunchKernelGGL((gm_zero_or_scale_kernel<T>), dim3(ge), dim3(kBlock), 0, s, n, k, (T)b, X, ldX);
-        TG_LAUNCH_CHECK();
-        res->succeeded = 1;
-        return TG_OK;
     }
     if (use_guess) hipLaunchKernelGGL((gm_load_kernel<T>), dim3(ge), dim3(kBlock), 0, s, n, k, (const T*)X, ldX, sol);
     else hipLaunchKernelGGL((gm_zero_kernel<T>), dim3(ge), dim3(kBlock), 0, s, N, sol);
@@ -1052,14 +1054,9 @@ int gmres_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d,
     };
     auto step = [&](hipStream_t st, int j) -> int { return delayed ? delayed_step(st, j, false) : arnoldi_step(st, j); };
     auto finish_cycle = [&](hipStream_t st) -> int { return delayed ? delayed_step(st, m, true) : TG_OK; };
-    struct CycleGraph {
-        hipStream_t cs = nullptr;
-        hipGraphExec_t ex = nullptr;
-        ~CycleGraph() {
-            if (ex) (void)hipGraphExecDestroy(ex);
-            if (cs) (void)hipStreamDestroy(cs);
-        }
-    } cyc;
+    // the captured cycle: this call's own, or the caller's (kept across calls)
+    GmresReuse own;
+    GmresReuse& cyc = reuse ? *reuse : own;
     int cycles = 0;
 
     auto solve = [&](bool implicit_zero, int& succ_out) -> int {
@@ -1091,10 +1088,10 @@ int gmres_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d,
                            (const double*)(w.small + L.inv), (const int*)nullptr);
         TG_LAUNCH_CHECK();
         GmresHdr hh{};
-        if (!host_ops && g_gmres_graph && cycles > 0) {
-            // every cycle after the first: its m steps as one captured graph (captured
-            // at the second cycle, replayed after that; steps after convergence return
-            // at once on the device flag)
+        if (!host_ops && g_gmres_graph && (cycles > 0 || reuse)) {
+            // every cycle after the first (with a caller's reuse state, every cycle): its
+            // m steps as one captured graph (captured once, replayed after that; steps
+            // after convergence return at once on the device flag)
             if (!cyc.ex) {
                 if (!cyc.cs) TG_HIP(hipStreamCreateWithFlags(&cyc.cs, hipStreamNonBlocking));
                 TG_HIP(hipStreamBeginCapture(cyc.cs, hipStreamCaptureModeThreadLocal));
@@ -1174,6 +1171,8 @@ int gmres_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d,
     TG_LAUNCH_CHECK();
     std::vector<double> beta((size_t)(mm + 1) * k);
     TG_HIP(hipMemcpyAsync(beta.data(), w.small + L.beta, beta.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (reuse && reuse->checked)  // the norms were not read before the cycles
+        TG_HIP(hipMemcpyAsync(bn.data(), w.small + L.bnorm, k * sizeof(double), hipMemcpyDeviceToHost, s));
     TG_HIP(hipStreamSynchronize(s));
     res->succeeded = succ;
     res->restarts = restarts;
@@ -1183,10 +1182,10 @@ int gmres_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d,
 
 template int gmres_solve_t<double>(const CsrView&, const OpRef&, int, const double*, const OpRef&, int, int32_t,
                                    int32_t, double, const double*, int64_t, double, double*, int64_t, double, int32_t,
-                                   int, void*, size_t, GmresResult*, hipStream_t);
+                                   int, void*, size_t, GmresResult*, hipStream_t, GmresReuse*);
 template int gmres_solve_t<float>(const CsrView&, const OpRef&, int, const float*, const OpRef&, int, int32_t,
                                   int32_t, double, const float*, int64_t, double, float*, int64_t, double, int32_t, int,
-                                  void*, size_t, GmresResult*, hipStream_t);
+                                  void*, size_t, GmresResult*, hipStream_t, GmresReuse*);
 
 int gmres_tune_set(const char* key, int32_t value, bool* known) {
     const std::string k = key;

@@ -132,11 +132,29 @@ struct GmresResult {
 
 size_t gmres_workspace_bytes(int32_t n, int32_t max_iter, int32_t k, size_t vs);
 
+// State a caller keeps across several gmres_solve_t calls on the same operator,
+// workspace, stream and shapes (the mixed solver's one-cycle inner solves): the
+// captured restart cycle, reused from the first call's first cycle on, and `checked`,
+// which says the right-hand side is known finite and non-zero, so the call neither
+// reads its norms back before the cycle nor handles those special cases.
+struct GmresReuse {
+    hipStream_t cs = nullptr;
+    hipGraphExec_t ex = nullptr;
+    bool checked = false;
+    GmresReuse() = default;
+    GmresReuse(const GmresReuse&) = delete;
+    GmresReuse& operator=(const GmresReuse&) = delete;
+    ~GmresReuse() {
+        if (ex) (void)hipGraphExecDestroy(ex);
+        if (cs) (void)hipStreamDestroy(cs);
+    }
+};
+
 template <typename T>
 int gmres_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, const OpRef& Pop, int side,
                   int32_t k, int32_t max_iter, double a, const T* Bm, int64_t ldB, double b, T* X, int64_t ldX,
                   double tol, int32_t max_restarts, int use_guess, void* ws, size_t ws_bytes, GmresResult* res,
-                  hipStream_t s);
+                  hipStream_t s, GmresReuse* reuse = nullptr);
 
 size_t cg_workspace_bytes(int32_t n, int32_t k, size_t vs, int prec_kind);
 

@@ -627,6 +627,10 @@ static int gmres_mixed(const CsrView& A64, const float* a32_in, int prec_kind, c
     bool ok = false;
     int cycles = 0;
     bool x_zero = !use_guess;
+    // the FP32 cycles: one captured cycle for all of them, and no read-back of the
+    // right-hand side's norms before each (r is checked finite and non-zero here)
+    GmresReuse reuse;
+    reuse.checked = true;
     for (;;) {
         int rc = residual(x_zero);
         if (rc) return rc;
@@ -647,7 +651,8 @@ static int gmres_mixed(const CsrView& A64, const float* a32_in, int prec_kind, c
         const double tol_in = std::max(0.5 * need, 1e-6);
         GmresResult gr{};
         rc = gmres_solve_t<float>(A32, A32op, prec_kind, (const float*)w.diag32, none, side, k, m, 1.0,
-                                  (const float*)w.r32, k, 0.0, w.d32, k, tol_in, 0, 0, w.inner, w.inner_bytes, &gr, s);
+                                  (const float*)w.r32, k, 0.0, w.d32, k, tol_in, 0, 0, w.inner, w.inner_bytes, &gr, s,
+                                  &reuse);
         if (rc) return rc;
         res->inner_total += gr.iterations;
         hipLaunchKernelGGL(mx_update_kernel, dim3(grid_m(N)), dim3(kBlock), 0, s, N, (const float*)w.d32, w.x64);
