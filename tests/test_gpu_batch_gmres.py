@@ -74,6 +74,12 @@ def test_batch_gmres_fixed_cycles_match_oracle(cuda, cycles):
     # restart lengths 5, 12 and 30, and two systems of 40 and 63 (the largest basis
     # bucket of the step kernels) in the same batch
     sysl = systems(9, gmax=40) + systems(2, seed=5105, gmax=40, ms=(40, 63))
+    # and blocks of 5 and 8 right-hand sides (eight column lanes in the reductions)
+    for kk, g in ((5, 17), (8, 23)):
+        rp, ci, val = gen.convection_diffusion_2d(g, 7.0)
+        n = g * g
+        d = 1.0 / sp.csr_matrix((val, ci, rp), shape=(n, n)).diagonal() if kk == 8 else None
+        sysl.append((rp.astype(np.int32), ci, val, d, gen.uniform(5200 + kk, n * kk).reshape(n, kk), n, kk, 12))
     X, res = run(sysl, cuda, 0.0, cycles - 1)
     for i, (rp, ci, val, d, B, n, k, m) in enumerate(sysl):
         Xo = np.zeros((n, k))

@@ -172,6 +172,28 @@ def test_gmres_fixed_cycles_match_oracle(cuda, gmres_delayed, m, cycles, side, p
     assert rel_err(gm.GetHessenbergMatrix().cpu().numpy(), H_o) <= TOL
 
 
+@pytest.mark.parametrize("k", [2, 8, 16])
+def test_gmres_fixed_cycles_wide_blocks(cuda, gmres_delayed, k):
+    """Blocks of 2, 8 and 16 right-hand sides (two, eight and sixteen column lanes in the
+    reductions over a wave) against the oracle after two cycles of 12 steps, left Jacobi."""
+    rp, ci, val = gen.convection_diffusion_2d(24, 10.0)
+    n, m, cycles = len(rp) - 1, 12, 2
+    d = 1.0 / sp.csr_matrix((val, ci, rp), shape=(n, n)).diagonal()
+    B = gen.uniform(5400 + k, n * k).reshape(n, k)
+    Xo = np.zeros((n, k))
+    ok_o, it_o, rs_o, rr_o, Q_o, H_o, beta_o, sol_o = oracle.gmres_ex(
+        n, rp, ci, val, B, Xo, k, m, 0.0, cycles - 1, prec_diag=d, side=0)
+    gm = GMRES(n, m, k, side=0)
+    Xd = torch.zeros(n, k, dtype=torch.float64, device=cuda)
+    gm(MatrixCSR(rp.astype(np.int32), ci, val, n, n), dev(d, cuda), 1.0, dev(B, cuda), 0.0, Xd, 0.0, cycles - 1)
+    print(f"GMRES delayed {gmres_delayed} k {k}")
+    assert gm.IterationCount() == it_o == m and gm.RestartCount() == rs_o == cycles - 1
+    assert rel_err(Xd.cpu().numpy(), Xo) <= TOL
+    assert res_err(gm.RelativeResiduals(), rr_o) <= TOL
+    assert rel_err(gm.GetBasis().cpu().numpy(), Q_o) <= TOL
+    assert rel_err(gm.GetHessenbergMatrix().cpu().numpy(), H_o) <= TOL
+
+
 @pytest.mark.parametrize("side", [0, 1], ids=["left", "right"])
 def test_gmres_long_restart_falls_back_to_cgs2(cuda, side):
     """Restart lengths above 32 take the CGS2 step whatever knob gmres_delayed says: the
