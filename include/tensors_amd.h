@@ -199,6 +199,11 @@ int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, doubl
  *                  every workgroup taking the same offsets of each panel, so a row's
  *                  neighbours one band away are reused from the L2), 0 = each
  *                  workgroup a contiguous chunk.  Results differ by rounding only.
+ *   "cg_graph":    tg_cg_solve and the mixed CG with device operators, 1 = after
+ *                  the first batch, blocks of 16 iterations run as one captured
+ *                  hipGraph (default; the CSR iteration forms), 0 = eager launches.
+ *                  Bitwise the same results.  tg_cg_solve_noalloc always launches
+ *                  eagerly (it creates no stream or graph).
  *   "gmres_graph": tg_gmres_solve with device operators, 1 = every restart cycle
  *                  after the first runs as one captured hipGraph (default), 0 =
  *                  eager launches.  Bitwise the same results.

@@ -566,16 +566,18 @@ size_t cg_workspace_bytes(int32_t n, int32_t k, size_t vs, int prec_kind) {
 // always take form 0.
 enum CgForm : int { kFourKernels = 0, kSpmmDot = 1, kFormDir = 2, kFormAuto = 3 };
 static int g_cg_fused = kFormAuto;
+static int g_cg_graph = 1;              // 1: blocks of kCgGraphIters iterations as captured hipGraphs
+constexpr int kCgGraphIters = 16;
 // rows of the forms' SpMM kernel: 1 = panel order where the band allows, 0 = contiguous chunks
 static int g_cg_panel = 1;
 
 int cg_tune_set(const char* key, int32_t value, bool* known) {
     const std::string k(key);
-    *known = k == "cg_fused" || k == "cg_panel";
+    *known = k == "cg_fused" || k == "cg_panel" || k == "cg_graph";
     if (!*known) return TG_OK;
-    if (k == "cg_panel") {
-        if (value != 0 && value != 1) return fail(TG_ERR_INVALID, "tg_tune_set: cg_panel must be 0 or 1");
-        g_cg_panel = value;
+    if (k == "cg_panel" || k == "cg_graph") {
+        if (value != 0 && value != 1) return fail(TG_ERR_INVALID, "tg_tune_set: %s must be 0 or 1", key);
+        (k == "cg_panel" ? g_cg_panel : g_cg_graph) = value;
         return TG_OK;
     }
     if (value < 0 || value > 3) return fail(TG_ERR_INVALID, "tg_tune_set: cg_fused must be 0, 1, 2 or 3 (auto)");
@@ -590,6 +592,7 @@ int32_t cg_tune_get(const char* key) {
     const std::string k(key);
     if (k == "cg_fused") return g_cg_fused;
     if (k == "cg_panel") return g_cg_panel;
+    if (k == "cg_graph") return g_cg_graph;
     return -1;
 }
 
@@ -662,7 +665,8 @@ static int grid_for(int64_t count) {
 template <typename T>
 int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, const OpRef& Pop, int32_t k,
                int32_t max_iter, double a, const T* Bm, int64_t ldB, double b, T* X, int64_t ldX, double tol,
-               int use_guess, void* ws, size_t ws_bytes, CGResult* res, hipStream_t s, const CGHook* hook) {
+               int use_guess, void* ws, size_t ws_bytes, CGResult* res, hipStream_t s, const CGHook* hook,
+               bool allow_graphs) {
     const int n = A.n;
     res->succeeded = 0;
     res->iterations = 0;
@@ -711,11 +715,11 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
     int panel = 0;
     if (form != kFourKernels && g_cg_panel && Ac.band >= 4 * (int64_t)kFusedBlocks && (int64_t)n >= 4 * Ac.band)
         panel = (int)Ac.band;
-    auto dir_spmm = [&]() -> int {
+    auto dir_spmm_on = [&](hipStream_t ss) -> int {
         if (Ac.rp_type == TG_I64)
-            return launch_dir<T, int64_t>(form, s, n, k, (const int64_t*)Ac.rp, Ac.ci, (const T*)Ac.val, r, dd, p,
+            return launch_dir<T, int64_t>(form, ss, n, k, (const int64_t*)Ac.rp, Ac.ci, (const T*)Ac.val, r, dd, p,
                                           p1, u, partial, st, panel);
-        return launch_dir<T, int32_t>(form, s, n, k, (const int32_t*)Ac.rp, Ac.ci, (const T*)Ac.val, r, dd, p, p1,
+        return launch_dir<T, int32_t>(form, ss, n, k, (const int32_t*)Ac.rp, Ac.ci, (const T*)Ac.val, r, dd, p, p1,
                                       u, partial, st, panel);
     };
 
@@ -795,44 +799,86 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
             TG_HIP(hipStreamSynchronize(s));
         }
     }
-    while (!hflag[1] && launched < max_iter) {
-        for (int it = 0; it < batch && launched < max_iter && fused; ++it, ++launched) {
-            rc = dir_spmm();
-            if (rc) break;
-            hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kFinalThreads), 0, s, k, kpad, kFusedBlocks,
+    // One iteration's launches on stream ss (every kernel returns at once when `done` is set).
+    auto one_iteration = [&](hipStream_t ss) -> int {
+        if (fused) {
+            int rc1 = dir_spmm_on(ss);
+            if (rc1) return rc1;
+            hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kFinalThreads), 0, ss, k, kpad, kFusedBlocks,
                                (const double*)partial, st);
             if (dd)
-                hipLaunchKernelGGL((cg_step_kernel<T, true>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, (const T*)p,
+                hipLaunchKernelGGL((cg_step_kernel<T, true>), dim3(B), dim3(kBlock), 0, ss, n, k, kpad, (const T*)p,
                                    (const T*)p1, (const T*)u, dd, x, r, partial, (const CGState*)st);
             else
-                hipLaunchKernelGGL((cg_step_kernel<T, false>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, (const T*)p,
+                hipLaunchKernelGGL((cg_step_kernel<T, false>), dim3(B), dim3(kBlock), 0, ss, n, k, kpad, (const T*)p,
                                    (const T*)p1, (const T*)u, dd, x, r, partial, (const CGState*)st);
-            hipLaunchKernelGGL(cg_rho_kernel, dim3(1), dim3(kFinalThreads), 0, s, k, kpad, B, (const double*)partial, st);
+            hipLaunchKernelGGL(cg_rho_kernel, dim3(1), dim3(kFinalThreads), 0, ss, k, kpad, B, (const double*)partial,
+                               st);
+            return TG_OK;
         }
-        for (int it = 0; it < batch && launched < max_iter && !fused; ++it, ++launched) {
-            if (form == kSpmmDot) {  // u = A p and partial(p.u) in one kernel
-                rc = dir_spmm();
-                if (rc) break;
-                hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kFinalThreads), 0, s, k, kpad, kFusedBlocks,
-                                   (const double*)partial, st);
-            } else {
-                rc = op_apply(Aop, p, u, k, &st->done, s);
-                if (rc) break;
-                hipLaunchKernelGGL((cg_dot_pu_kernel<T>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, (const T*)p,
-                                   (const T*)u, partial, (const CGState*)st);
-                hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kFinalThreads), 0, s, k, kpad, B,
-                                   (const double*)partial, st);
-            }
-            hipLaunchKernelGGL((cg_update_kernel<T>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, (const T*)u, dd, r,
-                               z, zmode, partial, (const CGState*)st);
-            if (zmode == 2) {
-                rc = apply_P();
-                if (rc) break;
-            }
-            hipLaunchKernelGGL(cg_rho_kernel, dim3(1), dim3(kFinalThreads), 0, s, k, kpad, B, (const double*)partial, st);
-            hipLaunchKernelGGL((cg_p_kernel<T>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, (const T*)z, p, x,
-                               (const CGState*)st);
+        if (form == kSpmmDot) {  // u = A p and partial(p.u) in one kernel
+            int rc1 = dir_spmm_on(ss);
+            if (rc1) return rc1;
+            hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kFinalThreads), 0, ss, k, kpad, kFusedBlocks,
+                               (const double*)partial, st);
+        } else {
+            int rc1 = op_apply(Aop, p, u, k, &st->done, ss);
+            if (rc1) return rc1;
+            hipLaunchKernelGGL((cg_dot_pu_kernel<T>), dim3(B), dim3(kBlock), 0, ss, n, k, kpad, (const T*)p,
+                               (const T*)u, partial, (const CGState*)st);
+            hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kFinalThreads), 0, ss, k, kpad, B,
+                               (const double*)partial, st);
         }
+        hipLaunchKernelGGL((cg_update_kernel<T>), dim3(B), dim3(kBlock), 0, ss, n, k, kpad, (const T*)u, dd, r, z,
+                           zmode, partial, (const CGState*)st);
+        if (zmode == 2) {
+            int rc1 = apply_P();
+            if (rc1) return rc1;
+        }
+        hipLaunchKernelGGL(cg_rho_kernel, dim3(1), dim3(kFinalThreads), 0, ss, k, kpad, B, (const double*)partial, st);
+        hipLaunchKernelGGL((cg_p_kernel<T>), dim3(B), dim3(kBlock), 0, ss, n, k, kpad, (const T*)z, p, x,
+                           (const CGState*)st);
+        return TG_OK;
+    };
+    // Blocks of kCgGraphIters iterations as one captured hipGraph (the CSR forms, whose
+    // launches never allocate), captured after the first batch and replayed; batches
+    // run as whole blocks where they can, the rest eagerly.  The same kernels in the
+    // same order as the eager launches.
+    const bool graphs = allow_graphs && g_cg_graph && form != kFourKernels && !host_ops &&
+                        (!hook || std::max(1, hook->every) % kCgGraphIters == 0);
+    struct IterGraph {
+        hipStream_t cs = nullptr;
+        hipGraphExec_t ex = nullptr;
+        ~IterGraph() {
+            if (ex) (void)hipGraphExecDestroy(ex);
+            if (cs) (void)hipStreamDestroy(cs);
+        }
+    } ig;
+    while (!hflag[1] && launched < max_iter) {
+        int todo = std::min(batch, max_iter - launched);
+        if (graphs && launched >= 4 && todo >= kCgGraphIters) {
+            if (!ig.ex) {
+                if (!ig.cs) TG_HIP(hipStreamCreateWithFlags(&ig.cs, hipStreamNonBlocking));
+                TG_HIP(hipStreamBeginCapture(ig.cs, hipStreamCaptureModeThreadLocal));
+                for (int it = 0; it < kCgGraphIters && rc == TG_OK; ++it) rc = one_iteration(ig.cs);
+                hipGraph_t g = nullptr;
+                const hipError_t ec = hipStreamEndCapture(ig.cs, &g);
+                if (rc) {
+                    if (g) (void)hipGraphDestroy(g);
+                    break;
+                }
+                if (ec != hipSuccess) return fail(TG_ERR_HIP, "cg: graph capture failed: %s", hipGetErrorString(ec));
+                const hipError_t ei = hipGraphInstantiate(&ig.ex, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (ei != hipSuccess) {
+                    ig.ex = nullptr;
+                    return fail(TG_ERR_HIP, "cg: graph instantiation failed: %s", hipGetErrorString(ei));
+                }
+            }
+            for (; todo >= kCgGraphIters; todo -= kCgGraphIters, launched += kCgGraphIters)
+                TG_HIP(hipGraphLaunch(ig.ex, s));
+        }
+        for (; todo > 0 && rc == TG_OK; --todo, ++launched) rc = one_iteration(s);
         if (rc) break;
         if (!fused) {  // x complete before the host looks at the state
             hipLaunchKernelGGL((cg_xfinish_kernel<T>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, (const T*)p, x,
@@ -882,10 +928,10 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
 
 template int cg_solve_t<double>(const CsrView&, const OpRef&, int, const double*, const OpRef&, int32_t, int32_t,
                                 double, const double*, int64_t, double, double*, int64_t, double, int, void*, size_t,
-                                CGResult*, hipStream_t, const CGHook*);
+                                CGResult*, hipStream_t, const CGHook*, bool);
 template int cg_solve_t<float>(const CsrView&, const OpRef&, int, const float*, const OpRef&, int32_t, int32_t,
                                double, const float*, int64_t, double, float*, int64_t, double, int, void*, size_t,
-                               CGResult*, hipStream_t, const CGHook*);
+                               CGResult*, hipStream_t, const CGHook*, bool);
 
 }  // namespace tg
 

@@ -173,9 +173,11 @@ struct CGHook {
         fn;
 };
 
+// allow_graphs = false keeps every iteration an eager launch: no stream or graph is
+// created (the allocation-free entry point).
 template <typename T>
 int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, const OpRef& Pop, int32_t k, int32_t max_iter, double a, const T* Bm,
                int64_t ldB, double b, T* X, int64_t ldX, double tol, int use_guess, void* ws, size_t ws_bytes,
-               CGResult* res, hipStream_t s, const CGHook* hook = nullptr);
+               CGResult* res, hipStream_t s, const CGHook* hook = nullptr, bool allow_graphs = true);
 
 }  // namespace tg

@@ -65,7 +65,8 @@ static int noalloc_entry(const CsrView& A, int32_t prec_kind, const void* prec_d
     OpRef Pop;
     CGResult res{};
     const int rc = cg_solve_t<T>(A, Aop, prec_kind, (const T*)prec_diag, Pop, nrhs, max_iter, a_coef, (const T*)B,
-                                 ldB, b_coef, (T*)X, ldX, tol, use_guess, ws, kWorkspaceBytes, &res, s);
+                                 ldB, b_coef, (T*)X, ldX, tol, use_guess, ws, kWorkspaceBytes, &res, s, nullptr,
+                                 /*allow_graphs=*/false);
     if (rc) return rc;
     if (succeeded) *succeeded = res.succeeded;
     if (iterations) *iterations = res.iterations;

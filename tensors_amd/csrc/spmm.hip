@@ -1051,7 +1051,7 @@ extern "C" int32_t tg_tune_get(const char* key) {
     if (k == "spmm_split") return g_spmm_split;
     if (k == "spmm_long_row") return g_spmm_long_row;
     if (k == "spmm_rows") return g_spmm_rows;
-    if (k == "cg_fused" || k == "cg_panel") return cg_tune_get(key);
+    if (k == "cg_fused" || k == "cg_panel" || k == "cg_graph") return cg_tune_get(key);
     if (k == "bsr_vector") return bsr_tune_get(key);
     if (k == "gmres_graph" || k == "gmres_delayed") return gmres_tune_get(key);
     if (k == "gbatch_big_tiles" || k == "gbatch_delayed") return gbatch_tune_get(key);

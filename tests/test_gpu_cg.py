@@ -9,7 +9,7 @@ import oracle
 from helpers import rel_err
 from tensors_amd import _capi as capi
 from tensors_amd import generators as gen
-from tensors_amd.solvers import ConjugateGradient
+from tensors_amd.solvers import ConjugateGradient, cg_solve_mixed
 from tensors_amd.sparse import MatrixCSR
 
 pytestmark = pytest.mark.gpu
@@ -192,6 +192,44 @@ def test_cg_fused_iteration_matches_four_kernel_path(cuda, fused_knob, form, k, 
         ok_o, it_o, _ = oracle.cg(n, rp, ci, val, B, Xo, k, 1000, tol, prec_diag=d)
         assert ok_o == 1 and abs(it1 - it_o) <= 1
         assert rel_err(X1, Xo) < 1e-8
+
+
+@pytest.mark.parametrize("form", [1, 2])
+@pytest.mark.parametrize("k", [1, 8])
+@pytest.mark.parametrize("prec", [False, True])
+def test_cg_graph_blocks_equal_eager_launches(cuda, fused_knob, form, k, prec):
+    """Knob cg_graph: after the first batch, blocks of 16 iterations run as one captured
+    hipGraph.  The same kernels in the same order as eager launches, so the solution, the
+    iteration count and the residuals are bitwise the same, including a solve that stops
+    inside a block and one that runs into max_iter; and the mixed solver (host hook every
+    16 iterations) likewise."""
+    rp, ci, val, n = grid_system(40, 0.05)
+    B = gen.uniform(977 + k, n * k).reshape(n, k)
+    d = 1.0 / sp.csr_matrix((val, ci, rp), shape=(n, n)).diagonal() if prec else None
+    A = MatrixCSR(rp, ci, val, n, n)
+    fused_knob(form)
+    before = int(capi.lib().tg_tune_get(b"cg_graph"))
+    out = {}
+    try:
+        for mode in (1, 0):
+            capi.call("tg_tune_set", b"cg_graph", mode)
+            res = []
+            for max_iter, tol in ((2000, 1e-10), (45, 1e-14)):
+                cg = ConjugateGradient(n, max_iter, k)
+                Xd = torch.zeros(n, k, dtype=torch.float64, device=cuda)
+                ok = cg(A, None if d is None else dev(d, cuda), 1.0, dev(B, cuda), 0.0, Xd, tol)
+                res.append((ok, cg.IterationCount(), cg.RelativeResiduals().tolist(), Xd.cpu()))
+            Xm = torch.zeros(n, k, dtype=torch.float64, device=cuda)
+            mixed = cg_solve_mixed(A, dev(B, cuda), Xm, tol=1e-10, P=None if d is None else dev(d, cuda))
+            res.append((mixed[:3], Xm.cpu()))
+            out[mode] = res
+    finally:
+        capi.call("tg_tune_set", b"cg_graph", before)
+    (a1, a2, am), (b1, b2, bm) = out[1], out[0]
+    assert a1[0] and a1[1] > 40 and not a2[0] and a2[1] == 45
+    for x, y in ((a1, b1), (a2, b2)):
+        assert x[:3] == y[:3] and torch.equal(x[3], y[3])
+    assert am[0] == bm[0] and torch.equal(am[1], bm[1])
 
 
 @pytest.mark.parametrize("n,k", [(1, 1), (1, 4), (2, 3), (3, 8)])
