@@ -193,7 +193,8 @@ int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, doubl
  *                  with P = I or Jacobi and mean row length <= 16: 0 = SpMM then
  *                  four vector kernels, 1 = p.u in the SpMM epilogue, 2 = p
  *                  update, SpMM and p.u in one kernel plus one vector kernel,
- *                  3 = default (form 1).  Results agree to rounding.
+ *                  3 = default (form 2 for blocks of up to 2^20 entries
+ *                  n * nrhs, form 1 above).  Results agree to rounding.
  *   "cg_panel":    rows of that iteration's SpMM kernel, 1 = panel order where
  *                  the band allows (default: the rows cut into panels of one band,
  *                  every workgroup taking the same offsets of each panel, so a row's

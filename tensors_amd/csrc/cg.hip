@@ -560,10 +560,11 @@ size_t cg_workspace_bytes(int32_t n, int32_t k, size_t vs, int prec_kind) {
 // Iteration form for CSR operators with P = I or Jacobi, short rows (mean length
 // <= 16) and no column slabs ("cg_fused"): 0 = four vector kernels after the
 // SpMM; 1 = p.u in the SpMM epilogue (kSpmmDot); 2 = direction, SpMM and p.u in
-// one kernel (kFormDir); 3 = default (kSpmmDot).  On the 128^3 Laplacian with 32
-// right-hand sides form 1 measured fastest in FP64 and FP32: gathering both z and
-// p_prev per nonzero (form 2) cost more than streaming p once.  Other operators
-// always take form 0.
+// one kernel (kFormDir); 3 = default (form 2 up to 2^20 entries n * nrhs, form 1
+// above).  On the 128^3 Laplacian with 32 right-hand sides form 1 measured fastest in
+// FP64 and FP32: gathering both z and p_prev per nonzero (form 2) cost more than
+// streaming p once; on small blocks form 2's one launch less per iteration wins.  Other
+// operators always take form 0.
 enum CgForm : int { kFourKernels = 0, kSpmmDot = 1, kFormDir = 2, kFormAuto = 3 };
 static int g_cg_fused = kFormAuto;
 static int g_cg_graph = 1;              // 1: blocks of kCgGraphIters iterations as captured hipGraphs
@@ -706,7 +707,10 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
     const bool csr_ok = !host_ops && Aop.csr && n > 0 && Ac.m == n && Ac.n == n && Ac.nnz <= 16 * (int64_t)n &&
                         !spmm_uses_slabs(Ac, k) && Ac.rp && (Ac.nnz == 0 || Ac.ci) &&
                         Ac.val_type == (sizeof(T) == 8 ? TG_F64 : TG_F32);
-    int form = g_cg_fused == kFormAuto ? kSpmmDot : g_cg_fused;
+    // auto: form 2 (four launches per iteration) for blocks of up to 2^20 entries, where
+    // an iteration is bound by the kernels' fixed costs (22^3 and 45^3 Laplacians: 12 %
+    // faster), form 1 above (its SpMM gathers one vector instead of two)
+    int form = g_cg_fused == kFormAuto ? ((int64_t)n * k <= (int64_t(1) << 20) ? kFormDir : kSpmmDot) : g_cg_fused;
     if (!csr_ok) form = kFourKernels;
     const bool fused = form == kFormDir;
     // panel order for the SpMM kernel of forms 1 / 2 (cg_dir_spmm_kernel): panels of
