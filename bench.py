@@ -66,6 +66,8 @@ def parse():
     ap.add_argument("--cg-fused", type=int, default=-1,
                     help="CG iteration form (tg_tune_set cg_fused): 0 four kernels, 1 p.u in the SpMM, "
                          "2 fused direction, -1 library default")
+    ap.add_argument("--cg-xdefer", type=int, default=-1,
+                    help="deferral depth of the CG x update (tg_tune_set cg_xdefer): 1, 2 or 4, -1 library default")
     ap.add_argument("--gmres-delayed", type=int, default=-1,
                     help="GMRES step (tg_tune_set gmres_delayed): 1 second Gram-Schmidt pass delayed to the next "
                          "step (two basis passes), 0 plain CGS2 (three)")
@@ -251,6 +253,8 @@ def cg_section(args, dev):
         capi.call("tg_tune_set", b"cg_fused", args.cg_fused)
     if args.cg_panel >= 0:
         capi.call("tg_tune_set", b"cg_panel", args.cg_panel)
+    if args.cg_xdefer >= 0:
+        capi.call("tg_tune_set", b"cg_xdefer", args.cg_xdefer)
     g, k, tol = args.cg_grid, 32, 1e-10
     rp, ci, val = gen.laplacian_3d(g, 1e-2)
     n = g ** 3

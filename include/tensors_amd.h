@@ -205,6 +205,10 @@ int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, doubl
  *                  hipGraph (default; the CSR iteration forms), 0 = eager launches.
  *                  Bitwise the same results.  tg_cg_solve_noalloc always launches
  *                  eagerly (it creates no stream or graph).
+ *   "cg_xdefer":   form 1's x += alpha p deferred over 1, 2 or 4 (default)
+ *                  iterations and applied in one pass from a ring of p, so three p
+ *                  steps in four neither read nor write x.  Bitwise the same
+ *                  results (the same fused multiply-adds in the same order).
  *   "gmres_graph": tg_gmres_solve with device operators, 1 = every restart cycle
  *                  after the first runs as one captured hipGraph (default), 0 =
  *                  eager launches.  Bitwise the same results.
@@ -258,7 +262,9 @@ int tg_axpby(int32_t m, int32_t nrhs, tg_dtype val_type, double alpha, const voi
  * all columns satisfy |r.z|_k <= tol^2 |b.Pb|_k, or max_iter iterations.
  * Special cases as the reference: a_coef == 0 or B == 0 -> X <- b_coef X,
  * succeeded; NaN in B -> nothing written, not succeeded.  x starts at 0.
- * workspace: device memory of tg_cg_workspace_bytes() bytes (not zeroed).
+ * workspace: device memory of tg_cg_workspace_bytes() bytes (not zeroed); a
+ * workspace of two n x nrhs blocks less (aligned to 256 bytes) is accepted too,
+ * with the x update then deferred over at most 2 iterations ("cg_xdefer").
  * Synchronises `stream` (the host polls convergence between batches of
  * iterations); *succeeded, *iterations and rel_residuals[nrhs] (sqrt(|r.z| /
  * |b.Pb|) per column, as ConjugateGradient::RelativeResiduals) are host

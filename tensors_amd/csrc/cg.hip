@@ -140,15 +140,20 @@ __global__ __launch_bounds__(kBlock) void cg_dot_pu_kernel(int n, int k, int kpa
     column_partial(acc, L, k, kpad, partial);
 }
 
-// alpha = rho / (p.u)   (:255-260)
+// alpha = rho / (p.u)   (:255-260); also kept in the alpha history (slot iter % xdefer)
+// for the deferred x update
 __global__ __launch_bounds__(kFinalThreads) void cg_alpha_kernel(int k, int kpad, int B, const double* __restrict__ partial,
-                                                          CGState* __restrict__ st) {
+                                                          double* __restrict__ ahist, CGState* __restrict__ st) {
     if (st->done) return;
     const int c = threadIdx.x % kpad, j = threadIdx.x / kpad, J = kFinalThreads / kpad;
     const double pu = column_total(partial, B, k, kpad, c, j, J);
     // guard: a column whose direction has vanished (p.u <= 0, an FP32 iteration past
     // the point its Krylov space is used up) stops moving instead of producing NaN
-    if (j == 0 && c < k) st->alpha[c] = (st->guard && !(pu > 0.0)) ? 0.0 : st->rho[c] / pu;
+    if (j == 0 && c < k) {
+        const double al = (st->guard && !(pu > 0.0)) ? 0.0 : st->rho[c] / pu;
+        st->alpha[c] = al;
+        ahist[(int64_t)(st->iter % st->xdefer) * k + c] = al;
+    }
 }
 
 template <typename T>
@@ -248,67 +253,127 @@ __global__ __launch_bounds__(kFinalThreads) void cg_rho_kernel(int k, int kpad, 
     __syncthreads();
     if (threadIdx.x == 0) {
         st->succeeded = conv;
-        st->iter = st->iter + 1;
-        st->done = (conv || st->iter >= st->max_iter) ? 1 : 0;
-        st->xpend = st->done;  // the p step (and its x update) will not run
+        const int it = st->iter + 1;
+        st->iter = it;
+        const int dn = (conv || it >= st->max_iter) ? 1 : 0;
+        st->done = dn;
+        // x += alpha p of the iterations [xfrom, it): in this iteration's p step once
+        // xdefer of them are pending, else later (after a stop, cg_xfinish_kernel: the p
+        // step does not run)
+        int cnt = 0;
+        if (!dn && it - st->xfrom >= st->xdefer) {
+            cnt = it - st->xfrom;
+            st->xfirst = st->xfrom;
+            st->xfrom = it;
+        }
+        st->xcnt = cnt;
     }
 }
 
-// p = z + beta p   (:321-335), and first x += alpha p with the old p (the update the
-// r step leaves out; rho has advanced iter, so iteration 0 is iter 1)
+constexpr int kMaxDefer = kCgRingSlots;  // ring slots of p for the deferred x update
+
+// The x update of the iterations [xf, xf + cnt) for the thread's R rows: x += alpha_t p_t
+// in iteration order, each the fused multiply-add of the unfused update (x starts at zero
+// without a guess: the first update is alpha p), so x is bitwise the same for any
+// deferral depth.  The p of the newest iteration may be passed in `own` (already loaded).
+template <typename T, int R>
+__device__ __forceinline__ void x_updates(int cnt, int xf, int D, bool first0, const T (&al)[kMaxDefer],
+                                          const T* ring, int64_t pstride, const int64_t (&e)[R], const T* own,
+                                          T (&xv)[R]) {
+    T q[kMaxDefer][R];
+#pragma unroll
+    for (int j = 0; j < kMaxDefer; ++j)
+        if (j < cnt && !(own && j == cnt - 1)) {
+            const T* pj = ring + (int64_t)((xf + j) % D) * pstride;
+#pragma unroll
+            for (int r = 0; r < R; ++r) q[j][r] = pj[e[r]];
+        }
+#pragma unroll
+    for (int j = 0; j < kMaxDefer; ++j)
+        if (j < cnt) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const T pv = (own && j == cnt - 1) ? own[r] : q[j][r];
+                xv[r] = x_step(first0 && j == 0, xv[r], al[j], pv);
+            }
+        }
+}
+
+// p = z + beta p   (:321-335), and the deferred x += alpha p (the update the r step
+// leaves out) of the iterations the rho step released.  rho has advanced iter, so
+// this is iteration t = iter - 1: p_t is read from ring slot t % D and p_{t+1} written
+// to slot (t + 1) % D, whose p_{t+1-D} is in x by then or added here.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void cg_p_kernel(int n, int k, int kpad, const T* __restrict__ z,
-                                                      T* __restrict__ p, T* __restrict__ x,
+__global__ __launch_bounds__(kBlock) void cg_p_kernel(int n, int k, int kpad, const T* __restrict__ z, T* ring,
+                                                      int64_t pstride, T* __restrict__ x,
+                                                      const double* __restrict__ ahist,
                                                       const CGState* __restrict__ st) {
     if (st->done) return;
     int i0, i1;
     block_row_range(n, i0, i1);
     const Lanes L = lanes_of(kpad);
-    const bool first = (st->iter == 1) && !st->use_guess;
+    const int D = st->xdefer, t = st->iter - 1;
+    const int cnt = st->xcnt, xf = st->xfirst;
+    const bool first0 = xf == 0 && !st->use_guess;
+    const T* pc = ring + (int64_t)(t % D) * pstride;
+    T* pn = ring + (int64_t)((t + 1) % D) * pstride;
     if (L.c < k) {
         const T be = (T)st->beta[L.c];
-        const T al = (T)st->alpha[L.c];
+        T al[kMaxDefer];
+#pragma unroll
+        for (int j = 0; j < kMaxDefer; ++j)
+            al[j] = j < cnt ? (T)ahist[(int64_t)((xf + j) % D) * k + L.c] : T(0);
         for (int ib = i0 + L.ro; ib < i1; ib += kRowsInFlight * L.rows) {
             T pv[kRowsInFlight], zv[kRowsInFlight], xv[kRowsInFlight];
+            int64_t e[kRowsInFlight];
 #pragma unroll
             for (int q = 0; q < kRowsInFlight; ++q) {  // loads of four rows first (clamped rows)
-                const int64_t e = (int64_t)min(ib + q * L.rows, i1 - 1) * k + L.c;
-                pv[q] = p[e];
-                zv[q] = z[e];
-                xv[q] = x[e];
+                e[q] = (int64_t)min(ib + q * L.rows, i1 - 1) * k + L.c;
+                pv[q] = pc[e[q]];
+                zv[q] = z[e[q]];
+                if (cnt > 0) xv[q] = x[e[q]];
             }
+            if (cnt > 0) x_updates<T, kRowsInFlight>(cnt, xf, D, first0, al, ring, pstride, e, pv, xv);
 #pragma unroll
             for (int q = 0; q < kRowsInFlight; ++q) {
-                const int i = ib + q * L.rows;
-                if (i < i1) {
-                    const int64_t e = (int64_t)i * k + L.c;
-                    x[e] = x_step(first, xv[q], al, pv[q]);
-                    p[e] = zv[q] + be * pv[q];
+                if (ib + q * L.rows < i1) {
+                    if (cnt > 0) x[e[q]] = xv[q];
+                    pn[e[q]] = zv[q] + be * pv[q];
                 }
             }
         }
     }
 }
 
-// The x update of an iteration after which the solve stopped (the p step did not run).
+// The deferred x updates still pending, [xfrom, iter) (after a stop the p step did
+// not run; at the end of a batch of iterations the last group may be short), so x is
+// complete before the host looks at it.  cg_xclear_kernel then advances xfrom.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void cg_xfinish_kernel(int n, int k, int kpad, const T* __restrict__ p,
-                                                            T* __restrict__ x, const CGState* __restrict__ st) {
-    if (!st->xpend) return;
+__global__ __launch_bounds__(kBlock) void cg_xfinish_kernel(int n, int k, int kpad, const T* __restrict__ ring,
+                                                            int64_t pstride, T* __restrict__ x,
+                                                            const double* __restrict__ ahist,
+                                                            const CGState* __restrict__ st) {
+    const int xf = st->xfrom, cnt = st->iter - xf, D = st->xdefer;
+    if (cnt <= 0) return;
     int i0, i1;
     block_row_range(n, i0, i1);
     const Lanes L = lanes_of(kpad);
-    const bool first = (st->iter == 1) && !st->use_guess;
+    const bool first0 = xf == 0 && !st->use_guess;
     if (L.c < k) {
-        const T al = (T)st->alpha[L.c];
+        T al[kMaxDefer];
+#pragma unroll
+        for (int j = 0; j < kMaxDefer; ++j)
+            al[j] = j < cnt ? (T)ahist[(int64_t)((xf + j) % D) * k + L.c] : T(0);
         for (int i = i0 + L.ro; i < i1; i += L.rows) {
-            const int64_t e = (int64_t)i * k + L.c;
-            x[e] = x_step(first, x[e], al, p[e]);
+            const int64_t e[1] = {(int64_t)i * k + L.c};
+            T xv[1] = {x[e[0]]};
+            x_updates<T, 1>(cnt, xf, D, first0, al, ring, pstride, e, nullptr, xv);
+            x[e[0]] = xv[0];
         }
     }
 }
 
-__global__ void cg_xclear_kernel(CGState* __restrict__ st) { st->xpend = 0; }
+__global__ void cg_xclear_kernel(CGState* __restrict__ st) { st->xfrom = st->iter; }
 
 // --- fused iteration forms (CSR operator, P = I or Jacobi) ------------------
 // Form 1 keeps the four-kernel order but computes partial(p.u) in the SpMM
@@ -337,13 +402,15 @@ __global__ void cg_xclear_kernel(CGState* __restrict__ st) { st->xpend = 0; }
 // offsets after xcd_swizzle) work on at about the same time, so p rows are reused
 // from the L2 instead of being fetched again for each neighbouring plane.
 // FORM = false is the plain "u = A p, partial(p.u)" step of the three-pass
-// iteration (p = p0 is formed by cg_p_kernel; r, d, p1 unused).
+// iteration (p formed by cg_p_kernel in ring slot iter % xdefer from p0 on, pstride
+// elements apart; r, d, p1 unused).
 template <typename T, typename RP, int G, int V, bool FORM, bool HAS_D>
 __global__ __launch_bounds__(kBlock) void cg_dir_spmm_kernel(int n, int k, const RP* __restrict__ rp,
                                                              const int32_t* __restrict__ ci, const T* __restrict__ a,
                                                              const T* __restrict__ r, const T* __restrict__ d,
                                                              T* __restrict__ p0, T* __restrict__ p1,
-                                                             T* __restrict__ u, double* __restrict__ partial,
+                                                             int64_t pstride, T* __restrict__ u,
+                                                             double* __restrict__ partial,
                                                              const CGState* __restrict__ st, int panel) {
     if (st->done) return;
     constexpr int S = kWave / G;  // rows in flight per wave
@@ -375,7 +442,8 @@ __global__ __launch_bounds__(kBlock) void cg_dir_spmm_kernel(int n, int k, const
         cnt = (int)((int64_t)n * (lb + 1) / B) - first;
     }
     const bool odd = FORM && (st->iter & 1) != 0;
-    const T* __restrict__ pp = (odd || !FORM) ? p0 : p1;  // FORM: written by the previous iteration
+    // FORM: written by the previous iteration; else ring slot iter % xdefer (p0 = slot 0)
+    const T* __restrict__ pp = FORM ? (odd ? p0 : p1) : p0 + (int64_t)(st->iter % st->xdefer) * pstride;
     T* __restrict__ pn = odd ? p1 : p0;
     for (int qb = wave * S; qb < cnt; qb += WAVES * S) {
         const int q = qb + s;
@@ -551,10 +619,13 @@ static int pow2ceil_i(int v) {
 // kernels, kFusedBlocks for the fused direction + SpMM kernel.
 constexpr int kPartialBlocks = kFusedBlocks > kSolverBlocks ? kFusedBlocks : kSolverBlocks;
 
-size_t cg_workspace_bytes(int32_t n, int32_t k, size_t vs, int prec_kind) {
+// Layout: state, partials, alpha history, then r, u, x (+ z) and the ring of p
+// (slot 0 = p; slot 1 = the fused iteration's second p; `slots` = 2 or kMaxDefer).
+size_t cg_workspace_bytes(int32_t n, int32_t k, size_t vs, int prec_kind, int slots) {
     const size_t vec = align256((size_t)n * k * vs);
-    const size_t nvec = prec_kind ? 6 : 5;  // r, u, p, x (+ z), second p of the fused iteration
-    return align256(sizeof(CGState)) + align256((size_t)kPartialBlocks * k * sizeof(double)) + nvec * vec;
+    const size_t nvec = (prec_kind ? 4 : 3) + (size_t)std::max(2, slots);
+    return align256(sizeof(CGState)) + align256((size_t)kPartialBlocks * k * sizeof(double)) +
+           align256((size_t)kMaxDefer * k * sizeof(double)) + nvec * vec;
 }
 
 // Iteration form for CSR operators with P = I or Jacobi, short rows (mean length
@@ -569,13 +640,24 @@ enum CgForm : int { kFourKernels = 0, kSpmmDot = 1, kFormDir = 2, kFormAuto = 3 
 static int g_cg_fused = kFormAuto;
 static int g_cg_graph = 1;              // 1: blocks of kCgGraphIters iterations as captured hipGraphs
 constexpr int kCgGraphIters = 16;
+// Deferral depth of form 1's x update ("cg_xdefer": 1, 2 or 4): x += alpha p of
+// `depth` iterations in one pass, from a ring of p, where the workspace holds the ring
+// (2 slots always, 4 in tg_cg_workspace_bytes).  Depth 4 skips the x read and write in
+// three iterations of four: 17 instead of 20 n x nrhs passes of the p step.
+static int g_cg_xdefer = kMaxDefer;
 // rows of the forms' SpMM kernel: 1 = panel order where the band allows, 0 = contiguous chunks
 static int g_cg_panel = 1;
 
 int cg_tune_set(const char* key, int32_t value, bool* known) {
     const std::string k(key);
-    *known = k == "cg_fused" || k == "cg_panel" || k == "cg_graph";
+    *known = k == "cg_fused" || k == "cg_panel" || k == "cg_graph" || k == "cg_xdefer";
     if (!*known) return TG_OK;
+    if (k == "cg_xdefer") {
+        if (value != 1 && value != 2 && value != kMaxDefer)
+            return fail(TG_ERR_INVALID, "tg_tune_set: cg_xdefer must be 1, 2 or %d", kMaxDefer);
+        g_cg_xdefer = value;
+        return TG_OK;
+    }
     if (k == "cg_panel" || k == "cg_graph") {
         if (value != 0 && value != 1) return fail(TG_ERR_INVALID, "tg_tune_set: %s must be 0 or 1", key);
         (k == "cg_panel" ? g_cg_panel : g_cg_graph) = value;
@@ -594,16 +676,17 @@ int32_t cg_tune_get(const char* key) {
     if (k == "cg_fused") return g_cg_fused;
     if (k == "cg_panel") return g_cg_panel;
     if (k == "cg_graph") return g_cg_graph;
+    if (k == "cg_xdefer") return g_cg_xdefer;
     return -1;
 }
 
 template <typename T, typename RP, int V>
 static int launch_dir_v(int G, int form, bool has_d, hipStream_t s, int n, int k, const RP* rp, const int32_t* ci,
-                        const T* a, const T* r, const T* d, T* p0, T* p1, T* u, double* partial, const CGState* st,
-                        int panel) {
+                        const T* a, const T* r, const T* d, T* p0, T* p1, int64_t pstride, T* u, double* partial,
+                        const CGState* st, int panel) {
 #define TG_DIRK(GG, FORM, HD)                                                                                  \
     hipLaunchKernelGGL((cg_dir_spmm_kernel<T, RP, GG, V, FORM, HD>), dim3(kFusedBlocks), dim3(kBlock), 0, s, n, k, \
-                       rp, ci, a, r, d, p0, p1, u, partial, st, panel)
+                       rp, ci, a, r, d, p0, p1, pstride, u, partial, st, panel)
 #define TG_DIR(GG)                     \
     case GG:                           \
         if (form != kFormDir)          \
@@ -633,7 +716,8 @@ static int launch_dir_v(int G, int form, bool has_d, hipStream_t s, int n, int k
 // allows), G = enough lanes for all k columns (at most 64; then k / (64 V) tiles).
 template <typename T, typename RP>
 static int launch_dir(int form, hipStream_t s, int n, int k, const RP* rp, const int32_t* ci, const T* a,
-                      const T* r, const T* d, T* p0, T* p1, T* u, double* partial, const CGState* st, int panel) {
+                      const T* r, const T* d, T* p0, T* p1, int64_t pstride, T* u, double* partial,
+                      const CGState* st, int panel) {
     int V = 1;
     if (sizeof(T) == 8) {
         V = (k % 2 == 0) ? 2 : 1;  // 4 doubles per lane measured slower on the 128^3 Laplacian
@@ -645,14 +729,14 @@ static int launch_dir(int form, hipStream_t s, int n, int k, const RP* rp, const
     const bool has_d = d != nullptr;
     switch (V) {
         case 1:
-            return launch_dir_v<T, RP, 1>(G, form, has_d, s, n, k, rp, ci, a, r, d, p0, p1, u, partial, st,
+            return launch_dir_v<T, RP, 1>(G, form, has_d, s, n, k, rp, ci, a, r, d, p0, p1, pstride, u, partial, st,
                                                   panel);
         case 2:
-            return launch_dir_v<T, RP, 2>(G, form, has_d, s, n, k, rp, ci, a, r, d, p0, p1, u, partial, st,
+            return launch_dir_v<T, RP, 2>(G, form, has_d, s, n, k, rp, ci, a, r, d, p0, p1, pstride, u, partial, st,
                                                   panel);
         default:
             if constexpr (sizeof(T) == 4)
-                return launch_dir_v<T, RP, 4>(G, form, has_d, s, n, k, rp, ci, a, r, d, p0, p1, u, partial, st,
+                return launch_dir_v<T, RP, 4>(G, form, has_d, s, n, k, rp, ci, a, r, d, p0, p1, pstride, u, partial, st,
                                                   panel);
             return fail(TG_ERR_INVALID, "internal: bad vector width");
     }
@@ -673,8 +757,9 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
     res->iterations = 0;
     if (k < 1 || k > kBlock) return fail(TG_ERR_INVALID, "cg: nrhs must be in [1, %d]", kBlock);
     if (n < 0 || A.m != n) return fail(TG_ERR_SHAPE, "cg: the operator must be square");
-    if (!ws || ws_bytes < cg_workspace_bytes(n, k, sizeof(T), prec_kind))
+    if (!ws || ws_bytes < cg_workspace_bytes(n, k, sizeof(T), prec_kind, 2))
         return fail(TG_ERR_WORKSPACE, "cg: workspace missing or smaller than tg_cg_workspace_bytes()");
+    const int slots = ws_bytes >= cg_workspace_bytes(n, k, sizeof(T), prec_kind, kMaxDefer) ? kMaxDefer : 2;
     if (prec_kind == 1 && !d) return fail(TG_ERR_INVALID, "cg: diagonal preconditioner without a diagonal");
     if (prec_kind == 2 && !Pop.fn) return fail(TG_ERR_INVALID, "cg: callback preconditioner without a function");
     max_iter = std::max(0, max_iter);
@@ -690,13 +775,16 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
     w += align256(sizeof(CGState));
     double* partial = (double*)w;
     w += align256((size_t)kPartialBlocks * k * sizeof(double));
+    double* ahist = (double*)w;
+    w += align256((size_t)kMaxDefer * k * sizeof(double));
     const size_t vec = align256((size_t)N * sizeof(T));
+    const int64_t pstride = (int64_t)(vec / sizeof(T));  // elements between ring slots
     T* r = (T*)w;
     T* u = (T*)(w + vec);
-    T* p = (T*)(w + 2 * vec);
-    T* x = (T*)(w + 3 * vec);
-    T* z = prec_kind ? (T*)(w + 4 * vec) : r;
-    T* p1 = (T*)(w + (prec_kind ? 5 : 4) * vec);
+    T* x = (T*)(w + 2 * vec);
+    T* z = prec_kind ? (T*)(w + 3 * vec) : r;
+    T* p = (T*)(w + (prec_kind ? 4 : 3) * vec);  // ring slot 0
+    T* p1 = p + pstride;                          // ring slot 1
     const T* dd = prec_kind == 1 ? d : nullptr;
     const int zmode = prec_kind;  // 0: z is r, 1: z = d r in the kernels, 2: z = P(r) by callback
     const int kpad = pow2ceil_i(k);
@@ -713,6 +801,9 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
     int form = g_cg_fused == kFormAuto ? ((int64_t)n * k <= (int64_t(1) << 20) ? kFormDir : kSpmmDot) : g_cg_fused;
     if (!csr_ok) form = kFourKernels;
     const bool fused = form == kFormDir;
+    // deferred x update: form 1 only (form 0's operator reads p at a fixed address, form 2
+    // updates x in its step kernel)
+    const int xdefer = form == kSpmmDot ? std::min(g_cg_xdefer, slots) : 1;
     // panel order for the SpMM kernel of forms 1 / 2 (cg_dir_spmm_kernel): panels of
     // one band, when that gives every workgroup at least 4 rows per panel and the
     // matrix at least 4 panels (the band as the caller measured it; none measured: off)
@@ -722,9 +813,9 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
     auto dir_spmm_on = [&](hipStream_t ss) -> int {
         if (Ac.rp_type == TG_I64)
             return launch_dir<T, int64_t>(form, ss, n, k, (const int64_t*)Ac.rp, Ac.ci, (const T*)Ac.val, r, dd, p,
-                                          p1, u, partial, st, panel);
+                                          p1, pstride, u, partial, st, panel);
         return launch_dir<T, int32_t>(form, ss, n, k, (const int32_t*)Ac.rp, Ac.ci, (const T*)Ac.val, r, dd, p, p1,
-                                      u, partial, st, panel);
+                                      pstride, u, partial, st, panel);
     };
 
     // z = P r by callback, then partial(r.z) -- the external-preconditioner step
@@ -740,6 +831,7 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
     h.max_iter = max_iter;
     h.use_guess = use_guess ? 1 : 0;
     h.guard = hook ? hook->guard : 0;
+    h.xdefer = xdefer;
     TG_HIP(hipMemcpyAsync(st, &h, sizeof(CGState), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL((cg_init_kernel<T>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, Bm, ldB, dd, r, z, x,
                        use_guess ? 0 : 1, zmode, partial);
@@ -787,6 +879,8 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
     // Iterations.  With device operators every kernel checks `done` itself and
     // the host polls in growing batches; host callbacks cannot check it, so
     // then the host polls after every iteration.
+    // The p the next iteration reads (ring slot iter % xdefer), for the hook.
+    auto p_next = [&](const CGState& hs) { return p + (int64_t)(hs.iter % xdefer) * pstride; };
     int launched = 0, batch = host_ops ? 1 : hook ? std::max(1, hook->every) : 4;
     int hflag[2] = {0, 0};
     TG_HIP(hipMemcpyAsync(hflag, &st->iter, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -796,7 +890,7 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
         TG_HIP(hipMemcpyAsync(&hs, st, sizeof(CGState), hipMemcpyDeviceToHost, s));
         TG_HIP(hipStreamSynchronize(s));
         bool stop = false;
-        rc = hook->fn(hs, r, z, x, fused ? nullptr : p, partial, st, &stop);
+        rc = hook->fn(hs, r, z, x, fused ? nullptr : p_next(hs), partial, st, &stop);
         if (rc) return rc;
         if (!stop) {
             TG_HIP(hipMemcpyAsync(hflag, &st->iter, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -809,7 +903,7 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
             int rc1 = dir_spmm_on(ss);
             if (rc1) return rc1;
             hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kFinalThreads), 0, ss, k, kpad, kFusedBlocks,
-                               (const double*)partial, st);
+                               (const double*)partial, ahist, st);
             if (dd)
                 hipLaunchKernelGGL((cg_step_kernel<T, true>), dim3(B), dim3(kBlock), 0, ss, n, k, kpad, (const T*)p,
                                    (const T*)p1, (const T*)u, dd, x, r, partial, (const CGState*)st);
@@ -824,14 +918,14 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
             int rc1 = dir_spmm_on(ss);
             if (rc1) return rc1;
             hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kFinalThreads), 0, ss, k, kpad, kFusedBlocks,
-                               (const double*)partial, st);
+                               (const double*)partial, ahist, st);
         } else {
             int rc1 = op_apply(Aop, p, u, k, &st->done, ss);
             if (rc1) return rc1;
             hipLaunchKernelGGL((cg_dot_pu_kernel<T>), dim3(B), dim3(kBlock), 0, ss, n, k, kpad, (const T*)p,
                                (const T*)u, partial, (const CGState*)st);
             hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kFinalThreads), 0, ss, k, kpad, B,
-                               (const double*)partial, st);
+                               (const double*)partial, ahist, st);
         }
         hipLaunchKernelGGL((cg_update_kernel<T>), dim3(B), dim3(kBlock), 0, ss, n, k, kpad, (const T*)u, dd, r, z,
                            zmode, partial, (const CGState*)st);
@@ -840,8 +934,8 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
             if (rc1) return rc1;
         }
         hipLaunchKernelGGL(cg_rho_kernel, dim3(1), dim3(kFinalThreads), 0, ss, k, kpad, B, (const double*)partial, st);
-        hipLaunchKernelGGL((cg_p_kernel<T>), dim3(B), dim3(kBlock), 0, ss, n, k, kpad, (const T*)z, p, x,
-                           (const CGState*)st);
+        hipLaunchKernelGGL((cg_p_kernel<T>), dim3(B), dim3(kBlock), 0, ss, n, k, kpad, (const T*)z, p, pstride, x,
+                           (const double*)ahist, (const CGState*)st);
         return TG_OK;
     };
     // Blocks of kCgGraphIters iterations as one captured hipGraph (the CSR forms, whose
@@ -885,8 +979,8 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
         for (; todo > 0 && rc == TG_OK; --todo, ++launched) rc = one_iteration(s);
         if (rc) break;
         if (!fused) {  // x complete before the host looks at the state
-            hipLaunchKernelGGL((cg_xfinish_kernel<T>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, (const T*)p, x,
-                               (const CGState*)st);
+            hipLaunchKernelGGL((cg_xfinish_kernel<T>), dim3(B), dim3(kBlock), 0, s, n, k, kpad, (const T*)p, pstride,
+                               x, (const double*)ahist, (const CGState*)st);
             hipLaunchKernelGGL(cg_xclear_kernel, dim3(1), dim3(1), 0, s, st);
         }
         if (hipGetLastError() != hipSuccess) {
@@ -905,7 +999,7 @@ int cg_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d, co
                 break;
             }
             bool stop = false;
-            rc = hook->fn(hs, r, z, x, fused ? nullptr : p, partial, st, &stop);
+            rc = hook->fn(hs, r, z, x, fused ? nullptr : p_next(hs), partial, st, &stop);
             if (rc || stop) break;
             if (hipMemcpyAsync(hflag, &st->iter, 2 * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
                 hipStreamSynchronize(s) != hipSuccess) {
@@ -943,7 +1037,7 @@ using namespace tg;
 
 extern "C" size_t tg_cg_workspace_bytes(int32_t n, int32_t nrhs, tg_dtype val_type, int32_t prec_kind) {
     if (n < 0 || nrhs < 1) return 0;
-    return cg_workspace_bytes(n, nrhs, val_type == TG_F32 ? 4 : 8, prec_kind ? 1 : 0);
+    return cg_workspace_bytes(n, nrhs, val_type == TG_F32 ? 4 : 8, prec_kind ? 1 : 0, kMaxDefer);
 }
 
 template <typename T>

@@ -106,9 +106,15 @@ struct CGState {
     int succeeded;
     int max_iter;
     int use_guess;
-    int xpend;  // the last iteration's x += alpha p is still to be applied (it stopped before the p step)
     int guard;  // alpha = 0 where p.u <= 0 (set by the mixed solver for its FP32 iteration)
-    int pad[9];
+    // Deferred x update (form 1): the p of iteration t sits in ring slot t % xdefer and its
+    // alpha in the alpha history; x holds the updates of the iterations before xfrom.  The
+    // p step of an iteration with xcnt > 0 adds those of [xfirst, xfirst + xcnt) to x.
+    int xdefer;
+    int xfrom;
+    int xfirst;
+    int xcnt;
+    int pad[6];
     double rho[kMaxCols];
     double rho_old[kMaxCols];
     double alpha[kMaxCols];
@@ -156,7 +162,11 @@ int gmres_solve_t(const CsrView& A, const OpRef& Aop, int prec_kind, const T* d,
                   double tol, int32_t max_restarts, int use_guess, void* ws, size_t ws_bytes, GmresResult* res,
                   hipStream_t s, GmresReuse* reuse = nullptr);
 
-size_t cg_workspace_bytes(int32_t n, int32_t k, size_t vs, int prec_kind);
+// Ring slots of p for the CG's deferred x update: cg_workspace_bytes with `slots` = 2 is
+// the least a solve accepts; with kCgRingSlots (what tg_cg_workspace_bytes reports) the
+// x update is deferred over up to kCgRingSlots iterations.
+constexpr int kCgRingSlots = 4;
+size_t cg_workspace_bytes(int32_t n, int32_t k, size_t vs, int prec_kind, int slots = kCgRingSlots);
 
 // Host hook of cg_solve_t: with a hook the host polls every `every` iterations and
 // calls fn with the device state copied to the host (hs), the device buffers of the

@@ -1051,12 +1051,13 @@ extern "C" int32_t tg_tune_get(const char* key) {
     if (k == "spmm_split") return g_spmm_split;
     if (k == "spmm_long_row") return g_spmm_long_row;
     if (k == "spmm_rows") return g_spmm_rows;
-    if (k == "cg_fused" || k == "cg_panel" || k == "cg_graph") return cg_tune_get(key);
-    if (k == "bsr_vector") return bsr_tune_get(key);
-    if (k == "gmres_graph" || k == "gmres_delayed") return gmres_tune_get(key);
-    if (k == "gbatch_big_tiles" || k == "gbatch_delayed") return gbatch_tune_get(key);
-    if (k == "serve_gmres_batch" || k == "serve_gmres_batch_rows") return serve_tune_get(key);
-    return batch_tune_get(key);
+    // the other modules' keys: each getter returns -1 for a key it does not know (no
+    // knob takes a negative value)
+    for (auto get : {cg_tune_get, bsr_tune_get, gmres_tune_get, gbatch_tune_get, serve_tune_get, batch_tune_get}) {
+        const int32_t v = get(key);
+        if (v != -1) return v;
+    }
+    return -1;
 }
 
 extern "C" int32_t tg_csr_default_blocks(int32_t m, int64_t nnz, int32_t nrhs) {

@@ -94,3 +94,29 @@ def test_convection_diffusion_is_m_matrix():
     assert np.all(off <= 0) and np.all(A.sum(1) >= -1e-12)
     assert not np.allclose(A, A.T)
     assert np.all(np.linalg.inv(A) >= -1e-12)  # inverse-positive
+
+
+def tuning_keys():
+    """The knob keys the header documents for tg_tune_set (lines ` *   "key":` or
+    ` *   "key1", "key2":` before the tg_tune_set declaration)."""
+    text = open(os.path.join(ROOT, "include", "tensors_amd.h")).read()
+    block = text[:text.index("int tg_tune_set(")]
+    keys = []
+    for m in re.finditer(r'^ \*   ((?:"[a-z0-9_]+"(?:, )?)+):', block, flags=re.M):
+        keys += re.findall(r'"([a-z0-9_]+)"', m.group(1))
+    return keys
+
+
+def test_every_documented_knob_reads_back():
+    """tg_tune_get knows every key tg_tune_set documents (a key missing from its dispatch
+    reads -1, and a caller that saves and restores a knob then fails), and setting a knob
+    to the value it reads is accepted.  Host globals only: no GPU call."""
+    keys = tuning_keys()
+    assert len(keys) >= 20 and "cg_xdefer" in keys and "gmres_delayed" in keys
+    L = capi.lib()
+    for key in keys:
+        v = int(L.tg_tune_get(key.encode()))
+        assert v >= 0, key
+        assert L.tg_tune_set(key.encode(), v) == 0, (key, capi.last_error())
+        assert int(L.tg_tune_get(key.encode())) == v, key
+    assert int(L.tg_tune_get(b"no_such_knob")) == -1

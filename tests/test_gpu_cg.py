@@ -232,6 +232,66 @@ def test_cg_graph_blocks_equal_eager_launches(cuda, fused_knob, form, k, prec):
     assert am[0] == bm[0] and torch.equal(am[1], bm[1])
 
 
+def _cg_raw(A, n, k, dtype, d, B, max_iter, tol, ws_bytes, cuda):
+    """tg_cg_solve with a workspace of exactly ws_bytes bytes."""
+    import ctypes as C
+    tg = capi.F64 if dtype == torch.float64 else capi.F32
+    rp, ci, vals = A.device_arrays(dtype)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=cuda)
+    X = torch.zeros(n, k, dtype=dtype, device=cuda)
+    ok, it = C.c_int32(0), C.c_int32(0)
+    rr = np.zeros(k)
+    capi.call("tg_cg_solve", n, A.NonzeroCount(), rp.data_ptr(), capi.I64 if rp.dtype == torch.int64 else capi.I32,
+              ci.data_ptr(), vals.data_ptr(), tg, 0 if d is None else 1, None if d is None else d.data_ptr(), k,
+              max_iter, 1.0, B.data_ptr(), k, 0.0, X.data_ptr(), k, tol, 0, ws.data_ptr(), ws_bytes, C.byref(ok),
+              C.byref(it), rr.ctypes.data, None)
+    torch.cuda.synchronize()
+    return bool(ok.value), it.value, rr.tolist(), X.cpu()
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32], ids=["f64", "f32"])
+@pytest.mark.parametrize("k", [1, 8])
+@pytest.mark.parametrize("prec", [False, True])
+def test_cg_deferred_x_update_is_bitwise(cuda, fused_knob, dtype, k, prec):
+    """Knob cg_xdefer (form 1): x += alpha p of 1, 2 or 4 iterations in one pass from a
+    ring of p.  The updates are the same fused multiply-adds in the same order, so the
+    solution, iteration count and residuals are bitwise those of the per-iteration update:
+    a converged solve (stopping inside a group of 4), one that runs into max_iter = 45, a
+    workspace with only the two ring slots every solve accepts (depth 2 then), and the
+    mixed solver, whose host hook folds x every 16 iterations."""
+    rp, ci, val, n = grid_system(40, 0.05)
+    B = gen.uniform(1313 + k, n * k).reshape(n, k)
+    d = 1.0 / sp.csr_matrix((val, ci, rp), shape=(n, n)).diagonal() if prec else None
+    A = MatrixCSR(rp, ci, val, n, n)
+    tol = 1e-10 if dtype == torch.float64 else 1e-5
+    vs = 8 if dtype == torch.float64 else 4
+    full = int(capi.lib().tg_cg_workspace_bytes(n, k, capi.F64 if vs == 8 else capi.F32, 0 if d is None else 1))
+    vec = (n * k * vs + 255) // 256 * 256
+    dd = None if d is None else dev(d, cuda, dtype)
+    Bd = dev(B, cuda, dtype)
+    fused_knob(1)
+    before = int(capi.lib().tg_tune_get(b"cg_xdefer"))
+    out = {}
+    try:
+        for depth in (1, 2, 4):
+            capi.call("tg_tune_set", b"cg_xdefer", depth)
+            res = [_cg_raw(A, n, k, dtype, dd, Bd, mi, t, full, cuda) for mi, t in ((2000, tol), (45, 1e-14))]
+            res.append(_cg_raw(A, n, k, dtype, dd, Bd, 2000, tol, full - 2 * vec, cuda))
+            if dtype == torch.float64:
+                Xm = torch.zeros(n, k, dtype=torch.float64, device=cuda)
+                mixed = cg_solve_mixed(A, Bd, Xm, tol=1e-10, P=dd)
+                res.append((mixed[0], mixed[1], mixed[2], Xm.cpu()))
+            out[depth] = res
+    finally:
+        capi.call("tg_tune_set", b"cg_xdefer", before)
+    conv, capped = out[1][0], out[1][1]
+    assert conv[0] and conv[1] > 40
+    assert not capped[0] and capped[1] == 45  # 45 = 11 groups of 4 and one pending update
+    for depth in (2, 4):
+        for a, b in zip(out[depth], out[1]):
+            assert a[:3] == b[:3] and torch.equal(a[3], b[3])
+
+
 @pytest.mark.parametrize("n,k", [(1, 1), (1, 4), (2, 3), (3, 8)])
 @pytest.mark.parametrize("prec", [False, True])
 def test_cg_tiny_systems(cuda, n, k, prec):
