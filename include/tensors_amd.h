@@ -171,11 +171,10 @@ int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, doubl
  *                  (default), 0 = fused direction and SpMM (bitwise the same).
  *                  The split form defers x += alpha p to the next direction pass:
  *                  change this knob only while no batch has systems in flight.
- *   "batch_xcd_order": 0 = the tile list of the batched engine in system-major
- *                  order, dealt round-robin over the XCDs (default: every XCD gets the
- *                  same mix of systems), 1 = each XCD takes a contiguous run of the
- *                  list (a tile's stencil neighbours share its L2, but the XCD whose
- *                  run holds the costlier systems finishes last).  Bitwise the same.
+ *   "batch_xcd_order": 0 = each XCD works through a contiguous run of the
+ *                  batched engine's system-major tile list (default: a tile's stencil
+ *                  neighbours are in flight on its XCD), 1 = each XCD's tiles spread
+ *                  over the whole list (measured 8-9 % slower).  Bitwise the same.
  *   "batch_graph": 1 = each tg_batch_step runs its iterations as one captured
  *                  hipGraph (re-captured when the set of active systems changes;
  *                  default), 0 = eager launches.  Bitwise the same results.

@@ -45,10 +45,12 @@ static int g_batch_stream = 0;  // SpMM kernel of the engine: 0 = a row per lane
 static int g_batch_split = 1;   // 1: direction pass + SpMM of p (three launches per iteration); 0: fused K1
 static int g_batch_idx16 = 1;   // 1: 16-bit column offsets in the split SpMM where they fit (read at add)
 static int g_batch_graph = 1;   // 1: the iterations of a step run as one captured hipGraph
-// 1: tile list laid out so each XCD takes a contiguous run of tiles; 0 (default): system-major
-// order dealt round-robin over the XCDs.  On 2 000 uneven 2D/3D systems the contiguous runs
-// measured 8-9% slower (the XCD whose run holds the costlier systems finishes last), so
-// the spread order is the default
+// Tile order over the XCDs.  The kernels take tile xcd_swizzle(b) in workgroup b, which
+// runs on XCD b % 8, so with the system-major list as it is (0, the default) each XCD
+// works through a contiguous run of it and a tile's stencil neighbours (the next tiles of
+// its system) are in flight on the same XCD.  1: the list permuted once more, which
+// spreads each XCD's tiles over the whole list; measured 8-9% slower on 2 000 uneven
+// 2D/3D systems (the neighbours then come from other XCDs' L2s).
 static int g_batch_xcd_order = 0;
 // SpMM kernel of the split iteration: 0 = row loop, 1 = software-pipelined row loop,
 // 2 = indices and values of a 256-row segment staged through LDS with coalesced loads
@@ -1261,12 +1263,12 @@ template <typename T> class CGBatch {
         for (int i = 0; i < (int)used_.size(); ++i)
             if (used_[i] && !parked_[i] && !states[i].done) tiles_of(i, tl);
         if (g_batch_xcd_order && tl.size() > (size_t)kXcds) {
-            // Workgroup b runs on XCD b % 8: lay the list out so that XCD x takes the
-            // contiguous run [x*q + min(x, r), ...) of the system-major tile order, so a
-            // tile's stencil neighbours (the next tiles of its system) share its L2.  Every
+            // The spread order: position b of the list gets the tile at position
+            // xcd_swizzle(b) of the system-major order, the same mapping the kernels apply
+            // once more, so each XCD's tiles come from all eight eighths of the list.  Every
             // reduction adds tile partials by tile index, so the results do not change.
-            // (the positions b < nt with b % 8 == x are exactly q + [x < r] many, the
-            // length of XCD x's run, so this is a permutation)
+            // (the positions b < nt with b % 8 == x are exactly q + [x < r] many, so this
+            // is a permutation)
             const int nt = (int)tl.size(), q = nt / kXcds, r = nt % kXcds;
             std::vector<Tile> perm(tl.size());
             for (int b = 0; b < nt; ++b) {

@@ -209,9 +209,10 @@ def test_graph_replayed_steps_equal_eager_steps(cuda, mixed):
 
 @pytest.mark.parametrize("mixed", [0, 1, 2])
 def test_xcd_tile_order_is_bitwise_neutral(cuda, mixed):
-    """The tile list laid out so each XCD takes a contiguous run of tiles (knob
-    batch_xcd_order = 1) changes only which XCD runs a tile: every reduction adds the tile
-    partials by tile index, so the results are bitwise those of the default order."""
+    """The spread tile order (knob batch_xcd_order = 1: each XCD's tiles drawn from the
+    whole list instead of a contiguous run) changes only which XCD runs a tile: every
+    reduction adds the tile partials by tile index, so the results are bitwise those of the
+    default order."""
     from tensors_amd import _capi as capi
 
     sysl = systems(9, seed=303)
