@@ -119,6 +119,9 @@ def parse():
                     help="GMRES requests of up to this many rows join the batched engine (larger: worker streams)")
     ap.add_argument("--serve-gmres-batch", type=int, default=1,
                     help="1: GMRES requests join the batched GMRES engine; 0: one worker stream per request")
+    ap.add_argument("--serve-gmres-mixed", type=int, default=1,
+                    help="1: GMRES requests on the worker streams run the mixed solver (FP32 cycles, FP64 "
+                         "refinement; tg_tune_set serve_gmres_mixed, the default); 0: tg_gmres_solve")
     ap.add_argument("--serve-requests", type=int, default=160)
     ap.add_argument("--serve-rates", type=str, default="0.25,0.5,1,2",
                     help="arrival rates as multiples of the measured saturated service rate")
@@ -573,6 +576,7 @@ def serve_section(args, dev):
 
     capi.call("tg_tune_set", b"serve_gmres_batch", int(args.serve_gmres_batch))
     capi.call("tg_tune_set", b"serve_gmres_batch_rows", int(args.serve_gmres_batch_rows))
+    capi.call("tg_tune_set", b"serve_gmres_mixed", int(args.serve_gmres_mixed))
     t_gen = time.perf_counter()
     reqs = serve_requests(args.serve_requests)
     t_gen = time.perf_counter() - t_gen
@@ -645,7 +649,8 @@ def serve_section(args, dev):
                     f"P = I, 1e-8, 100 restarts); host arrays in/out; 200 GB admission budget; GMRES "
                     + (f"requests of up to {args.serve_gmres_batch_rows} rows in the batched GMRES engine, larger "
                        f"ones on {args.serve_gmres_workers} worker streams" if args.serve_gmres_batch else
-                       f"requests on {args.serve_gmres_workers} worker streams"),
+                       f"requests on {args.serve_gmres_workers} worker streams")
+                    + (" (mixed GMRES: FP32 cycles, FP64 refinement)" if args.serve_gmres_mixed else ""),
         "generation_s": round(t_gen, 2),
         "saturated": sat,
         "poisson_sweep": sweep,

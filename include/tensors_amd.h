@@ -237,6 +237,10 @@ int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, doubl
  *                  serve_gmres_batch_rows rows (default 131072) join one batched
  *                  GMRES engine (tg_gbatch_*), larger ones run on the worker
  *                  threads; 0 = every request on a worker thread.
+ *   "serve_gmres_mixed": GMRES requests submitted afterwards that run on a worker
+ *                  thread: 1 = tg_gmres_solve_mixed (default: FP32 cycles, stop test
+ *                  on the FP64 true residual; the result's iterations count the
+ *                  FP32 Arnoldi steps of all cycles), 0 = tg_gmres_solve.
  * Process-wide; not thread-safe against concurrent SpMM calls. */
 int tg_tune_set(const char* key, int32_t value);
 int32_t tg_tune_get(const char* key);
@@ -540,8 +544,9 @@ size_t tg_gbatch_request_bytes(int32_t n, int32_t nrhs, int32_t restart);
  *   them together on one stream (Jacobi `inv_diag` optional, mixed as there);
  *   GMRES(restart) requests (P = I, left side) of up to serve_gmres_batch_rows
  *   rows join one batched GMRES engine (tg_gbatch_*) advanced by a thread of its
- *   own; larger ones (and all, with serve_gmres_batch = 0) run tg_gmres_solve on
- *   one of `gmres_workers` host threads, each with its own stream.
+ *   own; larger ones (and all, with serve_gmres_batch = 0) run
+ *   tg_gmres_solve_mixed (tg_gmres_solve with serve_gmres_mixed = 0) on one of
+ *   `gmres_workers` host threads, each with its own stream.
  * On completion X is downloaded and all of the request's device memory freed.
  * Results do not depend on what else is in flight.  Times are seconds since
  * tg_server_create on the host steady clock.
@@ -564,7 +569,8 @@ int tg_server_result(tg_server_t server, int64_t id, int32_t* status, int32_t* s
 int tg_server_release(tg_server_t server, int64_t id);
 int tg_server_stats(tg_server_t server, int64_t* bytes_in_use, int64_t* peak_bytes, int32_t* queued,
                     int32_t* running);
-/* Bytes a request is charged against the budget. */
+/* Bytes a request submitted now is charged against the budget (mixed: the CG
+ * request's batched-engine mode; GMRES follows the serve_gmres_* knobs). */
 size_t tg_server_request_bytes(int32_t kind, int32_t n, int64_t nnz, int32_t nrhs, int32_t has_diag, int32_t mixed,
                                int32_t restart);
 

@@ -40,11 +40,22 @@ namespace tg {
 // tiles), which the single-system solver reduces in two levels.
 static int g_serve_gmres_batch = 1;
 static int g_serve_gmres_batch_rows = 131072;  // 128 tiles of the batched engine
+// GMRES requests submitted from now on that run on a worker thread: 1 (default) = the
+// mixed solver (tg_gmres_solve_mixed: FP32 cycles, the FP64 true residual as stop
+// test), 0 = tg_gmres_solve.  The worker-thread requests are the large ones, whose
+// steps stream the basis, and several of them share the HBM bandwidth at once: FP32
+// halves those bytes (160-request mix: 58.8 -> 78.5 solves/s saturated).
+static int g_serve_gmres_mixed = 1;
 
 int serve_tune_set(const char* key, int32_t value, bool* known) {
     const std::string k(key);
-    *known = k == "serve_gmres_batch" || k == "serve_gmres_batch_rows";
+    *known = k == "serve_gmres_batch" || k == "serve_gmres_batch_rows" || k == "serve_gmres_mixed";
     if (!*known) return TG_OK;
+    if (k == "serve_gmres_mixed") {
+        if (value != 0 && value != 1) return fail(TG_ERR_INVALID, "tg_tune_set: serve_gmres_mixed must be 0 or 1");
+        g_serve_gmres_mixed = value;
+        return TG_OK;
+    }
     if (k == "serve_gmres_batch_rows") {
         if (value < 1) return fail(TG_ERR_INVALID, "tg_tune_set: serve_gmres_batch_rows must be >= 1");
         g_serve_gmres_batch_rows = value;
@@ -59,6 +70,7 @@ int32_t serve_tune_get(const char* key) {
     const std::string k(key);
     if (k == "serve_gmres_batch") return g_serve_gmres_batch;
     if (k == "serve_gmres_batch_rows") return g_serve_gmres_batch_rows;
+    if (k == "serve_gmres_mixed") return g_serve_gmres_mixed;
     return -1;
 }
 
@@ -427,15 +439,21 @@ struct Server {
     void solve_gmres(Request* r, hipStream_t s) {
         int rc = s ? TG_OK : fail(TG_ERR_HIP, "tg_server: no stream for a GMRES worker");
         void* ws = nullptr;
-        const size_t wsb = tg_gmres_workspace_bytes(r->n, r->restart, r->k, TG_F64);
+        const size_t wsb = r->mixed ? tg_gmres_mixed_workspace_bytes(r->n, r->nnz, r->restart, r->k, 0, 0)
+                                    : tg_gmres_workspace_bytes(r->n, r->restart, r->k, TG_F64);
         if (rc == TG_OK) rc = upload(r, s, false);
         if (rc == TG_OK && hipMallocAsync(&ws, wsb, s) != hipSuccess) rc = fail(TG_ERR_HIP, "tg_server: workspace");
         r->rel.assign(r->k, 0.0);
         if (rc == TG_OK) {
             DevPtrs d = ptrs(r, false);
-            rc = tg_gmres_solve(r->n, r->nnz, d.rp, TG_I32, d.ci, d.val, TG_F64, 0, nullptr, 0, r->k, r->restart, 1.0,
-                                d.B, r->k, 0.0, d.X, r->k, r->tol, r->max_restarts, 0, ws, wsb, &r->succeeded,
-                                &r->iterations, &r->restarts, r->rel.data(), s);
+            if (r->mixed)  // iterations: the FP32 Arnoldi steps over all cycles
+                rc = tg_gmres_solve_mixed(r->n, r->nnz, d.rp, TG_I32, d.ci, d.val, nullptr, 0, nullptr, 0, r->k,
+                                          r->restart, r->max_restarts, 1.0, d.B, r->k, 0.0, d.X, r->k, r->tol, 0, ws,
+                                          wsb, &r->succeeded, &r->restarts, &r->iterations, r->rel.data(), s);
+            else
+                rc = tg_gmres_solve(r->n, r->nnz, d.rp, TG_I32, d.ci, d.val, TG_F64, 0, nullptr, 0, r->k, r->restart,
+                                    1.0, d.B, r->k, 0.0, d.X, r->k, r->tol, r->max_restarts, 0, ws, wsb,
+                                    &r->succeeded, &r->iterations, &r->restarts, r->rel.data(), s);
             if (rc == TG_OK && hipMemcpyAsync(r->X, d.X, (size_t)r->n * r->k * 8, hipMemcpyDeviceToHost, s) != hipSuccess)
                 rc = fail(TG_ERR_HIP, "tg_server: download failed");
         }
@@ -462,6 +480,7 @@ extern "C" size_t tg_server_request_bytes(int32_t kind, int32_t n, int64_t nnz, 
     const size_t in = input_bytes(n, nnz, nrhs, kind == kCG && has_diag);
     if (kind == kCG) return in + tg_batch_request_bytes(n, nnz, nrhs, has_diag, mixed);
     if (gmres_goes_batched(n)) return in + tg_gbatch_request_bytes(n, nrhs, restart);
+    if (g_serve_gmres_mixed) return in + tg_gmres_mixed_workspace_bytes(n, nnz, restart, nrhs, 0, 0);
     return in + tg_gmres_workspace_bytes(n, restart, nrhs, TG_F64);
 }
 
@@ -586,7 +605,9 @@ extern "C" int tg_server_submit_gmres(tg_server_t h, int32_t n, int64_t nnz, con
     r->restart = restart;
     r->max_restarts = max_restarts;
     r->batched = h->s->gmres_batched && n <= h->s->gbatch_max_rows;
+    r->mixed = (!r->batched && g_serve_gmres_mixed) ? 1 : 0;
     r->bytes = r->batched ? input_bytes(n, nnz, nrhs, false) + tg_gbatch_request_bytes(n, nrhs, restart)
+               : r->mixed ? input_bytes(n, nnz, nrhs, false) + tg_gmres_mixed_workspace_bytes(n, nnz, restart, nrhs, 0, 0)
                           : input_bytes(n, nnz, nrhs, false) + tg_gmres_workspace_bytes(n, restart, nrhs, TG_F64);
     return submit(h, std::move(r), id_out);
 }

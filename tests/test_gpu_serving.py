@@ -45,13 +45,15 @@ def submit(srv, req):
     return srv.submit_gmres(rp, ci, val, B, tol=1e-8, restart=30, max_restarts=100)
 
 
-@pytest.fixture(params=[(1, 1 << 30), (0, 1 << 30), (1, 2000)], ids=["gmres_batched", "gmres_workers", "gmres_split"])
+@pytest.fixture(params=[(1, 1 << 30, 0), (0, 1 << 30, 0), (1, 2000, 0), (0, 1 << 30, 1), (1, 2000, 1)],
+                ids=["gmres_batched", "gmres_workers", "gmres_split", "gmres_workers_mixed", "gmres_split_mixed"])
 def gmres_mode(request):
     """Serving GMRES requests through the batched GMRES engine (every request here
     fits it), on per-request worker streams, or split by size (systems of up to 2000
     rows batched, the larger ones on workers, as the default split does at 131072
-    rows); restores the knobs afterwards."""
-    keys = (b"serve_gmres_batch", b"serve_gmres_batch_rows")
+    rows); the worker-stream requests with tg_gmres_solve or the mixed solver
+    (serve_gmres_mixed); restores the knobs afterwards."""
+    keys = (b"serve_gmres_batch", b"serve_gmres_batch_rows", b"serve_gmres_mixed")
     before = [int(capi.lib().tg_tune_get(k)) for k in keys]
     for k, v in zip(keys, request.param):
         capi.call("tg_tune_set", k, v)
