@@ -232,19 +232,19 @@ def test_cg_graph_blocks_equal_eager_launches(cuda, fused_knob, form, k, prec):
     assert am[0] == bm[0] and torch.equal(am[1], bm[1])
 
 
-def _cg_raw(A, n, k, dtype, d, B, max_iter, tol, ws_bytes, cuda):
-    """tg_cg_solve with a workspace of exactly ws_bytes bytes."""
+def _cg_raw(A, n, k, dtype, d, B, max_iter, tol, ws_bytes, cuda, X0=None):
+    """tg_cg_solve with a workspace of exactly ws_bytes bytes (X0: initial guess)."""
     import ctypes as C
     tg = capi.F64 if dtype == torch.float64 else capi.F32
     rp, ci, vals = A.device_arrays(dtype)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=cuda)
-    X = torch.zeros(n, k, dtype=dtype, device=cuda)
+    X = torch.zeros(n, k, dtype=dtype, device=cuda) if X0 is None else X0.clone()
     ok, it = C.c_int32(0), C.c_int32(0)
     rr = np.zeros(k)
     capi.call("tg_cg_solve", n, A.NonzeroCount(), rp.data_ptr(), capi.I64 if rp.dtype == torch.int64 else capi.I32,
               ci.data_ptr(), vals.data_ptr(), tg, 0 if d is None else 1, None if d is None else d.data_ptr(), k,
-              max_iter, 1.0, B.data_ptr(), k, 0.0, X.data_ptr(), k, tol, 0, ws.data_ptr(), ws_bytes, C.byref(ok),
-              C.byref(it), rr.ctypes.data, None)
+              max_iter, 1.0, B.data_ptr(), k, 0.0, X.data_ptr(), k, tol, 0 if X0 is None else 1, ws.data_ptr(),
+              ws_bytes, C.byref(ok), C.byref(it), rr.ctypes.data, None)
     torch.cuda.synchronize()
     return bool(ok.value), it.value, rr.tolist(), X.cpu()
 
@@ -257,8 +257,8 @@ def test_cg_deferred_x_update_is_bitwise(cuda, fused_knob, dtype, k, prec):
     ring of p.  The updates are the same fused multiply-adds in the same order, so the
     solution, iteration count and residuals are bitwise those of the per-iteration update:
     a converged solve (stopping inside a group of 4), one that runs into max_iter = 45, a
-    workspace with only the two ring slots every solve accepts (depth 2 then), and the
-    mixed solver, whose host hook folds x every 16 iterations."""
+    workspace with only the two ring slots every solve accepts (depth 2 then), a solve from
+    an initial guess, and the mixed solver, whose host hook folds x every 16 iterations."""
     rp, ci, val, n = grid_system(40, 0.05)
     B = gen.uniform(1313 + k, n * k).reshape(n, k)
     d = 1.0 / sp.csr_matrix((val, ci, rp), shape=(n, n)).diagonal() if prec else None
@@ -269,6 +269,7 @@ def test_cg_deferred_x_update_is_bitwise(cuda, fused_knob, dtype, k, prec):
     vec = (n * k * vs + 255) // 256 * 256
     dd = None if d is None else dev(d, cuda, dtype)
     Bd = dev(B, cuda, dtype)
+    X0 = dev(gen.uniform(1414 + k, n * k).reshape(n, k), cuda, dtype)
     fused_knob(1)
     before = int(capi.lib().tg_tune_get(b"cg_xdefer"))
     out = {}
@@ -277,6 +278,7 @@ def test_cg_deferred_x_update_is_bitwise(cuda, fused_knob, dtype, k, prec):
             capi.call("tg_tune_set", b"cg_xdefer", depth)
             res = [_cg_raw(A, n, k, dtype, dd, Bd, mi, t, full, cuda) for mi, t in ((2000, tol), (45, 1e-14))]
             res.append(_cg_raw(A, n, k, dtype, dd, Bd, 2000, tol, full - 2 * vec, cuda))
+            res.append(_cg_raw(A, n, k, dtype, dd, Bd, 2000, tol, full, cuda, X0=X0))  # x starts at a guess
             if dtype == torch.float64:
                 Xm = torch.zeros(n, k, dtype=torch.float64, device=cuda)
                 mixed = cg_solve_mixed(A, Bd, Xm, tol=1e-10, P=dd)
