@@ -273,9 +273,10 @@ __global__ __launch_bounds__(kFinalThreads) void cg_rho_kernel(int k, int kpad, 
 constexpr int kMaxDefer = kCgRingSlots;  // ring slots of p for the deferred x update
 
 // The x update of the iterations [xf, xf + cnt) for the thread's R rows: x += alpha_t p_t
-// in iteration order, each the fused multiply-add of the unfused update (x starts at zero
-// without a guess: the first update is alpha p), so x is bitwise the same for any
-// deferral depth.  The p of the newest iteration may be passed in `own` (already loaded).
+// in iteration order, each the same fused multiply-add the per-iteration update makes (x
+// starts at zero without a guess: the first update is alpha p), so x is bitwise the same
+// for any deferral depth.  The p of the newest iteration, xf + cnt - 1, may be passed in
+// `own` (already loaded).
 template <typename T, int R>
 __device__ __forceinline__ void x_updates(int cnt, int xf, int D, bool first0, const T (&al)[kMaxDefer],
                                           const T* ring, int64_t pstride, const int64_t (&e)[R], const T* own,
