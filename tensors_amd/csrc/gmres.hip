@@ -42,6 +42,7 @@ constexpr int kGmresMaxM = 63;     // restart length limit (m + 1 basis vectors 
 constexpr int kGB = 512;           // workgroups of the vector kernels (at most; fewer for small n)
 static int g_gmres_graph = 1;      // 1: cycles after the first run as one captured hipGraph
 static int g_gmres_delayed = 1;    // 1: CGS2 with the second pass delayed to the next step (two basis passes per step)
+static int g_gmres_split_dots = 1;  // 1: FP32 one-column dots pass of the 32-vector bucket with its two sides on alternate lanes
 
 struct GmresHdr {
     int iter;
@@ -352,6 +353,89 @@ __global__ __launch_bounds__(kBlock) void gm_ddots_kernel(int n, int k, int kpad
             }
         }
     dd_partial<NQMAX>(acc, j, k, kpad, partial);
+}
+
+// One column, FP32, 32-vector bucket: the dots pass with its two sides on alternate
+// lanes.  Even lanes accumulate a_i = Q_i . u and alpha = u . u, odd lanes b_i = Q_i . v
+// and beta = u . v, over the same rows (the pair's loads of a basis row are one
+// address), so a lane holds NQMAX + 1 FP64 accumulators instead of 2 NQMAX + 2 and the
+// reduce-scatter (over the lanes of one side: two "columns") half the values.  RU rows
+// per iteration: 1 (111 VGPRs, 4 waves per SIMD) measured faster on systems of 1e4 and
+// 9e4 rows, 2 (179 VGPRs, 2 waves) on 1e6 rows.
+template <int NQMAX>
+__device__ __forceinline__ void dd_partial_split(double (&v)[NQMAX + 1], int j, double* __restrict__ partial) {
+    extern __shared__ double gm_sh[];
+    const int nv = 2 * j + 2;
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    constexpr int R = NQMAX + 1;
+    {
+        const ScatterOut o = wave_scatter_sum<R>(v, 2);
+        const int side = lane % 2;
+        if ((lane % o.low) < 2)
+#pragma unroll
+            for (int i = 0; i < ScatterSizes<R>::A; ++i) {
+                const int r = o.base + i;
+                const int t = i >= o.count ? -1 : r < NQMAX ? (r < j ? side * j + r : -1) : 2 * j + side;
+                if (t >= 0) gm_sh[wave * nv + t] = v[i];
+            }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < nv; t += kBlock) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < kBlock / kWave; ++w) s += gm_sh[w * nv + t];
+        partial[(int64_t)blockIdx.x * nv + t] = s;
+    }
+}
+
+template <int NQMAX, int RU>
+__global__ __launch_bounds__(kBlock) void gm_ddots_split_kernel(int n, int j, int finish,
+                                                                const float* __restrict__ Q, int64_t qstride,
+                                                                const float* __restrict__ z,
+                                                                const float* __restrict__ d, float* __restrict__ vslot,
+                                                                double* __restrict__ partial,
+                                                                const GmresHdr* __restrict__ hd) {
+    if (hd->done) return;
+    int i0, i1;
+    grow_range(n, i0, i1);
+    const int side = threadIdx.x & 1, ro = threadIdx.x >> 1, rows = kBlock / 2;
+    double acc[NQMAX + 1];
+#pragma unroll
+    for (int r = 0; r < NQMAX + 1; ++r) acc[r] = 0.0;
+    const float* U = Q + (int64_t)j * qstride;
+    for (int r = i0 + ro; r < i1; r += rows * RU) {
+        float qi[RU][NQMAX], uv[RU], vv[RU];
+        int rr[RU];
+#pragma unroll
+        for (int u = 0; u < RU; ++u) {
+            rr[u] = r + u * rows < i1 ? r + u * rows : r;
+#pragma unroll
+            for (int i = 0; i < NQMAX; ++i) qi[u][i] = Q[(int64_t)max(0, min(i, j - 1)) * qstride + rr[u]];
+            uv[u] = U[rr[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < RU; ++u) {
+            const bool ok = r + u * rows < i1;
+            if (finish) {
+                vv[u] = uv[u];
+            } else if (d) {
+                vv[u] = d[rr[u]] * z[rr[u]];
+                if (ok && side) vslot[rr[u]] = vv[u];
+            } else {
+                vv[u] = vslot[rr[u]];
+            }
+            if (!ok) uv[u] = vv[u] = 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < RU; ++u) {
+            const float w = side ? vv[u] : uv[u];
+#pragma unroll
+            for (int i = 0; i < NQMAX; ++i)
+                if (i < j) acc[i] += (double)qi[u][i] * (double)w;
+            acc[NQMAX] += (double)uv[u] * (double)w;
+        }
+    }
+    dd_partial_split<NQMAX>(acc, j, partial);
 }
 
 // One workgroup: the scalar work of delayed step j (finish: the cycle's last column).
@@ -837,6 +921,17 @@ template <typename T, int NQMAX>
 static void launch_ddots(hipStream_t s, int gb, int n, int k, int kpad, int j, int finish, const T* Q, int64_t qs,
                          const T* z, const T* d, T* v, double* partial, const GmresHdr* hd) {
     const size_t lds = (size_t)(kBlock / kWave) * (2 * j + 2) * k * sizeof(double);
+    if constexpr (sizeof(T) == 4 && NQMAX == 32) {
+        if (k == 1 && g_gmres_split_dots) {
+            if (n >= (1 << 19))
+                hipLaunchKernelGGL((gm_ddots_split_kernel<NQMAX, 2>), dim3(gb), dim3(kBlock), lds, s, n, j, finish, Q,
+                                   qs, z, d, v, partial, hd);
+            else
+                hipLaunchKernelGGL((gm_ddots_split_kernel<NQMAX, 1>), dim3(gb), dim3(kBlock), lds, s, n, j, finish, Q,
+                                   qs, z, d, v, partial, hd);
+            return;
+        }
+    }
     hipLaunchKernelGGL((gm_ddots_kernel<T, NQMAX>), dim3(gb), dim3(kBlock), lds, s, n, k, kpad, j, finish, Q, qs, z,
                        d, v, partial, hd);
 }
@@ -1189,16 +1284,19 @@ template int gmres_solve_t<float>(const CsrView&, const OpRef&, int, const float
 
 int gmres_tune_set(const char* key, int32_t value, bool* known) {
     const std::string k = key;
-    *known = k == "gmres_graph" || k == "gmres_delayed";
+    *known = k == "gmres_graph" || k == "gmres_delayed" || k == "gmres_split_dots";
     if (!*known) return TG_OK;
     if (value != 0 && value != 1) return fail(TG_ERR_INVALID, "tg_tune_set: %s must be 0 or 1", key);
-    (k == "gmres_graph" ? g_gmres_graph : g_gmres_delayed) = value;
+    (k == "gmres_graph" ? g_gmres_graph : k == "gmres_delayed" ? g_gmres_delayed : g_gmres_split_dots) = value;
     return TG_OK;
 }
 
 int32_t gmres_tune_get(const char* key) {
     const std::string k = key;
-    return k == "gmres_graph" ? g_gmres_graph : k == "gmres_delayed" ? g_gmres_delayed : -1;
+    return k == "gmres_graph"        ? g_gmres_graph
+           : k == "gmres_delayed"    ? g_gmres_delayed
+           : k == "gmres_split_dots" ? g_gmres_split_dots
+                                     : -1;
 }
 
 }  // namespace tg

@@ -98,3 +98,36 @@ def test_mixed_gmres_special_cases(cuda):
     ok, *_ = gmres_solve_mixed(M, B, X, tol=1e-10, a=2.0, b=0.5)
     Xs = np.linalg.solve(A.toarray(), B.cpu().numpy())
     assert ok and rel_err(X.cpu().numpy(), 2.0 * Xs + 0.5 * X0.cpu().numpy()) < 1e-7
+
+
+@pytest.mark.parametrize("g,side,prec", [(60, 0, False), (60, 0, True), (40, 1, True), (800, 0, False)],
+                         ids=["plain", "left_jacobi", "right_jacobi", "large"])
+def test_mixed_gmres_split_dots_agree(cuda, g, side, prec):
+    """Knob gmres_split_dots: the FP32 dots pass of the 32-vector bucket with the u-side and
+    v-side products on alternate lanes (one row per iteration below 2^19 rows, two above)
+    against both sides on every lane.  Only the summation order differs, so both reach the
+    tolerance on the true FP64 residual in the same number of restarts (within one) and
+    the solutions agree far below the tolerance's scale."""
+    from tensors_amd import _capi as capi
+
+    rp, ci, val, n, A = problem(g, 20.0)
+    d = 1.0 / A.diagonal() if prec else None
+    B = gen.uniform(6006 + g, n).reshape(n, 1)
+    M = MatrixCSR(rp.astype(np.int32), ci, val, n, n)
+    Pd = torch.from_numpy(d).to(cuda) if prec else None
+    before = int(capi.lib().tg_tune_get(b"gmres_split_dots"))
+    out = {}
+    try:
+        for split in (1, 0):
+            capi.call("tg_tune_set", b"gmres_split_dots", split)
+            X = torch.zeros(n, 1, dtype=torch.float64, device=cuda)
+            ok, restarts, steps, rr = gmres_solve_mixed(M, torch.from_numpy(B).to(cuda), X, tol=1e-8, restart=30,
+                                                        max_restarts=300, P=Pd, side=side)
+            out[split] = (ok, restarts, rr, X.cpu().numpy())
+    finally:
+        capi.call("tg_tune_set", b"gmres_split_dots", before)
+    (ok1, r1, rr1, X1), (ok0, r0, rr0, X0) = out[1], out[0]
+    print(f"split dots g {g}: restarts {r1} / {r0}, residuals {rr1} / {rr0}")
+    assert ok1 and ok0 and np.all(rr1 <= 1e-8) and np.all(rr0 <= 1e-8)
+    assert r1 >= 1 and abs(r1 - r0) <= 1  # the 32-vector bucket ran (a whole cycle of 30 steps)
+    assert rel_err(X1, X0) < 1e-6
