@@ -42,7 +42,7 @@ constexpr int kGmresMaxM = 63;     // restart length limit (m + 1 basis vectors 
 constexpr int kGB = 512;           // workgroups of the vector kernels (at most; fewer for small n)
 static int g_gmres_graph = 1;      // 1: cycles after the first run as one captured hipGraph
 static int g_gmres_delayed = 1;    // 1: CGS2 with the second pass delayed to the next step (two basis passes per step)
-static int g_gmres_split_dots = 1;  // 1: FP32 one-column dots pass of the 32-vector bucket with its two sides on alternate lanes
+static int g_gmres_split_dots = 1;  // 1: one-column dots pass of the 32-vector bucket with its two sides on alternate lanes
 
 struct GmresHdr {
     int iter;
@@ -355,13 +355,13 @@ __global__ __launch_bounds__(kBlock) void gm_ddots_kernel(int n, int k, int kpad
     dd_partial<NQMAX>(acc, j, k, kpad, partial);
 }
 
-// One column, FP32, 32-vector bucket: the dots pass with its two sides on alternate
-// lanes.  Even lanes accumulate a_i = Q_i . u and alpha = u . u, odd lanes b_i = Q_i . v
+// One column, 32-vector bucket: the dots pass with its two sides on alternate lanes.  Even lanes accumulate a_i = Q_i . u and alpha = u . u, odd lanes b_i = Q_i . v
 // and beta = u . v, over the same rows (the pair's loads of a basis row are one
 // address), so a lane holds NQMAX + 1 FP64 accumulators instead of 2 NQMAX + 2 and the
-// reduce-scatter (over the lanes of one side: two "columns") half the values.  RU rows
-// per iteration: 1 (111 VGPRs, 4 waves per SIMD) measured faster on systems of 1e4 and
-// 9e4 rows, 2 (179 VGPRs, 2 waves) on 1e6 rows.
+// reduce-scatter (over the lanes of one side: two "columns") half the values.  FP32: RU
+// rows per iteration, 1 (111 VGPRs, 4 waves per SIMD) measured faster on systems of 1e4
+// and 9e4 rows, 2 (179 VGPRs, 2 waves) on 1e6 rows.  FP64: 1 (144 VGPRs, 3 waves), below
+// 2^19 rows.
 template <int NQMAX>
 __device__ __forceinline__ void dd_partial_split(double (&v)[NQMAX + 1], int j, double* __restrict__ partial) {
     extern __shared__ double gm_sh[];
@@ -388,11 +388,11 @@ __device__ __forceinline__ void dd_partial_split(double (&v)[NQMAX + 1], int j, 
     }
 }
 
-template <int NQMAX, int RU>
+template <typename T, int NQMAX, int RU>
 __global__ __launch_bounds__(kBlock) void gm_ddots_split_kernel(int n, int j, int finish,
-                                                                const float* __restrict__ Q, int64_t qstride,
-                                                                const float* __restrict__ z,
-                                                                const float* __restrict__ d, float* __restrict__ vslot,
+                                                                const T* __restrict__ Q, int64_t qstride,
+                                                                const T* __restrict__ z,
+                                                                const T* __restrict__ d, T* __restrict__ vslot,
                                                                 double* __restrict__ partial,
                                                                 const GmresHdr* __restrict__ hd) {
     if (hd->done) return;
@@ -402,9 +402,9 @@ __global__ __launch_bounds__(kBlock) void gm_ddots_split_kernel(int n, int j, in
     double acc[NQMAX + 1];
 #pragma unroll
     for (int r = 0; r < NQMAX + 1; ++r) acc[r] = 0.0;
-    const float* U = Q + (int64_t)j * qstride;
+    const T* U = Q + (int64_t)j * qstride;
     for (int r = i0 + ro; r < i1; r += rows * RU) {
-        float qi[RU][NQMAX], uv[RU], vv[RU];
+        T qi[RU][NQMAX], uv[RU], vv[RU];
         int rr[RU];
 #pragma unroll
         for (int u = 0; u < RU; ++u) {
@@ -424,11 +424,11 @@ __global__ __launch_bounds__(kBlock) void gm_ddots_split_kernel(int n, int j, in
             } else {
                 vv[u] = vslot[rr[u]];
             }
-            if (!ok) uv[u] = vv[u] = 0.0f;
+            if (!ok) uv[u] = vv[u] = T(0);
         }
 #pragma unroll
         for (int u = 0; u < RU; ++u) {
-            const float w = side ? vv[u] : uv[u];
+            const T w = side ? vv[u] : uv[u];
 #pragma unroll
             for (int i = 0; i < NQMAX; ++i)
                 if (i < j) acc[i] += (double)qi[u][i] * (double)w;
@@ -921,14 +921,19 @@ template <typename T, int NQMAX>
 static void launch_ddots(hipStream_t s, int gb, int n, int k, int kpad, int j, int finish, const T* Q, int64_t qs,
                          const T* z, const T* d, T* v, double* partial, const GmresHdr* hd) {
     const size_t lds = (size_t)(kBlock / kWave) * (2 * j + 2) * k * sizeof(double);
-    if constexpr (sizeof(T) == 4 && NQMAX == 32) {
-        if (k == 1 && g_gmres_split_dots) {
-            if (n >= (1 << 19))
-                hipLaunchKernelGGL((gm_ddots_split_kernel<NQMAX, 2>), dim3(gb), dim3(kBlock), lds, s, n, j, finish, Q,
-                                   qs, z, d, v, partial, hd);
-            else
-                hipLaunchKernelGGL((gm_ddots_split_kernel<NQMAX, 1>), dim3(gb), dim3(kBlock), lds, s, n, j, finish, Q,
-                                   qs, z, d, v, partial, hd);
+    if constexpr (NQMAX == 32) {
+        // FP64 on 2^19 rows and more: the unsplit pass (1e6 rows: 210 against 214 ms)
+        if (k == 1 && g_gmres_split_dots && (sizeof(T) == 4 || n < (1 << 19))) {
+            bool two = false;
+            if constexpr (sizeof(T) == 4) two = n >= (1 << 19);
+            if (two) {
+                if constexpr (sizeof(T) == 4)
+                    hipLaunchKernelGGL((gm_ddots_split_kernel<T, NQMAX, 2>), dim3(gb), dim3(kBlock), lds, s, n, j,
+                                       finish, Q, qs, z, d, v, partial, hd);
+            } else {
+                hipLaunchKernelGGL((gm_ddots_split_kernel<T, NQMAX, 1>), dim3(gb), dim3(kBlock), lds, s, n, j, finish,
+                                   Q, qs, z, d, v, partial, hd);
+            }
             return;
         }
     }

@@ -180,3 +180,34 @@ def test_gmres_tiny_systems(cuda, n, side):
     assert g(MatrixCSR(rp, ci, val, n, n), None, 1.0, dev(B, cuda), 0.0, X, 1e-10, 5)
     assert g.RestartCount() == 0 and g.IterationCount() <= n
     assert rel_err(X.cpu().numpy(), np.linalg.solve(A.toarray(), B)) < 1e-9
+
+
+@pytest.mark.parametrize("side,prec", [(0, False), (0, True), (1, True)], ids=["plain", "left_jacobi", "right_jacobi"])
+def test_gmres_split_dots_agree(cuda, side, prec):
+    """Knob gmres_split_dots in FP64 (one right-hand side, more than 16 basis vectors,
+    below 2^19 rows): the dots pass with its u-side and v-side products on alternate
+    lanes against both on every lane.  Only the summation order differs: same restarts,
+    solutions within rounding of each other and of the oracle's GMRES."""
+    from tensors_amd import _capi as capi
+
+    rp, ci, val, n = cd_system(50, 20.0)
+    B = gen.uniform(7007, n).reshape(n, 1)
+    d = 1.0 / sp.csr_matrix((val, ci, rp), shape=(n, n)).diagonal() if prec else None
+    A = MatrixCSR(rp, ci, val, n, n)
+    before = int(capi.lib().tg_tune_get(b"gmres_split_dots"))
+    out = {}
+    try:
+        for split in (1, 0):
+            capi.call("tg_tune_set", b"gmres_split_dots", split)
+            g = GMRES(n, 30, 1, side=side)
+            Xd = torch.zeros(n, 1, dtype=torch.float64, device=cuda)
+            ok = g(A, None if d is None else dev(d, cuda), 1.0, dev(B, cuda), 0.0, Xd, 1e-10, 200)
+            out[split] = (ok, g.RestartCount(), Xd.cpu().numpy())
+    finally:
+        capi.call("tg_tune_set", b"gmres_split_dots", before)
+    (ok1, r1, X1), (ok0, r0, X0) = out[1], out[0]
+    assert ok1 and ok0 and r1 >= 1 and abs(r1 - r0) <= 1
+    assert rel_err(X1, X0) < 1e-9
+    Xo = np.zeros((n, 1))
+    ok_o, it_o, rs_o, _ = oracle.gmres(n, rp, ci, val, B, Xo, 1, 30, 1e-10, 200, prec_diag=d, side=side)
+    assert ok_o == 1 and abs(r1 - rs_o) <= 1 and rel_err(X1, Xo) < 1e-7
