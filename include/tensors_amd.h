@@ -221,8 +221,9 @@ int tg_csr_spmm_plan(tg_csr_plan_t plan, const void* a, tg_dtype val_type, doubl
  *                  basis vectors (FP32: the cycles of tg_gmres_solve_mixed; FP64:
  *                  systems below 2^19 rows): 1 = the dots pass with its u-side and
  *                  v-side dot products on alternate lanes (default: half the
- *                  accumulators per lane), 0 = both on every lane.  Results agree to
- *                  rounding.
+ *                  accumulators per lane), and in FP32 the update pass with its two
+ *                  output vectors on alternate lanes; 0 = both on every lane.
+ *                  Results agree to rounding.
  *   "bsr_vector":  tg_bsr_spmm with row-major X / Y chunks, 1 = a lane per
  *                  (block row, 1-4 right-hand sides) with vector loads (default),
  *                  0 = a lane per output entry (the mapping of the other

@@ -610,6 +610,52 @@ __global__ __launch_bounds__(kBlock) void gm_dupdate_kernel(int n, int k, int kp
     }
 }
 
+// One column, 32-vector bucket, FP32: gm_dupdate_kernel with its two outputs on
+// alternate lanes (even: q_j, odd: the next pending vector) over the same rows, the
+// pair's loads of a basis row being one address; each output's operations and order are
+// those of gm_dupdate_kernel (the compiler's contraction of the products may differ, so
+// the vectors agree to rounding).  Mixed GMRES 1e4 / 9e4 / 1e6 rows 1-3 % faster; in FP64
+// the unsplit kernel measured faster on 1e6 rows.
+template <typename T, int NQMAX>
+__global__ __launch_bounds__(kBlock) void gm_dupdate_split_kernel(int n, int j, T* __restrict__ Q, int64_t qstride,
+                                                                  const double* __restrict__ small, GmresLayout L,
+                                                                  const GmresHdr* __restrict__ hd) {
+    if (hd->done && hd->iter != j) return;
+    const bool next = !hd->done;
+    const int side = threadIdx.x & 1;
+    if (side && !next) return;  // no pending vector to form
+    int i0, i1;
+    grow_range(n, i0, i1);
+    const int ro = threadIdx.x >> 1, rows = kBlock / 2;
+    T co[NQMAX];
+#pragma unroll
+    for (int i = 0; i < NQMAX; ++i) co[i] = i < j ? (T)small[(side ? L.dE : L.dA) + i] : T(0);
+    const T ir = (T)small[L.dsc];
+    const T cj = next ? (T)small[L.dsc + 1] : T(0);
+    T* U = Q + (int64_t)j * qstride;
+    T* V = Q + (int64_t)(j + 1) * qstride;
+    for (int r = i0 + ro; r < i1; r += rows) {
+        T qi[NQMAX];
+#pragma unroll
+        for (int i = 0; i < NQMAX; ++i) qi[i] = Q[(int64_t)max(0, min(i, j - 1)) * qstride + r];  // loads first
+        const T uv = U[r];
+        T out;
+        if (side) {
+            const T vv = V[r];
+            out = vv * ir - cj * uv;
+        } else {
+            out = uv * ir;
+        }
+#pragma unroll
+        for (int i = 0; i < NQMAX; ++i)
+            if (i < j) out -= co[i] * qi[i];
+        if (side)
+            V[r] = out;
+        else if (j > 0)
+            U[r] = out;
+    }
+}
+
 // H[j+1][j] = |q|; Givens (GMRES.hpp:603-658); beta; convergence; ++iter.  |q|^2 per
 // column is first summed from the B workgroup partials in gm_total_kernel's order (one
 // workgroup of kBlock threads, one column at a time).
@@ -958,6 +1004,8 @@ static void dupdate(hipStream_t s, int gb, int n, int k, int kpad, int j, T* Q, 
     else if (j <= 16)
         hipLaunchKernelGGL((gm_dupdate_kernel<T, 16>), dim3(gb), dim3(kBlock), 0, s, n, k, kpad, j, Q, qs, small, L,
                            hd);
+    else if (sizeof(T) == 4 && k == 1 && g_gmres_split_dots)  // FP64: the unsplit pass measured faster
+        hipLaunchKernelGGL((gm_dupdate_split_kernel<T, 32>), dim3(gb), dim3(kBlock), 0, s, n, j, Q, qs, small, L, hd);
     else
         hipLaunchKernelGGL((gm_dupdate_kernel<T, 32>), dim3(gb), dim3(kBlock), 0, s, n, k, kpad, j, Q, qs, small, L,
                            hd);
